@@ -1,0 +1,311 @@
+"""ctypes binding of libcapow.so (include/capow.h).
+
+This is the checking/benchmark layer: the product is the C ABI and its HIP
+kernels.  Loading fails loudly when the library has not been built; there is
+no CPU fallback.
+
+`PowerSpectrum` follows the reference's periodic call sequence
+(create_grids -> read + add_to_grid -> mean subtraction -> fftw_execute ->
+compute_periodic_power -> output_power, reference src/main.c:28-298) with
+the same parameter names (src/read_params.c:65-227).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcapow.so")
+
+CAPOW_OK = 0
+ERRORS = {1: "CAPOW_ERR_ARG", 2: "CAPOW_ERR_HIP", 3: "CAPOW_ERR_NOMEM", 4: "CAPOW_ERR_STATE",
+          5: "CAPOW_ERR_UNSUPPORTED", 6: "CAPOW_ERR_INTERNAL", 7: "CAPOW_ERR_IO",
+          110: "CAPOW_ERR_OUT_OF_BOX"}
+FP32, FP64 = 0, 1
+EDGE_REFERENCE, EDGE_PERIODIC = 0, 1
+SPEC_AUTO_A, SPEC_AUTO_B = 1, 2
+HOST, DEVICE = 0, 1
+
+
+class CapowError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (%d): %s" % (ERRORS.get(code, "?"), code, msg))
+        self.code = code
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int64), ("ny", C.c_int64), ("nz", C.c_int64),
+        ("xmin", C.c_double), ("xmax", C.c_double), ("ymin", C.c_double),
+        ("ymax", C.c_double), ("zmin", C.c_double), ("zmax", C.c_double),
+        ("interp_order", C.c_int32), ("do_interlacing", C.c_int32), ("los", C.c_int32),
+        ("momentum", C.c_int32), ("nk", C.c_int32), ("nmu", C.c_int32),
+        ("output_2d", C.c_int32), ("output_log", C.c_int32), ("odd_multipoles", C.c_int32),
+        ("mink", C.c_double), ("maxk", C.c_double), ("omega_m", C.c_double),
+        ("redshift", C.c_double),
+        ("precision", C.c_int32), ("edge_mode", C.c_int32), ("extra_spectra", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+class AssignStats(C.Structure):
+    _fields_ = [(n, C.c_double) for n in
+                ("nread", "nout", "ngrid", "nsq", "vr", "vr2", "vr3", "vr4", "alpha")]
+
+
+class Bins(C.Structure):
+    _fields_ = [("n_spectra", C.c_int32), ("codes", C.c_int32 * 3),
+                ("pk", C.POINTER(C.c_double)), ("nmodes", C.POINTER(C.c_int64)),
+                ("pk2d", C.POINTER(C.c_double)), ("nmodes2d", C.POINTER(C.c_int64))]
+
+
+# Exported symbols of include/capow.h with their ctypes signatures.
+_P = C.POINTER(Params)
+_V = C.c_void_p
+_D = C.POINTER(C.c_double)
+_I64 = C.POINTER(C.c_int64)
+SIGNATURES = {
+    "capow_create": ([_P, C.POINTER(_V)], C.c_int),
+    "capow_destroy": ([_V], C.c_int),
+    "capow_last_error": ([_V], C.c_char_p),
+    "capow_set_stream": ([_V, _V], C.c_int),
+    "capow_synchronize": ([_V], C.c_int),
+    "capow_set_timing": ([_V, C.c_int], C.c_int),
+    "capow_set_particles": ([_V, _V, _V, C.c_int64, C.c_int32, C.c_int32], C.c_int),
+    "capow_assign": ([_V, C.POINTER(AssignStats)], C.c_int),
+    "capow_fft": ([_V], C.c_int),
+    "capow_bin": ([_V, C.POINTER(Bins)], C.c_int),
+    "capow_power": ([_V, C.POINTER(AssignStats), C.POINTER(Bins)], C.c_int),
+    "capow_spectra": ([_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)], C.c_int),
+    "capow_scalars": ([_P, C.POINTER(AssignStats), C.c_int32, _D, _D], C.c_int),
+    "capow_finalize": ([_P, C.c_double, C.c_double, _D, _I64, _D, _I64,
+                        C.POINTER(C.c_int32)], C.c_int),
+    "capow_write_output": ([_P, C.c_double, C.c_double, _D, _I64, _D, _I64, C.c_char_p,
+                            C.c_int32], C.c_int),
+    "capow_grid_elems": ([_V, _I64], C.c_int),
+    "capow_get_grid": ([_V, C.c_int32, _V, C.c_int64], C.c_int),
+    "capow_set_grid": ([_V, C.c_int32, _V, C.c_int64], C.c_int),
+    "capow_mark_assigned": ([_V, C.POINTER(AssignStats)], C.c_int),
+    "capow_mark_transformed": ([_V], C.c_int),
+    "capow_kernel_times": ([_V, C.c_char_p, C.c_int32, _D, C.POINTER(C.c_int32), C.c_int32,
+                            C.POINTER(C.c_int32)], C.c_int),
+    "capow_reset_kernel_times": ([_V], C.c_int),
+}
+
+_lib = None
+
+
+def lib():
+    """Loads libcapow.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libcapow.so not built: run __graft_entry__.build() or "
+                              "python -m capow_amd.build_lib")
+        L = C.CDLL(LIB_PATH)
+        for name, (args, res) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+DEFAULTS = dict(nx=64, ny=64, nz=64, xmin=0.0, xmax=1000.0, ymin=0.0, ymax=1000.0,
+                zmin=0.0, zmax=1000.0, interp_order=2, do_interlacing=0, los=0, momentum=0,
+                nk=40, nmu=1, output_2d=0, output_log=0, odd_multipoles=0, mink=0.0,
+                maxk=0.4, omega_m=0.3, redshift=0.5, precision=FP64,
+                edge_mode=EDGE_REFERENCE, extra_spectra=0, reserved=0)
+
+
+def make_params(**kw):
+    d = dict(DEFAULTS)
+    unknown = set(kw) - set(d)
+    if unknown:
+        raise TypeError("unknown parameters: %s" % sorted(unknown))
+    d.update(kw)
+    return Params(**d)
+
+
+def spectra_codes(p):
+    n = C.c_int32()
+    codes = (C.c_int32 * 3)()
+    _check(None, lib().capow_spectra(C.byref(p), C.byref(n), codes))
+    return list(codes[:n.value])
+
+
+def _check(ctx, rc):
+    if rc != CAPOW_OK:
+        msg = lib().capow_last_error(ctx)
+        raise CapowError(rc, msg.decode() if msg else "")
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_D)
+
+
+def _iptr(a):
+    return a.ctypes.data_as(_I64)
+
+
+class PowerSpectrum:
+    """One context: device grids, sort buffers and tables for fixed params."""
+
+    def __init__(self, params=None, **kw):
+        self.p = params if params is not None else make_params(**kw)
+        self.ctx = _V()
+        _check(None, lib().capow_create(C.byref(self.p), C.byref(self.ctx)))
+        self.fp64 = self.p.precision == FP64
+        self.dtype = np.float64 if self.fp64 else np.float32
+        self.codes = spectra_codes(self.p)
+        self._keep = None
+        self.stats = None
+
+    def close(self):
+        if self.ctx:
+            lib().capow_destroy(self.ctx)
+            self.ctx = _V()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, rc):
+        _check(self.ctx, rc)
+
+    def set_timing(self, on=True):
+        self.check(lib().capow_set_timing(self.ctx, 1 if on else 0))
+
+    def set_stream(self, stream_ptr):
+        self.check(lib().capow_set_stream(self.ctx, _V(stream_ptr)))
+
+    def set_particles(self, xyzv, w=None):
+        """xyzv: (n, 4) rows of x y z v_los; numpy (copied) or a torch CUDA
+        tensor (used in place)."""
+        dt = 1 if self.fp64 else 0
+        if hasattr(xyzv, "data_ptr") and getattr(xyzv, "is_cuda", False):
+            assert xyzv.is_contiguous() and xyzv.shape[-1] == 4
+            wp = _V(w.data_ptr()) if w is not None else None
+            self._keep = (xyzv, w)
+            self.check(lib().capow_set_particles(self.ctx, _V(xyzv.data_ptr()), wp,
+                                                 xyzv.shape[0], dt, DEVICE))
+        else:
+            a = np.ascontiguousarray(xyzv, dtype=self.dtype)
+            assert a.ndim == 2 and a.shape[1] == 4
+            wa = None if w is None else np.ascontiguousarray(w, dtype=self.dtype)
+            self._keep = (a, wa)
+            self.check(lib().capow_set_particles(
+                self.ctx, a.ctypes.data_as(_V), wa.ctypes.data_as(_V) if wa is not None else None,
+                a.shape[0], dt, HOST))
+
+    def assign(self):
+        st = AssignStats()
+        self.check(lib().capow_assign(self.ctx, C.byref(st)))
+        self.stats = st
+        return st
+
+    def fft(self):
+        self.check(lib().capow_fft(self.ctx))
+
+    def _bins_struct(self):
+        p = self.p
+        ns = len(self.codes)
+        nmu = p.nmu if p.output_2d else 1
+        out = dict(pk=np.zeros(ns * 5 * p.nk), nmodes=np.zeros(p.nk, dtype=np.int64),
+                   pk2d=np.zeros(ns * p.nk * nmu), nmodes2d=np.zeros(p.nk * nmu, dtype=np.int64))
+        b = Bins(0, (C.c_int32 * 3)(), _dptr(out["pk"]), _iptr(out["nmodes"]),
+                 _dptr(out["pk2d"]), _iptr(out["nmodes2d"]))
+        return b, out
+
+    def bin(self):
+        """Raw per-bin sums: dict code -> {'pk': (5, nk), 'pk2d'}, plus 'nmodes'."""
+        b, out = self._bins_struct()
+        self.check(lib().capow_bin(self.ctx, C.byref(b)))
+        return self._unpack(b, out)
+
+    def _unpack(self, b, out):
+        p = self.p
+        ns = b.n_spectra
+        nmu = p.nmu if p.output_2d else 1
+        res = {"nmodes": out["nmodes"], "nmodes2d": out["nmodes2d"].reshape(p.nk, nmu),
+               "codes": list(b.codes[:ns])}
+        pk = out["pk"].reshape(ns, 5, p.nk)
+        pk2d = out["pk2d"].reshape(ns, p.nk, nmu)
+        for s in range(ns):
+            res[b.codes[s]] = {"pk": pk[s].copy(), "pk2d": pk2d[s].copy()}
+        return res
+
+    def power(self):
+        st = AssignStats()
+        b, out = self._bins_struct()
+        self.check(lib().capow_power(self.ctx, C.byref(st), C.byref(b)))
+        self.stats = st
+        return self._unpack(b, out)
+
+    def grid_elems(self):
+        n = C.c_int64()
+        self.check(lib().capow_grid_elems(self.ctx, C.byref(n)))
+        return n.value
+
+    def get_grid(self, slot):
+        a = np.empty(self.grid_elems(), dtype=self.dtype)
+        self.check(lib().capow_get_grid(self.ctx, slot, a.ctypes.data_as(_V), a.size))
+        return a
+
+    def set_grid(self, slot, arr):
+        a = np.ascontiguousarray(arr, dtype=self.dtype).ravel()
+        self.check(lib().capow_set_grid(self.ctx, slot, a.ctypes.data_as(_V), a.size))
+
+    def mark_assigned(self, stats=None):
+        self.check(lib().capow_mark_assigned(self.ctx, C.byref(stats) if stats else None))
+
+    def mark_transformed(self):
+        self.check(lib().capow_mark_transformed(self.ctx))
+
+    def synchronize(self):
+        self.check(lib().capow_synchronize(self.ctx))
+
+    def kernel_times(self):
+        names = C.create_string_buffer(4096)
+        ms = (C.c_double * 64)()
+        la = (C.c_int32 * 64)()
+        n = C.c_int32()
+        self.check(lib().capow_kernel_times(self.ctx, names, 4096, ms, la, 64, C.byref(n)))
+        labels = names.raw.split(b"\0")
+        return {labels[i].decode(): (ms[i], la[i]) for i in range(n.value)}
+
+    def reset_kernel_times(self):
+        self.check(lib().capow_reset_kernel_times(self.ctx))
+
+
+def scalars(p, stats, code):
+    shot, norm = C.c_double(), C.c_double()
+    _check(None, lib().capow_scalars(C.byref(p), C.byref(stats), code, C.byref(shot),
+                                     C.byref(norm)))
+    return shot.value, norm.value
+
+
+def finalize(p, shot, norm, pk5, nmodes, pk2d=None, nmodes2d=None):
+    """In place on copies; returns (pk5, nmodes, pk2d, nmodes2d, last_bin)."""
+    pk5 = np.ascontiguousarray(pk5, dtype=np.float64).copy()
+    nmodes = np.ascontiguousarray(nmodes, dtype=np.int64).copy()
+    pk2d = None if pk2d is None else np.ascontiguousarray(pk2d, dtype=np.float64).copy()
+    nmodes2d = None if nmodes2d is None else np.ascontiguousarray(nmodes2d, dtype=np.int64).copy()
+    last = C.c_int32()
+    _check(None, lib().capow_finalize(C.byref(p), shot, norm, _dptr(pk5), _iptr(nmodes),
+                                      _dptr(pk2d) if pk2d is not None else None,
+                                      _iptr(nmodes2d) if nmodes2d is not None else None,
+                                      C.byref(last)))
+    return pk5, nmodes, pk2d, nmodes2d, last.value
+
+
+def write_output(p, shot, norm, pk5, nmodes, path, pk2d=None, nmodes2d=None, precise=False):
+    pk5 = np.ascontiguousarray(pk5, dtype=np.float64)
+    nmodes = np.ascontiguousarray(nmodes, dtype=np.int64)
+    _check(None, lib().capow_write_output(
+        C.byref(p), shot, norm, _dptr(pk5), _iptr(nmodes),
+        _dptr(np.ascontiguousarray(pk2d, dtype=np.float64)) if pk2d is not None else None,
+        _iptr(np.ascontiguousarray(nmodes2d, dtype=np.int64)) if nmodes2d is not None else None,
+        path.encode(), 1 if precise else 0))

@@ -1,0 +1,218 @@
+// binning.hip -- fused deconvolution, interlace combine, power and Legendre
+// binning (K4) for gfx950.
+//
+// Replaces compute_periodic_power (reference src/main.c:302-468).  One pass
+// over the complex half-grid [NX][NY][NZ/2+1] of every deposited grid; per
+// mode:
+//   f = per-axis frequency with the reference's float casts (tables built on
+//       the host exactly as src/main.c:329-352 computes them),
+//   fktot = 2 pi sqrt(fx^2+fy^2+fz^2), mu, kbin = (int)((float)(fktot-Mink)
+//       /binwidth) -- all in double with no contraction, so bin indices and
+//       mode counts are bit-identical to the reference's x86 build,
+//   grid_cor = prod_axis (1/sinc)^(2 InterpOrder) (x 0.25 with interlacing),
+//   delta += delta_int * e^{i(ax+ay+az)} (per-axis phase tables),
+//   power of each requested spectrum, NM Hermitian weight, L_l(mu).
+// Accumulation: every wavefront owns a private copy of the bins in LDS when
+// they fit (adds in program order => bit-reproducible), else the workgroup
+// shares one copy; workgroup partials are then summed in a fixed order.
+#include "common.h"
+#include "kernels.h"
+
+namespace capow {
+
+template <typename T>
+struct CT;
+template <>
+struct CT<float> { typedef float2 t; };
+template <>
+struct CT<double> { typedef double2 t; };
+
+__device__ __forceinline__ double dmul(double a, double b) { return __dmul_rn(a, b); }
+__device__ __forceinline__ double dadd(double a, double b) { return __dadd_rn(a, b); }
+__device__ __forceinline__ double ddiv(double a, double b) { return __ddiv_rn(a, b); }
+
+template <typename T>
+__device__ __forceinline__ void bin_mode(int i, uint32_t r, const BinArgsDev& a,
+                                         const typename CT<T>::t* __restrict__ A,
+                                         const typename CT<T>::t* __restrict__ Ai,
+                                         const typename CT<T>::t* __restrict__ B,
+                                         const typename CT<T>::t* __restrict__ Bi,
+                                         double* accS, double* acc2, double* cnt, double* cnt2,
+                                         int nl, int n2d, uint32_t plane) {
+  typedef typename CT<T>::t C;
+  const int NK = a.nk;
+  const int NZH = a.nzh;
+  const double twopi = 2.0 * 3.14159265358979323846;  // the reference's 2.0*M_PI
+    const uint32_t j = a.fd_nzh.div(r);
+    const int k = (int)(r - j * (uint32_t)NZH);
+    const double fx = a.fx[i], fy = a.fy[j], fz = a.fz[k];
+    const double fktot = dmul(twopi, sqrt(dadd(dadd(dmul(fx, fx), dmul(fy, fy)), dmul(fz, fz))));
+    double mu;
+    if (fktot > 0) {
+      double fl = a.los == 1 ? fx : (a.los == 2 ? fy : fz);
+      mu = ddiv(dmul(twopi, fl), fktot);
+    } else {
+      mu = 0.0;
+    }
+    int kbin;
+    if (a.logbins) {
+      if (!(fktot > 0)) return;
+      kbin = (int)ddiv((double)__double2float_rn(log10(fktot) - a.mink), a.binwidth);
+    } else {
+      kbin = (int)ddiv((double)__double2float_rn(fktot - a.mink), a.binwidth);
+    }
+    if (!((kbin >= 0) && (kbin < NK) && (fktot <= a.min_nyq))) return;
+
+    // Hermitian weight (src/main.c:422-429)
+    double NM = 2.0;
+    if (k == 0) {
+      if (j == 0) {
+        if ((i != 0) && (i != a.nx / 2)) NM = 1.0;
+      } else {
+        if (i != a.nx / 2) NM = 1.0;
+      }
+    }
+    double cor = a.corx[i] * a.cory[j] * a.corz[k];
+    const int64_t idx = (int64_t)i * plane + r;
+    C va = A[idx];
+    double ar = (double)va.x, ai = (double)va.y, br = 0, bi = 0;
+    double cr = 1, ci = 0;
+    if (a.interlace) {
+      cor *= 0.25;
+      // e^{i(ax+ay+az)} from the per-axis phases
+      double2 px = a.phx[i], py = a.phy[j], pz = a.phz[k];
+      double qr = px.x * py.x - px.y * py.y, qi = px.x * py.y + px.y * py.x;
+      cr = qr * pz.x - qi * pz.y;
+      ci = qr * pz.y + qi * pz.x;
+      C vai = Ai[idx];
+      ar += (double)vai.x * cr - (double)vai.y * ci;
+      ai += (double)vai.x * ci + (double)vai.y * cr;
+    }
+    if (a.hasB) {
+      C vb = B[idx];
+      br = (double)vb.x;
+      bi = (double)vb.y;
+      if (a.interlace) {
+        C vbi = Bi[idx];
+        br += (double)vbi.x * cr - (double)vbi.y * ci;
+        bi += (double)vbi.x * ci + (double)vbi.y * cr;
+      }
+    }
+    const double mu2 = mu * mu;
+    const double L2 = 1.5 * mu * mu - 0.5;
+    const double L4 = 4.375 * mu * mu * mu * mu - 3.75 * mu * mu + 0.375;
+    const double L3 = 2.5 * mu * mu * mu - 1.5 * mu;
+    (void)mu2;
+    int mubin = 0;
+    bool in2d = false;
+    if (a.out2d) {
+      mubin = (int)(((mu + 1.0) / 2.0) * (double)(float)a.nmu);
+      if (mubin >= a.nmu) mubin = a.nmu - 1;
+      int b2 = kbin * a.nmu + mubin;
+      in2d = (b2 >= 0) && (b2 < NK * a.nmu);
+      mubin = b2;
+    }
+    for (int s = 0; s < a.nspec; s++) {
+      double p;
+      switch (a.kind[s]) {
+        case SPEC_AUTO_A: p = ar * ar + ai * ai; break;
+        case SPEC_AUTO_B: p = br * br + bi * bi; break;
+        case SPEC_CROSS_IM: p = ar * bi - ai * br; break;
+        default: p = ar * br + ai * bi; break;
+      }
+      p *= cor;
+      const double np = NM * p;
+      double* b = accS + s * nl * NK;
+      atomicAdd(&b[0 * NK + kbin], np);
+      atomicAdd(&b[1 * NK + kbin], NM * L2 * p);
+      atomicAdd(&b[2 * NK + kbin], NM * L4 * p);
+      if (a.odd) {
+        atomicAdd(&b[3 * NK + kbin], NM * mu * p);
+        atomicAdd(&b[4 * NK + kbin], NM * L3 * p);
+      }
+      if (in2d) atomicAdd(&acc2[s * n2d + mubin], np);
+    }
+    atomicAdd(&cnt[kbin], NM);
+    if (in2d) atomicAdd(&cnt2[mubin], NM);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_bin(const T* __restrict__ gA, const T* __restrict__ gAi,
+                                             const T* __restrict__ gB, const T* __restrict__ gBi,
+                                             BinArgsDev a, double* __restrict__ partials) {
+  typedef typename CT<T>::t C;
+  extern __shared__ double sm[];
+  const int NK = a.nk;
+  const int nl = a.odd ? 5 : 3;
+  const int F = a.nspec * nl;                       // sums per bin
+  const int n2d = a.out2d ? a.nk * a.nmu : 0;
+  const int per_copy = F * NK + a.nspec * n2d + NK + n2d;  // doubles (counts as double)
+  const int wave = threadIdx.x >> 6;
+  double* my = sm + (wave % a.copies) * per_copy;
+  for (int i = threadIdx.x; i < per_copy * a.copies; i += blockDim.x) sm[i] = 0.0;
+  __syncthreads();
+  double* accS = my;                       // [s][l'][NK]
+  double* acc2 = my + F * NK;              // [s][NK*NMU]
+  double* cnt = acc2 + a.nspec * n2d;      // [NK]
+  double* cnt2 = cnt + NK;                 // [NK*NMU]
+
+  const int NY = a.ny, NZH = a.nzh;
+  const uint32_t plane = (uint32_t)NY * (uint32_t)NZH;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const C* A = reinterpret_cast<const C*>(gA);
+  const C* Ai = reinterpret_cast<const C*>(gAi);
+  const C* B = reinterpret_cast<const C*>(gB);
+  const C* Bi = reinterpret_cast<const C*>(gBi);
+
+  {
+    const int64_t m0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int i = (int)(m0 / plane);
+    uint32_t r = (uint32_t)(m0 - (int64_t)i * plane);
+    const int si = (int)(stride / plane);
+    const uint32_t sr = (uint32_t)(stride - (int64_t)si * plane);
+    while (i < a.nx) {
+      bin_mode<T>(i, r, a, A, Ai, B, Bi, accS, acc2, cnt, cnt2, nl, n2d, plane);
+      i += si;
+      r += sr;
+      if (r >= plane) { r -= plane; i++; }
+    }
+  }
+  __syncthreads();
+  // sum the copies in a fixed order and write this workgroup's partial
+  double* out = partials + (int64_t)blockIdx.x * per_copy;
+  for (int q = threadIdx.x; q < per_copy; q += blockDim.x) {
+    double t = 0.0;
+    for (int c = 0; c < a.copies; c++) t += sm[c * per_copy + q];
+    out[q] = t;
+  }
+}
+
+// Fixed-order sum over workgroup partials.
+__global__ void k_bin_final(const double* __restrict__ partials, int nwg, int per_copy,
+                            double* __restrict__ out) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= per_copy) return;
+  double t = 0.0;
+  for (int w = 0; w < nwg; w++) t += partials[(int64_t)w * per_copy + q];
+  out[q] = t;
+}
+
+template <typename T>
+static int run_bin_t(BinLaunch& L) {
+  const int per_copy = L.per_copy;
+  const size_t lds = (size_t)per_copy * L.dev.copies * sizeof(double);
+  if (lds > 64 * 1024)
+    hipFuncSetAttribute((const void*)k_bin<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  TIMED(L.timer, "bin",
+        (k_bin<T><<<L.nwg, 256, lds, L.stream>>>((const T*)L.grid[0], (const T*)L.grid[1],
+                                                 (const T*)L.grid[2], (const T*)L.grid[3], L.dev,
+                                                 L.partials)));
+  TIMED(L.timer, "bin_final",
+        (k_bin_final<<<(per_copy + 255) / 256, 256, 0, L.stream>>>(L.partials, L.nwg, per_copy,
+                                                                    L.out)));
+  return hipGetLastError() == hipSuccess ? CAPOW_OK : CAPOW_ERR_HIP;
+}
+
+int run_bin(BinLaunch& L, bool fp64) { return fp64 ? run_bin_t<double>(L) : run_bin_t<float>(L); }
+
+}  // namespace capow

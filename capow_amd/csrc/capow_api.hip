@@ -1,0 +1,642 @@
+// capow_api.hip -- the C ABI of include/capow.h: context, validation,
+// device buffers, tables and stage launches.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "capow.h"
+#include "common.h"
+#include "kernels.h"
+
+using namespace capow;
+
+struct capow_ctx {
+  capow_params p;
+  Geom g;
+  bool fp64 = false;
+  size_t esz = 4;
+  hipStream_t stream = nullptr;
+  Timer timer;
+  std::string err;
+  int state = 0;  // 0 created, 1 particles, 2 assigned, 3 transformed
+  // grids: slots 0 A, 1 A_int, 2 B, 3 B_int
+  void* grid[4] = {nullptr, nullptr, nullptr, nullptr};
+  int64_t grid_elems = 0;
+  // particles
+  const void* parts = nullptr;
+  const void* wts = nullptr;
+  int64_t n = 0;
+  void* own_parts = nullptr;
+  size_t own_parts_cap = 0;
+  void* own_w = nullptr;
+  size_t own_w_cap = 0;
+  void* sorted = nullptr;
+  size_t sorted_cap = 0;
+  void* sorted_w = nullptr;
+  size_t sorted_w_cap = 0;
+  // buckets / sort
+  uint32_t *counts = nullptr, *offsets = nullptr, *cursor = nullptr, *bsum = nullptr,
+           *boff = nullptr;
+  int count_blocks = 2048;
+  double* partials = nullptr;
+  unsigned long long* gmax = nullptr;
+  unsigned int* errs = nullptr;
+  double* stats = nullptr;
+  // fft
+  FftPlan plan[3];
+  void* tw[3] = {nullptr, nullptr, nullptr};
+  // binning
+  BinArgsDev bin;
+  double* bin_tabs = nullptr;
+  double* bin_partials = nullptr;
+  double* bin_out = nullptr;
+  int bin_nwg = 1024;
+  int bin_per_copy = 0;
+  int nspec = 1;
+  int codes[3] = {0, 0, 0};
+  capow_assign_stats last{};
+};
+
+static std::string g_create_err;
+
+#define HIPCHK(ctx, call)                                                          \
+  do {                                                                             \
+    hipError_t _e = (call);                                                        \
+    if (_e != hipSuccess) {                                                        \
+      (ctx)->err = std::string(#call) + ": " + hipGetErrorString(_e);              \
+      return CAPOW_ERR_HIP;                                                        \
+    }                                                                              \
+  } while (0)
+
+static int fail(capow_ctx* c, int code, const std::string& m) {
+  if (c) c->err = m;
+  else g_create_err = m;
+  return code;
+}
+
+// Field exponents per Momentum (src/read_data.c:91-130): A = w v^a, B = w v^b.
+static void field_exps(int m, int* a, int* b) {
+  static const int ta[9] = {0, 0, 1, 0, 1, 2, 0, 1, 0};
+  static const int tb[9] = {-1, 1, -1, 2, 2, -1, 3, 3, 4};
+  *a = ta[m];
+  *b = tb[m];
+}
+
+static int auto_code(int e) { return e == 0 ? 0 : (e == 1 ? 2 : (e == 2 ? 5 : -1)); }
+
+extern "C" int capow_spectra(const capow_params* p, int32_t* n_spectra, int32_t codes[3]) {
+  if (!p || p->momentum < 0 || p->momentum > 8) return CAPOW_ERR_ARG;
+  int a, b;
+  field_exps(p->momentum, &a, &b);
+  int n = 0;
+  codes[n++] = p->momentum;
+  if (b >= 0 && (p->extra_spectra & CAPOW_SPEC_AUTO_A)) codes[n++] = auto_code(a);
+  if (b >= 0 && b <= 2 && (p->extra_spectra & CAPOW_SPEC_AUTO_B)) codes[n++] = auto_code(b);
+  *n_spectra = n;
+  return CAPOW_OK;
+}
+
+static int spec_kind(int code, int momentum) {
+  // the kind of combination a spectrum code needs from the (A, B) fields
+  if (code == momentum) {
+    if (code == 0 || code == 2 || code == 5) return SPEC_AUTO_A;
+    if (code == 1 || code == 4 || code == 6) return SPEC_CROSS_IM;
+    return SPEC_CROSS_RE;
+  }
+  int a, b;
+  field_exps(momentum, &a, &b);
+  return code == auto_code(a) ? SPEC_AUTO_A : SPEC_AUTO_B;
+}
+
+static bool make_plan(int64_t M, FftPlan* pl) {
+  pl->npass = 0;
+  static const int rad[5] = {8, 4, 2, 3, 5};
+  for (int q = 0; q < 5; q++) {
+    while (M % rad[q] == 0 && M > 1) {
+      if (pl->npass >= 24) return false;
+      pl->radix[pl->npass++] = rad[q];
+      M /= rad[q];
+    }
+  }
+  return M == 1;
+}
+
+static int pick_tile(int n, int tmax) {
+  if (n <= tmax) return n;
+  for (int t = tmax; t >= 4; t--)
+    if (n % t == 0) return t;
+  return -1;
+}
+
+// check_inputs (src/read_params.c:294-428) for the periodic tags, plus the
+// shapes the kernels handle.
+static int validate(const capow_params* p, std::string& m) {
+  if (p->los < 0 || p->los > 3) { m = "`LOS' is neither 0 (real-space), 1, 2 nor 3."; return CAPOW_ERR_ARG; }
+  if (p->interp_order < 1 || p->interp_order > 3) { m = "`InterpOrder' is neither 1, 2 nor 3."; return CAPOW_ERR_ARG; }
+  if (p->do_interlacing != 0 && p->do_interlacing != 1) { m = "`DoInterlacing' is neither 0 nor 1."; return CAPOW_ERR_ARG; }
+  if (p->output_2d != 0 && p->output_2d != 1) { m = "`Output2D' is neither 0 nor 1."; return CAPOW_ERR_ARG; }
+  if (p->output_2d && p->nmu <= 0) { m = "`NMU' is less than 1."; return CAPOW_ERR_ARG; }
+  if (p->output_log != 0 && p->output_log != 1) { m = "`OutputLog' is neither 0 nor 1."; return CAPOW_ERR_ARG; }
+  if (p->nx <= 0 || p->ny <= 0 || p->nz <= 0) { m = "`NX', `NY' or `NZ' is less than 1."; return CAPOW_ERR_ARG; }
+  if (p->nk <= 0) { m = "`NK' is less than 1."; return CAPOW_ERR_ARG; }
+  if (p->xmax < p->xmin || p->ymax < p->ymin || p->zmax < p->zmin) { m = "`XMAX' is less than `XMIN'."; return CAPOW_ERR_ARG; }
+  if (p->maxk < p->mink) { m = "`Maxk' is less than `Mink'."; return CAPOW_ERR_ARG; }
+  if (p->momentum < 0 || p->momentum > 8) { m = "`Momentum' must be 0..8."; return CAPOW_ERR_ARG; }
+  if (p->precision != CAPOW_FP32 && p->precision != CAPOW_FP64) { m = "precision must be CAPOW_FP32 or CAPOW_FP64."; return CAPOW_ERR_ARG; }
+  if (p->edge_mode != CAPOW_EDGE_REFERENCE && p->edge_mode != CAPOW_EDGE_PERIODIC) { m = "edge_mode must be CAPOW_EDGE_REFERENCE or CAPOW_EDGE_PERIODIC."; return CAPOW_ERR_ARG; }
+  if (p->nz % 2 != 0) { m = "NZ must be even (r2c row transform)."; return CAPOW_ERR_UNSUPPORTED; }
+  if (p->nx > 65536 || p->ny > 65536 || p->nz > 65536) { m = "grid dimension above 65536."; return CAPOW_ERR_UNSUPPORTED; }
+  FftPlan pl;
+  if (!make_plan(p->nx, &pl) || !make_plan(p->ny, &pl) || !make_plan(p->nz / 2, &pl)) {
+    m = "grid dimensions must factor into 2, 3 and 5";
+    return CAPOW_ERR_UNSUPPORTED;
+  }
+  const int64_t dims[3] = {p->nx, p->ny, p->nz};
+  const int tmax[3] = {8, 8, 32};
+  for (int a = 0; a < 3; a++) {
+    if (pick_tile((int)dims[a], tmax[a]) < 0) {
+      m = "grid dimension has no tile divisor in [4, 32]";
+      return CAPOW_ERR_UNSUPPORTED;
+    }
+  }
+  return CAPOW_OK;
+}
+
+template <typename T>
+static std::vector<T> twiddles(int64_t N) {
+  std::vector<T> v(2 * N);
+  for (int64_t j = 0; j < N; j++) {
+    double a = 2.0 * M_PI * (double)j / (double)N;
+    v[2 * j] = (T)cos(a);
+    v[2 * j + 1] = (T)(-sin(a));
+  }
+  return v;
+}
+
+static int setup(capow_ctx* c) {
+  const capow_params& p = c->p;
+  Geom& g = c->g;
+  memset(&g, 0, sizeof(g));
+  g.n[0] = (int)p.nx; g.n[1] = (int)p.ny; g.n[2] = (int)p.nz;
+  g.nzh = (int)(p.nz / 2 + 1);
+  g.rowlen = 2 * (int64_t)g.nzh;
+  const double lo[3] = {p.xmin, p.ymin, p.zmin}, hi[3] = {p.xmax, p.ymax, p.zmax};
+  for (int a = 0; a < 3; a++) {
+    g.lo[a] = lo[a];
+    g.hi[a] = hi[a];
+    g.L[a] = hi[a] - lo[a];
+    g.d[a] = (hi[a] - lo[a]) / (double)(unsigned long long)g.n[a];  // src/main.c:1126-1128
+  }
+  g.order = p.interp_order;
+  g.interlace = p.do_interlacing;
+  g.los = p.los;
+  g.edge = p.edge_mode;
+  g.periodic = (p.edge_mode == CAPOW_EDGE_PERIODIC) || (p.los == 0);
+  field_exps(p.momentum, &g.fa, &g.fb);
+  g.mean_sub_a = (g.fa == 0);
+  g.rsd_axis = (p.los >= 1 && p.los <= 3) ? p.los - 1 : -1;
+  g.rsd_f1 = 1.0 + p.redshift;
+  g.rsd_den = 100.0 * sqrt(p.omega_m * pow((1.0 + p.redshift), 3.0) + 1.0 - p.omega_m);
+  if (g.order == 3) { g.reach_lo = -1; g.reach_hi = g.interlace ? 2 : 1; }
+  else if (g.order == 2) { g.reach_lo = 0; g.reach_hi = g.interlace ? 2 : 1; }
+  else { g.reach_lo = 0; g.reach_hi = g.interlace ? 1 : 0; }
+  const int tmax[3] = {8, 8, 32};
+  for (int a = 0; a < 3; a++) {
+    int t = pick_tile(g.n[a], tmax[a]);
+    if (t < 0) return fail(c, CAPOW_ERR_UNSUPPORTED, "grid dimension has no tile divisor in [4, 32]");
+    g.tdim[a] = t;
+    g.ntile[a] = g.n[a] / t;
+  }
+  g.nbuckets = (int64_t)g.ntile[0] * g.ntile[1] * g.ntile[2] * 27;
+  c->fp64 = p.precision == CAPOW_FP64;
+  c->esz = c->fp64 ? 8 : 4;
+
+  // grids
+  c->grid_elems = (int64_t)p.nx * p.ny * g.rowlen;
+  const bool hasB = g.fb >= 0;
+  for (int s = 0; s < 4; s++) {
+    bool need = (s == 0) || (s == 1 && g.interlace) || (s == 2 && hasB) || (s == 3 && hasB && g.interlace);
+    if (need) HIPCHK(c, hipMalloc(&c->grid[s], (size_t)c->grid_elems * c->esz));
+  }
+  // sort buffers
+  HIPCHK(c, hipMalloc((void**)&c->counts, (size_t)g.nbuckets * 4));
+  HIPCHK(c, hipMalloc((void**)&c->offsets, (size_t)(g.nbuckets + 1) * 4));
+  HIPCHK(c, hipMalloc((void**)&c->cursor, (size_t)g.nbuckets * 4));
+  int64_t nblk = (g.nbuckets + 4095) / 4096;
+  HIPCHK(c, hipMalloc((void**)&c->bsum, (size_t)(nblk + 1) * 4));
+  HIPCHK(c, hipMalloc((void**)&c->boff, (size_t)(nblk + 1) * 4));
+  HIPCHK(c, hipMalloc((void**)&c->partials, (size_t)c->count_blocks * STAT_N * 8));
+  HIPCHK(c, hipMalloc((void**)&c->gmax, 2 * sizeof(unsigned long long)));
+  HIPCHK(c, hipMalloc((void**)&c->errs, 4 * sizeof(unsigned int)));
+  HIPCHK(c, hipMalloc((void**)&c->stats, STAT_TOTAL * 8));
+
+  // FFT plans + twiddles
+  if (!make_plan(p.nx, &c->plan[0]) || !make_plan(p.ny, &c->plan[1]) ||
+      !make_plan(p.nz / 2, &c->plan[2]))
+    return fail(c, CAPOW_ERR_UNSUPPORTED, "grid dimensions must factor into 2, 3 and 5");
+  if (p.nx * 8 > 8192 * 2 || p.ny * 8 > 8192 * 2 || (p.nz / 2) > 8192)
+    return fail(c, CAPOW_ERR_UNSUPPORTED, "FFT length above the per-workgroup limit");
+  const int64_t tn[3] = {p.nx, p.ny, p.nz};
+  for (int a = 0; a < 3; a++) {
+    if (c->fp64) {
+      auto v = twiddles<double>(tn[a]);
+      HIPCHK(c, hipMalloc(&c->tw[a], v.size() * 8));
+      HIPCHK(c, hipMemcpy(c->tw[a], v.data(), v.size() * 8, hipMemcpyHostToDevice));
+    } else {
+      auto v = twiddles<float>(tn[a]);
+      HIPCHK(c, hipMalloc(&c->tw[a], v.size() * 4));
+      HIPCHK(c, hipMemcpy(c->tw[a], v.data(), v.size() * 4, hipMemcpyHostToDevice));
+    }
+  }
+
+  // binning tables, built exactly as src/main.c:305-352 / 382-389
+  {
+    const unsigned long long NX = p.nx, NY = p.ny, NZ = p.nz;
+    const double dx = g.d[0], dy = g.d[1], dz = g.d[2];
+    double sx = 1.0 / ((float)NX * dx), sy = 1.0 / ((float)NY * dy), sz = 1.0 / ((float)NZ * dz);
+    const int nzh = g.nzh;
+    // layout: fx[NX] fy[NY] fz[nzh] corx corY corz phx(2NX) phy(2NY) phz(2nzh)
+    std::vector<double> tab;
+    size_t ofx = 0, ofy = NX, ofz = NX + NY, ocx = ofz + nzh, ocy = ocx + NX, ocz = ocy + NY,
+           opx = ocz + nzh, opy = opx + 2 * NX, opz = opy + 2 * NY, total = opz + 2 * nzh;
+    tab.resize(total);
+    const int po = 2 * p.interp_order;
+    auto fill = [&](int axis, size_t of, size_t oc, size_t op, unsigned long long N, int cnt,
+                    double s, double d) {
+      for (int i = 0; i < cnt; i++) {
+        double f;
+        if (axis == 2) f = (float)i * s;
+        else if ((unsigned long long)i <= N / 2) f = (float)i * s;
+        else f = ((float)i - (float)N) * s;
+        tab[of + i] = f;
+        double a = M_PI * f * d;
+        double sn = 1.0;
+        if (f != 0.0) sn = sin(a) / a;
+        tab[oc + i] = pow(1.0 / sn, po);
+        tab[op + 2 * i] = cos(a);
+        tab[op + 2 * i + 1] = sin(a);
+      }
+    };
+    fill(0, ofx, ocx, opx, NX, (int)NX, sx, dx);
+    fill(1, ofy, ocy, opy, NY, (int)NY, sy, dy);
+    fill(2, ofz, ocz, opz, NZ, nzh, sz, dz);
+    HIPCHK(c, hipMalloc((void**)&c->bin_tabs, total * 8));
+    HIPCHK(c, hipMemcpy(c->bin_tabs, tab.data(), total * 8, hipMemcpyHostToDevice));
+    BinArgsDev& b = c->bin;
+    memset(&b, 0, sizeof(b));
+    b.nx = (int)NX; b.ny = (int)NY; b.nzh = nzh;
+    b.fd_nzh.init((uint32_t)nzh);
+    b.nk = p.nk; b.nmu = p.output_2d ? p.nmu : 1;
+    b.out2d = p.output_2d; b.odd = p.odd_multipoles; b.logbins = p.output_log;
+    b.los = p.los; b.interlace = p.do_interlacing; b.hasB = hasB;
+    b.mink = p.mink;
+    b.binwidth = (p.maxk - p.mink) / (double)p.nk;  // src/main.c:305
+    double nyx = M_PI / dx, nyy = M_PI / dy, nyz = M_PI / dz, mn = nyx;
+    if (nyy < mn) mn = nyy;
+    if (nyz < mn) mn = nyz;
+    b.min_nyq = mn;
+    b.fx = c->bin_tabs + ofx; b.fy = c->bin_tabs + ofy; b.fz = c->bin_tabs + ofz;
+    b.corx = c->bin_tabs + ocx; b.cory = c->bin_tabs + ocy; b.corz = c->bin_tabs + ocz;
+    b.phx = (const double2*)(c->bin_tabs + opx);
+    b.phy = (const double2*)(c->bin_tabs + opy);
+    b.phz = (const double2*)(c->bin_tabs + opz);
+    int32_t ns, codes[3];
+    capow_spectra(&p, &ns, codes);
+    c->nspec = ns;
+    b.nspec = ns;
+    for (int s = 0; s < ns; s++) {
+      c->codes[s] = codes[s];
+      b.kind[s] = spec_kind(codes[s], p.momentum);
+    }
+    const int nl = p.odd_multipoles ? 5 : 3;
+    const int n2d = p.output_2d ? p.nk * p.nmu : 0;
+    c->bin_per_copy = ns * nl * p.nk + ns * n2d + p.nk + n2d;
+    const size_t bytes = (size_t)c->bin_per_copy * 8;
+    const size_t lds_max = 160 * 1024;
+    if (bytes > lds_max)
+      return fail(c, CAPOW_ERR_UNSUPPORTED, "NK/NMU too large for the LDS bin arrays");
+    b.copies = bytes * 4 <= lds_max ? 4 : (bytes * 2 <= lds_max ? 2 : 1);
+    int64_t modes = (int64_t)NX * NY * nzh;
+    int64_t want = (modes + 255) / 256;
+    c->bin_nwg = (int)(want < 1024 ? want : 1024);
+    HIPCHK(c, hipMalloc((void**)&c->bin_partials, (size_t)c->bin_nwg * bytes));
+    HIPCHK(c, hipMalloc((void**)&c->bin_out, bytes));
+  }
+  return CAPOW_OK;
+}
+
+static void release(capow_ctx* c) {
+  for (int s = 0; s < 4; s++) if (c->grid[s]) hipFree(c->grid[s]);
+  void* ps[] = {c->own_parts, c->own_w, c->sorted, c->sorted_w, c->counts, c->offsets,
+                c->cursor, c->bsum, c->boff, c->partials, c->gmax, c->errs, c->stats,
+                c->tw[0], c->tw[1], c->tw[2], c->bin_tabs, c->bin_partials, c->bin_out};
+  for (void* q : ps) if (q) hipFree(q);
+}
+
+extern "C" int capow_create(const capow_params* p, capow_ctx** out) {
+  if (!p || !out) return fail(nullptr, CAPOW_ERR_ARG, "null argument");
+  *out = nullptr;
+  std::string m;
+  int rc = validate(p, m);
+  if (rc) return fail(nullptr, rc, m);
+  capow_ctx* c = new capow_ctx();
+  c->p = *p;
+  rc = setup(c);
+  if (rc) {
+    g_create_err = c->err;
+    release(c);
+    delete c;
+    return rc;
+  }
+  *out = c;
+  return CAPOW_OK;
+}
+
+extern "C" int capow_destroy(capow_ctx* c) {
+  if (!c) return CAPOW_OK;
+  hipStreamSynchronize(c->stream);
+  release(c);
+  delete c;
+  return CAPOW_OK;
+}
+
+extern "C" const char* capow_last_error(const capow_ctx* c) {
+  return c ? c->err.c_str() : g_create_err.c_str();
+}
+
+extern "C" int capow_set_stream(capow_ctx* c, void* s) {
+  if (!c) return CAPOW_ERR_ARG;
+  c->stream = (hipStream_t)s;
+  c->timer.stream = c->stream;
+  return CAPOW_OK;
+}
+
+extern "C" int capow_synchronize(capow_ctx* c) {
+  if (!c) return CAPOW_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->timer.collect();
+  return CAPOW_OK;
+}
+
+extern "C" int capow_set_timing(capow_ctx* c, int on) {
+  if (!c) return CAPOW_ERR_ARG;
+  c->timer.on = on != 0;
+  c->timer.stream = c->stream;
+  return CAPOW_OK;
+}
+
+static int ensure(capow_ctx* c, void** buf, size_t* cap, size_t bytes) {
+  if (*cap >= bytes) return CAPOW_OK;
+  if (*buf) hipFree(*buf);
+  *buf = nullptr;
+  *cap = 0;
+  HIPCHK(c, hipMalloc(buf, bytes));
+  *cap = bytes;
+  return CAPOW_OK;
+}
+
+extern "C" int capow_set_particles(capow_ctx* c, const void* xyzv, const void* w, int64_t n,
+                                   int32_t dtype, int32_t where) {
+  if (!c) return CAPOW_ERR_ARG;
+  if (n < 0 || (n > 0 && !xyzv)) return fail(c, CAPOW_ERR_ARG, "bad particle array");
+  if ((dtype == CAPOW_DTYPE_F64) != c->fp64)
+    return fail(c, CAPOW_ERR_ARG, "particle dtype must match the context precision");
+  if (n > 0xFFFFFFF0LL) return fail(c, CAPOW_ERR_UNSUPPORTED, "more than 2^32-16 particles");
+  const size_t pb = (size_t)n * 4 * c->esz, wb = (size_t)n * c->esz;
+  if (where == CAPOW_HOST) {
+    int rc = ensure(c, &c->own_parts, &c->own_parts_cap, pb > 0 ? pb : 16);
+    if (rc) return rc;
+    if (n) HIPCHK(c, hipMemcpyAsync(c->own_parts, xyzv, pb, hipMemcpyHostToDevice, c->stream));
+    c->parts = c->own_parts;
+    if (w) {
+      rc = ensure(c, &c->own_w, &c->own_w_cap, wb > 0 ? wb : 16);
+      if (rc) return rc;
+      if (n) HIPCHK(c, hipMemcpyAsync(c->own_w, w, wb, hipMemcpyHostToDevice, c->stream));
+      c->wts = c->own_w;
+    } else {
+      c->wts = nullptr;
+    }
+  } else if (where == CAPOW_DEVICE) {
+    c->parts = xyzv;
+    c->wts = w;
+  } else {
+    return fail(c, CAPOW_ERR_ARG, "where must be CAPOW_HOST or CAPOW_DEVICE");
+  }
+  c->n = n;
+  int rc = ensure(c, &c->sorted, &c->sorted_cap, pb > 0 ? pb : 16);
+  if (rc) return rc;
+  if (w) {
+    rc = ensure(c, &c->sorted_w, &c->sorted_w_cap, wb > 0 ? wb : 16);
+    if (rc) return rc;
+  }
+  c->state = 1;
+  return CAPOW_OK;
+}
+
+static void fill_stats(capow_ctx* c, const double* s) {
+  capow_assign_stats& o = c->last;
+  o.nread = (double)c->n;
+  o.nout = s[STAT_NOUT];
+  o.ngrid = s[STAT_NGRID];
+  o.nsq = s[STAT_W];
+  o.vr = s[STAT_V1];
+  o.vr2 = s[STAT_V2];
+  o.vr3 = s[STAT_V3];
+  o.vr4 = s[STAT_V4];
+  o.alpha = s[STAT_ALPHA];
+}
+
+extern "C" int capow_assign(capow_ctx* c, capow_assign_stats* st) {
+  if (!c) return CAPOW_ERR_ARG;
+  if (c->state < 1) return fail(c, CAPOW_ERR_STATE, "capow_assign before capow_set_particles");
+  Geom& g = c->g;
+  g.has_w = c->wts != nullptr;
+  HIPCHK(c, hipMemsetAsync(c->counts, 0, (size_t)g.nbuckets * 4, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->gmax, 0, 2 * sizeof(unsigned long long), c->stream));
+  HIPCHK(c, hipMemsetAsync(c->errs, 0, 4 * sizeof(unsigned int), c->stream));
+  DepositArgs a;
+  a.g = &g;
+  a.stream = c->stream;
+  a.timer = &c->timer;
+  a.parts = c->parts;
+  a.wts = c->wts;
+  a.n = c->n;
+  int64_t want = (c->n + 255) / 256;
+  a.count_blocks = (int)(want < 1 ? 1 : (want < c->count_blocks ? want : c->count_blocks));
+  a.counts = c->counts;
+  a.offsets = c->offsets;
+  a.cursor = c->cursor;
+  a.bsum = c->bsum;
+  a.boff = c->boff;
+  a.sorted = c->sorted;
+  a.sorted_w = c->sorted_w;
+  a.partials = c->partials;
+  a.gmax = c->gmax;
+  a.errs = c->errs;
+  a.stats = c->stats;
+  for (int s = 0; s < 4; s++) a.grid[s] = c->grid[s];
+  int rc = run_deposit(a, c->fp64);
+  if (rc) return fail(c, rc, std::string("deposit launch failed: ") + hipGetErrorString(hipGetLastError()));
+  double s[STAT_TOTAL];
+  unsigned int e[4];
+  HIPCHK(c, hipMemcpyAsync(s, c->stats, sizeof s, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(e, c->errs, sizeof e, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->timer.collect();
+  fill_stats(c, s);
+  if (st) *st = c->last;
+  if (e[0]) {
+    c->state = 1;
+    return fail(c, CAPOW_ERR_OUT_OF_BOX,
+                "particle out of grid bounds with LOS = 0 (" + std::to_string(e[0]) + " particles)");
+  }
+  if (e[1]) return fail(c, CAPOW_ERR_INTERNAL, "deposit reach self-check failed");
+  c->state = 2;
+  return CAPOW_OK;
+}
+
+extern "C" int capow_fft(capow_ctx* c) {
+  if (!c) return CAPOW_ERR_ARG;
+  if (c->state < 2) return fail(c, CAPOW_ERR_STATE, "capow_fft before capow_assign");
+  if (c->state >= 3) return fail(c, CAPOW_ERR_STATE, "grids already transformed");
+  FftArgs a;
+  a.g = &c->g;
+  a.stream = c->stream;
+  a.timer = &c->timer;
+  a.ngrids = 0;
+  for (int s = 0; s < 4; s++)
+    if (c->grid[s]) a.grid[a.ngrids++] = c->grid[s];
+  for (int q = 0; q < 3; q++) {
+    a.plan[q] = c->plan[q];
+    a.tw[q] = c->tw[q];
+  }
+  int rc = run_fft(a, c->fp64);
+  if (rc) return fail(c, rc, std::string("fft launch failed: ") + hipGetErrorString(hipGetLastError()));
+  c->state = 3;
+  return CAPOW_OK;
+}
+
+extern "C" int capow_bin(capow_ctx* c, capow_bins* out) {
+  if (!c || !out) return CAPOW_ERR_ARG;
+  if (c->state < 3) return fail(c, CAPOW_ERR_STATE, "capow_bin before capow_fft");
+  BinLaunch L;
+  L.stream = c->stream;
+  L.timer = &c->timer;
+  for (int s = 0; s < 4; s++) L.grid[s] = c->grid[s] ? c->grid[s] : c->grid[0];
+  L.dev = c->bin;
+  L.nwg = c->bin_nwg;
+  L.per_copy = c->bin_per_copy;
+  L.partials = c->bin_partials;
+  L.out = c->bin_out;
+  int rc = run_bin(L, c->fp64);
+  if (rc) return fail(c, rc, std::string("binning launch failed: ") + hipGetErrorString(hipGetLastError()));
+  std::vector<double> h(c->bin_per_copy);
+  HIPCHK(c, hipMemcpyAsync(h.data(), c->bin_out, h.size() * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->timer.collect();
+  const capow_params& p = c->p;
+  const int nk = p.nk, nl = p.odd_multipoles ? 5 : 3, ns = c->nspec;
+  const int n2d = p.output_2d ? p.nk * p.nmu : 0;
+  out->n_spectra = ns;
+  for (int s = 0; s < 3; s++) out->codes[s] = s < ns ? c->codes[s] : -1;
+  // LDS order per spectrum: l = 0, 2, 4 [, 1, 3]
+  static const int lmap[5] = {0, 2, 4, 1, 3};
+  if (out->pk) {
+    for (int s = 0; s < ns; s++) {
+      for (int l = 0; l < 5; l++)
+        for (int k = 0; k < nk; k++) out->pk[(s * 5 + l) * nk + k] = 0.0;
+      for (int q = 0; q < nl; q++)
+        for (int k = 0; k < nk; k++)
+          out->pk[(s * 5 + lmap[q]) * nk + k] = h[(size_t)(s * nl + q) * nk + k];
+    }
+  }
+  const double* acc2 = h.data() + (size_t)ns * nl * nk;
+  const double* cnt = acc2 + (size_t)ns * n2d;
+  const double* cnt2 = cnt + nk;
+  if (out->nmodes)
+    for (int k = 0; k < nk; k++) out->nmodes[k] = (int64_t)cnt[k];
+  if (p.output_2d) {
+    if (out->pk2d)
+      for (int q = 0; q < ns * n2d; q++) out->pk2d[q] = acc2[q];
+    if (out->nmodes2d)
+      for (int q = 0; q < n2d; q++) out->nmodes2d[q] = (int64_t)cnt2[q];
+  }
+  return CAPOW_OK;
+}
+
+extern "C" int capow_power(capow_ctx* c, capow_assign_stats* st, capow_bins* out) {
+  int rc = capow_assign(c, st);
+  if (rc) return rc;
+  rc = capow_fft(c);
+  if (rc) return rc;
+  return capow_bin(c, out);
+}
+
+extern "C" int capow_grid_elems(const capow_ctx* c, int64_t* n) {
+  if (!c || !n) return CAPOW_ERR_ARG;
+  *n = c->grid_elems;
+  return CAPOW_OK;
+}
+
+extern "C" int capow_get_grid(capow_ctx* c, int32_t slot, void* host, int64_t n) {
+  if (!c || !host || slot < 0 || slot > 3) return CAPOW_ERR_ARG;
+  if (!c->grid[slot]) return fail(c, CAPOW_ERR_ARG, "grid slot not allocated");
+  if (n < c->grid_elems) return fail(c, CAPOW_ERR_ARG, "host buffer too small");
+  HIPCHK(c, hipMemcpyAsync(host, c->grid[slot], (size_t)c->grid_elems * c->esz,
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return CAPOW_OK;
+}
+
+extern "C" int capow_set_grid(capow_ctx* c, int32_t slot, const void* host, int64_t n) {
+  if (!c || !host || slot < 0 || slot > 3) return CAPOW_ERR_ARG;
+  if (!c->grid[slot]) return fail(c, CAPOW_ERR_ARG, "grid slot not allocated");
+  if (n < c->grid_elems) return fail(c, CAPOW_ERR_ARG, "host buffer too small");
+  HIPCHK(c, hipMemcpyAsync(c->grid[slot], host, (size_t)c->grid_elems * c->esz,
+                           hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return CAPOW_OK;
+}
+
+extern "C" int capow_mark_assigned(capow_ctx* c, const capow_assign_stats* st) {
+  if (!c) return CAPOW_ERR_ARG;
+  if (st) c->last = *st;
+  c->state = 2;
+  return CAPOW_OK;
+}
+
+extern "C" int capow_mark_transformed(capow_ctx* c) {
+  if (!c) return CAPOW_ERR_ARG;
+  c->state = 3;
+  return CAPOW_OK;
+}
+
+extern "C" int capow_kernel_times(const capow_ctx* c, char* names, int32_t names_len, double* ms,
+                                  int32_t* launches, int32_t max, int32_t* count) {
+  if (!c || !count) return CAPOW_ERR_ARG;
+  int n = (int)c->timer.names.size();
+  if (n > max) n = max;
+  int pos = 0;
+  for (int i = 0; i < n; i++) {
+    if (ms) ms[i] = c->timer.ms[i];
+    if (launches) launches[i] = c->timer.launches[i];
+    if (names) {
+      const std::string& s = c->timer.names[i];
+      if (pos + (int)s.size() + 1 <= names_len) {
+        memcpy(names + pos, s.c_str(), s.size() + 1);
+        pos += (int)s.size() + 1;
+      }
+    }
+  }
+  *count = n;
+  return CAPOW_OK;
+}
+
+extern "C" int capow_reset_kernel_times(capow_ctx* c) {
+  if (!c) return CAPOW_ERR_ARG;
+  c->timer.reset();
+  return CAPOW_OK;
+}

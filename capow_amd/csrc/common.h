@@ -1,0 +1,211 @@
+// common.h -- shared device/host definitions of the gfx950 P_l(k) path.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "capow.h"
+
+namespace capow {
+
+// ---------------------------------------------------------------------------
+// Grid geometry and mass-assignment setup, passed by value to kernels.
+// ---------------------------------------------------------------------------
+struct Geom {
+  int n[3];              // NX NY NZ
+  int nzh;               // NZ/2+1 complex per row
+  int64_t rowlen;        // 2*(NZ/2+1) reals per padded row
+  double lo[3], hi[3];   // box
+  double d[3];           // cell size (MAX-MIN)/N  (src/main.c:1126-1128)
+  double L[3];           // MAX-MIN
+  int order;             // 1 NGP, 2 CIC, 3 TSC
+  int interlace;         // 0/1
+  int los;               // 0..3
+  int edge;              // CAPOW_EDGE_*
+  int periodic;          // 1: taps wrap mod N (reference LOS=0 or EDGE_PERIODIC)
+  int fa, fb;            // field A = w v^fa, field B = w v^fb (fb < 0: none)
+  int mean_sub_a;        // A is mean-subtracted (src/main.c:214)
+  int has_w;             // explicit weights
+  int rsd_axis;          // -1, or 0/1/2 for LOS 1/2/3
+  double rsd_f1, rsd_den;// shift = v*f1/den   (src/read_data.c:67-76)
+  // deposit tiling
+  int tdim[3];           // tile extent per axis (cells)
+  int ntile[3];          // tiles per axis
+  int reach_lo, reach_hi;// taps of a particle lie in [key+reach_lo, key+reach_hi]
+  int64_t nbuckets;      // ntiles * 27
+};
+
+// Region of a key inside its tile along one axis: 0 = low edge (needed by the
+// tile below... i.e. the previous tile's high neighbour), 1 = middle,
+// 2 = high edge.  Low width = -reach_lo, high width = reach_hi.
+__host__ __device__ inline int region_of(int local, int td, int lw, int hw) {
+  if (local < lw) return 0;
+  if (local >= td - hw) return 2;
+  return 1;
+}
+
+// ---------------------------------------------------------------------------
+// Per-axis tap rules.
+//
+// For one coordinate u along one axis this returns whether the reference's
+// add_to_grid accepts it, the first tap index (unwrapped; taps are
+// first, first+1, ..., first+order-1) and the order weights, with weight 0
+// for every tap the reference drops.  Restates src/read_data.c:514-745 at
+// NTask = 1 (xmin = XMIN, xmax = XMAX, nx = NX + InterpOrder) followed by the
+// halo add of src/read_data.c:152-175 (LOS = 0: buffer slices wrap onto
+// slices 0..InterpOrder-1; LOS != 0: buffer dropped):
+//   NGP  accept [MIN, MAX); tap (int)s                          (519-541)
+//   CIC  accept [MIN, MAX); taps (int)s, +1, weights 1-f, f     (544-605)
+//   TSC  LOS = 0, x: u += L when u-MIN < d, accept [MIN+d, MAX+d)
+//        LOS != 0, x: accept [MIN, MAX+d); y, z: accept [MIN, MAX)
+//        centre c = (int)(s+0.5), f = s-c, weights 0.5(0.5-f)^2,
+//        0.75-f^2, 0.5(0.5+f)^2; the low tap is kept only when its
+//        index c-1 > 0                                           (608-743)
+// Periodic axes wrap taps mod N; non-periodic axes drop taps outside
+// [0, N).  EDGE_PERIODIC is textbook periodic assignment: positions wrapped
+// into the box, every tap kept.
+// `key` is the particle's bucket cell along the axis: the main-grid centre
+// (TSC) or floor cell (CIC/NGP), wrapped (periodic) or clamped.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct AxisOut {
+  int first;    // first tap (unwrapped)
+  T w[3];
+  bool ok;
+};
+
+template <typename T>
+__device__ __forceinline__ int fl2i(T v) { return (int)v; }  // trunc toward 0
+
+template <typename T, int ORDER>
+__device__ __forceinline__ AxisOut<T> axis_taps(T u, T lo, T hi, T d, T L, int n,
+                                                bool is_x, int los, int edge) {
+  AxisOut<T> o;
+  o.w[0] = o.w[1] = o.w[2] = T(0);
+  if (edge == CAPOW_EDGE_PERIODIC) {
+    // wrap into [lo, hi)
+    T r = u - lo;
+    if (r < T(0) || r >= L) {
+      r = r - L * floor(r / L);
+      if (r >= L) r = T(0);
+    }
+    T s = r / d;
+    o.ok = true;
+    if (ORDER == 3) {
+      int c = (int)floor(s + T(0.5));
+      T f = s - (T)c;
+      o.first = c - 1;
+      o.w[0] = T(0.5) * (T(0.5) - f) * (T(0.5) - f);
+      o.w[1] = T(0.75) - f * f;
+      o.w[2] = T(0.5) * (T(0.5) + f) * (T(0.5) + f);
+    } else if (ORDER == 2) {
+      int c = (int)floor(s);
+      T f = s - (T)c;
+      o.first = c;
+      o.w[0] = T(1) - f;
+      o.w[1] = f;
+    } else {
+      o.first = (int)floor(s);
+      o.w[0] = T(1);
+    }
+    return o;
+  }
+  const bool periodic = (los == 0);
+  if (ORDER == 3) {
+    if (is_x) {
+      if (periodic) {
+        if (u - lo < d) u += L;
+        o.ok = !((u < lo + d) || (u >= hi + d));
+      } else {
+        o.ok = !((u < lo) || (u >= hi + d));
+      }
+    } else {
+      o.ok = !((u < lo) || (u >= hi));
+    }
+    T s = (u - lo) / d;
+    int c = fl2i<T>(s + T(0.5));
+    T f = s - (T)c;
+    o.first = c - 1;
+    T wl = T(0.5) * (T(0.5) - f) * (T(0.5) - f);
+    T wc = T(0.75) - f * f;
+    T wh = T(0.5) * (T(0.5) + f) * (T(0.5) + f);
+    // low tap kept iff its pre-wrap index c-1 > 0 (both periodic and not)
+    o.w[0] = (c - 1 > 0) ? wl : T(0);
+    o.w[1] = wc;
+    o.w[2] = wh;
+    if (!periodic) {
+      // non-periodic: taps at index >= N land in dropped buffer slices (x)
+      // or fail the < N guards (y, z)
+      if (c - 1 >= n) o.w[0] = T(0);
+      if (c >= n) o.w[1] = T(0);
+      if (c + 1 >= n) o.w[2] = T(0);
+    }
+    return o;
+  } else if (ORDER == 2) {
+    o.ok = !((u < lo) || (u >= hi));
+    T s = (u - lo) / d;
+    int c = fl2i<T>(s);
+    T f = s - (T)c;
+    o.first = c;
+    o.w[0] = T(1) - f;
+    o.w[1] = f;
+    if (!periodic) {
+      if (c >= n) o.w[0] = T(0);
+      if (c + 1 >= n) o.w[1] = T(0);
+    }
+    return o;
+  } else {
+    o.ok = !((u < lo) || (u >= hi));
+    T s = (u - lo) / d;
+    int c = fl2i<T>(s);
+    if (!is_x && c == n) c = n - 1;   // src/read_data.c:534-535
+    o.first = c;
+    o.w[0] = T(1);
+    if (!periodic && c >= n) o.w[0] = T(0);
+    return o;
+  }
+}
+
+// Bucket key cell along an axis from the main-grid coordinate (already
+// shifted by the TSC periodic wrap where applicable).
+template <typename T, int ORDER>
+__device__ __forceinline__ int axis_key(T u, T lo, T hi, T d, T L, int n, bool is_x,
+                                        int los, int edge) {
+  T s;
+  if (edge == CAPOW_EDGE_PERIODIC) {
+    T r = u - lo;
+    if (r < T(0) || r >= L) {
+      r = r - L * floor(r / L);
+      if (r >= L) r = T(0);
+    }
+    s = r / d;
+  } else {
+    if (ORDER == 3 && is_x && los == 0 && (u - lo < d)) u += L;
+    s = (u - lo) / d;
+  }
+  int c = (ORDER == 3) ? (int)floor(s + T(0.5)) : (int)floor(s);
+  if (edge == CAPOW_EDGE_PERIODIC || los == 0) {
+    c %= n;
+    if (c < 0) c += n;
+  } else {
+    c = c < 0 ? 0 : (c >= n ? n - 1 : c);
+  }
+  return c;
+}
+
+// Fast unsigned 32-bit division by a runtime constant (round-up method).
+struct FastDiv {
+  uint32_t d, m, s;
+  __host__ void init(uint32_t div) {
+    d = div;
+    s = 0;
+    while ((1ull << s) < div) s++;
+    m = (uint32_t)(((1ull << 32) * ((1ull << s) - div)) / div + 1);
+  }
+  __device__ __forceinline__ uint32_t div(uint32_t x) const {
+    uint32_t t = __umulhi(x, m);
+    return (t + ((x - t) >> (s ? 1 : 0))) >> (s ? s - 1 : 0);
+  }
+};
+
+}  // namespace capow

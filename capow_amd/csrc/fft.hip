@@ -1,0 +1,315 @@
+// fft.hip -- in-place forward r2c 3-D FFT (K3) for gfx950.
+//
+// Replaces fftw_execute of the in-place r2c plans (reference
+// src/main.c:1200-1217, executed at src/main.c:266-271): unnormalised,
+// e^{-i k.x}, real input [NX][NY][NZ] in rows padded to 2(NZ/2+1), complex
+// output [NX][NY][NZ/2+1] in the same memory (FFTW's non-transposed layout).
+//
+// Three axis passes, each one workgroup per batch of 1-D transforms held in
+// LDS (radix 2/3/4/5/8 Stockham, butterflies in registers, one LDS buffer):
+//   z: rows are contiguous; a length-NZ real row is transformed as NZ/2
+//      complex values followed by the even/odd split, so a row is read and
+//      written once;
+//   y, x: B consecutive complex columns (64 B of contiguous memory per
+//      axis index) are gathered, transformed along the strided axis and
+//      written back in place.
+// Twiddles: exp(-2 pi i j / N) generated in double on the host, stored in the
+// working precision.
+#include "common.h"
+#include "kernels.h"
+
+namespace capow {
+
+template <typename T>
+struct C2;
+template <>
+struct C2<float> { typedef float2 t; };
+template <>
+struct C2<double> { typedef double2 t; };
+
+template <typename T>
+using cx = typename C2<T>::t;
+
+template <typename T>
+__device__ __forceinline__ cx<T> mk(T a, T b) { cx<T> r; r.x = a; r.y = b; return r; }
+template <typename T>
+__device__ __forceinline__ cx<T> add(cx<T> a, cx<T> b) { return mk<T>(a.x + b.x, a.y + b.y); }
+template <typename T>
+__device__ __forceinline__ cx<T> sub(cx<T> a, cx<T> b) { return mk<T>(a.x - b.x, a.y - b.y); }
+template <typename T>
+__device__ __forceinline__ cx<T> mul(cx<T> a, cx<T> b) {
+  return mk<T>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename T>
+__device__ __forceinline__ cx<T> mul_mi(cx<T> a) { return mk<T>(a.y, -a.x); }  // a * (-i)
+
+// Forward DFT of size R in registers: out[q] = sum_r v[r] exp(-2 pi i r q / R)
+template <typename T, int R>
+__device__ __forceinline__ void butterfly(cx<T>* v) {
+  if (R == 2) {
+    cx<T> a = v[0], b = v[1];
+    v[0] = add<T>(a, b);
+    v[1] = sub<T>(a, b);
+  } else if (R == 4) {
+    cx<T> s02 = add<T>(v[0], v[2]), d02 = sub<T>(v[0], v[2]);
+    cx<T> s13 = add<T>(v[1], v[3]), d13 = sub<T>(v[1], v[3]);
+    v[0] = add<T>(s02, s13);
+    v[2] = sub<T>(s02, s13);
+    v[1] = add<T>(d02, mul_mi<T>(d13));
+    v[3] = sub<T>(d02, mul_mi<T>(d13));
+  } else if (R == 8) {
+    const T h = (T)0.70710678118654752440084436210485;
+    cx<T> e[4] = {v[0], v[2], v[4], v[6]};
+    cx<T> o[4] = {v[1], v[3], v[5], v[7]};
+    butterfly<T, 4>(e);
+    butterfly<T, 4>(o);
+    // o[q] *= exp(-2 pi i q / 8)
+    o[1] = mk<T>(h * (o[1].x + o[1].y), h * (o[1].y - o[1].x));
+    o[2] = mul_mi<T>(o[2]);
+    o[3] = mk<T>(h * (o[3].y - o[3].x), -h * (o[3].x + o[3].y));
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      v[q] = add<T>(e[q], o[q]);
+      v[q + 4] = sub<T>(e[q], o[q]);
+    }
+  } else if (R == 3) {
+    const T s3 = (T)0.86602540378443864676372317075294;
+    cx<T> t1 = add<T>(v[1], v[2]), t2 = sub<T>(v[1], v[2]);
+    cx<T> a = v[0];
+    v[0] = add<T>(a, t1);
+    cx<T> m = mk<T>(a.x - T(0.5) * t1.x, a.y - T(0.5) * t1.y);
+    cx<T> r = mk<T>(s3 * t2.y, -s3 * t2.x);  // -i s3 t2
+    v[1] = add<T>(m, r);
+    v[2] = sub<T>(m, r);
+  } else if (R == 5) {
+    const T c1 = (T)0.30901699437494742410229341718282;   // cos 2pi/5
+    const T c2 = (T)-0.80901699437494742410229341718282;  // cos 4pi/5
+    const T s1 = (T)0.95105651629515357211643933337938;   // sin 2pi/5
+    const T s2 = (T)0.58778525229247312916870595463907;   // sin 4pi/5
+    cx<T> a = v[0];
+    cx<T> t1 = add<T>(v[1], v[4]), t2 = add<T>(v[2], v[3]);
+    cx<T> u1 = sub<T>(v[1], v[4]), u2 = sub<T>(v[2], v[3]);
+    v[0] = add<T>(a, add<T>(t1, t2));
+    cx<T> m1 = mk<T>(a.x + c1 * t1.x + c2 * t2.x, a.y + c1 * t1.y + c2 * t2.y);
+    cx<T> m2 = mk<T>(a.x + c2 * t1.x + c1 * t2.x, a.y + c2 * t1.y + c1 * t2.y);
+    // -i (s1 u1 + s2 u2) and -i (s2 u1 - s1 u2)
+    cx<T> n1 = mk<T>(s1 * u1.y + s2 * u2.y, -(s1 * u1.x + s2 * u2.x));
+    cx<T> n2 = mk<T>(s2 * u1.y - s1 * u2.y, -(s2 * u1.x - s1 * u2.x));
+    v[1] = add<T>(m1, n1);
+    v[4] = sub<T>(m1, n1);
+    v[2] = add<T>(m2, n2);
+    v[3] = sub<T>(m2, n2);
+  }
+}
+
+#define FFT_EPT 8   // complex elements per thread per pass
+
+// One Stockham pass of radix R over nseq sequences of length M held in LDS.
+// COL: element e of sequence s at buf[e*nseq + s] (sequence index fastest);
+// else at buf[s*M + e].  tw: table exp(-2 pi i j / (M*twmul)).
+template <typename T, int R, bool COL>
+__device__ __forceinline__ void pass(cx<T>* buf, int M, int Ns, int nseq,
+                                     const cx<T>* __restrict__ tw, int twmul) {
+  constexpr int KM = (FFT_EPT + R - 1) / R + 1;
+  const int nbf = M / R;
+  const int total = nseq * nbf;
+  const int span = M / (Ns * R);
+  cx<T> v[KM][R];
+#pragma unroll
+  for (int q = 0; q < KM; q++) {
+    int t = threadIdx.x + q * blockDim.x;
+    if (t < total) {
+      int s, j;
+      if (COL) { s = t % nseq; j = t / nseq; }
+      else { j = t % nbf; s = t / nbf; }
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        int e = j + r * nbf;
+        v[q][r] = buf[COL ? e * nseq + s : s * M + e];
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < KM; q++) {
+    int t = threadIdx.x + q * blockDim.x;
+    if (t < total) {
+      int s, j;
+      if (COL) { s = t % nseq; j = t / nseq; }
+      else { j = t % nbf; s = t / nbf; }
+      int k = j % Ns;
+      if (Ns > 1) {
+#pragma unroll
+        for (int r = 1; r < R; r++) v[q][r] = mul<T>(v[q][r], tw[(r * k * span) * twmul]);
+      }
+      butterfly<T, R>(v[q]);
+      int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        int e = base + r * Ns;
+        buf[COL ? e * nseq + s : s * M + e] = v[q][r];
+      }
+    }
+  }
+  __syncthreads();
+}
+
+template <typename T, bool COL>
+__device__ __forceinline__ void fft_lds(cx<T>* buf, int M, int nseq, const FftPlan& pl,
+                                        const cx<T>* __restrict__ tw, int twmul) {
+  int Ns = 1;
+  for (int p = 0; p < pl.npass; p++) {
+    switch (pl.radix[p]) {
+      case 2: pass<T, 2, COL>(buf, M, Ns, nseq, tw, twmul); break;
+      case 3: pass<T, 3, COL>(buf, M, Ns, nseq, tw, twmul); break;
+      case 4: pass<T, 4, COL>(buf, M, Ns, nseq, tw, twmul); break;
+      case 5: pass<T, 5, COL>(buf, M, Ns, nseq, tw, twmul); break;
+      default: pass<T, 8, COL>(buf, M, Ns, nseq, tw, twmul); break;
+    }
+    Ns *= pl.radix[p];
+  }
+}
+
+// z pass: nrows_per_wg rows of NZ reals -> NZ/2+1 complex each, in place.
+template <typename T>
+__global__ __launch_bounds__(1024) void k_fft_rows(T* __restrict__ g0, T* __restrict__ g1,
+                                                   T* __restrict__ g2, T* __restrict__ g3,
+                                                   int64_t nrows, int nz, int rows_per_wg,
+                                                   FftPlan pl, const cx<T>* __restrict__ twz) {
+  extern __shared__ unsigned char smem[];
+  cx<T>* buf = reinterpret_cast<cx<T>*>(smem);
+  T* gp = blockIdx.y == 0 ? g0 : blockIdx.y == 1 ? g1 : blockIdx.y == 2 ? g2 : g3;
+  const int M = nz / 2;
+  const int64_t rowlen = 2 * (int64_t)(M + 1);
+  const int64_t r0 = (int64_t)blockIdx.x * rows_per_wg;
+  const int nr = (int)min((int64_t)rows_per_wg, nrows - r0);
+  // load rows as complex pairs (x[2e], x[2e+1])
+  for (int t = threadIdx.x; t < rows_per_wg * M; t += blockDim.x) {
+    int s = t / M, e = t - s * M;
+    if (s < nr) {
+      const cx<T>* row = reinterpret_cast<const cx<T>*>(gp + (r0 + s) * rowlen);
+      buf[s * M + e] = row[e];
+    } else {
+      buf[s * M + e] = mk<T>(T(0), T(0));
+    }
+  }
+  __syncthreads();
+  fft_lds<T, false>(buf, M, rows_per_wg, pl, twz, 2);
+  // even/odd split: X[k] = E[k] + w^k O[k], w = exp(-2 pi i / NZ)
+  for (int t = threadIdx.x; t < nr * (M + 1); t += blockDim.x) {
+    int s = t / (M + 1), k = t - s * (M + 1);
+    cx<T> zk = buf[s * M + (k == M ? 0 : k)];
+    cx<T> zr = buf[s * M + (k == 0 ? 0 : M - k)];
+    // E = (zk + conj(zr))/2 ; O = (zk - conj(zr)) / (2i)
+    cx<T> E = mk<T>(T(0.5) * (zk.x + zr.x), T(0.5) * (zk.y - zr.y));
+    cx<T> O = mk<T>(T(0.5) * (zk.y + zr.y), -T(0.5) * (zk.x - zr.x));
+    cx<T> X = add<T>(E, mul<T>(twz[k], O));
+    cx<T>* row = reinterpret_cast<cx<T>*>(gp + (r0 + s) * rowlen);
+    row[k] = X;
+  }
+}
+
+// y / x pass: transforms of length N along an axis with stride astride
+// (complex elements), over columns [c0, c0+B) of plane p (plane stride
+// pstride), ncols columns per plane.
+template <typename T, int B>
+__global__ __launch_bounds__(1024) void k_fft_cols(T* __restrict__ g0, T* __restrict__ g1,
+                                                   T* __restrict__ g2, T* __restrict__ g3,
+                                                   int N, int64_t astride, int64_t pstride,
+                                                   int64_t ncols, int64_t groups_per_plane,
+                                                   FftPlan pl, const cx<T>* __restrict__ tw) {
+  extern __shared__ unsigned char smem[];
+  cx<T>* buf = reinterpret_cast<cx<T>*>(smem);
+  T* gp = blockIdx.y == 0 ? g0 : blockIdx.y == 1 ? g1 : blockIdx.y == 2 ? g2 : g3;
+  cx<T>* base = reinterpret_cast<cx<T>*>(gp);
+  const int64_t plane = blockIdx.x / groups_per_plane;
+  const int64_t c0 = (blockIdx.x - plane * groups_per_plane) * B;
+  const int nc = (int)min((int64_t)B, ncols - c0);
+  cx<T>* col0 = base + plane * pstride + c0;
+  for (int t = threadIdx.x; t < N * B; t += blockDim.x) {
+    int s = t % B, e = t / B;
+    buf[t] = s < nc ? col0[(int64_t)e * astride + s] : mk<T>(T(0), T(0));
+  }
+  __syncthreads();
+  fft_lds<T, true>(buf, N, B, pl, tw, 1);
+  for (int t = threadIdx.x; t < N * B; t += blockDim.x) {
+    int s = t % B, e = t / B;
+    if (s < nc) col0[(int64_t)e * astride + s] = buf[t];
+  }
+}
+
+template <typename T, int B>
+static void launch_cols(FftArgs& a, const char* name, T** gp, int N, int64_t astride,
+                        int64_t pstride, int64_t ncols, int64_t nplanes, const FftPlan& pl,
+                        const void* tw) {
+  int threads = (N * B + FFT_EPT - 1) / FFT_EPT;
+  threads = min(max(((threads + 63) / 64) * 64, 64), 1024);
+  int64_t gpp = (ncols + B - 1) / B;
+  dim3 grid((unsigned)(gpp * nplanes), a.ngrids);
+  size_t lds = (size_t)N * B * sizeof(cx<T>);
+  TIMED(a.timer, name,
+        (k_fft_cols<T, B><<<grid, threads, lds, a.stream>>>(gp[0], gp[1], gp[2], gp[3], N, astride,
+                                                            pstride, ncols, gpp, pl,
+                                                            (const cx<T>*)tw)));
+}
+
+// widest column batch B (contiguous bytes per axis index) that fits
+template <typename T>
+static void launch_cols_any(FftArgs& a, const char* name, T** gp, int N, int64_t astride,
+                            int64_t pstride, int64_t ncols, int64_t nplanes, const FftPlan& pl,
+                            const void* tw, int64_t lim) {
+  const int bmax = sizeof(T) == 4 ? 8 : 4;
+  if (bmax >= 8 && (int64_t)N * 8 <= lim)
+    launch_cols<T, 8>(a, name, gp, N, astride, pstride, ncols, nplanes, pl, tw);
+  else if ((int64_t)N * 4 <= lim)
+    launch_cols<T, 4>(a, name, gp, N, astride, pstride, ncols, nplanes, pl, tw);
+  else
+    launch_cols<T, 2>(a, name, gp, N, astride, pstride, ncols, nplanes, pl, tw);
+}
+#define TIMED_COLS(a, name, gp, N, astride, pstride, ncols, nplanes, pl, tw, lim) \
+  launch_cols_any<T>(a, name, gp, N, astride, pstride, ncols, nplanes, pl, tw, lim)
+
+template <typename T>
+static int run_fft_t(FftArgs& a) {
+  const Geom& g = *a.g;
+  hipStream_t st = a.stream;
+  T* gp[4];
+  for (int q = 0; q < 4; q++) gp[q] = q < a.ngrids ? (T*)a.grid[q] : (T*)a.grid[0];
+  const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+  const int nzh = nz / 2 + 1;
+  // z pass
+  {
+    const int M = nz / 2;
+    int rows = max(1, (FFT_EPT * 1024) / max(M, 1));
+    rows = min(rows, 16);
+    // keep the row batch within 64 KB of LDS
+    rows = max(1, min(rows, (int)((64 * 1024) / ((size_t)M * sizeof(cx<T>)))));
+    int threads = (rows * M + FFT_EPT - 1) / FFT_EPT;
+    threads = ((threads + 63) / 64) * 64;
+    threads = min(max(threads, 64), 1024);
+    while (rows > 1 && (int64_t)threads * FFT_EPT < (int64_t)rows * M) rows--;
+    int64_t nrows = (int64_t)nx * ny;
+    dim3 grid((unsigned)((nrows + rows - 1) / rows), a.ngrids);
+    size_t lds = (size_t)rows * M * sizeof(cx<T>);
+    TIMED(a.timer, "fft_z",
+          (k_fft_rows<T><<<grid, threads, lds, st>>>(gp[0], gp[1], gp[2], gp[3], nrows, nz, rows,
+                                                     a.plan[2], (const cx<T>*)a.tw[2])));
+  }
+  // y pass: planes = x index, columns = kz
+  {
+    int64_t lim = 8192;  // N*B complex per workgroup (FFT_EPT * 1024)
+    TIMED_COLS(a, "fft_y", gp, ny, (int64_t)nzh, (int64_t)ny * nzh, (int64_t)nzh, nx, a.plan[1],
+               a.tw[1], lim);
+  }
+  // x pass: one plane, columns = (y, kz) contiguous
+  {
+    int64_t lim = 8192;
+    int64_t ncols = (int64_t)ny * nzh;
+    TIMED_COLS(a, "fft_x", gp, nx, ncols, 0, ncols, 1, a.plan[0], a.tw[0], lim);
+  }
+  return hipGetLastError() == hipSuccess ? CAPOW_OK : CAPOW_ERR_HIP;
+}
+
+int run_fft(FftArgs& a, bool fp64) { return fp64 ? run_fft_t<double>(a) : run_fft_t<float>(a); }
+
+}  // namespace capow

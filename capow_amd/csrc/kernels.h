@@ -1,0 +1,146 @@
+// kernels.h -- launch arguments shared by the API layer and the kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "common.h"
+
+namespace capow {
+
+// Global sums written by k_stats_final (doubles).
+enum {
+  STAT_NGRID = 0,   // sum w over main-accepted particles
+  STAT_W = 1,       // sum w over all particles
+  STAT_V1 = 2, STAT_V2 = 3, STAT_V3 = 4, STAT_V4 = 5,
+  STAT_NOUT = 6,
+  STAT_N = 8,       // stride of block partials
+  STAT_ALPHA = 8,
+  STAT_SCALE_A = 9, STAT_SCALE_B = 10,
+  STAT_INVSCALE_A = 11, STAT_INVSCALE_B = 12,
+  STAT_TOTAL = 16
+};
+
+// Kernel timer: HIP events around launches on the context stream.
+struct Timer {
+  bool on = false;
+  hipStream_t stream = nullptr;
+  struct Rec { std::string name; hipEvent_t a, b; };
+  std::vector<Rec> pending;
+  std::vector<hipEvent_t> pool;
+  std::vector<std::string> names;
+  std::vector<double> ms;
+  std::vector<int> launches;
+  hipEvent_t ev() {
+    hipEvent_t e;
+    if (!pool.empty()) { e = pool.back(); pool.pop_back(); }
+    else hipEventCreate(&e);
+    return e;
+  }
+  void push(const char* n, hipEvent_t a, hipEvent_t b) { pending.push_back({n, a, b}); }
+  // after the stream is synchronised
+  void collect() {
+    for (auto& r : pending) {
+      float t = 0.f;
+      hipEventElapsedTime(&t, r.a, r.b);
+      size_t k = 0;
+      for (; k < names.size(); k++) if (names[k] == r.name) break;
+      if (k == names.size()) { names.push_back(r.name); ms.push_back(0.0); launches.push_back(0); }
+      ms[k] += t;
+      launches[k] += 1;
+      pool.push_back(r.a);
+      pool.push_back(r.b);
+    }
+    pending.clear();
+  }
+  void reset() { names.clear(); ms.clear(); launches.clear(); }
+  ~Timer() {
+    for (auto& r : pending) { hipEventDestroy(r.a); hipEventDestroy(r.b); }
+    for (auto e : pool) hipEventDestroy(e);
+  }
+};
+
+#define TIMED(timer, name, launch)                                  \
+  do {                                                              \
+    Timer* _t = (timer);                                            \
+    if (_t && _t->on) {                                             \
+      hipEvent_t _a = _t->ev(), _b = _t->ev();                      \
+      hipEventRecord(_a, _t->stream);                               \
+      launch;                                                       \
+      hipEventRecord(_b, _t->stream);                               \
+      _t->push(name, _a, _b);                                       \
+    } else {                                                        \
+      launch;                                                       \
+    }                                                               \
+  } while (0)
+
+struct DepositArgs {
+  const Geom* g;
+  hipStream_t stream;
+  Timer* timer;
+  const void* parts;      // n x 4 (x y z v_los)
+  const void* wts;        // n or null
+  int64_t n;
+  int count_blocks;
+  uint32_t* counts;       // nbuckets (zeroed)
+  uint32_t* offsets;      // nbuckets + 1
+  uint32_t* cursor;       // nbuckets
+  uint32_t* bsum;         // scan block sums
+  uint32_t* boff;         // scan block offsets (+1)
+  void* sorted;           // n x 4
+  void* sorted_w;         // n or null
+  double* partials;       // count_blocks * STAT_N
+  unsigned long long* gmax;  // 2 (zeroed)
+  unsigned int* errs;     // 2 (zeroed): [0] LOS=0 out-of-box, [1] reach violations
+  double* stats;          // STAT_TOTAL
+  void* grid[4];          // A, A_int, B, B_int (null when unused)
+};
+int run_deposit(DepositArgs& a, bool fp64);
+
+struct FftPlan {
+  int npass;
+  int radix[24];
+};
+
+struct FftArgs {
+  const Geom* g;
+  hipStream_t stream;
+  Timer* timer;
+  void* grid[4];          // compacted list of grids to transform
+  int ngrids;
+  FftPlan plan[3];        // x: NX, y: NY, z: NZ/2
+  const void* tw[3];      // x: NX entries, y: NY, z: NZ (exp(-2 pi i j/N))
+};
+int run_fft(FftArgs& a, bool fp64);
+
+enum { SPEC_AUTO_A = 0, SPEC_AUTO_B = 1, SPEC_CROSS_IM = 2, SPEC_CROSS_RE = 3 };
+
+struct BinArgsDev {
+  int nx, ny, nzh;
+  FastDiv fd_nzh;
+  int nk, nmu, out2d, odd, logbins, los, interlace, hasB;
+  int nspec;
+  int kind[3];
+  int copies;
+  double mink, binwidth, min_nyq;
+  const double *fx, *fy, *fz;
+  const double *corx, *cory, *corz;
+  const double2 *phx, *phy, *phz;
+};
+
+struct BinLaunch {
+  hipStream_t stream;
+  Timer* timer;
+  const void* grid[4];    // A, A_int, B, B_int (unused -> any valid pointer)
+  BinArgsDev dev;
+  int nwg;
+  int per_copy;
+  double* partials;       // nwg * per_copy
+  double* out;            // per_copy
+};
+int run_bin(BinLaunch& L, bool fp64);
+
+}  // namespace capow

@@ -1,0 +1,238 @@
+/*
+ * capow.h -- C ABI of the MI355X (gfx950) periodic-box power-spectrum path.
+ *
+ * The reference (CAPOW) has no library boundary: its periodic path is a set
+ * of internal C calls on globals,
+ *   read_parameterfile()        src/read_params.c:30      (parameters)
+ *   create_grids()              src/main.c:1123           (grids, FFT plans)
+ *   read_periodic_serial_ascii  src/read_data.c:32        (ingest + deposit)
+ *   add_to_grid()               src/read_data.c:514       (NGP/CIC/TSC)
+ *   mean subtraction            src/main.c:211-225
+ *   fftw_execute(plan...)       src/main.c:266-271        (r2c 3-D FFT)
+ *   compute_periodic_power()    src/main.c:302            (binning)
+ *   output_power()              src/main.c:971            (normalise, write)
+ * and it aborts through FatalError (src/main.c:1298-1303) on any error.
+ * This header replaces that sequence with an explicit context: every entry
+ * point returns a status (0 = CAPOW_OK) and never aborts; the host maps a
+ * nonzero status onto the reference's "Fatal Error" + exit(1).
+ *
+ * Conventions
+ *   - Plain C types only.  Host arrays belong to the caller; device memory
+ *     (grids, sorted particles, bins) belongs to the context.
+ *   - A context is not thread-safe: one context per host thread.
+ *   - All GPU work of a context runs on one HIP stream (the legacy default
+ *     stream unless capow_set_stream is called); calls that return host
+ *     results synchronise that stream, the others do not.
+ */
+#ifndef CAPOW_H
+#define CAPOW_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CAPOW_ABI_VERSION 1
+
+/* Status codes.  CAPOW_ERR_OUT_OF_BOX mirrors the reference's fatal exit for
+ * a LOS = 0 particle outside the box (src/read_data.c:81-84). */
+enum {
+  CAPOW_OK = 0,
+  CAPOW_ERR_ARG = 1,        /* invalid argument / parameter value          */
+  CAPOW_ERR_HIP = 2,        /* a HIP runtime call failed                   */
+  CAPOW_ERR_NOMEM = 3,      /* device or host allocation failed            */
+  CAPOW_ERR_STATE = 4,      /* call out of order (e.g. bin before fft)     */
+  CAPOW_ERR_UNSUPPORTED = 5,/* grid shape the kernels do not handle        */
+  CAPOW_ERR_INTERNAL = 6,   /* a kernel self-check failed                  */
+  CAPOW_ERR_IO = 7,         /* file could not be opened / written          */
+  CAPOW_ERR_OUT_OF_BOX = 110
+};
+
+/* Arithmetic form.  FP64 is the plain form: every stage in double, as the
+ * reference.  FP32: positions, deposit weights, grids and FFT in float;
+ * twiddles generated in double; bin accumulation, mode counts,
+ * normalisation and shot noise in double. */
+enum { CAPOW_FP32 = 0, CAPOW_FP64 = 1 };
+
+/* Edge rules of the mass assignment.  REFERENCE reproduces the reference's
+ * rules at one MPI rank exactly (TSC drops low taps landing on index 0,
+ * interlaced CIC/NGP reject shifted coordinates >= MAX, LOS != 0 is
+ * non-periodic: src/read_data.c:514-745 with the halo add of
+ * src/read_data.c:152-175).  PERIODIC is textbook periodic assignment. */
+enum { CAPOW_EDGE_REFERENCE = 0, CAPOW_EDGE_PERIODIC = 1 };
+
+/* Element type of arrays handed to capow_set_particles. */
+enum { CAPOW_DTYPE_F32 = 0, CAPOW_DTYPE_F64 = 1 };
+
+/* Where a pointer lives. */
+enum { CAPOW_HOST = 0, CAPOW_DEVICE = 1 };
+
+/* Extra spectra computed from the same grids in the same binning pass.  The
+ * deposited fields are fixed by `momentum` exactly as in the reference
+ * (src/read_data.c:91-130): field A (ddg) = w*v^a, field B (ddg_mom) =
+ * w*v^b.  The primary spectrum is always the reference's `momentum`
+ * spectrum.  CAPOW_SPEC_AUTO_A adds |A|^2 (the reference spectrum with
+ * Momentum = P_aa: 0, 2 or 5) and CAPOW_SPEC_AUTO_B adds |B|^2 (P_bb: 2 or
+ * 5; only when b <= 2). */
+enum { CAPOW_SPEC_AUTO_A = 1, CAPOW_SPEC_AUTO_B = 2 };
+
+/* Run parameters: the periodic subset of the reference's 41 tags
+ * (src/read_params.c:65-227, stored in src/vars.h:70-134) plus the
+ * arithmetic form and edge rules. */
+typedef struct capow_params {
+  int64_t nx, ny, nz;                  /* NX NY NZ                        */
+  double xmin, xmax, ymin, ymax, zmin, zmax;
+  int32_t interp_order;                /* InterpOrder: 1 NGP, 2 CIC, 3 TSC */
+  int32_t do_interlacing;              /* DoInterlacing 0/1               */
+  int32_t los;                         /* LOS 0 (real space, mu along z), 1..3 */
+  int32_t momentum;                    /* Momentum 0..8                   */
+  int32_t nk;                          /* NK                              */
+  int32_t nmu;                         /* NMU (used when output_2d)       */
+  int32_t output_2d;                   /* Output2D 0/1                    */
+  int32_t output_log;                  /* OutputLog 0/1                   */
+  int32_t odd_multipoles;              /* Odd_Multipoles 0/1              */
+  double mink, maxk;                   /* Mink Maxk                       */
+  double omega_m, redshift;            /* Omega_m Redshift (RSD shift)    */
+  int32_t precision;                   /* CAPOW_FP32 | CAPOW_FP64         */
+  int32_t edge_mode;                   /* CAPOW_EDGE_*                    */
+  int32_t extra_spectra;               /* OR of CAPOW_SPEC_*              */
+  int32_t reserved;
+} capow_params;
+
+/* Global sums of the assignment (reference src/read_data.c:132-138 and the
+ * NGRID/NOUT counters of src/read_data.c:37-88). */
+typedef struct capow_assign_stats {
+  double nread;        /* particles handed in                         */
+  double nout;         /* LOS != 0: outside the box after RSD         */
+  double ngrid;        /* sum of w over main-grid-accepted particles  */
+  double nsq;          /* sum w over all particles (Momentum != 0)     */
+  double vr, vr2, vr3, vr4;  /* sum w v^n over all particles           */
+  double alpha;        /* ngrid / (NX NY NZ)                           */
+} capow_assign_stats;
+
+/* Raw (un-normalised) per-bin sums of one binning pass, in host memory
+ * supplied by the caller.  For spectrum s (0 = primary, then AUTO_A, then
+ * AUTO_B when requested) and multipole l = 0..4:
+ *   pk[(s*5 + l)*nk + kbin]   (l = 1, 3 only filled with odd_multipoles)
+ *   nmodes[kbin]              (int64; identical for every spectrum)
+ *   pk2d[s*nk*nmu + kbin*nmu + mubin], nmodes2d[kbin*nmu + mubin]
+ * as in src/main.c:432-461.  The spectrum codes (reference Momentum value
+ * each spectrum equals) are written to codes[0..n_spectra). */
+typedef struct capow_bins {
+  int32_t n_spectra;
+  int32_t codes[3];
+  double *pk;          /* n_spectra*5*nk                               */
+  int64_t *nmodes;     /* nk                                           */
+  double *pk2d;        /* n_spectra*nk*nmu, or NULL                    */
+  int64_t *nmodes2d;   /* nk*nmu, or NULL                              */
+} capow_bins;
+
+typedef struct capow_ctx capow_ctx;
+
+/* ---- lifecycle --------------------------------------------------------- */
+
+/* Validates p as check_inputs() does (src/read_params.c:294-428), allocates
+ * the device grids in the reference's padded layout
+ * [NX][NY][2(NZ/2+1)] (src/main.c:1123-1217; no buffer slices -- the
+ * deposit wraps directly) and builds FFT and binning tables.
+ * Replaces create_grids() (src/main.c:1123). */
+int capow_create(const capow_params *p, capow_ctx **out);
+int capow_destroy(capow_ctx *ctx);
+
+/* Message of the last failure of ctx (or of capow_create when ctx = NULL). */
+const char *capow_last_error(const capow_ctx *ctx);
+
+/* Work on a caller-supplied hipStream_t (NULL = default stream). */
+int capow_set_stream(capow_ctx *ctx, void *hip_stream);
+int capow_synchronize(capow_ctx *ctx);
+
+/* Record HIP events around every kernel launch (0 = off, the default). */
+int capow_set_timing(capow_ctx *ctx, int enable);
+
+/* ---- stages ------------------------------------------------------------ */
+
+/* Particles as n rows of (x, y, z, v_los): 4 values per particle of type
+ * dtype, already with the LOS velocity column selected as
+ * src/read_data.c:65-76 does (vz for LOS 0 and 3, vx for 1, vy for 2).
+ * w: n weights of the same dtype, or NULL for unit weights (the reference
+ * hard-codes w = 1, src/read_data.c:60).  where = CAPOW_HOST copies to the
+ * device; CAPOW_DEVICE uses the pointers in place (they must stay valid
+ * until capow_assign returns).  dtype must match the context's precision
+ * (F32 for CAPOW_FP32, F64 for CAPOW_FP64). */
+int capow_set_particles(capow_ctx *ctx, const void *xyzv, const void *w,
+                        int64_t n, int32_t dtype, int32_t where);
+
+/* Mass assignment of every field the reference's `momentum` needs, with
+ * interlacing, followed by the mean subtraction; replaces the deposit loop
+ * of read_periodic_serial_ascii (src/read_data.c:56-175) and
+ * src/main.c:103-148 + 211-225.  The RSD shift of src/read_data.c:67-76 is
+ * applied on the device for LOS 1..3.  stats may be NULL. */
+int capow_assign(capow_ctx *ctx, capow_assign_stats *stats);
+
+/* In-place forward r2c 3-D FFT of every deposited grid, unnormalised,
+ * e^{-ik.x}; replaces fftw_execute at src/main.c:266-271. */
+int capow_fft(capow_ctx *ctx);
+
+/* Fused deconvolution, interlace combine, power, Legendre weighting and
+ * (k, mu) binning; replaces compute_periodic_power (src/main.c:302-468).
+ * Fills out (host memory) with raw sums; synchronises. */
+int capow_bin(capow_ctx *ctx, capow_bins *out);
+
+/* capow_assign + capow_fft + capow_bin. */
+int capow_power(capow_ctx *ctx, capow_assign_stats *stats, capow_bins *out);
+
+/* Number of spectra capow_bin produces and their reference Momentum codes. */
+int capow_spectra(const capow_params *p, int32_t *n_spectra, int32_t codes[3]);
+
+/* ---- host-side output math (no GPU) ------------------------------------ */
+
+/* shot and norm for spectrum code `code` from the assignment sums
+ * (src/main.c:103-148 at one rank). */
+int capow_scalars(const capow_params *p, const capow_assign_stats *st,
+                  int32_t code, double *shot, double *norm);
+
+/* In place on one spectrum's arrays (pk0..pk4 = 5 consecutive nk-vectors,
+ * nmodes, and the optional 2-D arrays): shot subtraction on P0,
+ * (2l+1)/(Nmodes norm) scaling, and zeroing of the highest filled bin
+ * (src/main.c:1014-1056).  Returns the zeroed bin index via *last_bin
+ * (-1 if none). */
+int capow_finalize(const capow_params *p, double shot, double norm,
+                   double *pk5, int64_t *nmodes, double *pk2d,
+                   int64_t *nmodes2d, int32_t *last_bin);
+
+/* Writes the reference's text file(s) (src/main.c:1058-1101): "%g" rows,
+ * and `path`_2D when output_2d.  precise != 0 uses "%.17g" (a parity side
+ * dump; not the reference format). */
+int capow_write_output(const capow_params *p, double shot, double norm,
+                       const double *pk5, const int64_t *nmodes,
+                       const double *pk2d, const int64_t *nmodes2d,
+                       const char *path, int32_t precise);
+
+/* ---- inspection (parity tests, dumps) ---------------------------------- */
+
+/* Grid slots: 0 = A (ddg), 1 = A interlaced, 2 = B (ddg_mom),
+ * 3 = B interlaced.  Element type is the context precision; the buffer is
+ * the padded [NX][NY][2(NZ/2+1)] array (real before capow_fft, complex
+ * after).  n = number of elements the host buffer holds. */
+int capow_grid_elems(const capow_ctx *ctx, int64_t *n);
+int capow_get_grid(capow_ctx *ctx, int32_t slot, void *host, int64_t n);
+int capow_set_grid(capow_ctx *ctx, int32_t slot, const void *host, int64_t n);
+/* Marks the grids as deposited (after capow_set_grid) with the given
+ * assignment sums, so capow_fft / capow_bin may run on uploaded grids. */
+int capow_mark_assigned(capow_ctx *ctx, const capow_assign_stats *st);
+/* Marks the grids as transformed (after uploading k-space grids). */
+int capow_mark_transformed(capow_ctx *ctx);
+
+/* Per-kernel GPU times of the last stages run with timing on: names
+ * (NUL-separated into `names`, up to names_len bytes), total ms and launch
+ * counts, up to max entries; returns the number of entries via *count. */
+int capow_kernel_times(const capow_ctx *ctx, char *names, int32_t names_len,
+                       double *ms, int32_t *launches, int32_t max,
+                       int32_t *count);
+int capow_reset_kernel_times(capow_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

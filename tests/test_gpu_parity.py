@@ -1,0 +1,328 @@
+"""GPU parity: the HIP path through the C ABI against the FP64 CPU oracle.
+
+Bars (stated per test):
+  * deposit grids (FP64 form): |gpu - oracle| <= 1e-10 * max|field value|
+    per cell (each tap is rounded to 2^-40 of the largest |field value|
+    before exact accumulation);
+  * FFT (FP64): <= 1e-12 * max|X|; (FP32): <= 2e-5 * max|X|;
+  * binning on identical k-space grids: mode counts bit-exact, sums within
+    1e-12 of the per-bin Cauchy-Schwarz scale;
+  * end to end (FP64): mode counts bit-exact, |dP_l(k)| <= 1e-10 * (|P_0(k)|
+    + shot/norm) (cross spectra: geometric mean of the two autos' scales);
+  * FP32 form vs FP64 form: <= 1e-4 of the same scale for k < k_Nyq/2.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from capow_amd import api, mocks
+
+pytestmark = pytest.mark.gpu
+
+L = 1000.0
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def oracle_params(O, p):
+    return O.params(nx=p.nx, ny=p.ny, nz=p.nz, xmin=p.xmin, xmax=p.xmax, ymin=p.ymin,
+                    ymax=p.ymax, zmin=p.zmin, zmax=p.zmax, interp_order=p.interp_order,
+                    do_interlacing=p.do_interlacing, los=p.los, momentum=p.momentum, nk=p.nk,
+                    nmu=p.nmu, output_2d=p.output_2d, output_log=p.output_log,
+                    odd_multipoles=p.odd_multipoles, mink=p.mink, maxk=p.maxk,
+                    omega_m=p.omega_m, redshift=p.redshift)
+
+
+def edge_particles(n, L, los, rng):
+    """Particles placed on the reference's edge rules: near 0, near L, in
+    [dx, 1.5 dx) and [0.5 dx, 1.5 dx) bands, plus exact lattice points."""
+    pts = []
+    for frac in (0.0, 1e-9, 0.3, 0.5, 0.9999, 1.0, 1.2, 1.49, 1.5, 2.0):
+        for axis in range(3):
+            q = rng.random(3) * L
+            q[axis] = frac * (L / n)
+            pts.append(q)
+            q2 = rng.random(3) * L
+            q2[axis] = L - frac * (L / n) - 1e-9 if frac > 0 else L * (1 - 1e-15)
+            pts.append(q2)
+    pts = np.array(pts)
+    pts = np.clip(pts, 0, np.nextafter(L, 0))
+    return pts
+
+
+def make_catalogue(n_grid, npart, los, seed, momentum):
+    rng = np.random.default_rng(seed)
+    pos, vel = mocks.uniform(npart, L, seed)
+    vel = rng.normal(0, 300.0, size=(npart, 3))
+    e = edge_particles(n_grid, L, los, rng)
+    pos = np.concatenate([pos, e])
+    vel = np.concatenate([vel, rng.normal(0, 300.0, size=(len(e), 3))])
+    if los == 0:
+        pos = np.clip(pos, 0, L)
+    return pos, vel
+
+
+CASES = [
+    # name, grid, order, interlace, los, momentum
+    ("cic_los0_m0", 32, 2, 0, 0, 0),
+    ("tsc_int_los0_m1", 32, 3, 1, 0, 1),
+    ("ngp_int_los0_m2", 24, 1, 1, 0, 2),
+    ("tsc_int_los3_m1", 32, 3, 1, 3, 1),
+    ("cic_int_los1_m3", 20, 2, 1, 1, 3),
+    ("ngp_los2_m4", 16, 1, 0, 2, 4),
+    ("tsc_los0_m8", 40, 3, 0, 0, 8),
+    ("cic_int_los0_m6", 48, 2, 1, 0, 6),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_deposit_matches_oracle(case, oracle):
+    name, n, order, inter, los, mom = case
+    say("deposit case", name)
+    p = api.make_params(nx=n, ny=n, nz=n, interp_order=order, do_interlacing=inter, los=los,
+                        momentum=mom, precision=api.FP64, nk=10, maxk=0.3)
+    pos, vel = make_catalogue(n, 20000, los, 7 + n, mom)
+    op = oracle_params(oracle, p)
+    rc, og, st = oracle.assign(op, pos, vel)
+    assert rc == 0
+    alpha, shot, norm = oracle.scalars(op, st)
+    oracle.mean_subtract(op, alpha, og)
+    ps = api.PowerSpectrum(p)
+    ps.set_particles(mocks.xyzv(pos, vel, los))
+    gst = ps.assign()
+    assert gst.ngrid == st.ngrid
+    assert gst.nread == st.nread and gst.nout == st.nout
+    if mom:
+        for q in ("nsq", "vr", "vr2", "vr3", "vr4"):
+            ref = getattr(st, q)
+            scale = np.abs(mocks.xyzv(pos, vel, los)[:, 3] ** int(q[-1] if q[-1].isdigit() else 1)).sum()
+            assert abs(getattr(gst, q) - ref) <= 1e-13 * max(1.0, scale), q
+    assert gst.alpha == pytest.approx(alpha, rel=1e-15)
+    vlos = mocks.xyzv(pos, vel, los)[:, 3]
+    fa = {0: 0, 1: 0, 2: 1, 3: 0, 4: 1, 5: 2, 6: 0, 7: 1, 8: 0}[mom]
+    fb = {1: 1, 3: 2, 4: 2, 6: 3, 7: 3, 8: 4}.get(mom)
+    for slot in range(4):
+        if og[slot] is None:
+            continue
+        e = fa if slot < 2 else fb
+        gmax = max(1e-300, np.abs(vlos ** e).max())
+        ref = oracle.real_view(op, og[slot])[:n, :, :n]
+        got = ps.get_grid(slot).reshape(n, n, -1)[:, :, :n]
+        err = np.abs(got - ref).max()
+        say("  slot", slot, "max err", err, "scale", gmax)
+        assert err <= 1e-10 * gmax, (name, slot, err)
+    ps.close()
+
+
+@pytest.mark.parametrize("prec", [api.FP64, api.FP32])
+@pytest.mark.parametrize("shape", [(16, 16, 16), (24, 20, 30), (64, 48, 32), (128, 128, 128),
+                                   (30, 40, 50)])
+def test_fft_matches_oracle(prec, shape, oracle):
+    nx, ny, nz = shape
+    say("fft case", prec, shape)
+    p = api.make_params(nx=nx, ny=ny, nz=nz, interp_order=1, precision=prec)
+    ps = api.PowerSpectrum(p)
+    rng = np.random.default_rng(nx * 7 + nz)
+    rowlen = 2 * (nz // 2 + 1)
+    g = np.zeros((nx, ny, rowlen))
+    g[:, :, :nz] = rng.standard_normal((nx, ny, nz))
+    ps.set_grid(0, g)
+    ps.mark_assigned()
+    ps.fft()
+    got = ps.get_grid(0).astype(np.float64).reshape(nx, ny, rowlen)
+    ref = g.ravel().copy()
+    oracle.fft_r2c_3d(ref, nx, ny, nz)
+    ref = ref.reshape(nx, ny, rowlen)
+    scale = np.abs(ref).max()
+    err = np.abs(got - ref).max() / scale
+    say("  rel err", err)
+    assert err <= (1e-12 if prec == api.FP64 else 2e-5)
+    ps.close()
+
+
+def _cs_scale(pk0_auto_a, pk0_auto_b):
+    return np.sqrt(np.abs(pk0_auto_a) * np.abs(pk0_auto_b))
+
+
+BIN_CASES = [
+    ("m0_cic", dict(interp_order=2, do_interlacing=0, momentum=0)),
+    ("m1_tsc_int_odd_2d", dict(interp_order=3, do_interlacing=1, momentum=1, odd_multipoles=1,
+                               output_2d=1, nmu=7, extra_spectra=3)),
+    ("m3_los1_log", dict(interp_order=2, do_interlacing=1, momentum=3, los=1, output_log=1,
+                         mink=-2.3, maxk=-0.5, nk=17)),
+    ("m2_los2_mink", dict(interp_order=1, do_interlacing=0, momentum=2, los=2, mink=0.013,
+                          maxk=0.31, nk=23)),
+]
+
+
+@pytest.mark.parametrize("case", BIN_CASES, ids=[c[0] for c in BIN_CASES])
+def test_binning_matches_oracle(case, oracle):
+    name, kw = case
+    say("binning case", name)
+    n = 32
+    base = dict(nx=n, ny=n, nz=n, nk=30, maxk=0.3, precision=api.FP64)
+    base.update(kw)
+    p = api.make_params(**base)
+    op = oracle_params(oracle, p)
+    pos, vel = make_catalogue(n, 30000, p.los, 3, p.momentum)
+    rc, og, st = oracle.assign(op, pos, vel)
+    assert rc == 0
+    alpha, shot, norm = oracle.scalars(op, st)
+    oracle.mean_subtract(op, alpha, og)
+    for g in og:
+        if g is not None:
+            oracle.fft_r2c_3d(g, n, n, n)
+    ob = oracle.compute_power(op, og)
+    ps = api.PowerSpectrum(p)
+    for slot in range(4):
+        if og[slot] is not None:
+            ps.set_grid(slot, oracle.real_view(op, og[slot])[:n])
+    ps.mark_transformed()
+    res = ps.bin()
+    assert np.array_equal(res["nmodes"], ob.nmodes), name
+    prim = res[p.momentum]["pk"]
+    autos = {}
+    # scale: the auto spectra when available, else |pk0|
+    for code in res["codes"]:
+        autos[code] = res[code]["pk"][0]
+    scale = np.abs(prim[0])
+    if len(res["codes"]) == 3:
+        scale = _cs_scale(autos[res["codes"][1]], autos[res["codes"][2]])
+    for l in ((0, 1, 2, 3, 4) if p.odd_multipoles else (0, 2, 4)):
+        d = np.abs(prim[l] - ob.pk[l])
+        assert np.all(d <= 1e-12 * scale + 1e-300), (name, l, (d / np.maximum(scale, 1e-300)).max())
+    if p.output_2d:
+        assert np.array_equal(res["nmodes2d"].ravel(), ob.nmodes2d)
+        d = np.abs(res[p.momentum]["pk2d"].ravel() - ob.pk2d)
+        assert d.max() <= 1e-12 * np.abs(ob.pk2d).max() * 10
+    ps.close()
+
+
+def _finalized(p, res, stats, code):
+    shot, norm = api.scalars(p, stats, code)
+    pk5, nm, _, _, last = api.finalize(p, shot, norm, res[code]["pk"].ravel(), res["nmodes"])
+    return pk5.reshape(5, -1), nm, shot / norm
+
+
+E2E = [
+    ("c0_like_cic_m0", 64, dict(interp_order=2, do_interlacing=0, momentum=0), 100000, "uniform"),
+    ("c1_like_tsc_int_m1", 64, dict(interp_order=3, do_interlacing=1, momentum=1,
+                                    odd_multipoles=1, extra_spectra=3), 32, "zeldovich"),
+    ("tsc_int_los3_m1", 48, dict(interp_order=3, do_interlacing=1, momentum=1, los=3,
+                                 extra_spectra=3), 24, "zeldovich"),
+]
+
+
+def _catalogue(kind, size, seed):
+    if kind == "uniform":
+        return mocks.uniform(size, L, seed)
+    return mocks.zeldovich(size, L, seed)
+
+
+@pytest.mark.parametrize("case", E2E, ids=[c[0] for c in E2E])
+def test_end_to_end_fp64(case, oracle):
+    name, n, kw, size, kind = case
+    say("e2e case", name)
+    base = dict(nx=n, ny=n, nz=n, nk=int(np.pi * n / L / 0.01), maxk=0.01 * int(np.pi * n / L / 0.01),
+                precision=api.FP64)
+    base.update(kw)
+    p = api.make_params(**base)
+    pos, vel = _catalogue(kind, size, 11)
+    ps = api.PowerSpectrum(p)
+    ps.set_particles(mocks.xyzv(pos, vel, p.los))
+    res = ps.power()
+    scales = {}
+    for code in res["codes"]:
+        op = oracle_params(oracle, p)
+        op.momentum = code
+        rc, ob, ost, oa, oshot, onorm = oracle.run(op, pos, vel)
+        assert rc == 0
+        assert np.array_equal(res["nmodes"], ob.nmodes), (name, code)
+        gpk, gnm, gsn = _finalized(p, res, ps.stats, code)
+        o5 = np.stack([ob.pk[l] for l in range(5)])
+        opk, onm, _, _, _ = api.finalize(p, oshot, onorm, o5.ravel(), ob.nmodes)
+        opk = opk.reshape(5, -1)
+        assert np.array_equal(gnm, onm)
+        if code in (0, 2, 5):
+            scales[code] = np.abs(opk[0]) + oshot / onorm
+            scale = scales[code]
+        else:
+            scale = None
+        if scale is None:
+            # cross: geometric mean of the autos (computed by the oracle)
+            sa = []
+            for auto in (0, 2):
+                op2 = oracle_params(oracle, p)
+                op2.momentum = auto
+                rc2, ob2, _, _, s2, n2 = oracle.run(op2, pos, vel)
+                a5 = np.stack([ob2.pk[l] for l in range(5)])
+                apk = api.finalize(p, s2, n2, a5.ravel(), ob2.nmodes)[0].reshape(5, -1)
+                sa.append(np.abs(apk[0]) + s2 / n2)
+            scale = np.sqrt(sa[0] * sa[1])
+        ls = (0, 1, 2, 3, 4) if p.odd_multipoles else (0, 2, 4)
+        for l in ls:
+            rel = np.abs(gpk[l] - opk[l]) / np.maximum(scale, 1e-300)
+            say("  code", code, "l", l, "max rel", rel.max())
+            assert rel.max() <= 1e-10, (name, code, l, rel.max())
+    ps.close()
+
+
+def test_fp32_form_error_table():
+    """FP32 form vs FP64 form on the same (float32-representable) catalogue:
+    every l and k bin below k_Nyq/2 within 1e-4 of the FP64 scale."""
+    say("fp32 vs fp64 error table")
+    n = 64
+    kw = dict(nx=n, ny=n, nz=n, interp_order=3, do_interlacing=1, momentum=1, extra_spectra=3,
+              nk=20, maxk=0.2)
+    pos, vel = mocks.zeldovich(64, L, 5)
+    pos = pos.astype(np.float32).astype(np.float64)
+    vel = vel.astype(np.float32).astype(np.float64)
+    out = {}
+    for prec in (api.FP64, api.FP32):
+        p = api.make_params(precision=prec, **kw)
+        ps = api.PowerSpectrum(p)
+        ps.set_particles(mocks.xyzv(pos, vel, 0))
+        res = ps.power()
+        out[prec] = {c: _finalized(p, res, ps.stats, c) for c in res["codes"]}
+        ps.close()
+    kc = (np.arange(20) + 0.5) * 0.01
+    knyq = np.pi * n / L
+    sel = kc < knyq / 2
+    s00 = np.abs(out[api.FP64][0][0][0]) + out[api.FP64][0][2]
+    s11 = np.abs(out[api.FP64][2][0][0]) + out[api.FP64][2][2]
+    for code in (0, 1, 2):
+        scale = s00 if code == 0 else (s11 if code == 2 else np.sqrt(s00 * s11))
+        for l in (0, 2, 4):
+            a = out[api.FP32][code][0][l]
+            b = out[api.FP64][code][0][l]
+            rel = np.abs(a - b) / scale
+            say("  code", code, "l", l, "max rel (k<kNyq/2)", rel[sel].max())
+            assert rel[sel].max() <= 1e-4
+
+
+def test_determinism():
+    """Same input twice: grids and bins bitwise identical (FP32 form)."""
+    say("determinism")
+    n = 64
+    p = api.make_params(nx=n, ny=n, nz=n, interp_order=3, do_interlacing=1, momentum=1,
+                        extra_spectra=3, precision=api.FP32, nk=20, maxk=0.2)
+    pos, vel = mocks.zeldovich(48, L, 9)
+    outs = []
+    for rep in range(2):
+        ps = api.PowerSpectrum(p)
+        ps.set_particles(mocks.xyzv(pos, vel, 0))
+        ps.assign()
+        grids = [ps.get_grid(s).copy() for s in range(4)]
+        ps.fft()
+        res = ps.bin()
+        outs.append((grids, res))
+        ps.close()
+    for s in range(4):
+        assert np.array_equal(outs[0][0][s].view(np.uint32), outs[1][0][s].view(np.uint32))
+    for code in outs[0][1]["codes"]:
+        assert np.array_equal(outs[0][1][code]["pk"].view(np.uint64),
+                              outs[1][1][code]["pk"].view(np.uint64))

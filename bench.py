@@ -1,0 +1,237 @@
+"""Benchmark of the periodic-box P_l(k) path on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): 134,217,728
+particles (a 512^3 lattice displaced by Zel'dovich displacements of a seeded
+Gaussian random field with P(k) = 2e4 (k/0.1)^-1.5, f = 0.5, aH = 100) in a
+2000 Mpc/h box, float32 positions/velocities, unit weights; 1024^3 grid,
+TSC + interlacing, density and line-of-sight momentum fields (reference
+Momentum = 1: grids delta, delta_int, p, p_int), spectra P00, P11 and the
+P01 cross, multipoles l = 0, 2, 4, dk = 0.005 h/Mpc (NK = 321).
+
+A step = one pass of the hot path with the catalogue resident in HBM:
+mass assignment (sort + tile deposit + mean subtraction), r2c FFT of the
+four grids, fused binning -> raw bins on the host.  `value` is the FP32
+form's ms per step; the FP64 plain form is timed beside it.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "ms per P_0,2,4(k) at 1024³ TSC+interlace, FP32 vs FP64 GPU; mocks/s batched"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(kernel, n_p, ng_reals, G, s):
+    """Compulsory HBM bytes per launch of each kernel (sort scratch excluded):
+    particles 16 B (x, y, z, v_los in FP32; 32 B in FP64), a grid N_g*s."""
+    bp = 4 * s
+    grid = ng_reals * s
+    return {
+        "deposit": n_p * bp + G * grid,        # read sorted particles, write G grids once
+        "fft_z": 2 * G * grid,                 # read + write every grid
+        "fft_y": 2 * G * grid,
+        "fft_x": 2 * G * grid,
+        "bin": G * grid,                       # read every grid once
+        "bucket_count": n_p * bp,
+        "bucket_scatter": 2 * n_p * bp,
+    }.get(kernel)
+
+
+def run_form(prec, parts, n_p, args, geom):
+    from capow_amd import api
+    nx = geom["n"]
+    p = api.make_params(nx=nx, ny=nx, nz=nx, xmin=0.0, xmax=geom["L"], ymin=0.0, ymax=geom["L"],
+                        zmin=0.0, zmax=geom["L"], interp_order=3, do_interlacing=1, los=0,
+                        momentum=1, nk=geom["nk"], maxk=geom["nk"] * geom["dk"], precision=prec,
+                        extra_spectra=api.SPEC_AUTO_A | api.SPEC_AUTO_B)
+    ps = api.PowerSpectrum(p)
+    ps.set_particles(parts)
+    import torch
+    for _ in range(args.warmup):
+        res = ps.power()
+    torch.cuda.synchronize()
+    ps.set_timing(True)
+    ps.reset_kernel_times()
+    # timing run with per-kernel HIP events (not the timed region)
+    res = ps.power()
+    ktimes = ps.kernel_times()
+    ps.set_timing(False)
+    torch.cuda.synchronize()
+    steps = args.steps if prec == api.FP32 else max(1, args.fp64_steps)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        res = ps.power()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    out = dict(ms=ms, kernels=ktimes, res=res, stats=ps.stats, params=p, steps=steps)
+    ps.close()
+    return out
+
+
+def cpu_baseline(geom_small, nl_small):
+    """The FP64 oracle on a bounded sample of the same workload: a 256^3
+    grid with the catalogue thinned to 1/64 (2,097,152 particles = 128^3
+    lattice, same box and spectrum), TSC + interlacing, Momentum = 1 grids,
+    three spectra.  Stage times are scaled to 1024^3 / 134M particles:
+    assignment x64 (particles), FFT x64*30/24 (N log N), binning x64."""
+    import ctypes as C
+    from capow_amd import mocks
+    from oracle import pyoracle as O
+    L = geom_small["L"]
+    n = geom_small["n"]
+    pos, vel = mocks.zeldovich(nl_small, L, 1)
+    pos = pos.astype(np.float32).astype(np.float64)
+    vel = vel.astype(np.float32).astype(np.float64)
+    p = O.params(nx=n, ny=n, nz=n, xmax=L, ymax=L, zmax=L, interp_order=3, do_interlacing=1,
+                 momentum=1, nk=geom_small["nk"], maxk=geom_small["nk"] * geom_small["dk"])
+    t0 = time.perf_counter()
+    rc, grids, st = O.assign(p, pos, vel)
+    alpha, shot, norm = O.scalars(p, st)
+    O.mean_subtract(p, alpha, grids)
+    t1 = time.perf_counter()
+    for g in grids:
+        O.fft_r2c_3d(g, n, n, n)
+    t2 = time.perf_counter()
+    for code in (1, 0, 2):
+        p.momentum = code
+        # the oracle bins one spectrum per pass, as the reference does
+        O.compute_power(p, grids if code == 1 else [grids[0], grids[1], None, None]
+                        if code == 0 else [grids[2], grids[3], None, None])
+    t3 = time.perf_counter()
+    scale_p = (512 ** 3) / float(nl_small ** 3)
+    scale_g = (1024 / n) ** 3
+    fft_scale = scale_g * (np.log2(1024 ** 3) / np.log2(n ** 3))
+    est = (t1 - t0) * scale_p + (t2 - t1) * fft_scale + (t3 - t2) * scale_g
+    del C
+    return dict(value=est * 1e3, unit="ms", cores=1, kind="port",
+                sample=("FP64 oracle, 1 thread: %d^3 grid, %d particles (Zel'dovich %d^3 "
+                        "lattice, L=2000), TSC+interlace, Momentum=1 grids, P01+P00+P11; "
+                        "measured assign %.2f s, 4 FFTs %.2f s, 3 binnings %.2f s; scaled to "
+                        "1024^3/134M (x%.0f particles, x%.0f FFT, x%.0f modes)" %
+                        (n, nl_small ** 3, nl_small, t1 - t0, t2 - t1, t3 - t2, scale_p,
+                         fft_scale, scale_g)))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--fp64-steps", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--grid", type=int, default=1024, help="grid size (1024 = the workload)")
+    ap.add_argument("--lattice", type=int, default=512)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: the periodic P_l(k) path runs on a single MI355X "
+              "(multi-GPU sharding is a design note, not built); run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from capow_amd import api, mocks
+    api.lib()
+    L = 2000.0
+    n = args.grid
+    dk = 0.005
+    nk = int(np.floor(np.pi * n / L / dk))
+    geom = dict(n=n, L=L, dk=dk, nk=nk)
+    log("generating Zel'dovich catalogue %d^3 on the GPU" % args.lattice)
+    t0 = time.perf_counter()
+    parts32 = mocks.zeldovich_torch(args.lattice, L, seed=1, device="cuda", dtype=torch.float32)
+    torch.cuda.synchronize()
+    n_p = parts32.shape[0]
+    log("catalogue: %d particles in %.1f s" % (n_p, time.perf_counter() - t0))
+
+    log("FP32 form")
+    f32 = run_form(api.FP32, parts32, n_p, args, geom)
+    log("  %.2f ms/step" % f32["ms"])
+    f64 = None
+    if not args.no_fp64:
+        parts64 = parts32.to(torch.float64)
+        log("FP64 plain form")
+        f64 = run_form(api.FP64, parts64, n_p, args, geom)
+        log("  %.2f ms/step" % f64["ms"])
+        del parts64
+        torch.cuda.empty_cache()
+
+    # roofline of the dominant kernel (FP32 form)
+    ng = n * n * 2 * (n // 2 + 1)
+    G = 4
+    kt = f32["kernels"]
+    rows = {}
+    for name, (ms, launches) in kt.items():
+        b = algorithmic_bytes(name, n_p, ng, G, 4)
+        if b is None:
+            rows[name] = dict(ms=ms, launches=launches)
+            continue
+        avg = ms / launches
+        rows[name] = dict(ms=ms, launches=launches, alg_bytes=b, gbs=b / (avg * 1e-3) / 1e9)
+    dom = max((k for k in rows if "alg_bytes" in rows[k]), key=lambda k: rows[k]["ms"])
+    d = rows[dom]
+    avg_ms = d["ms"] / d["launches"]
+    roofline = dict(bound="hbm", kernel=dom, achieved=round(d["gbs"], 1), peak=HBM_PEAK_GBS,
+                    unit="GB/s", frac=round(d["gbs"] / HBM_PEAK_GBS, 4), traffic=None,
+                    avg_launch_ms=round(avg_ms, 4), alg_bytes_per_launch=d["alg_bytes"])
+    assign_ms = sum(kt[k][0] for k in ("bucket_count", "stats_final", "scan", "bucket_scatter",
+                                        "deposit") if k in kt)
+    assign_bytes = n_p * 16 + G * ng * 4
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("CPU baseline (oracle sample)")
+        small = dict(n=256, L=L, dk=dk, nk=int(np.floor(np.pi * 256 / L / dk)))
+        cpu = cpu_baseline(small, 128)
+        log("  est %.0f ms" % cpu["value"])
+
+    res = f32["res"]
+    out = {
+        "metric": METRIC,
+        "value": round(f32["ms"], 3),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": f32["steps"],
+        "warmup": args.warmup,
+        "ms_per_step": round(f32["ms"], 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Zel'dovich catalogue generated on the GPU)",
+        "config": {
+            "workload": ("C2: %d particles (%d^3 lattice, Zel'dovich), L=%.0f Mpc/h, %d^3 grid, "
+                         "TSC+interlacing, density+momentum grids (4), P00/P11/P01, l=0,2,4, "
+                         "dk=%.3f, NK=%d" % (n_p, args.lattice, L, n, n, dk, nk)),
+            "particles": n_p, "grid": n, "interp": "TSC", "interlacing": True,
+            "spectra": res["codes"], "nk": nk,
+        },
+        "fp64_ms_per_step": round(f64["ms"], 3) if f64 else None,
+        "fp32_speedup_vs_fp64": round(f64["ms"] / f32["ms"], 3) if f64 else None,
+        "mparticles_per_s": round(n_p / (f32["ms"] * 1e-3) / 1e6, 1),
+        "assign_ms": round(assign_ms, 3),
+        "assign_frac_hbm": round(assign_bytes / (assign_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "kernels_fp32": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv)
+                             for kk, vv in v.items()} for k, v in rows.items()},
+        "kernels_fp64": ({k: dict(ms=round(v[0], 4), launches=v[1]) for k, v in f64["kernels"].items()}
+                         if f64 else None),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -60,7 +60,7 @@ __host__ __device__ inline int region_of(int local, int td, int lw, int hw) {
 //        LOS != 0, x: accept [MIN, MAX+d); y, z: accept [MIN, MAX)
 //        centre c = (int)(s+0.5), f = s-c, weights 0.5(0.5-f)^2,
 //        0.75-f^2, 0.5(0.5+f)^2; the low tap is kept only when its
-//        index c-1 > 0                                           (608-743)
+//        index, after the LOS = 0 wrap of -1 to N-1, is > 0      (608-743)
 // Periodic axes wrap taps mod N; non-periodic axes drop taps outside
 // [0, N).  EDGE_PERIODIC is textbook periodic assignment: positions wrapped
 // into the box, every tap kept.
@@ -129,8 +129,10 @@ __device__ __forceinline__ AxisOut<T> axis_taps(T u, T lo, T hi, T d, T L, int n
     T wl = T(0.5) * (T(0.5) - f) * (T(0.5) - f);
     T wc = T(0.75) - f * f;
     T wh = T(0.5) * (T(0.5) + f) * (T(0.5) + f);
-    // low tap kept iff its pre-wrap index c-1 > 0 (both periodic and not)
-    o.w[0] = (c - 1 > 0) ? wl : T(0);
+    // The reference keeps the low tap only when its index is > 0 after the
+    // LOS = 0 wrap (-1 -> N-1): periodic axes drop it exactly when c == 1
+    // (x: c-1 = slice 0; y/z: iy-1 = 0), non-periodic axes whenever c <= 1.
+    o.w[0] = (periodic ? (c != 1) : (c - 1 > 0)) ? wl : T(0);
     o.w[1] = wc;
     o.w[2] = wh;
     if (!periodic) {

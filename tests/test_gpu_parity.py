@@ -322,7 +322,10 @@ def test_determinism():
         outs.append((grids, res))
         ps.close()
     for s in range(4):
-        assert np.array_equal(outs[0][0][s].view(np.uint32), outs[1][0][s].view(np.uint32))
+        # real cells only: the row padding is not part of the real-space grid
+        a = outs[0][0][s].reshape(n, n, -1)[:, :, :n]
+        b = outs[1][0][s].reshape(n, n, -1)[:, :, :n]
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
     for code in outs[0][1]["codes"]:
         assert np.array_equal(outs[0][1][code]["pk"].view(np.uint64),
                               outs[1][1][code]["pk"].view(np.uint64))

@@ -77,6 +77,20 @@ def test_tsc_drops_low_tap_at_index_zero_y(oracle):
     assert g.sum() == pytest.approx(0.68, abs=1e-14)
 
 
+def test_tsc_periodic_wrap_low_y_keeps_low_tap(oracle):
+    # y = 0.3: centre 0, the low tap (-1) wraps to cell 15 and is kept
+    # (iyneighlow < 0 -> NY-1, then the "> 0" guard passes)
+    acc, g = deposit(oracle, 3, 8.0, 0.3, 8.0)
+    np.testing.assert_allclose([g[:, 15].sum(), g[:, 0].sum(), g[:, 1].sum()],
+                               [0.02, 0.66, 0.32], atol=1e-14)
+    acc, g = deposit(oracle, 3, 8.0, 8.0, 0.3)
+    np.testing.assert_allclose([g[:, :, 15].sum(), g[:, :, 0].sum(), g[:, :, 1].sum()],
+                               [0.02, 0.66, 0.32], atol=1e-14)
+    # the same position without the wrap (LOS != 0) drops it
+    acc, g = deposit(oracle, 3, 8.0, 0.3, 8.0, los=2)
+    assert g.sum() == pytest.approx(0.98, abs=1e-14)
+
+
 def test_tsc_periodic_wrap_low_x(oracle):
     # x = 0.3 < dx: shifted by L to 16.3 -> centre 16 == cell 0, low 15, high 1
     acc, g = deposit(oracle, 3, 0.3, 8.0, 8.0)

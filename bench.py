@@ -159,12 +159,16 @@ def main():
     log("FP32 form")
     f32 = run_form(api.FP32, parts32, n_p, args, geom)
     log("  %.2f ms/step" % f32["ms"])
+    for k, (ms, la) in sorted(f32["kernels"].items(), key=lambda kv: -kv[1][0]):
+        log("    %-16s %9.3f ms  (%d launches)" % (k, ms, la))
     f64 = None
     if not args.no_fp64:
         parts64 = parts32.to(torch.float64)
         log("FP64 plain form")
         f64 = run_form(api.FP64, parts64, n_p, args, geom)
         log("  %.2f ms/step" % f64["ms"])
+        for k, (ms, la) in sorted(f64["kernels"].items(), key=lambda kv: -kv[1][0]):
+            log("    %-16s %9.3f ms  (%d launches)" % (k, ms, la))
         del parts64
         torch.cuda.empty_cache()
 
@@ -214,7 +218,7 @@ def main():
         "config": {
             "workload": ("C2: %d particles (%d^3 lattice, Zel'dovich), L=%.0f Mpc/h, %d^3 grid, "
                          "TSC+interlacing, density+momentum grids (4), P00/P11/P01, l=0,2,4, "
-                         "dk=%.3f, NK=%d" % (n_p, args.lattice, L, n, n, dk, nk)),
+                         "dk=%.3f, NK=%d" % (n_p, args.lattice, L, n, dk, nk)),
             "particles": n_p, "grid": n, "interp": "TSC", "interlacing": True,
             "spectra": res["codes"], "nk": nk,
         },

@@ -64,7 +64,7 @@ def zeldovich(nl, L=1000.0, seed=1, dtype=np.float64, amp=AMP, slope=SLOPE):
     pos = np.empty((nl, nl, nl, 3))
     vel = np.empty((nl, nl, nl, 3))
     for a, ka in enumerate((kx, ky, kzz)):
-        psi = np.fft.irfftn(1j * ka * dk * inv, s=(nl, nl, nl))
+        psi = np.fft.irfftn(1j * ka * dk * inv, s=(nl, nl, nl), axes=(0, 1, 2))
         qa = np.broadcast_to(q.reshape([nl if b == a else 1 for b in range(3)]), (nl, nl, nl))
         pos[..., a] = qa + psi
         vel[..., a] = F_GROWTH * AH * psi

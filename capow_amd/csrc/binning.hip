@@ -31,110 +31,118 @@ __device__ __forceinline__ double dmul(double a, double b) { return __dmul_rn(a,
 __device__ __forceinline__ double dadd(double a, double b) { return __dadd_rn(a, b); }
 __device__ __forceinline__ double ddiv(double a, double b) { return __ddiv_rn(a, b); }
 
-template <typename T>
-__device__ __forceinline__ void bin_mode(int i, uint32_t r, const BinArgsDev& a,
-                                         const typename CT<T>::t* __restrict__ A,
-                                         const typename CT<T>::t* __restrict__ Ai,
-                                         const typename CT<T>::t* __restrict__ B,
-                                         const typename CT<T>::t* __restrict__ Bi,
-                                         double* accS, double* acc2, double* cnt, double* cnt2,
-                                         int nl, int n2d, uint32_t plane) {
-  typedef typename CT<T>::t C;
-  const int NK = a.nk;
-  const int NZH = a.nzh;
-  const double twopi = 2.0 * 3.14159265358979323846;  // the reference's 2.0*M_PI
-    const uint32_t j = a.fd_nzh.div(r);
-    const int k = (int)(r - j * (uint32_t)NZH);
-    const double fx = a.fx[i], fy = a.fy[j], fz = a.fz[k];
-    const double fktot = dmul(twopi, sqrt(dadd(dadd(dmul(fx, fx), dmul(fy, fy)), dmul(fz, fz))));
-    double mu;
-    if (fktot > 0) {
-      double fl = a.los == 1 ? fx : (a.los == 2 ? fy : fz);
-      mu = ddiv(dmul(twopi, fl), fktot);
-    } else {
-      mu = 0.0;
-    }
-    int kbin;
-    if (a.logbins) {
-      if (!(fktot > 0)) return;
-      kbin = (int)ddiv((double)__double2float_rn(log10(fktot) - a.mink), a.binwidth);
-    } else {
-      kbin = (int)ddiv((double)__double2float_rn(fktot - a.mink), a.binwidth);
-    }
-    if (!((kbin >= 0) && (kbin < NK) && (fktot <= a.min_nyq))) return;
-
-    // Hermitian weight (src/main.c:422-429)
-    double NM = 2.0;
-    if (k == 0) {
-      if (j == 0) {
-        if ((i != 0) && (i != a.nx / 2)) NM = 1.0;
-      } else {
-        if (i != a.nx / 2) NM = 1.0;
-      }
-    }
-    double cor = a.corx[i] * a.cory[j] * a.corz[k];
-    const int64_t idx = (int64_t)i * plane + r;
-    C va = A[idx];
-    double ar = (double)va.x, ai = (double)va.y, br = 0, bi = 0;
-    double cr = 1, ci = 0;
-    if (a.interlace) {
-      cor *= 0.25;
-      // e^{i(ax+ay+az)} from the per-axis phases
-      double2 px = a.phx[i], py = a.phy[j], pz = a.phz[k];
-      double qr = px.x * py.x - px.y * py.y, qi = px.x * py.y + px.y * py.x;
-      cr = qr * pz.x - qi * pz.y;
-      ci = qr * pz.y + qi * pz.x;
-      C vai = Ai[idx];
-      ar += (double)vai.x * cr - (double)vai.y * ci;
-      ai += (double)vai.x * ci + (double)vai.y * cr;
-    }
-    if (a.hasB) {
-      C vb = B[idx];
-      br = (double)vb.x;
-      bi = (double)vb.y;
-      if (a.interlace) {
-        C vbi = Bi[idx];
-        br += (double)vbi.x * cr - (double)vbi.y * ci;
-        bi += (double)vbi.x * ci + (double)vbi.y * cr;
-      }
-    }
-    const double mu2 = mu * mu;
-    const double L2 = 1.5 * mu * mu - 0.5;
-    const double L4 = 4.375 * mu * mu * mu * mu - 3.75 * mu * mu + 0.375;
-    const double L3 = 2.5 * mu * mu * mu - 1.5 * mu;
-    (void)mu2;
-    int mubin = 0;
-    bool in2d = false;
-    if (a.out2d) {
-      mubin = (int)(((mu + 1.0) / 2.0) * (double)(float)a.nmu);
-      if (mubin >= a.nmu) mubin = a.nmu - 1;
-      int b2 = kbin * a.nmu + mubin;
-      in2d = (b2 >= 0) && (b2 < NK * a.nmu);
-      mubin = b2;
-    }
-    for (int s = 0; s < a.nspec; s++) {
-      double p;
-      switch (a.kind[s]) {
-        case SPEC_AUTO_A: p = ar * ar + ai * ai; break;
-        case SPEC_AUTO_B: p = br * br + bi * bi; break;
-        case SPEC_CROSS_IM: p = ar * bi - ai * br; break;
-        default: p = ar * br + ai * bi; break;
-      }
-      p *= cor;
-      const double np = NM * p;
-      double* b = accS + s * nl * NK;
-      atomicAdd(&b[0 * NK + kbin], np);
-      atomicAdd(&b[1 * NK + kbin], NM * L2 * p);
-      atomicAdd(&b[2 * NK + kbin], NM * L4 * p);
-      if (a.odd) {
-        atomicAdd(&b[3 * NK + kbin], NM * mu * p);
-        atomicAdd(&b[4 * NK + kbin], NM * L3 * p);
-      }
-      if (in2d) atomicAdd(&acc2[s * n2d + mubin], np);
-    }
-    atomicAdd(&cnt[kbin], NM);
-    if (in2d) atomicAdd(&cnt2[mubin], NM);
+// Bin index of one mode, bit-identical to (int)((float)(fktot - Mink)/binwidth)
+// (src/main.c:368-373): multiply by the reciprocal and fall back to the exact
+// division only when the quotient lies within 1e-9 of an integer, where the
+// two could truncate differently.
+__device__ __forceinline__ int kbin_of(double fdk, const BinArgsDev& a) {
+  const double q = fdk * a.inv_binwidth;
+  const double tq = trunc(q);
+  const double fr = q - tq;
+  if (fabs(fr) < 1e-9 || fabs(fr) > 1.0 - 1e-9) return (int)ddiv(fdk, a.binwidth);
+  return (int)tq;
 }
+
+// Per-mode binning: frequencies, |k|, mu, bin, NM, deconvolution, interlace
+// phase, power of each spectrum, Legendre weights, accumulation.
+template <typename T>
+__device__ __forceinline__ void bin_one(int i, uint32_t r, typename CT<T>::t va,
+                                        typename CT<T>::t vai, typename CT<T>::t vb,
+                                        typename CT<T>::t vbi, const BinArgsDev& a,
+                                        double* accS, double* acc2, double* cnt, double* cnt2,
+                                        int nl, int n2d) {
+  const int NK = a.nk;
+  const double twopi = 2.0 * 3.14159265358979323846;  // the reference's 2.0*M_PI
+  const uint32_t j = a.fd_nzh.div(r);
+  const int k = (int)(r - j * (uint32_t)a.nzh);
+  const double fx = a.fx[i], fy = a.fy[j], fz = a.fz[k];
+  const double fktot = dmul(twopi, sqrt(dadd(dadd(dmul(fx, fx), dmul(fy, fy)), dmul(fz, fz))));
+  int kbin;
+  if (a.logbins) {
+    if (!(fktot > 0)) return;
+    kbin = (int)ddiv((double)__double2float_rn(log10(fktot) - a.mink), a.binwidth);
+  } else {
+    kbin = kbin_of((double)__double2float_rn(fktot - a.mink), a);
+  }
+  if (!((kbin >= 0) && (kbin < NK) && (fktot <= a.min_nyq))) return;
+  double mu = 0.0;
+  if (fktot > 0) {
+    const double fl = a.los == 1 ? fx : (a.los == 2 ? fy : fz);
+    if (a.out2d) {
+      mu = ddiv(dmul(twopi, fl), fktot);   // exact: feeds the mu-bin index
+    } else {
+      double rk = __builtin_amdgcn_rcp(fktot);
+      rk = rk * (2.0 - fktot * rk);
+      rk = rk * (2.0 - fktot * rk);
+      mu = (twopi * fl) * rk;
+    }
+  }
+  // Hermitian weight (src/main.c:422-429)
+  double NM = 2.0;
+  if (k == 0) {
+    if (j == 0) {
+      if ((i != 0) && (i != a.nx / 2)) NM = 1.0;
+    } else {
+      if (i != a.nx / 2) NM = 1.0;
+    }
+  }
+  const double cor = a.corx[i] * a.cory[j] * a.corz[k];  // x0.25 folded in corz
+  double ar = (double)va.x, ai = (double)va.y, br = 0, bi = 0;
+  double cr = 1, ci = 0;
+  if (a.interlace) {
+    const double2 px = a.phx[i], py = a.phy[j], pz = a.phz[k];
+    const double qr = px.x * py.x - px.y * py.y, qi = px.x * py.y + px.y * py.x;
+    cr = qr * pz.x - qi * pz.y;
+    ci = qr * pz.y + qi * pz.x;
+    ar += (double)vai.x * cr - (double)vai.y * ci;
+    ai += (double)vai.x * ci + (double)vai.y * cr;
+  }
+  if (a.hasB) {
+    br = (double)vb.x;
+    bi = (double)vb.y;
+    if (a.interlace) {
+      br += (double)vbi.x * cr - (double)vbi.y * ci;
+      bi += (double)vbi.x * ci + (double)vbi.y * cr;
+    }
+  }
+  const double mu2 = mu * mu;
+  const double L2 = 1.5 * mu2 - 0.5;
+  const double L4 = 4.375 * mu2 * mu2 - 3.75 * mu2 + 0.375;
+  const double L3 = 2.5 * mu2 * mu - 1.5 * mu;
+  int b2 = 0;
+  bool in2d = false;
+  if (a.out2d) {
+    int mubin = (int)(((mu + 1.0) / 2.0) * (double)(float)a.nmu);
+    if (mubin >= a.nmu) mubin = a.nmu - 1;
+    b2 = kbin * a.nmu + mubin;
+    in2d = (b2 >= 0) && (b2 < NK * a.nmu);
+  }
+  const double wc = NM * cor;
+#pragma unroll 3
+  for (int s = 0; s < a.nspec; s++) {
+    double p;
+    switch (a.kind[s]) {
+      case SPEC_AUTO_A: p = ar * ar + ai * ai; break;
+      case SPEC_AUTO_B: p = br * br + bi * bi; break;
+      case SPEC_CROSS_IM: p = ar * bi - ai * br; break;
+      default: p = ar * br + ai * bi; break;
+    }
+    const double np = wc * p;
+    double* b = accS + s * nl * NK + kbin;
+    atomicAdd(&b[0], np);
+    atomicAdd(&b[NK], L2 * np);
+    atomicAdd(&b[2 * NK], L4 * np);
+    if (a.odd) {
+      atomicAdd(&b[3 * NK], mu * np);
+      atomicAdd(&b[4 * NK], L3 * np);
+    }
+    if (in2d) atomicAdd(&acc2[s * n2d + b2], np);
+  }
+  atomicAdd(&cnt[kbin], NM);
+  if (in2d) atomicAdd(&cnt2[b2], NM);
+}
+
+#define BIN_U 4   // consecutive modes per lane per iteration (loads issued together)
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_bin(const T* __restrict__ gA, const T* __restrict__ gAi,
@@ -149,32 +157,40 @@ __global__ __launch_bounds__(256) void k_bin(const T* __restrict__ gA, const T* 
   const int per_copy = F * NK + a.nspec * n2d + NK + n2d;  // doubles (counts as double)
   const int wave = threadIdx.x >> 6;
   double* my = sm + (wave % a.copies) * per_copy;
-  for (int i = threadIdx.x; i < per_copy * a.copies; i += blockDim.x) sm[i] = 0.0;
+  for (int q = threadIdx.x; q < per_copy * a.copies; q += blockDim.x) sm[q] = 0.0;
   __syncthreads();
   double* accS = my;                       // [s][l'][NK]
   double* acc2 = my + F * NK;              // [s][NK*NMU]
   double* cnt = acc2 + a.nspec * n2d;      // [NK]
   double* cnt2 = cnt + NK;                 // [NK*NMU]
 
-  const int NY = a.ny, NZH = a.nzh;
-  const uint32_t plane = (uint32_t)NY * (uint32_t)NZH;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const uint32_t plane = (uint32_t)a.ny * (uint32_t)a.nzh;
+  const int64_t total = (int64_t)a.nx * plane;
   const C* A = reinterpret_cast<const C*>(gA);
   const C* Ai = reinterpret_cast<const C*>(gAi);
   const C* B = reinterpret_cast<const C*>(gB);
   const C* Bi = reinterpret_cast<const C*>(gBi);
-
-  {
-    const int64_t m0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const C zero = {T(0), T(0)};
+  // lane handles BIN_U consecutive modes; groups are strided over the grid
+  const int64_t gstride = (int64_t)gridDim.x * blockDim.x * BIN_U;
+  for (int64_t m0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * BIN_U; m0 < total;
+       m0 += gstride) {
+    C va[BIN_U], vai[BIN_U], vb[BIN_U], vbi[BIN_U];
+#pragma unroll
+    for (int u = 0; u < BIN_U; u++) {
+      const int64_t m = m0 + u;
+      const bool ok = m < total;
+      va[u] = ok ? A[m] : zero;
+      vai[u] = (ok && a.interlace) ? Ai[m] : zero;
+      vb[u] = (ok && a.hasB) ? B[m] : zero;
+      vbi[u] = (ok && a.hasB && a.interlace) ? Bi[m] : zero;
+    }
     int i = (int)(m0 / plane);
     uint32_t r = (uint32_t)(m0 - (int64_t)i * plane);
-    const int si = (int)(stride / plane);
-    const uint32_t sr = (uint32_t)(stride - (int64_t)si * plane);
-    while (i < a.nx) {
-      bin_mode<T>(i, r, a, A, Ai, B, Bi, accS, acc2, cnt, cnt2, nl, n2d, plane);
-      i += si;
-      r += sr;
-      if (r >= plane) { r -= plane; i++; }
+#pragma unroll
+    for (int u = 0; u < BIN_U; u++) {
+      if (m0 + u < total) bin_one<T>(i, r, va[u], vai[u], vb[u], vbi[u], a, accS, acc2, cnt, cnt2, nl, n2d);
+      if (++r == plane) { r = 0; i++; }
     }
   }
   __syncthreads();

@@ -124,10 +124,12 @@ static bool make_plan(int64_t M, FftPlan* pl) {
   return M == 1;
 }
 
-static int pick_tile(int n, int tmax) {
-  if (n <= tmax) return n;
+// Largest tile extent <= tmax dividing n (at least 4 unless n itself fits);
+// `even` for the z extent, whose cells are written in pairs.
+static int pick_tile(int n, int tmax, bool even = false) {
+  if (n <= tmax && (!even || n % 2 == 0)) return n;
   for (int t = tmax; t >= 4; t--)
-    if (n % t == 0) return t;
+    if (n % t == 0 && (!even || t % 2 == 0)) return t;
   return -1;
 }
 
@@ -155,9 +157,9 @@ static int validate(const capow_params* p, std::string& m) {
     return CAPOW_ERR_UNSUPPORTED;
   }
   const int64_t dims[3] = {p->nx, p->ny, p->nz};
-  const int tmax[3] = {8, 8, 32};
+  const int tmax[3] = {16, 16, p->precision == CAPOW_FP64 ? 16 : 32};
   for (int a = 0; a < 3; a++) {
-    if (pick_tile((int)dims[a], tmax[a]) < 0) {
+    if (pick_tile((int)dims[a], tmax[a], a == 2) < 0) {
       m = "grid dimension has no tile divisor in [4, 32]";
       return CAPOW_ERR_UNSUPPORTED;
     }
@@ -189,6 +191,7 @@ static int setup(capow_ctx* c) {
     g.hi[a] = hi[a];
     g.L[a] = hi[a] - lo[a];
     g.d[a] = (hi[a] - lo[a]) / (double)(unsigned long long)g.n[a];  // src/main.c:1126-1128
+    g.invd[a] = 1.0 / g.d[a];
   }
   g.order = p.interp_order;
   g.interlace = p.do_interlacing;
@@ -203,14 +206,15 @@ static int setup(capow_ctx* c) {
   if (g.order == 3) { g.reach_lo = -1; g.reach_hi = g.interlace ? 2 : 1; }
   else if (g.order == 2) { g.reach_lo = 0; g.reach_hi = g.interlace ? 2 : 1; }
   else { g.reach_lo = 0; g.reach_hi = g.interlace ? 1 : 0; }
-  const int tmax[3] = {8, 8, 32};
+  // deposit tile: one phase of accumulators (2 fields) in <= 64 KB of LDS
+  const int tmax[3] = {16, 16, p.precision == CAPOW_FP64 ? 16 : 32};
   for (int a = 0; a < 3; a++) {
-    int t = pick_tile(g.n[a], tmax[a]);
+    int t = pick_tile(g.n[a], tmax[a], a == 2);
     if (t < 0) return fail(c, CAPOW_ERR_UNSUPPORTED, "grid dimension has no tile divisor in [4, 32]");
     g.tdim[a] = t;
     g.ntile[a] = g.n[a] / t;
   }
-  g.nbuckets = (int64_t)g.ntile[0] * g.ntile[1] * g.ntile[2] * 27;
+  g.nbuckets = (int64_t)g.ntile[0] * g.ntile[1] * g.ntile[2];
   c->fp64 = p.precision == CAPOW_FP64;
   c->esz = c->fp64 ? 8 : 4;
 
@@ -224,7 +228,7 @@ static int setup(capow_ctx* c) {
   // sort buffers
   HIPCHK(c, hipMalloc((void**)&c->counts, (size_t)g.nbuckets * 4));
   HIPCHK(c, hipMalloc((void**)&c->offsets, (size_t)(g.nbuckets + 1) * 4));
-  HIPCHK(c, hipMalloc((void**)&c->cursor, (size_t)g.nbuckets * 4));
+  HIPCHK(c, hipMalloc((void**)&c->cursor, (size_t)g.nbuckets * 2 * 4));
   int64_t nblk = (g.nbuckets + 4095) / 4096;
   HIPCHK(c, hipMalloc((void**)&c->bsum, (size_t)(nblk + 1) * 4));
   HIPCHK(c, hipMalloc((void**)&c->boff, (size_t)(nblk + 1) * 4));
@@ -283,6 +287,8 @@ static int setup(capow_ctx* c) {
     fill(0, ofx, ocx, opx, NX, (int)NX, sx, dx);
     fill(1, ofy, ocy, opy, NY, (int)NY, sy, dy);
     fill(2, ofz, ocz, opz, NZ, nzh, sz, dz);
+    if (p.do_interlacing)  // grid_cor *= 0.25 (src/main.c:395), folded into the z table
+      for (int k = 0; k < nzh; k++) tab[ocz + k] *= 0.25;
     HIPCHK(c, hipMalloc((void**)&c->bin_tabs, total * 8));
     HIPCHK(c, hipMemcpy(c->bin_tabs, tab.data(), total * 8, hipMemcpyHostToDevice));
     BinArgsDev& b = c->bin;
@@ -294,6 +300,7 @@ static int setup(capow_ctx* c) {
     b.los = p.los; b.interlace = p.do_interlacing; b.hasB = hasB;
     b.mink = p.mink;
     b.binwidth = (p.maxk - p.mink) / (double)p.nk;  // src/main.c:305
+    b.inv_binwidth = 1.0 / b.binwidth;
     double nyx = M_PI / dx, nyy = M_PI / dy, nyz = M_PI / dz, mn = nyx;
     if (nyy < mn) mn = nyy;
     if (nyz < mn) mn = nyz;
@@ -320,7 +327,7 @@ static int setup(capow_ctx* c) {
       return fail(c, CAPOW_ERR_UNSUPPORTED, "NK/NMU too large for the LDS bin arrays");
     b.copies = bytes * 4 <= lds_max ? 4 : (bytes * 2 <= lds_max ? 2 : 1);
     int64_t modes = (int64_t)NX * NY * nzh;
-    int64_t want = (modes + 255) / 256;
+    int64_t want = (modes + 1023) / 1024;   // 256 lanes x 4 modes per iteration
     c->bin_nwg = (int)(want < 1024 ? want : 1024);
     HIPCHK(c, hipMalloc((void**)&c->bin_partials, (size_t)c->bin_nwg * bytes));
     HIPCHK(c, hipMalloc((void**)&c->bin_out, bytes));
@@ -404,7 +411,7 @@ extern "C" int capow_set_particles(capow_ctx* c, const void* xyzv, const void* w
   if (n < 0 || (n > 0 && !xyzv)) return fail(c, CAPOW_ERR_ARG, "bad particle array");
   if ((dtype == CAPOW_DTYPE_F64) != c->fp64)
     return fail(c, CAPOW_ERR_ARG, "particle dtype must match the context precision");
-  if (n > 0xFFFFFFF0LL) return fail(c, CAPOW_ERR_UNSUPPORTED, "more than 2^32-16 particles");
+  if (n > (1LL << 29)) return fail(c, CAPOW_ERR_UNSUPPORTED, "more than 2^29 particles (32-bit copy offsets)");
   const size_t pb = (size_t)n * 4 * c->esz, wb = (size_t)n * c->esz;
   if (where == CAPOW_HOST) {
     int rc = ensure(c, &c->own_parts, &c->own_parts_cap, pb > 0 ? pb : 16);
@@ -426,12 +433,6 @@ extern "C" int capow_set_particles(capow_ctx* c, const void* xyzv, const void* w
     return fail(c, CAPOW_ERR_ARG, "where must be CAPOW_HOST or CAPOW_DEVICE");
   }
   c->n = n;
-  int rc = ensure(c, &c->sorted, &c->sorted_cap, pb > 0 ? pb : 16);
-  if (rc) return rc;
-  if (w) {
-    rc = ensure(c, &c->sorted_w, &c->sorted_w_cap, wb > 0 ? wb : 16);
-    if (rc) return rc;
-  }
   c->state = 1;
   return CAPOW_OK;
 }
@@ -471,14 +472,28 @@ extern "C" int capow_assign(capow_ctx* c, capow_assign_stats* st) {
   a.cursor = c->cursor;
   a.bsum = c->bsum;
   a.boff = c->boff;
-  a.sorted = c->sorted;
-  a.sorted_w = c->sorted_w;
   a.partials = c->partials;
   a.gmax = c->gmax;
   a.errs = c->errs;
   a.stats = c->stats;
   for (int s = 0; s < 4; s++) a.grid[s] = c->grid[s];
-  int rc = run_deposit(a, c->fp64);
+  int rc = run_count(a, c->fp64);
+  if (rc) return fail(c, rc, std::string("deposit count failed: ") + hipGetErrorString(hipGetLastError()));
+  // total particle copies (one per reached tile) sizes the sorted buffer
+  uint32_t total = 0;
+  HIPCHK(c, hipMemcpyAsync(&total, c->offsets + g.nbuckets, 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const size_t rows = (size_t)total > 0 ? (size_t)total : 1;
+  rc = ensure(c, &c->sorted, &c->sorted_cap, rows * 4 * c->esz);
+  if (rc) return rc;
+  if (c->wts) {
+    rc = ensure(c, &c->sorted_w, &c->sorted_w_cap, rows * c->esz);
+    if (rc) return rc;
+  }
+  a.sorted = c->sorted;
+  a.sorted_w = c->sorted_w;
+  a.sorted_cap = (int64_t)(c->sorted_cap / (4 * c->esz));
+  rc = run_scatter_deposit(a, c->fp64);
   if (rc) return fail(c, rc, std::string("deposit launch failed: ") + hipGetErrorString(hipGetLastError()));
   double s[STAT_TOTAL];
   unsigned int e[4];

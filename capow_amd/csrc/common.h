@@ -17,6 +17,7 @@ struct Geom {
   int64_t rowlen;        // 2*(NZ/2+1) reals per padded row
   double lo[3], hi[3];   // box
   double d[3];           // cell size (MAX-MIN)/N  (src/main.c:1126-1128)
+  double invd[3];        // 1/d (FP32 form)
   double L[3];           // MAX-MIN
   int order;             // 1 NGP, 2 CIC, 3 TSC
   int interlace;         // 0/1
@@ -32,17 +33,8 @@ struct Geom {
   int tdim[3];           // tile extent per axis (cells)
   int ntile[3];          // tiles per axis
   int reach_lo, reach_hi;// taps of a particle lie in [key+reach_lo, key+reach_hi]
-  int64_t nbuckets;      // ntiles * 27
+  int64_t nbuckets;      // ntiles (one bucket per deposit tile)
 };
-
-// Region of a key inside its tile along one axis: 0 = low edge (needed by the
-// tile below... i.e. the previous tile's high neighbour), 1 = middle,
-// 2 = high edge.  Low width = -reach_lo, high width = reach_hi.
-__host__ __device__ inline int region_of(int local, int td, int lw, int hw) {
-  if (local < lw) return 0;
-  if (local >= td - hw) return 2;
-  return 1;
-}
 
 // ---------------------------------------------------------------------------
 // Per-axis tap rules.
@@ -77,9 +69,18 @@ struct AxisOut {
 template <typename T>
 __device__ __forceinline__ int fl2i(T v) { return (int)v; }  // trunc toward 0
 
+// Scaled coordinate (u - lo)/d.  FP64 divides exactly as the reference;
+// the FP32 form multiplies by the reciprocal (cell membership within an ulp
+// of a cell edge is part of the FP32 form's error).
+template <typename T>
+__device__ __forceinline__ T scaled(T r, T d, T invd) {
+  if constexpr (sizeof(T) == 4) return r * invd;
+  else return r / d;
+}
+
 template <typename T, int ORDER>
 __device__ __forceinline__ AxisOut<T> axis_taps(T u, T lo, T hi, T d, T L, int n,
-                                                bool is_x, int los, int edge) {
+                                                bool is_x, int los, int edge, T invd) {
   AxisOut<T> o;
   o.w[0] = o.w[1] = o.w[2] = T(0);
   if (edge == CAPOW_EDGE_PERIODIC) {
@@ -89,7 +90,7 @@ __device__ __forceinline__ AxisOut<T> axis_taps(T u, T lo, T hi, T d, T L, int n
       r = r - L * floor(r / L);
       if (r >= L) r = T(0);
     }
-    T s = r / d;
+    T s = scaled<T>(r, d, invd);
     o.ok = true;
     if (ORDER == 3) {
       int c = (int)floor(s + T(0.5));
@@ -122,7 +123,7 @@ __device__ __forceinline__ AxisOut<T> axis_taps(T u, T lo, T hi, T d, T L, int n
     } else {
       o.ok = !((u < lo) || (u >= hi));
     }
-    T s = (u - lo) / d;
+    T s = scaled<T>(u - lo, d, invd);
     int c = fl2i<T>(s + T(0.5));
     T f = s - (T)c;
     o.first = c - 1;
@@ -145,7 +146,7 @@ __device__ __forceinline__ AxisOut<T> axis_taps(T u, T lo, T hi, T d, T L, int n
     return o;
   } else if (ORDER == 2) {
     o.ok = !((u < lo) || (u >= hi));
-    T s = (u - lo) / d;
+    T s = scaled<T>(u - lo, d, invd);
     int c = fl2i<T>(s);
     T f = s - (T)c;
     o.first = c;
@@ -158,7 +159,7 @@ __device__ __forceinline__ AxisOut<T> axis_taps(T u, T lo, T hi, T d, T L, int n
     return o;
   } else {
     o.ok = !((u < lo) || (u >= hi));
-    T s = (u - lo) / d;
+    T s = scaled<T>(u - lo, d, invd);
     int c = fl2i<T>(s);
     if (!is_x && c == n) c = n - 1;   // src/read_data.c:534-535
     o.first = c;
@@ -172,7 +173,7 @@ __device__ __forceinline__ AxisOut<T> axis_taps(T u, T lo, T hi, T d, T L, int n
 // shifted by the TSC periodic wrap where applicable).
 template <typename T, int ORDER>
 __device__ __forceinline__ int axis_key(T u, T lo, T hi, T d, T L, int n, bool is_x,
-                                        int los, int edge) {
+                                        int los, int edge, T invd) {
   T s;
   if (edge == CAPOW_EDGE_PERIODIC) {
     T r = u - lo;
@@ -180,10 +181,10 @@ __device__ __forceinline__ int axis_key(T u, T lo, T hi, T d, T L, int n, bool i
       r = r - L * floor(r / L);
       if (r >= L) r = T(0);
     }
-    s = r / d;
+    s = scaled<T>(r, d, invd);
   } else {
     if (ORDER == 3 && is_x && los == 0 && (u - lo < d)) u += L;
-    s = (u - lo) / d;
+    s = scaled<T>(u - lo, d, invd);
   }
   int c = (ORDER == 3) ? (int)floor(s + T(0.5)) : (int)floor(s);
   if (edge == CAPOW_EDGE_PERIODIC || los == 0) {

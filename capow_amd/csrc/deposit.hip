@@ -6,15 +6,18 @@
 //
 // Design (one GPU holds the whole grid):
 //   1. k_bucket<COUNT>: per particle, apply the RSD shift, decide acceptance
-//      of the main and interlaced deposits, compute the bucket
-//      (deposit tile x 27 edge regions) and count it; block partials of the
-//      global sums (N gridded, sum w v^n) and the max |field| per field.
+//      of the main and interlaced deposits, find the deposit tiles its taps
+//      can reach (its own tile, plus the neighbour tile along each axis when
+//      the particle sits within the tap reach of that tile's edge: at most 8
+//      tiles) and count one copy per tile; block partials of the global sums
+//      (N gridded, sum w v^n) and the max |field| per field.
 //   2. k_stats_final: fixed-order reduction of the block partials, alpha,
 //      per-field fixed-point scales.
-//   3. exclusive scan of the bucket counts.
-//   4. k_bucket<SCATTER>: counting-sort scatter of the shifted particles.
-//   5. k_deposit: one workgroup per output tile gathers the particles of its
-//      own tile and of the edge regions of its 26 neighbours, accumulates
+//   3. exclusive scan of the per-tile copy counts.
+//   4. k_bucket<SCATTER>: counting-sort scatter of the shifted particles, one
+//      copy per reached tile, so every tile's particles are one contiguous
+//      range.
+//   5. k_deposit: one workgroup per output tile reads its range, accumulates
 //      every tap that lands in the tile into LDS, then writes the tile once
 //      (value - alpha for density fields).  No grid memset, no global
 //      atomics on the grid.
@@ -23,6 +26,8 @@
 // < 2^41 quanta) and accumulated in LDS as an integer-valued double.  Sums
 // of such values below 2^53 quanta are exact, so the grid does not depend
 // on the order in which particles or taps are added.
+#include <type_traits>
+
 #include "common.h"
 #include "kernels.h"
 
@@ -62,6 +67,19 @@ __device__ __forceinline__ void store4(T* p, int64_t i, T x, T y, T z, T v) {
   reinterpret_cast<V*>(p)[i] = q;
 }
 
+template <typename T>
+__device__ __forceinline__ void store2(T* p, T a, T b) {
+  if constexpr (sizeof(T) == 4) {
+    float2 q;
+    q.x = a; q.y = b;
+    *reinterpret_cast<float2*>(p) = q;
+  } else {
+    double2 q;
+    q.x = a; q.y = b;
+    *reinterpret_cast<double2*>(p) = q;
+  }
+}
+
 // Per-particle classification shared by the count and scatter passes.
 template <typename T, int ORDER>
 struct PClass {
@@ -69,7 +87,28 @@ struct PClass {
   bool acc_main, acc_int;
   int64_t bucket;  // -1: rejected by every deposit
   bool reach_ok;
+  int tile[3][2];  // reached tiles per axis
+  int nt[3];       // how many (1 or 2)
 };
+
+// Every kept tap must lie within [key+reach_lo, key+reach_hi] (mod N on
+// periodic axes), otherwise a tile that owns it would not read the particle.
+template <typename T, int ORDER>
+__device__ __forceinline__ bool reach_ok(const AxisOut<T>& o, int key, int n, const Geom& g) {
+  bool ok = true;
+#pragma unroll
+  for (int t = 0; t < ORDER; t++) {
+    if (o.w[t] == T(0)) continue;
+    int off = o.first + t - key;
+    if (g.periodic) {
+      off %= n;
+      if (off < 0) off += n;
+      if (off > n / 2) off -= n;
+    }
+    if (off < g.reach_lo || off > g.reach_hi) ok = false;
+  }
+  return ok;
+}
 
 template <typename T, int ORDER>
 __device__ __forceinline__ PClass<T, ORDER> classify(T x, T y, T z, T v, const Geom& g) {
@@ -86,16 +125,16 @@ __device__ __forceinline__ PClass<T, ORDER> classify(T x, T y, T z, T v, const G
   const T hi0 = (T)g.hi[0], hi1 = (T)g.hi[1], hi2 = (T)g.hi[2];
   const T d0 = (T)g.d[0], d1 = (T)g.d[1], d2 = (T)g.d[2];
   const T L0 = (T)g.L[0], L1 = (T)g.L[1], L2 = (T)g.L[2];
-  AxisOut<T> mx = axis_taps<T, ORDER>(x, lo0, hi0, d0, L0, g.n[0], true, g.los, g.edge);
-  AxisOut<T> my = axis_taps<T, ORDER>(y, lo1, hi1, d1, L1, g.n[1], false, g.los, g.edge);
-  AxisOut<T> mz = axis_taps<T, ORDER>(z, lo2, hi2, d2, L2, g.n[2], false, g.los, g.edge);
+  AxisOut<T> mx = axis_taps<T, ORDER>(x, lo0, hi0, d0, L0, g.n[0], true, g.los, g.edge, (T)g.invd[0]);
+  AxisOut<T> my = axis_taps<T, ORDER>(y, lo1, hi1, d1, L1, g.n[1], false, g.los, g.edge, (T)g.invd[1]);
+  AxisOut<T> mz = axis_taps<T, ORDER>(z, lo2, hi2, d2, L2, g.n[2], false, g.los, g.edge, (T)g.invd[2]);
   c.acc_main = mx.ok && my.ok && mz.ok;
-  AxisOut<T> ix, iy, iz;
+  AxisOut<T> ix = mx, iy = my, iz = mz;
   c.acc_int = false;
   if (g.interlace) {
-    ix = axis_taps<T, ORDER>(x + d0 / T(2), lo0, hi0, d0, L0, g.n[0], true, g.los, g.edge);
-    iy = axis_taps<T, ORDER>(y + d1 / T(2), lo1, hi1, d1, L1, g.n[1], false, g.los, g.edge);
-    iz = axis_taps<T, ORDER>(z + d2 / T(2), lo2, hi2, d2, L2, g.n[2], false, g.los, g.edge);
+    ix = axis_taps<T, ORDER>(x + d0 / T(2), lo0, hi0, d0, L0, g.n[0], true, g.los, g.edge, (T)g.invd[0]);
+    iy = axis_taps<T, ORDER>(y + d1 / T(2), lo1, hi1, d1, L1, g.n[1], false, g.los, g.edge, (T)g.invd[1]);
+    iz = axis_taps<T, ORDER>(z + d2 / T(2), lo2, hi2, d2, L2, g.n[2], false, g.los, g.edge, (T)g.invd[2]);
     c.acc_int = ix.ok && iy.ok && iz.ok;
   }
   c.reach_ok = true;
@@ -104,40 +143,43 @@ __device__ __forceinline__ PClass<T, ORDER> classify(T x, T y, T z, T v, const G
     return c;
   }
   int key[3];
-  key[0] = axis_key<T, ORDER>(x, lo0, hi0, d0, L0, g.n[0], true, g.los, g.edge);
-  key[1] = axis_key<T, ORDER>(y, lo1, hi1, d1, L1, g.n[1], false, g.los, g.edge);
-  key[2] = axis_key<T, ORDER>(z, lo2, hi2, d2, L2, g.n[2], false, g.los, g.edge);
-  // Self-check: every kept tap must lie within [key+reach_lo, key+reach_hi]
-  // (mod N on periodic axes), otherwise a tile that owns it would not read
-  // this particle.
-  const AxisOut<T>* ax[2][3] = {{&mx, &my, &mz}, {&ix, &iy, &iz}};
-  for (int gi = 0; gi < 2; gi++) {
-    if (gi == 0 ? !c.acc_main : !c.acc_int) continue;
-    for (int a = 0; a < 3; a++) {
-      const AxisOut<T>* o = ax[gi][a];
-      for (int t = 0; t < ORDER; t++) {
-        if (o->w[t] == T(0)) continue;
-        int idx = o->first + t;
-        int off = idx - key[a];
-        if (g.periodic) {
-          int n = g.n[a];
-          off %= n;
-          if (off < 0) off += n;
-          if (off > n / 2) off -= n;
-        }
-        if (off < g.reach_lo || off > g.reach_hi) c.reach_ok = false;
+  key[0] = axis_key<T, ORDER>(x, lo0, hi0, d0, L0, g.n[0], true, g.los, g.edge, (T)g.invd[0]);
+  key[1] = axis_key<T, ORDER>(y, lo1, hi1, d1, L1, g.n[1], false, g.los, g.edge, (T)g.invd[1]);
+  key[2] = axis_key<T, ORDER>(z, lo2, hi2, d2, L2, g.n[2], false, g.los, g.edge, (T)g.invd[2]);
+  if (c.acc_main) {
+    c.reach_ok &= reach_ok<T, ORDER>(mx, key[0], g.n[0], g);
+    c.reach_ok &= reach_ok<T, ORDER>(my, key[1], g.n[1], g);
+    c.reach_ok &= reach_ok<T, ORDER>(mz, key[2], g.n[2], g);
+  }
+  if (c.acc_int) {
+    c.reach_ok &= reach_ok<T, ORDER>(ix, key[0], g.n[0], g);
+    c.reach_ok &= reach_ok<T, ORDER>(iy, key[1], g.n[1], g);
+    c.reach_ok &= reach_ok<T, ORDER>(iz, key[2], g.n[2], g);
+  }
+  // Deposit tiles reached along each axis: the key's tile, plus the tile
+  // above when the key lies within reach_hi of its upper edge, plus the tile
+  // below when it lies within -reach_lo of its lower edge.
+  const int lw = -g.reach_lo, hw = g.reach_hi;
+  for (int a = 0; a < 3; a++) {
+    const int td = g.tdim[a], nt = g.ntile[a];
+    const int t0 = key[a] / td;
+    const int local = key[a] - t0 * td;
+    c.tile[a][0] = t0;
+    c.nt[a] = 1;
+    if (nt > 1) {
+      if (local >= td - hw) {
+        int q = t0 + 1;
+        if (q >= nt) q = g.periodic ? q - nt : -1;
+        if (q >= 0) c.tile[a][c.nt[a]++] = q;
+      }
+      if (local < lw) {
+        int q = t0 - 1;
+        if (q < 0) q = g.periodic ? q + nt : -1;
+        if (q >= 0) c.tile[a][c.nt[a]++] = q;
       }
     }
   }
-  int tile[3], reg[3];
-  const int lw = -g.reach_lo, hw = g.reach_hi;
-  for (int a = 0; a < 3; a++) {
-    tile[a] = key[a] / g.tdim[a];
-    int local = key[a] - tile[a] * g.tdim[a];
-    reg[a] = g.ntile[a] == 1 ? 1 : region_of(local, g.tdim[a], lw, hw);
-  }
-  int64_t t = ((int64_t)tile[2] * g.ntile[1] + tile[1]) * g.ntile[0] + tile[0];
-  c.bucket = t * 27 + reg[0] + 3 * reg[1] + 9 * reg[2];
+  c.bucket = 0;
   return c;
 }
 
@@ -155,13 +197,46 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
   return r;
 }
 
+// Adds one to ctr[key] for every active lane with has = true.  Lanes of a
+// wavefront that share a key are merged into one atomic (the leader adds the
+// group size); after 4 groups the remaining lanes fall back to one atomic
+// each.  RET: returns each lane's slot (old value + rank within its group).
+template <bool RET>
+__device__ __forceinline__ uint32_t wave_inc(uint32_t* __restrict__ ctr, int key, bool has) {
+  const int lane = __lane_id();
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  unsigned long long pending = __ballot(has);
+  uint32_t slot = 0;
+  for (int it = 0; pending; it++) {
+    if (it == 4) {
+      if ((pending >> lane) & 1ull) {
+        if (RET) slot = atomicAdd(&ctr[key], 1u);
+        else atomicAdd(&ctr[key], 1u);
+      }
+      break;
+    }
+    const int leader = __ffsll((long long)pending) - 1;
+    const int lk = __builtin_amdgcn_readlane(key, leader);
+    const unsigned long long m = __ballot(has && key == lk) & pending;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(&ctr[lk], (uint32_t)__popcll(m));
+    if (RET) {
+      base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
+      if ((m >> lane) & 1ull) slot = base + (uint32_t)__popcll(m & lt);
+    }
+    pending &= ~m;
+  }
+  return slot;
+}
+
 template <typename T, int ORDER, bool SCATTER>
 __global__ __launch_bounds__(256) void k_bucket(const T* __restrict__ parts,
                                                 const T* __restrict__ wts, int64_t n,
                                                 Geom g, uint32_t* __restrict__ counts,
                                                 uint32_t* __restrict__ cursor,
+                                                const uint32_t* __restrict__ off,
                                                 T* __restrict__ sorted,
-                                                T* __restrict__ sorted_w,
+                                                T* __restrict__ sorted_w, int64_t cap,
                                                 double* __restrict__ partials,
                                                 unsigned long long* __restrict__ gmax,
                                                 unsigned int* __restrict__ errs) {
@@ -175,16 +250,34 @@ __global__ __launch_bounds__(256) void k_bucket(const T* __restrict__ parts,
     load4<T>(parts, i, x, y, z, v);
     T w = g.has_w ? wts[i] : T(1);
     PClass<T, ORDER> c = classify<T, ORDER>(x, y, z, v, g);
-    if (SCATTER) {
-      if (c.bucket >= 0) {
-        uint32_t pos = atomicAdd(&cursor[c.bucket], 1u);
-        if ((int64_t)pos < n) {
+    const bool take = c.bucket >= 0;
+    // one copy per reached tile, in a fixed order (iz, iy, ix)
+    for (int q = 0; q < 8; q++) {
+      const int qx = q & 1, qy = (q >> 1) & 1, qz = q >> 2;
+      const bool has = take && qx < c.nt[0] && qy < c.nt[1] && qz < c.nt[2];
+      if (!__any(has)) continue;
+      int tile = -1;
+      if (has) tile = (c.tile[2][qz] * g.ntile[1] + c.tile[1][qy]) * g.ntile[0] + c.tile[0][qx];
+      if (SCATTER) {
+        // own copies fill each tile's range from the front, ghost copies
+        // from the back, so deposit waves are mostly homogeneous
+        uint32_t pos;
+        if (q == 0) {
+          const uint32_t r = wave_inc<true>(cursor, tile, has);
+          pos = has ? off[tile] + r : 0u;
+        } else {
+          const uint32_t r = wave_inc<true>(cursor + g.nbuckets, tile, has);
+          pos = has ? off[tile + 1] - 1u - r : 0u;
+        }
+        if (has && (int64_t)pos < cap) {
           store4<T>(sorted, pos, c.x, c.y, c.z, v);
           if (g.has_w) sorted_w[pos] = w;
         }
+      } else {
+        wave_inc<false>(counts, tile, has);
       }
-    } else {
-      if (c.bucket >= 0) atomicAdd(&counts[c.bucket], 1u);
+    }
+    if (!SCATTER) {
       if (!c.reach_ok) reach_bad++;
       // src/read_data.c:81-88 (on the shifted position)
       if ((c.x < (T)g.lo[0]) || (c.x > (T)g.hi[0]) || (c.y < (T)g.lo[1]) ||
@@ -210,6 +303,7 @@ __global__ __launch_bounds__(256) void k_bucket(const T* __restrict__ parts,
   }
   if (!SCATTER) {
     double r[7] = {s_ngrid, s_w, s_v1, s_v2, s_v3, s_v4, s_out};
+#pragma unroll
     for (int q = 0; q < 7; q++) {
       double t = block_sum<256>(r[q], sh);
       if (threadIdx.x == 0) partials[(int64_t)blockIdx.x * STAT_N + q] = t;
@@ -246,6 +340,7 @@ __global__ void k_stats_final(const double* __restrict__ partials, int nblocks,
       }
       stats[STAT_SCALE_A + f] = ldexp(1.0, e);
       stats[STAT_INVSCALE_A + f] = ldexp(1.0, -e);
+      stats[STAT_GMAX_A + f] = m;
     }
   }
 }
@@ -318,10 +413,45 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_apply(const uint32_t* __restric
 // ---------------------------------------------------------------------------
 // Tile deposit.
 // ---------------------------------------------------------------------------
-#define DEP_T 256
-#define MAX_BK 128
+#define DEP_T 512
 
-template <typename T, int ORDER>
+// Accumulator policy.  int (FP32 form): each tap is rounded to an int32
+// multiple of a per-tile quantum q chosen so that (copies in the tile) x
+// max|field| x (max tap weight) / q <= 2^30 -- no cell can overflow, and
+// integer sums are exact.  double (FP64 form): each tap is rounded to a
+// multiple of 2^-e, e from the field's max |value| so that one tap is below
+// 2^40 quanta; sums below 2^53 quanta are exact in ds_add_f64.
+template <typename ACC>
+struct Quant;
+template <>
+struct Quant<int> {
+  double scale, inv;
+  __device__ void init(uint32_t count, double gmax, double wmax, double /*gscale*/) {
+    const double bound = (double)(count ? count : 1) * (gmax > 0 ? gmax : 1.0) * wmax;
+    int ex;
+    frexp(bound, &ex);                 // bound < 2^ex
+    scale = ldexp(1.0, 30 - ex);       // bound * scale < 2^30
+    inv = ldexp(1.0, ex - 30);
+  }
+  template <typename T>
+  __device__ __forceinline__ int q(T vs, T w) const { return __float2int_rn((float)(vs * w)); }
+};
+template <>
+struct Quant<double> {
+  double scale, inv;
+  __device__ void init(uint32_t, double, double, double gscale) {
+    scale = gscale;
+    inv = 1.0 / gscale;   // power of two: exact
+  }
+  template <typename T>
+  __device__ __forceinline__ double q(T vs, T w) const { return rint((double)vs * (double)w); }
+};
+
+// One workgroup per TX x TY x TZ tile.  The tile's particle copies (own
+// first, then ghosts) are one contiguous range of the sorted array.  Phase 0
+// deposits the main grids (A, B), phase 1 the interlaced grids: LDS holds
+// one phase, i.e. nf x ncell accumulators.
+template <typename T, typename ACC, int ORDER>
 __global__ __launch_bounds__(DEP_T) void k_deposit(const T* __restrict__ sorted,
                                                    const T* __restrict__ sorted_w,
                                                    const uint32_t* __restrict__ off,
@@ -329,209 +459,167 @@ __global__ __launch_bounds__(DEP_T) void k_deposit(const T* __restrict__ sorted,
                                                    T* __restrict__ gAi, T* __restrict__ gB,
                                                    T* __restrict__ gBi,
                                                    const double* __restrict__ stats) {
-  extern __shared__ double acc[];  // [ngr][ncell]
-  __shared__ uint32_t s_bk_start[MAX_BK];
-  __shared__ uint32_t s_pref[MAX_BK + 1];
-  __shared__ uint32_t s_nbk;
-
+  extern __shared__ unsigned char smem[];
+  ACC* acc = reinterpret_cast<ACC*>(smem);  // [nf][ncell]
   const int TX = g.tdim[0], TY = g.tdim[1], TZ = g.tdim[2];
   const int ncell = TX * TY * TZ;
   const bool hasB = g.fb >= 0;
-  const int ngr = (hasB ? 2 : 1) * (g.interlace ? 2 : 1);
-  // slot layout in LDS: 0 A, 1 A_int (if interlace), then B, B_int
-  const int slotAi = 1, slotB = g.interlace ? 2 : 1, slotBi = 3;
+  const int nf = hasB ? 2 : 1;
 
-  // tile of this workgroup
-  int64_t tileid = blockIdx.x;
+  const int64_t tileid = blockIdx.x;
   const int ttx = (int)(tileid % g.ntile[0]);
   const int tty = (int)((tileid / g.ntile[0]) % g.ntile[1]);
   const int ttz = (int)(tileid / ((int64_t)g.ntile[0] * g.ntile[1]));
-  const int t0[3] = {ttx * TX, tty * TY, ttz * TZ};
+  const int t0x = ttx * TX, t0y = tty * TY, t0z = ttz * TZ;
+  const uint32_t s0 = off[tileid], s1 = off[tileid + 1];
 
-  for (int i = threadIdx.x; i < ngr * ncell; i += DEP_T) acc[i] = 0.0;
-  if (threadIdx.x == 0) s_nbk = 0;
-  __syncthreads();
-
-  // Enumerate the (neighbour tile, region) buckets whose particles can touch
-  // this tile: offset -1 needs the neighbour's high region, 0 all regions,
-  // +1 the low region.
-  const int tt[3] = {ttx, tty, ttz};
-  for (int cand = threadIdx.x; cand < 729; cand += DEP_T) {
-    int nc = cand / 27, r = cand % 27;
-    int o[3] = {nc % 3 - 1, (nc / 3) % 3 - 1, nc / 9 - 1};
-    int rr[3] = {r % 3, (r / 3) % 3, r / 9};
-    bool ok = true;
-    int nt[3];
-    for (int a = 0; a < 3 && ok; a++) {
-      if (g.ntile[a] == 1) {
-        ok = (o[a] == 0) && (rr[a] == 1);
-        nt[a] = 0;
-        continue;
-      }
-      if (o[a] == -1 && rr[a] != 2) ok = false;
-      if (o[a] == 1 && rr[a] != 0) ok = false;
-      int q = tt[a] + o[a];
-      if (q < 0 || q >= g.ntile[a]) {
-        if (g.periodic) q = (q + g.ntile[a]) % g.ntile[a];
-        else ok = false;
-      }
-      nt[a] = q;
-    }
-    if (!ok) continue;
-    int64_t b = (((int64_t)nt[2] * g.ntile[1] + nt[1]) * g.ntile[0] + nt[0]) * 27 + r;
-    uint32_t s0 = off[b], s1 = off[b + 1];
-    if (s1 > s0) {
-      uint32_t slot = atomicAdd(&s_nbk, 1u);
-      if (slot < MAX_BK) {
-        s_bk_start[slot] = s0;
-        s_pref[slot + 1] = s1 - s0;
-      }
-    }
-  }
-  __syncthreads();
-  const int nbk = (int)min(s_nbk, (uint32_t)MAX_BK);
-  if (threadIdx.x == 0) {
-    s_pref[0] = 0;
-    for (int q = 0; q < nbk; q++) s_pref[q + 1] += s_pref[q];
-  }
-  __syncthreads();
-  const uint32_t total = s_pref[nbk];
-
-  const double scA = stats[STAT_SCALE_A], scB = stats[STAT_SCALE_B];
+  const double wmax = ORDER == 3 ? 0.421875 : 1.0;
+  Quant<ACC> qa, qb;
+  qa.init(s1 - s0, stats[STAT_GMAX_A], wmax, stats[STAT_SCALE_A]);
+  qb.init(s1 - s0, stats[STAT_GMAX_B], wmax, stats[STAT_SCALE_B]);
+  const T scA = (T)qa.scale, scB = (T)qb.scale;
   const T d0 = (T)g.d[0], d1 = (T)g.d[1], d2 = (T)g.d[2];
+  const double alpha = stats[STAT_ALPHA];
 
-  for (uint32_t i = threadIdx.x; i < total; i += DEP_T) {
-    // locate the bucket holding flat index i
-    int lo = 0, hi = nbk - 1;
-    while (lo < hi) {
-      int mid = (lo + hi + 1) >> 1;
-      if (s_pref[mid] <= i) lo = mid;
-      else hi = mid - 1;
-    }
-    int64_t src = (int64_t)s_bk_start[lo] + (i - s_pref[lo]);
-    T x, y, z, v;
-    load4<T>(sorted, src, x, y, z, v);
-    T w = g.has_w ? sorted_w[src] : T(1);
-    const double va = (double)(w * vpow<T>(v, g.fa)) * scA;
-    const double vb = hasB ? (double)(w * vpow<T>(v, g.fb)) * scB : 0.0;
-    for (int gi = 0; gi < (g.interlace ? 2 : 1); gi++) {
-      T ux = gi ? x + d0 / T(2) : x;
-      T uy = gi ? y + d1 / T(2) : y;
-      T uz = gi ? z + d2 / T(2) : z;
-      AxisOut<T> a[3];
-      a[0] = axis_taps<T, ORDER>(ux, (T)g.lo[0], (T)g.hi[0], d0, (T)g.L[0], g.n[0], true, g.los, g.edge);
-      a[1] = axis_taps<T, ORDER>(uy, (T)g.lo[1], (T)g.hi[1], d1, (T)g.L[1], g.n[1], false, g.los, g.edge);
-      a[2] = axis_taps<T, ORDER>(uz, (T)g.lo[2], (T)g.hi[2], d2, (T)g.L[2], g.n[2], false, g.los, g.edge);
-      if (!(a[0].ok && a[1].ok && a[2].ok)) continue;
-      // local tile coordinates of each tap (or -1)
-      int loc[3][3];
-      for (int ax = 0; ax < 3; ax++) {
-        const int n = g.n[ax], td = g.tdim[ax];
-#pragma unroll
-        for (int t = 0; t < ORDER; t++) {
-          int gidx = a[ax].first + t;
-          if (g.periodic) {
-            if (gidx < 0) gidx += n;
-            else if (gidx >= n) gidx -= n;
-          }
-          int l = gidx - t0[ax];
-          loc[ax][t] = (a[ax].w[t] != T(0) && l >= 0 && l < td) ? l : -1;
-        }
+  for (int ph = 0; ph < (g.interlace ? 2 : 1); ph++) {
+    for (int c = threadIdx.x; c < nf * ncell; c += DEP_T) acc[c] = ACC(0);
+    __syncthreads();
+    for (uint32_t i = s0 + threadIdx.x; i < s1; i += DEP_T) {
+      T x, y, z, v;
+      load4<T>(sorted, i, x, y, z, v);
+      const T w = g.has_w ? sorted_w[i] : T(1);
+      const T va = w * vpow<T>(v, g.fa) * scA;
+      const T vb = hasB ? w * vpow<T>(v, g.fb) * scB : T(0);
+      if (ph) {
+        x += d0 / T(2);
+        y += d1 / T(2);
+        z += d2 / T(2);
       }
-      double* accA = acc + (gi ? slotAi : 0) * ncell;
-      double* accB = acc + (gi ? slotBi : slotB) * ncell;
+      const AxisOut<T> ax = axis_taps<T, ORDER>(x, (T)g.lo[0], (T)g.hi[0], d0, (T)g.L[0], g.n[0], true, g.los, g.edge, (T)g.invd[0]);
+      const AxisOut<T> ay = axis_taps<T, ORDER>(y, (T)g.lo[1], (T)g.hi[1], d1, (T)g.L[1], g.n[1], false, g.los, g.edge, (T)g.invd[1]);
+      const AxisOut<T> az = axis_taps<T, ORDER>(z, (T)g.lo[2], (T)g.hi[2], d2, (T)g.L[2], g.n[2], false, g.los, g.edge, (T)g.invd[2]);
+      if (!(ax.ok && ay.ok && az.ok)) continue;
+      // tile-local tap coordinates; weight 0 marks taps outside the tile
+      int lx[ORDER], ly[ORDER], lz[ORDER];
+      T wx[ORDER], wy[ORDER], wz[ORDER];
+#pragma unroll
+      for (int t = 0; t < ORDER; t++) {
+        int gx = ax.first + t, gy = ay.first + t, gz = az.first + t;
+        if (g.periodic) {
+          gx += gx < 0 ? g.n[0] : (gx >= g.n[0] ? -g.n[0] : 0);
+          gy += gy < 0 ? g.n[1] : (gy >= g.n[1] ? -g.n[1] : 0);
+          gz += gz < 0 ? g.n[2] : (gz >= g.n[2] ? -g.n[2] : 0);
+        }
+        lx[t] = gx - t0x;
+        ly[t] = gy - t0y;
+        lz[t] = gz - t0z;
+        wx[t] = ((unsigned)lx[t] < (unsigned)TX) ? ax.w[t] : T(0);
+        wy[t] = ((unsigned)ly[t] < (unsigned)TY) ? ay.w[t] : T(0);
+        wz[t] = ((unsigned)lz[t] < (unsigned)TZ) ? az.w[t] : T(0);
+      }
 #pragma unroll
       for (int tx = 0; tx < ORDER; tx++) {
-        if (loc[0][tx] < 0) continue;
 #pragma unroll
         for (int ty = 0; ty < ORDER; ty++) {
-          if (loc[1][ty] < 0) continue;
-          const T wxy = a[0].w[tx] * a[1].w[ty];
-          const int rowc = (loc[0][tx] * TY + loc[1][ty]) * TZ;
+          const T wxy = wx[tx] * wy[ty];
+          if (wxy == T(0)) continue;
+          const int row = (lx[tx] * TY + ly[ty]) * TZ;
+          const T pa = va * wxy, pb = vb * wxy;
 #pragma unroll
           for (int tz = 0; tz < ORDER; tz++) {
-            if (loc[2][tz] < 0) continue;
-            const double wt = (double)(wxy * a[2].w[tz]);
-            const int cell = rowc + loc[2][tz];
-            atomicAdd(&accA[cell], rint(va * wt));
-            if (hasB) atomicAdd(&accB[cell], rint(vb * wt));
+            if (wz[tz] == T(0)) continue;
+            const int cell = row + lz[tz];
+            atomicAdd(&acc[cell], qa.q(pa, wz[tz]));
+            if (hasB) atomicAdd(&acc[ncell + cell], qb.q(pb, wz[tz]));
           }
         }
       }
     }
-  }
-  __syncthreads();
-
-  // write the tile once: value/scale - alpha for density fields
-  const double alpha = stats[STAT_ALPHA];
-  const double iA = stats[STAT_INVSCALE_A], iB = stats[STAT_INVSCALE_B];
-  T* outs[4] = {gA, g.interlace ? gAi : gB, gB, gBi};
-  // map LDS slot -> (grid pointer, inv scale, mean)
-  for (int s = 0; s < ngr; s++) {
-    T* dst;
-    double isc, mean;
-    if (s == 0) { dst = gA; isc = iA; mean = g.mean_sub_a ? alpha : 0.0; }
-    else if (g.interlace && s == slotAi) { dst = gAi; isc = iA; mean = g.mean_sub_a ? alpha : 0.0; }
-    else if (s == slotB) { dst = gB; isc = iB; mean = 0.0; }
-    else { dst = gBi; isc = iB; mean = 0.0; }
-    const double* src = acc + s * ncell;
-    for (int c = threadIdx.x; c < ncell; c += DEP_T) {
-      int lz = c % TZ, ly = (c / TZ) % TY, lx = c / (TZ * TY);
-      int64_t gi = ((int64_t)(t0[0] + lx) * g.n[1] + (t0[1] + ly)) * g.rowlen + (t0[2] + lz);
-      dst[gi] = (T)(src[c] * isc - mean);
+    __syncthreads();
+    // write this phase's grids once: value * q - alpha for density fields
+    T* dA = ph ? gAi : gA;
+    T* dB = ph ? gBi : gB;
+    const double meanA = g.mean_sub_a ? alpha : 0.0;
+    for (int c2 = threadIdx.x; c2 < ncell / 2; c2 += DEP_T) {
+      const int c = 2 * c2;
+      const int lz = c % TZ, ly = (c / TZ) % TY, lxx = c / (TZ * TY);
+      const int64_t gi = ((int64_t)(t0x + lxx) * g.n[1] + (t0y + ly)) * g.rowlen + (t0z + lz);
+      T a0 = (T)((double)acc[c] * qa.inv - meanA), a1 = (T)((double)acc[c + 1] * qa.inv - meanA);
+      store2<T>(dA + gi, a0, a1);
+      if (hasB) {
+        T b0 = (T)((double)acc[ncell + c] * qb.inv), b1 = (T)((double)acc[ncell + c + 1] * qb.inv);
+        store2<T>(dB + gi, b0, b1);
+      }
     }
+    __syncthreads();
   }
-  (void)outs;
 }
 
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
 template <typename T, int ORDER>
-static int run_deposit_t(DepositArgs& a) {
+static int run_count_t(DepositArgs& a) {
   const Geom& g = *a.g;
   const int nb = a.count_blocks;
   hipStream_t st = a.stream;
   TIMED(a.timer, "bucket_count",
         (k_bucket<T, ORDER, false><<<nb, 256, 0, st>>>((const T*)a.parts, (const T*)a.wts, a.n, g,
-                                                        a.counts, nullptr, nullptr, nullptr,
+                                                        a.counts, nullptr, nullptr, nullptr, nullptr, 0,
                                                         a.partials, a.gmax, a.errs)));
   TIMED(a.timer, "stats_final",
         (k_stats_final<<<1, 256, 0, st>>>(a.partials, nb, a.gmax,
                                          (double)g.n[0] * (double)g.n[1] * (double)g.n[2],
                                          a.stats)));
-  // scan counts -> offsets (and cursor copy)
+  // scan counts -> offsets; zero the own/ghost cursors
   int64_t nbk = g.nbuckets;
   int64_t nblk = (nbk + SCAN_CHUNK - 1) / SCAN_CHUNK;
   if (nblk > SCAN_CHUNK) return CAPOW_ERR_UNSUPPORTED;
   TIMED(a.timer, "scan", (k_scan_reduce<<<(int)nblk, SCAN_T, 0, st>>>(a.counts, nbk, a.bsum)));
   TIMED(a.timer, "scan", (k_scan_apply<<<1, SCAN_T, 0, st>>>(a.bsum, nblk, nullptr, a.boff, nullptr)));
   TIMED(a.timer, "scan", (k_scan_apply<<<(int)nblk, SCAN_T, 0, st>>>(a.counts, nbk, a.boff, a.offsets,
-                                                                 a.cursor)));
+                                                                 nullptr)));
+  if (hipMemsetAsync(a.cursor, 0, (size_t)nbk * 2 * sizeof(uint32_t), st) != hipSuccess)
+    return CAPOW_ERR_HIP;
+  return hipGetLastError() == hipSuccess ? CAPOW_OK : CAPOW_ERR_HIP;
+}
+
+template <typename T, int ORDER>
+static int run_scatter_t(DepositArgs& a) {
+  const Geom& g = *a.g;
+  const int nb = a.count_blocks;
+  hipStream_t st = a.stream;
   TIMED(a.timer, "bucket_scatter",
         (k_bucket<T, ORDER, true><<<nb, 256, 0, st>>>((const T*)a.parts, (const T*)a.wts, a.n, g,
-                                                       nullptr, a.cursor, (T*)a.sorted,
-                                                       (T*)a.sorted_w, nullptr, nullptr, nullptr)));
-  const int ngr = (g.fb >= 0 ? 2 : 1) * (g.interlace ? 2 : 1);
-  const size_t lds = (size_t)ngr * g.tdim[0] * g.tdim[1] * g.tdim[2] * sizeof(double);
+                                                       nullptr, a.cursor, a.offsets, (T*)a.sorted,
+                                                       (T*)a.sorted_w, a.sorted_cap, nullptr,
+                                                       nullptr, nullptr)));
+  typedef typename std::conditional<sizeof(T) == 4, int, double>::type ACC;
+  const int nf = g.fb >= 0 ? 2 : 1;
+  const size_t lds = (size_t)nf * g.tdim[0] * g.tdim[1] * g.tdim[2] * sizeof(ACC);
   const int64_t ntiles = (int64_t)g.ntile[0] * g.ntile[1] * g.ntile[2];
+  if (lds > 64 * 1024)
+    hipFuncSetAttribute((const void*)k_deposit<T, ACC, ORDER>,
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   TIMED(a.timer, "deposit",
-        (k_deposit<T, ORDER><<<(unsigned)ntiles, DEP_T, lds, st>>>(
+        (k_deposit<T, ACC, ORDER><<<(unsigned)ntiles, DEP_T, lds, st>>>(
             (const T*)a.sorted, (const T*)a.sorted_w, a.offsets, g, (T*)a.grid[0],
             (T*)a.grid[1], (T*)a.grid[2], (T*)a.grid[3], a.stats)));
   return hipGetLastError() == hipSuccess ? CAPOW_OK : CAPOW_ERR_HIP;
 }
 
-int run_deposit(DepositArgs& a, bool fp64) {
-  switch (a.g->order * 2 + (fp64 ? 1 : 0)) {
-    case 2: return run_deposit_t<float, 1>(a);
-    case 3: return run_deposit_t<double, 1>(a);
-    case 4: return run_deposit_t<float, 2>(a);
-    case 5: return run_deposit_t<double, 2>(a);
-    case 6: return run_deposit_t<float, 3>(a);
-    case 7: return run_deposit_t<double, 3>(a);
-  }
+#define DISPATCH(fn)                                \
+  switch (a.g->order * 2 + (fp64 ? 1 : 0)) {        \
+    case 2: return fn<float, 1>(a);                 \
+    case 3: return fn<double, 1>(a);                \
+    case 4: return fn<float, 2>(a);                 \
+    case 5: return fn<double, 2>(a);                \
+    case 6: return fn<float, 3>(a);                 \
+    case 7: return fn<double, 3>(a);                \
+  }                                                 \
   return CAPOW_ERR_ARG;
-}
+
+int run_count(DepositArgs& a, bool fp64) { DISPATCH(run_count_t) }
+int run_scatter_deposit(DepositArgs& a, bool fp64) { DISPATCH(run_scatter_t) }
 
 }  // namespace capow

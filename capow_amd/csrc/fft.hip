@@ -17,6 +17,7 @@
 // working precision.
 #include "common.h"
 #include "kernels.h"
+#include "twiddle_table.h"
 
 namespace capow {
 
@@ -238,6 +239,236 @@ __global__ __launch_bounds__(1024) void k_fft_cols(T* __restrict__ g0, T* __rest
   }
 }
 
+// ---------------------------------------------------------------------------
+// Register-resident two-level FFT (N = N1 * N2).
+//
+// x[n1 + N1 n2] -> X[k2 + N2 k1]:
+//   1. for each n1: Y[n1][k2] = DFT_N2 over n2        (registers)
+//   2. Y[n1][k2] *= exp(-2 pi i n1 k2 / N)           (twiddle table)
+//   3. transpose through LDS (one write, one read; rows padded by one
+//      element so the strided side is free of bank conflicts)
+//   4. for each k2: X[k2 + N2 k1] = DFT_N1 over n1    (registers)
+// A thread holds E = lcm(N1, N2) values: E/N2 first-level and E/N1
+// second-level transforms.  In-register DFTs are fully unrolled mixed-radix
+// (4, 2, 3) decimation in time with constant twiddles from twiddle_table.h.
+// ---------------------------------------------------------------------------
+constexpr int cgcd(int a, int b) { return b == 0 ? a : cgcd(b, a % b); }
+constexpr int clcm(int a, int b) { return a / cgcd(a, b) * b; }
+
+template <typename T>
+__device__ __forceinline__ cx<T> twc(int R, int m) {
+  // exp(-2 pi i m / R) for compile-time R | 192 (m constant after unrolling)
+  m %= R;
+  if (m == 0) return mk<T>(T(1), T(0));
+  const int idx = m * (TWC_LEN / R);
+  return mk<T>((T)TWC_RE[idx], (T)TWC_IM[idx]);
+}
+
+template <typename T>
+__device__ __forceinline__ cx<T> mul_twc(cx<T> a, int R, int m) {
+  m %= R;
+  if (m == 0) return a;
+  if (4 * m == R) return mul_mi<T>(a);                 // -i
+  if (2 * m == R) return mk<T>(-a.x, -a.y);            // -1
+  if (4 * m == 3 * R) return mk<T>(-a.y, a.x);         // +i
+  return mul<T>(a, twc<T>(R, m));
+}
+
+template <typename T, int R>
+struct RegDFT {
+  static __device__ __forceinline__ void run(const cx<T>* in, cx<T>* out) {
+    if constexpr (R == 1) {
+      out[0] = in[0];
+    } else if constexpr (R == 2 || R == 3 || R == 4 || R == 8) {
+      cx<T> v[R];
+#pragma unroll
+      for (int i = 0; i < R; i++) v[i] = in[i];
+      butterfly<T, R>(v);
+#pragma unroll
+      for (int i = 0; i < R; i++) out[i] = v[i];
+    } else {
+      constexpr int P = (R % 8 == 0 && R > 16) ? 8 : ((R % 4 == 0) ? 4 : ((R % 2 == 0) ? 2 : 3));
+      constexpr int Q = R / P;
+      cx<T> so[P][Q];
+#pragma unroll
+      for (int n1 = 0; n1 < P; n1++) {
+        cx<T> si[Q];
+#pragma unroll
+        for (int n2 = 0; n2 < Q; n2++) si[n2] = in[n1 + P * n2];
+        RegDFT<T, Q>::run(si, so[n1]);
+      }
+#pragma unroll
+      for (int k2 = 0; k2 < Q; k2++) {
+        cx<T> t[P];
+#pragma unroll
+        for (int n1 = 0; n1 < P; n1++) t[n1] = mul_twc<T>(so[n1][k2], R, n1 * k2);
+        butterfly<T, P>(t);
+#pragma unroll
+        for (int k1 = 0; k1 < P; k1++) out[k2 + Q * k1] = t[k1];
+      }
+    }
+  }
+};
+
+// LDS index of Y[n1][k2] for sequence c (B sequences interleaved).  The k2
+// position is rotated by n1 so that the write side (lanes differing in n1)
+// spreads over the banks; the read side (lanes differing in k2) stays
+// contiguous.
+template <int N1, int N2, int B>
+__device__ __forceinline__ int tidx(int n1, int k2, int c) {
+  return n1 * (N2 * B) + ((k2 + n1) % N2) * B + c;
+}
+
+// Columns: B consecutive complex columns, N = N1*N2 along a strided axis.
+// Threads: (N / E) per column * B; lane -> (c = lane % B, t = lane / B).
+template <typename T, int N1, int N2, int B>
+__global__ __launch_bounds__((N1 * N2 / clcm(N1, N2)) * B) void k_fft_cols2(
+    T* __restrict__ g0, T* __restrict__ g1, T* __restrict__ g2, T* __restrict__ g3,
+    int64_t astride, int64_t pstride, int64_t ncols, int64_t groups_per_plane,
+    const cx<T>* __restrict__ tw, int64_t nblocks_x) {
+  constexpr int N = N1 * N2;
+  constexpr int E = clcm(N1, N2);
+  constexpr int TS = N / E;            // threads per sequence
+  constexpr int G1 = E / N2, G2 = E / N1;
+  extern __shared__ unsigned char smem[];
+  cx<T>* lds = reinterpret_cast<cx<T>*>(smem);
+  T* gp = blockIdx.y == 0 ? g0 : blockIdx.y == 1 ? g1 : blockIdx.y == 2 ? g2 : g3;
+  // XCD-aware block order: consecutive logical blocks on the same XCD
+  int64_t bid = blockIdx.x;
+  if ((nblocks_x & 7) == 0) bid = (bid & 7) * (nblocks_x >> 3) + (bid >> 3);
+  const int64_t plane = bid / groups_per_plane;
+  const int64_t c0 = (bid - plane * groups_per_plane) * B;
+  const int c = threadIdx.x % B, t = threadIdx.x / B;
+  const bool live = c0 + c < ncols;
+  cx<T>* col = reinterpret_cast<cx<T>*>(gp) + plane * pstride + c0 + c;
+  cx<T> v[E];
+  // step 1: load x[n1 + N1 n2], n1 = t + TS g
+#pragma unroll
+  for (int g = 0; g < G1; g++)
+#pragma unroll
+    for (int n2 = 0; n2 < N2; n2++) {
+      const int n = (t + TS * g) + N1 * n2;
+      v[g * N2 + n2] = live ? col[(int64_t)n * astride] : mk<T>(T(0), T(0));
+    }
+#pragma unroll
+  for (int g = 0; g < G1; g++) {
+    cx<T> o[N2];
+    RegDFT<T, N2>::run(&v[g * N2], o);
+    const int n1 = t + TS * g;
+#pragma unroll
+    for (int k2 = 0; k2 < N2; k2++) {
+      cx<T> y = (k2 == 0 || n1 == 0) ? o[k2] : mul<T>(o[k2], tw[n1 * k2]);
+      lds[tidx<N1, N2, B>(n1, k2, c)] = y;
+    }
+  }
+  __syncthreads();
+  // step 3: k2 = t + TS h, all n1
+#pragma unroll
+  for (int h = 0; h < G2; h++) {
+    const int k2 = t + TS * h;
+    cx<T> a[N1], o[N1];
+#pragma unroll
+    for (int n1 = 0; n1 < N1; n1++) a[n1] = lds[tidx<N1, N2, B>(n1, k2, c)];
+    RegDFT<T, N1>::run(a, o);
+    if (live) {
+#pragma unroll
+      for (int k1 = 0; k1 < N1; k1++) col[(int64_t)(k2 + N2 * k1) * astride] = o[k1];
+    }
+  }
+}
+
+// Rows (z pass): B rows of NZ reals -> NZ/2+1 complex, M = NZ/2 = N1*N2.
+// lane -> (t = lane % TS, c = lane / TS) so a wave reads TS consecutive
+// complex values of each row.
+template <typename T, int N1, int N2, int B>
+__global__ __launch_bounds__((N1 * N2 / clcm(N1, N2)) * B) void k_fft_rows2(
+    T* __restrict__ g0, T* __restrict__ g1, T* __restrict__ g2, T* __restrict__ g3,
+    int64_t nrows, const cx<T>* __restrict__ twz,
+    int64_t nblocks_x) {
+  constexpr int M = N1 * N2;
+  constexpr int E = clcm(N1, N2);
+  constexpr int TS = M / E;
+  constexpr int G1 = E / N2, G2 = E / N1;
+  extern __shared__ unsigned char smem[];
+  cx<T>* lds = reinterpret_cast<cx<T>*>(smem);
+  T* gp = blockIdx.y == 0 ? g0 : blockIdx.y == 1 ? g1 : blockIdx.y == 2 ? g2 : g3;
+  int64_t bid = blockIdx.x;
+  if ((nblocks_x & 7) == 0) bid = (bid & 7) * (nblocks_x >> 3) + (bid >> 3);
+  const int t = threadIdx.x % TS, c = threadIdx.x / TS;
+  const int64_t row = bid * B + c;
+  const bool live = row < nrows;
+  const int64_t rowlen = 2 * (int64_t)(M + 1);
+  cx<T>* r = reinterpret_cast<cx<T>*>(gp + (live ? row : 0) * rowlen);
+  cx<T> v[E];
+#pragma unroll
+  for (int g = 0; g < G1; g++)
+#pragma unroll
+    for (int n2 = 0; n2 < N2; n2++) {
+      const int n = (t + TS * g) + N1 * n2;
+      v[g * N2 + n2] = live ? r[n] : mk<T>(T(0), T(0));
+    }
+  // LDS holds B rows of M values (padded rows for the transpose)
+  constexpr int RS = N1 * (N2 + 1);   // per-row stride of the transpose buffer
+#pragma unroll
+  for (int g = 0; g < G1; g++) {
+    cx<T> o[N2];
+    RegDFT<T, N2>::run(&v[g * N2], o);
+    const int n1 = t + TS * g;
+#pragma unroll
+    for (int k2 = 0; k2 < N2; k2++) {
+      cx<T> y = (k2 == 0 || n1 == 0) ? o[k2] : mul<T>(o[k2], twz[2 * n1 * k2]);
+      lds[c * RS + n1 * (N2 + 1) + k2] = y;
+    }
+  }
+  __syncthreads();
+  cx<T> z[E];
+#pragma unroll
+  for (int h = 0; h < G2; h++) {
+    const int k2 = t + TS * h;
+    cx<T> a[N1];
+#pragma unroll
+    for (int n1 = 0; n1 < N1; n1++) a[n1] = lds[c * RS + n1 * (N2 + 1) + k2];
+    RegDFT<T, N1>::run(a, &z[h * N1]);
+  }
+  __syncthreads();
+  // Z[k2 + N2 k1] back to LDS in natural order (row stride M + 1)
+  cx<T>* zr = lds + c * (M + 1);
+#pragma unroll
+  for (int h = 0; h < G2; h++) {
+    const int k2 = t + TS * h;
+#pragma unroll
+    for (int k1 = 0; k1 < N1; k1++) zr[k2 + N2 * k1] = z[h * N1 + k1];
+  }
+  __syncthreads();
+  // even/odd split: X[k] = E[k] + w^k O[k], k = 0..M
+  if (live) {
+    for (int k = t; k <= M; k += TS) {
+      cx<T> zk = zr[k == M ? 0 : k];
+      cx<T> zm = zr[k == 0 ? 0 : M - k];
+      cx<T> Ev = mk<T>(T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y));
+      cx<T> Ov = mk<T>(T(0.5) * (zk.y + zm.y), -T(0.5) * (zk.x - zm.x));
+      r[k] = add<T>(Ev, mul<T>(twz[k], Ov));
+    }
+  }
+}
+
+// Sizes served by the register kernels: N -> (N1, N2).
+static bool reg_split(int N, int* n1, int* n2) {
+  switch (N) {
+    case 16: *n1 = 4; *n2 = 4; return true;
+    case 32: *n1 = 4; *n2 = 8; return true;
+    case 64: *n1 = 8; *n2 = 8; return true;
+    case 96: *n1 = 8; *n2 = 12; return true;
+    case 128: *n1 = 8; *n2 = 16; return true;
+    case 192: *n1 = 8; *n2 = 24; return true;
+    case 256: *n1 = 16; *n2 = 16; return true;
+    case 384: *n1 = 16; *n2 = 24; return true;
+    case 512: *n1 = 16; *n2 = 32; return true;
+    case 1024: *n1 = 32; *n2 = 32; return true;
+  }
+  return false;
+}
+
 template <typename T, int B>
 static void launch_cols(FftArgs& a, const char* name, T** gp, int N, int64_t astride,
                         int64_t pstride, int64_t ncols, int64_t nplanes, const FftPlan& pl,
@@ -269,6 +500,83 @@ static void launch_cols_any(FftArgs& a, const char* name, T** gp, int N, int64_t
 #define TIMED_COLS(a, name, gp, N, astride, pstride, ncols, nplanes, pl, tw, lim) \
   launch_cols_any<T>(a, name, gp, N, astride, pstride, ncols, nplanes, pl, tw, lim)
 
+// Columns per workgroup for the register kernels: at least one wavefront
+// of threads, 64 contiguous bytes per axis index when possible, <= 64 KB LDS.
+template <typename T, int N1, int N2>
+constexpr int cols_B() {
+  constexpr int N = N1 * N2, E = clcm(N1, N2), TS = N / E;
+  int b = sizeof(T) == 4 ? 8 : 4;
+  while (TS * b < 64) b *= 2;
+  while (b > 1 && (size_t)N * b * sizeof(cx<T>) > 64 * 1024) b /= 2;
+  return b;
+}
+
+template <typename T, int N1, int N2>
+constexpr int rows_B() {
+  constexpr int M = N1 * N2, E = clcm(N1, N2), TS = M / E;
+  int b = 1;
+  while (TS * b < 256 && (size_t)(2 * b) * (M + N1) * sizeof(cx<T>) <= 40 * 1024) b *= 2;
+  return b;
+}
+
+template <typename T, int N1, int N2>
+static void launch_cols2(FftArgs& a, const char* name, T** gp, int64_t astride, int64_t pstride,
+                         int64_t ncols, int64_t nplanes, const void* tw) {
+  constexpr int N = N1 * N2, E = clcm(N1, N2), TS = N / E;
+  constexpr int B = cols_B<T, N1, N2>();
+  const size_t lds = (size_t)N * B * sizeof(cx<T>);
+  const int64_t gpp = (ncols + B - 1) / B;
+  const int64_t nbx = gpp * nplanes;
+  dim3 grid((unsigned)nbx, a.ngrids);
+  TIMED(a.timer, name,
+        (k_fft_cols2<T, N1, N2, B><<<grid, TS * B, lds, a.stream>>>(
+            gp[0], gp[1], gp[2], gp[3], astride, pstride, ncols, gpp, (const cx<T>*)tw, nbx)));
+}
+
+template <typename T, int N1, int N2>
+static void launch_rows2(FftArgs& a, T** gp, int64_t nrows, const void* twz) {
+  constexpr int M = N1 * N2, E = clcm(N1, N2), TS = M / E;
+  constexpr int B = rows_B<T, N1, N2>();
+  const size_t lds = (size_t)B * (M + N1) * sizeof(cx<T>);
+  const int64_t nbx = (nrows + B - 1) / B;
+  dim3 grid((unsigned)nbx, a.ngrids);
+  TIMED(a.timer, "fft_z",
+        (k_fft_rows2<T, N1, N2, B><<<grid, TS * B, lds, a.stream>>>(
+            gp[0], gp[1], gp[2], gp[3], nrows, (const cx<T>*)twz, nbx)));
+}
+
+#define CAPOW_FFT_SIZES(X) X(4, 4) X(4, 8) X(8, 8) X(8, 12) X(8, 16) X(8, 24) X(16, 16) \
+  X(16, 24) X(16, 32) X(32, 32)
+
+template <typename T>
+static bool try_cols2(FftArgs& a, const char* name, T** gp, int N, int64_t astride,
+                      int64_t pstride, int64_t ncols, int64_t nplanes, const void* tw) {
+  int n1, n2;
+  if (!reg_split(N, &n1, &n2)) return false;
+#define X(A, B_)                                                                     \
+  if (n1 == A && n2 == B_) {                                                          \
+    launch_cols2<T, A, B_>(a, name, gp, astride, pstride, ncols, nplanes, tw);        \
+    return true;                                                                      \
+  }
+  CAPOW_FFT_SIZES(X)
+#undef X
+  return false;
+}
+
+template <typename T>
+static bool try_rows2(FftArgs& a, T** gp, int M, int64_t nrows, const void* twz) {
+  int n1, n2;
+  if (!reg_split(M, &n1, &n2)) return false;
+#define X(A, B_)                                   \
+  if (n1 == A && n2 == B_) {                        \
+    launch_rows2<T, A, B_>(a, gp, nrows, twz);      \
+    return true;                                    \
+  }
+  CAPOW_FFT_SIZES(X)
+#undef X
+  return false;
+}
+
 template <typename T>
 static int run_fft_t(FftArgs& a) {
   const Geom& g = *a.g;
@@ -277,8 +585,9 @@ static int run_fft_t(FftArgs& a) {
   for (int q = 0; q < 4; q++) gp[q] = q < a.ngrids ? (T*)a.grid[q] : (T*)a.grid[0];
   const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
   const int nzh = nz / 2 + 1;
+  const int64_t nrows = (int64_t)nx * ny;
   // z pass
-  {
+  if (!(a.fast && try_rows2<T>(a, gp, nz / 2, nrows, a.tw[2]))) {
     const int M = nz / 2;
     int rows = max(1, (FFT_EPT * 1024) / max(M, 1));
     rows = min(rows, 16);
@@ -288,7 +597,6 @@ static int run_fft_t(FftArgs& a) {
     threads = ((threads + 63) / 64) * 64;
     threads = min(max(threads, 64), 1024);
     while (rows > 1 && (int64_t)threads * FFT_EPT < (int64_t)rows * M) rows--;
-    int64_t nrows = (int64_t)nx * ny;
     dim3 grid((unsigned)((nrows + rows - 1) / rows), a.ngrids);
     size_t lds = (size_t)rows * M * sizeof(cx<T>);
     TIMED(a.timer, "fft_z",
@@ -296,15 +604,16 @@ static int run_fft_t(FftArgs& a) {
                                                      a.plan[2], (const cx<T>*)a.tw[2])));
   }
   // y pass: planes = x index, columns = kz
-  {
+  if (!(a.fast && try_cols2<T>(a, "fft_y", gp, ny, (int64_t)nzh, (int64_t)ny * nzh,
+                               (int64_t)nzh, nx, a.tw[1]))) {
     int64_t lim = 8192;  // N*B complex per workgroup (FFT_EPT * 1024)
     TIMED_COLS(a, "fft_y", gp, ny, (int64_t)nzh, (int64_t)ny * nzh, (int64_t)nzh, nx, a.plan[1],
                a.tw[1], lim);
   }
   // x pass: one plane, columns = (y, kz) contiguous
-  {
+  const int64_t ncols = (int64_t)ny * nzh;
+  if (!(a.fast && try_cols2<T>(a, "fft_x", gp, nx, ncols, 0, ncols, 1, a.tw[0]))) {
     int64_t lim = 8192;
-    int64_t ncols = (int64_t)ny * nzh;
     TIMED_COLS(a, "fft_x", gp, nx, ncols, 0, ncols, 1, a.plan[0], a.tw[0], lim);
   }
   return hipGetLastError() == hipSuccess ? CAPOW_OK : CAPOW_ERR_HIP;

@@ -21,6 +21,7 @@ enum {
   STAT_ALPHA = 8,
   STAT_SCALE_A = 9, STAT_SCALE_B = 10,
   STAT_INVSCALE_A = 11, STAT_INVSCALE_B = 12,
+  STAT_GMAX_A = 13, STAT_GMAX_B = 14,
   STAT_TOTAL = 16
 };
 
@@ -85,20 +86,24 @@ struct DepositArgs {
   const void* wts;        // n or null
   int64_t n;
   int count_blocks;
-  uint32_t* counts;       // nbuckets (zeroed)
-  uint32_t* offsets;      // nbuckets + 1
-  uint32_t* cursor;       // nbuckets
+  uint32_t* counts;       // ntiles (zeroed)
+  uint32_t* offsets;      // ntiles + 1
+  uint32_t* cursor;       // 2*ntiles: own-copy and ghost-copy cursors
   uint32_t* bsum;         // scan block sums
   uint32_t* boff;         // scan block offsets (+1)
-  void* sorted;           // n x 4
-  void* sorted_w;         // n or null
+  void* sorted;           // sorted_cap x 4 (one row per particle copy)
+  void* sorted_w;         // sorted_cap or null
+  int64_t sorted_cap;     // rows the sorted buffers hold
   double* partials;       // count_blocks * STAT_N
   unsigned long long* gmax;  // 2 (zeroed)
   unsigned int* errs;     // 2 (zeroed): [0] LOS=0 out-of-box, [1] reach violations
   double* stats;          // STAT_TOTAL
   void* grid[4];          // A, A_int, B, B_int (null when unused)
 };
-int run_deposit(DepositArgs& a, bool fp64);
+// count + stats + scan; then (after the host has sized the sorted buffer
+// from offsets[ntiles]) scatter + tile deposit
+int run_count(DepositArgs& a, bool fp64);
+int run_scatter_deposit(DepositArgs& a, bool fp64);
 
 struct FftPlan {
   int npass;
@@ -113,6 +118,7 @@ struct FftArgs {
   int ngrids;
   FftPlan plan[3];        // x: NX, y: NY, z: NZ/2
   const void* tw[3];      // x: NX entries, y: NY, z: NZ (exp(-2 pi i j/N))
+  bool fast = true;       // register two-level kernels where the size allows
 };
 int run_fft(FftArgs& a, bool fp64);
 
@@ -125,7 +131,7 @@ struct BinArgsDev {
   int nspec;
   int kind[3];
   int copies;
-  double mink, binwidth, min_nyq;
+  double mink, binwidth, inv_binwidth, min_nyq;
   const double *fx, *fy, *fz;
   const double *corx, *cory, *corz;
   const double2 *phx, *phy, *phz;

@@ -43,7 +43,6 @@ def test_struct_layouts_match_header():
     (dict(output_2d=1, nmu=0), "NMU"), (dict(output_log=3), "OutputLog"), (dict(nk=0), "NK"),
     (dict(nx=0), "NX"), (dict(xmax=-1.0), "XMAX"), (dict(maxk=-1.0), "Maxk"),
     (dict(momentum=9), "Momentum"), (dict(nz=33), "NZ"), (dict(nx=14 * 11), "2, 3 and 5"),
-    (dict(nx=27), "tile"),
 ])
 def test_create_rejects_invalid_parameters(bad, msg):
     p = api.make_params(**bad)

@@ -2,6 +2,7 @@
 // device buffers, tables and stage launches.
 #include <math.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <string>
@@ -58,6 +59,7 @@ struct capow_ctx {
   int nspec = 1;
   int codes[3] = {0, 0, 0};
   capow_assign_stats last{};
+  int fft_col_mul = 1;   // tuning knob: CAPOW_FFT_COL_MUL=2 doubles the y/x column batch
 };
 
 static std::string g_create_err;
@@ -351,6 +353,7 @@ extern "C" int capow_create(const capow_params* p, capow_ctx** out) {
   if (rc) return fail(nullptr, rc, m);
   capow_ctx* c = new capow_ctx();
   c->p = *p;
+  if (const char* e = getenv("CAPOW_FFT_COL_MUL")) c->fft_col_mul = atoi(e) == 2 ? 2 : 1;
   rc = setup(c);
   if (rc) {
     g_create_err = c->err;
@@ -528,6 +531,7 @@ extern "C" int capow_fft(capow_ctx* c) {
     a.plan[q] = c->plan[q];
     a.tw[q] = c->tw[q];
   }
+  a.col_mul = c->fft_col_mul;
   int rc = run_fft(a, c->fp64);
   if (rc) return fail(c, rc, std::string("fft launch failed: ") + hipGetErrorString(hipGetLastError()));
   c->state = 3;

@@ -160,24 +160,24 @@ __device__ __forceinline__ PClass<T, ORDER> classify(T x, T y, T z, T v, const G
   // above when the key lies within reach_hi of its upper edge, plus the tile
   // below when it lies within -reach_lo of its lower edge.
   const int lw = -g.reach_lo, hw = g.reach_hi;
+#pragma unroll
   for (int a = 0; a < 3; a++) {
     const int td = g.tdim[a], nt = g.ntile[a];
     const int t0 = key[a] / td;
     const int local = key[a] - t0 * td;
-    c.tile[a][0] = t0;
-    c.nt[a] = 1;
+    int second = -1;
     if (nt > 1) {
       if (local >= td - hw) {
-        int q = t0 + 1;
-        if (q >= nt) q = g.periodic ? q - nt : -1;
-        if (q >= 0) c.tile[a][c.nt[a]++] = q;
-      }
-      if (local < lw) {
-        int q = t0 - 1;
-        if (q < 0) q = g.periodic ? q + nt : -1;
-        if (q >= 0) c.tile[a][c.nt[a]++] = q;
+        second = t0 + 1;
+        if (second >= nt) second = g.periodic ? second - nt : -1;
+      } else if (local < lw) {
+        second = t0 - 1;
+        if (second < 0) second = g.periodic ? second + nt : -1;
       }
     }
+    c.tile[a][0] = t0;
+    c.tile[a][1] = second;
+    c.nt[a] = second >= 0 ? 2 : 1;
   }
   c.bucket = 0;
   return c;
@@ -257,7 +257,12 @@ __global__ __launch_bounds__(256) void k_bucket(const T* __restrict__ parts,
       const bool has = take && qx < c.nt[0] && qy < c.nt[1] && qz < c.nt[2];
       if (!__any(has)) continue;
       int tile = -1;
-      if (has) tile = (c.tile[2][qz] * g.ntile[1] + c.tile[1][qy]) * g.ntile[0] + c.tile[0][qx];
+      if (has) {
+        const int tx = qx ? c.tile[0][1] : c.tile[0][0];
+        const int ty = qy ? c.tile[1][1] : c.tile[1][0];
+        const int tz = qz ? c.tile[2][1] : c.tile[2][0];
+        tile = (tz * g.ntile[1] + ty) * g.ntile[0] + tx;
+      }
       if (SCATTER) {
         // own copies fill each tile's range from the front, ghost copies
         // from the back, so deposit waves are mostly homogeneous
@@ -447,6 +452,14 @@ struct Quant<double> {
   __device__ __forceinline__ double q(T vs, T w) const { return rint((double)vs * (double)w); }
 };
 
+// z-index swizzle of LDS tile row `rowi`: an even XOR mask below TZ when TZ
+// is a power of two, so rows that differ only in (x, y) land in different
+// LDS banks (without it every row starts on one of two bank groups); z pairs
+// (2j, 2j+1) stay adjacent.
+__device__ __forceinline__ int zswz(int rowi, int TZ) {
+  return ((TZ & (TZ - 1)) == 0 && TZ >= 4) ? ((rowi << 1) & (TZ - 2)) : 0;
+}
+
 // One workgroup per TX x TY x TZ tile.  The tile's particle copies (own
 // first, then ghosts) are one contiguous range of the sorted array.  Phase 0
 // deposits the main grids (A, B), phase 1 the interlaced grids: LDS holds
@@ -523,12 +536,14 @@ __global__ __launch_bounds__(DEP_T) void k_deposit(const T* __restrict__ sorted,
         for (int ty = 0; ty < ORDER; ty++) {
           const T wxy = wx[tx] * wy[ty];
           if (wxy == T(0)) continue;
-          const int row = (lx[tx] * TY + ly[ty]) * TZ;
+          const int rowi = lx[tx] * TY + ly[ty];
+          const int row = rowi * TZ;
+          const int swz = zswz(rowi, TZ);
           const T pa = va * wxy, pb = vb * wxy;
 #pragma unroll
           for (int tz = 0; tz < ORDER; tz++) {
             if (wz[tz] == T(0)) continue;
-            const int cell = row + lz[tz];
+            const int cell = row + (lz[tz] ^ swz);
             atomicAdd(&acc[cell], qa.q(pa, wz[tz]));
             if (hasB) atomicAdd(&acc[ncell + cell], qb.q(pb, wz[tz]));
           }
@@ -542,12 +557,14 @@ __global__ __launch_bounds__(DEP_T) void k_deposit(const T* __restrict__ sorted,
     const double meanA = g.mean_sub_a ? alpha : 0.0;
     for (int c2 = threadIdx.x; c2 < ncell / 2; c2 += DEP_T) {
       const int c = 2 * c2;
-      const int lz = c % TZ, ly = (c / TZ) % TY, lxx = c / (TZ * TY);
+      const int lz = c % TZ, rowi = c / TZ;
+      const int ly = rowi % TY, lxx = rowi / TY;
       const int64_t gi = ((int64_t)(t0x + lxx) * g.n[1] + (t0y + ly)) * g.rowlen + (t0z + lz);
-      T a0 = (T)((double)acc[c] * qa.inv - meanA), a1 = (T)((double)acc[c + 1] * qa.inv - meanA);
+      const int cs = rowi * TZ + (lz ^ zswz(rowi, TZ));   // lz even, swizzle even: pair stays adjacent
+      T a0 = (T)((double)acc[cs] * qa.inv - meanA), a1 = (T)((double)acc[cs + 1] * qa.inv - meanA);
       store2<T>(dA + gi, a0, a1);
       if (hasB) {
-        T b0 = (T)((double)acc[ncell + c] * qb.inv), b1 = (T)((double)acc[ncell + c + 1] * qb.inv);
+        T b0 = (T)((double)acc[ncell + cs] * qb.inv), b1 = (T)((double)acc[ncell + cs + 1] * qb.inv);
         store2<T>(dB + gi, b0, b1);
       }
     }

@@ -519,12 +519,15 @@ constexpr int rows_B() {
   return b;
 }
 
-template <typename T, int N1, int N2>
+template <typename T, int N1, int N2, int MUL>
 static void launch_cols2(FftArgs& a, const char* name, T** gp, int64_t astride, int64_t pstride,
                          int64_t ncols, int64_t nplanes, const void* tw) {
   constexpr int N = N1 * N2, E = clcm(N1, N2), TS = N / E;
-  constexpr int B = cols_B<T, N1, N2>();
+  constexpr int B = cols_B<T, N1, N2>() * MUL;
   const size_t lds = (size_t)N * B * sizeof(cx<T>);
+  if (lds > 64 * 1024)
+    hipFuncSetAttribute((const void*)k_fft_cols2<T, N1, N2, B>,
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   const int64_t gpp = (ncols + B - 1) / B;
   const int64_t nbx = gpp * nplanes;
   dim3 grid((unsigned)nbx, a.ngrids);
@@ -555,7 +558,10 @@ static bool try_cols2(FftArgs& a, const char* name, T** gp, int N, int64_t astri
   if (!reg_split(N, &n1, &n2)) return false;
 #define X(A, B_)                                                                     \
   if (n1 == A && n2 == B_) {                                                          \
-    launch_cols2<T, A, B_>(a, name, gp, astride, pstride, ncols, nplanes, tw);        \
+    if (a.col_mul == 2 && (size_t)A * B_ * cols_B<T, A, B_>() * 2 * sizeof(cx<T>) <= 160 * 1024) \
+      launch_cols2<T, A, B_, 2>(a, name, gp, astride, pstride, ncols, nplanes, tw);   \
+    else                                                                              \
+      launch_cols2<T, A, B_, 1>(a, name, gp, astride, pstride, ncols, nplanes, tw);   \
     return true;                                                                      \
   }
   CAPOW_FFT_SIZES(X)

@@ -119,6 +119,7 @@ struct FftArgs {
   FftPlan plan[3];        // x: NX, y: NY, z: NZ/2
   const void* tw[3];      // x: NX entries, y: NY, z: NZ (exp(-2 pi i j/N))
   bool fast = true;       // register two-level kernels where the size allows
+  int col_mul = 1;        // column batch multiplier of the y/x passes (1 or 2)
 };
 int run_fft(FftArgs& a, bool fp64);
 

@@ -2,8 +2,8 @@
 // binning (K4) for gfx950.
 //
 // Replaces compute_periodic_power (reference src/main.c:302-468).  One pass
-// over the complex half-grid [NX][NY][NZ/2+1] of every deposited grid; per
-// mode:
+// over the complex half-grid [NX][NY][NZ/2+1] (rows padded to rowc complex)
+// of every deposited grid; per mode:
 //   f = per-axis frequency with the reference's float casts (tables built on
 //       the host exactly as src/main.c:329-352 computes them),
 //   fktot = 2 pi sqrt(fx^2+fy^2+fz^2), mu, kbin = (int)((float)(fktot-Mink)
@@ -53,8 +53,9 @@ __device__ __forceinline__ void bin_one(int i, uint32_t r, typename CT<T>::t va,
                                         int nl, int n2d) {
   const int NK = a.nk;
   const double twopi = 2.0 * 3.14159265358979323846;  // the reference's 2.0*M_PI
-  const uint32_t j = a.fd_nzh.div(r);
-  const int k = (int)(r - j * (uint32_t)a.nzh);
+  const uint32_t j = a.fd_rowc.div(r);
+  const int k = (int)(r - j * (uint32_t)a.rowc);
+  if (k >= a.nzh) return;   // row padding
   const double fx = a.fx[i], fy = a.fy[j], fz = a.fz[k];
   const double fktot = dmul(twopi, sqrt(dadd(dadd(dmul(fx, fx), dmul(fy, fy)), dmul(fz, fz))));
   int kbin;
@@ -164,7 +165,7 @@ __global__ __launch_bounds__(256) void k_bin(const T* __restrict__ gA, const T* 
   double* cnt = acc2 + a.nspec * n2d;      // [NK]
   double* cnt2 = cnt + NK;                 // [NK*NMU]
 
-  const uint32_t plane = (uint32_t)a.ny * (uint32_t)a.nzh;
+  const uint32_t plane = (uint32_t)a.ny * (uint32_t)a.rowc;   // padded rows
   const int64_t total = (int64_t)a.nx * plane;
   const C* A = reinterpret_cast<const C*>(gA);
   const C* Ai = reinterpret_cast<const C*>(gAi);

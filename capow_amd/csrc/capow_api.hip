@@ -41,7 +41,7 @@ struct capow_ctx {
   // buckets / sort
   uint32_t *counts = nullptr, *offsets = nullptr, *cursor = nullptr, *bsum = nullptr,
            *boff = nullptr;
-  int count_blocks = 2048;
+  int count_blocks = 1024;   // bucket-kernel blocks: 4 per CU
   double* partials = nullptr;
   unsigned long long* gmax = nullptr;
   unsigned int* errs = nullptr;
@@ -59,7 +59,7 @@ struct capow_ctx {
   int nspec = 1;
   int codes[3] = {0, 0, 0};
   capow_assign_stats last{};
-  int fft_col_mul = 1;   // tuning knob: CAPOW_FFT_COL_MUL=2 doubles the y/x column batch
+  int fft_col_mul = 2;   // column batch multiplier of the y/x FFT passes (CAPOW_FFT_COL_MUL=1 halves it)
 };
 
 static std::string g_create_err;
@@ -160,11 +160,19 @@ static int validate(const capow_params* p, std::string& m) {
   }
   const int64_t dims[3] = {p->nx, p->ny, p->nz};
   const int tmax[3] = {16, 16, p->precision == CAPOW_FP64 ? 16 : 32};
+  int64_t ntiles = 1;
   for (int a = 0; a < 3; a++) {
-    if (pick_tile((int)dims[a], tmax[a], a == 2) < 0) {
+    const int t = pick_tile((int)dims[a], tmax[a], a == 2);
+    if (t < 0) {
       m = "grid dimension has no tile divisor in [4, 32]";
       return CAPOW_ERR_UNSUPPORTED;
     }
+    ntiles *= dims[a] / t;
+  }
+  // the tile-offset scan covers 4096 x 4096 tiles
+  if (ntiles > (int64_t)4096 * 4096) {
+    m = "grid has more than 16,777,216 deposit tiles";
+    return CAPOW_ERR_UNSUPPORTED;
   }
   return CAPOW_OK;
 }
@@ -186,7 +194,12 @@ static int setup(capow_ctx* c) {
   memset(&g, 0, sizeof(g));
   g.n[0] = (int)p.nx; g.n[1] = (int)p.ny; g.n[2] = (int)p.nz;
   g.nzh = (int)(p.nz / 2 + 1);
-  g.rowlen = 2 * (int64_t)g.nzh;
+  {
+    // rows start on 128-byte lines: 16 complex (FP32) or 8 (FP64)
+    const int64_t align = p.precision == CAPOW_FP64 ? 8 : 16;
+    g.rowc = ((int64_t)g.nzh + align - 1) / align * align;
+    g.rowlen = 2 * g.rowc;
+  }
   const double lo[3] = {p.xmin, p.ymin, p.zmin}, hi[3] = {p.xmax, p.ymax, p.zmax};
   for (int a = 0; a < 3; a++) {
     g.lo[a] = lo[a];
@@ -214,6 +227,7 @@ static int setup(capow_ctx* c) {
     int t = pick_tile(g.n[a], tmax[a], a == 2);
     if (t < 0) return fail(c, CAPOW_ERR_UNSUPPORTED, "grid dimension has no tile divisor in [4, 32]");
     g.tdim[a] = t;
+    g.tdiv[a].init((uint32_t)t);
     g.ntile[a] = g.n[a] / t;
   }
   g.nbuckets = (int64_t)g.ntile[0] * g.ntile[1] * g.ntile[2];
@@ -221,11 +235,13 @@ static int setup(capow_ctx* c) {
   c->esz = c->fp64 ? 8 : 4;
 
   // grids
-  c->grid_elems = (int64_t)p.nx * p.ny * g.rowlen;
+  // host layout: the in-place r2c layout [NX][NY][2(NZ/2+1)]; device rows
+  // are padded to rowlen reals
+  c->grid_elems = (int64_t)p.nx * p.ny * 2 * g.nzh;
   const bool hasB = g.fb >= 0;
   for (int s = 0; s < 4; s++) {
     bool need = (s == 0) || (s == 1 && g.interlace) || (s == 2 && hasB) || (s == 3 && hasB && g.interlace);
-    if (need) HIPCHK(c, hipMalloc(&c->grid[s], (size_t)c->grid_elems * c->esz));
+    if (need) HIPCHK(c, hipMalloc(&c->grid[s], (size_t)p.nx * p.ny * g.rowlen * c->esz));
   }
   // sort buffers
   HIPCHK(c, hipMalloc((void**)&c->counts, (size_t)g.nbuckets * 4));
@@ -297,6 +313,8 @@ static int setup(capow_ctx* c) {
     memset(&b, 0, sizeof(b));
     b.nx = (int)NX; b.ny = (int)NY; b.nzh = nzh;
     b.fd_nzh.init((uint32_t)nzh);
+    b.rowc = (int)g.rowc;
+    b.fd_rowc.init((uint32_t)g.rowc);
     b.nk = p.nk; b.nmu = p.output_2d ? p.nmu : 1;
     b.out2d = p.output_2d; b.odd = p.odd_multipoles; b.logbins = p.output_log;
     b.los = p.los; b.interlace = p.do_interlacing; b.hasB = hasB;
@@ -353,7 +371,7 @@ extern "C" int capow_create(const capow_params* p, capow_ctx** out) {
   if (rc) return fail(nullptr, rc, m);
   capow_ctx* c = new capow_ctx();
   c->p = *p;
-  if (const char* e = getenv("CAPOW_FFT_COL_MUL")) c->fft_col_mul = atoi(e) == 2 ? 2 : 1;
+  if (const char* e = getenv("CAPOW_FFT_COL_MUL")) c->fft_col_mul = atoi(e) == 1 ? 1 : 2;
   rc = setup(c);
   if (rc) {
     g_create_err = c->err;
@@ -468,7 +486,7 @@ extern "C" int capow_assign(capow_ctx* c, capow_assign_stats* st) {
   a.parts = c->parts;
   a.wts = c->wts;
   a.n = c->n;
-  int64_t want = (c->n + 255) / 256;
+  int64_t want = (c->n + BK_CHUNK - 1) / BK_CHUNK;
   a.count_blocks = (int)(want < 1 ? 1 : (want < c->count_blocks ? want : c->count_blocks));
   a.counts = c->counts;
   a.offsets = c->offsets;
@@ -604,8 +622,9 @@ extern "C" int capow_get_grid(capow_ctx* c, int32_t slot, void* host, int64_t n)
   if (!c || !host || slot < 0 || slot > 3) return CAPOW_ERR_ARG;
   if (!c->grid[slot]) return fail(c, CAPOW_ERR_ARG, "grid slot not allocated");
   if (n < c->grid_elems) return fail(c, CAPOW_ERR_ARG, "host buffer too small");
-  HIPCHK(c, hipMemcpyAsync(host, c->grid[slot], (size_t)c->grid_elems * c->esz,
-                           hipMemcpyDeviceToHost, c->stream));
+  const size_t w = (size_t)2 * c->g.nzh * c->esz, pitch = (size_t)c->g.rowlen * c->esz;
+  HIPCHK(c, hipMemcpy2DAsync(host, w, c->grid[slot], pitch, w, (size_t)c->p.nx * c->p.ny,
+                             hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return CAPOW_OK;
 }
@@ -614,8 +633,9 @@ extern "C" int capow_set_grid(capow_ctx* c, int32_t slot, const void* host, int6
   if (!c || !host || slot < 0 || slot > 3) return CAPOW_ERR_ARG;
   if (!c->grid[slot]) return fail(c, CAPOW_ERR_ARG, "grid slot not allocated");
   if (n < c->grid_elems) return fail(c, CAPOW_ERR_ARG, "host buffer too small");
-  HIPCHK(c, hipMemcpyAsync(c->grid[slot], host, (size_t)c->grid_elems * c->esz,
-                           hipMemcpyHostToDevice, c->stream));
+  const size_t w = (size_t)2 * c->g.nzh * c->esz, pitch = (size_t)c->g.rowlen * c->esz;
+  HIPCHK(c, hipMemcpy2DAsync(c->grid[slot], pitch, host, w, w, (size_t)c->p.nx * c->p.ny,
+                             hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return CAPOW_OK;
 }

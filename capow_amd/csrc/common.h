@@ -8,13 +8,31 @@
 
 namespace capow {
 
+// Fast unsigned 32-bit division by a runtime constant (round-up method).
+struct FastDiv {
+  uint32_t d, m, s;
+  __host__ void init(uint32_t div) {
+    d = div;
+    s = 0;
+    while ((1ull << s) < div) s++;
+    m = (uint32_t)(((1ull << 32) * ((1ull << s) - div)) / div + 1);
+  }
+  __device__ __forceinline__ uint32_t div(uint32_t x) const {
+    uint32_t t = __umulhi(x, m);
+    return (t + ((x - t) >> (s ? 1 : 0))) >> (s ? s - 1 : 0);
+  }
+};
+
 // ---------------------------------------------------------------------------
 // Grid geometry and mass-assignment setup, passed by value to kernels.
 // ---------------------------------------------------------------------------
 struct Geom {
   int n[3];              // NX NY NZ
   int nzh;               // NZ/2+1 complex per row
-  int64_t rowlen;        // 2*(NZ/2+1) reals per padded row
+  int64_t rowc;          // complex elements per row in device memory: NZ/2+1
+                         // rounded up to a 128-byte multiple, so every row
+                         // starts on a cache line
+  int64_t rowlen;        // 2*rowc reals per padded row
   double lo[3], hi[3];   // box
   double d[3];           // cell size (MAX-MIN)/N  (src/main.c:1126-1128)
   double invd[3];        // 1/d (FP32 form)
@@ -31,6 +49,7 @@ struct Geom {
   double rsd_f1, rsd_den;// shift = v*f1/den   (src/read_data.c:67-76)
   // deposit tiling
   int tdim[3];           // tile extent per axis (cells)
+  FastDiv tdiv[3];       // division by tdim
   int ntile[3];          // tiles per axis
   int reach_lo, reach_hi;// taps of a particle lie in [key+reach_lo, key+reach_hi]
   int64_t nbuckets;      // ntiles (one bucket per deposit tile)
@@ -78,11 +97,20 @@ __device__ __forceinline__ T scaled(T r, T d, T invd) {
   else return r / d;
 }
 
+// Centre (TSC) or floor (CIC, NGP) cell of one coordinate and whether the
+// reference accepts it; the one place where the cell arithmetic lives, so
+// the sort keys and the deposit taps agree bit for bit.
+template <typename T>
+struct AxisCell {
+  T s;      // scaled coordinate (u - MIN)/d after the TSC periodic x shift
+  int c;    // unwrapped cell: TSC (int)(s + 0.5), CIC/NGP (int)s
+  bool ok;
+};
+
 template <typename T, int ORDER>
-__device__ __forceinline__ AxisOut<T> axis_taps(T u, T lo, T hi, T d, T L, int n,
-                                                bool is_x, int los, int edge, T invd) {
-  AxisOut<T> o;
-  o.w[0] = o.w[1] = o.w[2] = T(0);
+__device__ __forceinline__ AxisCell<T> axis_cell(T u, T lo, T hi, T d, T L, int n, bool is_x,
+                                                 int los, int edge, T invd) {
+  AxisCell<T> o;
   if (edge == CAPOW_EDGE_PERIODIC) {
     // wrap into [lo, hi)
     T r = u - lo;
@@ -90,32 +118,16 @@ __device__ __forceinline__ AxisOut<T> axis_taps(T u, T lo, T hi, T d, T L, int n
       r = r - L * floor(r / L);
       if (r >= L) r = T(0);
     }
-    T s = scaled<T>(r, d, invd);
+    o.s = scaled<T>(r, d, invd);
     o.ok = true;
-    if (ORDER == 3) {
-      int c = (int)floor(s + T(0.5));
-      T f = s - (T)c;
-      o.first = c - 1;
-      o.w[0] = T(0.5) * (T(0.5) - f) * (T(0.5) - f);
-      o.w[1] = T(0.75) - f * f;
-      o.w[2] = T(0.5) * (T(0.5) + f) * (T(0.5) + f);
-    } else if (ORDER == 2) {
-      int c = (int)floor(s);
-      T f = s - (T)c;
-      o.first = c;
-      o.w[0] = T(1) - f;
-      o.w[1] = f;
-    } else {
-      o.first = (int)floor(s);
-      o.w[0] = T(1);
-    }
+    o.c = (ORDER == 3) ? (int)floor(o.s + T(0.5)) : (int)floor(o.s);
     return o;
   }
   const bool periodic = (los == 0);
   if (ORDER == 3) {
     if (is_x) {
       if (periodic) {
-        if (u - lo < d) u += L;
+        u = (u - lo < d) ? u + L : u;
         o.ok = !((u < lo + d) || (u >= hi + d));
       } else {
         o.ok = !((u < lo) || (u >= hi + d));
@@ -123,92 +135,71 @@ __device__ __forceinline__ AxisOut<T> axis_taps(T u, T lo, T hi, T d, T L, int n
     } else {
       o.ok = !((u < lo) || (u >= hi));
     }
-    T s = scaled<T>(u - lo, d, invd);
-    int c = fl2i<T>(s + T(0.5));
-    T f = s - (T)c;
+    o.s = scaled<T>(u - lo, d, invd);
+    o.c = fl2i<T>(o.s + T(0.5));
+  } else {
+    o.ok = !((u < lo) || (u >= hi));
+    o.s = scaled<T>(u - lo, d, invd);
+    o.c = fl2i<T>(o.s);
+    if (ORDER == 1 && !is_x && o.c == n) o.c = n - 1;   // src/read_data.c:534-535
+  }
+  return o;
+}
+
+template <typename T, int ORDER>
+__device__ __forceinline__ AxisOut<T> axis_taps(T u, T lo, T hi, T d, T L, int n,
+                                                bool is_x, int los, int edge, T invd) {
+  const AxisCell<T> ac = axis_cell<T, ORDER>(u, lo, hi, d, L, n, is_x, los, edge, invd);
+  AxisOut<T> o;
+  o.ok = ac.ok;
+  o.w[0] = o.w[1] = o.w[2] = T(0);
+  const int c = ac.c;
+  const T f = ac.s - (T)c;
+  const bool periodic = (edge == CAPOW_EDGE_PERIODIC) || (los == 0);
+  if (ORDER == 3) {
     o.first = c - 1;
-    T wl = T(0.5) * (T(0.5) - f) * (T(0.5) - f);
-    T wc = T(0.75) - f * f;
-    T wh = T(0.5) * (T(0.5) + f) * (T(0.5) + f);
+    const T wl = T(0.5) * (T(0.5) - f) * (T(0.5) - f);
+    const T wc = T(0.75) - f * f;
+    const T wh = T(0.5) * (T(0.5) + f) * (T(0.5) + f);
+    if (edge == CAPOW_EDGE_PERIODIC) {
+      o.w[0] = wl;
+      o.w[1] = wc;
+      o.w[2] = wh;
+      return o;
+    }
     // The reference keeps the low tap only when its index is > 0 after the
     // LOS = 0 wrap (-1 -> N-1): periodic axes drop it exactly when c == 1
     // (x: c-1 = slice 0; y/z: iy-1 = 0), non-periodic axes whenever c <= 1.
-    o.w[0] = (periodic ? (c != 1) : (c - 1 > 0)) ? wl : T(0);
-    o.w[1] = wc;
-    o.w[2] = wh;
-    if (!periodic) {
-      // non-periodic: taps at index >= N land in dropped buffer slices (x)
-      // or fail the < N guards (y, z)
-      if (c - 1 >= n) o.w[0] = T(0);
-      if (c >= n) o.w[1] = T(0);
-      if (c + 1 >= n) o.w[2] = T(0);
-    }
-    return o;
+    // Non-periodic axes also drop taps at index >= N (x: dropped buffer
+    // slices; y, z: the < N guards).
+    o.w[0] = ((periodic ? (c != 1) : (c - 1 > 0)) && (periodic || c - 1 < n)) ? wl : T(0);
+    o.w[1] = (periodic || c < n) ? wc : T(0);
+    o.w[2] = (periodic || c + 1 < n) ? wh : T(0);
   } else if (ORDER == 2) {
-    o.ok = !((u < lo) || (u >= hi));
-    T s = scaled<T>(u - lo, d, invd);
-    int c = fl2i<T>(s);
-    T f = s - (T)c;
     o.first = c;
-    o.w[0] = T(1) - f;
-    o.w[1] = f;
-    if (!periodic) {
-      if (c >= n) o.w[0] = T(0);
-      if (c + 1 >= n) o.w[1] = T(0);
-    }
-    return o;
+    o.w[0] = (periodic || c < n) ? T(1) - f : T(0);
+    o.w[1] = (periodic || c + 1 < n) ? f : T(0);
   } else {
-    o.ok = !((u < lo) || (u >= hi));
-    T s = scaled<T>(u - lo, d, invd);
-    int c = fl2i<T>(s);
-    if (!is_x && c == n) c = n - 1;   // src/read_data.c:534-535
     o.first = c;
-    o.w[0] = T(1);
-    if (!periodic && c >= n) o.w[0] = T(0);
-    return o;
+    o.w[0] = (periodic || c < n) ? T(1) : T(0);
   }
+  return o;
 }
 
-// Bucket key cell along an axis from the main-grid coordinate (already
-// shifted by the TSC periodic wrap where applicable).
-template <typename T, int ORDER>
-__device__ __forceinline__ int axis_key(T u, T lo, T hi, T d, T L, int n, bool is_x,
-                                        int los, int edge, T invd) {
-  T s;
-  if (edge == CAPOW_EDGE_PERIODIC) {
-    T r = u - lo;
-    if (r < T(0) || r >= L) {
-      r = r - L * floor(r / L);
-      if (r >= L) r = T(0);
+// Bucket key cell along an axis: the cell of axis_cell, wrapped mod N on
+// periodic axes (accepted coordinates give c in [-1, N + 2]), clamped to
+// [0, N) otherwise.
+__device__ __forceinline__ int cell_key(int c, int n, bool periodic) {
+  if (periodic) {
+    if (c >= n && c < 2 * n) c -= n;
+    else if (c < 0 && c >= -n) c += n;
+    else if ((unsigned)c >= (unsigned)n) {
+      c %= n;
+      if (c < 0) c += n;
     }
-    s = scaled<T>(r, d, invd);
-  } else {
-    if (ORDER == 3 && is_x && los == 0 && (u - lo < d)) u += L;
-    s = scaled<T>(u - lo, d, invd);
+    return c;
   }
-  int c = (ORDER == 3) ? (int)floor(s + T(0.5)) : (int)floor(s);
-  if (edge == CAPOW_EDGE_PERIODIC || los == 0) {
-    c %= n;
-    if (c < 0) c += n;
-  } else {
-    c = c < 0 ? 0 : (c >= n ? n - 1 : c);
-  }
-  return c;
+  return c < 0 ? 0 : (c >= n ? n - 1 : c);
 }
-
-// Fast unsigned 32-bit division by a runtime constant (round-up method).
-struct FastDiv {
-  uint32_t d, m, s;
-  __host__ void init(uint32_t div) {
-    d = div;
-    s = 0;
-    while ((1ull << s) < div) s++;
-    m = (uint32_t)(((1ull << 32) * ((1ull << s) - div)) / div + 1);
-  }
-  __device__ __forceinline__ uint32_t div(uint32_t x) const {
-    uint32_t t = __umulhi(x, m);
-    return (t + ((x - t) >> (s ? 1 : 0))) >> (s ? s - 1 : 0);
-  }
-};
 
 }  // namespace capow

@@ -22,10 +22,10 @@
 //      (value - alpha for density fields).  No grid memset, no global
 //      atomics on the grid.
 // Determinism: every tap contribution is rounded to an integer multiple of
-// a per-field quantum 2^-e (e from the field's max |value|, so one tap is
-// < 2^41 quanta) and accumulated in LDS as an integer-valued double.  Sums
-// of such values below 2^53 quanta are exact, so the grid does not depend
-// on the order in which particles or taps are added.
+// a quantum (FP32 form: per tile and field, int32 taps in one int64
+// accumulator per cell; FP64 form: per field, integer-valued doubles), and
+// integer sums do not depend on the order in which particles or taps are
+// added, so the grid does not depend on the sort order.
 #include <type_traits>
 
 #include "common.h"
@@ -91,23 +91,30 @@ struct PClass {
   int nt[3];       // how many (1 or 2)
 };
 
-// Every kept tap must lie within [key+reach_lo, key+reach_hi] (mod N on
-// periodic axes), otherwise a tile that owns it would not read the particle.
-template <typename T, int ORDER>
-__device__ __forceinline__ bool reach_ok(const AxisOut<T>& o, int key, int n, const Geom& g) {
-  bool ok = true;
-#pragma unroll
-  for (int t = 0; t < ORDER; t++) {
-    if (o.w[t] == T(0)) continue;
-    int off = o.first + t - key;
-    if (g.periodic) {
-      off %= n;
-      if (off < 0) off += n;
-      if (off > n / 2) off -= n;
-    }
-    if (off < g.reach_lo || off > g.reach_hi) ok = false;
+// Canonical periodic offset in [n/2 - n + 1, n/2] of an offset within one
+// period either side.
+__device__ __forceinline__ int canon_off(int off, int n) {
+  const int h = n / 2;
+  off = off > h ? off - n : off;
+  off = off <= h - n ? off + n : off;
+  return off;
+}
+
+// Taps of one deposit (main or interlaced) along an axis span
+// [first, first + ORDER - 1] (dropped taps only narrow this, and
+// non-periodic axes drop taps outside [0, N)); all of them must lie within
+// [key + reach_lo, key + reach_hi], otherwise a tile that owns a tap would
+// not hold a copy of the particle.
+template <int ORDER>
+__device__ __forceinline__ bool reach_ok(int first, int key, int n, const Geom& g) {
+  int a = first, b = first + ORDER - 1;
+  if (!g.periodic) {
+    a = a < 0 ? 0 : a;
+    b = b > n - 1 ? n - 1 : b;
+    if (a > b) return true;   // every tap dropped
+    return a - key >= g.reach_lo && b - key <= g.reach_hi;
   }
-  return ok;
+  return canon_off(a - key, n) >= g.reach_lo && canon_off(b - key, n) <= g.reach_hi;
 }
 
 template <typename T, int ORDER>
@@ -121,65 +128,54 @@ __device__ __forceinline__ PClass<T, ORDER> classify(T x, T y, T z, T v, const G
     else z += sh;
   }
   c.x = x; c.y = y; c.z = z;
-  const T lo0 = (T)g.lo[0], lo1 = (T)g.lo[1], lo2 = (T)g.lo[2];
-  const T hi0 = (T)g.hi[0], hi1 = (T)g.hi[1], hi2 = (T)g.hi[2];
-  const T d0 = (T)g.d[0], d1 = (T)g.d[1], d2 = (T)g.d[2];
-  const T L0 = (T)g.L[0], L1 = (T)g.L[1], L2 = (T)g.L[2];
-  AxisOut<T> mx = axis_taps<T, ORDER>(x, lo0, hi0, d0, L0, g.n[0], true, g.los, g.edge, (T)g.invd[0]);
-  AxisOut<T> my = axis_taps<T, ORDER>(y, lo1, hi1, d1, L1, g.n[1], false, g.los, g.edge, (T)g.invd[1]);
-  AxisOut<T> mz = axis_taps<T, ORDER>(z, lo2, hi2, d2, L2, g.n[2], false, g.los, g.edge, (T)g.invd[2]);
-  c.acc_main = mx.ok && my.ok && mz.ok;
-  AxisOut<T> ix = mx, iy = my, iz = mz;
-  c.acc_int = false;
-  if (g.interlace) {
-    ix = axis_taps<T, ORDER>(x + d0 / T(2), lo0, hi0, d0, L0, g.n[0], true, g.los, g.edge, (T)g.invd[0]);
-    iy = axis_taps<T, ORDER>(y + d1 / T(2), lo1, hi1, d1, L1, g.n[1], false, g.los, g.edge, (T)g.invd[1]);
-    iz = axis_taps<T, ORDER>(z + d2 / T(2), lo2, hi2, d2, L2, g.n[2], false, g.los, g.edge, (T)g.invd[2]);
-    c.acc_int = ix.ok && iy.ok && iz.ok;
+  const T u[3] = {x, y, z};
+  bool okm = true, oki = (g.interlace != 0);
+  int cm[3], ci[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    const T lo = (T)g.lo[a], hi = (T)g.hi[a], d = (T)g.d[a], L = (T)g.L[a], invd = (T)g.invd[a];
+    const AxisCell<T> m = axis_cell<T, ORDER>(u[a], lo, hi, d, L, g.n[a], a == 0, g.los, g.edge, invd);
+    okm = okm && m.ok;
+    cm[a] = m.c;
+    ci[a] = m.c;
+    if (g.interlace) {
+      const AxisCell<T> i = axis_cell<T, ORDER>(u[a] + d / T(2), lo, hi, d, L, g.n[a], a == 0, g.los,
+                                                g.edge, invd);
+      oki = oki && i.ok;
+      ci[a] = i.c;
+    }
   }
+  c.acc_main = okm;
+  c.acc_int = oki;
+  c.bucket = (okm || oki) ? 0 : -1;
   c.reach_ok = true;
-  if (!c.acc_main && !c.acc_int) {
-    c.bucket = -1;
-    return c;
-  }
-  int key[3];
-  key[0] = axis_key<T, ORDER>(x, lo0, hi0, d0, L0, g.n[0], true, g.los, g.edge, (T)g.invd[0]);
-  key[1] = axis_key<T, ORDER>(y, lo1, hi1, d1, L1, g.n[1], false, g.los, g.edge, (T)g.invd[1]);
-  key[2] = axis_key<T, ORDER>(z, lo2, hi2, d2, L2, g.n[2], false, g.los, g.edge, (T)g.invd[2]);
-  if (c.acc_main) {
-    c.reach_ok &= reach_ok<T, ORDER>(mx, key[0], g.n[0], g);
-    c.reach_ok &= reach_ok<T, ORDER>(my, key[1], g.n[1], g);
-    c.reach_ok &= reach_ok<T, ORDER>(mz, key[2], g.n[2], g);
-  }
-  if (c.acc_int) {
-    c.reach_ok &= reach_ok<T, ORDER>(ix, key[0], g.n[0], g);
-    c.reach_ok &= reach_ok<T, ORDER>(iy, key[1], g.n[1], g);
-    c.reach_ok &= reach_ok<T, ORDER>(iz, key[2], g.n[2], g);
-  }
   // Deposit tiles reached along each axis: the key's tile, plus the tile
   // above when the key lies within reach_hi of its upper edge, plus the tile
   // below when it lies within -reach_lo of its lower edge.
   const int lw = -g.reach_lo, hw = g.reach_hi;
+  const bool per = g.periodic != 0;
 #pragma unroll
   for (int a = 0; a < 3; a++) {
-    const int td = g.tdim[a], nt = g.ntile[a];
-    const int t0 = key[a] / td;
-    const int local = key[a] - t0 * td;
+    const int n = g.n[a], td = g.tdim[a], nt = g.ntile[a];
+    const int key = cell_key(cm[a], n, per);
+    const int t0 = (int)g.tdiv[a].div((uint32_t)key);
+    const int local = key - t0 * td;
     int second = -1;
     if (nt > 1) {
-      if (local >= td - hw) {
-        second = t0 + 1;
-        if (second >= nt) second = g.periodic ? second - nt : -1;
-      } else if (local < lw) {
-        second = t0 - 1;
-        if (second < 0) second = g.periodic ? second + nt : -1;
-      }
+      const int first_m = ORDER == 3 ? cm[a] - 1 : cm[a];
+      const int first_i = ORDER == 3 ? ci[a] - 1 : ci[a];
+      const bool r = (!okm || reach_ok<ORDER>(first_m, key, n, g)) &&
+                     (!oki || reach_ok<ORDER>(first_i, key, n, g));
+      c.reach_ok = c.reach_ok && r;
+      int up = t0 + 1, dn = t0 - 1;
+      up = up >= nt ? (per ? up - nt : -1) : up;
+      dn = dn < 0 ? (per ? dn + nt : -1) : dn;
+      second = local >= td - hw ? up : (local < lw ? dn : -1);
     }
     c.tile[a][0] = t0;
     c.tile[a][1] = second;
     c.nt[a] = second >= 0 ? 2 : 1;
   }
-  c.bucket = 0;
   return c;
 }
 
@@ -197,32 +193,83 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
   return r;
 }
 
-// Adds one to ctr[key] for every active lane with has = true.  Lanes of a
-// wavefront that share a key are merged into one atomic (the leader adds the
-// group size); after 4 groups the remaining lanes fall back to one atomic
-// each.  RET: returns each lane's slot (old value + rank within its group).
+// Block-local aggregation of the counting sort.  A block takes a chunk of
+// BK_CHUNK consecutive particles; every copy (particle, reached tile) gets a
+// key -- the tile in the count pass, 2*tile + ghost in the scatter pass --
+// and is counted in an LDS hash table (open addressing, linear probing).
+// Lanes of a wavefront that share a key are merged first (the leader adds
+// the group size), so a slot sees one LDS atomic per group.  At the end of
+// the chunk each occupied slot issues ONE global atomic for all its copies:
+// with catalogues stored in spatial order (simulation snapshots, lattices)
+// a chunk reaches a few dozen tiles, so the device-scope atomics drop from
+// one per wavefront group to one per (chunk, key).  A key that finds no
+// free slot within BK_PROBE probes falls back to its own global atomic.
+#define BK_T 256
+#define BK_K 4
+#define BK_HS_BITS 12
+#define BK_HS (1 << BK_HS_BITS)
+#define BK_PROBE 32
+#define BK_EMPTY 0xffffffffu
+#define BK_AGG 8
+static_assert(BK_T * BK_K == BK_CHUNK, "chunk size");
+static_assert(BK_T * BK_K <= (1 << (32 - BK_HS_BITS - 1)), "rank field");
+
+__device__ __forceinline__ int hs_find(uint32_t* __restrict__ hkey, uint32_t key) {
+  uint32_t h = (key * 2654435761u) >> (32 - BK_HS_BITS);
+#pragma unroll 1
+  for (int p = 0; p < BK_PROBE; p++) {
+    const uint32_t cur = hkey[h];
+    if (cur == key) return (int)h;
+    if (cur == BK_EMPTY) {
+      const uint32_t old = atomicCAS(&hkey[h], BK_EMPTY, key);
+      if (old == BK_EMPTY || old == key) return (int)h;
+    }
+    h = (h + 1) & (BK_HS - 1);
+  }
+  return -1;
+}
+
+// Counts one copy of `key` for every lane with has = true.  Returns the slot
+// (-1: table full, the caller takes the global path) and, for RET, the
+// lane's rank among the slot's copies in this chunk.
 template <bool RET>
-__device__ __forceinline__ uint32_t wave_inc(uint32_t* __restrict__ ctr, int key, bool has) {
+__device__ __forceinline__ int hs_add(uint32_t* __restrict__ hkey, uint32_t* __restrict__ hcnt,
+                                      uint32_t key, bool has, uint32_t& rank) {
   const int lane = __lane_id();
   const unsigned long long lt = (1ull << lane) - 1ull;
   unsigned long long pending = __ballot(has);
-  uint32_t slot = 0;
+  int slot = -1;
+  rank = 0;
+#pragma unroll 1
   for (int it = 0; pending; it++) {
-    if (it == 4) {
+    if (it == BK_AGG) {
+      // many distinct keys in this wavefront: one LDS insert per lane
       if ((pending >> lane) & 1ull) {
-        if (RET) slot = atomicAdd(&ctr[key], 1u);
-        else atomicAdd(&ctr[key], 1u);
+        slot = hs_find(hkey, key);
+        if (slot >= 0) {
+          if (RET) rank = atomicAdd(&hcnt[slot], 1u);
+          else atomicAdd(&hcnt[slot], 1u);
+        }
       }
       break;
     }
     const int leader = __ffsll((long long)pending) - 1;
-    const int lk = __builtin_amdgcn_readlane(key, leader);
+    const uint32_t lk = (uint32_t)__builtin_amdgcn_readlane((int)key, leader);
     const unsigned long long m = __ballot(has && key == lk) & pending;
+    int s = 0;
     uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(&ctr[lk], (uint32_t)__popcll(m));
-    if (RET) {
-      base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
-      if ((m >> lane) & 1ull) slot = base + (uint32_t)__popcll(m & lt);
+    if (lane == leader) {
+      s = hs_find(hkey, lk);
+      if (s >= 0) {
+        if (RET) base = atomicAdd(&hcnt[s], (uint32_t)__popcll(m));
+        else atomicAdd(&hcnt[s], (uint32_t)__popcll(m));
+      }
+    }
+    s = __builtin_amdgcn_readlane(s, leader);
+    if (RET) base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
+    if ((m >> lane) & 1ull) {
+      slot = s;
+      if (RET) rank = base + (uint32_t)__popcll(m & lt);
     }
     pending &= ~m;
   }
@@ -230,87 +277,134 @@ __device__ __forceinline__ uint32_t wave_inc(uint32_t* __restrict__ ctr, int key
 }
 
 template <typename T, int ORDER, bool SCATTER>
-__global__ __launch_bounds__(256) void k_bucket(const T* __restrict__ parts,
-                                                const T* __restrict__ wts, int64_t n,
-                                                Geom g, uint32_t* __restrict__ counts,
-                                                uint32_t* __restrict__ cursor,
-                                                const uint32_t* __restrict__ off,
-                                                T* __restrict__ sorted,
-                                                T* __restrict__ sorted_w, int64_t cap,
-                                                double* __restrict__ partials,
-                                                unsigned long long* __restrict__ gmax,
-                                                unsigned int* __restrict__ errs) {
-  __shared__ double sh[256];
+__global__ __launch_bounds__(BK_T) void k_bucket(const T* __restrict__ parts,
+                                                 const T* __restrict__ wts, int64_t n,
+                                                 Geom g, uint32_t* __restrict__ counts,
+                                                 uint32_t* __restrict__ cursor,
+                                                 const uint32_t* __restrict__ off,
+                                                 T* __restrict__ sorted,
+                                                 T* __restrict__ sorted_w, int64_t cap,
+                                                 double* __restrict__ partials,
+                                                 unsigned long long* __restrict__ gmax,
+                                                 unsigned int* __restrict__ errs) {
+  __shared__ uint32_t hkey[BK_HS];
+  __shared__ uint32_t hcnt[BK_HS];   // copies per slot; in the scatter pass, then the slot's base
+  __shared__ double sh[BK_T];
   double s_ngrid = 0, s_w = 0, s_v1 = 0, s_v2 = 0, s_v3 = 0, s_v4 = 0, s_out = 0;
   double m_a = 0, m_b = 0;
   unsigned int oob = 0, reach_bad = 0;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    T x, y, z, v;
-    load4<T>(parts, i, x, y, z, v);
-    T w = g.has_w ? wts[i] : T(1);
-    PClass<T, ORDER> c = classify<T, ORDER>(x, y, z, v, g);
-    const bool take = c.bucket >= 0;
-    // one copy per reached tile, in a fixed order (iz, iy, ix)
-    for (int q = 0; q < 8; q++) {
-      const int qx = q & 1, qy = (q >> 1) & 1, qz = q >> 2;
-      const bool has = take && qx < c.nt[0] && qy < c.nt[1] && qz < c.nt[2];
-      if (!__any(has)) continue;
-      int tile = -1;
-      if (has) {
-        const int tx = qx ? c.tile[0][1] : c.tile[0][0];
-        const int ty = qy ? c.tile[1][1] : c.tile[1][0];
-        const int tz = qz ? c.tile[2][1] : c.tile[2][0];
-        tile = (tz * g.ntile[1] + ty) * g.ntile[0] + tx;
+  const int64_t nchunks = (n + BK_CHUNK - 1) / BK_CHUNK;
+  const uint32_t nb = (uint32_t)g.nbuckets;
+  for (int64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+    for (int s = threadIdx.x; s < BK_HS; s += BK_T) {
+      hkey[s] = BK_EMPTY;
+      hcnt[s] = 0;
+    }
+    __syncthreads();
+    // scatter pass: the chunk's copies wait in registers for their slot bases
+    T px[BK_K], py[BK_K], pz[BK_K], pv[BK_K], pw[BK_K];
+    uint32_t pr[BK_K][8];   // slot | rank << BK_HS_BITS; BK_EMPTY: no copy / written
+#pragma unroll
+    for (int k = 0; k < BK_K; k++) {
+      const int64_t i = ch * BK_CHUNK + (int64_t)k * BK_T + threadIdx.x;
+      const bool valid = i < n;
+      T x = T(0), y = T(0), z = T(0), v = T(0), w = T(1);
+      if (valid) {
+        load4<T>(parts, i, x, y, z, v);
+        if (g.has_w) w = wts[i];
       }
-      if (SCATTER) {
+      PClass<T, ORDER> c = classify<T, ORDER>(x, y, z, v, g);
+      const bool take = valid && c.bucket >= 0;
+      px[k] = c.x; py[k] = c.y; pz[k] = c.z; pv[k] = v; pw[k] = w;
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        pr[k][q] = BK_EMPTY;
+        const int qx = q & 1, qy = (q >> 1) & 1, qz = q >> 2;
+        const bool has = take && qx < c.nt[0] && qy < c.nt[1] && qz < c.nt[2];
+        if (!__any(has)) continue;
+        int tile = 0;
+        if (has) {
+          const int tx = qx ? c.tile[0][1] : c.tile[0][0];
+          const int ty = qy ? c.tile[1][1] : c.tile[1][0];
+          const int tz = qz ? c.tile[2][1] : c.tile[2][0];
+          tile = (tz * g.ntile[1] + ty) * g.ntile[0] + tx;
+        }
         // own copies fill each tile's range from the front, ghost copies
         // from the back, so deposit waves are mostly homogeneous
-        uint32_t pos;
-        if (q == 0) {
-          const uint32_t r = wave_inc<true>(cursor, tile, has);
-          pos = has ? off[tile] + r : 0u;
-        } else {
-          const uint32_t r = wave_inc<true>(cursor + g.nbuckets, tile, has);
-          pos = has ? off[tile + 1] - 1u - r : 0u;
+        const uint32_t key = SCATTER ? 2u * (uint32_t)tile + (q ? 1u : 0u) : (uint32_t)tile;
+        uint32_t rank;
+        const int slot = hs_add<SCATTER>(hkey, hcnt, key, has, rank);
+        if (has && slot < 0) {
+          if (SCATTER) {
+            const uint32_t r = atomicAdd(&cursor[(q ? nb : 0u) + (uint32_t)tile], 1u);
+            const uint32_t pos = q ? off[tile + 1] - 1u - r : off[tile] + r;
+            if ((int64_t)pos < cap) {
+              store4<T>(sorted, pos, c.x, c.y, c.z, v);
+              if (g.has_w) sorted_w[pos] = w;
+            }
+          } else {
+            atomicAdd(&counts[tile], 1u);
+          }
+        } else if (has) {
+          pr[k][q] = (uint32_t)slot | (rank << BK_HS_BITS);
         }
-        if (has && (int64_t)pos < cap) {
-          store4<T>(sorted, pos, c.x, c.y, c.z, v);
-          if (g.has_w) sorted_w[pos] = w;
+      }
+      if (!SCATTER && valid) {
+        if (!c.reach_ok) reach_bad++;
+        // src/read_data.c:81-88 (on the shifted position)
+        if ((c.x < (T)g.lo[0]) || (c.x > (T)g.hi[0]) || (c.y < (T)g.lo[1]) ||
+            (c.y > (T)g.hi[1]) || (c.z < (T)g.lo[2]) || (c.z > (T)g.hi[2])) {
+          if (g.los == 0) oob++;
+          else s_out += 1.0;
         }
-      } else {
-        wave_inc<false>(counts, tile, has);
+        if (c.acc_main) s_ngrid += (double)w;
+        // moments over every particle read (src/read_data.c:132-138)
+        s_w += (double)w;
+        s_v1 += (double)(w * v);
+        s_v2 += (double)(w * v * v);
+        s_v3 += (double)(w * v * v * v);
+        s_v4 += (double)(w * v * v * v * v);
+        if (c.acc_main || c.acc_int) {
+          m_a = fmax(m_a, fabs((double)(w * vpow<T>(v, g.fa))));
+          if (g.fb >= 0) m_b = fmax(m_b, fabs((double)(w * vpow<T>(v, g.fb))));
+        }
       }
     }
-    if (!SCATTER) {
-      if (!c.reach_ok) reach_bad++;
-      // src/read_data.c:81-88 (on the shifted position)
-      if ((c.x < (T)g.lo[0]) || (c.x > (T)g.hi[0]) || (c.y < (T)g.lo[1]) ||
-          (c.y > (T)g.hi[1]) || (c.z < (T)g.lo[2]) || (c.z > (T)g.hi[2])) {
-        if (g.los == 0) oob++;
-        else s_out += 1.0;
-      }
-      if (c.acc_main) s_ngrid += (double)w;
-      // moments over every particle read (src/read_data.c:132-138)
-      double wd = (double)w, vd = (double)v;
-      s_w += wd;
-      s_v1 += (double)(w * v);
-      s_v2 += (double)(w * v * v);
-      s_v3 += (double)(w * v * v * v);
-      s_v4 += (double)(w * v * v * v * v);
-      if (c.acc_main || c.acc_int) {
-        double fa = fabs((double)(w * vpow<T>(v, g.fa)));
-        m_a = fmax(m_a, fa);
-        if (g.fb >= 0) m_b = fmax(m_b, fabs((double)(w * vpow<T>(v, g.fb))));
-      }
-      (void)vd;
+    __syncthreads();
+    // one global atomic per occupied slot
+    for (int s = threadIdx.x; s < BK_HS; s += BK_T) {
+      const uint32_t key = hkey[s];
+      if (key == BK_EMPTY) continue;
+      if (SCATTER) hcnt[s] = atomicAdd(&cursor[(key & 1u) ? nb + (key >> 1) : (key >> 1)], hcnt[s]);
+      else atomicAdd(&counts[key], hcnt[s]);
     }
+    if (SCATTER) {
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < BK_K; k++) {
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+          const uint32_t e = pr[k][q];
+          if (e == BK_EMPTY) continue;
+          const uint32_t slot = e & (BK_HS - 1);
+          const uint32_t key = hkey[slot];
+          const uint32_t tile = key >> 1;
+          const uint32_t r = hcnt[slot] + (e >> BK_HS_BITS);
+          const uint32_t pos = (key & 1u) ? off[tile + 1] - 1u - r : off[tile] + r;
+          if ((int64_t)pos < cap) {
+            store4<T>(sorted, pos, px[k], py[k], pz[k], pv[k]);
+            if (g.has_w) sorted_w[pos] = pw[k];
+          }
+        }
+      }
+    }
+    __syncthreads();
   }
   if (!SCATTER) {
     double r[7] = {s_ngrid, s_w, s_v1, s_v2, s_v3, s_v4, s_out};
 #pragma unroll
     for (int q = 0; q < 7; q++) {
-      double t = block_sum<256>(r[q], sh);
+      double t = block_sum<BK_T>(r[q], sh);
       if (threadIdx.x == 0) partials[(int64_t)blockIdx.x * STAT_N + q] = t;
     }
     // max |field| (non-negative doubles order as their bit patterns)
@@ -419,52 +513,51 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_apply(const uint32_t* __restric
 // Tile deposit.
 // ---------------------------------------------------------------------------
 #define DEP_T 512
+#define DEP_K 4       // particle copies per thread kept in registers for both phases
+#define DEP_TRASH 64  // per-lane sink cells for taps outside the tile
+// LDS bytes of the largest tile: FP32 16x16x32 (int64 per cell), FP64
+// 16x16x16 (two doubles per cell), each with the sink cells
+#define DEP_LDS_MAX (2 * (16 * 16 * 16 + DEP_TRASH) * 8)
+static_assert((16 * 16 * 32 + DEP_TRASH) * 8 <= DEP_LDS_MAX, "FP32 tile");
 
-// Accumulator policy.  int (FP32 form): each tap is rounded to an int32
-// multiple of a per-tile quantum q chosen so that (copies in the tile) x
-// max|field| x (max tap weight) / q <= 2^30 -- no cell can overflow, and
-// integer sums are exact.  double (FP64 form): each tap is rounded to a
+// Exact accumulation.  FP32 form: each tap is rounded to an int32 multiple
+// of a per-tile quantum q chosen so that (copies in the tile) x max|field| x
+// (max tap weight) / q <= 2^30, so no partial sum of either field leaves
+// (-2^31, 2^31); fields A and B of a cell share one int64 accumulator,
+// A * 2^32 + B, updated with one 64-bit LDS atomic per tap (the low word
+// read as int32 is B, the rest is A).  FP64 form: each tap is rounded to a
 // multiple of 2^-e, e from the field's max |value| so that one tap is below
-// 2^40 quanta; sums below 2^53 quanta are exact in ds_add_f64.
-template <typename ACC>
-struct Quant;
-template <>
-struct Quant<int> {
-  double scale, inv;
-  __device__ void init(uint32_t count, double gmax, double wmax, double /*gscale*/) {
-    const double bound = (double)(count ? count : 1) * (gmax > 0 ? gmax : 1.0) * wmax;
-    int ex;
-    frexp(bound, &ex);                 // bound < 2^ex
-    scale = ldexp(1.0, 30 - ex);       // bound * scale < 2^30
-    inv = ldexp(1.0, ex - 30);
-  }
-  template <typename T>
-  __device__ __forceinline__ int q(T vs, T w) const { return __float2int_rn((float)(vs * w)); }
-};
-template <>
-struct Quant<double> {
-  double scale, inv;
-  __device__ void init(uint32_t, double, double, double gscale) {
-    scale = gscale;
-    inv = 1.0 / gscale;   // power of two: exact
-  }
-  template <typename T>
-  __device__ __forceinline__ double q(T vs, T w) const { return rint((double)vs * (double)w); }
-};
+// 2^40 quanta, and summed with ds_add_f64; sums below 2^53 quanta are exact.
+// Integer sums do not depend on the order of the additions.
+__device__ __forceinline__ double tile_scale(uint32_t count, double gmax, double wmax) {
+  const double bound = (double)(count ? count : 1) * (gmax > 0 ? gmax : 1.0) * wmax;
+  int ex;
+  frexp(bound, &ex);               // bound < 2^ex
+  return ldexp(1.0, 30 - ex);      // bound * scale < 2^30
+}
 
-// z-index swizzle of LDS tile row `rowi`: an even XOR mask below TZ when TZ
-// is a power of two, so rows that differ only in (x, y) land in different
-// LDS banks (without it every row starts on one of two bank groups); z pairs
-// (2j, 2j+1) stay adjacent.
-__device__ __forceinline__ int zswz(int rowi, int TZ) {
-  return ((TZ & (TZ - 1)) == 0 && TZ >= 4) ? ((rowi << 1) & (TZ - 2)) : 0;
+// floor(x + 0.5) as int32 in one instruction (ties round up); the rounding
+// rule only has to be the same for every tap.
+__device__ __forceinline__ int round_rpi(float x) {
+  int r;
+  asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+  return r;
+}
+
+// XOR mask on the z index of LDS row `rowi` (hash of the row, below TZ when
+// TZ is a power of two): taps of neighbouring particles in different rows
+// land in different LDS banks.  An aligned cell pair (2j, 2j+1) maps to an
+// aligned pair, swapped when the mask is odd.
+__device__ __forceinline__ int zmask(int rowi, int TZ) {
+  return ((TZ & (TZ - 1)) == 0) ? (int)((__umul24((uint32_t)rowi, 0x9E3779u) >> 12) & (uint32_t)(TZ - 1)) : 0;
 }
 
 // One workgroup per TX x TY x TZ tile.  The tile's particle copies (own
 // first, then ghosts) are one contiguous range of the sorted array.  Phase 0
 // deposits the main grids (A, B), phase 1 the interlaced grids: LDS holds
-// one phase, i.e. nf x ncell accumulators.
-template <typename T, typename ACC, int ORDER>
+// one phase.  Taps outside the tile go to the lane's sink cell, so the tap
+// loop has no branches.
+template <typename T, int ORDER, bool HASB>
 __global__ __launch_bounds__(DEP_T) void k_deposit(const T* __restrict__ sorted,
                                                    const T* __restrict__ sorted_w,
                                                    const uint32_t* __restrict__ off,
@@ -472,12 +565,19 @@ __global__ __launch_bounds__(DEP_T) void k_deposit(const T* __restrict__ sorted,
                                                    T* __restrict__ gAi, T* __restrict__ gB,
                                                    T* __restrict__ gBi,
                                                    const double* __restrict__ stats) {
-  extern __shared__ unsigned char smem[];
-  ACC* acc = reinterpret_cast<ACC*>(smem);  // [nf][ncell]
+  constexpr bool F32 = sizeof(T) == 4;
+  // static LDS (its base folds into the tap addresses): the largest tile's
+  // accumulators, checked on the host
+  __shared__ __align__(16) unsigned char smem[DEP_LDS_MAX];
   const int TX = g.tdim[0], TY = g.tdim[1], TZ = g.tdim[2];
   const int ncell = TX * TY * TZ;
-  const bool hasB = g.fb >= 0;
-  const int nf = hasB ? 2 : 1;
+  const int nacc = ncell + DEP_TRASH;
+  // FP32: int64 [nacc]; FP64: double A [nacc], B [nacc]
+  unsigned long long* acc64 = reinterpret_cast<unsigned long long*>(smem);
+  double* accA = reinterpret_cast<double*>(smem);
+  double* accB = accA + nacc;
+  const int lane = threadIdx.x & 63;
+  const int trash = ncell + lane;
 
   const int64_t tileid = blockIdx.x;
   const int ttx = (int)(tileid % g.ntile[0]);
@@ -487,65 +587,99 @@ __global__ __launch_bounds__(DEP_T) void k_deposit(const T* __restrict__ sorted,
   const uint32_t s0 = off[tileid], s1 = off[tileid + 1];
 
   const double wmax = ORDER == 3 ? 0.421875 : 1.0;
-  Quant<ACC> qa, qb;
-  qa.init(s1 - s0, stats[STAT_GMAX_A], wmax, stats[STAT_SCALE_A]);
-  qb.init(s1 - s0, stats[STAT_GMAX_B], wmax, stats[STAT_SCALE_B]);
-  const T scA = (T)qa.scale, scB = (T)qb.scale;
+  const double scA = F32 ? tile_scale(s1 - s0, stats[STAT_GMAX_A], wmax) : stats[STAT_SCALE_A];
+  const double scB = F32 ? tile_scale(s1 - s0, stats[STAT_GMAX_B], wmax) : stats[STAT_SCALE_B];
+  const double invA = 1.0 / scA, invB = 1.0 / scB;   // powers of two: exact
   const T d0 = (T)g.d[0], d1 = (T)g.d[1], d2 = (T)g.d[2];
   const double alpha = stats[STAT_ALPHA];
+  const bool single = (s1 - s0) <= (uint32_t)(DEP_T * DEP_K);
 
+  T px[DEP_K], py[DEP_K], pz[DEP_K], pa[DEP_K], pb[DEP_K];
+  bool ph_has[DEP_K];
   for (int ph = 0; ph < (g.interlace ? 2 : 1); ph++) {
-    for (int c = threadIdx.x; c < nf * ncell; c += DEP_T) acc[c] = ACC(0);
+    {
+      uint4* z4 = reinterpret_cast<uint4*>(smem);
+      const int n4 = F32 ? nacc / 2 : nacc;   // 16-byte words
+      for (int c = threadIdx.x; c < n4; c += DEP_T) z4[c] = make_uint4(0u, 0u, 0u, 0u);
+    }
     __syncthreads();
-    for (uint32_t i = s0 + threadIdx.x; i < s1; i += DEP_T) {
-      T x, y, z, v;
-      load4<T>(sorted, i, x, y, z, v);
-      const T w = g.has_w ? sorted_w[i] : T(1);
-      const T va = w * vpow<T>(v, g.fa) * scA;
-      const T vb = hasB ? w * vpow<T>(v, g.fb) * scB : T(0);
-      if (ph) {
-        x += d0 / T(2);
-        y += d1 / T(2);
-        z += d2 / T(2);
-      }
-      const AxisOut<T> ax = axis_taps<T, ORDER>(x, (T)g.lo[0], (T)g.hi[0], d0, (T)g.L[0], g.n[0], true, g.los, g.edge, (T)g.invd[0]);
-      const AxisOut<T> ay = axis_taps<T, ORDER>(y, (T)g.lo[1], (T)g.hi[1], d1, (T)g.L[1], g.n[1], false, g.los, g.edge, (T)g.invd[1]);
-      const AxisOut<T> az = axis_taps<T, ORDER>(z, (T)g.lo[2], (T)g.hi[2], d2, (T)g.L[2], g.n[2], false, g.los, g.edge, (T)g.invd[2]);
-      if (!(ax.ok && ay.ok && az.ok)) continue;
-      // tile-local tap coordinates; weight 0 marks taps outside the tile
-      int lx[ORDER], ly[ORDER], lz[ORDER];
-      T wx[ORDER], wy[ORDER], wz[ORDER];
+    for (uint32_t base = s0; base < s1; base += DEP_T * DEP_K) {
+      if (ph == 0 || !single) {
 #pragma unroll
-      for (int t = 0; t < ORDER; t++) {
-        int gx = ax.first + t, gy = ay.first + t, gz = az.first + t;
-        if (g.periodic) {
-          gx += gx < 0 ? g.n[0] : (gx >= g.n[0] ? -g.n[0] : 0);
-          gy += gy < 0 ? g.n[1] : (gy >= g.n[1] ? -g.n[1] : 0);
-          gz += gz < 0 ? g.n[2] : (gz >= g.n[2] ? -g.n[2] : 0);
+        for (int k = 0; k < DEP_K; k++) {
+          const uint32_t i = base + (uint32_t)(k * DEP_T) + threadIdx.x;
+          ph_has[k] = i < s1;
+          T x = T(0), y = T(0), z = T(0), v = T(0), w = T(1);
+          if (ph_has[k]) {
+            load4<T>(sorted, i, x, y, z, v);
+            if (g.has_w) w = sorted_w[i];
+          }
+          px[k] = x; py[k] = y; pz[k] = z;
+          pa[k] = w * vpow<T>(v, g.fa) * (T)scA;
+          pb[k] = HASB ? w * vpow<T>(v, g.fb) * (T)scB : T(0);
         }
-        lx[t] = gx - t0x;
-        ly[t] = gy - t0y;
-        lz[t] = gz - t0z;
-        wx[t] = ((unsigned)lx[t] < (unsigned)TX) ? ax.w[t] : T(0);
-        wy[t] = ((unsigned)ly[t] < (unsigned)TY) ? ay.w[t] : T(0);
-        wz[t] = ((unsigned)lz[t] < (unsigned)TZ) ? az.w[t] : T(0);
       }
 #pragma unroll
-      for (int tx = 0; tx < ORDER; tx++) {
+      for (int k = 0; k < DEP_K; k++) {
+        if (!ph_has[k]) continue;
+        T x = px[k], y = py[k], z = pz[k];
+        if (ph) {
+          x += d0 / T(2);
+          y += d1 / T(2);
+          z += d2 / T(2);
+        }
+        const AxisOut<T> ax = axis_taps<T, ORDER>(x, (T)g.lo[0], (T)g.hi[0], d0, (T)g.L[0], g.n[0], true, g.los, g.edge, (T)g.invd[0]);
+        const AxisOut<T> ay = axis_taps<T, ORDER>(y, (T)g.lo[1], (T)g.hi[1], d1, (T)g.L[1], g.n[1], false, g.los, g.edge, (T)g.invd[1]);
+        const AxisOut<T> az = axis_taps<T, ORDER>(z, (T)g.lo[2], (T)g.hi[2], d2, (T)g.L[2], g.n[2], false, g.los, g.edge, (T)g.invd[2]);
+        const bool ok = ax.ok && ay.ok && az.ok;
+        const T va = ok ? pa[k] : T(0), vb = ok ? pb[k] : T(0);
+        // tile-local tap coordinates; taps outside the tile are invalid
+        int lx[ORDER], ly[ORDER], lz[ORDER], lz8[ORDER];
+        bool vx[ORDER], vy[ORDER], vz[ORDER];
 #pragma unroll
-        for (int ty = 0; ty < ORDER; ty++) {
-          const T wxy = wx[tx] * wy[ty];
-          if (wxy == T(0)) continue;
-          const int rowi = lx[tx] * TY + ly[ty];
-          const int row = rowi * TZ;
-          const int swz = zswz(rowi, TZ);
-          const T pa = va * wxy, pb = vb * wxy;
+        for (int t = 0; t < ORDER; t++) {
+          int gx = ax.first + t, gy = ay.first + t, gz = az.first + t;
+          if (g.periodic) {
+            gx += gx < 0 ? g.n[0] : (gx >= g.n[0] ? -g.n[0] : 0);
+            gy += gy < 0 ? g.n[1] : (gy >= g.n[1] ? -g.n[1] : 0);
+            gz += gz < 0 ? g.n[2] : (gz >= g.n[2] ? -g.n[2] : 0);
+          }
+          lx[t] = gx - t0x;
+          ly[t] = gy - t0y;
+          lz[t] = gz - t0z;
+          vx[t] = (unsigned)lx[t] < (unsigned)TX;
+          vy[t] = (unsigned)ly[t] < (unsigned)TY;
+          vz[t] = (unsigned)lz[t] < (unsigned)TZ;
+          lz8[t] = lz[t] * 8;
+        }
 #pragma unroll
-          for (int tz = 0; tz < ORDER; tz++) {
-            if (wz[tz] == T(0)) continue;
-            const int cell = row + (lz[tz] ^ swz);
-            atomicAdd(&acc[cell], qa.q(pa, wz[tz]));
-            if (hasB) atomicAdd(&acc[ncell + cell], qb.q(pb, wz[tz]));
+        for (int tx = 0; tx < ORDER; tx++) {
+#pragma unroll
+          for (int ty = 0; ty < ORDER; ty++) {
+            const bool vxy = vx[tx] && vy[ty];
+            const int rowi = lx[tx] * TY + ly[ty];
+            const int m8 = zmask(rowi, TZ) * 8;
+            const int row8 = rowi * TZ * 8;
+            const T wxy = ax.w[tx] * ay.w[ty];
+            const T qa0 = va * wxy, qb0 = vb * wxy;
+#pragma unroll
+            for (int tz = 0; tz < ORDER; tz++) {
+              // byte offset of the cell's accumulator (8 bytes per cell)
+              const int cell8 = (vxy && vz[tz]) ? row8 + (lz8[tz] ^ m8) : trash * 8;
+              if constexpr (F32) {
+                const int qa = round_rpi((float)(qa0 * az.w[tz]));
+                const int qb = HASB ? round_rpi((float)(qb0 * az.w[tz])) : 0;
+                // A * 2^32 + B with B sign-extended
+                const uint32_t hi = (uint32_t)qa + (uint32_t)(qb >> 31);
+                const unsigned long long packed = ((unsigned long long)hi << 32) | (uint32_t)qb;
+                atomicAdd(reinterpret_cast<unsigned long long*>(smem + cell8), packed);
+              } else {
+                atomicAdd(reinterpret_cast<double*>(smem + cell8), rint((double)qa0 * (double)az.w[tz]));
+                if (HASB)
+                  atomicAdd(reinterpret_cast<double*>(smem + (size_t)nacc * 8 + cell8),
+                            rint((double)qb0 * (double)az.w[tz]));
+              }
+            }
           }
         }
       }
@@ -555,17 +689,40 @@ __global__ __launch_bounds__(DEP_T) void k_deposit(const T* __restrict__ sorted,
     T* dA = ph ? gAi : gA;
     T* dB = ph ? gBi : gB;
     const double meanA = g.mean_sub_a ? alpha : 0.0;
+    // cell pair c2 = (lxx*TY + ly)*(TZ/2) + lz/2, advanced by DEP_T pairs per
+    // iteration with carries instead of divisions
+    const int TZ2 = TZ / 2;
+    const int step_row = DEP_T / TZ2, step_z = DEP_T - step_row * TZ2;
+    const int step_x = step_row / TY, step_y = step_row - step_x * TY;
+    int zp = (int)threadIdx.x % TZ2, rowi0 = (int)threadIdx.x / TZ2;
+    int ly = rowi0 % TY, lxx = rowi0 / TY;
     for (int c2 = threadIdx.x; c2 < ncell / 2; c2 += DEP_T) {
-      const int c = 2 * c2;
-      const int lz = c % TZ, rowi = c / TZ;
-      const int ly = rowi % TY, lxx = rowi / TY;
+      const int lz = 2 * zp, rowi = lxx * TY + ly;
       const int64_t gi = ((int64_t)(t0x + lxx) * g.n[1] + (t0y + ly)) * g.rowlen + (t0z + lz);
-      const int cs = rowi * TZ + (lz ^ zswz(rowi, TZ));   // lz even, swizzle even: pair stays adjacent
-      T a0 = (T)((double)acc[cs] * qa.inv - meanA), a1 = (T)((double)acc[cs + 1] * qa.inv - meanA);
-      store2<T>(dA + gi, a0, a1);
-      if (hasB) {
-        T b0 = (T)((double)acc[ncell + cs] * qb.inv), b1 = (T)((double)acc[ncell + cs + 1] * qb.inv);
-        store2<T>(dB + gi, b0, b1);
+      const int m = zmask(rowi, TZ);
+      const int cs = rowi * TZ + ((lz ^ m) & ~1);   // aligned pair holding z = lz, lz+1
+      const bool swap = (m & 1) != 0;
+      zp += step_z;
+      ly += step_y;
+      lxx += step_x;
+      if (zp >= TZ2) { zp -= TZ2; ly += 1; }
+      if (ly >= TY) { ly -= TY; lxx += 1; }
+      if constexpr (F32) {
+        const ulonglong2 q = *reinterpret_cast<const ulonglong2*>(acc64 + cs);
+        const long long p0 = (long long)(swap ? q.y : q.x), p1 = (long long)(swap ? q.x : q.y);
+        const int b0 = (int)(unsigned)(p0 & 0xffffffffll), b1 = (int)(unsigned)(p1 & 0xffffffffll);
+        const int a0 = (int)((p0 - (long long)b0) >> 32), a1 = (int)((p1 - (long long)b1) >> 32);
+        store2<T>(dA + gi, (T)((double)a0 * invA - meanA), (T)((double)a1 * invA - meanA));
+        if (HASB) store2<T>(dB + gi, (T)((double)b0 * invB), (T)((double)b1 * invB));
+      } else {
+        const double2 qa = *reinterpret_cast<const double2*>(accA + cs);
+        const double a0 = swap ? qa.y : qa.x, a1 = swap ? qa.x : qa.y;
+        store2<T>(dA + gi, (T)(a0 * invA - meanA), (T)(a1 * invA - meanA));
+        if (HASB) {
+          const double2 qb = *reinterpret_cast<const double2*>(accB + cs);
+          const double b0 = swap ? qb.y : qb.x, b1 = swap ? qb.x : qb.y;
+          store2<T>(dB + gi, (T)(b0 * invB), (T)(b1 * invB));
+        }
       }
     }
     __syncthreads();
@@ -581,7 +738,7 @@ static int run_count_t(DepositArgs& a) {
   const int nb = a.count_blocks;
   hipStream_t st = a.stream;
   TIMED(a.timer, "bucket_count",
-        (k_bucket<T, ORDER, false><<<nb, 256, 0, st>>>((const T*)a.parts, (const T*)a.wts, a.n, g,
+        (k_bucket<T, ORDER, false><<<nb, BK_T, 0, st>>>((const T*)a.parts, (const T*)a.wts, a.n, g,
                                                         a.counts, nullptr, nullptr, nullptr, nullptr, 0,
                                                         a.partials, a.gmax, a.errs)));
   TIMED(a.timer, "stats_final",
@@ -607,21 +764,27 @@ static int run_scatter_t(DepositArgs& a) {
   const int nb = a.count_blocks;
   hipStream_t st = a.stream;
   TIMED(a.timer, "bucket_scatter",
-        (k_bucket<T, ORDER, true><<<nb, 256, 0, st>>>((const T*)a.parts, (const T*)a.wts, a.n, g,
+        (k_bucket<T, ORDER, true><<<nb, BK_T, 0, st>>>((const T*)a.parts, (const T*)a.wts, a.n, g,
                                                        nullptr, a.cursor, a.offsets, (T*)a.sorted,
                                                        (T*)a.sorted_w, a.sorted_cap, nullptr,
                                                        nullptr, nullptr)));
-  typedef typename std::conditional<sizeof(T) == 4, int, double>::type ACC;
-  const int nf = g.fb >= 0 ? 2 : 1;
-  const size_t lds = (size_t)nf * g.tdim[0] * g.tdim[1] * g.tdim[2] * sizeof(ACC);
+  const int ncell = g.tdim[0] * g.tdim[1] * g.tdim[2];
+  const size_t nacc = (size_t)ncell + DEP_TRASH;
+  const size_t lds = sizeof(T) == 4 ? nacc * 8 : 2 * nacc * 8;
   const int64_t ntiles = (int64_t)g.ntile[0] * g.ntile[1] * g.ntile[2];
-  if (lds > 64 * 1024)
-    hipFuncSetAttribute((const void*)k_deposit<T, ACC, ORDER>,
-                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  TIMED(a.timer, "deposit",
-        (k_deposit<T, ACC, ORDER><<<(unsigned)ntiles, DEP_T, lds, st>>>(
-            (const T*)a.sorted, (const T*)a.sorted_w, a.offsets, g, (T*)a.grid[0],
-            (T*)a.grid[1], (T*)a.grid[2], (T*)a.grid[3], a.stats)));
+  const bool hasB = g.fb >= 0;
+  if (lds > DEP_LDS_MAX) return CAPOW_ERR_UNSUPPORTED;
+  if (hasB) {
+    TIMED(a.timer, "deposit",
+          (k_deposit<T, ORDER, true><<<(unsigned)ntiles, DEP_T, 0, st>>>(
+              (const T*)a.sorted, (const T*)a.sorted_w, a.offsets, g, (T*)a.grid[0],
+              (T*)a.grid[1], (T*)a.grid[2], (T*)a.grid[3], a.stats)));
+  } else {
+    TIMED(a.timer, "deposit",
+          (k_deposit<T, ORDER, false><<<(unsigned)ntiles, DEP_T, 0, st>>>(
+              (const T*)a.sorted, (const T*)a.sorted_w, a.offsets, g, (T*)a.grid[0],
+              (T*)a.grid[1], (T*)a.grid[2], (T*)a.grid[3], a.stats)));
+  }
   return hipGetLastError() == hipSuccess ? CAPOW_OK : CAPOW_ERR_HIP;
 }
 

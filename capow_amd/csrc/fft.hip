@@ -176,12 +176,12 @@ template <typename T>
 __global__ __launch_bounds__(1024) void k_fft_rows(T* __restrict__ g0, T* __restrict__ g1,
                                                    T* __restrict__ g2, T* __restrict__ g3,
                                                    int64_t nrows, int nz, int rows_per_wg,
-                                                   FftPlan pl, const cx<T>* __restrict__ twz) {
+                                                   FftPlan pl, const cx<T>* __restrict__ twz,
+                                                   int64_t rowlen) {
   extern __shared__ unsigned char smem[];
   cx<T>* buf = reinterpret_cast<cx<T>*>(smem);
   T* gp = blockIdx.y == 0 ? g0 : blockIdx.y == 1 ? g1 : blockIdx.y == 2 ? g2 : g3;
   const int M = nz / 2;
-  const int64_t rowlen = 2 * (int64_t)(M + 1);
   const int64_t r0 = (int64_t)blockIdx.x * rows_per_wg;
   const int nr = (int)min((int64_t)rows_per_wg, nrows - r0);
   // load rows as complex pairs (x[2e], x[2e+1])
@@ -384,7 +384,7 @@ template <typename T, int N1, int N2, int B>
 __global__ __launch_bounds__((N1 * N2 / clcm(N1, N2)) * B) void k_fft_rows2(
     T* __restrict__ g0, T* __restrict__ g1, T* __restrict__ g2, T* __restrict__ g3,
     int64_t nrows, const cx<T>* __restrict__ twz,
-    int64_t nblocks_x) {
+    int64_t nblocks_x, int64_t rowlen) {
   constexpr int M = N1 * N2;
   constexpr int E = clcm(N1, N2);
   constexpr int TS = M / E;
@@ -397,7 +397,6 @@ __global__ __launch_bounds__((N1 * N2 / clcm(N1, N2)) * B) void k_fft_rows2(
   const int t = threadIdx.x % TS, c = threadIdx.x / TS;
   const int64_t row = bid * B + c;
   const bool live = row < nrows;
-  const int64_t rowlen = 2 * (int64_t)(M + 1);
   cx<T>* r = reinterpret_cast<cx<T>*>(gp + (live ? row : 0) * rowlen);
   cx<T> v[E];
 #pragma unroll
@@ -537,7 +536,7 @@ static void launch_cols2(FftArgs& a, const char* name, T** gp, int64_t astride, 
 }
 
 template <typename T, int N1, int N2>
-static void launch_rows2(FftArgs& a, T** gp, int64_t nrows, const void* twz) {
+static void launch_rows2(FftArgs& a, T** gp, int64_t nrows, const void* twz, int64_t rowlen) {
   constexpr int M = N1 * N2, E = clcm(N1, N2), TS = M / E;
   constexpr int B = rows_B<T, N1, N2>();
   const size_t lds = (size_t)B * (M + N1) * sizeof(cx<T>);
@@ -545,7 +544,7 @@ static void launch_rows2(FftArgs& a, T** gp, int64_t nrows, const void* twz) {
   dim3 grid((unsigned)nbx, a.ngrids);
   TIMED(a.timer, "fft_z",
         (k_fft_rows2<T, N1, N2, B><<<grid, TS * B, lds, a.stream>>>(
-            gp[0], gp[1], gp[2], gp[3], nrows, (const cx<T>*)twz, nbx)));
+            gp[0], gp[1], gp[2], gp[3], nrows, (const cx<T>*)twz, nbx, rowlen)));
 }
 
 #define CAPOW_FFT_SIZES(X) X(4, 4) X(4, 8) X(8, 8) X(8, 12) X(8, 16) X(8, 24) X(16, 16) \
@@ -570,12 +569,12 @@ static bool try_cols2(FftArgs& a, const char* name, T** gp, int N, int64_t astri
 }
 
 template <typename T>
-static bool try_rows2(FftArgs& a, T** gp, int M, int64_t nrows, const void* twz) {
+static bool try_rows2(FftArgs& a, T** gp, int M, int64_t nrows, const void* twz, int64_t rowlen) {
   int n1, n2;
   if (!reg_split(M, &n1, &n2)) return false;
 #define X(A, B_)                                   \
   if (n1 == A && n2 == B_) {                        \
-    launch_rows2<T, A, B_>(a, gp, nrows, twz);      \
+    launch_rows2<T, A, B_>(a, gp, nrows, twz, rowlen); \
     return true;                                    \
   }
   CAPOW_FFT_SIZES(X)
@@ -593,7 +592,8 @@ static int run_fft_t(FftArgs& a) {
   const int nzh = nz / 2 + 1;
   const int64_t nrows = (int64_t)nx * ny;
   // z pass
-  if (!(a.fast && try_rows2<T>(a, gp, nz / 2, nrows, a.tw[2]))) {
+  const int64_t rowc = g.rowc;
+  if (!(a.fast && try_rows2<T>(a, gp, nz / 2, nrows, a.tw[2], g.rowlen))) {
     const int M = nz / 2;
     int rows = max(1, (FFT_EPT * 1024) / max(M, 1));
     rows = min(rows, 16);
@@ -607,17 +607,18 @@ static int run_fft_t(FftArgs& a) {
     size_t lds = (size_t)rows * M * sizeof(cx<T>);
     TIMED(a.timer, "fft_z",
           (k_fft_rows<T><<<grid, threads, lds, st>>>(gp[0], gp[1], gp[2], gp[3], nrows, nz, rows,
-                                                     a.plan[2], (const cx<T>*)a.tw[2])));
+                                                     a.plan[2], (const cx<T>*)a.tw[2], g.rowlen)));
   }
   // y pass: planes = x index, columns = kz
-  if (!(a.fast && try_cols2<T>(a, "fft_y", gp, ny, (int64_t)nzh, (int64_t)ny * nzh,
+  if (!(a.fast && try_cols2<T>(a, "fft_y", gp, ny, rowc, (int64_t)ny * rowc,
                                (int64_t)nzh, nx, a.tw[1]))) {
     int64_t lim = 8192;  // N*B complex per workgroup (FFT_EPT * 1024)
-    TIMED_COLS(a, "fft_y", gp, ny, (int64_t)nzh, (int64_t)ny * nzh, (int64_t)nzh, nx, a.plan[1],
+    TIMED_COLS(a, "fft_y", gp, ny, rowc, (int64_t)ny * rowc, (int64_t)nzh, nx, a.plan[1],
                a.tw[1], lim);
   }
-  // x pass: one plane, columns = (y, kz) contiguous
-  const int64_t ncols = (int64_t)ny * nzh;
+  // x pass: one plane, columns = (y, kz) over the padded rows (the padding
+  // columns share the rows' last cache line and are never read as modes)
+  const int64_t ncols = (int64_t)ny * rowc;
   if (!(a.fast && try_cols2<T>(a, "fft_x", gp, nx, ncols, 0, ncols, 1, a.tw[0]))) {
     int64_t lim = 8192;
     TIMED_COLS(a, "fft_x", gp, nx, ncols, 0, ncols, 1, a.plan[0], a.tw[0], lim);

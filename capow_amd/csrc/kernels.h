@@ -78,6 +78,9 @@ struct Timer {
     }                                                               \
   } while (0)
 
+// particles per bucket-kernel block chunk (one LDS aggregation round)
+#define BK_CHUNK 1024
+
 struct DepositArgs {
   const Geom* g;
   hipStream_t stream;
@@ -128,6 +131,8 @@ enum { SPEC_AUTO_A = 0, SPEC_AUTO_B = 1, SPEC_CROSS_IM = 2, SPEC_CROSS_RE = 3 };
 struct BinArgsDev {
   int nx, ny, nzh;
   FastDiv fd_nzh;
+  int rowc;               // complex per row in device memory (>= nzh)
+  FastDiv fd_rowc;
   int nk, nmu, out2d, odd, logbins, los, interlace, hasB;
   int nspec;
   int kind[3];

@@ -12,9 +12,9 @@
 //   grid_cor = prod_axis (1/sinc)^(2 InterpOrder) (x 0.25 with interlacing),
 //   delta += delta_int * e^{i(ax+ay+az)} (per-axis phase tables),
 //   power of each requested spectrum, NM Hermitian weight, L_l(mu).
-// Accumulation: every wavefront owns a private copy of the bins in LDS when
-// they fit (adds in program order => bit-reproducible), else the workgroup
-// shares one copy; workgroup partials are then summed in a fixed order.
+// Accumulation: every wavefront owns a private copy of the bins in LDS (adds
+// in program order => bit-reproducible); workgroup partials are then summed
+// in a fixed order.
 #include "common.h"
 #include "kernels.h"
 
@@ -43,21 +43,46 @@ __device__ __forceinline__ int kbin_of(double fdk, const BinArgsDev& a) {
   return (int)tq;
 }
 
-// Per-mode binning: frequencies, |k|, mu, bin, NM, deconvolution, interlace
-// phase, power of each spectrum, Legendre weights, accumulation.
+// Per-row quantities of one (i, j) row of modes: frequencies, fx^2+fy^2,
+// the x*y deconvolution and interlace phase products.
+struct RowVals {
+  int i, j;
+  double fx, fy, fxy, corxy, qr, qi;
+};
+
+__device__ __forceinline__ RowVals row_vals(int i, int j, const double* __restrict__ tabs,
+                                            const BinArgsDev& a) {
+  RowVals R;
+  R.i = i;
+  R.j = j;
+  R.fx = tabs[a.ofx + i];
+  R.fy = tabs[a.ofy + j];
+  R.fxy = dadd(dmul(R.fx, R.fx), dmul(R.fy, R.fy));
+  R.corxy = tabs[a.ocx + i] * tabs[a.ocy + j];
+  R.qr = 1.0;
+  R.qi = 0.0;
+  if (a.interlace) {
+    const double pxr = tabs[a.opx + 2 * i], pxi = tabs[a.opx + 2 * i + 1];
+    const double pyr = tabs[a.opy + 2 * j], pyi = tabs[a.opy + 2 * j + 1];
+    R.qr = pxr * pyr - pxi * pyi;
+    R.qi = pxr * pyi + pxi * pyr;
+  }
+  return R;
+}
+
+// Per-mode binning: |k|, mu, bin, NM, deconvolution, interlace phase, power
+// of each spectrum, Legendre weights, accumulation into this wave's bins.
 template <typename T>
-__device__ __forceinline__ void bin_one(int i, uint32_t r, typename CT<T>::t va,
-                                        typename CT<T>::t vai, typename CT<T>::t vb,
-                                        typename CT<T>::t vbi, const BinArgsDev& a,
-                                        double* accS, double* acc2, double* cnt, double* cnt2,
-                                        int nl, int n2d) {
+__device__ __forceinline__ void bin_mode(const RowVals& R, int k, double fz, double corz,
+                                         double pzr, double pzi, typename CT<T>::t va,
+                                         typename CT<T>::t vai, typename CT<T>::t vb,
+                                         typename CT<T>::t vbi, const BinArgsDev& a,
+                                         double* accS, double* acc2, double* cnt, double* cnt2,
+                                         int nl, int n2d) {
   const int NK = a.nk;
   const double twopi = 2.0 * 3.14159265358979323846;  // the reference's 2.0*M_PI
-  const uint32_t j = a.fd_rowc.div(r);
-  const int k = (int)(r - j * (uint32_t)a.rowc);
-  if (k >= a.nzh) return;   // row padding
-  const double fx = a.fx[i], fy = a.fy[j], fz = a.fz[k];
-  const double fktot = dmul(twopi, sqrt(dadd(dadd(dmul(fx, fx), dmul(fy, fy)), dmul(fz, fz))));
+  const int i = R.i, j = R.j;
+  const double fktot = dmul(twopi, sqrt(dadd(R.fxy, dmul(fz, fz))));
   int kbin;
   if (a.logbins) {
     if (!(fktot > 0)) return;
@@ -68,7 +93,7 @@ __device__ __forceinline__ void bin_one(int i, uint32_t r, typename CT<T>::t va,
   if (!((kbin >= 0) && (kbin < NK) && (fktot <= a.min_nyq))) return;
   double mu = 0.0;
   if (fktot > 0) {
-    const double fl = a.los == 1 ? fx : (a.los == 2 ? fy : fz);
+    const double fl = a.los == 1 ? R.fx : (a.los == 2 ? R.fy : fz);
     if (a.out2d) {
       mu = ddiv(dmul(twopi, fl), fktot);   // exact: feeds the mu-bin index
     } else {
@@ -87,14 +112,12 @@ __device__ __forceinline__ void bin_one(int i, uint32_t r, typename CT<T>::t va,
       if (i != a.nx / 2) NM = 1.0;
     }
   }
-  const double cor = a.corx[i] * a.cory[j] * a.corz[k];  // x0.25 folded in corz
+  const double cor = R.corxy * corz;  // x0.25 folded in corz
   double ar = (double)va.x, ai = (double)va.y, br = 0, bi = 0;
   double cr = 1, ci = 0;
   if (a.interlace) {
-    const double2 px = a.phx[i], py = a.phy[j], pz = a.phz[k];
-    const double qr = px.x * py.x - px.y * py.y, qi = px.x * py.y + px.y * py.x;
-    cr = qr * pz.x - qi * pz.y;
-    ci = qr * pz.y + qi * pz.x;
+    cr = R.qr * pzr - R.qi * pzi;
+    ci = R.qr * pzi + R.qi * pzr;
     ar += (double)vai.x * cr - (double)vai.y * ci;
     ai += (double)vai.x * ci + (double)vai.y * cr;
   }
@@ -143,12 +166,31 @@ __device__ __forceinline__ void bin_one(int i, uint32_t r, typename CT<T>::t va,
   if (in2d) atomicAdd(&cnt2[b2], NM);
 }
 
-#define BIN_U 4   // consecutive modes per lane per iteration (loads issued together)
-
+// Flat-index form: r is the offset within plane i of the padded layout.
 template <typename T>
-__global__ __launch_bounds__(256) void k_bin(const T* __restrict__ gA, const T* __restrict__ gAi,
+__device__ __forceinline__ void bin_one(int i, uint32_t r, typename CT<T>::t va,
+                                        typename CT<T>::t vai, typename CT<T>::t vb,
+                                        typename CT<T>::t vbi, const double* __restrict__ tabs,
+                                        const BinArgsDev& a, double* accS, double* acc2,
+                                        double* cnt, double* cnt2, int nl, int n2d) {
+  const uint32_t j = a.fd_rowc.div(r);
+  const int k = (int)(r - j * (uint32_t)a.rowc);
+  if (k >= a.nzh) return;   // row padding
+  const RowVals R = row_vals(i, (int)j, tabs, a);
+  bin_mode<T>(R, k, tabs[a.ofz + k], tabs[a.ocz + k], tabs[a.opz + 2 * k], tabs[a.opz + 2 * k + 1],
+              va, vai, vb, vbi, a, accS, acc2, cnt, cnt2, nl, n2d);
+}
+
+// One workgroup = `copies` wavefronts, each with a private copy of the bins
+// in LDS (adds in program order => bit-reproducible).  A lane handles U
+// consecutive modes per iteration; the next iteration's loads are issued
+// before the current modes are binned, so every wave keeps two groups of
+// loads in flight (the LDS copies leave few waves per CU to hide latency).
+template <typename T, int U>
+__global__ __launch_bounds__(512) void k_bin(const T* __restrict__ gA, const T* __restrict__ gAi,
                                              const T* __restrict__ gB, const T* __restrict__ gBi,
-                                             BinArgsDev a, double* __restrict__ partials) {
+                                             const double* __restrict__ tabs, BinArgsDev a,
+                                             double* __restrict__ partials) {
   typedef typename CT<T>::t C;
   extern __shared__ double sm[];
   const int NK = a.nk;
@@ -157,7 +199,7 @@ __global__ __launch_bounds__(256) void k_bin(const T* __restrict__ gA, const T* 
   const int n2d = a.out2d ? a.nk * a.nmu : 0;
   const int per_copy = F * NK + a.nspec * n2d + NK + n2d;  // doubles (counts as double)
   const int wave = threadIdx.x >> 6;
-  double* my = sm + (wave % a.copies) * per_copy;
+  double* my = sm + wave * per_copy;
   for (int q = threadIdx.x; q < per_copy * a.copies; q += blockDim.x) sm[q] = 0.0;
   __syncthreads();
   double* accS = my;                       // [s][l'][NK]
@@ -172,26 +214,46 @@ __global__ __launch_bounds__(256) void k_bin(const T* __restrict__ gA, const T* 
   const C* B = reinterpret_cast<const C*>(gB);
   const C* Bi = reinterpret_cast<const C*>(gBi);
   const C zero = {T(0), T(0)};
-  // lane handles BIN_U consecutive modes; groups are strided over the grid
-  const int64_t gstride = (int64_t)gridDim.x * blockDim.x * BIN_U;
-  for (int64_t m0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * BIN_U; m0 < total;
-       m0 += gstride) {
-    C va[BIN_U], vai[BIN_U], vb[BIN_U], vbi[BIN_U];
+  const bool il = a.interlace != 0, hb = a.hasB != 0;
+  // lane handles U consecutive modes; groups are strided over the grid
+  const int64_t gstride = (int64_t)gridDim.x * blockDim.x * U;
+  int64_t m0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * U;
+  C va[U], vai[U], vb[U], vbi[U];
 #pragma unroll
-    for (int u = 0; u < BIN_U; u++) {
-      const int64_t m = m0 + u;
+  for (int u = 0; u < U; u++) {
+    const int64_t m = m0 + u;
+    const bool ok = m < total;
+    va[u] = ok ? A[m] : zero;
+    vai[u] = (ok && il) ? Ai[m] : zero;
+    vb[u] = (ok && hb) ? B[m] : zero;
+    vbi[u] = (ok && hb && il) ? Bi[m] : zero;
+  }
+  for (; m0 < total; m0 += gstride) {
+    // prefetch the next group
+    C na[U], nai[U], nb[U], nbi[U];
+    const int64_t m1 = m0 + gstride;
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int64_t m = m1 + u;
       const bool ok = m < total;
-      va[u] = ok ? A[m] : zero;
-      vai[u] = (ok && a.interlace) ? Ai[m] : zero;
-      vb[u] = (ok && a.hasB) ? B[m] : zero;
-      vbi[u] = (ok && a.hasB && a.interlace) ? Bi[m] : zero;
+      na[u] = ok ? A[m] : zero;
+      nai[u] = (ok && il) ? Ai[m] : zero;
+      nb[u] = (ok && hb) ? B[m] : zero;
+      nbi[u] = (ok && hb && il) ? Bi[m] : zero;
     }
     int i = (int)(m0 / plane);
     uint32_t r = (uint32_t)(m0 - (int64_t)i * plane);
 #pragma unroll
-    for (int u = 0; u < BIN_U; u++) {
-      if (m0 + u < total) bin_one<T>(i, r, va[u], vai[u], vb[u], vbi[u], a, accS, acc2, cnt, cnt2, nl, n2d);
+    for (int u = 0; u < U; u++) {
+      if (m0 + u < total) bin_one<T>(i, r, va[u], vai[u], vb[u], vbi[u], tabs, a, accS, acc2, cnt, cnt2, nl, n2d);
       if (++r == plane) { r = 0; i++; }
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      va[u] = na[u];
+      vai[u] = nai[u];
+      vb[u] = nb[u];
+      vbi[u] = nbi[u];
     }
   }
   __syncthreads();
@@ -200,6 +262,125 @@ __global__ __launch_bounds__(256) void k_bin(const T* __restrict__ gA, const T* 
   for (int q = threadIdx.x; q < per_copy; q += blockDim.x) {
     double t = 0.0;
     for (int c = 0; c < a.copies; c++) t += sm[c * per_copy + q];
+    out[q] = t;
+  }
+}
+
+// Row form.  A wave takes batches of 64 consecutive (i, j) rows; for each
+// row the lanes cover kz = lane + 64u, u < NUF (coalesced loads), with the
+// row's x/y quantities wave-uniform and the z tables in LDS.  The rest of
+// each row, kz in [64 NUF, NZ/2+1), is binned once per batch with lane l on
+// row l of the batch.  PREF: the next row's loads are issued before the
+// current row is binned.
+template <typename T, int NUF, bool PREF>
+__global__ __launch_bounds__(512) void k_bin_rows(const T* __restrict__ gA,
+                                                  const T* __restrict__ gAi,
+                                                  const T* __restrict__ gB,
+                                                  const T* __restrict__ gBi,
+                                                  const double* __restrict__ tabs, BinArgsDev a,
+                                                  double* __restrict__ partials) {
+  typedef typename CT<T>::t C;
+  extern __shared__ double sm[];
+  const int NK = a.nk;
+  const int nl = a.odd ? 5 : 3;
+  const int F = a.nspec * nl;
+  const int n2d = a.out2d ? a.nk * a.nmu : 0;
+  const int per_copy = F * NK + a.nspec * n2d + NK + n2d;
+  const int copies = a.copies;
+  const int nzh = a.nzh;
+  double* zt = sm + copies * per_copy;   // [nzh][4]: fz, corz, phz.re, phz.im
+  for (int q = threadIdx.x; q < per_copy * copies; q += blockDim.x) sm[q] = 0.0;
+  for (int k = threadIdx.x; k < nzh; k += blockDim.x) {
+    zt[4 * k] = tabs[a.ofz + k];
+    zt[4 * k + 1] = tabs[a.ocz + k];
+    zt[4 * k + 2] = tabs[a.opz + 2 * k];
+    zt[4 * k + 3] = tabs[a.opz + 2 * k + 1];
+  }
+  __syncthreads();
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63;
+  double* my = sm + wave * per_copy;
+  double* accS = my;
+  double* acc2 = my + F * NK;
+  double* cnt = acc2 + a.nspec * n2d;
+  double* cnt2 = cnt + NK;
+  const C* A = reinterpret_cast<const C*>(gA);
+  const C* Ai = reinterpret_cast<const C*>(gAi);
+  const C* B = reinterpret_cast<const C*>(gB);
+  const C* Bi = reinterpret_cast<const C*>(gBi);
+  const C zero = {T(0), T(0)};
+  const bool il = a.interlace != 0, hb = a.hasB != 0;
+  const int64_t nrows = (int64_t)a.nx * a.ny;
+  const int64_t nbatch = (nrows + 63) / 64;
+  const int64_t rowc = a.rowc;
+  const int rem0 = 64 * NUF;
+  const int nw = gridDim.x * copies;
+  for (int64_t bt = (int64_t)blockIdx.x * copies + wave; bt < nbatch; bt += nw) {
+    const int64_t r0 = bt * 64;
+    const int nr = (int)min((int64_t)64, nrows - r0);
+    if constexpr (NUF > 0) {
+      C va[NUF], vai[NUF], vb[NUF], vbi[NUF];
+      auto load = [&](int64_t row, C* xa, C* xai, C* xb, C* xbi) {
+        const int64_t base = row * rowc + lane;
+#pragma unroll
+        for (int u = 0; u < NUF; u++) {
+          const int64_t m = base + 64 * u;
+          xa[u] = A[m];
+          xai[u] = il ? Ai[m] : zero;
+          xb[u] = hb ? B[m] : zero;
+          xbi[u] = (hb && il) ? Bi[m] : zero;
+        }
+      };
+      load(r0, va, vai, vb, vbi);
+      for (int rr = 0; rr < nr; rr++) {
+        const int64_t row = r0 + rr;
+        C na[NUF], nai[NUF], nb[NUF], nbi[NUF];
+        if (PREF && rr + 1 < nr) load(row + 1, na, nai, nb, nbi);
+        const int i = (int)(row / a.ny), j = (int)(row - (int64_t)i * a.ny);
+        const RowVals R = row_vals(i, j, tabs, a);
+#pragma unroll
+        for (int u = 0; u < NUF; u++) {
+          const int k = lane + 64 * u;
+          const double4 z = *reinterpret_cast<const double4*>(zt + 4 * k);
+          bin_mode<T>(R, k, z.x, z.y, z.z, z.w, va[u], vai[u], vb[u], vbi[u], a, accS, acc2, cnt,
+                      cnt2, nl, n2d);
+        }
+        if (rr + 1 < nr) {
+          if (PREF) {
+#pragma unroll
+            for (int u = 0; u < NUF; u++) {
+              va[u] = na[u];
+              vai[u] = nai[u];
+              vb[u] = nb[u];
+              vbi[u] = nbi[u];
+            }
+          } else {
+            load(row + 1, va, vai, vb, vbi);
+          }
+        }
+      }
+    }
+    // the rest of each row: lane l on row r0 + l
+    if (lane < nr) {
+      const int64_t row = r0 + lane;
+      const int i = (int)(row / a.ny), j = (int)(row - (int64_t)i * a.ny);
+      const RowVals R = row_vals(i, j, tabs, a);
+      for (int k = rem0; k < nzh; k++) {
+        const int64_t m = row * rowc + k;
+        const C xa = A[m];
+        const C xai = il ? Ai[m] : zero;
+        const C xb = hb ? B[m] : zero;
+        const C xbi = (hb && il) ? Bi[m] : zero;
+        const double4 z = *reinterpret_cast<const double4*>(zt + 4 * k);
+        bin_mode<T>(R, k, z.x, z.y, z.z, z.w, xa, xai, xb, xbi, a, accS, acc2, cnt, cnt2, nl, n2d);
+      }
+    }
+  }
+  __syncthreads();
+  double* out = partials + (int64_t)blockIdx.x * per_copy;
+  for (int q = threadIdx.x; q < per_copy; q += blockDim.x) {
+    double t = 0.0;
+    for (int c = 0; c < copies; c++) t += sm[c * per_copy + q];
     out[q] = t;
   }
 }
@@ -214,16 +395,42 @@ __global__ void k_bin_final(const double* __restrict__ partials, int nwg, int pe
   out[q] = t;
 }
 
+template <typename T, int NUF>
+static void launch_rows(BinLaunch& L, size_t lds) {
+  // FP64 rows hold at most 8 slots in registers (16 would spill)
+  constexpr int NU = (sizeof(T) == 8 && NUF > 8) ? 8 : NUF;
+  constexpr bool PREF = sizeof(T) == 4 && NU <= 8;
+  const void* fn = (const void*)k_bin_rows<T, NU, PREF>;
+  if (lds > 64 * 1024) hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  TIMED(L.timer, "bin",
+        (k_bin_rows<T, NU, PREF><<<L.nwg, 64 * L.dev.copies, lds, L.stream>>>(
+            (const T*)L.grid[0], (const T*)L.grid[1], (const T*)L.grid[2], (const T*)L.grid[3],
+            L.tabs, L.dev, L.partials)));
+}
+
 template <typename T>
 static int run_bin_t(BinLaunch& L) {
   const int per_copy = L.per_copy;
-  const size_t lds = (size_t)per_copy * L.dev.copies * sizeof(double);
-  if (lds > 64 * 1024)
-    hipFuncSetAttribute((const void*)k_bin<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  TIMED(L.timer, "bin",
-        (k_bin<T><<<L.nwg, 256, lds, L.stream>>>((const T*)L.grid[0], (const T*)L.grid[1],
-                                                 (const T*)L.grid[2], (const T*)L.grid[3], L.dev,
-                                                 L.partials)));
+  const size_t bins = (size_t)per_copy * L.dev.copies * sizeof(double);
+  const size_t lds = bins + (size_t)L.dev.nzh * 4 * sizeof(double);
+  const int nuf = L.dev.nzh / 64;
+  if (lds <= 160 * 1024 && L.row_form) {
+    // the largest instantiated full-slot count not above nzh / 64
+    if (nuf >= 16 && sizeof(T) == 4) launch_rows<T, 16>(L, lds);
+    else if (nuf >= 8) launch_rows<T, 8>(L, lds);
+    else if (nuf >= 4) launch_rows<T, 4>(L, lds);
+    else if (nuf >= 2) launch_rows<T, 2>(L, lds);
+    else if (nuf >= 1) launch_rows<T, 1>(L, lds);
+    else launch_rows<T, 0>(L, lds);
+  } else {
+    constexpr int U = sizeof(T) == 4 ? 8 : 4;
+    if (bins > 64 * 1024)
+      hipFuncSetAttribute((const void*)k_bin<T, U>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bins);
+    TIMED(L.timer, "bin",
+          (k_bin<T, U><<<L.nwg, 64 * L.dev.copies, bins, L.stream>>>(
+              (const T*)L.grid[0], (const T*)L.grid[1], (const T*)L.grid[2], (const T*)L.grid[3],
+              L.tabs, L.dev, L.partials)));
+  }
   TIMED(L.timer, "bin_final",
         (k_bin_final<<<(per_copy + 255) / 256, 256, 0, L.stream>>>(L.partials, L.nwg, per_copy,
                                                                     L.out)));

@@ -56,6 +56,7 @@ struct capow_ctx {
   double* bin_out = nullptr;
   int bin_nwg = 1024;
   int bin_per_copy = 0;
+  bool bin_row_form = true;
   int nspec = 1;
   int codes[3] = {0, 0, 0};
   capow_assign_stats last{};
@@ -231,6 +232,13 @@ static int setup(capow_ctx* c) {
     g.ntile[a] = g.n[a] / t;
   }
   g.nbuckets = (int64_t)g.ntile[0] * g.ntile[1] * g.ntile[2];
+  // LDS hash probes per key in the counting sort; CAPOW_SORT_PROBES=0 sends
+  // every copy through the global counters (the table-full path)
+  g.sort_probes = 32;
+  if (const char* e = getenv("CAPOW_SORT_PROBES")) {
+    const int v = atoi(e);
+    g.sort_probes = v < 0 ? 0 : (v > 32 ? 32 : v);
+  }
   c->fp64 = p.precision == CAPOW_FP64;
   c->esz = c->fp64 ? 8 : 4;
 
@@ -325,11 +333,9 @@ static int setup(capow_ctx* c) {
     if (nyy < mn) mn = nyy;
     if (nyz < mn) mn = nyz;
     b.min_nyq = mn;
-    b.fx = c->bin_tabs + ofx; b.fy = c->bin_tabs + ofy; b.fz = c->bin_tabs + ofz;
-    b.corx = c->bin_tabs + ocx; b.cory = c->bin_tabs + ocy; b.corz = c->bin_tabs + ocz;
-    b.phx = (const double2*)(c->bin_tabs + opx);
-    b.phy = (const double2*)(c->bin_tabs + opy);
-    b.phz = (const double2*)(c->bin_tabs + opz);
+    b.ofx = (int64_t)ofx; b.ofy = (int64_t)ofy; b.ofz = (int64_t)ofz;
+    b.ocx = (int64_t)ocx; b.ocy = (int64_t)ocy; b.ocz = (int64_t)ocz;
+    b.opx = (int64_t)opx; b.opy = (int64_t)opy; b.opz = (int64_t)opz;
     int32_t ns, codes[3];
     capow_spectra(&p, &ns, codes);
     c->nspec = ns;
@@ -345,10 +351,22 @@ static int setup(capow_ctx* c) {
     const size_t lds_max = 160 * 1024;
     if (bytes > lds_max)
       return fail(c, CAPOW_ERR_UNSUPPORTED, "NK/NMU too large for the LDS bin arrays");
-    b.copies = bytes * 4 <= lds_max ? 4 : (bytes * 2 <= lds_max ? 2 : 1);
-    int64_t modes = (int64_t)NX * NY * nzh;
-    int64_t want = (modes + 1023) / 1024;   // 256 lanes x 4 modes per iteration
-    c->bin_nwg = (int)(want < 1024 ? want : 1024);
+    // one wavefront per private bin copy, as many as fit in one CU's LDS
+    // beside the z tables (up to 8); a persistent grid of one workgroup per CU
+    const size_t ztab = (size_t)nzh * 4 * 8;
+    c->bin_row_form = bytes + ztab <= lds_max;
+    int copies = (int)((lds_max - (c->bin_row_form ? ztab : 0)) / bytes);
+    b.copies = copies > 8 ? 8 : copies;
+    int dev = 0, ncu = 256;
+    if (hipGetDevice(&dev) == hipSuccess)
+      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    if (ncu < 1) ncu = 256;
+    const int64_t modes = (int64_t)NX * NY * g.rowc;
+    const int64_t per_wg = (int64_t)64 * b.copies * (c->fp64 ? 4 : 8);   // modes per iteration
+    const int64_t nbatch = ((int64_t)NX * NY + 63) / 64;                   // row form: 64-row batches
+    const int64_t want = c->bin_row_form ? (nbatch + b.copies - 1) / b.copies
+                                         : (modes + per_wg - 1) / per_wg;
+    c->bin_nwg = (int)(want < ncu ? want : ncu);
     HIPCHK(c, hipMalloc((void**)&c->bin_partials, (size_t)c->bin_nwg * bytes));
     HIPCHK(c, hipMalloc((void**)&c->bin_out, bytes));
   }
@@ -564,6 +582,8 @@ extern "C" int capow_bin(capow_ctx* c, capow_bins* out) {
   L.timer = &c->timer;
   for (int s = 0; s < 4; s++) L.grid[s] = c->grid[s] ? c->grid[s] : c->grid[0];
   L.dev = c->bin;
+  L.tabs = c->bin_tabs;
+  L.row_form = c->bin_row_form;
   L.nwg = c->bin_nwg;
   L.per_copy = c->bin_per_copy;
   L.partials = c->bin_partials;

@@ -53,6 +53,7 @@ struct Geom {
   int ntile[3];          // tiles per axis
   int reach_lo, reach_hi;// taps of a particle lie in [key+reach_lo, key+reach_hi]
   int64_t nbuckets;      // ntiles (one bucket per deposit tile)
+  int sort_probes;       // LDS hash probes per key in the sort (0: always global)
 };
 
 // ---------------------------------------------------------------------------

@@ -197,27 +197,26 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
 // BK_CHUNK consecutive particles; every copy (particle, reached tile) gets a
 // key -- the tile in the count pass, 2*tile + ghost in the scatter pass --
 // and is counted in an LDS hash table (open addressing, linear probing).
-// Lanes of a wavefront that share a key are merged first (the leader adds
-// the group size), so a slot sees one LDS atomic per group.  At the end of
-// the chunk each occupied slot issues ONE global atomic for all its copies:
+// Runs of lanes with the same key are merged first (the run head adds the
+// run length), so a slot sees one LDS atomic per run.  At the end of the
+// chunk each occupied slot issues ONE global atomic for all its copies:
 // with catalogues stored in spatial order (simulation snapshots, lattices)
-// a chunk reaches a few dozen tiles, so the device-scope atomics drop from
-// one per wavefront group to one per (chunk, key).  A key that finds no
-// free slot within BK_PROBE probes falls back to its own global atomic.
+// a chunk reaches a few dozen tiles, so the device-scope atomics drop to one
+// per (chunk, key).  A key that finds no free slot within BK_PROBE probes
+// falls back to the global counter.
 #define BK_T 256
 #define BK_K 4
 #define BK_HS_BITS 12
 #define BK_HS (1 << BK_HS_BITS)
 #define BK_PROBE 32
 #define BK_EMPTY 0xffffffffu
-#define BK_AGG 8
 static_assert(BK_T * BK_K == BK_CHUNK, "chunk size");
 static_assert(BK_T * BK_K <= (1 << (32 - BK_HS_BITS - 1)), "rank field");
 
-__device__ __forceinline__ int hs_find(uint32_t* __restrict__ hkey, uint32_t key) {
+__device__ __forceinline__ int hs_find(uint32_t* __restrict__ hkey, uint32_t key, int probes) {
   uint32_t h = (key * 2654435761u) >> (32 - BK_HS_BITS);
 #pragma unroll 1
-  for (int p = 0; p < BK_PROBE; p++) {
+  for (int p = 0; p < probes; p++) {
     const uint32_t cur = hkey[h];
     if (cur == key) return (int)h;
     if (cur == BK_EMPTY) {
@@ -229,49 +228,54 @@ __device__ __forceinline__ int hs_find(uint32_t* __restrict__ hkey, uint32_t key
   return -1;
 }
 
-// Counts one copy of `key` for every lane with has = true.  Returns the slot
-// (-1: table full, the caller takes the global path) and, for RET, the
-// lane's rank among the slot's copies in this chunk.
+// Counts one copy of `key` for every lane with has = true.  Lanes are
+// merged into runs: a run is a maximal block of consecutive lanes with has
+// and the same key (catalogues stored in spatial order give long runs).
+// Only the run heads touch the LDS table, each adding its run length; a
+// head whose key finds no free slot adds the run to the global counter
+// `gctr` (the key's own counter) instead.  For RET the other lanes take the
+// head's slot and base with one permute and add their offset in the run.
+// Returns the slot, or -1 when the run went to gctr (for RET, `rank` is
+// then the lane's rank among gctr's copies); rank: the lane's rank among
+// the slot's copies in this chunk.
 template <bool RET>
 __device__ __forceinline__ int hs_add(uint32_t* __restrict__ hkey, uint32_t* __restrict__ hcnt,
-                                      uint32_t key, bool has, uint32_t& rank) {
+                                      uint32_t key, bool has, uint32_t* gctr, int probes,
+                                      uint32_t& rank) {
   const int lane = __lane_id();
-  const unsigned long long lt = (1ull << lane) - 1ull;
-  unsigned long long pending = __ballot(has);
+  const uint32_t pkey = (uint32_t)__shfl_up((int)key, 1);
+  const unsigned long long hm = __ballot(has);
+  const bool phas = lane > 0 && ((hm >> (lane - 1)) & 1ull);
+  const bool head = has && !(phas && pkey == key);
+  const unsigned long long heads = __ballot(head);
+  // run end: the next lane that starts a run or has no copy
+  const unsigned long long stops = heads | ~hm;
   int slot = -1;
+  uint32_t base = 0;
+  if (head) {
+    const unsigned long long after = lane == 63 ? 0ull : (stops >> (lane + 1));
+    const int len = after ? __ffsll((long long)after) : 64 - lane;
+    slot = hs_find(hkey, key, probes);
+    if (slot >= 0) {
+      if (RET) base = atomicAdd(&hcnt[slot], (uint32_t)len);
+      else atomicAdd(&hcnt[slot], (uint32_t)len);
+    } else {
+      if (RET) base = atomicAdd(gctr, (uint32_t)len);
+      else atomicAdd(gctr, (uint32_t)len);
+    }
+  }
   rank = 0;
-#pragma unroll 1
-  for (int it = 0; pending; it++) {
-    if (it == BK_AGG) {
-      // many distinct keys in this wavefront: one LDS insert per lane
-      if ((pending >> lane) & 1ull) {
-        slot = hs_find(hkey, key);
-        if (slot >= 0) {
-          if (RET) rank = atomicAdd(&hcnt[slot], 1u);
-          else atomicAdd(&hcnt[slot], 1u);
-        }
-      }
-      break;
+  if (RET) {
+    // the lane's head: the highest head at or below it
+    const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+    const unsigned long long below = heads & upto;
+    const int h = below ? 63 - __clzll((long long)below) : lane;
+    const int hs = __shfl(slot, h);
+    const uint32_t hb = (uint32_t)__shfl((int)base, h);
+    if (has) {
+      slot = hs;
+      rank = hb + (uint32_t)(lane - h);
     }
-    const int leader = __ffsll((long long)pending) - 1;
-    const uint32_t lk = (uint32_t)__builtin_amdgcn_readlane((int)key, leader);
-    const unsigned long long m = __ballot(has && key == lk) & pending;
-    int s = 0;
-    uint32_t base = 0;
-    if (lane == leader) {
-      s = hs_find(hkey, lk);
-      if (s >= 0) {
-        if (RET) base = atomicAdd(&hcnt[s], (uint32_t)__popcll(m));
-        else atomicAdd(&hcnt[s], (uint32_t)__popcll(m));
-      }
-    }
-    s = __builtin_amdgcn_readlane(s, leader);
-    if (RET) base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
-    if ((m >> lane) & 1ull) {
-      slot = s;
-      if (RET) rank = base + (uint32_t)__popcll(m & lt);
-    }
-    pending &= ~m;
   }
   return slot;
 }
@@ -333,20 +337,19 @@ __global__ __launch_bounds__(BK_T) void k_bucket(const T* __restrict__ parts,
         // from the back, so deposit waves are mostly homogeneous
         const uint32_t key = SCATTER ? 2u * (uint32_t)tile + (q ? 1u : 0u) : (uint32_t)tile;
         uint32_t rank;
-        const int slot = hs_add<SCATTER>(hkey, hcnt, key, has, rank);
-        if (has && slot < 0) {
-          if (SCATTER) {
-            const uint32_t r = atomicAdd(&cursor[(q ? nb : 0u) + (uint32_t)tile], 1u);
-            const uint32_t pos = q ? off[tile + 1] - 1u - r : off[tile] + r;
+        uint32_t* gctr = SCATTER ? cursor + (q ? nb : 0u) + (uint32_t)tile : counts + tile;
+        const int slot = hs_add<SCATTER>(hkey, hcnt, key, has, gctr, g.sort_probes, rank);
+        if (SCATTER && has) {
+          if (slot < 0) {
+            // table full: the run took its ranks from the global cursor
+            const uint32_t pos = q ? off[tile + 1] - 1u - rank : off[tile] + rank;
             if ((int64_t)pos < cap) {
               store4<T>(sorted, pos, c.x, c.y, c.z, v);
               if (g.has_w) sorted_w[pos] = w;
             }
           } else {
-            atomicAdd(&counts[tile], 1u);
+            pr[k][q] = (uint32_t)slot | (rank << BK_HS_BITS);
           }
-        } else if (has) {
-          pr[k][q] = (uint32_t)slot | (rank << BK_HS_BITS);
         }
       }
       if (!SCATTER && valid) {

@@ -138,9 +138,9 @@ struct BinArgsDev {
   int kind[3];
   int copies;
   double mink, binwidth, inv_binwidth, min_nyq;
-  const double *fx, *fy, *fz;
-  const double *corx, *cory, *corz;
-  const double2 *phx, *phy, *phz;
+  // offsets (doubles) of the per-axis tables in the table buffer:
+  // frequencies f, deconvolution cor, interlace phases ph (re, im pairs)
+  int64_t ofx, ofy, ofz, ocx, ocy, ocz, opx, opy, opz;
 };
 
 struct BinLaunch {
@@ -148,6 +148,8 @@ struct BinLaunch {
   Timer* timer;
   const void* grid[4];    // A, A_int, B, B_int (unused -> any valid pointer)
   BinArgsDev dev;
+  const double* tabs;     // per-axis tables (offsets in dev)
+  bool row_form;          // row-walking kernel (z tables in LDS)
   int nwg;
   int per_copy;
   double* partials;       // nwg * per_copy

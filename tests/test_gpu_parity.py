@@ -329,3 +329,43 @@ def test_determinism():
     for code in outs[0][1]["codes"]:
         assert np.array_equal(outs[0][1][code]["pk"].view(np.uint64),
                               outs[1][1][code]["pk"].view(np.uint64))
+
+
+def _grids_and_bins(p, xyzv):
+    ps = api.PowerSpectrum(p)
+    ps.set_particles(xyzv)
+    ps.assign()
+    n = p.nz
+    grids = [ps.get_grid(s).reshape(p.nx, p.ny, -1)[:, :, :n].copy() for s in range(4)]
+    ps.fft()
+    res = ps.bin()
+    ps.close()
+    return grids, res
+
+
+@pytest.mark.parametrize("prec", [api.FP32, api.FP64])
+def test_sort_order_and_table_overflow_do_not_change_grids(prec, monkeypatch):
+    """The deposit sums exact integers, so the grids and bins do not depend
+    on the catalogue order or on which path the counting sort takes: the
+    LDS-aggregated path, a shuffled catalogue (no runs of equal keys) and
+    the global-counter path (CAPOW_SORT_PROBES=0, as when the table is full)
+    give bitwise identical results.  Grid 96^3 spans several tiles per axis
+    (ghost copies on every face)."""
+    say("sort paths", prec)
+    n = 96
+    p = api.make_params(nx=n, ny=n, nz=n, interp_order=3, do_interlacing=1, momentum=1,
+                        extra_spectra=3, precision=prec, nk=20, maxk=0.2)
+    pos, vel = mocks.zeldovich(64, L, 11)
+    xyzv = mocks.xyzv(pos, vel, 0)
+    perm = np.random.default_rng(3).permutation(len(xyzv))
+    base = _grids_and_bins(p, xyzv)
+    shuffled = _grids_and_bins(p, xyzv[perm])
+    monkeypatch.setenv("CAPOW_SORT_PROBES", "0")
+    forced = _grids_and_bins(p, xyzv)
+    monkeypatch.delenv("CAPOW_SORT_PROBES")
+    u = np.uint32 if prec == api.FP32 else np.uint64
+    for other in (shuffled, forced):
+        for s in range(4):
+            assert np.array_equal(base[0][s].view(u), other[0][s].view(u)), s
+        for code in base[1]["codes"]:
+            assert np.array_equal(base[1][code]["pk"], other[1][code]["pk"]), code

@@ -18,6 +18,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -28,6 +29,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "ms per P_0,2,4(k) at 1024³ TSC+interlace, FP32 vs FP64 GPU; mocks/s batched"
 HBM_PEAK_GBS = 8000.0
+COPY_BYTES = 1 << 30
+TRAFFIC_FILE = "c2_fp32_traffic.md"
 
 
 def log(*a):
@@ -63,30 +66,69 @@ def run_form(prec, parts, n_p, args, geom):
     for _ in range(args.warmup):
         res = ps.power()
     torch.cuda.synchronize()
+    steps = args.steps if prec == api.FP32 else max(1, args.fp64_steps)
+    # timed region: per-kernel HIP events on the context's stream, recorded
+    # around every launch of the K steps (the step time is the host clock
+    # around the K steps, which includes the event records)
     ps.set_timing(True)
     ps.reset_kernel_times()
-    # timing run with per-kernel HIP events (not the timed region)
-    res = ps.power()
-    ktimes = ps.kernel_times()
-    ps.set_timing(False)
     torch.cuda.synchronize()
-    steps = args.steps if prec == api.FP32 else max(1, args.fp64_steps)
     t0 = time.perf_counter()
     for _ in range(steps):
         res = ps.power()
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / steps
+    ktimes = ps.kernel_times()
+    ps.set_timing(False)
     out = dict(ms=ms, kernels=ktimes, res=res, stats=ps.stats, params=p, steps=steps)
     ps.close()
     return out
 
 
+def profile_pass(parts, geom):
+    """One FP32 step of the workload, then a hipMemcpy of COPY_BYTES (the
+    counter calibration), for rocprofv3 --pmc runs."""
+    import torch
+    from capow_amd import api
+    nx = geom["n"]
+    p = api.make_params(nx=nx, ny=nx, nz=nx, xmax=geom["L"], ymax=geom["L"], zmax=geom["L"],
+                        interp_order=3, do_interlacing=1, los=0, momentum=1, nk=geom["nk"],
+                        maxk=geom["nk"] * geom["dk"], precision=api.FP32,
+                        extra_spectra=api.SPEC_AUTO_A | api.SPEC_AUTO_B)
+    ps = api.PowerSpectrum(p)
+    ps.set_particles(parts)
+    ps.power()
+    torch.cuda.synchronize()
+    ps.close()
+    src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    log("profile pass done")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (FETCH_SIZE + WRITE_SIZE, calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", TRAFFIC_FILE)
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    m = re.search(r"```json\s*(\{.*?\})\s*```", text, re.S)
+    if not m:
+        return None
+    return json.loads(m.group(1)).get("traffic_bytes", {}).get(kernel)
+
+
 def cpu_baseline(geom_small, nl_small):
-    """The FP64 oracle on a bounded sample of the same workload: a 256^3
-    grid with the catalogue thinned to 1/64 (2,097,152 particles = 128^3
-    lattice, same box and spectrum), TSC + interlacing, Momentum = 1 grids,
-    three spectra.  Stage times are scaled to 1024^3 / 134M particles:
-    assignment x64 (particles), FFT x64*30/24 (N log N), binning x64."""
+    """The FP64 oracle on a bounded sample of the same workload (about 10 s
+    of one core): a 256^3 grid and a 200^3 Zel'dovich lattice (8,000,000
+    particles, same box and spectrum), TSC + interlacing, Momentum = 1
+    grids, three spectra.  Stage times are scaled to 1024^3 / 134M
+    particles: assignment by the particle ratio, FFT by N log N, binning by
+    the mode ratio."""
     import ctypes as C
     from capow_amd import mocks
     from oracle import pyoracle as O
@@ -111,7 +153,7 @@ def cpu_baseline(geom_small, nl_small):
         O.compute_power(p, grids if code == 1 else [grids[0], grids[1], None, None]
                         if code == 0 else [grids[2], grids[3], None, None])
     t3 = time.perf_counter()
-    scale_p = (512 ** 3) / float(nl_small ** 3)
+    scale_p = (512 ** 3) / float(nl_small ** 3)   # 134,217,728 / sample particles
     scale_g = (1024 / n) ** 3
     fft_scale = scale_g * (np.log2(1024 ** 3) / np.log2(n ** 3))
     est = (t1 - t0) * scale_p + (t2 - t1) * fft_scale + (t3 - t2) * scale_g
@@ -135,6 +177,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--grid", type=int, default=1024, help="grid size (1024 = the workload)")
     ap.add_argument("--lattice", type=int, default=512)
+    ap.add_argument("--profile-pass", action="store_true",
+                    help="for rocprofv3 counter runs: one FP32 step and a %d-byte calibration "
+                         "copy, no timing" % COPY_BYTES)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the periodic P_l(k) path runs on a single MI355X "
@@ -156,11 +201,15 @@ def main():
     n_p = parts32.shape[0]
     log("catalogue: %d particles in %.1f s" % (n_p, time.perf_counter() - t0))
 
+    if args.profile_pass:
+        profile_pass(parts32, geom)
+        return
+
     log("FP32 form")
     f32 = run_form(api.FP32, parts32, n_p, args, geom)
     log("  %.2f ms/step" % f32["ms"])
     for k, (ms, la) in sorted(f32["kernels"].items(), key=lambda kv: -kv[1][0]):
-        log("    %-16s %9.3f ms  (%d launches)" % (k, ms, la))
+        log("    %-16s %9.3f ms/launch  (%d launches)" % (k, ms / la, la))
     f64 = None
     if not args.no_fp64:
         parts64 = parts32.to(torch.float64)
@@ -168,7 +217,7 @@ def main():
         f64 = run_form(api.FP64, parts64, n_p, args, geom)
         log("  %.2f ms/step" % f64["ms"])
         for k, (ms, la) in sorted(f64["kernels"].items(), key=lambda kv: -kv[1][0]):
-            log("    %-16s %9.3f ms  (%d launches)" % (k, ms, la))
+            log("    %-16s %9.3f ms/launch  (%d launches)" % (k, ms / la, la))
         del parts64
         torch.cuda.empty_cache()
 
@@ -180,25 +229,28 @@ def main():
     for name, (ms, launches) in kt.items():
         b = algorithmic_bytes(name, n_p, ng, G, 4)
         if b is None:
-            rows[name] = dict(ms=ms, launches=launches)
+            rows[name] = dict(ms=ms, launches=launches, ms_per_launch=ms / launches)
             continue
         avg = ms / launches
-        rows[name] = dict(ms=ms, launches=launches, alg_bytes=b, gbs=b / (avg * 1e-3) / 1e9)
+        rows[name] = dict(ms=ms, launches=launches, ms_per_launch=avg, alg_bytes=b,
+                          gbs=b / (avg * 1e-3) / 1e9)
     dom = max((k for k in rows if "alg_bytes" in rows[k]), key=lambda k: rows[k]["ms"])
     d = rows[dom]
     avg_ms = d["ms"] / d["launches"]
+    tb = measured_traffic(dom)
     roofline = dict(bound="hbm", kernel=dom, achieved=round(d["gbs"], 1), peak=HBM_PEAK_GBS,
-                    unit="GB/s", frac=round(d["gbs"] / HBM_PEAK_GBS, 4), traffic=None,
+                    unit="GB/s", frac=round(d["gbs"] / HBM_PEAK_GBS, 4),
+                    traffic=(round(tb) if tb else None),
                     avg_launch_ms=round(avg_ms, 4), alg_bytes_per_launch=d["alg_bytes"])
     assign_ms = sum(kt[k][0] for k in ("bucket_count", "stats_final", "scan", "bucket_scatter",
-                                        "deposit") if k in kt)
+                                        "deposit") if k in kt) / f32["steps"]
     assign_bytes = n_p * 16 + G * ng * 4
 
     cpu = None
     if not args.no_cpu_baseline:
         log("CPU baseline (oracle sample)")
         small = dict(n=256, L=L, dk=dk, nk=int(np.floor(np.pi * 256 / L / dk)))
-        cpu = cpu_baseline(small, 128)
+        cpu = cpu_baseline(small, 200)
         log("  est %.0f ms" % cpu["value"])
 
     res = f32["res"]
@@ -229,8 +281,8 @@ def main():
         "assign_frac_hbm": round(assign_bytes / (assign_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "kernels_fp32": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv)
                              for kk, vv in v.items()} for k, v in rows.items()},
-        "kernels_fp64": ({k: dict(ms=round(v[0], 4), launches=v[1]) for k, v in f64["kernels"].items()}
-                         if f64 else None),
+        "kernels_fp64": ({k: dict(ms=round(v[0], 4), launches=v[1], ms_per_launch=round(v[0] / v[1], 4))
+                          for k, v in f64["kernels"].items()} if f64 else None),
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -321,7 +321,8 @@ __device__ __forceinline__ int tidx(int n1, int k2, int c) {
 
 // Columns: B consecutive complex columns, N = N1*N2 along a strided axis.
 // Threads: (N / E) per column * B; lane -> (c = lane % B, t = lane / B).
-template <typename T, int N1, int N2, int B>
+// AX names the axis (1: y, 0: x) so that profiles tell the passes apart.
+template <typename T, int N1, int N2, int B, int AX>
 __global__ __launch_bounds__((N1 * N2 / clcm(N1, N2)) * B) void k_fft_cols2(
     T* __restrict__ g0, T* __restrict__ g1, T* __restrict__ g2, T* __restrict__ g3,
     int64_t astride, int64_t pstride, int64_t ncols, int64_t groups_per_plane,
@@ -518,20 +519,20 @@ constexpr int rows_B() {
   return b;
 }
 
-template <typename T, int N1, int N2, int MUL>
+template <typename T, int N1, int N2, int MUL, int AX>
 static void launch_cols2(FftArgs& a, const char* name, T** gp, int64_t astride, int64_t pstride,
                          int64_t ncols, int64_t nplanes, const void* tw) {
   constexpr int N = N1 * N2, E = clcm(N1, N2), TS = N / E;
   constexpr int B = cols_B<T, N1, N2>() * MUL;
   const size_t lds = (size_t)N * B * sizeof(cx<T>);
   if (lds > 64 * 1024)
-    hipFuncSetAttribute((const void*)k_fft_cols2<T, N1, N2, B>,
+    hipFuncSetAttribute((const void*)k_fft_cols2<T, N1, N2, B, AX>,
                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   const int64_t gpp = (ncols + B - 1) / B;
   const int64_t nbx = gpp * nplanes;
   dim3 grid((unsigned)nbx, a.ngrids);
   TIMED(a.timer, name,
-        (k_fft_cols2<T, N1, N2, B><<<grid, TS * B, lds, a.stream>>>(
+        (k_fft_cols2<T, N1, N2, B, AX><<<grid, TS * B, lds, a.stream>>>(
             gp[0], gp[1], gp[2], gp[3], astride, pstride, ncols, gpp, (const cx<T>*)tw, nbx)));
 }
 
@@ -550,7 +551,7 @@ static void launch_rows2(FftArgs& a, T** gp, int64_t nrows, const void* twz, int
 #define CAPOW_FFT_SIZES(X) X(4, 4) X(4, 8) X(8, 8) X(8, 12) X(8, 16) X(8, 24) X(16, 16) \
   X(16, 24) X(16, 32) X(32, 32)
 
-template <typename T>
+template <typename T, int AX>
 static bool try_cols2(FftArgs& a, const char* name, T** gp, int N, int64_t astride,
                       int64_t pstride, int64_t ncols, int64_t nplanes, const void* tw) {
   int n1, n2;
@@ -558,9 +559,9 @@ static bool try_cols2(FftArgs& a, const char* name, T** gp, int N, int64_t astri
 #define X(A, B_)                                                                     \
   if (n1 == A && n2 == B_) {                                                          \
     if (a.col_mul == 2 && (size_t)A * B_ * cols_B<T, A, B_>() * 2 * sizeof(cx<T>) <= 160 * 1024) \
-      launch_cols2<T, A, B_, 2>(a, name, gp, astride, pstride, ncols, nplanes, tw);   \
+      launch_cols2<T, A, B_, 2, AX>(a, name, gp, astride, pstride, ncols, nplanes, tw); \
     else                                                                              \
-      launch_cols2<T, A, B_, 1>(a, name, gp, astride, pstride, ncols, nplanes, tw);   \
+      launch_cols2<T, A, B_, 1, AX>(a, name, gp, astride, pstride, ncols, nplanes, tw); \
     return true;                                                                      \
   }
   CAPOW_FFT_SIZES(X)
@@ -610,7 +611,7 @@ static int run_fft_t(FftArgs& a) {
                                                      a.plan[2], (const cx<T>*)a.tw[2], g.rowlen)));
   }
   // y pass: planes = x index, columns = kz
-  if (!(a.fast && try_cols2<T>(a, "fft_y", gp, ny, rowc, (int64_t)ny * rowc,
+  if (!(a.fast && try_cols2<T, 1>(a, "fft_y", gp, ny, rowc, (int64_t)ny * rowc,
                                (int64_t)nzh, nx, a.tw[1]))) {
     int64_t lim = 8192;  // N*B complex per workgroup (FFT_EPT * 1024)
     TIMED_COLS(a, "fft_y", gp, ny, rowc, (int64_t)ny * rowc, (int64_t)nzh, nx, a.plan[1],
@@ -619,7 +620,7 @@ static int run_fft_t(FftArgs& a) {
   // x pass: one plane, columns = (y, kz) over the padded rows (the padding
   // columns share the rows' last cache line and are never read as modes)
   const int64_t ncols = (int64_t)ny * rowc;
-  if (!(a.fast && try_cols2<T>(a, "fft_x", gp, nx, ncols, 0, ncols, 1, a.tw[0]))) {
+  if (!(a.fast && try_cols2<T, 0>(a, "fft_x", gp, nx, ncols, 0, ncols, 1, a.tw[0]))) {
     int64_t lim = 8192;
     TIMED_COLS(a, "fft_x", gp, nx, ncols, 0, ncols, 1, a.plan[0], a.tw[0], lim);
   }

@@ -85,6 +85,49 @@ def run_form(prec, parts, n_p, args, geom):
     return out
 
 
+def run_batch(args):
+    """Batched mocks: `args.mocks` catalogues in a 500 Mpc/h box, particle
+    counts lognormal (median 5e5, sigma_ln 0.6, clipped to [1e5, 3e6]), each
+    half Zel'dovich lattice (P(k) as above, per-mock seed) and half uniform;
+    grids alternate 128^3 (NK = 80) and 192^3 (NK = 120), CIC + interlacing,
+    density P_0,2,4(k), dk = 0.01, FP32 grids with FP64 bins.  All catalogues
+    are resident on the GPU before the timed run."""
+    import torch
+    from capow_amd import api, mocks
+    Lb = 500.0
+    counts = mocks.lognormal_counts(args.mocks, seed=2024)
+    cfgs = [dict(nx=g, ny=g, nz=g, xmax=Lb, ymax=Lb, zmax=Lb, interp_order=2, do_interlacing=1,
+                 momentum=0, nk=nk, maxk=nk * 0.01, precision=api.FP32)
+            for g, nk in ((128, 80), (192, 120))]
+    which = [i % 2 for i in range(args.mocks)]
+    t0 = time.perf_counter()
+    cats = [mocks.mixed_mock_torch(int(n), Lb, seed=1000 + i) for i, n in enumerate(counts)]
+    torch.cuda.synchronize()
+    ntot = int(sum(c.shape[0] for c in cats))
+    log("batch: %d mocks, %d particles generated in %.1f s" % (len(cats), ntot, time.perf_counter() - t0))
+    b = api.Batch(cfgs, max_particles=int(counts.max()), nstreams=args.mock_streams)
+    b.run(cats[:16], which[:16])          # warmup
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res, stats, ms = b.run(cats, which)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    b.close()
+    del cats
+    torch.cuda.empty_cache()
+    out = dict(mocks=len(counts), mocks_per_s=round(len(counts) / wall, 1),
+               wall_s=round(wall, 4), p50_ms=round(float(np.percentile(ms, 50)), 3),
+               p99_ms=round(float(np.percentile(ms, 99)), 3), particles=ntot,
+               mparticles_per_s=round(ntot / wall / 1e6, 1), streams=args.mock_streams,
+               grids="128^3 (NK 80) / 192^3 (NK 120) alternating", box=Lb,
+               interp="CIC+interlacing", dtype="f32 grids, f64 bins",
+               per_mock_ms=("GPU time from a mock's first to last stage on its stream, "
+                            "sharing the GPU with the other streams"))
+    log("batch: %.1f mocks/s, p50 %.3f ms, p99 %.3f ms per mock" % (out["mocks_per_s"],
+                                                                   out["p50_ms"], out["p99_ms"]))
+    return out
+
+
 def profile_pass(parts, geom):
     """One FP32 step of the workload, then a hipMemcpy of COPY_BYTES (the
     counter calibration), for rocprofv3 --pmc runs."""
@@ -177,6 +220,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--grid", type=int, default=1024, help="grid size (1024 = the workload)")
     ap.add_argument("--lattice", type=int, default=512)
+    ap.add_argument("--mocks", type=int, default=512, help="batched mocks (0: skip)")
+    ap.add_argument("--mock-streams", type=int, default=4)
     ap.add_argument("--profile-pass", action="store_true",
                     help="for rocprofv3 counter runs: one FP32 step and a %d-byte calibration "
                          "copy, no timing" % COPY_BYTES)
@@ -246,6 +291,10 @@ def main():
                                         "deposit") if k in kt) / f32["steps"]
     assign_bytes = n_p * 16 + G * ng * 4
 
+    del parts32
+    torch.cuda.empty_cache()
+    batch = run_batch(args) if args.mocks > 0 else None
+
     cpu = None
     if not args.no_cpu_baseline:
         log("CPU baseline (oracle sample)")
@@ -284,6 +333,7 @@ def main():
         "kernels_fp64": ({k: dict(ms=round(v[0], 4), launches=v[1], ms_per_launch=round(v[0] / v[1], 4))
                           for k, v in f64["kernels"].items()} if f64 else None),
         "roofline": roofline,
+        "batched_mocks": batch,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -59,6 +59,11 @@ class Bins(C.Structure):
                 ("pk2d", C.POINTER(C.c_double)), ("nmodes2d", C.POINTER(C.c_int64))]
 
 
+class Mock(C.Structure):
+    _fields_ = [("xyzv", C.c_void_p), ("w", C.c_void_p), ("n", C.c_int64),
+                ("config", C.c_int32), ("reserved", C.c_int32)]
+
+
 # Exported symbols of include/capow.h with their ctypes signatures.
 _P = C.POINTER(Params)
 _V = C.c_void_p
@@ -90,15 +95,36 @@ SIGNATURES = {
     "capow_kernel_times": ([_V, C.c_char_p, C.c_int32, _D, C.POINTER(C.c_int32), C.c_int32,
                             C.POINTER(C.c_int32)], C.c_int),
     "capow_reset_kernel_times": ([_V], C.c_int),
+    "capow_batch_create": ([_P, C.c_int32, C.c_int64, C.c_int32, C.POINTER(_V)], C.c_int),
+    "capow_batch_run": ([_V, C.POINTER(Mock), C.c_int32, C.POINTER(Bins),
+                         C.POINTER(AssignStats), C.POINTER(C.c_float)], C.c_int),
+    "capow_batch_destroy": ([_V], C.c_int),
+    "capow_batch_last_error": ([_V], C.c_char_p),
 }
 
 _lib = None
+
+
+def _init_torch_device():
+    """torch ships its own HIP runtime beside the system one libcapow links;
+    in one process both work only when torch's initializes the GPU first, so
+    the first use of the library does that (a no-op without torch or GPU)."""
+    try:
+        import torch
+    except ImportError:
+        return
+    try:
+        if torch.cuda.is_available():
+            torch.cuda.init()
+    except Exception:
+        pass
 
 
 def lib():
     """Loads libcapow.so; raises if it has not been built."""
     global _lib
     if _lib is None:
+        _init_torch_device()
         if not os.path.exists(LIB_PATH):
             raise ImportError("libcapow.so not built: run __graft_entry__.build() or "
                               "python -m capow_amd.build_lib")
@@ -309,3 +335,75 @@ def write_output(p, shot, norm, pk5, nmodes, path, pk2d=None, nmodes2d=None, pre
         _dptr(np.ascontiguousarray(pk2d, dtype=np.float64)) if pk2d is not None else None,
         _iptr(np.ascontiguousarray(nmodes2d, dtype=np.int64)) if nmodes2d is not None else None,
         path.encode(), 1 if precise else 0))
+
+
+class Batch:
+    """Many catalogues through one batch runner (capow_batch_*): catalogue i
+    runs on stream i % nstreams with no host synchronisation between
+    catalogues.  `configs` are parameter dicts or Params (e.g. one per grid
+    size); catalogues are device tensors of (n, 4) rows x y z v_los."""
+
+    def __init__(self, configs, max_particles, nstreams=4):
+        ps = [c if isinstance(c, Params) else make_params(**c) for c in configs]
+        self.configs = ps
+        arr = (Params * len(ps))(*ps)
+        h = _V()
+        rc = lib().capow_batch_create(arr, len(ps), int(max_particles), int(nstreams), C.byref(h))
+        if rc:
+            raise CapowError(rc, lib().capow_batch_last_error(None).decode())
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().capow_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, catalogues, configs, weights=None):
+        """catalogues: list of device tensors; configs: list of config indices.
+        Returns (results, stats, ms): per-mock raw bins in PowerSpectrum.bin()'s
+        form, AssignStats and GPU ms per mock."""
+        n = len(catalogues)
+        mocks = (Mock * n)()
+        outs, bins = [], (Bins * n)()
+        keep = []
+        for i, (t, q) in enumerate(zip(catalogues, configs)):
+            w = weights[i] if weights is not None else None
+            mocks[i] = Mock(t.data_ptr(), w.data_ptr() if w is not None else None,
+                            int(t.shape[0]), int(q), 0)
+            p = self.configs[q]
+            ns = C.c_int32()
+            codes = (C.c_int32 * 3)()
+            lib().capow_spectra(C.byref(p), C.byref(ns), codes)
+            nmu = p.nmu if p.output_2d else 1
+            out = {"pk": np.zeros(ns.value * 5 * p.nk), "nmodes": np.zeros(p.nk, dtype=np.int64),
+                   "pk2d": np.zeros(ns.value * p.nk * nmu),
+                   "nmodes2d": np.zeros(p.nk * nmu, dtype=np.int64)}
+            keep.append(out)
+            bins[i] = Bins(0, (C.c_int32 * 3)(), _dptr(out["pk"]), _iptr(out["nmodes"]),
+                           _dptr(out["pk2d"]) if p.output_2d else None,
+                           _iptr(out["nmodes2d"]) if p.output_2d else None)
+        stats = (AssignStats * n)()
+        ms = (C.c_float * n)()
+        rc = lib().capow_batch_run(self.h, mocks, n, bins, stats, ms)
+        if rc:
+            raise CapowError(rc, lib().capow_batch_last_error(self.h).decode())
+        res = []
+        for i in range(n):
+            p = self.configs[configs[i]]
+            b, out = bins[i], keep[i]
+            nsp = b.n_spectra
+            nmu = p.nmu if p.output_2d else 1
+            r = {"nmodes": out["nmodes"], "nmodes2d": out["nmodes2d"].reshape(p.nk, nmu),
+                 "codes": list(b.codes[:nsp])}
+            pk = out["pk"].reshape(nsp, 5, p.nk)
+            pk2d = out["pk2d"].reshape(nsp, p.nk, nmu)
+            for s in range(nsp):
+                r[b.codes[s]] = {"pk": pk[s].copy(), "pk2d": pk2d[s].copy()}
+            res.append(r)
+        return res, list(stats), np.array(ms[:], dtype=np.float64)

@@ -60,6 +60,11 @@ struct capow_ctx {
   int nspec = 1;
   int codes[3] = {0, 0, 0};
   capow_assign_stats last{};
+  // pinned host staging of the stats, error flags and bins of the last call
+  double* h_stats = nullptr;
+  unsigned int* h_errs = nullptr;
+  double* h_bins = nullptr;
+  bool cap_fixed = false;   // sorted buffer capacity fixed (batch); overflow is flagged
   int fft_col_mul = 2;   // column batch multiplier of the y/x FFT passes (CAPOW_FFT_COL_MUL=1 halves it)
 };
 
@@ -262,6 +267,8 @@ static int setup(capow_ctx* c) {
   HIPCHK(c, hipMalloc((void**)&c->gmax, 2 * sizeof(unsigned long long)));
   HIPCHK(c, hipMalloc((void**)&c->errs, 4 * sizeof(unsigned int)));
   HIPCHK(c, hipMalloc((void**)&c->stats, STAT_TOTAL * 8));
+  HIPCHK(c, hipHostMalloc((void**)&c->h_stats, STAT_TOTAL * 8, hipHostMallocDefault));
+  HIPCHK(c, hipHostMalloc((void**)&c->h_errs, 4 * sizeof(unsigned int), hipHostMallocDefault));
 
   // FFT plans + twiddles
   if (!make_plan(p.nx, &c->plan[0]) || !make_plan(p.ny, &c->plan[1]) ||
@@ -369,6 +376,7 @@ static int setup(capow_ctx* c) {
     c->bin_nwg = (int)(want < ncu ? want : ncu);
     HIPCHK(c, hipMalloc((void**)&c->bin_partials, (size_t)c->bin_nwg * bytes));
     HIPCHK(c, hipMalloc((void**)&c->bin_out, bytes));
+    HIPCHK(c, hipHostMalloc((void**)&c->h_bins, bytes, hipHostMallocDefault));
   }
   return CAPOW_OK;
 }
@@ -379,6 +387,8 @@ static void release(capow_ctx* c) {
                 c->cursor, c->bsum, c->boff, c->partials, c->gmax, c->errs, c->stats,
                 c->tw[0], c->tw[1], c->tw[2], c->bin_tabs, c->bin_partials, c->bin_out};
   for (void* q : ps) if (q) hipFree(q);
+  void* hs[] = {c->h_stats, c->h_errs, c->h_bins};
+  for (void* q : hs) if (q) hipHostFree(q);
 }
 
 extern "C" int capow_create(const capow_params* p, capow_ctx** out) {
@@ -489,8 +499,12 @@ static void fill_stats(capow_ctx* c, const double* s) {
   o.alpha = s[STAT_ALPHA];
 }
 
-extern "C" int capow_assign(capow_ctx* c, capow_assign_stats* st) {
-  if (!c) return CAPOW_ERR_ARG;
+// Enqueues the assignment on the context stream: sort, deposit, and
+// asynchronous copies of the sums and error flags into h_stats / h_errs.
+// exact: size the sorted buffer from the copy count (one synchronisation);
+// otherwise the buffer keeps its capacity and an overflow is flagged in
+// h_errs[2] (the caller re-runs that catalogue).
+static int assign_enqueue(capow_ctx* c, double* h_stats, unsigned int* h_errs, bool exact) {
   if (c->state < 1) return fail(c, CAPOW_ERR_STATE, "capow_assign before capow_set_particles");
   Geom& g = c->g;
   g.has_w = c->wts != nullptr;
@@ -518,38 +532,66 @@ extern "C" int capow_assign(capow_ctx* c, capow_assign_stats* st) {
   for (int s = 0; s < 4; s++) a.grid[s] = c->grid[s];
   int rc = run_count(a, c->fp64);
   if (rc) return fail(c, rc, std::string("deposit count failed: ") + hipGetErrorString(hipGetLastError()));
-  // total particle copies (one per reached tile) sizes the sorted buffer
-  uint32_t total = 0;
-  HIPCHK(c, hipMemcpyAsync(&total, c->offsets + g.nbuckets, 4, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  const size_t rows = (size_t)total > 0 ? (size_t)total : 1;
-  rc = ensure(c, &c->sorted, &c->sorted_cap, rows * 4 * c->esz);
-  if (rc) return rc;
-  if (c->wts) {
-    rc = ensure(c, &c->sorted_w, &c->sorted_w_cap, rows * c->esz);
+  if (exact) {
+    // total particle copies (one per reached tile) sizes the sorted buffer
+    uint32_t total = 0;
+    HIPCHK(c, hipMemcpyAsync(&total, c->offsets + g.nbuckets, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const size_t rows = (size_t)total > 0 ? (size_t)total : 1;
+    rc = ensure(c, &c->sorted, &c->sorted_cap, rows * 4 * c->esz);
     if (rc) return rc;
+    if (c->wts) {
+      rc = ensure(c, &c->sorted_w, &c->sorted_w_cap, rows * c->esz);
+      if (rc) return rc;
+    }
+  } else if (!c->sorted || (c->wts && !c->sorted_w)) {
+    return fail(c, CAPOW_ERR_STATE, "sorted buffer not reserved");
   }
   a.sorted = c->sorted;
   a.sorted_w = c->sorted_w;
-  a.sorted_cap = (int64_t)(c->sorted_cap / (4 * c->esz));
+  int64_t cap = (int64_t)(c->sorted_cap / (4 * c->esz));
+  if (c->wts) {
+    const int64_t capw = (int64_t)(c->sorted_w_cap / c->esz);
+    cap = capw < cap ? capw : cap;
+  }
+  a.sorted_cap = cap;
   rc = run_scatter_deposit(a, c->fp64);
   if (rc) return fail(c, rc, std::string("deposit launch failed: ") + hipGetErrorString(hipGetLastError()));
-  double s[STAT_TOTAL];
-  unsigned int e[4];
-  HIPCHK(c, hipMemcpyAsync(s, c->stats, sizeof s, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipMemcpyAsync(e, c->errs, sizeof e, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  c->timer.collect();
-  fill_stats(c, s);
-  if (st) *st = c->last;
-  if (e[0]) {
-    c->state = 1;
-    return fail(c, CAPOW_ERR_OUT_OF_BOX,
-                "particle out of grid bounds with LOS = 0 (" + std::to_string(e[0]) + " particles)");
-  }
-  if (e[1]) return fail(c, CAPOW_ERR_INTERNAL, "deposit reach self-check failed");
+  HIPCHK(c, hipMemcpyAsync(h_stats, c->stats, STAT_TOTAL * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(h_errs, c->errs, 4 * sizeof(unsigned int), hipMemcpyDeviceToHost, c->stream));
   c->state = 2;
   return CAPOW_OK;
+}
+
+// After the stream has completed: sums into the stats, error flags into a
+// status.  CAPOW_ERR_NOMEM: the sorted buffer overflowed (fixed capacity).
+static int assign_finish(capow_ctx* c, const double* h_stats, const unsigned int* h_errs,
+                         capow_assign_stats* st) {
+  fill_stats(c, h_stats);
+  if (st) *st = c->last;
+  if (h_errs[0]) {
+    c->state = 1;
+    return fail(c, CAPOW_ERR_OUT_OF_BOX,
+                "particle out of grid bounds with LOS = 0 (" + std::to_string(h_errs[0]) + " particles)");
+  }
+  if (h_errs[2]) {
+    c->state = 1;
+    return fail(c, CAPOW_ERR_NOMEM, "sorted particle buffer too small for this catalogue");
+  }
+  if (h_errs[1]) {
+    c->state = 1;
+    return fail(c, CAPOW_ERR_INTERNAL, "deposit reach self-check failed");
+  }
+  return CAPOW_OK;
+}
+
+extern "C" int capow_assign(capow_ctx* c, capow_assign_stats* st) {
+  if (!c) return CAPOW_ERR_ARG;
+  int rc = assign_enqueue(c, c->h_stats, c->h_errs, true);
+  if (rc) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->timer.collect();
+  return assign_finish(c, c->h_stats, c->h_errs, st);
 }
 
 extern "C" int capow_fft(capow_ctx* c) {
@@ -574,8 +616,8 @@ extern "C" int capow_fft(capow_ctx* c) {
   return CAPOW_OK;
 }
 
-extern "C" int capow_bin(capow_ctx* c, capow_bins* out) {
-  if (!c || !out) return CAPOW_ERR_ARG;
+// Enqueues the binning and an asynchronous copy of the bins into h_bins.
+static int bin_enqueue(capow_ctx* c, double* h_bins) {
   if (c->state < 3) return fail(c, CAPOW_ERR_STATE, "capow_bin before capow_fft");
   BinLaunch L;
   L.stream = c->stream;
@@ -590,16 +632,18 @@ extern "C" int capow_bin(capow_ctx* c, capow_bins* out) {
   L.out = c->bin_out;
   int rc = run_bin(L, c->fp64);
   if (rc) return fail(c, rc, std::string("binning launch failed: ") + hipGetErrorString(hipGetLastError()));
-  std::vector<double> h(c->bin_per_copy);
-  HIPCHK(c, hipMemcpyAsync(h.data(), c->bin_out, h.size() * 8, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  c->timer.collect();
+  HIPCHK(c, hipMemcpyAsync(h_bins, c->bin_out, (size_t)c->bin_per_copy * 8, hipMemcpyDeviceToHost,
+                           c->stream));
+  return CAPOW_OK;
+}
+
+// Unpacks staged bins (LDS order per spectrum: l = 0, 2, 4 [, 1, 3]).
+static void bin_finish(capow_ctx* c, const double* h, capow_bins* out) {
   const capow_params& p = c->p;
   const int nk = p.nk, nl = p.odd_multipoles ? 5 : 3, ns = c->nspec;
   const int n2d = p.output_2d ? p.nk * p.nmu : 0;
   out->n_spectra = ns;
   for (int s = 0; s < 3; s++) out->codes[s] = s < ns ? c->codes[s] : -1;
-  // LDS order per spectrum: l = 0, 2, 4 [, 1, 3]
   static const int lmap[5] = {0, 2, 4, 1, 3};
   if (out->pk) {
     for (int s = 0; s < ns; s++) {
@@ -610,7 +654,7 @@ extern "C" int capow_bin(capow_ctx* c, capow_bins* out) {
           out->pk[(s * 5 + lmap[q]) * nk + k] = h[(size_t)(s * nl + q) * nk + k];
     }
   }
-  const double* acc2 = h.data() + (size_t)ns * nl * nk;
+  const double* acc2 = h + (size_t)ns * nl * nk;
   const double* cnt = acc2 + (size_t)ns * n2d;
   const double* cnt2 = cnt + nk;
   if (out->nmodes)
@@ -621,6 +665,15 @@ extern "C" int capow_bin(capow_ctx* c, capow_bins* out) {
     if (out->nmodes2d)
       for (int q = 0; q < n2d; q++) out->nmodes2d[q] = (int64_t)cnt2[q];
   }
+}
+
+extern "C" int capow_bin(capow_ctx* c, capow_bins* out) {
+  if (!c || !out) return CAPOW_ERR_ARG;
+  int rc = bin_enqueue(c, c->h_bins);
+  if (rc) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->timer.collect();
+  bin_finish(c, c->h_bins, out);
   return CAPOW_OK;
 }
 
@@ -698,4 +751,171 @@ extern "C" int capow_reset_kernel_times(capow_ctx* c) {
   if (!c) return CAPOW_ERR_ARG;
   c->timer.reset();
   return CAPOW_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batches of catalogues.
+// ---------------------------------------------------------------------------
+struct capow_batch {
+  std::vector<capow_params> cfg;
+  int nconf = 0, nstreams = 0;
+  int64_t max_n = 0;
+  std::vector<hipStream_t> streams;
+  std::vector<capow_ctx*> ctx;   // [stream * nconf + config]
+  size_t stride = 0;             // doubles of staging per mock
+  double* stage = nullptr;
+  size_t stage_mocks = 0;
+  std::vector<hipEvent_t> ev;
+  std::string err;
+};
+
+static int bfail(capow_batch* b, int code, const std::string& m) {
+  if (b) b->err = m;
+  else g_create_err = m;
+  return code;
+}
+
+extern "C" int capow_batch_destroy(capow_batch* b) {
+  if (!b) return CAPOW_OK;
+  for (hipStream_t s : b->streams) hipStreamSynchronize(s);
+  for (capow_ctx* c : b->ctx) capow_destroy(c);
+  for (hipEvent_t e : b->ev) hipEventDestroy(e);
+  for (hipStream_t s : b->streams) hipStreamDestroy(s);
+  if (b->stage) hipHostFree(b->stage);
+  delete b;
+  return CAPOW_OK;
+}
+
+extern "C" const char* capow_batch_last_error(const capow_batch* b) {
+  return b ? b->err.c_str() : g_create_err.c_str();
+}
+
+extern "C" int capow_batch_create(const capow_params* configs, int32_t nconf,
+                                  int64_t max_particles, int32_t nstreams, capow_batch** out) {
+  if (!configs || !out || nconf < 1 || nstreams < 1 || max_particles < 0)
+    return bfail(nullptr, CAPOW_ERR_ARG, "bad batch arguments");
+  if (max_particles > (1LL << 29))
+    return bfail(nullptr, CAPOW_ERR_UNSUPPORTED, "more than 2^29 particles per catalogue");
+  *out = nullptr;
+  capow_batch* b = new capow_batch();
+  b->cfg.assign(configs, configs + nconf);
+  b->nconf = nconf;
+  b->nstreams = nstreams;
+  b->max_n = max_particles;
+  size_t per_copy_max = 0;
+  for (int s = 0; s < nstreams; s++) {
+    hipStream_t st;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+      capow_batch_destroy(b);
+      return bfail(nullptr, CAPOW_ERR_HIP, "hipStreamCreate failed");
+    }
+    b->streams.push_back(st);
+    for (int q = 0; q < nconf; q++) {
+      capow_ctx* c = nullptr;
+      int rc = capow_create(&configs[q], &c);
+      if (rc) {
+        std::string m = g_create_err;
+        capow_batch_destroy(b);
+        return bfail(nullptr, rc, "batch config " + std::to_string(q) + ": " + m);
+      }
+      b->ctx.push_back(c);
+      capow_set_stream(c, st);
+      c->cap_fixed = true;
+      // two copies per particle (tile + ghost copies are ~1.3 at these sizes)
+      const size_t rows = (size_t)2 * (size_t)max_particles + 4096;
+      int r1 = ensure(c, &c->sorted, &c->sorted_cap, rows * 4 * c->esz);
+      int r2 = r1 ? r1 : ensure(c, &c->sorted_w, &c->sorted_w_cap, rows * c->esz);
+      if (r1 || r2) {
+        std::string m = c->err;
+        capow_batch_destroy(b);
+        return bfail(nullptr, CAPOW_ERR_NOMEM, "batch sort buffers: " + m);
+      }
+      if ((size_t)c->bin_per_copy > per_copy_max) per_copy_max = (size_t)c->bin_per_copy;
+    }
+  }
+  b->stride = STAT_TOTAL + 2 + per_copy_max;
+  *out = b;
+  return CAPOW_OK;
+}
+
+extern "C" int capow_batch_run(capow_batch* b, const capow_mock* mocks, int32_t n,
+                               capow_bins* out, capow_assign_stats* stats, float* ms) {
+  if (!b || (n > 0 && (!mocks || !out))) return bfail(b, CAPOW_ERR_ARG, "bad batch run arguments");
+  for (int i = 0; i < n; i++) {
+    if (mocks[i].config < 0 || mocks[i].config >= b->nconf)
+      return bfail(b, CAPOW_ERR_ARG, "mock " + std::to_string(i) + ": config out of range");
+    if (mocks[i].n < 0 || mocks[i].n > b->max_n)
+      return bfail(b, CAPOW_ERR_ARG, "mock " + std::to_string(i) + ": particle count above the batch maximum");
+  }
+  if (b->stage_mocks < (size_t)n) {
+    if (b->stage) hipHostFree(b->stage);
+    b->stage = nullptr;
+    b->stage_mocks = 0;
+    if (hipHostMalloc((void**)&b->stage, (size_t)n * b->stride * 8, hipHostMallocDefault) != hipSuccess)
+      return bfail(b, CAPOW_ERR_NOMEM, "batch staging allocation failed");
+    b->stage_mocks = (size_t)n;
+  }
+  while (b->ev.size() < (size_t)2 * n) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return bfail(b, CAPOW_ERR_HIP, "hipEventCreate failed");
+    b->ev.push_back(e);
+  }
+  int first = CAPOW_OK;
+  std::vector<int> launched(n, 0);
+  for (int i = 0; i < n; i++) {
+    const int s = i % b->nstreams;
+    capow_ctx* c = b->ctx[(size_t)s * b->nconf + mocks[i].config];
+    double* st = b->stage + (size_t)i * b->stride;
+    unsigned int* er = reinterpret_cast<unsigned int*>(st + STAT_TOTAL);
+    double* bins = st + STAT_TOTAL + 2;
+    hipEventRecord(b->ev[2 * i], b->streams[s]);
+    int rc = capow_set_particles(c, mocks[i].xyzv, mocks[i].w, mocks[i].n,
+                                 c->fp64 ? CAPOW_DTYPE_F64 : CAPOW_DTYPE_F32, CAPOW_DEVICE);
+    if (!rc) rc = assign_enqueue(c, st, er, false);
+    if (!rc) rc = capow_fft(c);
+    if (!rc) rc = bin_enqueue(c, bins);
+    hipEventRecord(b->ev[2 * i + 1], b->streams[s]);
+    if (rc) {
+      if (!first) {
+        first = rc;
+        b->err = "mock " + std::to_string(i) + ": " + c->err;
+      }
+      continue;
+    }
+    launched[i] = 1;
+  }
+  for (hipStream_t s : b->streams)
+    if (hipStreamSynchronize(s) != hipSuccess)
+      return bfail(b, CAPOW_ERR_HIP, std::string("batch stream: ") + hipGetErrorString(hipGetLastError()));
+  for (int i = 0; i < n; i++) {
+    if (!launched[i]) continue;
+    const int s = i % b->nstreams;
+    capow_ctx* c = b->ctx[(size_t)s * b->nconf + mocks[i].config];
+    const double* st = b->stage + (size_t)i * b->stride;
+    const unsigned int* er = reinterpret_cast<const unsigned int*>(st + STAT_TOTAL);
+    c->n = mocks[i].n;   // the context ran later mocks since
+    int rc = assign_finish(c, st, er, stats ? &stats[i] : nullptr);
+    if (rc == CAPOW_ERR_NOMEM) {
+      // the catalogue needed more sorted copies than reserved: run it alone
+      // with an exactly sized buffer
+      int r = capow_set_particles(c, mocks[i].xyzv, mocks[i].w, mocks[i].n,
+                                  c->fp64 ? CAPOW_DTYPE_F64 : CAPOW_DTYPE_F32, CAPOW_DEVICE);
+      if (!r) r = capow_assign(c, stats ? &stats[i] : nullptr);
+      if (!r) r = capow_fft(c);
+      if (!r) r = capow_bin(c, &out[i]);
+      rc = r;
+    } else if (!rc) {
+      bin_finish(c, st + STAT_TOTAL + 2, &out[i]);
+    }
+    if (ms) {
+      float t = 0.f;
+      hipEventElapsedTime(&t, b->ev[2 * i], b->ev[2 * i + 1]);
+      ms[i] = t;
+    }
+    if (rc && !first) {
+      first = rc;
+      b->err = "mock " + std::to_string(i) + ": " + c->err;
+    }
+  }
+  return first;
 }

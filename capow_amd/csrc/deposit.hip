@@ -567,7 +567,8 @@ __global__ __launch_bounds__(DEP_T) void k_deposit(const T* __restrict__ sorted,
                                                    Geom g, T* __restrict__ gA,
                                                    T* __restrict__ gAi, T* __restrict__ gB,
                                                    T* __restrict__ gBi,
-                                                   const double* __restrict__ stats) {
+                                                   const double* __restrict__ stats,
+                                                   int64_t cap, unsigned int* __restrict__ errs) {
   constexpr bool F32 = sizeof(T) == 4;
   // static LDS (its base folds into the tap addresses): the largest tile's
   // accumulators, checked on the host
@@ -587,7 +588,12 @@ __global__ __launch_bounds__(DEP_T) void k_deposit(const T* __restrict__ sorted,
   const int tty = (int)((tileid / g.ntile[0]) % g.ntile[1]);
   const int ttz = (int)(tileid / ((int64_t)g.ntile[0] * g.ntile[1]));
   const int t0x = ttx * TX, t0y = tty * TY, t0z = ttz * TZ;
-  const uint32_t s0 = off[tileid], s1 = off[tileid + 1];
+  // copies beyond the sorted buffer's capacity were not stored: flag the
+  // overflow and deposit only what is there (the result is then discarded)
+  const uint32_t total = off[(int64_t)g.ntile[0] * g.ntile[1] * g.ntile[2]];
+  if (tileid == 0 && threadIdx.x == 0 && (int64_t)total > cap) errs[2] = 1u;
+  const uint32_t s0 = (uint32_t)min((int64_t)off[tileid], cap);
+  const uint32_t s1 = (uint32_t)min((int64_t)off[tileid + 1], cap);
 
   const double wmax = ORDER == 3 ? 0.421875 : 1.0;
   const double scA = F32 ? tile_scale(s1 - s0, stats[STAT_GMAX_A], wmax) : stats[STAT_SCALE_A];
@@ -781,12 +787,12 @@ static int run_scatter_t(DepositArgs& a) {
     TIMED(a.timer, "deposit",
           (k_deposit<T, ORDER, true><<<(unsigned)ntiles, DEP_T, 0, st>>>(
               (const T*)a.sorted, (const T*)a.sorted_w, a.offsets, g, (T*)a.grid[0],
-              (T*)a.grid[1], (T*)a.grid[2], (T*)a.grid[3], a.stats)));
+              (T*)a.grid[1], (T*)a.grid[2], (T*)a.grid[3], a.stats, a.sorted_cap, a.errs)));
   } else {
     TIMED(a.timer, "deposit",
           (k_deposit<T, ORDER, false><<<(unsigned)ntiles, DEP_T, 0, st>>>(
               (const T*)a.sorted, (const T*)a.sorted_w, a.offsets, g, (T*)a.grid[0],
-              (T*)a.grid[1], (T*)a.grid[2], (T*)a.grid[3], a.stats)));
+              (T*)a.grid[1], (T*)a.grid[2], (T*)a.grid[3], a.stats, a.sorted_cap, a.errs)));
   }
   return hipGetLastError() == hipSuccess ? CAPOW_OK : CAPOW_ERR_HIP;
 }

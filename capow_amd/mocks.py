@@ -117,3 +117,30 @@ def xyzv(pos, vel, los=0):
     (src/read_data.c:65-76: vz for LOS 0 and 3, vx for 1, vy for 2)."""
     col = {0: 2, 1: 0, 2: 1, 3: 2}[los]
     return np.concatenate([pos, vel[:, col:col + 1]], axis=1)
+
+
+def lognormal_counts(nmocks, seed, median=5e5, sigma_ln=0.6, lo=1e5, hi=3e6):
+    """Per-mock particle counts: lognormal with the given median and sigma of
+    ln n, clipped to [lo, hi] (seeded, numpy Philox)."""
+    rng = np.random.Generator(np.random.Philox(seed))
+    n = np.exp(np.log(median) + sigma_ln * rng.standard_normal(nmocks))
+    return np.clip(np.round(n), lo, hi).astype(np.int64)
+
+
+def mixed_mock_torch(n, L, seed, device="cuda", dtype=None, frac_clustered=0.5):
+    """One mock of n points on the GPU: a Zel'dovich lattice of nl^3 points
+    (power law as above; nl = (frac_clustered n)^(1/3) rounded to a multiple
+    of 8, so that a few FFT sizes serve every mock) plus uniform points for
+    the rest (zero velocity).  Rows x y z v_z."""
+    import torch
+    dtype = dtype or torch.float32
+    nl = int(round((frac_clustered * n) ** (1.0 / 3.0) / 8.0)) * 8
+    nl = max(8, min(nl, int(n ** (1.0 / 3.0)) // 8 * 8 if n >= 512 else 8))
+    z = zeldovich_torch(nl, L, seed=seed, device=device, dtype=dtype)
+    nu = max(0, int(n) - z.shape[0])
+    g = torch.Generator(device=device)
+    g.manual_seed(seed + 7919)
+    u = torch.rand((nu, 3), generator=g, device=device, dtype=torch.float64) * L
+    u = torch.where(u >= L, torch.zeros_like(u), u).to(dtype)
+    u = torch.cat([u, torch.zeros((nu, 1), device=device, dtype=dtype)], dim=1)
+    return torch.cat([z, u], dim=0).contiguous()

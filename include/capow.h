@@ -185,6 +185,39 @@ int capow_power(capow_ctx *ctx, capow_assign_stats *stats, capow_bins *out);
 /* Number of spectra capow_bin produces and their reference Momentum codes. */
 int capow_spectra(const capow_params *p, int32_t *n_spectra, int32_t codes[3]);
 
+/* ---- batches of catalogues (mock runs) --------------------------------- */
+/* No counterpart in the reference, which measures one catalogue per
+ * process run (src/main.c:28-298); a batch runs many catalogues through the
+ * same stages (capow_assign, capow_fft, capow_bin) without host
+ * synchronisation between them: catalogue i goes to stream i % nstreams,
+ * each stream owning one context per parameter set. */
+
+typedef struct capow_batch capow_batch;
+
+typedef struct {
+  const void *xyzv;    /* device rows x y z v_los (the configs' precision) */
+  const void *w;       /* device weights, or NULL for unit weights         */
+  int64_t n;           /* particles                                        */
+  int32_t config;      /* index of the parameter set                       */
+  int32_t reserved;
+} capow_mock;
+
+/* configs: nconf parameter sets (e.g. two grid sizes); max_particles: the
+ * largest catalogue (sizes the per-context sort buffers: 2 copies per
+ * particle, a catalogue that needs more is re-run alone); nstreams >= 1
+ * concurrent streams (default priority). */
+int capow_batch_create(const capow_params *configs, int32_t nconf,
+                       int64_t max_particles, int32_t nstreams,
+                       capow_batch **out);
+/* Runs n catalogues.  out[i] (caller-owned arrays, as for capow_bin) gets
+ * mock i's raw bins; stats[i] (or NULL) its assignment sums; ms[i] (or
+ * NULL) the GPU time from its first to its last stage on its stream.
+ * Returns the first failing mock's status (all mocks are attempted). */
+int capow_batch_run(capow_batch *b, const capow_mock *mocks, int32_t n,
+                    capow_bins *out, capow_assign_stats *stats, float *ms);
+int capow_batch_destroy(capow_batch *b);
+const char *capow_batch_last_error(const capow_batch *b);
+
 /* ---- host-side output math (no GPU) ------------------------------------ */
 
 /* shot and norm for spectrum code `code` from the assignment sums

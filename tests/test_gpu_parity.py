@@ -369,3 +369,41 @@ def test_sort_order_and_table_overflow_do_not_change_grids(prec, monkeypatch):
             assert np.array_equal(base[0][s].view(u), other[0][s].view(u)), s
         for code in base[1]["codes"]:
             assert np.array_equal(base[1][code]["pk"], other[1][code]["pk"]), code
+
+
+def test_batch_matches_single_contexts():
+    """capow_batch_run (several streams, two grid configs, no host
+    synchronisation between mocks) gives bitwise the bins and sums of
+    running each catalogue alone through capow_power; a catalogue whose
+    copies overflow the batch's reserved sort buffer (every particle next to
+    a tile corner) is re-run alone and matches too."""
+    import torch
+    say("batch vs single contexts")
+    Lb = 500.0
+    cfgs = [dict(nx=32, ny=32, nz=32, xmax=Lb, ymax=Lb, zmax=Lb, interp_order=2, do_interlacing=1,
+                 momentum=0, nk=20, maxk=0.2, precision=api.FP32),
+            dict(nx=48, ny=48, nz=48, xmax=Lb, ymax=Lb, zmax=Lb, interp_order=2, do_interlacing=1,
+                 momentum=0, nk=30, maxk=0.3, precision=api.FP32)]
+    counts = [3000, 5000, 4000, 6000, 2500, 4500]
+    cats = [mocks.mixed_mock_torch(n, Lb, seed=100 + i) for i, n in enumerate(counts)]
+    # 6000 particles in the last cell of a tile in x and y (cell 15.5 of 32^3,
+    # tiles of 16 cells; z is one tile): 4 copies each, above the 2 per
+    # particle the batch reserves
+    corner = torch.full((6000, 4), 15.5 * Lb / 32, device="cuda", dtype=torch.float32)
+    corner[:, 3] = 0
+    cats.append(corner)
+    which = [0, 1, 0, 1, 0, 1, 0]
+    b = api.Batch(cfgs, max_particles=6000, nstreams=3)
+    res, stats, ms = b.run(cats, which)
+    b.close()
+    assert len(res) == len(cats) and np.all(ms > 0)
+    for i, (cat, q) in enumerate(zip(cats, which)):
+        ps = api.PowerSpectrum(api.make_params(**cfgs[q]))
+        ps.set_particles(cat)
+        ref = ps.power()
+        st = ps.stats
+        ps.close()
+        assert stats[i].ngrid == st.ngrid and stats[i].nread == st.nread, i
+        for code in ref["codes"]:
+            assert np.array_equal(res[i][code]["pk"], ref[code]["pk"]), (i, code)
+        assert np.array_equal(res[i]["nmodes"], ref["nmodes"]), i

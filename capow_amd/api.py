@@ -95,6 +95,7 @@ SIGNATURES = {
     "capow_kernel_times": ([_V, C.c_char_p, C.c_int32, _D, C.POINTER(C.c_int32), C.c_int32,
                             C.POINTER(C.c_int32)], C.c_int),
     "capow_reset_kernel_times": ([_V], C.c_int),
+    "capow_backward": ([_V, _D, _V, _V, C.c_int32], C.c_int),
     "capow_batch_create": ([_P, C.c_int32, C.c_int64, C.c_int32, C.POINTER(_V)], C.c_int),
     "capow_batch_run": ([_V, C.POINTER(Mock), C.c_int32, C.POINTER(Bins),
                          C.POINTER(AssignStats), C.POINTER(C.c_float)], C.c_int),
@@ -269,6 +270,30 @@ class PowerSpectrum:
         self.check(lib().capow_power(self.ctx, C.byref(st), C.byref(b)))
         self.stats = st
         return self._unpack(b, out)
+
+    def backward(self, dL_draw, want_w=True):
+        """dL/d(raw bins) (array [n_spectra][5][nk], PowerSpectrum.bin()'s
+        order) -> (dL/d(x, y, z, v_los) as (n, 4), dL/dw as (n,) or None) for
+        the particles last set: torch tensors on the device for device
+        particles, numpy arrays otherwise.  Recomputes the forward."""
+        G = np.ascontiguousarray(dL_draw, dtype=np.float64).ravel()
+        ns = len(self.codes)
+        if G.size != ns * 5 * self.p.nk:
+            raise ValueError("dL_draw must hold n_spectra * 5 * nk values")
+        xyzv = self._keep[0]
+        n = xyzv.shape[0]
+        if hasattr(xyzv, "data_ptr") and getattr(xyzv, "is_cuda", False):
+            import torch
+            gx = torch.empty((n, 4), dtype=xyzv.dtype, device=xyzv.device)
+            gw = torch.empty((n,), dtype=xyzv.dtype, device=xyzv.device) if want_w else None
+            self.check(lib().capow_backward(self.ctx, _dptr(G), _V(gx.data_ptr()),
+                                            _V(gw.data_ptr()) if gw is not None else None, DEVICE))
+            return gx, gw
+        gx = np.empty((n, 4), dtype=self.dtype)
+        gw = np.empty((n,), dtype=self.dtype) if want_w else None
+        self.check(lib().capow_backward(self.ctx, _dptr(G), gx.ctypes.data_as(_V),
+                                        gw.ctypes.data_as(_V) if gw is not None else None, HOST))
+        return gx, gw
 
     def grid_elems(self):
         n = C.c_int64()

@@ -15,63 +15,12 @@
 // Accumulation: every wavefront owns a private copy of the bins in LDS (adds
 // in program order => bit-reproducible); workgroup partials are then summed
 // in a fixed order.
-#include "common.h"
-#include "kernels.h"
+#include "binning_math.h"
 
 namespace capow {
 
-template <typename T>
-struct CT;
-template <>
-struct CT<float> { typedef float2 t; };
-template <>
-struct CT<double> { typedef double2 t; };
-
-__device__ __forceinline__ double dmul(double a, double b) { return __dmul_rn(a, b); }
-__device__ __forceinline__ double dadd(double a, double b) { return __dadd_rn(a, b); }
-__device__ __forceinline__ double ddiv(double a, double b) { return __ddiv_rn(a, b); }
-
-// Bin index of one mode, bit-identical to (int)((float)(fktot - Mink)/binwidth)
-// (src/main.c:368-373): multiply by the reciprocal and fall back to the exact
-// division only when the quotient lies within 1e-9 of an integer, where the
-// two could truncate differently.
-__device__ __forceinline__ int kbin_of(double fdk, const BinArgsDev& a) {
-  const double q = fdk * a.inv_binwidth;
-  const double tq = trunc(q);
-  const double fr = q - tq;
-  if (fabs(fr) < 1e-9 || fabs(fr) > 1.0 - 1e-9) return (int)ddiv(fdk, a.binwidth);
-  return (int)tq;
-}
-
-// Per-row quantities of one (i, j) row of modes: frequencies, fx^2+fy^2,
-// the x*y deconvolution and interlace phase products.
-struct RowVals {
-  int i, j;
-  double fx, fy, fxy, corxy, qr, qi;
-};
-
-__device__ __forceinline__ RowVals row_vals(int i, int j, const double* __restrict__ tabs,
-                                            const BinArgsDev& a) {
-  RowVals R;
-  R.i = i;
-  R.j = j;
-  R.fx = tabs[a.ofx + i];
-  R.fy = tabs[a.ofy + j];
-  R.fxy = dadd(dmul(R.fx, R.fx), dmul(R.fy, R.fy));
-  R.corxy = tabs[a.ocx + i] * tabs[a.ocy + j];
-  R.qr = 1.0;
-  R.qi = 0.0;
-  if (a.interlace) {
-    const double pxr = tabs[a.opx + 2 * i], pxi = tabs[a.opx + 2 * i + 1];
-    const double pyr = tabs[a.opy + 2 * j], pyi = tabs[a.opy + 2 * j + 1];
-    R.qr = pxr * pyr - pxi * pyi;
-    R.qi = pxr * pyi + pxi * pyr;
-  }
-  return R;
-}
-
-// Per-mode binning: |k|, mu, bin, NM, deconvolution, interlace phase, power
-// of each spectrum, Legendre weights, accumulation into this wave's bins.
+// Per-mode binning: geometry (mode_geom), interlace combine, power of each
+// spectrum, Legendre weights, accumulation into this wave's bins.
 template <typename T>
 __device__ __forceinline__ void bin_mode(const RowVals& R, int k, double fz, double corz,
                                          double pzr, double pzi, typename CT<T>::t va,
@@ -79,45 +28,13 @@ __device__ __forceinline__ void bin_mode(const RowVals& R, int k, double fz, dou
                                          typename CT<T>::t vbi, const BinArgsDev& a,
                                          double* accS, double* acc2, double* cnt, double* cnt2,
                                          int nl, int n2d) {
+  const ModeGeom G = mode_geom(R, k, fz, corz, pzr, pzi, a);
+  if (!G.ok) return;
   const int NK = a.nk;
-  const double twopi = 2.0 * 3.14159265358979323846;  // the reference's 2.0*M_PI
-  const int i = R.i, j = R.j;
-  const double fktot = dmul(twopi, sqrt(dadd(R.fxy, dmul(fz, fz))));
-  int kbin;
-  if (a.logbins) {
-    if (!(fktot > 0)) return;
-    kbin = (int)ddiv((double)__double2float_rn(log10(fktot) - a.mink), a.binwidth);
-  } else {
-    kbin = kbin_of((double)__double2float_rn(fktot - a.mink), a);
-  }
-  if (!((kbin >= 0) && (kbin < NK) && (fktot <= a.min_nyq))) return;
-  double mu = 0.0;
-  if (fktot > 0) {
-    const double fl = a.los == 1 ? R.fx : (a.los == 2 ? R.fy : fz);
-    if (a.out2d) {
-      mu = ddiv(dmul(twopi, fl), fktot);   // exact: feeds the mu-bin index
-    } else {
-      double rk = __builtin_amdgcn_rcp(fktot);
-      rk = rk * (2.0 - fktot * rk);
-      rk = rk * (2.0 - fktot * rk);
-      mu = (twopi * fl) * rk;
-    }
-  }
-  // Hermitian weight (src/main.c:422-429)
-  double NM = 2.0;
-  if (k == 0) {
-    if (j == 0) {
-      if ((i != 0) && (i != a.nx / 2)) NM = 1.0;
-    } else {
-      if (i != a.nx / 2) NM = 1.0;
-    }
-  }
-  const double cor = R.corxy * corz;  // x0.25 folded in corz
+  const int kbin = G.kbin;
+  const double mu = G.mu, NM = G.NM, cor = G.cor, cr = G.cr, ci = G.ci;
   double ar = (double)va.x, ai = (double)va.y, br = 0, bi = 0;
-  double cr = 1, ci = 0;
   if (a.interlace) {
-    cr = R.qr * pzr - R.qi * pzi;
-    ci = R.qr * pzi + R.qi * pzr;
     ar += (double)vai.x * cr - (double)vai.y * ci;
     ai += (double)vai.x * ci + (double)vai.y * cr;
   }
@@ -133,14 +50,8 @@ __device__ __forceinline__ void bin_mode(const RowVals& R, int k, double fz, dou
   const double L2 = 1.5 * mu2 - 0.5;
   const double L4 = 4.375 * mu2 * mu2 - 3.75 * mu2 + 0.375;
   const double L3 = 2.5 * mu2 * mu - 1.5 * mu;
-  int b2 = 0;
-  bool in2d = false;
-  if (a.out2d) {
-    int mubin = (int)(((mu + 1.0) / 2.0) * (double)(float)a.nmu);
-    if (mubin >= a.nmu) mubin = a.nmu - 1;
-    b2 = kbin * a.nmu + mubin;
-    in2d = (b2 >= 0) && (b2 < NK * a.nmu);
-  }
+  const int b2 = G.b2;
+  const bool in2d = G.in2d;
   const double wc = NM * cor;
 #pragma unroll 3
   for (int s = 0; s < a.nspec; s++) {

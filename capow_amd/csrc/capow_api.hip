@@ -65,6 +65,14 @@ struct capow_ctx {
   unsigned int* h_errs = nullptr;
   double* h_bins = nullptr;
   bool cap_fixed = false;   // sorted buffer capacity fixed (batch); overflow is flagged
+  // backward buffers
+  double* bwd_G = nullptr;
+  size_t bwd_G_cap = 0;
+  double* bwd_g0 = nullptr;
+  void* bwd_out = nullptr;
+  size_t bwd_out_cap = 0;
+  void* bwd_w = nullptr;
+  size_t bwd_w_cap = 0;
   int fft_col_mul = 2;   // column batch multiplier of the y/x FFT passes (CAPOW_FFT_COL_MUL=1 halves it)
 };
 
@@ -387,6 +395,8 @@ static void release(capow_ctx* c) {
                 c->cursor, c->bsum, c->boff, c->partials, c->gmax, c->errs, c->stats,
                 c->tw[0], c->tw[1], c->tw[2], c->bin_tabs, c->bin_partials, c->bin_out};
   for (void* q : ps) if (q) hipFree(q);
+  void* bs[] = {c->bwd_G, c->bwd_g0, c->bwd_out, c->bwd_w};
+  for (void* q : bs) if (q) hipFree(q);
   void* hs[] = {c->h_stats, c->h_errs, c->h_bins};
   for (void* q : hs) if (q) hipHostFree(q);
 }
@@ -750,6 +760,88 @@ extern "C" int capow_kernel_times(const capow_ctx* c, char* names, int32_t names
 extern "C" int capow_reset_kernel_times(capow_ctx* c) {
   if (!c) return CAPOW_ERR_ARG;
   c->timer.reset();
+  return CAPOW_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Backward: dL/d(raw bins) -> dL/d(x, y, z, v_los), dL/dw.
+// ---------------------------------------------------------------------------
+extern "C" int capow_backward(capow_ctx* c, const double* dL_draw, void* grad_xyzv, void* grad_w,
+                              int32_t where) {
+  if (!c) return CAPOW_ERR_ARG;
+  if (!dL_draw || !grad_xyzv) return fail(c, CAPOW_ERR_ARG, "null gradient array");
+  if (where != CAPOW_HOST && where != CAPOW_DEVICE)
+    return fail(c, CAPOW_ERR_ARG, "where must be CAPOW_HOST or CAPOW_DEVICE");
+  if (c->state < 1) return fail(c, CAPOW_ERR_STATE, "capow_backward before capow_set_particles");
+  // the forward grids, recomputed (nothing is stored between the passes)
+  int rc = capow_assign(c, nullptr);
+  if (rc) return rc;
+  rc = capow_fft(c);
+  if (rc) return rc;
+  const size_t gbytes = (size_t)c->nspec * 5 * c->p.nk * 8;
+  rc = ensure(c, (void**)&c->bwd_G, &c->bwd_G_cap, gbytes);
+  if (rc) return rc;
+  if (!c->bwd_g0) HIPCHK(c, hipMalloc((void**)&c->bwd_g0, 8 * sizeof(double)));
+  HIPCHK(c, hipMemcpyAsync(c->bwd_G, dL_draw, gbytes, hipMemcpyHostToDevice, c->stream));
+  BinLaunch L;
+  L.stream = c->stream;
+  L.timer = &c->timer;
+  for (int s = 0; s < 4; s++) L.grid[s] = c->grid[s] ? c->grid[s] : c->grid[0];
+  L.dev = c->bin;
+  L.tabs = c->bin_tabs;
+  L.row_form = c->bin_row_form;
+  L.nwg = c->bin_nwg;
+  L.per_copy = c->bin_per_copy;
+  L.partials = c->bin_partials;
+  L.out = c->bin_out;
+  rc = run_bin_backward(L, c->bwd_G, c->bwd_g0, c->fp64);
+  if (rc) return fail(c, rc, std::string("binning backward failed: ") + hipGetErrorString(hipGetLastError()));
+  FftArgs f;
+  f.g = &c->g;
+  f.stream = c->stream;
+  f.timer = &c->timer;
+  f.ngrids = 0;
+  for (int s = 0; s < 4; s++)
+    if (c->grid[s]) f.grid[f.ngrids++] = c->grid[s];
+  for (int q = 0; q < 3; q++) {
+    f.plan[q] = c->plan[q];
+    f.tw[q] = c->tw[q];
+  }
+  f.col_mul = c->fft_col_mul;
+  rc = run_ifft(f, c->fp64);
+  c->state = 1;   // the grids now hold gradients
+  if (rc == CAPOW_ERR_UNSUPPORTED)
+    return fail(c, rc, "the backward needs register FFT sizes (16..1024 of 2^a 3^b with a split)");
+  if (rc) return fail(c, rc, std::string("inverse fft failed: ") + hipGetErrorString(hipGetLastError()));
+  GatherArgs ga;
+  ga.g = &c->g;
+  ga.stream = c->stream;
+  ga.timer = &c->timer;
+  ga.parts = c->parts;
+  ga.wts = c->wts;
+  ga.n = c->n;
+  for (int s = 0; s < 4; s++) ga.grid[s] = c->grid[s] ? c->grid[s] : c->grid[0];
+  ga.g0 = c->bwd_g0;
+  const size_t ob = (size_t)(c->n > 0 ? c->n : 1) * 4 * c->esz, wb = (size_t)(c->n > 0 ? c->n : 1) * c->esz;
+  if (where == CAPOW_DEVICE) {
+    ga.gout = grad_xyzv;
+    ga.gw = grad_w;
+  } else {
+    rc = ensure(c, &c->bwd_out, &c->bwd_out_cap, ob);
+    if (!rc && grad_w) rc = ensure(c, &c->bwd_w, &c->bwd_w_cap, wb);
+    if (rc) return rc;
+    ga.gout = c->bwd_out;
+    ga.gw = grad_w ? c->bwd_w : nullptr;
+  }
+  c->g.has_w = c->wts != nullptr;
+  rc = run_gather(ga, c->fp64);
+  if (rc) return fail(c, rc, std::string("gather failed: ") + hipGetErrorString(hipGetLastError()));
+  if (where == CAPOW_HOST && c->n > 0) {
+    HIPCHK(c, hipMemcpyAsync(grad_xyzv, c->bwd_out, (size_t)c->n * 4 * c->esz, hipMemcpyDeviceToHost, c->stream));
+    if (grad_w) HIPCHK(c, hipMemcpyAsync(grad_w, c->bwd_w, (size_t)c->n * c->esz, hipMemcpyDeviceToHost, c->stream));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->timer.collect();
   return CAPOW_OK;
 }
 

@@ -83,6 +83,7 @@ template <typename T>
 struct AxisOut {
   int first;    // first tap (unwrapped)
   T w[3];
+  T dw[3];      // d w / d s (scaled coordinate), zero for dropped taps
   bool ok;
 };
 
@@ -154,6 +155,7 @@ __device__ __forceinline__ AxisOut<T> axis_taps(T u, T lo, T hi, T d, T L, int n
   AxisOut<T> o;
   o.ok = ac.ok;
   o.w[0] = o.w[1] = o.w[2] = T(0);
+  o.dw[0] = o.dw[1] = o.dw[2] = T(0);
   const int c = ac.c;
   const T f = ac.s - (T)c;
   const bool periodic = (edge == CAPOW_EDGE_PERIODIC) || (los == 0);
@@ -162,24 +164,30 @@ __device__ __forceinline__ AxisOut<T> axis_taps(T u, T lo, T hi, T d, T L, int n
     const T wl = T(0.5) * (T(0.5) - f) * (T(0.5) - f);
     const T wc = T(0.75) - f * f;
     const T wh = T(0.5) * (T(0.5) + f) * (T(0.5) + f);
-    if (edge == CAPOW_EDGE_PERIODIC) {
-      o.w[0] = wl;
-      o.w[1] = wc;
-      o.w[2] = wh;
-      return o;
+    bool k0 = true, k1 = true, k2 = true;
+    if (edge != CAPOW_EDGE_PERIODIC) {
+      // The reference keeps the low tap only when its index is > 0 after the
+      // LOS = 0 wrap (-1 -> N-1): periodic axes drop it exactly when c == 1
+      // (x: c-1 = slice 0; y/z: iy-1 = 0), non-periodic axes whenever c <= 1.
+      // Non-periodic axes also drop taps at index >= N (x: dropped buffer
+      // slices; y, z: the < N guards).
+      k0 = (periodic ? (c != 1) : (c - 1 > 0)) && (periodic || c - 1 < n);
+      k1 = periodic || c < n;
+      k2 = periodic || c + 1 < n;
     }
-    // The reference keeps the low tap only when its index is > 0 after the
-    // LOS = 0 wrap (-1 -> N-1): periodic axes drop it exactly when c == 1
-    // (x: c-1 = slice 0; y/z: iy-1 = 0), non-periodic axes whenever c <= 1.
-    // Non-periodic axes also drop taps at index >= N (x: dropped buffer
-    // slices; y, z: the < N guards).
-    o.w[0] = ((periodic ? (c != 1) : (c - 1 > 0)) && (periodic || c - 1 < n)) ? wl : T(0);
-    o.w[1] = (periodic || c < n) ? wc : T(0);
-    o.w[2] = (periodic || c + 1 < n) ? wh : T(0);
+    o.w[0] = k0 ? wl : T(0);
+    o.w[1] = k1 ? wc : T(0);
+    o.w[2] = k2 ? wh : T(0);
+    o.dw[0] = k0 ? -(T(0.5) - f) : T(0);
+    o.dw[1] = k1 ? T(-2) * f : T(0);
+    o.dw[2] = k2 ? T(0.5) + f : T(0);
   } else if (ORDER == 2) {
     o.first = c;
-    o.w[0] = (periodic || c < n) ? T(1) - f : T(0);
-    o.w[1] = (periodic || c + 1 < n) ? f : T(0);
+    const bool k0 = periodic || c < n, k1 = periodic || c + 1 < n;
+    o.w[0] = k0 ? T(1) - f : T(0);
+    o.w[1] = k1 ? f : T(0);
+    o.dw[0] = k0 ? T(-1) : T(0);
+    o.dw[1] = k1 ? T(1) : T(0);
   } else {
     o.first = c;
     o.w[0] = (periodic || c < n) ? T(1) : T(0);

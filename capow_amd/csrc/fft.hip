@@ -321,7 +321,9 @@ __device__ __forceinline__ int tidx(int n1, int k2, int c) {
 
 // Columns: B consecutive complex columns, N = N1*N2 along a strided axis.
 // Threads: (N / E) per column * B; lane -> (c = lane % B, t = lane / B).
-// AX names the axis (1: y, 0: x) so that profiles tell the passes apart.
+// AX names the axis (1: y, 0: x) so that profiles tell the passes apart;
+// AX + 2 is the inverse transform (conjugated input and output: the
+// unnormalised sum with e^{+2 pi i nk/N}) used by the adjoint.
 template <typename T, int N1, int N2, int B, int AX>
 __global__ __launch_bounds__((N1 * N2 / clcm(N1, N2)) * B) void k_fft_cols2(
     T* __restrict__ g0, T* __restrict__ g1, T* __restrict__ g2, T* __restrict__ g3,
@@ -331,6 +333,7 @@ __global__ __launch_bounds__((N1 * N2 / clcm(N1, N2)) * B) void k_fft_cols2(
   constexpr int E = clcm(N1, N2);
   constexpr int TS = N / E;            // threads per sequence
   constexpr int G1 = E / N2, G2 = E / N1;
+  constexpr bool INV = AX >= 2;
   extern __shared__ unsigned char smem[];
   cx<T>* lds = reinterpret_cast<cx<T>*>(smem);
   T* gp = blockIdx.y == 0 ? g0 : blockIdx.y == 1 ? g1 : blockIdx.y == 2 ? g2 : g3;
@@ -349,7 +352,9 @@ __global__ __launch_bounds__((N1 * N2 / clcm(N1, N2)) * B) void k_fft_cols2(
 #pragma unroll
     for (int n2 = 0; n2 < N2; n2++) {
       const int n = (t + TS * g) + N1 * n2;
-      v[g * N2 + n2] = live ? col[(int64_t)n * astride] : mk<T>(T(0), T(0));
+      cx<T> q = live ? col[(int64_t)n * astride] : mk<T>(T(0), T(0));
+      if (INV) q.y = -q.y;
+      v[g * N2 + n2] = q;
     }
 #pragma unroll
   for (int g = 0; g < G1; g++) {
@@ -373,7 +378,8 @@ __global__ __launch_bounds__((N1 * N2 / clcm(N1, N2)) * B) void k_fft_cols2(
     RegDFT<T, N1>::run(a, o);
     if (live) {
 #pragma unroll
-      for (int k1 = 0; k1 < N1; k1++) col[(int64_t)(k2 + N2 * k1) * astride] = o[k1];
+      for (int k1 = 0; k1 < N1; k1++)
+        col[(int64_t)(k2 + N2 * k1) * astride] = INV ? mk<T>(o[k1].x, -o[k1].y) : o[k1];
     }
   }
 }
@@ -450,6 +456,87 @@ __global__ __launch_bounds__((N1 * N2 / clcm(N1, N2)) * B) void k_fft_rows2(
       r[k] = add<T>(Ev, mul<T>(twz[k], Ov));
     }
   }
+}
+
+// Adjoint of the z pass (r2c rows) for the backward of the binning: for a
+// gradient G[k], k = 0..M, on the stored half spectrum of a row, writes the
+// real row g[n] = Re sum_{k=0..M} G[k] e^{+2 pi i k n / NZ}, n < NZ = 2M.
+// With X[0] = Re G[0], X[M] = Re G[M] and X[k] = G[k]/2 otherwise, g is the
+// inverse real transform of the Hermitian spectrum X, computed as one
+// complex inverse FFT of length M of F[k] = Fe[k] + i Fo[k],
+//   Fe[k] = X[k] + conj(X[M-k]),  Fo[k] = (X[k] - conj(X[M-k])) e^{+2 pi i k/NZ},
+// whose real and imaginary parts are g[2m] and g[2m+1]; the inverse FFT is
+// the forward register FFT on conjugated input and output.
+template <typename T, int N1, int N2, int B>
+__global__ __launch_bounds__((N1 * N2 / clcm(N1, N2)) * B) void k_fft_rows2_adj(
+    T* __restrict__ g0, T* __restrict__ g1, T* __restrict__ g2, T* __restrict__ g3,
+    int64_t nrows, const cx<T>* __restrict__ twz, int64_t nblocks_x, int64_t rowlen) {
+  constexpr int M = N1 * N2;
+  constexpr int E = clcm(N1, N2);
+  constexpr int TS = M / E;
+  constexpr int G1 = E / N2, G2 = E / N1;
+  constexpr int RS = N1 * (N2 + 1);   // per-row LDS stride (>= M + 1)
+  extern __shared__ unsigned char smem[];
+  cx<T>* lds = reinterpret_cast<cx<T>*>(smem);
+  T* gp = blockIdx.y == 0 ? g0 : blockIdx.y == 1 ? g1 : blockIdx.y == 2 ? g2 : g3;
+  int64_t bid = blockIdx.x;
+  if ((nblocks_x & 7) == 0) bid = (bid & 7) * (nblocks_x >> 3) + (bid >> 3);
+  const int t = threadIdx.x % TS, c = threadIdx.x / TS;
+  const int64_t row = bid * B + c;
+  const bool live = row < nrows;
+  cx<T>* r = reinterpret_cast<cx<T>*>(gp + (live ? row : 0) * rowlen);
+  cx<T>* xr = lds + c * RS;
+  for (int k = t; k <= M; k += TS) xr[k] = live ? r[k] : mk<T>(T(0), T(0));
+  __syncthreads();
+  cx<T> v[E];
+#pragma unroll
+  for (int g = 0; g < G1; g++)
+#pragma unroll
+    for (int n2 = 0; n2 < N2; n2++) {
+      const int n = (t + TS * g) + N1 * n2;      // 0 .. M-1
+      cx<T> a = xr[n], b = xr[M - n];
+      a = (n == 0) ? mk<T>(a.x, T(0)) : mk<T>(T(0.5) * a.x, T(0.5) * a.y);
+      b = (n == 0) ? mk<T>(b.x, T(0)) : mk<T>(T(0.5) * b.x, T(0.5) * b.y);   // M - n == M iff n == 0
+      const cx<T> bc = mk<T>(b.x, -b.y);
+      const cx<T> fe = add<T>(a, bc);
+      const cx<T> w = mk<T>(twz[n].x, -twz[n].y);                           // e^{+2 pi i n/NZ}
+      const cx<T> fo = mul<T>(sub<T>(a, bc), w);
+      // F = Fe + i Fo, conjugated for the forward FFT
+      v[g * N2 + n2] = mk<T>(fe.x - fo.y, -(fe.y + fo.x));
+    }
+  __syncthreads();
+#pragma unroll
+  for (int g = 0; g < G1; g++) {
+    cx<T> o[N2];
+    RegDFT<T, N2>::run(&v[g * N2], o);
+    const int n1 = t + TS * g;
+#pragma unroll
+    for (int k2 = 0; k2 < N2; k2++) {
+      cx<T> y = (k2 == 0 || n1 == 0) ? o[k2] : mul<T>(o[k2], twz[2 * n1 * k2]);
+      lds[c * RS + n1 * (N2 + 1) + k2] = y;
+    }
+  }
+  __syncthreads();
+  cx<T> z[E];
+#pragma unroll
+  for (int h = 0; h < G2; h++) {
+    const int k2 = t + TS * h;
+    cx<T> a[N1];
+#pragma unroll
+    for (int n1 = 0; n1 < N1; n1++) a[n1] = lds[c * RS + n1 * (N2 + 1) + k2];
+    RegDFT<T, N1>::run(a, &z[h * N1]);
+  }
+  __syncthreads();
+  cx<T>* zr = lds + c * (M + 1);
+#pragma unroll
+  for (int h = 0; h < G2; h++) {
+    const int k2 = t + TS * h;
+#pragma unroll
+    for (int k1 = 0; k1 < N1; k1++) zr[k2 + N2 * k1] = z[h * N1 + k1];
+  }
+  __syncthreads();
+  if (live)
+    for (int m = t; m < M; m += TS) r[m] = mk<T>(zr[m].x, -zr[m].y);   // g[2m], g[2m+1]
 }
 
 // Sizes served by the register kernels: N -> (N1, N2).
@@ -626,6 +713,50 @@ static int run_fft_t(FftArgs& a) {
   }
   return hipGetLastError() == hipSuccess ? CAPOW_OK : CAPOW_ERR_HIP;
 }
+
+template <typename T, int N1, int N2>
+static void launch_rows2_adj(FftArgs& a, T** gp, int64_t nrows, const void* twz, int64_t rowlen) {
+  constexpr int M = N1 * N2, E = clcm(N1, N2), TS = M / E;
+  constexpr int B = rows_B<T, N1, N2>();
+  const size_t lds = (size_t)B * (M + N1) * sizeof(cx<T>);
+  const int64_t nbx = (nrows + B - 1) / B;
+  dim3 grid((unsigned)nbx, a.ngrids);
+  TIMED(a.timer, "ifft_z",
+        (k_fft_rows2_adj<T, N1, N2, B><<<grid, TS * B, lds, a.stream>>>(
+            gp[0], gp[1], gp[2], gp[3], nrows, (const cx<T>*)twz, nbx, rowlen)));
+}
+
+// Adjoint of run_fft on gradient grids: inverse x and y passes (conjugated
+// register kernels), then the z-pass adjoint.  Register sizes only.
+template <typename T>
+static int run_ifft_t(FftArgs& a) {
+  const Geom& g = *a.g;
+  T* gp[4];
+  for (int q = 0; q < 4; q++) gp[q] = q < a.ngrids ? (T*)a.grid[q] : (T*)a.grid[0];
+  const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+  const int nzh = nz / 2 + 1;
+  const int64_t rowc = g.rowc, nrows = (int64_t)nx * ny;
+  int n1, n2;
+  if (!reg_split(nx, &n1, &n2) || !reg_split(ny, &n1, &n2) || !reg_split(nz / 2, &n1, &n2))
+    return CAPOW_ERR_UNSUPPORTED;
+  const int64_t ncols = (int64_t)ny * rowc;
+  if (!try_cols2<T, 2>(a, "ifft_x", gp, nx, ncols, 0, ncols, 1, a.tw[0])) return CAPOW_ERR_UNSUPPORTED;
+  if (!try_cols2<T, 3>(a, "ifft_y", gp, ny, rowc, (int64_t)ny * rowc, (int64_t)nzh, nx, a.tw[1]))
+    return CAPOW_ERR_UNSUPPORTED;
+  reg_split(nz / 2, &n1, &n2);
+  bool done = false;
+#define X(A, B_)                                                   \
+  if (!done && n1 == A && n2 == B_) {                              \
+    launch_rows2_adj<T, A, B_>(a, gp, nrows, a.tw[2], g.rowlen);   \
+    done = true;                                                   \
+  }
+  CAPOW_FFT_SIZES(X)
+#undef X
+  if (!done) return CAPOW_ERR_UNSUPPORTED;
+  return hipGetLastError() == hipSuccess ? CAPOW_OK : CAPOW_ERR_HIP;
+}
+
+int run_ifft(FftArgs& a, bool fp64) { return fp64 ? run_ifft_t<double>(a) : run_ifft_t<float>(a); }
 
 int run_fft(FftArgs& a, bool fp64) { return fp64 ? run_fft_t<double>(a) : run_fft_t<float>(a); }
 

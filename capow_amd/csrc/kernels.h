@@ -125,6 +125,8 @@ struct FftArgs {
   int col_mul = 1;        // column batch multiplier of the y/x passes (1 or 2)
 };
 int run_fft(FftArgs& a, bool fp64);
+// adjoint of run_fft on gradient grids (register FFT sizes only)
+int run_ifft(FftArgs& a, bool fp64);
 
 enum { SPEC_AUTO_A = 0, SPEC_AUTO_B = 1, SPEC_CROSS_IM = 2, SPEC_CROSS_RE = 3 };
 
@@ -156,5 +158,23 @@ struct BinLaunch {
   double* out;            // per_copy
 };
 int run_bin(BinLaunch& L, bool fp64);
+
+// Backward of the binning: dL/draw (G: [n_spectra][5][NK]) -> gradients of
+// the k-space grids, in place; g0: the k = 0 gradients (8 doubles).
+int run_bin_backward(const BinLaunch& L, const double* G, double* g0, bool fp64);
+
+struct GatherArgs {
+  const Geom* g;
+  hipStream_t stream;
+  Timer* timer;
+  const void* parts;      // n x 4
+  const void* wts;        // n or null
+  int64_t n;
+  const void* grid[4];    // real-space gradient grids (A, A_int, B, B_int)
+  const double* g0;       // k = 0 gradients from the binning backward
+  void* gout;             // n x 4: dL/dx, dL/dy, dL/dz, dL/dv_los
+  void* gw;               // n: dL/dw (or null)
+};
+int run_gather(const GatherArgs& a, bool fp64);
 
 }  // namespace capow

@@ -185,6 +185,22 @@ int capow_power(capow_ctx *ctx, capow_assign_stats *stats, capow_bins *out);
 /* Number of spectra capow_bin produces and their reference Momentum codes. */
 int capow_spectra(const capow_params *p, int32_t *n_spectra, int32_t codes[3]);
 
+/* ---- adjoint ------------------------------------------------------------ */
+/* No counterpart in the reference (it computes no derivatives).  Given
+ * dL_draw = dL/d(raw bins) in capow_bins.pk's layout ([n_spectra][5][nk],
+ * host array; entries of multipoles not computed are ignored), recomputes
+ * the assignment and FFT of the particles last set with capow_set_particles
+ * and writes dL/d(x, y, z, v_los) as n x 4 rows into grad_xyzv and dL/dw
+ * into grad_w (NULL: not wanted), both in the context precision and on the
+ * host or the device (`where`).  The gradient covers the deposit (fields
+ * w v^a, RSD shift of the line-of-sight coordinate), the mean subtraction,
+ * the r2c FFT and the binning; the host-side normalisation (shot noise,
+ * N^2/V) is the caller's to differentiate.  2-D bins carry no gradient.
+ * Register FFT sizes only (CAPOW_ERR_UNSUPPORTED otherwise).  Afterwards the
+ * grids hold gradients (capow_assign must run again before capow_fft). */
+int capow_backward(capow_ctx *ctx, const double *dL_draw, void *grad_xyzv,
+                   void *grad_w, int32_t where);
+
 /* ---- batches of catalogues (mock runs) --------------------------------- */
 /* No counterpart in the reference, which measures one catalogue per
  * process run (src/main.c:28-298); a batch runs many catalogues through the

@@ -128,6 +128,50 @@ def run_batch(args):
     return out
 
 
+def run_adjoint(args):
+    """Gradient configuration: 16,777,216 Zel'dovich particles (256^3
+    lattice, L = 1000 Mpc/h), 512^3 grid, TSC + interlacing, density P_0,2,4,
+    FP32 form: one forward (capow_power) and one backward (capow_backward:
+    assignment and FFT recomputed, binning backward, inverse FFT, gather),
+    G = dL/d(raw bins) fixed."""
+    import torch
+    from capow_amd import api, mocks
+    Lb, n = 1000.0, 512
+    parts = mocks.zeldovich_torch(256, Lb, seed=1, device="cuda", dtype=torch.float32)
+    w = torch.ones(parts.shape[0], device="cuda", dtype=torch.float32)
+    p = api.make_params(nx=n, ny=n, nz=n, xmax=Lb, ymax=Lb, zmax=Lb, interp_order=3,
+                        do_interlacing=1, momentum=0, nk=160, maxk=1.6, precision=api.FP32)
+    ps = api.PowerSpectrum(p)
+    ps.set_particles(parts, w)
+    G = np.random.default_rng(1).normal(size=(1, 5, 160))
+    ps.power()
+    ps.backward(G)                     # warmup
+    torch.cuda.synchronize()
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ps.set_particles(parts, w)
+        ps.power()
+    torch.cuda.synchronize()
+    fwd = (time.perf_counter() - t0) * 1e3 / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ps.set_particles(parts, w)
+        gx, gw = ps.backward(G)
+    torch.cuda.synchronize()
+    bwd = (time.perf_counter() - t0) * 1e3 / reps
+    ok = bool(torch.isfinite(gx).all().item() and torch.isfinite(gw).all().item())
+    ps.close()
+    del parts, w, gx, gw
+    torch.cuda.empty_cache()
+    out = dict(workload="16,777,216 Zel'dovich particles, 512^3, TSC+interlacing, P_0,2,4, FP32",
+               forward_ms=round(fwd, 3), backward_ms=round(bwd, 3),
+               backward_includes="assignment + FFT recomputed, binning backward, inverse FFT, gather",
+               gradients_finite=ok)
+    log("adjoint: forward %.2f ms, backward %.2f ms" % (fwd, bwd))
+    return out
+
+
 def profile_pass(parts, geom):
     """One FP32 step of the workload, then a hipMemcpy of COPY_BYTES (the
     counter calibration), for rocprofv3 --pmc runs."""
@@ -222,6 +266,7 @@ def main():
     ap.add_argument("--lattice", type=int, default=512)
     ap.add_argument("--mocks", type=int, default=512, help="batched mocks (0: skip)")
     ap.add_argument("--mock-streams", type=int, default=4)
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--profile-pass", action="store_true",
                     help="for rocprofv3 counter runs: one FP32 step and a %d-byte calibration "
                          "copy, no timing" % COPY_BYTES)
@@ -294,6 +339,7 @@ def main():
     del parts32
     torch.cuda.empty_cache()
     batch = run_batch(args) if args.mocks > 0 else None
+    adjoint = None if args.no_adjoint else run_adjoint(args)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -334,6 +380,7 @@ def main():
                           for k, v in f64["kernels"].items()} if f64 else None),
         "roofline": roofline,
         "batched_mocks": batch,
+        "adjoint": adjoint,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -177,13 +177,101 @@ __global__ __launch_bounds__(512) void k_bin(const T* __restrict__ gA, const T* 
   }
 }
 
+// Branch-free part of one mode for the common binning (linear k bins, no
+// 2-D bins): bin index (without the near-integer exact fallback), whether
+// the mode is binned, NM * cor * power of each spectrum, L_2, L_4 and mu.  Written without early exits so that the
+// compiler can interleave the long FP64 chains of several modes.
+struct ModeVals {
+  int kbin;
+  bool ok, near;
+  double np[3];
+  double L2, L4, mu;
+};
+
+template <typename T>
+__device__ __forceinline__ ModeVals mode_vals(const RowVals& R, int k, double fz, double corz,
+                                              double pzr, double pzi, typename CT<T>::t va,
+                                              typename CT<T>::t vai, typename CT<T>::t vb,
+                                              typename CT<T>::t vbi, const BinArgsDev& a) {
+  ModeVals V;
+  const double twopi = 2.0 * 3.14159265358979323846;
+  const double fktot = dmul(twopi, sqrt(dadd(R.fxy, dmul(fz, fz))));
+  const double q = (double)__double2float_rn(fktot - a.mink) * a.inv_binwidth;
+  const double tq = trunc(q);
+  const double fr = q - tq;
+  V.near = fabs(fr) < 1e-9 || fabs(fr) > 1.0 - 1e-9;
+  V.kbin = (int)tq;
+  V.ok = (V.kbin >= 0) && (V.kbin < a.nk) && (fktot <= a.min_nyq);
+  const double fl = a.los == 1 ? R.fx : (a.los == 2 ? R.fy : fz);
+  double rk = __builtin_amdgcn_rcp(fktot > 0 ? fktot : 1.0);
+  rk = rk * (2.0 - fktot * rk);
+  rk = rk * (2.0 - fktot * rk);
+  const double mu = fktot > 0 ? (twopi * fl) * rk : 0.0;
+  const bool k0 = k == 0;
+  const bool nm1 = k0 && ((R.j == 0) ? ((R.i != 0) && (R.i != a.nx / 2)) : (R.i != a.nx / 2));
+  const double wc = (nm1 ? 1.0 : 2.0) * (R.corxy * corz);
+  double cr = 1.0, ci = 0.0;
+  double ar = (double)va.x, ai = (double)va.y, br = 0, bi = 0;
+  if (a.interlace) {
+    cr = R.qr * pzr - R.qi * pzi;
+    ci = R.qr * pzi + R.qi * pzr;
+    ar += (double)vai.x * cr - (double)vai.y * ci;
+    ai += (double)vai.x * ci + (double)vai.y * cr;
+  }
+  if (a.hasB) {
+    br = (double)vb.x;
+    bi = (double)vb.y;
+    if (a.interlace) {
+      br += (double)vbi.x * cr - (double)vbi.y * ci;
+      bi += (double)vbi.x * ci + (double)vbi.y * cr;
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < 3; s++) {
+    double p;
+    switch (a.kind[s]) {
+      case SPEC_AUTO_A: p = ar * ar + ai * ai; break;
+      case SPEC_AUTO_B: p = br * br + bi * bi; break;
+      case SPEC_CROSS_IM: p = ar * bi - ai * br; break;
+      default: p = ar * br + ai * bi; break;
+    }
+    V.np[s] = wc * p;
+  }
+  const double mu2 = mu * mu;
+  V.mu = mu;
+  V.L2 = 1.5 * mu2 - 0.5;
+  V.L4 = 4.375 * mu2 * mu2 - 3.75 * mu2 + 0.375;
+  return V;
+}
+
+// Accumulates one evaluated mode into this wave's bins.
+__device__ __forceinline__ void mode_add(const ModeVals& V, const BinArgsDev& a, double* accS,
+                                         double* cnt, int nl, double nmw) {
+  const int NK = a.nk;
+#pragma unroll
+  for (int s = 0; s < 3; s++) {
+    if (s >= a.nspec) break;
+    const double np = V.np[s];
+    double* b = accS + s * nl * NK + V.kbin;
+    atomicAdd(&b[0], np);
+    atomicAdd(&b[NK], V.L2 * np);
+    atomicAdd(&b[2 * NK], V.L4 * np);
+    if (a.odd) {
+      const double L3 = 2.5 * V.mu * V.mu * V.mu - 1.5 * V.mu;
+      atomicAdd(&b[3 * NK], V.mu * np);
+      atomicAdd(&b[4 * NK], L3 * np);
+    }
+  }
+  atomicAdd(&cnt[V.kbin], nmw);
+}
+
 // Row form.  A wave takes batches of 64 consecutive (i, j) rows; for each
 // row the lanes cover kz = lane + 64u, u < NUF (coalesced loads), with the
 // row's x/y quantities wave-uniform and the z tables in LDS.  The rest of
 // each row, kz in [64 NUF, NZ/2+1), is binned once per batch with lane l on
 // row l of the batch.  PREF: the next row's loads are issued before the
 // current row is binned.
-template <typename T, int NUF, bool PREF>
+template <typename T, int NUF, bool PREF, bool FAST>
 __global__ __launch_bounds__(512) void k_bin_rows(const T* __restrict__ gA,
                                                   const T* __restrict__ gAi,
                                                   const T* __restrict__ gB,
@@ -249,12 +337,50 @@ __global__ __launch_bounds__(512) void k_bin_rows(const T* __restrict__ gA,
         if (PREF && rr + 1 < nr) load(row + 1, na, nai, nb, nbi);
         const int i = (int)(row / a.ny), j = (int)(row - (int64_t)i * a.ny);
         const RowVals R = row_vals(i, j, tabs, a);
+        if (FAST) {
+          // groups of modes evaluated branch-free, then accumulated
+          // modes per group: as many independent chains as the registers allow
+          constexpr int GRP = NUF < 2 ? NUF : (NUF >= 8 ? 2 : (NUF < 4 ? NUF : 4));
 #pragma unroll
-        for (int u = 0; u < NUF; u++) {
-          const int k = lane + 64 * u;
-          const double4 z = *reinterpret_cast<const double4*>(zt + 4 * k);
-          bin_mode<T>(R, k, z.x, z.y, z.z, z.w, va[u], vai[u], vb[u], vbi[u], a, accS, acc2, cnt,
-                      cnt2, nl, n2d);
+          for (int u0 = 0; u0 < NUF; u0 += GRP) {
+            ModeVals V[GRP];
+#pragma unroll
+            for (int q = 0; q < GRP; q++) {
+              const int u = u0 + q;
+              const int k = lane + 64 * u;
+              const double4 z = *reinterpret_cast<const double4*>(zt + 4 * k);
+              V[q] = mode_vals<T>(R, k, z.x, z.y, z.z, z.w, va[u], vai[u], vb[u], vbi[u], a);
+            }
+            bool anynear = false;
+#pragma unroll
+            for (int q = 0; q < GRP; q++) anynear |= V[q].near;
+            if (__any(anynear)) {
+#pragma unroll
+              for (int q = 0; q < GRP; q++)
+                if (V[q].near) {
+                  // the exact division where the quotient is within 1e-9 of
+                  // an integer (the bin index of the reference's division)
+                  const double fz = zt[4 * (lane + 64 * (u0 + q))];
+                  const double fk = dmul(2.0 * 3.14159265358979323846, sqrt(dadd(R.fxy, dmul(fz, fz))));
+                  V[q].kbin = (int)ddiv((double)__double2float_rn(fk - a.mink), a.binwidth);
+                  V[q].ok = (fk <= a.min_nyq) && V[q].kbin >= 0 && V[q].kbin < a.nk;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < GRP; q++) {
+              const int k = lane + 64 * (u0 + q);
+              const bool nm1 = k == 0 && ((j == 0) ? ((i != 0) && (i != a.nx / 2)) : (i != a.nx / 2));
+              if (V[q].ok) mode_add(V[q], a, accS, cnt, nl, nm1 ? 1.0 : 2.0);
+            }
+          }
+        } else {
+#pragma unroll
+          for (int u = 0; u < NUF; u++) {
+            const int k = lane + 64 * u;
+            const double4 z = *reinterpret_cast<const double4*>(zt + 4 * k);
+            bin_mode<T>(R, k, z.x, z.y, z.z, z.w, va[u], vai[u], vb[u], vbi[u], a, accS, acc2, cnt,
+                        cnt2, nl, n2d);
+          }
         }
         if (rr + 1 < nr) {
           if (PREF) {
@@ -311,12 +437,20 @@ static void launch_rows(BinLaunch& L, size_t lds) {
   // FP64 rows hold at most 8 slots in registers (16 would spill)
   constexpr int NU = (sizeof(T) == 8 && NUF > 8) ? 8 : NUF;
   constexpr bool PREF = sizeof(T) == 4 && NU <= 8;
-  const void* fn = (const void*)k_bin_rows<T, NU, PREF>;
+  // the branch-free path covers linear k bins without 2-D bins
+  const bool fast = !L.dev.logbins && !L.dev.out2d;
+  const void* fn = fast ? (const void*)k_bin_rows<T, NU, PREF, true> : (const void*)k_bin_rows<T, NU, PREF, false>;
   if (lds > 64 * 1024) hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  TIMED(L.timer, "bin",
-        (k_bin_rows<T, NU, PREF><<<L.nwg, 64 * L.dev.copies, lds, L.stream>>>(
-            (const T*)L.grid[0], (const T*)L.grid[1], (const T*)L.grid[2], (const T*)L.grid[3],
-            L.tabs, L.dev, L.partials)));
+  if (fast)
+    TIMED(L.timer, "bin",
+          (k_bin_rows<T, NU, PREF, true><<<L.nwg, 64 * L.dev.copies, lds, L.stream>>>(
+              (const T*)L.grid[0], (const T*)L.grid[1], (const T*)L.grid[2], (const T*)L.grid[3],
+              L.tabs, L.dev, L.partials)));
+  else
+    TIMED(L.timer, "bin",
+          (k_bin_rows<T, NU, PREF, false><<<L.nwg, 64 * L.dev.copies, lds, L.stream>>>(
+              (const T*)L.grid[0], (const T*)L.grid[1], (const T*)L.grid[2], (const T*)L.grid[3],
+              L.tabs, L.dev, L.partials)));
 }
 
 template <typename T>

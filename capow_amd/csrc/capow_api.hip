@@ -25,6 +25,11 @@ struct capow_ctx {
   int state = 0;  // 0 created, 1 particles, 2 assigned, 3 transformed
   // grids: slots 0 A, 1 A_int, 2 B, 3 B_int
   void* grid[4] = {nullptr, nullptr, nullptr, nullptr};
+  // out-of-place FFT: a second buffer per grid (swapped with the grid after
+  // the transform); null when the in-place passes are used
+  void* scratch[4] = {nullptr, nullptr, nullptr, nullptr};
+  int64_t fft_px = 0, fft_py = 0;   // row pitches of the intermediate layouts
+  int fft_rows_mul = 0;             // CAPOW_FFT_ROWS_MUL=1|2 forces 64|128-byte write runs
   int64_t grid_elems = 0;
   // particles
   const void* parts = nullptr;
@@ -260,9 +265,42 @@ static int setup(capow_ctx* c) {
   // are padded to rowlen reals
   c->grid_elems = (int64_t)p.nx * p.ny * 2 * g.nzh;
   const bool hasB = g.fb >= 0;
+  // intermediate layouts of the out-of-place FFT: [x][kz][y] and [kz][ky][x]
+  // with 128-byte row pitches; every buffer holds the largest layout
+  {
+    const int64_t al = c->fp64 ? 8 : 16;
+    c->fft_py = (p.ny + al - 1) / al * al;
+    c->fft_px = (p.nx + al - 1) / al * al;
+  }
+  int64_t cap_c = (int64_t)p.nx * p.ny * g.rowc;
+  const int64_t l1 = (int64_t)p.nx * g.nzh * c->fft_py, l2 = (int64_t)g.nzh * p.ny * c->fft_px;
+  if (l1 > cap_c) cap_c = l1;
+  if (l2 > cap_c) cap_c = l2;
+  const size_t gbytes = (size_t)cap_c * 2 * c->esz;
+  int ngr = 0;
   for (int s = 0; s < 4; s++) {
     bool need = (s == 0) || (s == 1 && g.interlace) || (s == 2 && hasB) || (s == 3 && hasB && g.interlace);
-    if (need) HIPCHK(c, hipMalloc(&c->grid[s], (size_t)p.nx * p.ny * g.rowlen * c->esz));
+    if (need) {
+      HIPCHK(c, hipMalloc(&c->grid[s], gbytes));
+      ngr++;
+    }
+  }
+  bool want_scratch = true;
+  if (const char* e = getenv("CAPOW_FFT_TRANSPOSED")) want_scratch = atoi(e) != 0;
+  if (want_scratch) {
+    // only with room to spare: the second set of grids plus 16 GB
+    size_t fre = 0, tot = 0;
+    if (hipMemGetInfo(&fre, &tot) == hipSuccess && fre > (size_t)ngr * gbytes + ((size_t)16 << 30)) {
+      for (int s = 0; s < 4; s++) {
+        if (!c->grid[s]) continue;
+        if (hipMalloc(&c->scratch[s], gbytes) != hipSuccess) {
+          hipGetLastError();
+          for (int q = 0; q < 4; q++)
+            if (c->scratch[q]) { hipFree(c->scratch[q]); c->scratch[q] = nullptr; }
+          break;
+        }
+      }
+    }
   }
   // sort buffers
   HIPCHK(c, hipMalloc((void**)&c->counts, (size_t)g.nbuckets * 4));
@@ -391,6 +429,7 @@ static int setup(capow_ctx* c) {
 
 static void release(capow_ctx* c) {
   for (int s = 0; s < 4; s++) if (c->grid[s]) hipFree(c->grid[s]);
+  for (int s = 0; s < 4; s++) if (c->scratch[s]) hipFree(c->scratch[s]);
   void* ps[] = {c->own_parts, c->own_w, c->sorted, c->sorted_w, c->counts, c->offsets,
                 c->cursor, c->bsum, c->boff, c->partials, c->gmax, c->errs, c->stats,
                 c->tw[0], c->tw[1], c->tw[2], c->bin_tabs, c->bin_partials, c->bin_out};
@@ -410,6 +449,7 @@ extern "C" int capow_create(const capow_params* p, capow_ctx** out) {
   capow_ctx* c = new capow_ctx();
   c->p = *p;
   if (const char* e = getenv("CAPOW_FFT_COL_MUL")) c->fft_col_mul = atoi(e) == 1 ? 1 : 2;
+  if (const char* e = getenv("CAPOW_FFT_ROWS_MUL")) c->fft_rows_mul = atoi(e) == 2 ? 2 : (atoi(e) == 1 ? 1 : 0);
   rc = setup(c);
   if (rc) {
     g_create_err = c->err;
@@ -620,8 +660,21 @@ extern "C" int capow_fft(capow_ctx* c) {
     a.tw[q] = c->tw[q];
   }
   a.col_mul = c->fft_col_mul;
+  int ns = 0;
+  for (int s = 0; s < 4; s++)
+    if (c->grid[s]) a.scratch[ns++] = c->scratch[s];
+  if (!c->scratch[0]) a.scratch[0] = nullptr;
+  a.px = c->fft_px;
+  a.py = c->fft_py;
+  a.rows_mul = c->fft_rows_mul;
   int rc = run_fft(a, c->fp64);
   if (rc) return fail(c, rc, std::string("fft launch failed: ") + hipGetErrorString(hipGetLastError()));
+  if (a.swapped)
+    for (int s = 0; s < 4; s++) {
+      void* t = c->grid[s];
+      c->grid[s] = c->scratch[s];
+      c->scratch[s] = t;
+    }
   c->state = 3;
   return CAPOW_OK;
 }

@@ -539,6 +539,105 @@ __global__ __launch_bounds__((N1 * N2 / clcm(N1, N2)) * B) void k_fft_rows2_adj(
     for (int m = t; m < M; m += TS) r[m] = mk<T>(zr[m].x, -zr[m].y);   // g[2m], g[2m+1]
 }
 
+// Row pass with a transposed write (the three passes of the out-of-place
+// pipeline).  A workgroup transforms B rows that are B consecutive values of
+// an "inner" index at one "outer" index:
+//   input row (outer, inner) at in + outer*in_so + inner*in_sb (complex),
+//     contiguous, N = N1*N2 complex values (R2C: 2N reals, an r2c of length
+//     2N giving N+1 outputs by the even/odd split);
+//   output k of that row at out + outer*out_so + k*out_sk + inner,
+// so every write is a run of B consecutive complex values (the transpose of
+// the B rows, staged in LDS).  Every read is a whole row.
+template <typename T, int N1, int N2, int B, bool R2C>
+__global__ __launch_bounds__((N1 * N2 / clcm(N1, N2)) * B) void k_fft_rows_t(
+    const T* __restrict__ i0, const T* __restrict__ i1, const T* __restrict__ i2,
+    const T* __restrict__ i3, T* __restrict__ o0, T* __restrict__ o1, T* __restrict__ o2,
+    T* __restrict__ o3, int64_t in_so, int64_t in_sb, int64_t out_so, int64_t out_sk,
+    int64_t n_inner, int64_t inner_groups, const cx<T>* __restrict__ tw, int64_t nblocks_x) {
+  constexpr int N = N1 * N2;
+  constexpr int E = clcm(N1, N2);
+  constexpr int TS = N / E;
+  constexpr int G1 = E / N2, G2 = E / N1;
+  constexpr int RS = N1 * (N2 + 1);   // per-row LDS stride of the transpose (>= N + 1)
+  constexpr int K = R2C ? N + 1 : N;  // outputs per row
+  extern __shared__ unsigned char smem[];
+  cx<T>* lds = reinterpret_cast<cx<T>*>(smem);
+  const int gsel = blockIdx.y;
+  const T* gin = gsel == 0 ? i0 : gsel == 1 ? i1 : gsel == 2 ? i2 : i3;
+  T* gout = gsel == 0 ? o0 : gsel == 1 ? o1 : gsel == 2 ? o2 : o3;
+  int64_t bid = blockIdx.x;
+  if ((nblocks_x & 7) == 0) bid = (bid & 7) * (nblocks_x >> 3) + (bid >> 3);
+  const int64_t outer = bid / inner_groups;
+  const int64_t inner0 = (bid - outer * inner_groups) * B;
+  const int t = threadIdx.x % TS, c = threadIdx.x / TS;
+  const bool live = inner0 + c < n_inner;
+  const cx<T>* row = reinterpret_cast<const cx<T>*>(gin) + outer * in_so + (live ? inner0 + c : 0) * in_sb;
+  cx<T> v[E];
+#pragma unroll
+  for (int g = 0; g < G1; g++)
+#pragma unroll
+    for (int n2 = 0; n2 < N2; n2++) {
+      const int n = (t + TS * g) + N1 * n2;
+      v[g * N2 + n2] = live ? row[n] : mk<T>(T(0), T(0));
+    }
+#pragma unroll
+  for (int g = 0; g < G1; g++) {
+    cx<T> o[N2];
+    RegDFT<T, N2>::run(&v[g * N2], o);
+    const int n1 = t + TS * g;
+#pragma unroll
+    for (int k2 = 0; k2 < N2; k2++) {
+      // R2C: the 2N-point table, step 2; C2C: the N-point table
+      cx<T> y = (k2 == 0 || n1 == 0) ? o[k2] : mul<T>(o[k2], tw[(R2C ? 2 : 1) * n1 * k2]);
+      lds[c * RS + n1 * (N2 + 1) + k2] = y;
+    }
+  }
+  __syncthreads();
+  cx<T> z[E];
+#pragma unroll
+  for (int h = 0; h < G2; h++) {
+    const int k2 = t + TS * h;
+    cx<T> a[N1];
+#pragma unroll
+    for (int n1 = 0; n1 < N1; n1++) a[n1] = lds[c * RS + n1 * (N2 + 1) + k2];
+    RegDFT<T, N1>::run(a, &z[h * N1]);
+  }
+  __syncthreads();
+  // natural order, row stride N + 1
+  cx<T>* zr = lds + c * (N + 1);
+#pragma unroll
+  for (int h = 0; h < G2; h++) {
+    const int k2 = t + TS * h;
+#pragma unroll
+    for (int k1 = 0; k1 < N1; k1++) zr[k2 + N2 * k1] = z[h * N1 + k1];
+  }
+  __syncthreads();
+  if (R2C) {
+    // even/odd split X[k] = E[k] + w^k O[k], k = 0..N (w = e^{-2 pi i/2N}):
+    // X[k] and X[N-k] need only z[k] and z[N-k] (z[N] = z[0]), so one
+    // thread computes the pair and overwrites both in place
+    for (int kk = t; kk <= N / 2; kk += TS) {
+      const cx<T> zk = zr[kk];
+      const cx<T> zm = zr[kk == 0 ? 0 : N - kk];
+      const cx<T> Ev = mk<T>(T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y));
+      const cx<T> Ov = mk<T>(T(0.5) * (zk.y + zm.y), -T(0.5) * (zk.x - zm.x));
+      const cx<T> Em = mk<T>(T(0.5) * (zm.x + zk.x), T(0.5) * (zm.y - zk.y));
+      const cx<T> Om = mk<T>(T(0.5) * (zm.y + zk.y), -T(0.5) * (zm.x - zk.x));
+      const cx<T> xk = add<T>(Ev, mul<T>(tw[kk], Ov));
+      const cx<T> xm = add<T>(Em, mul<T>(tw[N - kk], Om));
+      zr[kk] = xk;
+      zr[N - kk] = xm;   // kk == 0: X[N]; kk == N/2: the same value twice
+    }
+    __syncthreads();
+  }
+  // transposed write: runs of B consecutive inner values per output k
+  cx<T>* ob = reinterpret_cast<cx<T>*>(gout) + outer * out_so + inner0;
+  for (int q = threadIdx.x; q < K * B; q += TS * B) {
+    const int b = q % B, kk = q / B;
+    if (inner0 + b < n_inner) ob[(int64_t)kk * out_sk + b] = lds[b * (N + 1) + kk];
+  }
+}
+
 // Sizes served by the register kernels: N -> (N1, N2).
 static bool reg_split(int N, int* n1, int* n2) {
   switch (N) {
@@ -670,10 +769,91 @@ static bool try_rows2(FftArgs& a, T** gp, int M, int64_t nrows, const void* twz,
   return false;
 }
 
+template <typename T, int N1, int N2, bool R2C, int MUL>
+static void launch_rows_t(FftArgs& a, const char* name, T* const* in, T* const* out, int64_t n_outer,
+                          int64_t n_inner, int64_t in_so, int64_t in_sb, int64_t out_so,
+                          int64_t out_sk, const void* tw) {
+  constexpr int N = N1 * N2, E = clcm(N1, N2), TS = N / E;
+  constexpr int B = (sizeof(T) == 4 ? 8 : 4) * MUL;   // 64- (MUL 2: 128-) byte write runs
+  const size_t lds = (size_t)B * (N + N1) * sizeof(cx<T>);
+  if (lds > 64 * 1024)
+    hipFuncSetAttribute((const void*)k_fft_rows_t<T, N1, N2, B, R2C>,
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  const int64_t groups = (n_inner + B - 1) / B;
+  const int64_t nbx = n_outer * groups;
+  dim3 grid((unsigned)nbx, a.ngrids);
+  TIMED(a.timer, name,
+        (k_fft_rows_t<T, N1, N2, B, R2C><<<grid, TS * B, lds, a.stream>>>(
+            in[0], in[1], in[2], in[3], out[0], out[1], out[2], out[3], in_so, in_sb, out_so,
+            out_sk, n_inner, groups, (const cx<T>*)tw, nbx)));
+}
+
+template <typename T, bool R2C>
+static bool try_rows_t(FftArgs& a, const char* name, int N, T* const* in, T* const* out,
+                       int64_t n_outer, int64_t n_inner, int64_t in_so, int64_t in_sb,
+                       int64_t out_so, int64_t out_sk, const void* tw, int mul) {
+  int n1, n2;
+  if (!reg_split(N, &n1, &n2)) return false;
+#define X(A, B_)                                                                               \
+  if (n1 == A && n2 == B_) {                                                                    \
+    if (mul == 2 && (size_t)8 * (sizeof(T) == 4 ? 8 : 4) * 2 * (A * B_ + A) <= 160 * 1024)     \
+      launch_rows_t<T, A, B_, R2C, 2>(a, name, in, out, n_outer, n_inner, in_so, in_sb, out_so,  \
+                                      out_sk, tw);                                             \
+    else                                                                                       \
+      launch_rows_t<T, A, B_, R2C, 1>(a, name, in, out, n_outer, n_inner, in_so, in_sb, out_so,  \
+                                      out_sk, tw);                                             \
+    return true;                                                                               \
+  }
+  CAPOW_FFT_SIZES(X)
+#undef X
+  return false;
+}
+
+// Out-of-place pipeline: every pass reads whole rows and writes 64-byte
+// runs of the next pass's row axis.
+//   z (r2c): grid [x][y][z]   -> scratch [x][kz][y]   (row pitch py)
+//   y (c2c): scratch          -> grid    [kz][ky][x]  (row pitch px)
+//   x (c2c): grid             -> scratch [kx][ky][kz] (row pitch rowc)
+// The result is in the scratch buffers; the caller swaps them with the grids.
+template <typename T>
+static bool run_fft_transposed(FftArgs& a) {
+  const Geom& g = *a.g;
+  const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+  const int64_t nzh = nz / 2 + 1, rowc = g.rowc, px = a.px, py = a.py;
+  int n1, n2;
+  if (!reg_split(nz / 2, &n1, &n2) || !reg_split(ny, &n1, &n2) || !reg_split(nx, &n1, &n2))
+    return false;
+  T* G[4];
+  T* S[4];
+  for (int q = 0; q < 4; q++) {
+    G[q] = q < a.ngrids ? (T*)a.grid[q] : (T*)a.grid[0];
+    S[q] = q < a.ngrids ? (T*)a.scratch[q] : (T*)a.scratch[0];
+  }
+  // write runs: 128 bytes for the z and x passes, 64 for y (measured best
+  // at 1024^3 in both precisions); rows_mul overrides all three
+  const int mz = a.rows_mul ? a.rows_mul : 2, my = a.rows_mul ? a.rows_mul : 1,
+            mx = a.rows_mul ? a.rows_mul : 2;
+  if (!try_rows_t<T, true>(a, "fft_z", nz / 2, G, S, nx, ny, (int64_t)ny * rowc, rowc,
+                           nzh * py, py, a.tw[2], mz))
+    return false;
+  if (!try_rows_t<T, false>(a, "fft_y", ny, S, G, nzh, nx, py, nzh * py, (int64_t)ny * px, px,
+                            a.tw[1], my))
+    return false;
+  if (!try_rows_t<T, false>(a, "fft_x", nx, G, S, ny, nzh, px, (int64_t)ny * px, rowc,
+                            (int64_t)ny * rowc, a.tw[0], mx))
+    return false;
+  return true;
+}
+
 template <typename T>
 static int run_fft_t(FftArgs& a) {
   const Geom& g = *a.g;
   hipStream_t st = a.stream;
+  a.swapped = false;
+  if (a.fast && a.scratch[0] && run_fft_transposed<T>(a)) {
+    a.swapped = true;
+    return hipGetLastError() == hipSuccess ? CAPOW_OK : CAPOW_ERR_HIP;
+  }
   T* gp[4];
   for (int q = 0; q < 4; q++) gp[q] = q < a.ngrids ? (T*)a.grid[q] : (T*)a.grid[0];
   const int nx = g.n[0], ny = g.n[1], nz = g.n[2];

@@ -123,6 +123,13 @@ struct FftArgs {
   const void* tw[3];      // x: NX entries, y: NY, z: NZ (exp(-2 pi i j/N))
   bool fast = true;       // register two-level kernels where the size allows
   int col_mul = 1;        // column batch multiplier of the y/x passes (1 or 2)
+  // out-of-place pipeline (null: in place): scratch grids of the same
+  // capacity, row pitches of the intermediate layouts; swapped = the result
+  // is in the scratch grids
+  void* scratch[4] = {nullptr, nullptr, nullptr, nullptr};
+  int64_t px = 0, py = 0;
+  bool swapped = false;
+  int rows_mul = 0;       // rows per workgroup multiplier of the out-of-place passes (0: per pass)
 };
 int run_fft(FftArgs& a, bool fp64);
 // adjoint of run_fft on gradient grids (register FFT sizes only)

@@ -560,7 +560,7 @@ __device__ __forceinline__ int zmask(int rowi, int TZ) {
 // deposits the main grids (A, B), phase 1 the interlaced grids: LDS holds
 // one phase.  Taps outside the tile go to the lane's sink cell, so the tap
 // loop has no branches.
-template <typename T, int ORDER, bool HASB>
+template <typename T, int ORDER, bool HASB, int TYC, int TZC>
 __global__ __launch_bounds__(DEP_T) void k_deposit(const T* __restrict__ sorted,
                                                    const T* __restrict__ sorted_w,
                                                    const uint32_t* __restrict__ off,
@@ -573,7 +573,9 @@ __global__ __launch_bounds__(DEP_T) void k_deposit(const T* __restrict__ sorted,
   // static LDS (its base folds into the tap addresses): the largest tile's
   // accumulators, checked on the host
   __shared__ __align__(16) unsigned char smem[DEP_LDS_MAX];
-  const int TX = g.tdim[0], TY = g.tdim[1], TZ = g.tdim[2];
+  // TYC, TZC: compile-time tile extents (0: the runtime ones)
+  const int TX = g.tdim[0];
+  const int TY = TYC ? TYC : g.tdim[1], TZ = TZC ? TZC : g.tdim[2];
   const int ncell = TX * TY * TZ;
   const int nacc = ncell + DEP_TRASH;
   // FP32: int64 [nacc]; FP64: double A [nacc], B [nacc]
@@ -783,17 +785,25 @@ static int run_scatter_t(DepositArgs& a) {
   const int64_t ntiles = (int64_t)g.ntile[0] * g.ntile[1] * g.ntile[2];
   const bool hasB = g.fb >= 0;
   if (lds > DEP_LDS_MAX) return CAPOW_ERR_UNSUPPORTED;
+  // the default tile shapes get compile-time extents (shifts instead of
+  // integer multiplies in the tap addressing)
+  const int ty = g.tdim[1], tz = g.tdim[2];
+  const bool t1632 = ty == 16 && tz == 32, t1616 = ty == 16 && tz == 16;
+#define DEP_LAUNCH(HB, TYC, TZC)                                                               \
+  TIMED(a.timer, "deposit",                                                                    \
+        (k_deposit<T, ORDER, HB, TYC, TZC><<<(unsigned)ntiles, DEP_T, 0, st>>>(                 \
+            (const T*)a.sorted, (const T*)a.sorted_w, a.offsets, g, (T*)a.grid[0],              \
+            (T*)a.grid[1], (T*)a.grid[2], (T*)a.grid[3], a.stats, a.sorted_cap, a.errs)))
   if (hasB) {
-    TIMED(a.timer, "deposit",
-          (k_deposit<T, ORDER, true><<<(unsigned)ntiles, DEP_T, 0, st>>>(
-              (const T*)a.sorted, (const T*)a.sorted_w, a.offsets, g, (T*)a.grid[0],
-              (T*)a.grid[1], (T*)a.grid[2], (T*)a.grid[3], a.stats, a.sorted_cap, a.errs)));
+    if (sizeof(T) == 4 && t1632) DEP_LAUNCH(true, 16, 32);
+    else if (sizeof(T) == 8 && t1616) DEP_LAUNCH(true, 16, 16);
+    else DEP_LAUNCH(true, 0, 0);
   } else {
-    TIMED(a.timer, "deposit",
-          (k_deposit<T, ORDER, false><<<(unsigned)ntiles, DEP_T, 0, st>>>(
-              (const T*)a.sorted, (const T*)a.sorted_w, a.offsets, g, (T*)a.grid[0],
-              (T*)a.grid[1], (T*)a.grid[2], (T*)a.grid[3], a.stats, a.sorted_cap, a.errs)));
+    if (sizeof(T) == 4 && t1632) DEP_LAUNCH(false, 16, 32);
+    else if (sizeof(T) == 8 && t1616) DEP_LAUNCH(false, 16, 16);
+    else DEP_LAUNCH(false, 0, 0);
   }
+#undef DEP_LAUNCH
   return hipGetLastError() == hipSuccess ? CAPOW_OK : CAPOW_ERR_HIP;
 }
 

@@ -188,6 +188,45 @@ struct ModeVals {
   double L2, L4, mu;
 };
 
+// Linear k bin of a mode by multiplication with 1/binwidth; `near` flags a
+// quotient within 1e-9 of an integer, where the product may differ from the
+// reference's division and kbin_exact decides.
+struct KBin {
+  int kbin;
+  bool ok, near;
+  double fktot;
+};
+
+__device__ __forceinline__ KBin kbin_fast(double fxy, double fz, const BinArgsDev& a) {
+  KBin r;
+  r.fktot = dmul(2.0 * 3.14159265358979323846, sqrt(dadd(fxy, dmul(fz, fz))));
+  const double q = (double)__double2float_rn(r.fktot - a.mink) * a.inv_binwidth;
+  const double tq = trunc(q);
+  const double fr = q - tq;
+  r.near = fabs(fr) < 1e-9 || fabs(fr) > 1.0 - 1e-9;
+  r.kbin = (int)tq;
+  r.ok = (r.kbin >= 0) && (r.kbin < a.nk) && (r.fktot <= a.min_nyq);
+  return r;
+}
+
+// The reference's division (src/main.c:372, 375) for the near-integer case.
+__device__ __forceinline__ KBin kbin_exact(double fxy, double fz, const BinArgsDev& a) {
+  KBin r;
+  r.fktot = dmul(2.0 * 3.14159265358979323846, sqrt(dadd(fxy, dmul(fz, fz))));
+  r.kbin = (int)ddiv((double)__double2float_rn(r.fktot - a.mink), a.binwidth);
+  r.ok = (r.fktot <= a.min_nyq) && r.kbin >= 0 && r.kbin < a.nk;
+  r.near = false;
+  return r;
+}
+
+// The reference's Hermitian weight NM (src/main.c:422-429): 2, except 1 in
+// the kz = 0 plane where a mode's partner (-i, -j, 0) is stored as well
+// (the rows i = 0, NX/2 of j = 0 and i = NX/2 keep 2, as there).
+__device__ __forceinline__ double nm_weight(int i, int j, int k, int nx) {
+  const bool nm1 = k == 0 && ((j == 0) ? ((i != 0) && (i != nx / 2)) : (i != nx / 2));
+  return nm1 ? 1.0 : 2.0;
+}
+
 template <typename T>
 __device__ __forceinline__ ModeVals mode_vals(const RowVals& R, int k, double fz, double corz,
                                               double pzr, double pzi, typename CT<T>::t va,
@@ -195,21 +234,17 @@ __device__ __forceinline__ ModeVals mode_vals(const RowVals& R, int k, double fz
                                               typename CT<T>::t vbi, const BinArgsDev& a) {
   ModeVals V;
   const double twopi = 2.0 * 3.14159265358979323846;
-  const double fktot = dmul(twopi, sqrt(dadd(R.fxy, dmul(fz, fz))));
-  const double q = (double)__double2float_rn(fktot - a.mink) * a.inv_binwidth;
-  const double tq = trunc(q);
-  const double fr = q - tq;
-  V.near = fabs(fr) < 1e-9 || fabs(fr) > 1.0 - 1e-9;
-  V.kbin = (int)tq;
-  V.ok = (V.kbin >= 0) && (V.kbin < a.nk) && (fktot <= a.min_nyq);
+  const KBin kb = kbin_fast(R.fxy, fz, a);
+  const double fktot = kb.fktot;
+  V.near = kb.near;
+  V.kbin = kb.kbin;
+  V.ok = kb.ok;
   const double fl = a.los == 1 ? R.fx : (a.los == 2 ? R.fy : fz);
   double rk = __builtin_amdgcn_rcp(fktot > 0 ? fktot : 1.0);
   rk = rk * (2.0 - fktot * rk);
   rk = rk * (2.0 - fktot * rk);
   const double mu = fktot > 0 ? (twopi * fl) * rk : 0.0;
-  const bool k0 = k == 0;
-  const bool nm1 = k0 && ((R.j == 0) ? ((R.i != 0) && (R.i != a.nx / 2)) : (R.i != a.nx / 2));
-  const double wc = (nm1 ? 1.0 : 2.0) * (R.corxy * corz);
+  const double wc = nm_weight(R.i, R.j, k, a.nx) * (R.corxy * corz);
   double cr = 1.0, ci = 0.0;
   double ar = (double)va.x, ai = (double)va.y, br = 0, bi = 0;
   if (a.interlace) {
@@ -244,9 +279,10 @@ __device__ __forceinline__ ModeVals mode_vals(const RowVals& R, int k, double fz
   return V;
 }
 
-// Accumulates one evaluated mode into this wave's bins.
+// Accumulates one evaluated mode into this wave's bins (the mode counts
+// come from k_bin_counts).
 __device__ __forceinline__ void mode_add(const ModeVals& V, const BinArgsDev& a, double* accS,
-                                         double* cnt, int nl, double nmw) {
+                                         int nl) {
   const int NK = a.nk;
 #pragma unroll
   for (int s = 0; s < 3; s++) {
@@ -262,15 +298,36 @@ __device__ __forceinline__ void mode_add(const ModeVals& V, const BinArgsDev& a,
       atomicAdd(&b[4 * NK], L3 * np);
     }
   }
-  atomicAdd(&cnt[V.kbin], nmw);
 }
 
-// Row form.  A wave takes batches of 64 consecutive (i, j) rows; for each
-// row the lanes cover kz = lane + 64u, u < NUF (coalesced loads), with the
-// row's x/y quantities wave-uniform and the z tables in LDS.  The rest of
-// each row, kz in [64 NUF, NZ/2+1), is binned once per batch with lane l on
-// row l of the batch.  PREF: the next row's loads are issued before the
-// current row is binned.
+// V consecutive complex values from p (16-byte loads where they fit: p is
+// 16-byte aligned whenever V > 1).
+template <typename C, int V>
+__device__ __forceinline__ void load_run(const C* __restrict__ p, C* out) {
+  if constexpr (sizeof(C) == 8 && V >= 2) {
+#pragma unroll
+    for (int u = 0; u < V; u += 2) {
+      const float4 q = *reinterpret_cast<const float4*>(p + u);
+      out[u].x = q.x;
+      out[u].y = q.y;
+      out[u + 1].x = q.z;
+      out[u + 1].y = q.w;
+    }
+  } else {
+#pragma unroll
+    for (int u = 0; u < V; u++) out[u] = p[u];
+  }
+}
+
+// Row form.  A wave takes batches of 64 consecutive (i, j) rows, with the
+// row's x/y quantities wave-uniform.  Lane l covers kz = p 64V + V l + u
+// (slot s = p V + u, u < V, p < NUF / V): V consecutive modes per lane, so
+// that one instruction's 64 lanes span the whole row and fall into many
+// different k bins (adjacent kz share bins wherever kz << k, and same-bin
+// lanes serialise the LDS adds).  The z tables sit in LDS in slot-major
+// order [s][lane].  The rest of each row, kz in [64 NUF, NZ/2+1), is binned
+// once per batch with lane l on row l of the batch.  PREF: the next row's
+// loads are issued before the current row is binned.
 template <typename T, int NUF, bool PREF, bool FAST>
 __global__ __launch_bounds__(512) void k_bin_rows(const T* __restrict__ gA,
                                                   const T* __restrict__ gAi,
@@ -285,20 +342,33 @@ __global__ __launch_bounds__(512) void k_bin_rows(const T* __restrict__ gA,
   const int F = a.nspec * nl;
   const int n2d = a.out2d ? a.nk * a.nmu : 0;
   const int per_copy = F * NK + a.nspec * n2d + NK + n2d;
+  // LDS doubles per wave: the branch-free form holds only the sums (its mode
+  // counts come from k_bin_counts)
+  const int lcp = FAST ? F * NK : per_copy;
   const int copies = a.copies;
   const int nzh = a.nzh;
-  double* zt = sm + copies * per_copy;   // [nzh][4]: fz, corz, phz.re, phz.im
-  for (int q = threadIdx.x; q < per_copy * copies; q += blockDim.x) sm[q] = 0.0;
-  for (int k = threadIdx.x; k < nzh; k += blockDim.x) {
-    zt[4 * k] = tabs[a.ofz + k];
-    zt[4 * k + 1] = tabs[a.ocz + k];
-    zt[4 * k + 2] = tabs[a.opz + 2 * k];
-    zt[4 * k + 3] = tabs[a.opz + 2 * k + 1];
+  constexpr int V = NUF < (sizeof(T) == 4 ? 8 : 4) ? NUF : (sizeof(T) == 4 ? 8 : 4);
+  const int rem0 = 64 * NUF;
+  // z tables, 4 doubles per kz (fz, corz, phz.re, phz.im): slots [s][lane]
+  // for kz < 64 NUF, then the rest of the row in kz order
+  double* zt = sm + copies * lcp;
+  double* ztr = zt + 4 * rem0;
+  for (int q = threadIdx.x; q < lcp * copies; q += blockDim.x) sm[q] = 0.0;
+  for (int q = threadIdx.x; q < nzh; q += blockDim.x) {
+    int k = q;
+    if (q < rem0) {
+      const int sl = q >> 6, l = q & 63;
+      k = (sl / V) * 64 * V + V * l + (sl % V);
+    }
+    zt[4 * q] = tabs[a.ofz + k];
+    zt[4 * q + 1] = tabs[a.ocz + k];
+    zt[4 * q + 2] = tabs[a.opz + 2 * k];
+    zt[4 * q + 3] = tabs[a.opz + 2 * k + 1];
   }
   __syncthreads();
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int lane = threadIdx.x & 63;
-  double* my = sm + wave * per_copy;
+  double* my = sm + wave * lcp;
   double* accS = my;
   double* acc2 = my + F * NK;
   double* cnt = acc2 + a.nspec * n2d;
@@ -312,7 +382,6 @@ __global__ __launch_bounds__(512) void k_bin_rows(const T* __restrict__ gA,
   const int64_t nrows = (int64_t)a.nx * a.ny;
   const int64_t nbatch = (nrows + 63) / 64;
   const int64_t rowc = a.rowc;
-  const int rem0 = 64 * NUF;
   const int nw = gridDim.x * copies;
   for (int64_t bt = (int64_t)blockIdx.x * copies + wave; bt < nbatch; bt += nw) {
     const int64_t r0 = bt * 64;
@@ -320,14 +389,23 @@ __global__ __launch_bounds__(512) void k_bin_rows(const T* __restrict__ gA,
     if constexpr (NUF > 0) {
       C va[NUF], vai[NUF], vb[NUF], vbi[NUF];
       auto load = [&](int64_t row, C* xa, C* xai, C* xb, C* xbi) {
-        const int64_t base = row * rowc + lane;
+        const int64_t base = row * rowc + V * lane;
 #pragma unroll
-        for (int u = 0; u < NUF; u++) {
-          const int64_t m = base + 64 * u;
-          xa[u] = A[m];
-          xai[u] = il ? Ai[m] : zero;
-          xb[u] = hb ? B[m] : zero;
-          xbi[u] = (hb && il) ? Bi[m] : zero;
+        for (int p = 0; p < NUF / V; p++) {
+          const int64_t m = base + 64 * V * p;
+          load_run<C, V>(A + m, xa + p * V);
+          if (il) load_run<C, V>(Ai + m, xai + p * V);
+          else
+#pragma unroll
+            for (int u = 0; u < V; u++) xai[p * V + u] = zero;
+          if (hb) load_run<C, V>(B + m, xb + p * V);
+          else
+#pragma unroll
+            for (int u = 0; u < V; u++) xb[p * V + u] = zero;
+          if (hb && il) load_run<C, V>(Bi + m, xbi + p * V);
+          else
+#pragma unroll
+            for (int u = 0; u < V; u++) xbi[p * V + u] = zero;
         }
       };
       load(r0, va, vai, vb, vbi);
@@ -343,41 +421,35 @@ __global__ __launch_bounds__(512) void k_bin_rows(const T* __restrict__ gA,
           constexpr int GRP = NUF < 2 ? NUF : (NUF >= 8 ? 2 : (NUF < 4 ? NUF : 4));
 #pragma unroll
           for (int u0 = 0; u0 < NUF; u0 += GRP) {
-            ModeVals V[GRP];
+            ModeVals MV[GRP];
 #pragma unroll
             for (int q = 0; q < GRP; q++) {
               const int u = u0 + q;
-              const int k = lane + 64 * u;
-              const double4 z = *reinterpret_cast<const double4*>(zt + 4 * k);
-              V[q] = mode_vals<T>(R, k, z.x, z.y, z.z, z.w, va[u], vai[u], vb[u], vbi[u], a);
+              const int k = (u / V) * 64 * V + V * lane + (u % V);
+              const double4 z = *reinterpret_cast<const double4*>(zt + 4 * (64 * u + lane));
+              MV[q] = mode_vals<T>(R, k, z.x, z.y, z.z, z.w, va[u], vai[u], vb[u], vbi[u], a);
             }
             bool anynear = false;
 #pragma unroll
-            for (int q = 0; q < GRP; q++) anynear |= V[q].near;
+            for (int q = 0; q < GRP; q++) anynear |= MV[q].near;
             if (__any(anynear)) {
 #pragma unroll
               for (int q = 0; q < GRP; q++)
-                if (V[q].near) {
-                  // the exact division where the quotient is within 1e-9 of
-                  // an integer (the bin index of the reference's division)
-                  const double fz = zt[4 * (lane + 64 * (u0 + q))];
-                  const double fk = dmul(2.0 * 3.14159265358979323846, sqrt(dadd(R.fxy, dmul(fz, fz))));
-                  V[q].kbin = (int)ddiv((double)__double2float_rn(fk - a.mink), a.binwidth);
-                  V[q].ok = (fk <= a.min_nyq) && V[q].kbin >= 0 && V[q].kbin < a.nk;
+                if (MV[q].near) {
+                  const KBin e = kbin_exact(R.fxy, zt[4 * (64 * (u0 + q) + lane)], a);
+                  MV[q].kbin = e.kbin;
+                  MV[q].ok = e.ok;
                 }
             }
 #pragma unroll
-            for (int q = 0; q < GRP; q++) {
-              const int k = lane + 64 * (u0 + q);
-              const bool nm1 = k == 0 && ((j == 0) ? ((i != 0) && (i != a.nx / 2)) : (i != a.nx / 2));
-              if (V[q].ok) mode_add(V[q], a, accS, cnt, nl, nm1 ? 1.0 : 2.0);
-            }
+            for (int q = 0; q < GRP; q++)
+              if (MV[q].ok) mode_add(MV[q], a, accS, nl);
           }
         } else {
 #pragma unroll
           for (int u = 0; u < NUF; u++) {
-            const int k = lane + 64 * u;
-            const double4 z = *reinterpret_cast<const double4*>(zt + 4 * k);
+            const int k = (u / V) * 64 * V + V * lane + (u % V);
+            const double4 z = *reinterpret_cast<const double4*>(zt + 4 * (64 * u + lane));
             bin_mode<T>(R, k, z.x, z.y, z.z, z.w, va[u], vai[u], vb[u], vbi[u], a, accS, acc2, cnt,
                         cnt2, nl, n2d);
           }
@@ -408,8 +480,19 @@ __global__ __launch_bounds__(512) void k_bin_rows(const T* __restrict__ gA,
         const C xai = il ? Ai[m] : zero;
         const C xb = hb ? B[m] : zero;
         const C xbi = (hb && il) ? Bi[m] : zero;
-        const double4 z = *reinterpret_cast<const double4*>(zt + 4 * k);
-        bin_mode<T>(R, k, z.x, z.y, z.z, z.w, xa, xai, xb, xbi, a, accS, acc2, cnt, cnt2, nl, n2d);
+        const double4 z = *reinterpret_cast<const double4*>(ztr + 4 * (k - rem0));
+        if (FAST) {
+          ModeVals mv = mode_vals<T>(R, k, z.x, z.y, z.z, z.w, xa, xai, xb, xbi, a);
+          if (mv.near) {
+            const KBin e = kbin_exact(R.fxy, z.x, a);
+            mv.kbin = e.kbin;
+            mv.ok = e.ok;
+          }
+          if (mv.ok) mode_add(mv, a, accS, nl);
+        } else {
+          bin_mode<T>(R, k, z.x, z.y, z.z, z.w, xa, xai, xb, xbi, a, accS, acc2, cnt, cnt2, nl,
+                      n2d);
+        }
       }
     }
   }
@@ -417,16 +500,51 @@ __global__ __launch_bounds__(512) void k_bin_rows(const T* __restrict__ gA,
   double* out = partials + (int64_t)blockIdx.x * per_copy;
   for (int q = threadIdx.x; q < per_copy; q += blockDim.x) {
     double t = 0.0;
-    for (int c = 0; c < copies; c++) t += sm[c * per_copy + q];
+    if (q < lcp)
+      for (int c = 0; c < copies; c++) t += sm[c * lcp + q];
     out[q] = t;
   }
 }
 
-// Fixed-order sum over workgroup partials.
+// Mode counts of the branch-free form (they depend on the geometry and the
+// k bins only): one wave per row, kz over the lanes; the same bin decisions
+// as k_bin_rows (kbin_fast, kbin_exact near integers).  Counts are integers
+// held exactly in doubles, so the atomic order does not change the result.
+__global__ __launch_bounds__(256) void k_bin_counts(const double* __restrict__ tabs,
+                                                    BinArgsDev a, double* __restrict__ counts) {
+  extern __shared__ double sc[];
+  for (int q = threadIdx.x; q < a.nk; q += blockDim.x) sc[q] = 0.0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int64_t nrows = (int64_t)a.nx * a.ny;
+  const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
+  for (int64_t row = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); row < nrows;
+       row += nw) {
+    const int i = (int)(row / a.ny), j = (int)(row - (int64_t)i * a.ny);
+    const double fx = tabs[a.ofx + i], fy = tabs[a.ofy + j];
+    const double fxy = dadd(dmul(fx, fx), dmul(fy, fy));
+    for (int k = lane; k < a.nzh; k += 64) {
+      KBin kb = kbin_fast(fxy, tabs[a.ofz + k], a);
+      if (kb.near) kb = kbin_exact(fxy, tabs[a.ofz + k], a);
+      if (kb.ok) atomicAdd(&sc[kb.kbin], nm_weight(i, j, k, a.nx));
+    }
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < a.nk; q += blockDim.x)
+    if (sc[q] != 0.0) atomicAdd(&counts[q], sc[q]);
+}
+
+// Fixed-order sum over workgroup partials; with `counts`, the mode counts
+// [cnt_off, cnt_off + nk) are taken from there.
 __global__ void k_bin_final(const double* __restrict__ partials, int nwg, int per_copy,
+                            const double* __restrict__ counts, int cnt_off, int nk,
                             double* __restrict__ out) {
   int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= per_copy) return;
+  if (counts && q >= cnt_off && q < cnt_off + nk) {
+    out[q] = counts[q - cnt_off];
+    return;
+  }
   double t = 0.0;
   for (int w = 0; w < nwg; w++) t += partials[(int64_t)w * per_copy + q];
   out[q] = t;
@@ -457,9 +575,23 @@ template <typename T>
 static int run_bin_t(BinLaunch& L) {
   const int per_copy = L.per_copy;
   const size_t bins = (size_t)per_copy * L.dev.copies * sizeof(double);
-  const size_t lds = bins + (size_t)L.dev.nzh * 4 * sizeof(double);
   const int nuf = L.dev.nzh / 64;
-  if (lds <= 160 * 1024 && L.row_form) {
+  const bool fast = L.row_form && !L.dev.logbins && !L.dev.out2d;
+  const int nl = L.dev.odd ? 5 : 3;
+  const int F = L.dev.nspec * nl;
+  const size_t lcp = fast ? (size_t)F * L.dev.nk : (size_t)per_copy;
+  const size_t lds = lcp * L.dev.copies * sizeof(double) + (size_t)L.dev.nzh * 4 * sizeof(double);
+  if (fast && !*L.counts_valid) {
+    // once per context: the counts depend on the geometry and bins only
+    hipMemsetAsync(L.counts, 0, (size_t)L.dev.nk * sizeof(double), L.stream);
+    const int64_t nrows = (int64_t)L.dev.nx * L.dev.ny;
+    const int nblk = (int)std::min<int64_t>((nrows + 3) / 4, 4096);
+    k_bin_counts<<<nblk, 256, (size_t)L.dev.nk * sizeof(double), L.stream>>>(L.tabs, L.dev,
+                                                                           L.counts);
+    if (hipGetLastError() != hipSuccess) return CAPOW_ERR_HIP;
+    *L.counts_valid = true;
+  }
+  if (L.row_form && lds <= 160 * 1024) {
     // the largest instantiated full-slot count not above nzh / 64
     if (nuf >= 16 && sizeof(T) == 4) launch_rows<T, 16>(L, lds);
     else if (nuf >= 8) launch_rows<T, 8>(L, lds);
@@ -477,8 +609,10 @@ static int run_bin_t(BinLaunch& L) {
               L.tabs, L.dev, L.partials)));
   }
   TIMED(L.timer, "bin_final",
-        (k_bin_final<<<(per_copy + 255) / 256, 256, 0, L.stream>>>(L.partials, L.nwg, per_copy,
-                                                                    L.out)));
+        (k_bin_final<<<(per_copy + 255) / 256, 256, 0, L.stream>>>(
+            L.partials, L.nwg, per_copy, fast ? L.counts : nullptr,
+            F * L.dev.nk + L.dev.nspec * (L.dev.out2d ? L.dev.nk * L.dev.nmu : 0), L.dev.nk,
+            L.out)));
   return hipGetLastError() == hipSuccess ? CAPOW_OK : CAPOW_ERR_HIP;
 }
 

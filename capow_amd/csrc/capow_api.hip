@@ -59,6 +59,8 @@ struct capow_ctx {
   double* bin_tabs = nullptr;
   double* bin_partials = nullptr;
   double* bin_out = nullptr;
+  double* bin_counts = nullptr;     // mode counts of the branch-free binning
+  bool bin_counts_valid = false;
   int bin_nwg = 1024;
   int bin_per_copy = 0;
   bool bin_row_form = true;
@@ -405,10 +407,14 @@ static int setup(capow_ctx* c) {
     if (bytes > lds_max)
       return fail(c, CAPOW_ERR_UNSUPPORTED, "NK/NMU too large for the LDS bin arrays");
     // one wavefront per private bin copy, as many as fit in one CU's LDS
-    // beside the z tables (up to 8); a persistent grid of one workgroup per CU
+    // beside the z tables (up to 8); a persistent grid of one workgroup per CU.
+    // The branch-free row form (linear bins, no 2-D bins) keeps only the
+    // sums in LDS: its mode counts are computed once per context.
     const size_t ztab = (size_t)nzh * 4 * 8;
     c->bin_row_form = bytes + ztab <= lds_max;
-    int copies = (int)((lds_max - (c->bin_row_form ? ztab : 0)) / bytes);
+    const bool fast = c->bin_row_form && !b.logbins && !b.out2d;
+    const size_t lbytes = fast ? (size_t)ns * nl * p.nk * 8 : bytes;
+    int copies = (int)((lds_max - (c->bin_row_form ? ztab : 0)) / lbytes);
     b.copies = copies > 8 ? 8 : copies;
     int dev = 0, ncu = 256;
     if (hipGetDevice(&dev) == hipSuccess)
@@ -422,6 +428,8 @@ static int setup(capow_ctx* c) {
     c->bin_nwg = (int)(want < ncu ? want : ncu);
     HIPCHK(c, hipMalloc((void**)&c->bin_partials, (size_t)c->bin_nwg * bytes));
     HIPCHK(c, hipMalloc((void**)&c->bin_out, bytes));
+    HIPCHK(c, hipMalloc((void**)&c->bin_counts, (size_t)p.nk * 8));
+    c->bin_counts_valid = false;
     HIPCHK(c, hipHostMalloc((void**)&c->h_bins, bytes, hipHostMallocDefault));
   }
   return CAPOW_OK;
@@ -432,7 +440,8 @@ static void release(capow_ctx* c) {
   for (int s = 0; s < 4; s++) if (c->scratch[s]) hipFree(c->scratch[s]);
   void* ps[] = {c->own_parts, c->own_w, c->sorted, c->sorted_w, c->counts, c->offsets,
                 c->cursor, c->bsum, c->boff, c->partials, c->gmax, c->errs, c->stats,
-                c->tw[0], c->tw[1], c->tw[2], c->bin_tabs, c->bin_partials, c->bin_out};
+                c->tw[0], c->tw[1], c->tw[2], c->bin_tabs, c->bin_partials, c->bin_out,
+                c->bin_counts};
   for (void* q : ps) if (q) hipFree(q);
   void* bs[] = {c->bwd_G, c->bwd_g0, c->bwd_out, c->bwd_w};
   for (void* q : bs) if (q) hipFree(q);
@@ -693,6 +702,8 @@ static int bin_enqueue(capow_ctx* c, double* h_bins) {
   L.per_copy = c->bin_per_copy;
   L.partials = c->bin_partials;
   L.out = c->bin_out;
+  L.counts = c->bin_counts;
+  L.counts_valid = &c->bin_counts_valid;
   int rc = run_bin(L, c->fp64);
   if (rc) return fail(c, rc, std::string("binning launch failed: ") + hipGetErrorString(hipGetLastError()));
   HIPCHK(c, hipMemcpyAsync(h_bins, c->bin_out, (size_t)c->bin_per_copy * 8, hipMemcpyDeviceToHost,
@@ -847,6 +858,8 @@ extern "C" int capow_backward(capow_ctx* c, const double* dL_draw, void* grad_xy
   L.per_copy = c->bin_per_copy;
   L.partials = c->bin_partials;
   L.out = c->bin_out;
+  L.counts = c->bin_counts;
+  L.counts_valid = &c->bin_counts_valid;
   rc = run_bin_backward(L, c->bwd_G, c->bwd_g0, c->fp64);
   if (rc) return fail(c, rc, std::string("binning backward failed: ") + hipGetErrorString(hipGetLastError()));
   FftArgs f;

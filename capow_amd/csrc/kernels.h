@@ -163,6 +163,8 @@ struct BinLaunch {
   int per_copy;
   double* partials;       // nwg * per_copy
   double* out;            // per_copy
+  double* counts;         // nk: mode counts of the branch-free form
+  bool* counts_valid;     // counts computed for this geometry
 };
 int run_bin(BinLaunch& L, bool fp64);
 

@@ -36,6 +36,7 @@ STAGE_PATTERNS = [
     ("fft_yx", r"k_fft_rows_t<.*false>"),
     ("fft_z", r"k_fft_rows"),
     ("fft_yx", r"k_fft_cols"),
+    ("bin_counts", r"k_bin_counts"),
     ("bin", r"k_bin(_rows)?<"),
     ("bin_final", r"k_bin_final"),
 ]

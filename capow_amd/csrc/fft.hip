@@ -540,46 +540,31 @@ __global__ __launch_bounds__((N1 * N2 / clcm(N1, N2)) * B) void k_fft_rows2_adj(
 }
 
 // Row pass with a transposed write (the three passes of the out-of-place
-// pipeline).  A workgroup transforms B rows that are B consecutive values of
-// an "inner" index at one "outer" index:
+// pipeline).  A batch is B rows that are B consecutive values of an "inner"
+// index at one "outer" index:
 //   input row (outer, inner) at in + outer*in_so + inner*in_sb (complex),
 //     contiguous, N = N1*N2 complex values (R2C: 2N reals, an r2c of length
 //     2N giving N+1 outputs by the even/odd split);
 //   output k of that row at out + outer*out_so + k*out_sk + inner,
 // so every write is a run of B consecutive complex values (the transpose of
 // the B rows, staged in LDS).  Every read is a whole row.
-template <typename T, int N1, int N2, int B, bool R2C>
-__global__ __launch_bounds__((N1 * N2 / clcm(N1, N2)) * B) void k_fft_rows_t(
-    const T* __restrict__ i0, const T* __restrict__ i1, const T* __restrict__ i2,
-    const T* __restrict__ i3, T* __restrict__ o0, T* __restrict__ o1, T* __restrict__ o2,
-    T* __restrict__ o3, int64_t in_so, int64_t in_sb, int64_t out_so, int64_t out_sk,
-    int64_t n_inner, int64_t inner_groups, const cx<T>* __restrict__ tw, int64_t nblocks_x) {
-  constexpr int N = N1 * N2;
-  constexpr int E = clcm(N1, N2);
-  constexpr int TS = N / E;
-  constexpr int G1 = E / N2, G2 = E / N1;
-  constexpr int RS = N1 * (N2 + 1);   // per-row LDS stride of the transpose (>= N + 1)
-  constexpr int K = R2C ? N + 1 : N;  // outputs per row
-  extern __shared__ unsigned char smem[];
-  cx<T>* lds = reinterpret_cast<cx<T>*>(smem);
-  const int gsel = blockIdx.y;
-  const T* gin = gsel == 0 ? i0 : gsel == 1 ? i1 : gsel == 2 ? i2 : i3;
-  T* gout = gsel == 0 ? o0 : gsel == 1 ? o1 : gsel == 2 ? o2 : o3;
-  int64_t bid = blockIdx.x;
-  if ((nblocks_x & 7) == 0) bid = (bid & 7) * (nblocks_x >> 3) + (bid >> 3);
-  const int64_t outer = bid / inner_groups;
-  const int64_t inner0 = (bid - outer * inner_groups) * B;
-  const int t = threadIdx.x % TS, c = threadIdx.x / TS;
-  const bool live = inner0 + c < n_inner;
-  const cx<T>* row = reinterpret_cast<const cx<T>*>(gin) + outer * in_so + (live ? inner0 + c : 0) * in_sb;
-  cx<T> v[E];
-#pragma unroll
-  for (int g = 0; g < G1; g++)
-#pragma unroll
-    for (int n2 = 0; n2 < N2; n2++) {
-      const int n = (t + TS * g) + N1 * n2;
-      v[g * N2 + n2] = live ? row[n] : mk<T>(T(0), T(0));
-    }
+//
+// Persistent: the batches of all grids form one list (batch b of grid g is
+// item g*nbx + b); with gridDim.x a multiple of 8, workgroup w runs on XCD
+// w % 8 and walks that XCD's contiguous eighth of the list, interleaved with
+// the XCD's other workgroups.  The 135 KB LDS of the 1024-point FP32 pass
+// leaves one workgroup per CU, so the next batch's rows are loaded (PF)
+// while the current batch is transformed and written; and since a thread's
+// first-stage index n1 is fixed (G1 = 1), its twiddles w^(n1 k2) are loaded
+// once into registers (TWR) instead of once per row.
+// First stage of a batch: the N2-point DFTs of the thread's rows, twiddled,
+// into the LDS transpose (row c at c*RS).
+template <typename T, int N1, int N2, bool R2C, bool TW_REG>
+__device__ __forceinline__ void rt_stage1(cx<T>* v, const cx<T>* twr,
+                                          const cx<T>* __restrict__ tw, cx<T>* lds, int t,
+                                          int c) {
+  constexpr int N = N1 * N2, E = clcm(N1, N2), TS = N / E, G1 = E / N2;
+  constexpr int RS = N1 * (N2 + 1);
 #pragma unroll
   for (int g = 0; g < G1; g++) {
     cx<T> o[N2];
@@ -588,11 +573,24 @@ __global__ __launch_bounds__((N1 * N2 / clcm(N1, N2)) * B) void k_fft_rows_t(
 #pragma unroll
     for (int k2 = 0; k2 < N2; k2++) {
       // R2C: the 2N-point table, step 2; C2C: the N-point table
-      cx<T> y = (k2 == 0 || n1 == 0) ? o[k2] : mul<T>(o[k2], tw[(R2C ? 2 : 1) * n1 * k2]);
+      cx<T> y;
+      if constexpr (TW_REG) y = k2 == 0 ? o[k2] : mul<T>(o[k2], twr[k2]);
+      else y = (k2 == 0 || n1 == 0) ? o[k2] : mul<T>(o[k2], tw[(R2C ? 2 : 1) * n1 * k2]);
       lds[c * RS + n1 * (N2 + 1) + k2] = y;
     }
   }
-  __syncthreads();
+}
+
+// The rest of a batch after the first stage's barrier: N1-point DFTs,
+// natural order in LDS (row stride N + 1), the r2c split, the transposed
+// write.  Ends with the LDS still in use by the write (the caller syncs
+// before reusing it).
+template <typename T, int N1, int N2, int B, bool R2C>
+__device__ __forceinline__ void rt_rest(const cx<T>* __restrict__ tw, cx<T>* lds, int t, int c,
+                                        cx<T>* ob, int64_t out_sk, int64_t nlive) {
+  constexpr int N = N1 * N2, E = clcm(N1, N2), TS = N / E, G2 = E / N1;
+  constexpr int RS = N1 * (N2 + 1);
+  constexpr int K = R2C ? N + 1 : N;  // outputs per row
   cx<T> z[E];
 #pragma unroll
   for (int h = 0; h < G2; h++) {
@@ -603,7 +601,6 @@ __global__ __launch_bounds__((N1 * N2 / clcm(N1, N2)) * B) void k_fft_rows_t(
     RegDFT<T, N1>::run(a, &z[h * N1]);
   }
   __syncthreads();
-  // natural order, row stride N + 1
   cx<T>* zr = lds + c * (N + 1);
 #pragma unroll
   for (int h = 0; h < G2; h++) {
@@ -631,10 +628,102 @@ __global__ __launch_bounds__((N1 * N2 / clcm(N1, N2)) * B) void k_fft_rows_t(
     __syncthreads();
   }
   // transposed write: runs of B consecutive inner values per output k
-  cx<T>* ob = reinterpret_cast<cx<T>*>(gout) + outer * out_so + inner0;
   for (int q = threadIdx.x; q < K * B; q += TS * B) {
     const int b = q % B, kk = q / B;
-    if (inner0 + b < n_inner) ob[(int64_t)kk * out_sk + b] = lds[b * (N + 1) + kk];
+    if (b < nlive) ob[(int64_t)kk * out_sk + b] = lds[b * (N + 1) + kk];
+  }
+}
+
+// PERSIST = false: one batch per workgroup (the same XCD mapping); FP64
+// uses it, as the loop form needs about twice the registers there.
+// outer_fast: batch b of a grid is (outer = b % n_outer, group = b / n_outer)
+// instead of (outer = b / groups, group = b % groups).
+template <typename T, int N1, int N2, int B, bool R2C, bool PERSIST, bool PF, bool TWR>
+__global__ __launch_bounds__((N1 * N2 / clcm(N1, N2)) * B) void k_fft_rows_t(
+    const T* __restrict__ i0, const T* __restrict__ i1, const T* __restrict__ i2,
+    const T* __restrict__ i3, T* __restrict__ o0, T* __restrict__ o1, T* __restrict__ o2,
+    T* __restrict__ o3, int64_t in_so, int64_t in_sb, int64_t out_so, int64_t out_sk,
+    int64_t n_inner, int64_t inner_groups, const cx<T>* __restrict__ tw, int64_t nbx,
+    int ngrids, int outer_fast) {
+  constexpr int N = N1 * N2;
+  constexpr int E = clcm(N1, N2);
+  constexpr int TS = N / E;
+  constexpr int G1 = E / N2;
+  constexpr bool TW_REG = TWR && G1 == 1;
+  extern __shared__ unsigned char smem[];
+  cx<T>* lds = reinterpret_cast<cx<T>*>(smem);
+  const int t = threadIdx.x % TS, c = threadIdx.x / TS;
+  const int64_t total = nbx * ngrids;
+  const int64_t n_outer = nbx / inner_groups;
+  int64_t L0, Lend, step;
+  if ((gridDim.x & 7) == 0) {
+    const int64_t M = gridDim.x >> 3;
+    const int64_t chunk = (total + 7) >> 3;
+    const int64_t xcd = blockIdx.x & 7;
+    L0 = xcd * chunk + (blockIdx.x >> 3);
+    Lend = min(total, (xcd + 1) * chunk);
+    step = M;
+  } else {
+    L0 = blockIdx.x;
+    Lend = total;
+    step = gridDim.x;
+  }
+  cx<T> twr[TW_REG ? N2 : 1];
+  if constexpr (TW_REG) {
+#pragma unroll
+    for (int k2 = 0; k2 < N2; k2++) twr[k2] = tw[(R2C ? 2 : 1) * t * k2];
+  }
+  cx<T> v[E];
+  auto load = [&](int64_t item) {
+    const int gsel = (int)(item / nbx);
+    const int64_t bid = item - (int64_t)gsel * nbx;
+    const T* gin = gsel == 0 ? i0 : gsel == 1 ? i1 : gsel == 2 ? i2 : i3;
+    const int64_t ig = outer_fast ? bid / n_outer : bid - (bid / inner_groups) * inner_groups;
+    const int64_t outer = outer_fast ? bid - ig * n_outer : bid / inner_groups;
+    const int64_t inner0 = ig * B;
+    const bool live = inner0 + c < n_inner;
+    const cx<T>* row = reinterpret_cast<const cx<T>*>(gin) + outer * in_so +
+                       (live ? inner0 + c : 0) * in_sb;
+#pragma unroll
+    for (int g = 0; g < G1; g++)
+#pragma unroll
+      for (int n2 = 0; n2 < N2; n2++) {
+        const int n = (t + TS * g) + N1 * n2;
+        v[g * N2 + n2] = live ? row[n] : mk<T>(T(0), T(0));
+      }
+  };
+  // output of a batch: the start of its transposed runs, its live rows
+  auto dest = [&](int64_t item, int64_t& nlive) {
+    const int gsel = (int)(item / nbx);
+    const int64_t bid = item - (int64_t)gsel * nbx;
+    T* gout = gsel == 0 ? o0 : gsel == 1 ? o1 : gsel == 2 ? o2 : o3;
+    const int64_t ig = outer_fast ? bid / n_outer : bid - (bid / inner_groups) * inner_groups;
+    const int64_t outer = outer_fast ? bid - ig * n_outer : bid / inner_groups;
+    nlive = n_inner - ig * B;
+    return reinterpret_cast<cx<T>*>(gout) + outer * out_so + ig * B;
+  };
+  if constexpr (!PERSIST) {
+    if (L0 >= Lend) return;
+    load(L0);
+    rt_stage1<T, N1, N2, R2C, TW_REG>(v, twr, tw, lds, t, c);
+    __syncthreads();
+    int64_t nlive;
+    cx<T>* ob = dest(L0, nlive);
+    rt_rest<T, N1, N2, B, R2C>(tw, lds, t, c, ob, out_sk, nlive);
+  } else {
+    if (L0 < Lend) load(L0);
+    for (int64_t item = L0; item < Lend; item += step) {
+      if (!PF && item != L0) load(item);
+      rt_stage1<T, N1, N2, R2C, TW_REG>(v, twr, tw, lds, t, c);
+      __syncthreads();
+      // v is free: the next batch's loads overlap the rest of this one
+      if (PF && item + step < Lend) load(item + step);
+      int64_t nlive;
+      cx<T>* ob = dest(item, nlive);
+      rt_rest<T, N1, N2, B, R2C>(tw, lds, t, c, ob, out_sk, nlive);
+      // the next batch's first stage overwrites the LDS
+      __syncthreads();
+    }
   }
 }
 
@@ -775,17 +864,27 @@ static void launch_rows_t(FftArgs& a, const char* name, T* const* in, T* const* 
                           int64_t out_sk, const void* tw) {
   constexpr int N = N1 * N2, E = clcm(N1, N2), TS = N / E;
   constexpr int B = (sizeof(T) == 4 ? 8 : 4) * MUL;   // 64- (MUL 2: 128-) byte write runs
+  // FP32: persistent, prefetch and register twiddles (FP64 has no registers
+  // to spare: one batch per workgroup)
+  constexpr bool PS = sizeof(T) == 4, PF = sizeof(T) == 4, TWR = sizeof(T) == 4;
+  const void* fn = (const void*)k_fft_rows_t<T, N1, N2, B, R2C, PS, PF, TWR>;
   const size_t lds = (size_t)B * (N + N1) * sizeof(cx<T>);
   if (lds > 64 * 1024)
-    hipFuncSetAttribute((const void*)k_fft_rows_t<T, N1, N2, B, R2C>,
-                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   const int64_t groups = (n_inner + B - 1) / B;
   const int64_t nbx = n_outer * groups;
-  dim3 grid((unsigned)nbx, a.ngrids);
+  const int64_t total = nbx * a.ngrids;
+  int per_cu = 1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, TS * B, lds) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  int64_t nblk = (int64_t)(a.ncu > 0 ? a.ncu : 256) * per_cu;
+  if (!PS || a.persist == 0 || nblk > total) nblk = total;
+  const int ofast = (a.order >> (name[4] == 'z' ? 0 : name[4] == 'y' ? 1 : 2)) & 1;
   TIMED(a.timer, name,
-        (k_fft_rows_t<T, N1, N2, B, R2C><<<grid, TS * B, lds, a.stream>>>(
+        (k_fft_rows_t<T, N1, N2, B, R2C, PS, PF, TWR><<<(unsigned)nblk, TS * B, lds, a.stream>>>(
             in[0], in[1], in[2], in[3], out[0], out[1], out[2], out[3], in_so, in_sb, out_so,
-            out_sk, n_inner, groups, (const cx<T>*)tw, nbx)));
+            out_sk, n_inner, groups, (const cx<T>*)tw, nbx, a.ngrids, ofast)));
 }
 
 template <typename T, bool R2C>
@@ -829,9 +928,9 @@ static bool run_fft_transposed(FftArgs& a) {
     G[q] = q < a.ngrids ? (T*)a.grid[q] : (T*)a.grid[0];
     S[q] = q < a.ngrids ? (T*)a.scratch[q] : (T*)a.scratch[0];
   }
-  // write runs: 128 bytes for the z and x passes, 64 for y (measured best
-  // at 1024^3 in both precisions); rows_mul overrides all three
-  const int mz = a.rows_mul ? a.rows_mul : 2, my = a.rows_mul ? a.rows_mul : 1,
+  // write runs: 128 bytes (FP64: 64 for the y pass; measured best at 1024^3);
+  // rows_mul overrides all three
+  const int mz = a.rows_mul ? a.rows_mul : 2, my = a.rows_mul ? a.rows_mul : (sizeof(T) == 4 ? 2 : 1),
             mx = a.rows_mul ? a.rows_mul : 2;
   if (!try_rows_t<T, true>(a, "fft_z", nz / 2, G, S, nx, ny, (int64_t)ny * rowc, rowc,
                            nzh * py, py, a.tw[2], mz))

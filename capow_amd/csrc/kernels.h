@@ -130,6 +130,9 @@ struct FftArgs {
   int64_t px = 0, py = 0;
   bool swapped = false;
   int rows_mul = 0;       // rows per workgroup multiplier of the out-of-place passes (0: per pass)
+  int ncu = 256;          // compute units (persistent grids)
+  int persist = 1;        // 0: one workgroup per batch instead of a persistent grid
+  int order = 0;          // bit per pass (z, y, x): batches with the outer index fastest
 };
 int run_fft(FftArgs& a, bool fp64);
 // adjoint of run_fft on gradient grids (register FFT sizes only)

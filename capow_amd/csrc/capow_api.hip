@@ -197,7 +197,7 @@ static int validate(const capow_params* p, std::string& m) {
     }
     ntiles *= dims[a] / t;
   }
-  // the tile-offset scan covers 4096 x 4096 tiles
+  // sort keys tile * 27 + border class stay within 32 bits
   if (ntiles > (int64_t)4096 * 4096) {
     m = "grid has more than 16,777,216 deposit tiles";
     return CAPOW_ERR_UNSUPPORTED;
@@ -258,7 +258,8 @@ static int setup(capow_ctx* c) {
     g.tdiv[a].init((uint32_t)t);
     g.ntile[a] = g.n[a] / t;
   }
-  g.nbuckets = (int64_t)g.ntile[0] * g.ntile[1] * g.ntile[2];
+  // one bucket per (tile, border class)
+  g.nbuckets = (int64_t)g.ntile[0] * g.ntile[1] * g.ntile[2] * 27;
   // LDS hash probes per key in the counting sort; CAPOW_SORT_PROBES=0 sends
   // every copy through the global counters (the table-full path)
   g.sort_probes = 32;
@@ -315,7 +316,7 @@ static int setup(capow_ctx* c) {
   // sort buffers
   HIPCHK(c, hipMalloc((void**)&c->counts, (size_t)g.nbuckets * 4));
   HIPCHK(c, hipMalloc((void**)&c->offsets, (size_t)(g.nbuckets + 1) * 4));
-  HIPCHK(c, hipMalloc((void**)&c->cursor, (size_t)g.nbuckets * 2 * 4));
+  HIPCHK(c, hipMalloc((void**)&c->cursor, (size_t)g.nbuckets * 4));
   int64_t nblk = (g.nbuckets + 4095) / 4096;
   HIPCHK(c, hipMalloc((void**)&c->bsum, (size_t)(nblk + 1) * 4));
   HIPCHK(c, hipMalloc((void**)&c->boff, (size_t)(nblk + 1) * 4));
@@ -603,11 +604,8 @@ static int assign_enqueue(capow_ctx* c, double* h_stats, unsigned int* h_errs, b
   int rc = run_count(a, c->fp64);
   if (rc) return fail(c, rc, std::string("deposit count failed: ") + hipGetErrorString(hipGetLastError()));
   if (exact) {
-    // total particle copies (one per reached tile) sizes the sorted buffer
-    uint32_t total = 0;
-    HIPCHK(c, hipMemcpyAsync(&total, c->offsets + g.nbuckets, 4, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    const size_t rows = (size_t)total > 0 ? (size_t)total : 1;
+    // one sorted copy per accepted particle: n rows always suffice
+    const size_t rows = c->n > 0 ? (size_t)c->n : 1;
     rc = ensure(c, &c->sorted, &c->sorted_cap, rows * 4 * c->esz);
     if (rc) return rc;
     if (c->wts) {
@@ -995,8 +993,8 @@ extern "C" int capow_batch_create(const capow_params* configs, int32_t nconf,
       b->ctx.push_back(c);
       capow_set_stream(c, st);
       c->cap_fixed = true;
-      // two copies per particle (tile + ghost copies are ~1.3 at these sizes)
-      const size_t rows = (size_t)2 * (size_t)max_particles + 4096;
+      // one sorted copy per particle
+      const size_t rows = (size_t)max_particles + 4096;
       int r1 = ensure(c, &c->sorted, &c->sorted_cap, rows * 4 * c->esz);
       int r2 = r1 ? r1 : ensure(c, &c->sorted_w, &c->sorted_w_cap, rows * c->esz);
       if (r1 || r2) {

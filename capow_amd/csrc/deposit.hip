@@ -6,21 +6,21 @@
 //
 // Design (one GPU holds the whole grid):
 //   1. k_bucket<COUNT>: per particle, apply the RSD shift, decide acceptance
-//      of the main and interlaced deposits, find the deposit tiles its taps
-//      can reach (its own tile, plus the neighbour tile along each axis when
-//      the particle sits within the tap reach of that tile's edge: at most 8
-//      tiles) and count one copy per tile; block partials of the global sums
-//      (N gridded, sum w v^n) and the max |field| per field.
+//      of the main and interlaced deposits, find its key: its tile and its
+//      border class (per axis: whether its taps reach the tile below, the
+//      tile above, or neither; 27 classes) and count it; block partials of
+//      the global sums (N gridded, sum w v^n) and the max |field| per field.
 //   2. k_stats_final: fixed-order reduction of the block partials, alpha,
 //      per-field fixed-point scales.
-//   3. exclusive scan of the per-tile copy counts.
+//   3. exclusive scan of the per-(tile, class) counts.
 //   4. k_bucket<SCATTER>: counting-sort scatter of the shifted particles, one
-//      copy per reached tile, so every tile's particles are one contiguous
-//      range.
-//   5. k_deposit: one workgroup per output tile reads its range, accumulates
-//      every tap that lands in the tile into LDS, then writes the tile once
-//      (value - alpha for density fields).  No grid memset, no global
-//      atomics on the grid.
+//      copy each, so every (tile, class) is one contiguous range and every
+//      tile's own particles are the contiguous range of its 27 classes.
+//   5. k_deposit: one workgroup per output tile reads its own range and, from
+//      each of its 26 neighbours, the class ranges whose taps reach it (63
+//      ranges in all), accumulates every tap that lands in the tile into
+//      LDS, then writes the tile once (value - alpha for density fields).
+//      No grid memset, no global atomics on the grid.
 // Determinism: every tap contribution is rounded to an integer multiple of
 // a quantum (FP32 form: per tile and field, int32 taps in one int64
 // accumulator per cell; FP64 form: per field, integer-valued doubles), and
@@ -81,14 +81,20 @@ __device__ __forceinline__ void store2(T* p, T a, T b) {
 }
 
 // Per-particle classification shared by the count and scatter passes.
+// The sort key is tile * 27 + border class: per axis, class 0 when the
+// particle's taps reach the tile below (key cell within -reach_lo of the
+// tile's lower edge), 2 when they reach the tile above (within reach_hi of
+// the upper edge), 1 otherwise (and always 1 along an axis with a single
+// tile); class = cx*9 + cy*3 + cz.  Every particle has exactly one sorted
+// copy; the deposit of a tile reads its own 27 classes and, from each
+// neighbour, the classes that reach it.
 template <typename T, int ORDER>
 struct PClass {
   T x, y, z;       // position after the RSD shift
   bool acc_main, acc_int;
-  int64_t bucket;  // -1: rejected by every deposit
+  bool take;       // accepted by at least one deposit
   bool reach_ok;
-  int tile[3][2];  // reached tiles per axis
-  int nt[3];       // how many (1 or 2)
+  uint32_t key;    // tile * 27 + class
 };
 
 // Canonical periodic offset in [n/2 - n + 1, n/2] of an offset within one
@@ -147,35 +153,34 @@ __device__ __forceinline__ PClass<T, ORDER> classify(T x, T y, T z, T v, const G
   }
   c.acc_main = okm;
   c.acc_int = oki;
-  c.bucket = (okm || oki) ? 0 : -1;
+  c.take = okm || oki;
   c.reach_ok = true;
-  // Deposit tiles reached along each axis: the key's tile, plus the tile
-  // above when the key lies within reach_hi of its upper edge, plus the tile
-  // below when it lies within -reach_lo of its lower edge.
+  // the key's tile, and per axis whether the taps reach the tile above
+  // (the key lies within reach_hi of its upper edge) or below (within
+  // -reach_lo of its lower edge)
   const int lw = -g.reach_lo, hw = g.reach_hi;
   const bool per = g.periodic != 0;
+  int tc[3], cl[3];
 #pragma unroll
   for (int a = 0; a < 3; a++) {
     const int n = g.n[a], td = g.tdim[a], nt = g.ntile[a];
     const int key = cell_key(cm[a], n, per);
     const int t0 = (int)g.tdiv[a].div((uint32_t)key);
     const int local = key - t0 * td;
-    int second = -1;
+    cl[a] = 1;
     if (nt > 1) {
       const int first_m = ORDER == 3 ? cm[a] - 1 : cm[a];
       const int first_i = ORDER == 3 ? ci[a] - 1 : ci[a];
       const bool r = (!okm || reach_ok<ORDER>(first_m, key, n, g)) &&
                      (!oki || reach_ok<ORDER>(first_i, key, n, g));
       c.reach_ok = c.reach_ok && r;
-      int up = t0 + 1, dn = t0 - 1;
-      up = up >= nt ? (per ? up - nt : -1) : up;
-      dn = dn < 0 ? (per ? dn + nt : -1) : dn;
-      second = local >= td - hw ? up : (local < lw ? dn : -1);
+      cl[a] = local >= td - hw ? 2 : (local < lw ? 0 : 1);
     }
-    c.tile[a][0] = t0;
-    c.tile[a][1] = second;
-    c.nt[a] = second >= 0 ? 2 : 1;
+    tc[a] = t0;
   }
+  const uint32_t tile = ((uint32_t)tc[2] * (uint32_t)g.ntile[1] + (uint32_t)tc[1]) *
+                            (uint32_t)g.ntile[0] + (uint32_t)tc[0];
+  c.key = tile * 27u + (uint32_t)(cl[0] * 9 + cl[1] * 3 + cl[2]);
   return c;
 }
 
@@ -194,9 +199,8 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
 }
 
 // Block-local aggregation of the counting sort.  A block takes a chunk of
-// BK_CHUNK consecutive particles; every copy (particle, reached tile) gets a
-// key -- the tile in the count pass, 2*tile + ghost in the scatter pass --
-// and is counted in an LDS hash table (open addressing, linear probing).
+// BK_CHUNK consecutive particles; every particle's key (tile * 27 + border
+// class) is counted in an LDS hash table (open addressing, linear probing).
 // Runs of lanes with the same key are merged first (the run head adds the
 // run length), so a slot sees one LDS atomic per run.  At the end of the
 // chunk each occupied slot issues ONE global atomic for all its copies:
@@ -298,16 +302,16 @@ __global__ __launch_bounds__(BK_T) void k_bucket(const T* __restrict__ parts,
   double m_a = 0, m_b = 0;
   unsigned int oob = 0, reach_bad = 0;
   const int64_t nchunks = (n + BK_CHUNK - 1) / BK_CHUNK;
-  const uint32_t nb = (uint32_t)g.nbuckets;
   for (int64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
     for (int s = threadIdx.x; s < BK_HS; s += BK_T) {
       hkey[s] = BK_EMPTY;
       hcnt[s] = 0;
     }
     __syncthreads();
-    // scatter pass: the chunk's copies wait in registers for their slot bases
+    // scatter pass: the chunk's particles wait in registers for their slot
+    // bases
     T px[BK_K], py[BK_K], pz[BK_K], pv[BK_K], pw[BK_K];
-    uint32_t pr[BK_K][8];   // slot | rank << BK_HS_BITS; BK_EMPTY: no copy / written
+    uint32_t pr[BK_K];   // slot | rank << BK_HS_BITS; BK_EMPTY: none / written
 #pragma unroll
     for (int k = 0; k < BK_K; k++) {
       const int64_t i = ch * BK_CHUNK + (int64_t)k * BK_T + threadIdx.x;
@@ -318,38 +322,23 @@ __global__ __launch_bounds__(BK_T) void k_bucket(const T* __restrict__ parts,
         if (g.has_w) w = wts[i];
       }
       PClass<T, ORDER> c = classify<T, ORDER>(x, y, z, v, g);
-      const bool take = valid && c.bucket >= 0;
+      const bool take = valid && c.take;
       px[k] = c.x; py[k] = c.y; pz[k] = c.z; pv[k] = v; pw[k] = w;
-#pragma unroll
-      for (int q = 0; q < 8; q++) {
-        pr[k][q] = BK_EMPTY;
-        const int qx = q & 1, qy = (q >> 1) & 1, qz = q >> 2;
-        const bool has = take && qx < c.nt[0] && qy < c.nt[1] && qz < c.nt[2];
-        if (!__any(has)) continue;
-        int tile = 0;
-        if (has) {
-          const int tx = qx ? c.tile[0][1] : c.tile[0][0];
-          const int ty = qy ? c.tile[1][1] : c.tile[1][0];
-          const int tz = qz ? c.tile[2][1] : c.tile[2][0];
-          tile = (tz * g.ntile[1] + ty) * g.ntile[0] + tx;
-        }
-        // own copies fill each tile's range from the front, ghost copies
-        // from the back, so deposit waves are mostly homogeneous
-        const uint32_t key = SCATTER ? 2u * (uint32_t)tile + (q ? 1u : 0u) : (uint32_t)tile;
-        uint32_t rank;
-        uint32_t* gctr = SCATTER ? cursor + (q ? nb : 0u) + (uint32_t)tile : counts + tile;
-        const int slot = hs_add<SCATTER>(hkey, hcnt, key, has, gctr, g.sort_probes, rank);
-        if (SCATTER && has) {
-          if (slot < 0) {
-            // table full: the run took its ranks from the global cursor
-            const uint32_t pos = q ? off[tile + 1] - 1u - rank : off[tile] + rank;
-            if ((int64_t)pos < cap) {
-              store4<T>(sorted, pos, c.x, c.y, c.z, v);
-              if (g.has_w) sorted_w[pos] = w;
-            }
-          } else {
-            pr[k][q] = (uint32_t)slot | (rank << BK_HS_BITS);
+      pr[k] = BK_EMPTY;
+      const uint32_t key = take ? c.key : 0u;
+      uint32_t rank;
+      uint32_t* gctr = SCATTER ? cursor + key : counts + key;
+      const int slot = hs_add<SCATTER>(hkey, hcnt, key, take, gctr, g.sort_probes, rank);
+      if (SCATTER && take) {
+        if (slot < 0) {
+          // table full: the run took its ranks from the global cursor
+          const uint32_t pos = off[key] + rank;
+          if ((int64_t)pos < cap) {
+            store4<T>(sorted, pos, c.x, c.y, c.z, v);
+            if (g.has_w) sorted_w[pos] = w;
           }
+        } else {
+          pr[k] = (uint32_t)slot | (rank << BK_HS_BITS);
         }
       }
       if (!SCATTER && valid) {
@@ -378,26 +367,21 @@ __global__ __launch_bounds__(BK_T) void k_bucket(const T* __restrict__ parts,
     for (int s = threadIdx.x; s < BK_HS; s += BK_T) {
       const uint32_t key = hkey[s];
       if (key == BK_EMPTY) continue;
-      if (SCATTER) hcnt[s] = atomicAdd(&cursor[(key & 1u) ? nb + (key >> 1) : (key >> 1)], hcnt[s]);
+      if (SCATTER) hcnt[s] = atomicAdd(&cursor[key], hcnt[s]);
       else atomicAdd(&counts[key], hcnt[s]);
     }
     if (SCATTER) {
       __syncthreads();
 #pragma unroll
       for (int k = 0; k < BK_K; k++) {
-#pragma unroll
-        for (int q = 0; q < 8; q++) {
-          const uint32_t e = pr[k][q];
-          if (e == BK_EMPTY) continue;
-          const uint32_t slot = e & (BK_HS - 1);
-          const uint32_t key = hkey[slot];
-          const uint32_t tile = key >> 1;
-          const uint32_t r = hcnt[slot] + (e >> BK_HS_BITS);
-          const uint32_t pos = (key & 1u) ? off[tile + 1] - 1u - r : off[tile] + r;
-          if ((int64_t)pos < cap) {
-            store4<T>(sorted, pos, px[k], py[k], pz[k], pv[k]);
-            if (g.has_w) sorted_w[pos] = pw[k];
-          }
+        const uint32_t e = pr[k];
+        if (e == BK_EMPTY) continue;
+        const uint32_t slot = e & (BK_HS - 1);
+        const uint32_t key = hkey[slot];
+        const uint32_t pos = off[key] + hcnt[slot] + (e >> BK_HS_BITS);
+        if ((int64_t)pos < cap) {
+          store4<T>(sorted, pos, px[k], py[k], pz[k], pv[k]);
+          if (g.has_w) sorted_w[pos] = pw[k];
         }
       }
     }
@@ -512,9 +496,80 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_apply(const uint32_t* __restric
   }
 }
 
+// Exclusive scan of n block sums by one workgroup, chunk after chunk with a
+// running carry (any n).
+__global__ __launch_bounds__(SCAN_T) void k_scan_carry(const uint32_t* __restrict__ in, int64_t n,
+                                                       uint32_t* __restrict__ out) {
+  __shared__ uint32_t sh[SCAN_T];
+  uint32_t carry = 0;
+  for (int64_t c0 = 0; c0 < n; c0 += SCAN_CHUNK) {
+    const int64_t base = c0 + (int64_t)threadIdx.x * SCAN_PER;
+    uint32_t v[SCAN_PER];
+    uint32_t sum = 0;
+    for (int q = 0; q < SCAN_PER; q++) {
+      const int64_t i = base + q;
+      v[q] = i < n ? in[i] : 0u;
+      sum += v[q];
+    }
+    uint32_t tot;
+    uint32_t ex = block_excl_scan(sum, sh, &tot) + carry;
+    for (int q = 0; q < SCAN_PER; q++) {
+      const int64_t i = base + q;
+      if (i < n) out[i] = ex;
+      ex += v[q];
+    }
+    carry += tot;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Tile deposit.
 // ---------------------------------------------------------------------------
+
+// The particle ranges a tile's deposit reads: its own 27 classes, then for
+// each neighbour offset (dx, dy, dz) the classes cx*9 + cy*3 + cz whose taps
+// reach back into this tile (per axis: any class for d = 0, class 2 for the
+// tile below (d = -1), class 0 for the tile above (d = +1)), merged into
+// runs of consecutive classes: 1 + 2 x-faces + 2*3 y-faces + 2*9 z-faces +
+// 4 xy-edges + 4*3 xz-edges + 4*3 yz-edges + 8 corners = 63 runs.
+struct DepRange {
+  signed char d[3];
+  unsigned char lo, hi;   // classes [lo, hi) of the neighbour
+};
+struct DepRanges {
+  DepRange r[64];
+  int n;
+};
+constexpr DepRanges make_dep_ranges() {
+  DepRanges t{};
+  t.n = 0;
+  t.r[t.n++] = DepRange{{0, 0, 0}, 0, 27};
+  for (int dx = -1; dx <= 1; dx++)
+    for (int dy = -1; dy <= 1; dy++)
+      for (int dz = -1; dz <= 1; dz++) {
+        if (dx == 0 && dy == 0 && dz == 0) continue;
+        const int first = t.n;
+        for (int cx = 0; cx < 3; cx++) {
+          if (dx != 0 && cx != (dx < 0 ? 2 : 0)) continue;
+          for (int cy = 0; cy < 3; cy++) {
+            if (dy != 0 && cy != (dy < 0 ? 2 : 0)) continue;
+            const int lo = cx * 9 + cy * 3 + (dz == 0 ? 0 : (dz < 0 ? 2 : 0));
+            const int hi = dz == 0 ? lo + 3 : lo + 1;
+            if (t.n > first && t.r[t.n - 1].hi == lo) {
+              t.r[t.n - 1].hi = (unsigned char)hi;   // contiguous: extend the run
+            } else {
+              t.r[t.n++] = DepRange{{(signed char)dx, (signed char)dy, (signed char)dz},
+                                    (unsigned char)lo, (unsigned char)hi};
+            }
+          }
+        }
+      }
+  return t;
+}
+constexpr DepRanges kDepRanges = make_dep_ranges();
+static_assert(kDepRanges.n == 63, "63 ranges per tile");
+__constant__ DepRanges c_dep_ranges = kDepRanges;
+
 #define DEP_T 512
 #define DEP_K 4       // particle copies per thread kept in registers for both phases
 #define DEP_TRASH 64  // per-lane sink cells for taps outside the tile
@@ -592,18 +647,67 @@ __global__ __launch_bounds__(DEP_T) void k_deposit(const T* __restrict__ sorted,
   const int t0x = ttx * TX, t0y = tty * TY, t0z = ttz * TZ;
   // copies beyond the sorted buffer's capacity were not stored: flag the
   // overflow and deposit only what is there (the result is then discarded)
-  const uint32_t total = off[(int64_t)g.ntile[0] * g.ntile[1] * g.ntile[2]];
-  if (tileid == 0 && threadIdx.x == 0 && (int64_t)total > cap) errs[2] = 1u;
-  const uint32_t s0 = (uint32_t)min((int64_t)off[tileid], cap);
-  const uint32_t s1 = (uint32_t)min((int64_t)off[tileid + 1], cap);
+  if (tileid == 0 && threadIdx.x == 0 && (int64_t)off[g.nbuckets] > cap) errs[2] = 1u;
+  // the tile's 63 ranges (wave 0): start and exclusive prefix of the lengths
+  __shared__ uint32_t r_start[64], r_pre[65];
+  if (threadIdx.x < 64) {
+    const int r = threadIdx.x;
+    uint32_t st = 0, len = 0;
+    if (r < c_dep_ranges.n) {
+      const DepRange d = c_dep_ranges.r[r];
+      const int tt[3] = {ttx, tty, ttz};
+      int nb[3];
+      bool ok = true;
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+        int q = tt[a] + d.d[a];
+        if (q < 0 || q >= g.ntile[a]) {
+          if (g.periodic) q += q < 0 ? g.ntile[a] : -g.ntile[a];
+          else ok = false;
+        }
+        nb[a] = q;
+      }
+      if (ok) {
+        const int64_t b = (((int64_t)nb[2] * g.ntile[1] + nb[1]) * g.ntile[0] + nb[0]) * 27;
+        const uint32_t s0 = (uint32_t)min((int64_t)off[b + d.lo], cap);
+        const uint32_t s1 = (uint32_t)min((int64_t)off[b + d.hi], cap);
+        st = s0;
+        len = s1 - s0;
+      }
+    }
+    uint32_t inc = len;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t u = (uint32_t)__shfl_up((int)inc, o);
+      if (r >= o) inc += u;
+    }
+    r_start[r] = st;
+    r_pre[r] = inc - len;
+    if (r == 63) r_pre[64] = inc;
+  }
+  __syncthreads();
+  const uint32_t total = r_pre[64], own = r_pre[1];
+  // sorted index of the f-th particle of the tile's ranges (f < total): the
+  // last range whose prefix is <= f holds it
+  auto pidx = [&](uint32_t f) -> uint32_t {
+    if (f < own) return r_start[0] + f;
+    int lo = 1, hi = 62;
+#pragma unroll 1
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (r_pre[mid] <= f) lo = mid;
+      else hi = mid - 1;
+    }
+    return r_start[lo] + (f - r_pre[lo]);
+  };
 
   const double wmax = ORDER == 3 ? 0.421875 : 1.0;
-  const double scA = F32 ? tile_scale(s1 - s0, stats[STAT_GMAX_A], wmax) : stats[STAT_SCALE_A];
-  const double scB = F32 ? tile_scale(s1 - s0, stats[STAT_GMAX_B], wmax) : stats[STAT_SCALE_B];
+  const double scA = F32 ? tile_scale(total, stats[STAT_GMAX_A], wmax) : stats[STAT_SCALE_A];
+  const double scB = F32 ? tile_scale(total, stats[STAT_GMAX_B], wmax) : stats[STAT_SCALE_B];
   const double invA = 1.0 / scA, invB = 1.0 / scB;   // powers of two: exact
   const T d0 = (T)g.d[0], d1 = (T)g.d[1], d2 = (T)g.d[2];
   const double alpha = stats[STAT_ALPHA];
-  const bool single = (s1 - s0) <= (uint32_t)(DEP_T * DEP_K);
+  const bool single = total <= (uint32_t)(DEP_T * DEP_K);
 
   T px[DEP_K], py[DEP_K], pz[DEP_K], pa[DEP_K], pb[DEP_K];
   bool ph_has[DEP_K];
@@ -614,14 +718,15 @@ __global__ __launch_bounds__(DEP_T) void k_deposit(const T* __restrict__ sorted,
       for (int c = threadIdx.x; c < n4; c += DEP_T) z4[c] = make_uint4(0u, 0u, 0u, 0u);
     }
     __syncthreads();
-    for (uint32_t base = s0; base < s1; base += DEP_T * DEP_K) {
+    for (uint32_t base = 0; base < total; base += DEP_T * DEP_K) {
       if (ph == 0 || !single) {
 #pragma unroll
         for (int k = 0; k < DEP_K; k++) {
-          const uint32_t i = base + (uint32_t)(k * DEP_T) + threadIdx.x;
-          ph_has[k] = i < s1;
+          const uint32_t f = base + (uint32_t)(k * DEP_T) + threadIdx.x;
+          ph_has[k] = f < total;
           T x = T(0), y = T(0), z = T(0), v = T(0), w = T(1);
           if (ph_has[k]) {
+            const uint32_t i = pidx(f);
             load4<T>(sorted, i, x, y, z, v);
             if (g.has_w) w = sorted_w[i];
           }
@@ -759,12 +864,11 @@ static int run_count_t(DepositArgs& a) {
   // scan counts -> offsets; zero the own/ghost cursors
   int64_t nbk = g.nbuckets;
   int64_t nblk = (nbk + SCAN_CHUNK - 1) / SCAN_CHUNK;
-  if (nblk > SCAN_CHUNK) return CAPOW_ERR_UNSUPPORTED;
   TIMED(a.timer, "scan", (k_scan_reduce<<<(int)nblk, SCAN_T, 0, st>>>(a.counts, nbk, a.bsum)));
-  TIMED(a.timer, "scan", (k_scan_apply<<<1, SCAN_T, 0, st>>>(a.bsum, nblk, nullptr, a.boff, nullptr)));
+  TIMED(a.timer, "scan", (k_scan_carry<<<1, SCAN_T, 0, st>>>(a.bsum, nblk, a.boff)));
   TIMED(a.timer, "scan", (k_scan_apply<<<(int)nblk, SCAN_T, 0, st>>>(a.counts, nbk, a.boff, a.offsets,
                                                                  nullptr)));
-  if (hipMemsetAsync(a.cursor, 0, (size_t)nbk * 2 * sizeof(uint32_t), st) != hipSuccess)
+  if (hipMemsetAsync(a.cursor, 0, (size_t)nbk * sizeof(uint32_t), st) != hipSuccess)
     return CAPOW_ERR_HIP;
   return hipGetLastError() == hipSuccess ? CAPOW_OK : CAPOW_ERR_HIP;
 }

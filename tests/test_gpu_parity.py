@@ -374,9 +374,9 @@ def test_sort_order_and_table_overflow_do_not_change_grids(prec, monkeypatch):
 def test_batch_matches_single_contexts():
     """capow_batch_run (several streams, two grid configs, no host
     synchronisation between mocks) gives bitwise the bins and sums of
-    running each catalogue alone through capow_power; a catalogue whose
-    copies overflow the batch's reserved sort buffer (every particle next to
-    a tile corner) is re-run alone and matches too."""
+    running each catalogue alone through capow_power, including one with
+    every particle next to a tile corner (its taps reach four tiles, so the
+    corner, edge and face class ranges of the sort carry all of it)."""
     import torch
     say("batch vs single contexts")
     Lb = 500.0
@@ -387,8 +387,7 @@ def test_batch_matches_single_contexts():
     counts = [3000, 5000, 4000, 6000, 2500, 4500]
     cats = [mocks.mixed_mock_torch(n, Lb, seed=100 + i) for i, n in enumerate(counts)]
     # 6000 particles in the last cell of a tile in x and y (cell 15.5 of 32^3,
-    # tiles of 16 cells; z is one tile): 4 copies each, above the 2 per
-    # particle the batch reserves
+    # tiles of 16 cells; z is one tile): deposited by 4 tiles each
     corner = torch.full((6000, 4), 15.5 * Lb / 32, device="cuda", dtype=torch.float32)
     corner[:, 3] = 0
     cats.append(corner)

@@ -585,9 +585,12 @@ __device__ __forceinline__ void rt_stage1(cx<T>* v, const cx<T>* twr,
 // natural order in LDS (row stride N + 1), the r2c split, the transposed
 // write.  Ends with the LDS still in use by the write (the caller syncs
 // before reusing it).
+// tws (R2C): the split twiddles w^k, k = 0..N/2, in LDS (w^(N-k) =
+// -conj(w^k)); null: read both from the table in global memory.
 template <typename T, int N1, int N2, int B, bool R2C>
-__device__ __forceinline__ void rt_rest(const cx<T>* __restrict__ tw, cx<T>* lds, int t, int c,
-                                        cx<T>* ob, int64_t out_sk, int64_t nlive) {
+__device__ __forceinline__ void rt_rest(const cx<T>* __restrict__ tw, const cx<T>* tws,
+                                        cx<T>* lds, int t, int c, cx<T>* ob, int64_t out_sk,
+                                        int64_t nlive) {
   constexpr int N = N1 * N2, E = clcm(N1, N2), TS = N / E, G2 = E / N1;
   constexpr int RS = N1 * (N2 + 1);
   constexpr int K = R2C ? N + 1 : N;  // outputs per row
@@ -620,8 +623,16 @@ __device__ __forceinline__ void rt_rest(const cx<T>* __restrict__ tw, cx<T>* lds
       const cx<T> Ov = mk<T>(T(0.5) * (zk.y + zm.y), -T(0.5) * (zk.x - zm.x));
       const cx<T> Em = mk<T>(T(0.5) * (zm.x + zk.x), T(0.5) * (zm.y - zk.y));
       const cx<T> Om = mk<T>(T(0.5) * (zm.y + zk.y), -T(0.5) * (zm.x - zk.x));
-      const cx<T> xk = add<T>(Ev, mul<T>(tw[kk], Ov));
-      const cx<T> xm = add<T>(Em, mul<T>(tw[N - kk], Om));
+      cx<T> wk, wm;
+      if (tws) {
+        wk = tws[kk];
+        wm = mk<T>(-wk.x, wk.y);
+      } else {
+        wk = tw[kk];
+        wm = tw[N - kk];
+      }
+      const cx<T> xk = add<T>(Ev, mul<T>(wk, Ov));
+      const cx<T> xm = add<T>(Em, mul<T>(wm, Om));
       zr[kk] = xk;
       zr[N - kk] = xm;   // kk == 0: X[N]; kk == N/2: the same value twice
     }
@@ -673,6 +684,13 @@ __global__ __launch_bounds__((N1 * N2 / clcm(N1, N2)) * B) void k_fft_rows_t(
 #pragma unroll
     for (int k2 = 0; k2 < N2; k2++) twr[k2] = tw[(R2C ? 2 : 1) * t * k2];
   }
+  // persistent r2c: the split twiddles once per workgroup, after the
+  // transpose area (the first batch's barrier orders the fill)
+  cx<T>* tws = nullptr;
+  if constexpr (R2C && PERSIST) {
+    tws = lds + B * (N + N1);
+    for (int k = threadIdx.x; k <= N / 2; k += TS * B) tws[k] = tw[k];
+  }
   cx<T> v[E];
   auto load = [&](int64_t item) {
     const int gsel = (int)(item / nbx);
@@ -709,7 +727,7 @@ __global__ __launch_bounds__((N1 * N2 / clcm(N1, N2)) * B) void k_fft_rows_t(
     __syncthreads();
     int64_t nlive;
     cx<T>* ob = dest(L0, nlive);
-    rt_rest<T, N1, N2, B, R2C>(tw, lds, t, c, ob, out_sk, nlive);
+    rt_rest<T, N1, N2, B, R2C>(tw, nullptr, lds, t, c, ob, out_sk, nlive);
   } else {
     if (L0 < Lend) load(L0);
     for (int64_t item = L0; item < Lend; item += step) {
@@ -720,7 +738,7 @@ __global__ __launch_bounds__((N1 * N2 / clcm(N1, N2)) * B) void k_fft_rows_t(
       if (PF && item + step < Lend) load(item + step);
       int64_t nlive;
       cx<T>* ob = dest(item, nlive);
-      rt_rest<T, N1, N2, B, R2C>(tw, lds, t, c, ob, out_sk, nlive);
+      rt_rest<T, N1, N2, B, R2C>(tw, tws, lds, t, c, ob, out_sk, nlive);
       // the next batch's first stage overwrites the LDS
       __syncthreads();
     }
@@ -868,7 +886,7 @@ static void launch_rows_t(FftArgs& a, const char* name, T* const* in, T* const* 
   // to spare: one batch per workgroup)
   constexpr bool PS = sizeof(T) == 4, PF = sizeof(T) == 4, TWR = sizeof(T) == 4;
   const void* fn = (const void*)k_fft_rows_t<T, N1, N2, B, R2C, PS, PF, TWR>;
-  const size_t lds = (size_t)B * (N + N1) * sizeof(cx<T>);
+  const size_t lds = ((size_t)B * (N + N1) + (R2C && PS ? N / 2 + 1 : 0)) * sizeof(cx<T>);
   if (lds > 64 * 1024)
     hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   const int64_t groups = (n_inner + B - 1) / B;
@@ -895,7 +913,8 @@ static bool try_rows_t(FftArgs& a, const char* name, int N, T* const* in, T* con
   if (!reg_split(N, &n1, &n2)) return false;
 #define X(A, B_)                                                                               \
   if (n1 == A && n2 == B_) {                                                                    \
-    if (mul == 2 && (size_t)8 * (sizeof(T) == 4 ? 8 : 4) * 2 * (A * B_ + A) <= 160 * 1024)     \
+    if (mul == 2 && (size_t)(sizeof(T) == 4 ? 16 : 8) * sizeof(cx<T>) * (A * B_ + A) +           \
+                        (R2C ? (size_t)(A * B_ / 2 + 1) * sizeof(cx<T>) : 0) <= 160 * 1024)      \
       launch_rows_t<T, A, B_, R2C, 2>(a, name, in, out, n_outer, n_inner, in_so, in_sb, out_so,  \
                                       out_sk, tw);                                             \
     else                                                                                       \

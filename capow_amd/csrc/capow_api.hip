@@ -30,6 +30,7 @@ struct capow_ctx {
   void* scratch[4] = {nullptr, nullptr, nullptr, nullptr};
   int64_t fft_px = 0, fft_py = 0;   // row pitches of the intermediate layouts
   int fft_persist = 1;              // persistent row passes (CAPOW_FFT_PERSIST=0: off)
+  bool dep_halo = true;             // halo-row deposit (CAPOW_DEPOSIT_HALO=0: off)
   // batch order of the row passes, bit per pass (z, y, x): outer index
   // fastest (FP32 default z and x: 128-byte write runs of neighbouring
   // outer rows land close together; measured 45.7 -> 42.7-44.9 ms at 1024^3;
@@ -470,6 +471,7 @@ extern "C" int capow_create(const capow_params* p, capow_ctx** out) {
   if (const char* e = getenv("CAPOW_FFT_COL_MUL")) c->fft_col_mul = atoi(e) == 1 ? 1 : 2;
   if (const char* e = getenv("CAPOW_FFT_ROWS_MUL")) c->fft_rows_mul = atoi(e) == 2 ? 2 : (atoi(e) == 1 ? 1 : 0);
   if (const char* e = getenv("CAPOW_FFT_PERSIST")) c->fft_persist = atoi(e) != 0;
+  if (const char* e = getenv("CAPOW_DEPOSIT_HALO")) c->dep_halo = atoi(e) != 0;
   if (const char* e = getenv("CAPOW_FFT_ORDER")) c->fft_order = atoi(e) & 7;
   rc = setup(c);
   if (rc) {
@@ -601,6 +603,7 @@ static int assign_enqueue(capow_ctx* c, double* h_stats, unsigned int* h_errs, b
   a.errs = c->errs;
   a.stats = c->stats;
   for (int s = 0; s < 4; s++) a.grid[s] = c->grid[s];
+  a.halo = c->dep_halo;
   int rc = run_count(a, c->fp64);
   if (rc) return fail(c, rc, std::string("deposit count failed: ") + hipGetErrorString(hipGetLastError()));
   if (exact) {

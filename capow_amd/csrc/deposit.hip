@@ -610,12 +610,20 @@ __device__ __forceinline__ int zmask(int rowi, int TZ) {
   return ((TZ & (TZ - 1)) == 0) ? (int)((__umul24((uint32_t)rowi, 0x9E3779u) >> 12) & (uint32_t)(TZ - 1)) : 0;
 }
 
-// One workgroup per TX x TY x TZ tile.  The tile's particle copies (own
-// first, then ghosts) are one contiguous range of the sorted array.  Phase 0
-// deposits the main grids (A, B), phase 1 the interlaced grids: LDS holds
-// one phase.  Taps outside the tile go to the lane's sink cell, so the tap
-// loop has no branches.
-template <typename T, int ORDER, bool HASB, int TYC, int TZC>
+// One workgroup per TX x TY x TZ tile, reading its own particles first, then
+// the neighbours' particles whose taps reach it.  Phase 0 deposits the main
+// grids (A, B), phase 1 the interlaced grids: LDS holds one phase.  The tap
+// loop has no branches: taps outside the tile go to the lane's sink cell,
+// or, with HALO (FP32 16x16x32 tiles, two or more tiles along z), every LDS
+// row has DEP_ZLO cells below and DEP_ZHI above the tile's z range, so that
+// all z taps of a particle land in its row at consecutive addresses (halo
+// cells are not written out), and rows outside the tile in x or y add zeros
+// to a row of the tile.  APOS (field A = w v^0 with no weights: A >= 0):
+// the int64 accumulator is B * 2^32 + A, so the int32 taps need no sign
+// borrow.
+#define DEP_ZLO 4   // halo cells below the tile's z range (taps reach -3)
+#define DEP_ZHI 2   // above (taps reach TZ + 2: the next row's first cells)
+template <typename T, int ORDER, bool HASB, int TYC, int TZC, bool HALO, bool APOS>
 __global__ __launch_bounds__(DEP_T) void k_deposit(const T* __restrict__ sorted,
                                                    const T* __restrict__ sorted_w,
                                                    const uint32_t* __restrict__ off,
@@ -625,9 +633,15 @@ __global__ __launch_bounds__(DEP_T) void k_deposit(const T* __restrict__ sorted,
                                                    const double* __restrict__ stats,
                                                    int64_t cap, unsigned int* __restrict__ errs) {
   constexpr bool F32 = sizeof(T) == 4;
-  // static LDS (its base folds into the tap addresses): the largest tile's
-  // accumulators, checked on the host
-  __shared__ __align__(16) unsigned char smem[DEP_LDS_MAX];
+  // static LDS (its base folds into the tap addresses): the accumulators of
+  // the largest tile (TX <= 16; with compile-time extents, of this shape),
+  // checked on the host
+  constexpr int RZ = TZC + DEP_ZLO + DEP_ZHI;   // HALO row length (cells)
+  constexpr int LDSB = HALO ? (16 * TYC * RZ + 2) * 8
+                            : ((TYC && TZC) ? (16 * TYC * TZC + DEP_TRASH) * 8 * (F32 ? 1 : 2)
+                                            : DEP_LDS_MAX);
+  static_assert(!HALO || (F32 && TYC && TZC), "halo rows: FP32, compile-time extents");
+  __shared__ __align__(16) unsigned char smem[LDSB];
   // TYC, TZC: compile-time tile extents (0: the runtime ones)
   const int TX = g.tdim[0];
   const int TY = TYC ? TYC : g.tdim[1], TZ = TZC ? TZC : g.tdim[2];
@@ -714,7 +728,7 @@ __global__ __launch_bounds__(DEP_T) void k_deposit(const T* __restrict__ sorted,
   for (int ph = 0; ph < (g.interlace ? 2 : 1); ph++) {
     {
       uint4* z4 = reinterpret_cast<uint4*>(smem);
-      const int n4 = F32 ? nacc / 2 : nacc;   // 16-byte words
+      const int n4 = HALO ? (TX * TY * RZ + 2) / 2 : (F32 ? nacc / 2 : nacc);   // 16-byte words
       for (int c = threadIdx.x; c < n4; c += DEP_T) z4[c] = make_uint4(0u, 0u, 0u, 0u);
     }
     __syncthreads();
@@ -749,6 +763,58 @@ __global__ __launch_bounds__(DEP_T) void k_deposit(const T* __restrict__ sorted,
         const AxisOut<T> az = axis_taps<T, ORDER>(z, (T)g.lo[2], (T)g.hi[2], d2, (T)g.L[2], g.n[2], false, g.los, g.edge, (T)g.invd[2]);
         const bool ok = ax.ok && ay.ok && az.ok;
         const T va = ok ? pa[k] : T(0), vb = ok ? pb[k] : T(0);
+        if constexpr (HALO) {
+          // x, y: tile-local tap rows; rows outside the tile add zeros to row
+          // `lane` (a row of the tile: TX * TY >= 64)
+          int lx[ORDER], ly[ORDER];
+          bool vx[ORDER], vy[ORDER];
+#pragma unroll
+          for (int t = 0; t < ORDER; t++) {
+            int gx = ax.first + t, gy = ay.first + t;
+            if (g.periodic) {
+              gx += gx < 0 ? g.n[0] : (gx >= g.n[0] ? -g.n[0] : 0);
+              gy += gy < 0 ? g.n[1] : (gy >= g.n[1] ? -g.n[1] : 0);
+            }
+            lx[t] = gx - t0x;
+            ly[t] = gy - t0y;
+            vx[t] = (unsigned)lx[t] < (unsigned)TX;
+            vy[t] = (unsigned)ly[t] < (unsigned)TY;
+          }
+          // z: the first tap relative to the tile (periodic: the nearest
+          // image); a particle of this tile's lists has it in [-3, TZ], the
+          // clamp only keeps any other in the halo
+          int lz0 = az.first - t0z;
+          if (g.periodic) lz0 = canon_off(lz0, g.n[2]);
+          lz0 = lz0 < -3 ? -3 : (lz0 > TZ ? TZ : lz0);
+          const int zb8 = (lz0 + DEP_ZLO) * 8;
+#pragma unroll
+          for (int tx = 0; tx < ORDER; tx++) {
+#pragma unroll
+            for (int ty = 0; ty < ORDER; ty++) {
+              const bool vxy = vx[tx] && vy[ty];
+              const int rowi = vxy ? lx[tx] * TY + ly[ty] : lane;
+              const T wxy = vxy ? ax.w[tx] * ay.w[ty] : T(0);
+              const T qa0 = va * wxy, qb0 = vb * wxy;
+              unsigned char* rb = smem + rowi * (RZ * 8) + zb8;
+#pragma unroll
+              for (int tz = 0; tz < ORDER; tz++) {
+                const int qa = round_rpi((float)(qa0 * az.w[tz]));
+                const int qb = HASB ? round_rpi((float)(qb0 * az.w[tz])) : 0;
+                unsigned long long packed;
+                if constexpr (APOS) {
+                  // B * 2^32 + A, A >= 0
+                  packed = ((unsigned long long)(uint32_t)qb << 32) | (uint32_t)qa;
+                } else {
+                  // A * 2^32 + B with B sign-extended
+                  const uint32_t hi = (uint32_t)qa + (uint32_t)(qb >> 31);
+                  packed = ((unsigned long long)hi << 32) | (uint32_t)qb;
+                }
+                atomicAdd(reinterpret_cast<unsigned long long*>(rb + tz * 8), packed);
+              }
+            }
+          }
+          continue;
+        }
         // tile-local tap coordinates; taps outside the tile are invalid
         int lx[ORDER], ly[ORDER], lz[ORDER], lz8[ORDER];
         bool vx[ORDER], vy[ORDER], vz[ORDER];
@@ -815,8 +881,9 @@ __global__ __launch_bounds__(DEP_T) void k_deposit(const T* __restrict__ sorted,
     for (int c2 = threadIdx.x; c2 < ncell / 2; c2 += DEP_T) {
       const int lz = 2 * zp, rowi = lxx * TY + ly;
       const int64_t gi = ((int64_t)(t0x + lxx) * g.n[1] + (t0y + ly)) * g.rowlen + (t0z + lz);
-      const int m = zmask(rowi, TZ);
-      const int cs = rowi * TZ + ((lz ^ m) & ~1);   // aligned pair holding z = lz, lz+1
+      const int m = HALO ? 0 : zmask(rowi, TZ);
+      // aligned pair holding z = lz, lz+1
+      const int cs = HALO ? rowi * RZ + DEP_ZLO + lz : rowi * TZ + ((lz ^ m) & ~1);
       const bool swap = (m & 1) != 0;
       zp += step_z;
       ly += step_y;
@@ -826,8 +893,18 @@ __global__ __launch_bounds__(DEP_T) void k_deposit(const T* __restrict__ sorted,
       if constexpr (F32) {
         const ulonglong2 q = *reinterpret_cast<const ulonglong2*>(acc64 + cs);
         const long long p0 = (long long)(swap ? q.y : q.x), p1 = (long long)(swap ? q.x : q.y);
-        const int b0 = (int)(unsigned)(p0 & 0xffffffffll), b1 = (int)(unsigned)(p1 & 0xffffffffll);
-        const int a0 = (int)((p0 - (long long)b0) >> 32), a1 = (int)((p1 - (long long)b1) >> 32);
+        int a0, a1, b0, b1;
+        if (APOS && HALO) {
+          a0 = (int)(unsigned)(p0 & 0xffffffffll);
+          a1 = (int)(unsigned)(p1 & 0xffffffffll);
+          b0 = (int)(p0 >> 32);
+          b1 = (int)(p1 >> 32);
+        } else {
+          b0 = (int)(unsigned)(p0 & 0xffffffffll);
+          b1 = (int)(unsigned)(p1 & 0xffffffffll);
+          a0 = (int)((p0 - (long long)b0) >> 32);
+          a1 = (int)((p1 - (long long)b1) >> 32);
+        }
         store2<T>(dA + gi, (T)((double)a0 * invA - meanA), (T)((double)a1 * invA - meanA));
         if (HASB) store2<T>(dB + gi, (T)((double)b0 * invB), (T)((double)b1 * invB));
       } else {
@@ -893,19 +970,34 @@ static int run_scatter_t(DepositArgs& a) {
   // integer multiplies in the tap addressing)
   const int ty = g.tdim[1], tz = g.tdim[2];
   const bool t1632 = ty == 16 && tz == 32, t1616 = ty == 16 && tz == 16;
-#define DEP_LAUNCH(HB, TYC, TZC)                                                               \
+#define DEP_LAUNCH(HB, TYC, TZC, HALO, APOS)                                                   \
   TIMED(a.timer, "deposit",                                                                    \
-        (k_deposit<T, ORDER, HB, TYC, TZC><<<(unsigned)ntiles, DEP_T, 0, st>>>(                 \
+        (k_deposit<T, ORDER, HB, TYC, TZC, HALO, APOS><<<(unsigned)ntiles, DEP_T, 0, st>>>(     \
             (const T*)a.sorted, (const T*)a.sorted_w, a.offsets, g, (T*)a.grid[0],              \
             (T*)a.grid[1], (T*)a.grid[2], (T*)a.grid[3], a.stats, a.sorted_cap, a.errs)))
-  if (hasB) {
-    if (sizeof(T) == 4 && t1632) DEP_LAUNCH(true, 16, 32);
-    else if (sizeof(T) == 8 && t1616) DEP_LAUNCH(true, 16, 16);
-    else DEP_LAUNCH(true, 0, 0);
+  // halo rows: FP32 16x16x32 tiles with two or more tiles along z (one tile
+  // wraps its own z taps); A >= 0 when it is the plain count (no weights)
+  const bool halo = a.halo && sizeof(T) == 4 && t1632 && g.ntile[2] >= 2 && g.tdim[0] * 16 >= 64;
+  const bool apos = !g.has_w && g.fa == 0;
+  bool done = false;
+  if constexpr (sizeof(T) == 4) {
+    if (halo) {
+      if (hasB && apos) DEP_LAUNCH(true, 16, 32, true, true);
+      else if (hasB) DEP_LAUNCH(true, 16, 32, true, false);
+      else if (apos) DEP_LAUNCH(false, 16, 32, true, true);
+      else DEP_LAUNCH(false, 16, 32, true, false);
+      done = true;
+    }
+  }
+  if (done) {
+  } else if (hasB) {
+    if (sizeof(T) == 4 && t1632) DEP_LAUNCH(true, 16, 32, false, false);
+    else if (sizeof(T) == 8 && t1616) DEP_LAUNCH(true, 16, 16, false, false);
+    else DEP_LAUNCH(true, 0, 0, false, false);
   } else {
-    if (sizeof(T) == 4 && t1632) DEP_LAUNCH(false, 16, 32);
-    else if (sizeof(T) == 8 && t1616) DEP_LAUNCH(false, 16, 16);
-    else DEP_LAUNCH(false, 0, 0);
+    if (sizeof(T) == 4 && t1632) DEP_LAUNCH(false, 16, 32, false, false);
+    else if (sizeof(T) == 8 && t1616) DEP_LAUNCH(false, 16, 16, false, false);
+    else DEP_LAUNCH(false, 0, 0, false, false);
   }
 #undef DEP_LAUNCH
   return hipGetLastError() == hipSuccess ? CAPOW_OK : CAPOW_ERR_HIP;

@@ -89,9 +89,9 @@ struct DepositArgs {
   const void* wts;        // n or null
   int64_t n;
   int count_blocks;
-  uint32_t* counts;       // ntiles (zeroed)
-  uint32_t* offsets;      // ntiles + 1
-  uint32_t* cursor;       // 2*ntiles: own-copy and ghost-copy cursors
+  uint32_t* counts;       // nbuckets = ntiles * 27 (zeroed)
+  uint32_t* offsets;      // nbuckets + 1
+  uint32_t* cursor;       // nbuckets: scatter cursors
   uint32_t* bsum;         // scan block sums
   uint32_t* boff;         // scan block offsets (+1)
   void* sorted;           // sorted_cap x 4 (one row per particle copy)
@@ -102,9 +102,9 @@ struct DepositArgs {
   unsigned int* errs;     // 2 (zeroed): [0] LOS=0 out-of-box, [1] reach violations
   double* stats;          // STAT_TOTAL
   void* grid[4];          // A, A_int, B, B_int (null when unused)
+  bool halo = true;       // halo-row deposit where the tile shape allows
 };
-// count + stats + scan; then (after the host has sized the sorted buffer
-// from offsets[ntiles]) scatter + tile deposit
+// count + stats + scan; then scatter + tile deposit
 int run_count(DepositArgs& a, bool fp64);
 int run_scatter_deposit(DepositArgs& a, bool fp64);
 

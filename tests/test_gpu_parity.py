@@ -118,6 +118,64 @@ def test_deposit_matches_oracle(case, oracle):
     ps.close()
 
 
+HALO_CASES = [
+    # name, (nx, ny, nz), order, interlace, los, momentum, weights
+    ("tsc_int_los0_m1", (64, 64, 64), 3, 1, 0, 1, False),
+    ("cic_int_los3_m3", (96, 64, 128), 2, 1, 3, 3, False),
+    ("ngp_los0_m2", (64, 48, 64), 1, 0, 0, 2, False),
+    ("tsc_los1_m1_w", (64, 32, 96), 3, 0, 1, 1, True),
+    ("tsc_int_los2_m6", (48, 64, 64), 3, 1, 2, 6, False),
+    ("cic_los0_m0_w", (32, 64, 64), 2, 0, 0, 0, True),
+]
+
+
+@pytest.mark.parametrize("case", HALO_CASES, ids=[c[0] for c in HALO_CASES])
+def test_fp32_halo_deposit_matches_plain_tiles(case, monkeypatch):
+    """FP32 tiles of 16x16x32 with two or more tiles along z take the
+    halo-row deposit (and, for an unweighted field w v^0, the packing
+    without sign borrow).  Both round the same float products to the same
+    integer quanta as the plain tile deposit, whose FP64 twin matches the
+    oracle, and integer sums do not depend on the order: the grids must be
+    bitwise equal.  Particles sit on every edge rule of every axis, and
+    weights (when used) take both signs."""
+    name, (nx, ny, nz), order, inter, los, mom, weighted = case
+    say("halo deposit case", name)
+    p = api.make_params(nx=nx, ny=ny, nz=nz, xmax=L, ymax=L, zmax=L, interp_order=order,
+                        do_interlacing=inter, los=los, momentum=mom, precision=api.FP32,
+                        nk=10, maxk=0.3)
+    rng = np.random.default_rng(nx + 3 * ny + 7 * nz)
+    pos, vel = make_catalogue(nx, 30000, los, nz, mom)
+    extra = np.concatenate([edge_particles(ny, L, los, rng), edge_particles(nz, L, los, rng)])
+    pos = np.concatenate([pos, extra])
+    vel = np.concatenate([vel, rng.normal(0, 300.0, size=(len(extra), 3))])
+    xyzv = mocks.xyzv(pos, vel, los)
+    w = rng.uniform(-0.5, 2.0, size=len(pos)) if weighted else None
+
+    def grids():
+        ps = api.PowerSpectrum(p)
+        ps.set_particles(xyzv, w)
+        ps.assign()
+        out = []
+        for s in range(4):
+            try:
+                # real cells only: the deposit does not write the row padding
+                out.append(ps.get_grid(s).reshape(nx, ny, -1)[:, :, :nz].copy())
+            except RuntimeError:
+                out.append(None)
+        ps.close()
+        return out
+
+    halo = grids()
+    monkeypatch.setenv("CAPOW_DEPOSIT_HALO", "0")
+    plain = grids()
+    monkeypatch.delenv("CAPOW_DEPOSIT_HALO")
+    assert any(g is not None for g in halo)
+    for s in range(4):
+        assert (halo[s] is None) == (plain[s] is None), s
+        if halo[s] is not None:
+            assert np.array_equal(halo[s].view(np.uint32), plain[s].view(np.uint32)), (name, s)
+
+
 @pytest.mark.parametrize("prec", [api.FP64, api.FP32])
 @pytest.mark.parametrize("shape", [(16, 16, 16), (24, 20, 30), (64, 48, 32), (128, 128, 128),
                                    (30, 40, 50)])

@@ -54,7 +54,7 @@ struct capow_ctx {
   // buckets / sort
   uint32_t *counts = nullptr, *offsets = nullptr, *cursor = nullptr, *bsum = nullptr,
            *boff = nullptr;
-  int count_blocks = 1024;   // bucket-kernel blocks: 4 per CU
+  int count_blocks = 3072;   // bucket-kernel blocks: 12 per CU (measured best at C2)
   double* partials = nullptr;
   unsigned long long* gmax = nullptr;
   unsigned int* errs = nullptr;

@@ -210,7 +210,7 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
 // falls back to the global counter.
 #define BK_T 256
 #define BK_K 4
-#define BK_HS_BITS 12
+#define BK_HS_BITS 11
 #define BK_HS (1 << BK_HS_BITS)
 #define BK_PROBE 32
 #define BK_EMPTY 0xffffffffu

@@ -32,8 +32,9 @@ STAGE_PATTERNS = [
     ("scan", r"k_scan_"),
     ("bucket_scatter", r"k_bucket<.*true>"),
     ("deposit", r"k_deposit"),
-    # transposed pipeline: the r2c row pass is z, the two c2c passes y then x
-    ("fft_yx", r"k_fft_rows_t<.*false>"),
+    # transposed pipeline: the r2c row pass (5th template argument true) is
+    # z, the two c2c passes y then x
+    ("fft_yx", r"k_fft_rows_t<[^,]+, \d+, \d+, \d+, false"),
     ("fft_z", r"k_fft_rows"),
     ("fft_yx", r"k_fft_cols"),
     ("bin_counts", r"k_bin_counts"),

@@ -106,7 +106,7 @@ def run_batch(args):
     ntot = int(sum(c.shape[0] for c in cats))
     log("batch: %d mocks, %d particles generated in %.1f s" % (len(cats), ntot, time.perf_counter() - t0))
     b = api.Batch(cfgs, max_particles=int(counts.max()), nstreams=args.mock_streams)
-    b.run(cats[:16], which[:16])          # warmup
+    b.run(cats, which)                    # warmup: one untimed pass over every catalogue
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     res, stats, ms = b.run(cats, which)
@@ -265,7 +265,7 @@ def main():
     ap.add_argument("--grid", type=int, default=1024, help="grid size (1024 = the workload)")
     ap.add_argument("--lattice", type=int, default=512)
     ap.add_argument("--mocks", type=int, default=512, help="batched mocks (0: skip)")
-    ap.add_argument("--mock-streams", type=int, default=4)
+    ap.add_argument("--mock-streams", type=int, default=6)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--profile-pass", action="store_true",
                     help="for rocprofv3 counter runs: one FP32 step and a %d-byte calibration "

@@ -927,11 +927,14 @@ static bool try_rows_t(FftArgs& a, const char* name, int N, T* const* in, T* con
   return false;
 }
 
-// Out-of-place pipeline: every pass reads whole rows and writes 64-byte
-// runs of the next pass's row axis.
+// Out-of-place pipeline: every pass reads whole rows and writes runs of the
+// next pass's row axis.
 //   z (r2c): grid [x][y][z]   -> scratch [x][kz][y]   (row pitch py)
-//   y (c2c): scratch          -> grid    [kz][ky][x]  (row pitch px)
+//   y (c2c): scratch          -> grid    [ky][kz][x]  (row pitch px; FP64:
+//                                          [kz][ky][x])
 //   x (c2c): grid             -> scratch [kx][ky][kz] (row pitch rowc)
+// ([ky][kz][x] puts the rows of one x-pass batch -- consecutive kz at one
+// ky -- next to each other: FP32 x pass 7.6 -> 7.0 ms at 1024^3.)
 // The result is in the scratch buffers; the caller swaps them with the grids.
 template <typename T>
 static bool run_fft_transposed(FftArgs& a) {
@@ -954,11 +957,16 @@ static bool run_fft_transposed(FftArgs& a) {
   if (!try_rows_t<T, true>(a, "fft_z", nz / 2, G, S, nx, ny, (int64_t)ny * rowc, rowc,
                            nzh * py, py, a.tw[2], mz))
     return false;
-  if (!try_rows_t<T, false>(a, "fft_y", ny, S, G, nzh, nx, py, nzh * py, (int64_t)ny * px, px,
-                            a.tw[1], my))
+  // FP32: [ky][kz][x] between the y and x passes; FP64 (64-byte y runs):
+  // [kz][ky][x], which keeps a y batch's output runs in one plane
+  const bool kyz = sizeof(T) == 4;
+  const int64_t y_so = kyz ? px : (int64_t)ny * px, y_sk = kyz ? nzh * px : px;
+  const int64_t x_so = kyz ? nzh * px : px, x_sb = kyz ? px : (int64_t)ny * px;
+  if (!try_rows_t<T, false>(a, "fft_y", ny, S, G, nzh, nx, py, nzh * py, y_so, y_sk, a.tw[1],
+                            my))
     return false;
-  if (!try_rows_t<T, false>(a, "fft_x", nx, G, S, ny, nzh, px, (int64_t)ny * px, rowc,
-                            (int64_t)ny * rowc, a.tw[0], mx))
+  if (!try_rows_t<T, false>(a, "fft_x", nx, G, S, ny, nzh, x_so, x_sb, rowc, (int64_t)ny * rowc,
+                            a.tw[0], mx))
     return false;
   return true;
 }

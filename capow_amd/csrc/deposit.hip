@@ -67,12 +67,16 @@ __device__ __forceinline__ void store4(T* p, int64_t i, T x, T y, T z, T v) {
   reinterpret_cast<V*>(p)[i] = q;
 }
 
+// FP32: a non-temporal store (the grid is read back only by the FFT's first
+// pass, long after the write-out; streaming it past the caches measured
+// about 3% faster for the whole deposit).  FP64: a plain store (no gain).
 template <typename T>
 __device__ __forceinline__ void store2(T* p, T a, T b) {
   if constexpr (sizeof(T) == 4) {
-    float2 q;
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    f2 q;
     q.x = a; q.y = b;
-    *reinterpret_cast<float2*>(p) = q;
+    __builtin_nontemporal_store(q, reinterpret_cast<f2*>(p));
   } else {
     double2 q;
     q.x = a; q.y = b;

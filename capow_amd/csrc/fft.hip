@@ -638,10 +638,23 @@ __device__ __forceinline__ void rt_rest(const cx<T>* __restrict__ tw, const cx<T
     }
     __syncthreads();
   }
-  // transposed write: runs of B consecutive inner values per output k
+  // transposed write: runs of B consecutive inner values per output k.
+  // FP64 complex-to-complex passes stream it past the caches (non-temporal;
+  // measured faster there, slower for FP32 and for the FP64 r2c pass).
+  constexpr bool NT = sizeof(T) == 8 && !R2C;
   for (int q = threadIdx.x; q < K * B; q += TS * B) {
     const int b = q % B, kk = q / B;
-    if (b < nlive) ob[(int64_t)kk * out_sk + b] = lds[b * (N + 1) + kk];
+    if (b < nlive) {
+      const cx<T> val = lds[b * (N + 1) + kk];
+      if constexpr (NT) {
+        typedef double d2 __attribute__((ext_vector_type(2)));
+        d2 w;
+        w.x = val.x; w.y = val.y;
+        __builtin_nontemporal_store(w, reinterpret_cast<d2*>(ob + (int64_t)kk * out_sk + b));
+      } else {
+        ob[(int64_t)kk * out_sk + b] = val;
+      }
+    }
   }
 }
 

@@ -506,6 +506,300 @@ __global__ __launch_bounds__(512) void k_bin_rows(const T* __restrict__ gA,
   }
 }
 
+// ---- Mirror-group form of the branch-free binning ----
+//
+// With f(N - i) = -f(i) exactly in the frequency tables and the deconvolution
+// even in f (both checked on the host), the rows (i, j), (NX - i, j),
+// (i, NY - j) and (NX - i, NY - j) share |k|, the k bin, the window
+// correction and the Hermitian weight NM of every kz (NX, NY even); mu agrees
+// up to its sign, so L_2 and L_4 agree, and with the line of sight along z
+// mu itself does.  Only the interlace phase differs from row to row.  A
+// group's per-mode geometry is therefore evaluated once, its (up to) four
+// powers are summed, and the sum takes one set of LDS adds.
+
+// The rows of one group: canonical (i, j), 0 <= i <= NX/2, 0 <= j <= NY/2,
+// and its partners where they are distinct rows (live).  Dead slots repeat
+// the canonical row and are read as zero.
+struct Group {
+  int i, j;
+  double fx, fy, fxy, corxy;
+  int64_t row[4];
+  bool live[4];
+  double qr[4], qi[4];   // interlace phase of each row (x phase * y phase)
+};
+
+__device__ __forceinline__ void group_rows(int g, int hy1, const BinArgsDev& a, int& i, int& j,
+                                           int64_t* row, bool* live) {
+  i = g / hy1;
+  j = g - i * hy1;
+  const bool mx = i > 0 && 2 * i < a.nx, my = j > 0 && 2 * j < a.ny;
+  const int i2 = mx ? a.nx - i : i, j2 = my ? a.ny - j : j;
+  row[0] = (int64_t)i * a.ny + j;
+  row[1] = (int64_t)i2 * a.ny + j;
+  row[2] = (int64_t)i * a.ny + j2;
+  row[3] = (int64_t)i2 * a.ny + j2;
+  live[0] = true;
+  live[1] = mx;
+  live[2] = my;
+  live[3] = mx && my;
+}
+
+__device__ __forceinline__ Group group_of(int g, int hy1, const double* __restrict__ tabs,
+                                          const BinArgsDev& a) {
+  Group G;
+  group_rows(g, hy1, a, G.i, G.j, G.row, G.live);
+  G.fx = tabs[a.ofx + G.i];
+  G.fy = tabs[a.ofy + G.j];
+  G.fxy = dadd(dmul(G.fx, G.fx), dmul(G.fy, G.fy));
+  G.corxy = tabs[a.ocx + G.i] * tabs[a.ocy + G.j];
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    G.qr[r] = 1.0;
+    G.qi[r] = 0.0;
+    if (a.interlace) {
+      const int ii = (int)(G.row[r] / a.ny), jj = (int)(G.row[r] - (int64_t)ii * a.ny);
+      const double pxr = tabs[a.opx + 2 * ii], pxi = tabs[a.opx + 2 * ii + 1];
+      const double pyr = tabs[a.opy + 2 * jj], pyi = tabs[a.opy + 2 * jj + 1];
+      G.qr[r] = pxr * pyr - pxi * pyi;
+      G.qi[r] = pxr * pyi + pxi * pyr;
+    }
+  }
+  return G;
+}
+
+// One mode kz of a group: v[r][0..3] = A, A_int, B, B_int of row r (zero for
+// dead rows and absent grids).  np[s] = NM cor * (sum over the rows of the
+// power of spectrum s).
+template <typename C>
+__device__ __forceinline__ ModeVals group_mode(const Group& G, int k, double fz, double corz,
+                                               double pzr, double pzi, const C (&v)[4][4],
+                                               const BinArgsDev& a) {
+  ModeVals V;
+  const double twopi = 2.0 * 3.14159265358979323846;
+  const KBin kb = kbin_fast(G.fxy, fz, a);
+  const double fktot = kb.fktot;
+  V.near = kb.near;
+  V.kbin = kb.kbin;
+  V.ok = kb.ok;
+  const double fl = a.los == 1 ? G.fx : (a.los == 2 ? G.fy : fz);
+  double rk = __builtin_amdgcn_rcp(fktot > 0 ? fktot : 1.0);
+  rk = rk * (2.0 - fktot * rk);
+  rk = rk * (2.0 - fktot * rk);
+  const double mu = fktot > 0 ? (twopi * fl) * rk : 0.0;
+  const double wc = nm_weight(G.i, G.j, k, a.nx) * (G.corxy * corz);
+  double ps[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    double ar = (double)v[r][0].x, ai = (double)v[r][0].y, br = 0, bi = 0;
+    double cr = 1.0, ci = 0.0;
+    if (a.interlace) {
+      cr = G.qr[r] * pzr - G.qi[r] * pzi;
+      ci = G.qr[r] * pzi + G.qi[r] * pzr;
+      ar += (double)v[r][1].x * cr - (double)v[r][1].y * ci;
+      ai += (double)v[r][1].x * ci + (double)v[r][1].y * cr;
+    }
+    if (a.hasB) {
+      br = (double)v[r][2].x;
+      bi = (double)v[r][2].y;
+      if (a.interlace) {
+        br += (double)v[r][3].x * cr - (double)v[r][3].y * ci;
+        bi += (double)v[r][3].x * ci + (double)v[r][3].y * cr;
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < 3; s++) {
+      double p;
+      switch (a.kind[s]) {
+        case SPEC_AUTO_A: p = ar * ar + ai * ai; break;
+        case SPEC_AUTO_B: p = br * br + bi * bi; break;
+        case SPEC_CROSS_IM: p = ar * bi - ai * br; break;
+        default: p = ar * br + ai * bi; break;
+      }
+      ps[s] += p;
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < 3; s++) V.np[s] = wc * ps[s];
+  const double mu2 = mu * mu;
+  V.mu = mu;
+  V.L2 = 1.5 * mu2 - 0.5;
+  V.L4 = 4.375 * mu2 * mu2 - 3.75 * mu2 + 0.375;
+  return V;
+}
+
+// The four grids of the live rows of a group at MPL consecutive modes
+// k0 .. k0 + MPL - 1 of every row (one 8 * MPL-byte (FP32) or 16-byte (FP64)
+// load per row and grid; lanes beyond the row load nothing).
+template <typename C, int MPL>
+__device__ __forceinline__ void group_load(const C* __restrict__ A, const C* __restrict__ Ai,
+                                           const C* __restrict__ B, const C* __restrict__ Bi,
+                                           const int64_t* row, const bool* live, int64_t rowc,
+                                           int k0, int nzh, bool il, bool hb, C (&v)[4][4][MPL]) {
+  const C zero = {0, 0};
+  const bool in = k0 < nzh;
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const int64_t m = row[r] * rowc + k0;
+    const bool lv = live[r] && in;
+    const C* src[4] = {A, Ai, B, Bi};
+    const bool use[4] = {true, il, hb, hb && il};
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+      if (lv && use[g]) {
+        load_run<C, MPL>(src[g] + m, v[r][g]);
+      } else {
+#pragma unroll
+        for (int u = 0; u < MPL; u++) v[r][g][u] = zero;
+      }
+    }
+  }
+}
+
+// A wave walks groups g = w0, w0 + nw, ...; each group in chunks of 64 MPL
+// modes (lane l: kz = 64 MPL c + MPL l + u); the next chunk's loads are in
+// flight while the current one is binned.  Modes kz >= ktail (the few at the
+// end of the row that do not fill a chunk) are binned afterwards with one
+// group per lane.
+template <typename T, int MPL>
+__global__ __launch_bounds__(512) void k_bin_mirror(const T* __restrict__ gA,
+                                                    const T* __restrict__ gAi,
+                                                    const T* __restrict__ gB,
+                                                    const T* __restrict__ gBi,
+                                                    const double* __restrict__ tabs, BinArgsDev a,
+                                                    int nch, double* __restrict__ partials) {
+  typedef typename CT<T>::t C;
+  extern __shared__ double sm[];
+  constexpr int CH = 64 * MPL;
+  const int NK = a.nk;
+  const int nl = a.odd ? 5 : 3;
+  const int F = a.nspec * nl;
+  const int lcp = F * NK;
+  const int per_copy = F * NK + NK;
+  const int copies = a.copies;
+  const int nzh = a.nzh;
+  const int ktail = nch * CH;
+  double* zt = sm + copies * lcp;   // fz, corz, phz.re, phz.im per kz
+  for (int q = threadIdx.x; q < lcp * copies; q += blockDim.x) sm[q] = 0.0;
+  for (int q = threadIdx.x; q < nzh; q += blockDim.x) {
+    zt[4 * q] = tabs[a.ofz + q];
+    zt[4 * q + 1] = tabs[a.ocz + q];
+    zt[4 * q + 2] = tabs[a.opz + 2 * q];
+    zt[4 * q + 3] = tabs[a.opz + 2 * q + 1];
+  }
+  __syncthreads();
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63;
+  double* accS = sm + wave * lcp;
+  const C* A = reinterpret_cast<const C*>(gA);
+  const C* Ai = reinterpret_cast<const C*>(gAi);
+  const C* B = reinterpret_cast<const C*>(gB);
+  const C* Bi = reinterpret_cast<const C*>(gBi);
+  const bool il = a.interlace != 0, hb = a.hasB != 0;
+  const int64_t rowc = a.rowc;
+  const int hy1 = a.ny / 2 + 1;
+  const int ng = (a.nx / 2 + 1) * hy1;
+  const int nw = gridDim.x * copies;
+  const int w0 = blockIdx.x * copies + wave;
+  if (nch > 0 && w0 < ng) {
+    C cur[4][4][MPL];
+    {
+      int i0, j0;
+      int64_t row[4];
+      bool live[4];
+      group_rows(w0, hy1, a, i0, j0, row, live);
+      group_load<C, MPL>(A, Ai, B, Bi, row, live, rowc, MPL * lane, nzh, il, hb, cur);
+    }
+    for (int g = w0; g < ng; g += nw) {
+      const Group G = group_of(g, hy1, tabs, a);
+      for (int c = 0; c < nch; c++) {
+        const int gn = c + 1 < nch ? g : g + nw, cn = c + 1 < nch ? c + 1 : 0;
+        C nxt[4][4][MPL];
+        if (gn < ng) {
+          int in_, jn;
+          int64_t row[4];
+          bool live[4];
+          group_rows(gn, hy1, a, in_, jn, row, live);
+          group_load<C, MPL>(A, Ai, B, Bi, row, live, rowc, cn * CH + MPL * lane, nzh, il, hb,
+                             nxt);
+        }
+        ModeVals MV[MPL];
+#pragma unroll
+        for (int u = 0; u < MPL; u++) {
+          const int k = c * CH + MPL * lane + u;
+          const double4 z = *reinterpret_cast<const double4*>(zt + 4 * k);
+          C v[4][4];
+#pragma unroll
+          for (int r = 0; r < 4; r++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) v[r][q] = cur[r][q][u];
+          MV[u] = group_mode<C>(G, k, z.x, z.y, z.z, z.w, v, a);
+        }
+        bool anynear = false;
+#pragma unroll
+        for (int u = 0; u < MPL; u++) anynear |= MV[u].near;
+        if (__any(anynear)) {
+#pragma unroll
+          for (int u = 0; u < MPL; u++)
+            if (MV[u].near) {
+              const KBin e = kbin_exact(G.fxy, zt[4 * (c * CH + MPL * lane + u)], a);
+              MV[u].kbin = e.kbin;
+              MV[u].ok = e.ok;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < MPL; u++)
+          if (MV[u].ok) mode_add(MV[u], a, accS, nl);
+        if (gn < ng) {
+#pragma unroll
+          for (int r = 0; r < 4; r++)
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+#pragma unroll
+              for (int u = 0; u < MPL; u++) cur[r][q][u] = nxt[r][q][u];
+        }
+      }
+    }
+  }
+  if (ktail < nzh) {
+    // the end of every row: one group per lane
+    const C zero = {0, 0};
+    for (int gb = w0 * 64; gb < ng; gb += nw * 64) {
+      const int g = gb + lane;
+      if (g < ng) {
+        const Group G = group_of(g, hy1, tabs, a);
+        for (int k = ktail; k < nzh; k++) {
+          C v[4][4];
+#pragma unroll
+          for (int r = 0; r < 4; r++) {
+            const int64_t m = G.row[r] * rowc + k;
+            const bool lv = G.live[r];
+            v[r][0] = lv ? A[m] : zero;
+            v[r][1] = lv && il ? Ai[m] : zero;
+            v[r][2] = lv && hb ? B[m] : zero;
+            v[r][3] = lv && hb && il ? Bi[m] : zero;
+          }
+          const double4 z = *reinterpret_cast<const double4*>(zt + 4 * k);
+          ModeVals mv = group_mode<C>(G, k, z.x, z.y, z.z, z.w, v, a);
+          if (mv.near) {
+            const KBin e = kbin_exact(G.fxy, z.x, a);
+            mv.kbin = e.kbin;
+            mv.ok = e.ok;
+          }
+          if (mv.ok) mode_add(mv, a, accS, nl);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  double* out = partials + (int64_t)blockIdx.x * per_copy;
+  for (int q = threadIdx.x; q < per_copy; q += blockDim.x) {
+    double t = 0.0;
+    if (q < lcp)
+      for (int c = 0; c < copies; c++) t += sm[c * lcp + q];
+    out[q] = t;
+  }
+}
+
 // Mode counts of the branch-free form (they depend on the geometry and the
 // k bins only): one wave per row, kz over the lanes; the same bin decisions
 // as k_bin_rows (kbin_fast, kbin_exact near integers).  Counts are integers
@@ -591,7 +885,24 @@ static int run_bin_t(BinLaunch& L) {
     if (hipGetLastError() != hipSuccess) return CAPOW_ERR_HIP;
     *L.counts_valid = true;
   }
-  if (L.row_form && lds <= 160 * 1024) {
+  if (fast && L.mirror && lds <= 160 * 1024) {
+    // FP32: two modes per lane (one 16-byte load per row and grid) once a row
+    // fills a 128-mode chunk; FP64: one
+    constexpr int M2 = sizeof(T) == 4 ? 2 : 1;
+    const bool two = M2 == 2 && L.dev.nzh >= 128;
+    const void* fn = two ? (const void*)k_bin_mirror<T, M2> : (const void*)k_bin_mirror<T, 1>;
+    if (lds > 64 * 1024) hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (two)
+      TIMED(L.timer, "bin",
+            (k_bin_mirror<T, M2><<<L.nwg, 64 * L.dev.copies, lds, L.stream>>>(
+                (const T*)L.grid[0], (const T*)L.grid[1], (const T*)L.grid[2], (const T*)L.grid[3],
+                L.tabs, L.dev, L.dev.nzh / (64 * M2), L.partials)));
+    else
+      TIMED(L.timer, "bin",
+            (k_bin_mirror<T, 1><<<L.nwg, 64 * L.dev.copies, lds, L.stream>>>(
+                (const T*)L.grid[0], (const T*)L.grid[1], (const T*)L.grid[2], (const T*)L.grid[3],
+                L.tabs, L.dev, L.dev.nzh / 64, L.partials)));
+  } else if (L.row_form && lds <= 160 * 1024) {
     // the largest instantiated full-slot count not above nzh / 64
     if (nuf >= 16 && sizeof(T) == 4) launch_rows<T, 16>(L, lds);
     else if (nuf >= 8) launch_rows<T, 8>(L, lds);

@@ -72,6 +72,8 @@ struct capow_ctx {
   int bin_nwg = 1024;
   int bin_per_copy = 0;
   bool bin_row_form = true;
+  bool bin_mirror = false;  // mirror-group binning (set up in setup())
+  int bin_mirror_env = -1;  // CAPOW_BIN_MIRROR: 0 off, 1 on where it applies
   int nspec = 1;
   int codes[3] = {0, 0, 0};
   capow_assign_stats last{};
@@ -380,6 +382,18 @@ static int setup(capow_ctx* c) {
     fill(2, ofz, ocz, opz, NZ, nzh, sz, dz);
     if (p.do_interlacing)  // grid_cor *= 0.25 (src/main.c:395), folded into the z table
       for (int k = 0; k < nzh; k++) tab[ocz + k] *= 0.25;
+    // mirror-group binning: f(N - i) = -f(i) and cor(N - i) = cor(i) exactly
+    // on x and y, even NX and NY, and mu's sign irrelevant (line of sight
+    // along z, or even multipoles only)
+    {
+      bool sym = NX % 2 == 0 && NY % 2 == 0;
+      for (unsigned long long i = 1; sym && 2 * i < NX; i++)
+        sym = tab[ofx + NX - i] == -tab[ofx + i] && tab[ocx + NX - i] == tab[ocx + i];
+      for (unsigned long long j = 1; sym && 2 * j < NY; j++)
+        sym = tab[ofy + NY - j] == -tab[ofy + j] && tab[ocy + NY - j] == tab[ocy + j];
+      const bool mu_even = !p.odd_multipoles || (p.los != 1 && p.los != 2);
+      c->bin_mirror = sym && mu_even && c->bin_mirror_env != 0;
+    }
     HIPCHK(c, hipMalloc((void**)&c->bin_tabs, total * 8));
     HIPCHK(c, hipMemcpy(c->bin_tabs, tab.data(), total * 8, hipMemcpyHostToDevice));
     BinArgsDev& b = c->bin;
@@ -473,6 +487,7 @@ extern "C" int capow_create(const capow_params* p, capow_ctx** out) {
   if (const char* e = getenv("CAPOW_FFT_PERSIST")) c->fft_persist = atoi(e) != 0;
   if (const char* e = getenv("CAPOW_DEPOSIT_HALO")) c->dep_halo = atoi(e) != 0;
   if (const char* e = getenv("CAPOW_FFT_ORDER")) c->fft_order = atoi(e) & 7;
+  if (const char* e = getenv("CAPOW_BIN_MIRROR")) c->bin_mirror_env = atoi(e) != 0;
   rc = setup(c);
   if (rc) {
     g_create_err = c->err;
@@ -713,6 +728,7 @@ static int bin_enqueue(capow_ctx* c, double* h_bins) {
   L.dev = c->bin;
   L.tabs = c->bin_tabs;
   L.row_form = c->bin_row_form;
+  L.mirror = c->bin_mirror;
   L.nwg = c->bin_nwg;
   L.per_copy = c->bin_per_copy;
   L.partials = c->bin_partials;
@@ -869,6 +885,7 @@ extern "C" int capow_backward(capow_ctx* c, const double* dL_draw, void* grad_xy
   L.dev = c->bin;
   L.tabs = c->bin_tabs;
   L.row_form = c->bin_row_form;
+  L.mirror = c->bin_mirror;
   L.nwg = c->bin_nwg;
   L.per_copy = c->bin_per_copy;
   L.partials = c->bin_partials;

@@ -162,6 +162,7 @@ struct BinLaunch {
   BinArgsDev dev;
   const double* tabs;     // per-axis tables (offsets in dev)
   bool row_form;          // row-walking kernel (z tables in LDS)
+  bool mirror;            // mirror-group form (symmetric tables, even NX, NY)
   int nwg;
   int per_copy;
   double* partials;       // nwg * per_copy

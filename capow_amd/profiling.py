@@ -38,7 +38,7 @@ STAGE_PATTERNS = [
     ("fft_z", r"k_fft_rows"),
     ("fft_yx", r"k_fft_cols"),
     ("bin_counts", r"k_bin_counts"),
-    ("bin", r"k_bin(_rows)?<"),
+    ("bin", r"k_bin(_rows|_mirror)?<"),
     ("bin_final", r"k_bin_final"),
 ]
 

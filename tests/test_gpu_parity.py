@@ -362,6 +362,83 @@ def test_fp32_form_error_table():
             assert rel[sel].max() <= 1e-4
 
 
+MIRROR_CASES = [
+    # FP32 rows of 129 modes: two modes per lane, one chunk and a 1-mode tail
+    ("fp32_two_per_lane", api.FP32, (64, 48, 256),
+     dict(interp_order=3, do_interlacing=1, momentum=1, extra_spectra=3)),
+    # 101 modes per row: one 64-mode chunk and a 37-mode tail (one group per lane)
+    ("fp32_long_tail", api.FP32, (32, 40, 200), dict(interp_order=2, do_interlacing=1, momentum=0)),
+    ("fp64_chunk_and_tail", api.FP64, (48, 64, 160),
+     dict(interp_order=3, do_interlacing=1, momentum=2)),
+    # odd multipoles with the line of sight along z: mu is the same on all four rows
+    ("fp64_odd_los_z", api.FP64, (32, 32, 128),
+     dict(interp_order=3, do_interlacing=1, momentum=1, odd_multipoles=1, los=3, extra_spectra=3)),
+    # odd multipoles along x: mu changes sign between mirror rows -> row form
+    ("fp32_odd_los_x", api.FP32, (32, 32, 64),
+     dict(interp_order=2, do_interlacing=1, momentum=1, odd_multipoles=1, los=1)),
+    # no interlacing, momentum cross with its autos, odd NX (row form)
+    ("fp32_odd_nx", api.FP32, (45, 32, 64), dict(interp_order=1, momentum=3, extra_spectra=3)),
+]
+
+
+@pytest.mark.parametrize("case", MIRROR_CASES, ids=[c[0] for c in MIRROR_CASES])
+def test_mirror_group_binning(case, oracle, monkeypatch):
+    """Binning of the mirror rows (i, j), (NX-i, j), (i, NY-j), (NX-i, NY-j)
+    together, against the oracle's compute_power and against the row form
+    (CAPOW_BIN_MIRROR=0) on the same k-space grids: mode counts exact, sums
+    within 1e-12 of the bin's scale."""
+    name, prec, (nx, ny, nz), kw = case
+    say("mirror binning case", name)
+    base = dict(nx=nx, ny=ny, nz=nz, nk=40, maxk=0.2, precision=prec)
+    base.update(kw)
+    p = api.make_params(**base)
+    op = oracle_params(oracle, p)
+    rng = np.random.default_rng(nx + 3 * ny + 7 * nz)
+    has_b = p.momentum not in (0, 2, 5)
+    il = bool(p.do_interlacing)
+    og = []
+    for s, used in enumerate((True, il, has_b, has_b and il)):
+        if not used:
+            og.append(None)
+            continue
+        g = oracle.new_grid(op)
+        v = oracle.real_view(op, g)
+        # float32-representable, so that both precisions see the same values
+        v[:nx] = rng.standard_normal(v[:nx].shape).astype(np.float32)
+        og.append(g)
+    ob = oracle.compute_power(op, og)
+    res = {}
+    for form in ("mirror", "rows"):
+        if form == "rows":
+            monkeypatch.setenv("CAPOW_BIN_MIRROR", "0")
+        ps = api.PowerSpectrum(p)
+        for s in range(4):
+            if og[s] is not None:
+                ps.set_grid(s, oracle.real_view(op, og[s])[:nx])
+        ps.mark_transformed()
+        res[form] = ps.bin()
+        ps.close()
+    monkeypatch.delenv("CAPOW_BIN_MIRROR")
+    ls = (0, 1, 2, 3, 4) if p.odd_multipoles else (0, 2, 4)
+    for form, r in res.items():
+        assert np.array_equal(r["nmodes"], ob.nmodes), (name, form)
+        prim = r[p.momentum]["pk"]
+        scale = np.abs(prim[0])
+        if len(r["codes"]) == 3:
+            scale = _cs_scale(r[r["codes"][1]]["pk"][0], r[r["codes"][2]]["pk"][0])
+        for l in ls:
+            d = np.abs(prim[l] - ob.pk[l])
+            assert np.all(d <= 1e-12 * scale + 1e-300), (name, form, l)
+    a, b = res["mirror"], res["rows"]
+    for code in a["codes"]:
+        scale = np.abs(b[code]["pk"][0])
+        if code != p.momentum and len(a["codes"]) == 3:
+            scale = _cs_scale(b[a["codes"][1]]["pk"][0], b[a["codes"][2]]["pk"][0])
+        for l in ls:
+            d = np.abs(a[code]["pk"][l] - b[code]["pk"][l])
+            assert np.all(d <= 1e-12 * (scale + np.abs(b[code]["pk"][0])) + 1e-300), (name, code, l)
+
+
 def test_determinism():
     """Same input twice: grids and bins bitwise identical (FP32 form)."""
     say("determinism")

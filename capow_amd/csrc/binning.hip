@@ -655,10 +655,11 @@ __device__ __forceinline__ void group_load(const C* __restrict__ A, const C* __r
   }
 }
 
-// A wave walks groups g = w0, w0 + nw, ...; each group in chunks of 64 MPL
-// modes (lane l: kz = 64 MPL c + MPL l + u); the next chunk's loads are in
-// flight while the current one is binned.  Modes kz >= ktail (the few at the
-// end of the row that do not fill a chunk) are binned afterwards with one
+// A wave walks groups g = w0, w0 + nw, ...; each group in nch chunks of
+// 64 MPL modes (lane l: kz = 64 MPL c + MPL l + u; the last chunk may end
+// past the row, its lanes there bin nothing); the next chunk's loads are in
+// flight while the current one is binned.  Modes kz >= 64 MPL nch (a few at
+// the end of the row that do not fill a chunk) are binned afterwards with one
 // group per lane.
 template <typename T, int MPL>
 __global__ __launch_bounds__(512) void k_bin_mirror(const T* __restrict__ gA,
@@ -726,7 +727,7 @@ __global__ __launch_bounds__(512) void k_bin_mirror(const T* __restrict__ gA,
 #pragma unroll
         for (int u = 0; u < MPL; u++) {
           const int k = c * CH + MPL * lane + u;
-          const double4 z = *reinterpret_cast<const double4*>(zt + 4 * k);
+          const double4 z = *reinterpret_cast<const double4*>(zt + 4 * (k < nzh ? k : nzh - 1));
           C v[4][4];
 #pragma unroll
           for (int r = 0; r < 4; r++)
@@ -741,14 +742,15 @@ __global__ __launch_bounds__(512) void k_bin_mirror(const T* __restrict__ gA,
 #pragma unroll
           for (int u = 0; u < MPL; u++)
             if (MV[u].near) {
-              const KBin e = kbin_exact(G.fxy, zt[4 * (c * CH + MPL * lane + u)], a);
+              const int k = c * CH + MPL * lane + u;
+              const KBin e = kbin_exact(G.fxy, zt[4 * (k < nzh ? k : nzh - 1)], a);
               MV[u].kbin = e.kbin;
               MV[u].ok = e.ok;
             }
         }
 #pragma unroll
         for (int u = 0; u < MPL; u++)
-          if (MV[u].ok) mode_add(MV[u], a, accS, nl);
+          if (MV[u].ok && c * CH + MPL * lane + u < nzh) mode_add(MV[u], a, accS, nl);
         if (gn < ng) {
 #pragma unroll
           for (int r = 0; r < 4; r++)
@@ -844,6 +846,10 @@ __global__ void k_bin_final(const double* __restrict__ partials, int nwg, int pe
   out[q] = t;
 }
 
+// Chunks per row of the mirror-group form: a remainder of 16 modes or more
+// takes a partial chunk (coalesced loads), a smaller one the tail pass.
+static int mirror_chunks(int nzh, int ch) { return nzh / ch + (nzh % ch >= 16 ? 1 : 0); }
+
 template <typename T, int NUF>
 static void launch_rows(BinLaunch& L, size_t lds) {
   // FP64 rows hold at most 8 slots in registers (16 would spill)
@@ -896,12 +902,12 @@ static int run_bin_t(BinLaunch& L) {
       TIMED(L.timer, "bin",
             (k_bin_mirror<T, M2><<<L.nwg, 64 * L.dev.copies, lds, L.stream>>>(
                 (const T*)L.grid[0], (const T*)L.grid[1], (const T*)L.grid[2], (const T*)L.grid[3],
-                L.tabs, L.dev, L.dev.nzh / (64 * M2), L.partials)));
+                L.tabs, L.dev, mirror_chunks(L.dev.nzh, 64 * M2), L.partials)));
     else
       TIMED(L.timer, "bin",
             (k_bin_mirror<T, 1><<<L.nwg, 64 * L.dev.copies, lds, L.stream>>>(
                 (const T*)L.grid[0], (const T*)L.grid[1], (const T*)L.grid[2], (const T*)L.grid[3],
-                L.tabs, L.dev, L.dev.nzh / 64, L.partials)));
+                L.tabs, L.dev, mirror_chunks(L.dev.nzh, 64), L.partials)));
   } else if (L.row_form && lds <= 160 * 1024) {
     // the largest instantiated full-slot count not above nzh / 64
     if (nuf >= 16 && sizeof(T) == 4) launch_rows<T, 16>(L, lds);

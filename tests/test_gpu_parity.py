@@ -366,7 +366,7 @@ MIRROR_CASES = [
     # FP32 rows of 129 modes: two modes per lane, one chunk and a 1-mode tail
     ("fp32_two_per_lane", api.FP32, (64, 48, 256),
      dict(interp_order=3, do_interlacing=1, momentum=1, extra_spectra=3)),
-    # 101 modes per row: one 64-mode chunk and a 37-mode tail (one group per lane)
+    # 101 modes per row: one 64-mode chunk and a partial chunk of 37
     ("fp32_long_tail", api.FP32, (32, 40, 200), dict(interp_order=2, do_interlacing=1, momentum=0)),
     ("fp64_chunk_and_tail", api.FP64, (48, 64, 160),
      dict(interp_order=3, do_interlacing=1, momentum=2)),

@@ -127,4 +127,196 @@ __device__ __forceinline__ ModeGeom mode_geom(const RowVals& R, int k, double fz
   return G;
 }
 
+// Branch-free part of one mode for the common binning (linear k bins, no
+// 2-D bins): bin index (without the near-integer exact fallback), whether
+// the mode is binned, NM * cor * power of each spectrum, L_2, L_4 and mu.  Written without early exits so that the
+// compiler can interleave the long FP64 chains of several modes.
+struct ModeVals {
+  int kbin;
+  bool ok, near;
+  double np[3];
+  double L2, L4, mu;
+};
+
+// Linear k bin of a mode by multiplication with 1/binwidth; `near` flags a
+// quotient within 1e-9 of an integer, where the product may differ from the
+// reference's division and kbin_exact decides.
+struct KBin {
+  int kbin;
+  bool ok, near;
+  double fktot;
+};
+
+__device__ __forceinline__ KBin kbin_fast(double fxy, double fz, const BinArgsDev& a) {
+  KBin r;
+  r.fktot = dmul(2.0 * 3.14159265358979323846, sqrt(dadd(fxy, dmul(fz, fz))));
+  const double q = (double)__double2float_rn(r.fktot - a.mink) * a.inv_binwidth;
+  const double tq = trunc(q);
+  const double fr = q - tq;
+  r.near = fabs(fr) < 1e-9 || fabs(fr) > 1.0 - 1e-9;
+  r.kbin = (int)tq;
+  r.ok = (r.kbin >= 0) && (r.kbin < a.nk) && (r.fktot <= a.min_nyq);
+  return r;
+}
+
+// The reference's division (src/main.c:372, 375) for the near-integer case.
+__device__ __forceinline__ KBin kbin_exact(double fxy, double fz, const BinArgsDev& a) {
+  KBin r;
+  r.fktot = dmul(2.0 * 3.14159265358979323846, sqrt(dadd(fxy, dmul(fz, fz))));
+  r.kbin = (int)ddiv((double)__double2float_rn(r.fktot - a.mink), a.binwidth);
+  r.ok = (r.fktot <= a.min_nyq) && r.kbin >= 0 && r.kbin < a.nk;
+  r.near = false;
+  return r;
+}
+
+// The reference's Hermitian weight NM (src/main.c:422-429): 2, except 1 in
+// the kz = 0 plane where a mode's partner (-i, -j, 0) is stored as well
+// (the rows i = 0, NX/2 of j = 0 and i = NX/2 keep 2, as there).
+__device__ __forceinline__ double nm_weight(int i, int j, int k, int nx) {
+  const bool nm1 = k == 0 && ((j == 0) ? ((i != 0) && (i != nx / 2)) : (i != nx / 2));
+  return nm1 ? 1.0 : 2.0;
+}
+
+// Accumulates one evaluated mode into this wave's bins (the mode counts
+// come from k_bin_counts).
+__device__ __forceinline__ void mode_add(const ModeVals& V, const BinArgsDev& a, double* accS,
+                                         int nl) {
+  const int NK = a.nk;
+#pragma unroll
+  for (int s = 0; s < 3; s++) {
+    if (s >= a.nspec) break;
+    const double np = V.np[s];
+    double* b = accS + s * nl * NK + V.kbin;
+    atomicAdd(&b[0], np);
+    atomicAdd(&b[NK], V.L2 * np);
+    atomicAdd(&b[2 * NK], V.L4 * np);
+    if (a.odd) {
+      const double L3 = 2.5 * V.mu * V.mu * V.mu - 1.5 * V.mu;
+      atomicAdd(&b[3 * NK], V.mu * np);
+      atomicAdd(&b[4 * NK], L3 * np);
+    }
+  }
+}
+
+// ---- Mirror-group form of the branch-free binning ----
+//
+// With f(N - i) = -f(i) exactly in the frequency tables and the deconvolution
+// even in f (both checked on the host), the rows (i, j), (NX - i, j),
+// (i, NY - j) and (NX - i, NY - j) share |k|, the k bin, the window
+// correction and the Hermitian weight NM of every kz (NX, NY even); mu agrees
+// up to its sign, so L_2 and L_4 agree, and with the line of sight along z
+// mu itself does.  Only the interlace phase differs from row to row.  A
+// group's per-mode geometry is therefore evaluated once, its (up to) four
+// powers are summed, and the sum takes one set of LDS adds.
+
+// The rows of one group: canonical (i, j), 0 <= i <= NX/2, 0 <= j <= NY/2,
+// and its partners where they are distinct rows (live).  Dead slots repeat
+// the canonical row and are read as zero.
+struct Group {
+  int i, j;
+  double fx, fy, fxy, corxy;
+  int64_t row[4];
+  bool live[4];
+  double qr[4], qi[4];   // interlace phase of each row (x phase * y phase)
+};
+
+__device__ __forceinline__ void group_rows(int g, int hy1, const BinArgsDev& a, int& i, int& j,
+                                           int64_t* row, bool* live) {
+  i = g / hy1;
+  j = g - i * hy1;
+  const bool mx = i > 0 && 2 * i < a.nx, my = j > 0 && 2 * j < a.ny;
+  const int i2 = mx ? a.nx - i : i, j2 = my ? a.ny - j : j;
+  row[0] = (int64_t)i * a.ny + j;
+  row[1] = (int64_t)i2 * a.ny + j;
+  row[2] = (int64_t)i * a.ny + j2;
+  row[3] = (int64_t)i2 * a.ny + j2;
+  live[0] = true;
+  live[1] = mx;
+  live[2] = my;
+  live[3] = mx && my;
+}
+
+__device__ __forceinline__ Group group_of(int g, int hy1, const double* __restrict__ tabs,
+                                          const BinArgsDev& a) {
+  Group G;
+  group_rows(g, hy1, a, G.i, G.j, G.row, G.live);
+  G.fx = tabs[a.ofx + G.i];
+  G.fy = tabs[a.ofy + G.j];
+  G.fxy = dadd(dmul(G.fx, G.fx), dmul(G.fy, G.fy));
+  G.corxy = tabs[a.ocx + G.i] * tabs[a.ocy + G.j];
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    G.qr[r] = 1.0;
+    G.qi[r] = 0.0;
+    if (a.interlace) {
+      const int ii = (int)(G.row[r] / a.ny), jj = (int)(G.row[r] - (int64_t)ii * a.ny);
+      const double pxr = tabs[a.opx + 2 * ii], pxi = tabs[a.opx + 2 * ii + 1];
+      const double pyr = tabs[a.opy + 2 * jj], pyi = tabs[a.opy + 2 * jj + 1];
+      G.qr[r] = pxr * pyr - pxi * pyi;
+      G.qi[r] = pxr * pyi + pxi * pyr;
+    }
+  }
+  return G;
+}
+
+// One mode kz of a group: v[r][0..3] = A, A_int, B, B_int of row r (zero for
+// dead rows and absent grids).  np[s] = NM cor * (sum over the rows of the
+// power of spectrum s).
+template <typename C>
+__device__ __forceinline__ ModeVals group_mode(const Group& G, int k, double fz, double corz,
+                                               double pzr, double pzi, const C (&v)[4][4],
+                                               const BinArgsDev& a) {
+  ModeVals V;
+  const double twopi = 2.0 * 3.14159265358979323846;
+  const KBin kb = kbin_fast(G.fxy, fz, a);
+  const double fktot = kb.fktot;
+  V.near = kb.near;
+  V.kbin = kb.kbin;
+  V.ok = kb.ok;
+  const double fl = a.los == 1 ? G.fx : (a.los == 2 ? G.fy : fz);
+  double rk = __builtin_amdgcn_rcp(fktot > 0 ? fktot : 1.0);
+  rk = rk * (2.0 - fktot * rk);
+  rk = rk * (2.0 - fktot * rk);
+  const double mu = fktot > 0 ? (twopi * fl) * rk : 0.0;
+  const double wc = nm_weight(G.i, G.j, k, a.nx) * (G.corxy * corz);
+  double ps[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    double ar = (double)v[r][0].x, ai = (double)v[r][0].y, br = 0, bi = 0;
+    double cr = 1.0, ci = 0.0;
+    if (a.interlace) {
+      cr = G.qr[r] * pzr - G.qi[r] * pzi;
+      ci = G.qr[r] * pzi + G.qi[r] * pzr;
+      ar += (double)v[r][1].x * cr - (double)v[r][1].y * ci;
+      ai += (double)v[r][1].x * ci + (double)v[r][1].y * cr;
+    }
+    if (a.hasB) {
+      br = (double)v[r][2].x;
+      bi = (double)v[r][2].y;
+      if (a.interlace) {
+        br += (double)v[r][3].x * cr - (double)v[r][3].y * ci;
+        bi += (double)v[r][3].x * ci + (double)v[r][3].y * cr;
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < 3; s++) {
+      double p;
+      switch (a.kind[s]) {
+        case SPEC_AUTO_A: p = ar * ar + ai * ai; break;
+        case SPEC_AUTO_B: p = br * br + bi * bi; break;
+        case SPEC_CROSS_IM: p = ar * bi - ai * br; break;
+        default: p = ar * br + ai * bi; break;
+      }
+      ps[s] += p;
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < 3; s++) V.np[s] = wc * ps[s];
+  const double mu2 = mu * mu;
+  V.mu = mu;
+  V.L2 = 1.5 * mu2 - 0.5;
+  V.L4 = 4.375 * mu2 * mu2 - 3.75 * mu2 + 0.375;
+  return V;
+}
+
 }  // namespace capow

@@ -680,11 +680,7 @@ extern "C" int capow_assign(capow_ctx* c, capow_assign_stats* st) {
   return assign_finish(c, c->h_stats, c->h_errs, st);
 }
 
-extern "C" int capow_fft(capow_ctx* c) {
-  if (!c) return CAPOW_ERR_ARG;
-  if (c->state < 2) return fail(c, CAPOW_ERR_STATE, "capow_fft before capow_assign");
-  if (c->state >= 3) return fail(c, CAPOW_ERR_STATE, "grids already transformed");
-  FftArgs a;
+static void fft_args(capow_ctx* c, FftArgs& a) {
   a.g = &c->g;
   a.stream = c->stream;
   a.timer = &c->timer;
@@ -706,6 +702,14 @@ extern "C" int capow_fft(capow_ctx* c) {
   a.ncu = c->ncu;
   a.persist = c->fft_persist;
   a.order = c->fft_order;
+}
+
+extern "C" int capow_fft(capow_ctx* c) {
+  if (!c) return CAPOW_ERR_ARG;
+  if (c->state < 2) return fail(c, CAPOW_ERR_STATE, "capow_fft before capow_assign");
+  if (c->state >= 3) return fail(c, CAPOW_ERR_STATE, "grids already transformed");
+  FftArgs a;
+  fft_args(c, a);
   int rc = run_fft(a, c->fp64);
   if (rc) return fail(c, rc, std::string("fft launch failed: ") + hipGetErrorString(hipGetLastError()));
   if (a.swapped)
@@ -718,10 +722,7 @@ extern "C" int capow_fft(capow_ctx* c) {
   return CAPOW_OK;
 }
 
-// Enqueues the binning and an asynchronous copy of the bins into h_bins.
-static int bin_enqueue(capow_ctx* c, double* h_bins) {
-  if (c->state < 3) return fail(c, CAPOW_ERR_STATE, "capow_bin before capow_fft");
-  BinLaunch L;
+static void bin_launch(capow_ctx* c, BinLaunch& L) {
   L.stream = c->stream;
   L.timer = &c->timer;
   for (int s = 0; s < 4; s++) L.grid[s] = c->grid[s] ? c->grid[s] : c->grid[0];
@@ -735,6 +736,13 @@ static int bin_enqueue(capow_ctx* c, double* h_bins) {
   L.out = c->bin_out;
   L.counts = c->bin_counts;
   L.counts_valid = &c->bin_counts_valid;
+}
+
+// Enqueues the binning and an asynchronous copy of the bins into h_bins.
+static int bin_enqueue(capow_ctx* c, double* h_bins) {
+  if (c->state < 3) return fail(c, CAPOW_ERR_STATE, "capow_bin before capow_fft");
+  BinLaunch L;
+  bin_launch(c, L);
   int rc = run_bin(L, c->fp64);
   if (rc) return fail(c, rc, std::string("binning launch failed: ") + hipGetErrorString(hipGetLastError()));
   HIPCHK(c, hipMemcpyAsync(h_bins, c->bin_out, (size_t)c->bin_per_copy * 8, hipMemcpyDeviceToHost,
@@ -783,6 +791,7 @@ extern "C" int capow_bin(capow_ctx* c, capow_bins* out) {
 }
 
 extern "C" int capow_power(capow_ctx* c, capow_assign_stats* st, capow_bins* out) {
+  if (!c || !out) return CAPOW_ERR_ARG;
   int rc = capow_assign(c, st);
   if (rc) return rc;
   rc = capow_fft(c);

@@ -581,19 +581,12 @@ __device__ __forceinline__ void rt_stage1(cx<T>* v, const cx<T>* twr,
   }
 }
 
-// The rest of a batch after the first stage's barrier: N1-point DFTs,
-// natural order in LDS (row stride N + 1), the r2c split, the transposed
-// write.  Ends with the LDS still in use by the write (the caller syncs
-// before reusing it).
-// tws (R2C): the split twiddles w^k, k = 0..N/2, in LDS (w^(N-k) =
-// -conj(w^k)); null: read both from the table in global memory.
-template <typename T, int N1, int N2, int B, bool R2C>
-__device__ __forceinline__ void rt_rest(const cx<T>* __restrict__ tw, const cx<T>* tws,
-                                        cx<T>* lds, int t, int c, cx<T>* ob, int64_t out_sk,
-                                        int64_t nlive) {
+// Second stage after the first stage's barrier: the N1-point DFTs, then row
+// c in natural order at lds + c (N + 1) (ends with a barrier).
+template <typename T, int N1, int N2>
+__device__ __forceinline__ void rt_stage2(cx<T>* lds, int t, int c) {
   constexpr int N = N1 * N2, E = clcm(N1, N2), TS = N / E, G2 = E / N1;
   constexpr int RS = N1 * (N2 + 1);
-  constexpr int K = R2C ? N + 1 : N;  // outputs per row
   cx<T> z[E];
 #pragma unroll
   for (int h = 0; h < G2; h++) {
@@ -612,6 +605,22 @@ __device__ __forceinline__ void rt_rest(const cx<T>* __restrict__ tw, const cx<T
     for (int k1 = 0; k1 < N1; k1++) zr[k2 + N2 * k1] = z[h * N1 + k1];
   }
   __syncthreads();
+}
+
+// The rest of a batch after the first stage's barrier: N1-point DFTs,
+// natural order in LDS (row stride N + 1), the r2c split, the transposed
+// write.  Ends with the LDS still in use by the write (the caller syncs
+// before reusing it).
+// tws (R2C): the split twiddles w^k, k = 0..N/2, in LDS (w^(N-k) =
+// -conj(w^k)); null: read both from the table in global memory.
+template <typename T, int N1, int N2, int B, bool R2C>
+__device__ __forceinline__ void rt_rest(const cx<T>* __restrict__ tw, const cx<T>* tws,
+                                        cx<T>* lds, int t, int c, cx<T>* ob, int64_t out_sk,
+                                        int64_t nlive) {
+  constexpr int N = N1 * N2, E = clcm(N1, N2), TS = N / E;
+  constexpr int K = R2C ? N + 1 : N;  // outputs per row
+  rt_stage2<T, N1, N2>(lds, t, c);
+  cx<T>* zr = lds + c * (N + 1);
   if (R2C) {
     // even/odd split X[k] = E[k] + w^k O[k], k = 0..N (w = e^{-2 pi i/2N}):
     // X[k] and X[N-k] need only z[k] and z[N-k] (z[N] = z[0]), so one

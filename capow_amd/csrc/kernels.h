@@ -171,6 +171,10 @@ struct BinLaunch {
   bool* counts_valid;     // counts computed for this geometry
 };
 int run_bin(BinLaunch& L, bool fp64);
+// The pieces of run_bin around the accumulation kernel: the once-per-context
+// mode counts of the branch-free forms, and the final sum of the partials.
+int run_bin_counts(BinLaunch& L);
+int run_bin_final(BinLaunch& L, bool fast);
 
 // Backward of the binning: dL/draw (G: [n_spectra][5][NK]) -> gradients of
 // the k-space grids, in place; g0: the k = 0 gradients (8 doubles).

@@ -469,7 +469,7 @@ __device__ __forceinline__ void group_load(const C* __restrict__ A, const C* __r
 // flight while the current one is binned.  Modes kz >= 64 MPL nch (a few at
 // the end of the row that do not fill a chunk) are binned afterwards with one
 // group per lane.
-template <typename T, int MPL>
+template <typename T, int MPL, bool IL, bool HB>
 __global__ __launch_bounds__(512) void k_bin_mirror(const T* __restrict__ gA,
                                                     const T* __restrict__ gAi,
                                                     const T* __restrict__ gB,
@@ -503,7 +503,7 @@ __global__ __launch_bounds__(512) void k_bin_mirror(const T* __restrict__ gA,
   const C* Ai = reinterpret_cast<const C*>(gAi);
   const C* B = reinterpret_cast<const C*>(gB);
   const C* Bi = reinterpret_cast<const C*>(gBi);
-  const bool il = a.interlace != 0, hb = a.hasB != 0;
+  constexpr bool il = IL, hb = HB;
   const int64_t rowc = a.rowc;
   const int hy1 = a.ny / 2 + 1;
   const int ng = (a.nx / 2 + 1) * hy1;
@@ -541,7 +541,7 @@ __global__ __launch_bounds__(512) void k_bin_mirror(const T* __restrict__ gA,
           for (int r = 0; r < 4; r++)
 #pragma unroll
             for (int q = 0; q < 4; q++) v[r][q] = cur[r][q][u];
-          MV[u] = group_mode<C>(G, k, z.x, z.y, z.z, z.w, v, a);
+          MV[u] = group_mode<C, IL, HB>(G, k, z.x, z.y, z.z, z.w, v, a);
         }
         bool anynear = false;
 #pragma unroll
@@ -589,7 +589,7 @@ __global__ __launch_bounds__(512) void k_bin_mirror(const T* __restrict__ gA,
             v[r][3] = lv && hb && il ? Bi[m] : zero;
           }
           const double4 z = *reinterpret_cast<const double4*>(zt + 4 * k);
-          ModeVals mv = group_mode<C>(G, k, z.x, z.y, z.z, z.w, v, a);
+          ModeVals mv = group_mode<C, IL, HB>(G, k, z.x, z.y, z.z, z.w, v, a);
           if (mv.near) {
             const KBin e = kbin_exact(G.fxy, z.x, a);
             mv.kbin = e.kbin;
@@ -658,6 +658,27 @@ __global__ void k_bin_final(const double* __restrict__ partials, int nwg, int pe
 // takes a partial chunk (coalesced loads), a smaller one the tail pass.
 static int mirror_chunks(int nzh, int ch) { return nzh / ch + (nzh % ch >= 16 ? 1 : 0); }
 
+// FP32: two modes per lane (one 16-byte load per row and grid) once a row
+// fills a 128-mode chunk; FP64: one.
+template <typename T, bool IL, bool HB>
+static void launch_mirror(BinLaunch& L, size_t lds) {
+  constexpr int M2 = sizeof(T) == 4 ? 2 : 1;
+  const bool two = M2 == 2 && L.dev.nzh >= 128;
+  const void* fn = two ? (const void*)k_bin_mirror<T, M2, IL, HB>
+                       : (const void*)k_bin_mirror<T, 1, IL, HB>;
+  if (lds > 64 * 1024) hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (two)
+    TIMED(L.timer, "bin",
+          (k_bin_mirror<T, M2, IL, HB><<<L.nwg, 64 * L.dev.copies, lds, L.stream>>>(
+              (const T*)L.grid[0], (const T*)L.grid[1], (const T*)L.grid[2], (const T*)L.grid[3],
+              L.tabs, L.dev, mirror_chunks(L.dev.nzh, 64 * M2), L.partials)));
+  else
+    TIMED(L.timer, "bin",
+          (k_bin_mirror<T, 1, IL, HB><<<L.nwg, 64 * L.dev.copies, lds, L.stream>>>(
+              (const T*)L.grid[0], (const T*)L.grid[1], (const T*)L.grid[2], (const T*)L.grid[3],
+              L.tabs, L.dev, mirror_chunks(L.dev.nzh, 64), L.partials)));
+}
+
 template <typename T, int NUF>
 static void launch_rows(BinLaunch& L, size_t lds) {
   // FP64 rows hold at most 8 slots in registers (16 would spill)
@@ -721,22 +742,11 @@ static int run_bin_t(BinLaunch& L) {
     if (rc) return rc;
   }
   if (fast && L.mirror && lds <= 160 * 1024) {
-    // FP32: two modes per lane (one 16-byte load per row and grid) once a row
-    // fills a 128-mode chunk; FP64: one
-    constexpr int M2 = sizeof(T) == 4 ? 2 : 1;
-    const bool two = M2 == 2 && L.dev.nzh >= 128;
-    const void* fn = two ? (const void*)k_bin_mirror<T, M2> : (const void*)k_bin_mirror<T, 1>;
-    if (lds > 64 * 1024) hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (two)
-      TIMED(L.timer, "bin",
-            (k_bin_mirror<T, M2><<<L.nwg, 64 * L.dev.copies, lds, L.stream>>>(
-                (const T*)L.grid[0], (const T*)L.grid[1], (const T*)L.grid[2], (const T*)L.grid[3],
-                L.tabs, L.dev, mirror_chunks(L.dev.nzh, 64 * M2), L.partials)));
-    else
-      TIMED(L.timer, "bin",
-            (k_bin_mirror<T, 1><<<L.nwg, 64 * L.dev.copies, lds, L.stream>>>(
-                (const T*)L.grid[0], (const T*)L.grid[1], (const T*)L.grid[2], (const T*)L.grid[3],
-                L.tabs, L.dev, mirror_chunks(L.dev.nzh, 64), L.partials)));
+    const bool il = L.dev.interlace != 0, hb = L.dev.hasB != 0;
+    if (il && hb) launch_mirror<T, true, true>(L, lds);
+    else if (il) launch_mirror<T, true, false>(L, lds);
+    else if (hb) launch_mirror<T, false, true>(L, lds);
+    else launch_mirror<T, false, false>(L, lds);
   } else if (L.row_form && lds <= 160 * 1024) {
     // the largest instantiated full-slot count not above nzh / 64
     if (nuf >= 16 && sizeof(T) == 4) launch_rows<T, 16>(L, lds);

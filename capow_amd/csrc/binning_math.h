@@ -261,8 +261,11 @@ __device__ __forceinline__ Group group_of(int g, int hy1, const double* __restri
 
 // One mode kz of a group: v[r][0..3] = A, A_int, B, B_int of row r (zero for
 // dead rows and absent grids).  np[s] = NM cor * (sum over the rows of the
-// power of spectrum s).
-template <typename C>
+// power of spectrum s).  IL (interlacing) and HB (B grids) are the
+// context's flags as template arguments; the four products |A|^2, |B|^2,
+// Im(A* B), Re(A* B) are summed over the rows and mapped onto the spectra
+// once per group.
+template <typename C, bool IL, bool HB>
 __device__ __forceinline__ ModeVals group_mode(const Group& G, int k, double fz, double corz,
                                                double pzr, double pzi, const C (&v)[4][4],
                                                const BinArgsDev& a) {
@@ -279,39 +282,41 @@ __device__ __forceinline__ ModeVals group_mode(const Group& G, int k, double fz,
   rk = rk * (2.0 - fktot * rk);
   const double mu = fktot > 0 ? (twopi * fl) * rk : 0.0;
   const double wc = nm_weight(G.i, G.j, k, a.nx) * (G.corxy * corz);
-  double ps[3] = {0.0, 0.0, 0.0};
+  double saa = 0.0, sbb = 0.0, sim = 0.0, sre = 0.0;
 #pragma unroll
   for (int r = 0; r < 4; r++) {
     double ar = (double)v[r][0].x, ai = (double)v[r][0].y, br = 0, bi = 0;
     double cr = 1.0, ci = 0.0;
-    if (a.interlace) {
+    if constexpr (IL) {
       cr = G.qr[r] * pzr - G.qi[r] * pzi;
       ci = G.qr[r] * pzi + G.qi[r] * pzr;
       ar += (double)v[r][1].x * cr - (double)v[r][1].y * ci;
       ai += (double)v[r][1].x * ci + (double)v[r][1].y * cr;
     }
-    if (a.hasB) {
+    saa += ar * ar + ai * ai;
+    if constexpr (HB) {
       br = (double)v[r][2].x;
       bi = (double)v[r][2].y;
-      if (a.interlace) {
+      if constexpr (IL) {
         br += (double)v[r][3].x * cr - (double)v[r][3].y * ci;
         bi += (double)v[r][3].x * ci + (double)v[r][3].y * cr;
       }
-    }
-#pragma unroll
-    for (int s = 0; s < 3; s++) {
-      double p;
-      switch (a.kind[s]) {
-        case SPEC_AUTO_A: p = ar * ar + ai * ai; break;
-        case SPEC_AUTO_B: p = br * br + bi * bi; break;
-        case SPEC_CROSS_IM: p = ar * bi - ai * br; break;
-        default: p = ar * br + ai * bi; break;
-      }
-      ps[s] += p;
+      sbb += br * br + bi * bi;
+      sim += ar * bi - ai * br;
+      sre += ar * br + ai * bi;
     }
   }
 #pragma unroll
-  for (int s = 0; s < 3; s++) V.np[s] = wc * ps[s];
+  for (int s = 0; s < 3; s++) {
+    double p;
+    switch (a.kind[s]) {
+      case SPEC_AUTO_A: p = saa; break;
+      case SPEC_AUTO_B: p = sbb; break;
+      case SPEC_CROSS_IM: p = sim; break;
+      default: p = sre; break;
+    }
+    V.np[s] = wc * p;
+  }
   const double mu2 = mu * mu;
   V.mu = mu;
   V.L2 = 1.5 * mu2 - 0.5;

@@ -230,13 +230,15 @@ __device__ __forceinline__ ModeVals mode_vals(const RowVals& R, int k, double fz
 }
 
 // V consecutive complex values from p (16-byte loads where they fit: p is
-// 16-byte aligned whenever V > 1).
+// 16-byte aligned whenever V > 1).  FP32: non-temporal loads (every value is
+// read once; measured 3.11 -> 3.03 ms for the C2 binning).
 template <typename C, int V>
 __device__ __forceinline__ void load_run(const C* __restrict__ p, C* out) {
   if constexpr (sizeof(C) == 8 && V >= 2) {
 #pragma unroll
     for (int u = 0; u < V; u += 2) {
-      const float4 q = *reinterpret_cast<const float4*>(p + u);
+      typedef float f4v __attribute__((ext_vector_type(4)));
+      const f4v q = __builtin_nontemporal_load(reinterpret_cast<const f4v*>(p + u));
       out[u].x = q.x;
       out[u].y = q.y;
       out[u + 1].x = q.z;

@@ -796,6 +796,9 @@ __global__ __launch_bounds__(DEP_T) void k_deposit(const T* __restrict__ sorted,
 #pragma unroll
             for (int ty = 0; ty < ORDER; ty++) {
               const bool vxy = vx[tx] && vy[ty];
+              // a row outside the tile for every lane (ghost particles of
+              // one border class share the pattern): nothing to add
+              if (!__any(vxy)) continue;
               const int rowi = vxy ? lx[tx] * TY + ly[ty] : lane;
               const T wxy = vxy ? ax.w[tx] * ay.w[ty] : T(0);
               const T qa0 = va * wxy, qb0 = vb * wxy;
@@ -843,6 +846,7 @@ __global__ __launch_bounds__(DEP_T) void k_deposit(const T* __restrict__ sorted,
 #pragma unroll
           for (int ty = 0; ty < ORDER; ty++) {
             const bool vxy = vx[tx] && vy[ty];
+            if (!__any(vxy)) continue;   // outside the tile for every lane
             const int rowi = lx[tx] * TY + ly[ty];
             const int m8 = zmask(rowi, TZ) * 8;
             const int row8 = rowi * TZ * 8;

@@ -200,13 +200,14 @@ __device__ __forceinline__ AxisOut<T> axis_taps(T u, T lo, T hi, T d, T L, int n
 // [0, N) otherwise.
 __device__ __forceinline__ int cell_key(int c, int n, bool periodic) {
   if (periodic) {
-    if (c >= n && c < 2 * n) c -= n;
-    else if (c < 0 && c >= -n) c += n;
-    else if ((unsigned)c >= (unsigned)n) {
-      c %= n;
-      if (c < 0) c += n;
+    // one period either side by a select; further out (never for accepted
+    // coordinates) by the remainder
+    int r = c >= n ? c - n : (c < 0 ? c + n : c);
+    if ((unsigned)r >= (unsigned)n) {
+      r = c % n;
+      if (r < 0) r += n;
     }
-    return c;
+    return r;
   }
   return c < 0 ? 0 : (c >= n ? n - 1 : c);
 }

@@ -439,6 +439,40 @@ def test_mirror_group_binning(case, oracle, monkeypatch):
             assert np.all(d <= 1e-12 * (scale + np.abs(b[code]["pk"][0])) + 1e-300), (name, code, l)
 
 
+@pytest.mark.parametrize("prec", [api.FP64, api.FP32])
+@pytest.mark.parametrize("order,inter", [(1, 1), (2, 0), (2, 1), (3, 0)])
+def test_lattice_power_is_minus_shot(prec, order, inter):
+    """The known answer of tests/test_oracle_power.py::
+    test_lattice_power_is_minus_shot through the whole GPU path (capow_power
+    and the host finalisation): a 2-cell-spaced lattice of particles that no
+    edge rule touches has no power below k_L = pi/dx, so P_0 = -V/N there
+    (FP64 within 1e-10, FP32 form within 1e-4 of V/N) and P_2 = P_4 = 0;
+    the mode counts are bit-exact with the oracle's (checked elsewhere)."""
+    say("lattice case", prec, order, inter)
+    n, nk, maxk = 32, 20, 0.2
+    dx = L / n
+    m = n // 2
+    q = np.arange(m) * 2 * dx
+    X, Y, Z = np.meshgrid(q + 0.3 * dx, q + 0.1 * dx, q + 0.45 * dx, indexing="ij")
+    pos = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+    vel = np.zeros_like(pos)
+    p = api.make_params(nx=n, ny=n, nz=n, xmax=L, ymax=L, zmax=L, interp_order=order,
+                        do_interlacing=inter, momentum=0, nk=nk, maxk=maxk, precision=prec)
+    ps = api.PowerSpectrum(p)
+    ps.set_particles(mocks.xyzv(pos, vel, 0))
+    res = ps.power()
+    pk, nm, pshot = _finalized(p, res, ps.stats, 0)
+    ps.close()
+    assert pshot == pytest.approx(L ** 3 / m ** 3, rel=1e-14)
+    edges_hi = (np.arange(nk) + 1) * (maxk / nk)
+    sel = (edges_hi <= np.pi / dx) & (np.asarray(nm) > 0)
+    assert sel.sum() >= 8
+    tol = 1e-10 if prec == api.FP64 else 1e-4
+    np.testing.assert_allclose(pk[0][sel], -pshot, rtol=tol)
+    assert np.abs(pk[2][sel]).max() <= tol * pshot
+    assert np.abs(pk[4][sel]).max() <= tol * pshot
+
+
 def test_determinism():
     """Same input twice: grids and bins bitwise identical (FP32 form)."""
     say("determinism")

@@ -149,3 +149,46 @@ def test_output_text_known_answer(oracle, tmp_path):
     text = open(path).read()
     assert text == ("# shot = 0.5\n# norm = 4\n# k  pk0  pk2  pk4  Nmodes\n"
                     "0.05 1.5 -0.25 0 2\n0.15 2e-07 1.23457e+06 3 5\n0.25 0 0 0 0\n")
+
+
+# A periodic lattice of particles, spacing 2 cells, offsets chosen so that no
+# edge rule of the reference drops a tap or a particle (the TSC low tap at
+# index 0, the interlaced >= MAX rejection): its density is 2-cell periodic,
+# so every Fourier mode with all |k_i| below the lattice harmonic
+# k_L = pi/dx vanishes, and after the shot-noise subtraction
+# (src/main.c:1014-1056: P_0 = sum/(norm Nmodes) - shot/norm, shot = N,
+# norm = N^2/V) P_0 = -V/N exactly in every bin below k_L while
+# P_2 = P_4 = 0.  Interlaced TSC always meets one of the edge rules and is
+# left out.
+@pytest.mark.parametrize("order,inter", [(1, 0), (1, 1), (2, 0), (2, 1), (3, 0)])
+def test_lattice_power_is_minus_shot(order, inter, oracle):
+    n, L, nk, maxk = 32, 1000.0, 20, 0.2
+    dx = L / n
+    m = n // 2
+    q = np.arange(m) * 2 * dx
+    X, Y, Z = np.meshgrid(q + 0.3 * dx, q + 0.1 * dx, q + 0.45 * dx, indexing="ij")
+    pos = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+    vel = np.zeros_like(pos)
+    p = oracle.params(nx=n, ny=n, nz=n, interp_order=order, do_interlacing=inter, nk=nk, maxk=maxk)
+    rc, b, st, alpha, shot, norm = oracle.run(p, pos, vel)
+    assert rc == 0 and st.ngrid == m ** 3
+    raw0 = b.pk[0].copy()
+    oracle.finalize(p, shot, norm, b)
+    kl = np.pi / dx
+    edges_hi = (np.arange(nk) + 1) * (maxk / nk)
+    sel = (edges_hi <= kl) & (b.nmodes > 0)
+    assert sel.sum() >= 8
+    # the raw sums of these bins are FFT round-off of a grid of O(1) values
+    assert np.abs(raw0[sel]).max() <= 1e-12 * b.nmodes[sel].max() * shot
+    pshot = L ** 3 / m ** 3
+    assert shot / norm == pytest.approx(pshot, rel=1e-14)
+    np.testing.assert_allclose(b.pk[0][sel], -pshot, rtol=1e-12)
+    assert np.abs(b.pk[2][sel]).max() <= 1e-12 * pshot
+    assert np.abs(b.pk[4][sel]).max() <= 1e-12 * pshot
+    # control: one particle moved by one cell breaks the 2-cell periodicity
+    # (for every assignment order) and puts power into every one of these bins
+    pos2 = pos.copy()
+    pos2[m * m * 3 + 5, 1] += dx
+    rc, b2, _, _, shot2, norm2 = oracle.run(p, pos2, vel)
+    assert rc == 0
+    assert np.all(np.abs(b2.pk[0][sel]) > 1e6 * np.abs(raw0[sel]).max() + 1e-300)

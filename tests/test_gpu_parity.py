@@ -272,6 +272,8 @@ E2E = [
                                     odd_multipoles=1, extra_spectra=3), 32, "zeldovich"),
     ("tsc_int_los3_m1", 48, dict(interp_order=3, do_interlacing=1, momentum=1, los=3,
                                  extra_spectra=3), 24, "zeldovich"),
+    ("c1_like_128_tsc_int_m1", 128, dict(interp_order=3, do_interlacing=1, momentum=1,
+                                         odd_multipoles=1, extra_spectra=3), 64, "zeldovich"),
 ]
 
 

@@ -331,11 +331,11 @@ def test_end_to_end_fp64(case, oracle):
     ps.close()
 
 
-def test_fp32_form_error_table():
+@pytest.mark.parametrize("n", [64, 128])
+def test_fp32_form_error_table(n):
     """FP32 form vs FP64 form on the same (float32-representable) catalogue:
     every l and k bin below k_Nyq/2 within 1e-4 of the FP64 scale."""
     say("fp32 vs fp64 error table")
-    n = 64
     kw = dict(nx=n, ny=n, nz=n, interp_order=3, do_interlacing=1, momentum=1, extra_spectra=3,
               nk=20, maxk=0.2)
     pos, vel = mocks.zeldovich(64, L, 5)

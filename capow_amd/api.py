@@ -18,9 +18,10 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libcapow.so")
 
 CAPOW_OK = 0
+CAPOW_ERR_IO, CAPOW_ERR_PARSE, CAPOW_ERR_OUT_OF_BOX = 7, 8, 110
 ERRORS = {1: "CAPOW_ERR_ARG", 2: "CAPOW_ERR_HIP", 3: "CAPOW_ERR_NOMEM", 4: "CAPOW_ERR_STATE",
           5: "CAPOW_ERR_UNSUPPORTED", 6: "CAPOW_ERR_INTERNAL", 7: "CAPOW_ERR_IO",
-          110: "CAPOW_ERR_OUT_OF_BOX"}
+          8: "CAPOW_ERR_PARSE", 110: "CAPOW_ERR_OUT_OF_BOX"}
 FP32, FP64 = 0, 1
 EDGE_REFERENCE, EDGE_PERIODIC = 0, 1
 SPEC_AUTO_A, SPEC_AUTO_B = 1, 2
@@ -87,6 +88,9 @@ SIGNATURES = {
                         C.POINTER(C.c_int32)], C.c_int),
     "capow_write_output": ([_P, C.c_double, C.c_double, _D, _I64, _D, _I64, C.c_char_p,
                             C.c_int32], C.c_int),
+    "capow_read_ascii": ([C.c_char_p, C.c_int32, _D, _D, C.c_int32, C.c_int32, C.POINTER(_V),
+                          _I64, C.c_char_p, C.c_int32, _D], C.c_int),
+    "capow_free_rows": ([_V], None),
     "capow_grid_elems": ([_V, _I64], C.c_int),
     "capow_get_grid": ([_V, C.c_int32, _V, C.c_int64], C.c_int),
     "capow_set_grid": ([_V, C.c_int32, _V, C.c_int64], C.c_int),
@@ -360,6 +364,39 @@ def write_output(p, shot, norm, pk5, nmodes, path, pk2d=None, nmodes2d=None, pre
         _dptr(np.ascontiguousarray(pk2d, dtype=np.float64)) if pk2d is not None else None,
         _iptr(np.ascontiguousarray(nmodes2d, dtype=np.int64)) if nmodes2d is not None else None,
         path.encode(), 1 if precise else 0))
+
+
+def read_ascii(path, los=0, lo=(0.0, 0.0, 0.0), hi=(1000.0, 1000.0, 1000.0), precision=FP64,
+               nthreads=0):
+    """The periodic ASCII catalogue at `path` as an (n, 4) array of
+    (x, y, z, v_los) rows (capow_read_ascii: the reference's fgets/sscanf
+    semantics, parsed over `nthreads` threads).  Raises CapowError with the
+    first malformed line, or the first particle outside [lo, hi] for LOS 0."""
+    lo_a = (C.c_double * 3)(*lo)
+    hi_a = (C.c_double * 3)(*hi)
+    rows = _V()
+    n = C.c_int64()
+    bad = C.create_string_buffer(2048)
+    xyz = (C.c_double * 3)()
+    fp64 = precision == FP64
+    rc = lib().capow_read_ascii(os.fsencode(path), los, lo_a, hi_a, 1 if fp64 else 0, nthreads,
+                                C.byref(rows), C.byref(n), bad, len(bad), xyz)
+    if rc != CAPOW_OK:
+        if rc == CAPOW_ERR_OUT_OF_BOX:
+            msg = "particle out of grid bounds: x=%f, y=%f, z=%f" % tuple(xyz)
+        elif rc == CAPOW_ERR_PARSE:
+            msg = "error reading file: %s" % bad.value.decode(errors="replace")
+        else:
+            msg = "cannot read %s" % path
+        raise CapowError(rc, msg)
+    try:
+        dt = np.float64 if fp64 else np.float32
+        count = n.value * 4
+        buf = (C.c_char * (count * np.dtype(dt).itemsize)).from_address(rows.value)
+        out = np.frombuffer(buf, dtype=dt, count=count).reshape(n.value, 4).copy()
+    finally:
+        lib().capow_free_rows(rows)
+    return out
 
 
 class Batch:

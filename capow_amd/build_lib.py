@@ -26,7 +26,7 @@ HIP_FLAGS = [
     "-Wno-unused-result", "-Wno-unused-value",
 ]
 HIP_SOURCES = ["capow_api.hip", "deposit.hip", "fft.hip", "binning.hip", "adjoint.hip"]
-C_SOURCES = ["capow_host.c"]
+C_SOURCES = ["capow_host.c", "capow_ascii.c"]
 
 
 def _run(cmd):
@@ -47,7 +47,7 @@ def build(verbose=False, jobs=4):
         objs.append(o)
     for s in C_SOURCES:
         o = os.path.join(OBJ, s.replace(".c", ".o"))
-        cmds.append(["gcc", "-O2", "-fPIC", "-std=gnu11", "-Wall", "-I", INC, "-c",
+        cmds.append(["gcc", "-O2", "-fPIC", "-pthread", "-std=gnu11", "-Wall", "-I", INC, "-c",
                      os.path.join(CSRC, s), "-o", o])
         objs.append(o)
     with cf.ThreadPoolExecutor(jobs) as ex:
@@ -55,7 +55,7 @@ def build(verbose=False, jobs=4):
             if verbose and r.stderr:
                 print(r.stderr, file=sys.stderr)
     tmp = LIB + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lpthread"])
     os.replace(tmp, LIB)
     host_srcs = sorted(f for f in os.listdir(HOST) if f.endswith(".c")) if os.path.isdir(HOST) else []
     if host_srcs:

@@ -1,0 +1,234 @@
+/* capow_ascii.c -- parallel reader of the periodic-box ASCII catalogue.
+ *
+ * Replaces the parse loop of read_periodic_serial_ascii (reference
+ * src/read_data.c:32-88) with the same semantics, split over threads:
+ *   - the file is read as fgets with the reference's 2000-character buffer
+ *     reads it: each physical line in pieces of at most 1999 characters (the
+ *     last piece keeps the newline), and every piece is one "line";
+ *   - a piece starting with '#' is skipped;
+ *   - a piece must hold six numbers, converted by strtod as sscanf("%lf")
+ *     does (extra columns ignored), or the read fails with that piece;
+ *   - v_los is column 4, 5 or 6 for LOS = 1, 2, otherwise (read_data.c:65-66);
+ *   - with LOS = 0 a particle outside [MIN, MAX] on any axis is fatal.
+ * The file is mapped, cut into one range per thread at line starts, and
+ * every thread parses its range into its own rows; errors are reported for
+ * the first failing line in file order, so the result does not depend on
+ * the thread count.
+ */
+#define _GNU_SOURCE
+#include <errno.h>
+#include <fcntl.h>
+#include <pthread.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include "capow.h"
+
+#define LINE_BUF 2000   /* the reference's fgets buffer (src/read_data.c:35) */
+
+typedef struct {
+  const char *base;
+  size_t beg, end;          /* byte range of whole physical lines */
+  int32_t los, fp64;
+  double lo[3], hi[3];
+  unsigned char *rows;      /* (x, y, z, v_los) per accepted line */
+  size_t n, cap;
+  int err;                  /* CAPOW_OK or the first error in this range */
+  size_t err_off, err_len;  /* the failing piece */
+  double err_xyz[3];
+} chunk_t;
+
+/* Six numbers as sscanf(buf, "%lf %lf %lf %lf %lf %lf") reads them. */
+static int parse6(const char *s, double *v) {
+  char *end;
+  for (int q = 0; q < 6; q++) {
+    v[q] = strtod(s, &end);
+    if (end == s) return q;
+    s = end;
+  }
+  return 6;
+}
+
+static int push_row(chunk_t *c, const double *v) {
+  const size_t esz = c->fp64 ? 8 : 4;
+  if (c->n == c->cap) {
+    const size_t cap = c->cap ? 2 * c->cap : 4096;
+    unsigned char *r = (unsigned char *)realloc(c->rows, cap * 4 * esz);
+    if (!r) return CAPOW_ERR_NOMEM;
+    c->rows = r;
+    c->cap = cap;
+  }
+  const double tvr = (c->los == 1) ? v[3] : (c->los == 2 ? v[4] : v[5]);
+  if (c->fp64) {
+    double *r = (double *)c->rows + 4 * c->n;
+    r[0] = v[0]; r[1] = v[1]; r[2] = v[2]; r[3] = tvr;
+  } else {
+    float *r = (float *)c->rows + 4 * c->n;
+    r[0] = (float)v[0]; r[1] = (float)v[1]; r[2] = (float)v[2]; r[3] = (float)tvr;
+  }
+  c->n++;
+  return CAPOW_OK;
+}
+
+static void *parse_range(void *arg) {
+  chunk_t *c = (chunk_t *)arg;
+  char buf[LINE_BUF];
+  size_t pos = c->beg;
+  while (pos < c->end) {
+    /* one fgets piece: up to LINE_BUF - 1 characters, through the newline */
+    size_t len = 0;
+    while (len < LINE_BUF - 1 && pos + len < c->end) {
+      const char ch = c->base[pos + len];
+      len++;
+      if (ch == '\n') break;
+    }
+    const size_t off = pos;
+    pos += len;
+    if (c->base[off] == '#') continue;
+    memcpy(buf, c->base + off, len);
+    buf[len] = 0;
+    double v[6];
+    if (parse6(buf, v) != 6) {
+      c->err = CAPOW_ERR_PARSE;
+      c->err_off = off;
+      c->err_len = len;
+      return NULL;
+    }
+    if (c->los == 0 && ((v[0] < c->lo[0]) || (v[0] > c->hi[0]) || (v[1] < c->lo[1]) ||
+                        (v[1] > c->hi[1]) || (v[2] < c->lo[2]) || (v[2] > c->hi[2]))) {
+      c->err = CAPOW_ERR_OUT_OF_BOX;
+      c->err_off = off;
+      c->err_len = len;
+      c->err_xyz[0] = v[0];
+      c->err_xyz[1] = v[1];
+      c->err_xyz[2] = v[2];
+      return NULL;
+    }
+    if (push_row(c, v) != CAPOW_OK) {
+      c->err = CAPOW_ERR_NOMEM;
+      c->err_off = off;
+      c->err_len = 0;
+      return NULL;
+    }
+  }
+  return NULL;
+}
+
+int capow_read_ascii(const char *path, int32_t los, const double lo[3], const double hi[3],
+                     int32_t fp64, int32_t nthreads, void **rows_out, int64_t *n_out,
+                     char *bad_line, int32_t bad_len, double bad_xyz[3]) {
+  if (!path || !lo || !hi || !rows_out || !n_out) return CAPOW_ERR_ARG;
+  *rows_out = NULL;
+  *n_out = 0;
+  const int fd = open(path, O_RDONLY);
+  if (fd < 0) return CAPOW_ERR_IO;
+  struct stat sb;
+  if (fstat(fd, &sb) != 0) {
+    close(fd);
+    return CAPOW_ERR_IO;
+  }
+  const size_t size = (size_t)sb.st_size;
+  const size_t esz = fp64 ? 8 : 4;
+  if (size == 0) {
+    close(fd);
+    *rows_out = malloc(4 * esz);
+    return *rows_out ? CAPOW_OK : CAPOW_ERR_NOMEM;
+  }
+  const char *base = (const char *)mmap(NULL, size, PROT_READ, MAP_PRIVATE, fd, 0);
+  close(fd);
+  if (base == MAP_FAILED) return CAPOW_ERR_IO;
+  madvise((void *)base, size, MADV_SEQUENTIAL);
+
+  int nt = nthreads;
+  if (nt <= 0) {
+    const long cpus = sysconf(_SC_NPROCESSORS_ONLN);
+    nt = cpus > 0 ? (int)cpus : 1;
+    if (nt > 32) nt = 32;
+  }
+  /* at least 1 MB per thread */
+  const size_t max_nt = size / (1u << 20) + 1;
+  if ((size_t)nt > max_nt) nt = (int)max_nt;
+
+  chunk_t *ch = (chunk_t *)calloc((size_t)nt, sizeof(chunk_t));
+  pthread_t *th = (pthread_t *)calloc((size_t)nt, sizeof(pthread_t));
+  if (!ch || !th) {
+    free(ch);
+    free(th);
+    munmap((void *)base, size);
+    return CAPOW_ERR_NOMEM;
+  }
+  /* ranges start at the beginning of a physical line */
+  size_t prev = 0;
+  for (int t = 0; t < nt; t++) {
+    size_t b = prev;
+    size_t e = (t == nt - 1) ? size : (size_t)((unsigned __int128)size * (t + 1) / nt);
+    if (e < b) e = b;
+    while (e < size && e > 0 && base[e - 1] != '\n') e++;
+    ch[t].base = base;
+    ch[t].beg = b;
+    ch[t].end = e;
+    ch[t].los = los;
+    ch[t].fp64 = fp64;
+    for (int a = 0; a < 3; a++) {
+      ch[t].lo[a] = lo[a];
+      ch[t].hi[a] = hi[a];
+    }
+    prev = e;
+  }
+  int started = 0;
+  for (int t = 1; t < nt; t++) {
+    if (pthread_create(&th[t], NULL, parse_range, &ch[t]) != 0) break;
+    started = t;
+  }
+  parse_range(&ch[0]);
+  /* ranges whose thread could not start are parsed here */
+  for (int t = started + 1; t < nt; t++) parse_range(&ch[t]);
+  for (int t = 1; t <= started; t++) pthread_join(th[t], NULL);
+
+  int rc = CAPOW_OK;
+  size_t total = 0;
+  for (int t = 0; t < nt; t++) {
+    if (ch[t].err != CAPOW_OK) {
+      /* ranges are in file order: the first failing range holds the first
+         failing line */
+      rc = ch[t].err;
+      if (bad_line && bad_len > 0) {
+        size_t l = ch[t].err_len < (size_t)(bad_len - 1) ? ch[t].err_len : (size_t)(bad_len - 1);
+        memcpy(bad_line, base + ch[t].err_off, l);
+        bad_line[l] = 0;
+      }
+      if (bad_xyz)
+        for (int a = 0; a < 3; a++) bad_xyz[a] = ch[t].err_xyz[a];
+      break;
+    }
+    total += ch[t].n;
+  }
+  unsigned char *out = NULL;
+  if (rc == CAPOW_OK) {
+    out = (unsigned char *)malloc((total ? total : 1) * 4 * esz);
+    if (!out) {
+      rc = CAPOW_ERR_NOMEM;
+    } else {
+      size_t o = 0;
+      for (int t = 0; t < nt; t++) {
+        if (ch[t].n) memcpy(out + o * 4 * esz, ch[t].rows, ch[t].n * 4 * esz);
+        o += ch[t].n;
+      }
+    }
+  }
+  for (int t = 0; t < nt; t++) free(ch[t].rows);
+  free(ch);
+  free(th);
+  munmap((void *)base, size);
+  if (rc == CAPOW_OK) {
+    *rows_out = out;
+    *n_out = (int64_t)total;
+  }
+  return rc;
+}
+
+void capow_free_rows(void *rows) { free(rows); }

@@ -217,63 +217,33 @@ static void banner(const char *t) {
   fflush(stdout);
 }
 
-/* Parses "x y z vx vy vz" (extra columns ignored) like sscanf("%lf"x6). */
-static int parse6(const char *s, double *v) {
-  char *end;
-  for (int q = 0; q < 6; q++) {
-    errno = 0;
-    v[q] = strtod(s, &end);
-    if (end == s) return q;
-    s = end;
-  }
-  return 6;
-}
-
 /* src/read_data.c:32-88: read the catalogue, pick v_los by LOS, check the
- * box for LOS = 0.  Returns rows of (x, y, z, v_los) in the working
+ * box for LOS = 0 (capow_read_ascii: the same line semantics, parsed over
+ * one thread per CPU).  Returns rows of (x, y, z, v_los) in the working
  * precision.  The RSD shift and the deposit run on the GPU. */
 static void *read_catalogue(const char *path, int fp64, long long *n_out) {
-  FILE *fp = fopen(path, "r");
-  if (!fp) {
+  const double lo[3] = {XMIN, YMIN, ZMIN}, hi[3] = {XMAX, YMAX, ZMAX};
+  void *rows = NULL;
+  int64_t n = 0;
+  char bad[2000];
+  double xyz[3] = {0, 0, 0};
+  const int rc = capow_read_ascii(path, LOS, lo, hi, fp64, 0, &rows, &n, bad, (int32_t)sizeof bad, xyz);
+  if (rc == CAPOW_ERR_IO) {
     printf("Task %d cannot open input file\n", 0);
     FatalError("read_data", 96);
   }
-  size_t cap = 1 << 20, n = 0, esz = fp64 ? 8 : 4;
-  unsigned char *rows = (unsigned char *)malloc(cap * 4 * esz);
-  char buf[2000];
-  while (fgets(buf, (int)sizeof buf, fp)) {
-    if (strncmp(buf, "#", 1) == 0) continue;
-    double v[6];
-    if (parse6(buf, v) != 6) {
-      printf("Task %d has error reading file: %s\n", 0, buf);
-      FatalError("read_data", 102);
-    }
-    double tvr = (LOS == 1) ? v[3] : (LOS == 2 ? v[4] : v[5]);
-    if (LOS == 0) {
-      if ((v[0] < XMIN) || (v[0] > XMAX) || (v[1] < YMIN) || (v[1] > YMAX) || (v[2] < ZMIN) ||
-          (v[2] > ZMAX)) {
-        printf("Task %d has particle out of grid bounds: x=%lf, y=%lf, z=%lf\n", 0, v[0], v[1], v[2]);
-        FatalError("read_data", 110);
-      }
-    }
-    if (n == cap) {
-      cap *= 2;
-      rows = (unsigned char *)realloc(rows, cap * 4 * esz);
-      if (!rows) {
-        printf("Task 0 unable to allocate memory for the catalogue\n");
-        FatalError("read_data", 96);
-      }
-    }
-    if (fp64) {
-      double *r = (double *)rows + 4 * n;
-      r[0] = v[0]; r[1] = v[1]; r[2] = v[2]; r[3] = tvr;
-    } else {
-      float *r = (float *)rows + 4 * n;
-      r[0] = (float)v[0]; r[1] = (float)v[1]; r[2] = (float)v[2]; r[3] = (float)tvr;
-    }
-    n++;
+  if (rc == CAPOW_ERR_PARSE) {
+    printf("Task %d has error reading file: %s\n", 0, bad);
+    FatalError("read_data", 102);
   }
-  fclose(fp);
+  if (rc == CAPOW_ERR_OUT_OF_BOX) {
+    printf("Task %d has particle out of grid bounds: x=%lf, y=%lf, z=%lf\n", 0, xyz[0], xyz[1], xyz[2]);
+    FatalError("read_data", 110);
+  }
+  if (rc != CAPOW_OK) {
+    printf("Task 0 unable to allocate memory for the catalogue\n");
+    FatalError("read_data", 96);
+  }
   *n_out = (long long)n;
   return rows;
 }
@@ -334,7 +304,7 @@ int main(int argc, char **argv) {
     FatalError("read_data", 110);
   }
   check(ctx, rc, "read_data", 133);
-  free(rows);
+  capow_free_rows(rows);
   if (LOS == 0) printf("Task %d read %lf particles, gridded %lf\n", 0, st.nread, st.ngrid);
   else printf("Task %d read %lf particles, gridded %lf (%lf out of bounds due to RSD)\n", 0, st.nread, st.ngrid, st.nout);
 

@@ -46,6 +46,7 @@ enum {
   CAPOW_ERR_UNSUPPORTED = 5,/* grid shape the kernels do not handle        */
   CAPOW_ERR_INTERNAL = 6,   /* a kernel self-check failed                  */
   CAPOW_ERR_IO = 7,         /* file could not be opened / written          */
+  CAPOW_ERR_PARSE = 8,      /* catalogue line without six numbers          */
   CAPOW_ERR_OUT_OF_BOX = 110
 };
 
@@ -233,6 +234,26 @@ int capow_batch_run(capow_batch *b, const capow_mock *mocks, int32_t n,
                     capow_bins *out, capow_assign_stats *stats, float *ms);
 int capow_batch_destroy(capow_batch *b);
 const char *capow_batch_last_error(const capow_batch *b);
+
+/* ---- catalogue input (no GPU) ------------------------------------------ */
+
+/* Reads the periodic ASCII catalogue the way read_periodic_serial_ascii
+ * parses it (src/read_data.c:32-88): fgets pieces of at most 1999
+ * characters, '#' pieces skipped, six numbers per piece ("x y z vx vy vz",
+ * extra columns ignored), v_los = vx, vy or vz for LOS 1, 2, else vz; with
+ * LOS = 0 a particle outside [lo, hi] is fatal.  Parsed by nthreads threads
+ * (<= 0: one per CPU, at most 32) over line-aligned ranges of the mapped
+ * file; the result and the reported error do not depend on the thread count.
+ * On success *rows holds n rows of (x, y, z, v_los) in float64 (fp64) or
+ * float32, allocated by the library: release them with capow_free_rows.
+ * Errors: CAPOW_ERR_IO (cannot open), CAPOW_ERR_PARSE (the first piece in
+ * file order without six numbers, copied to bad_line), CAPOW_ERR_OUT_OF_BOX
+ * (LOS = 0: the first particle outside the box, its position in bad_xyz and
+ * its piece in bad_line), CAPOW_ERR_NOMEM.  bad_line / bad_xyz may be NULL. */
+int capow_read_ascii(const char *path, int32_t los, const double lo[3], const double hi[3],
+                     int32_t fp64, int32_t nthreads, void **rows, int64_t *n,
+                     char *bad_line, int32_t bad_len, double bad_xyz[3]);
+void capow_free_rows(void *rows);
 
 /* ---- host-side output math (no GPU) ------------------------------------ */
 

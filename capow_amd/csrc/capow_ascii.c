@@ -19,6 +19,7 @@
 #include <errno.h>
 #include <fcntl.h>
 #include <pthread.h>
+#include <sched.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -145,9 +146,17 @@ int capow_read_ascii(const char *path, int32_t los, const double lo[3], const do
 
   int nt = nthreads;
   if (nt <= 0) {
-    const long cpus = sysconf(_SC_NPROCESSORS_ONLN);
-    nt = cpus > 0 ? (int)cpus : 1;
-    if (nt > 32) nt = 32;
+    /* OMP_NUM_THREADS, else the CPUs this process may run on; at most 16 */
+    const char *e = getenv("OMP_NUM_THREADS");
+    if (e && atoi(e) > 0) {
+      nt = atoi(e);
+    } else {
+      cpu_set_t set;
+      CPU_ZERO(&set);
+      nt = sched_getaffinity(0, sizeof set, &set) == 0 ? CPU_COUNT(&set) : 1;
+    }
+    if (nt > 16) nt = 16;
+    if (nt < 1) nt = 1;
   }
   /* at least 1 MB per thread */
   const size_t max_nt = size / (1u << 20) + 1;

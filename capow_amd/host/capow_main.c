@@ -219,7 +219,7 @@ static void banner(const char *t) {
 
 /* src/read_data.c:32-88: read the catalogue, pick v_los by LOS, check the
  * box for LOS = 0 (capow_read_ascii: the same line semantics, parsed over
- * one thread per CPU).  Returns rows of (x, y, z, v_los) in the working
+ * up to 16 threads).  Returns rows of (x, y, z, v_los) in the working
  * precision.  The RSD shift and the deposit run on the GPU. */
 static void *read_catalogue(const char *path, int fp64, long long *n_out) {
   const double lo[3] = {XMIN, YMIN, ZMIN}, hi[3] = {XMAX, YMAX, ZMAX};

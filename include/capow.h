@@ -242,8 +242,9 @@ const char *capow_batch_last_error(const capow_batch *b);
  * characters, '#' pieces skipped, six numbers per piece ("x y z vx vy vz",
  * extra columns ignored), v_los = vx, vy or vz for LOS 1, 2, else vz; with
  * LOS = 0 a particle outside [lo, hi] is fatal.  Parsed by nthreads threads
- * (<= 0: one per CPU, at most 32) over line-aligned ranges of the mapped
- * file; the result and the reported error do not depend on the thread count.
+ * (<= 0: OMP_NUM_THREADS if set, else the CPUs the process may run on; at
+ * most 16) over line-aligned ranges of the mapped file; the result and the
+ * reported error do not depend on the thread count.
  * On success *rows holds n rows of (x, y, z, v_los) in float64 (fp64) or
  * float32, allocated by the library: release them with capow_free_rows.
  * Errors: CAPOW_ERR_IO (cannot open), CAPOW_ERR_PARSE (the first piece in

@@ -630,7 +630,7 @@ __global__ __launch_bounds__(256) void k_bin_counts(const double* __restrict__ t
     const double fx = tabs[a.ofx + i], fy = tabs[a.ofy + j];
     const double fxy = dadd(dmul(fx, fx), dmul(fy, fy));
     for (int k = lane; k < a.nzh; k += 64) {
-      KBin kb = kbin_fast(fxy, tabs[a.ofz + k], a);
+      KBin kb = kbin_any(fxy, tabs[a.ofz + k], a);
       if (kb.near) kb = kbin_exact(fxy, tabs[a.ofz + k], a);
       if (kb.ok) atomicAdd(&sc[kb.kbin], nm_weight(i, j, k, a.nx));
     }
@@ -734,16 +734,22 @@ static int run_bin_t(BinLaunch& L) {
   const int per_copy = L.per_copy;
   const size_t bins = (size_t)per_copy * L.dev.copies * sizeof(double);
   const int nuf = L.dev.nzh / 64;
+  // branch-free row form (linear bins) and mirror-group form (linear or log
+  // bins): LDS copies of the sums only, mode counts from k_bin_counts
   const bool fast = L.row_form && !L.dev.logbins && !L.dev.out2d;
+  const bool mirror = L.row_form && L.mirror && !L.dev.out2d;
+  const bool sums = fast || mirror;
   const int nl = L.dev.odd ? 5 : 3;
   const int F = L.dev.nspec * nl;
-  const size_t lcp = fast ? (size_t)F * L.dev.nk : (size_t)per_copy;
+  const size_t lcp = sums ? (size_t)F * L.dev.nk : (size_t)per_copy;
   const size_t lds = lcp * L.dev.copies * sizeof(double) + (size_t)L.dev.nzh * 4 * sizeof(double);
-  if (fast) {
+  if (sums) {
     const int rc = run_bin_counts(L);
     if (rc) return rc;
   }
-  if (fast && L.mirror && lds <= 160 * 1024) {
+  if (mirror) {
+    // the host sizes the copies of the sums to fit beside the z tables
+    if (lds > 160 * 1024) return CAPOW_ERR_UNSUPPORTED;
     const bool il = L.dev.interlace != 0, hb = L.dev.hasB != 0;
     if (il && hb) launch_mirror<T, true, true>(L, lds);
     else if (il) launch_mirror<T, true, false>(L, lds);
@@ -766,7 +772,7 @@ static int run_bin_t(BinLaunch& L) {
               (const T*)L.grid[0], (const T*)L.grid[1], (const T*)L.grid[2], (const T*)L.grid[3],
               L.tabs, L.dev, L.partials)));
   }
-  return run_bin_final(L, fast);
+  return run_bin_final(L, sums);
 }
 
 int run_bin(BinLaunch& L, bool fp64) { return fp64 ? run_bin_t<double>(L) : run_bin_t<float>(L); }

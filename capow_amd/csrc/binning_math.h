@@ -169,6 +169,28 @@ __device__ __forceinline__ KBin kbin_exact(double fxy, double fz, const BinArgsD
   return r;
 }
 
+// Log k bin (OutputLog, src/main.c:369-371): (int)((float)(log10|k| - Mink)
+// / binwidth) with the exact division, as mode_geom computes it; k = 0 is not
+// binned.
+__device__ __forceinline__ KBin kbin_log(double fxy, double fz, const BinArgsDev& a) {
+  KBin r;
+  r.fktot = dmul(2.0 * 3.14159265358979323846, sqrt(dadd(fxy, dmul(fz, fz))));
+  r.near = false;
+  r.kbin = 0;
+  r.ok = false;
+  if (r.fktot > 0) {
+    r.kbin = (int)ddiv((double)__double2float_rn(log10(r.fktot) - a.mink), a.binwidth);
+    r.ok = (r.fktot <= a.min_nyq) && r.kbin >= 0 && r.kbin < a.nk;
+  }
+  return r;
+}
+
+// The bin of a mode in the branch-free forms: linear bins by multiplication
+// (near-integer quotients flagged for kbin_exact), log bins exactly.
+__device__ __forceinline__ KBin kbin_any(double fxy, double fz, const BinArgsDev& a) {
+  return a.logbins ? kbin_log(fxy, fz, a) : kbin_fast(fxy, fz, a);
+}
+
 // The reference's Hermitian weight NM (src/main.c:422-429): 2, except 1 in
 // the kz = 0 plane where a mode's partner (-i, -j, 0) is stored as well
 // (the rows i = 0, NX/2 of j = 0 and i = NX/2 keep 2, as there).
@@ -271,7 +293,7 @@ __device__ __forceinline__ ModeVals group_mode(const Group& G, int k, double fz,
                                                const BinArgsDev& a) {
   ModeVals V;
   const double twopi = 2.0 * 3.14159265358979323846;
-  const KBin kb = kbin_fast(G.fxy, fz, a);
+  const KBin kb = kbin_any(G.fxy, fz, a);
   const double fktot = kb.fktot;
   V.near = kb.near;
   V.kbin = kb.kbin;

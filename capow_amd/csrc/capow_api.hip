@@ -436,7 +436,9 @@ static int setup(capow_ctx* c) {
     // sums in LDS: its mode counts are computed once per context.
     const size_t ztab = (size_t)nzh * 4 * 8;
     c->bin_row_form = bytes + ztab <= lds_max;
-    const bool fast = c->bin_row_form && !b.logbins && !b.out2d;
+    // the branch-free row form (linear bins) and the mirror-group form
+    // (linear or log bins) keep only the sums in LDS
+    const bool fast = c->bin_row_form && !b.out2d && (!b.logbins || c->bin_mirror);
     const size_t lbytes = fast ? (size_t)ns * nl * p.nk * 8 : bytes;
     int copies = (int)((lds_max - (c->bin_row_form ? ztab : 0)) / lbytes);
     b.copies = copies > 8 ? 8 : copies;

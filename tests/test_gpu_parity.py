@@ -380,6 +380,13 @@ MIRROR_CASES = [
      dict(interp_order=2, do_interlacing=1, momentum=1, odd_multipoles=1, los=1)),
     # no interlacing, momentum cross with its autos, odd NX (row form)
     ("fp32_odd_nx", api.FP32, (45, 32, 64), dict(interp_order=1, momentum=3, extra_spectra=3)),
+    # log bins (OutputLog): one exact log10 bin per group
+    ("fp64_log_bins", api.FP64, (64, 48, 64),
+     dict(interp_order=2, do_interlacing=1, momentum=1, extra_spectra=3, output_log=1,
+          mink=-2.5, maxk=-0.6, nk=25)),
+    ("fp32_log_two_per_lane", api.FP32, (32, 32, 256),
+     dict(interp_order=3, do_interlacing=1, momentum=0, output_log=1, mink=-2.5, maxk=-0.9,
+          nk=20)),
 ]
 
 

@@ -471,25 +471,58 @@ __device__ __forceinline__ void group_load(const C* __restrict__ A, const C* __r
 // flight while the current one is binned.  Modes kz >= 64 MPL nch (a few at
 // the end of the row that do not fill a chunk) are binned afterwards with one
 // group per lane.
-template <typename T, int MPL, bool IL, bool HB>
+// Accumulation of one evaluated group mode in the mirror-group kernel.
+// MODE 0: the 1-D sums into this wave's LDS copy; MODE 1: also the mode's
+// bound into the copy's extra NK entries (the per-k scale of MODE 2);
+// MODE 2: the (k, mu) sums as integer quanta of their k bin's scale
+// (np * qinv[kbin] rounded) into the workgroup's one int64 table: integer
+// adds do not depend on their order, so all waves share it.
+template <int MODE>
+__device__ __forceinline__ void mirror_accum(const ModeVals& mv, double fl, const BinArgsDev& a,
+                                             double* accS, unsigned long long* acc2i,
+                                             const double* __restrict__ qinv, int nl, int FNK,
+                                             int n2d) {
+  if constexpr (MODE < 2) {
+    mode_add(mv, a, accS, nl);
+    if constexpr (MODE == 1) atomicAdd(&accS[FNK + mv.kbin], mv.bound);
+  } else {
+    bool in2d;
+    const int b2 = mu_bin2d(mv.kbin, fl, mv.fktot, a, in2d);
+    if (!in2d) return;
+    const double iq = qinv[mv.kbin];
+#pragma unroll
+    for (int s = 0; s < 3; s++) {
+      if (s >= a.nspec) break;
+      atomicAdd(&acc2i[s * n2d + b2], (unsigned long long)__double2ll_rn(mv.np[s] * iq));
+    }
+  }
+}
+
+template <typename T, int MPL, bool IL, bool HB, int MODE>
 __global__ __launch_bounds__(512) void k_bin_mirror(const T* __restrict__ gA,
                                                     const T* __restrict__ gAi,
                                                     const T* __restrict__ gB,
                                                     const T* __restrict__ gBi,
                                                     const double* __restrict__ tabs, BinArgsDev a,
-                                                    int nch, double* __restrict__ partials) {
+                                                    int nch, double* __restrict__ partials,
+                                                    const double* __restrict__ qinv,
+                                                    long long* __restrict__ partials2) {
   typedef typename CT<T>::t C;
   extern __shared__ double sm[];
   constexpr int CH = 64 * MPL;
   const int NK = a.nk;
   const int nl = a.odd ? 5 : 3;
   const int F = a.nspec * nl;
-  const int lcp = F * NK;
-  const int per_copy = F * NK + NK;
-  const int copies = a.copies;
+  const int n2d = a.out2d ? a.nk * a.nmu : 0;
+  // MODE < 2: one copy of the sums (and bounds) per wave; MODE 2: one int64
+  // (k, mu) table per workgroup
+  const int lcp = MODE == 2 ? a.nspec * n2d : F * NK + (MODE == 1 ? NK : 0);
+  const int per_copy = F * NK + a.nspec * n2d + NK + n2d;
+  const int copies = MODE == 2 ? 1 : a.copies;
   const int nzh = a.nzh;
   const int ktail = nch * CH;
   double* zt = sm + copies * lcp;   // fz, corz, phz.re, phz.im per kz
+  unsigned long long* acc2i = reinterpret_cast<unsigned long long*>(sm);
   for (int q = threadIdx.x; q < lcp * copies; q += blockDim.x) sm[q] = 0.0;
   for (int q = threadIdx.x; q < nzh; q += blockDim.x) {
     zt[4 * q] = tabs[a.ofz + q];
@@ -500,7 +533,7 @@ __global__ __launch_bounds__(512) void k_bin_mirror(const T* __restrict__ gA,
   __syncthreads();
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int lane = threadIdx.x & 63;
-  double* accS = sm + wave * lcp;
+  double* accS = sm + (MODE == 2 ? 0 : wave) * lcp;
   const C* A = reinterpret_cast<const C*>(gA);
   const C* Ai = reinterpret_cast<const C*>(gAi);
   const C* B = reinterpret_cast<const C*>(gB);
@@ -509,8 +542,9 @@ __global__ __launch_bounds__(512) void k_bin_mirror(const T* __restrict__ gA,
   const int64_t rowc = a.rowc;
   const int hy1 = a.ny / 2 + 1;
   const int ng = (a.nx / 2 + 1) * hy1;
-  const int nw = gridDim.x * copies;
-  const int w0 = blockIdx.x * copies + wave;
+  const int wpb = blockDim.x >> 6;   // waves per workgroup
+  const int nw = gridDim.x * wpb;
+  const int w0 = blockIdx.x * wpb + wave;
   if (nch > 0 && w0 < ng) {
     C cur[4][4][MPL];
     {
@@ -560,7 +594,11 @@ __global__ __launch_bounds__(512) void k_bin_mirror(const T* __restrict__ gA,
         }
 #pragma unroll
         for (int u = 0; u < MPL; u++)
-          if (MV[u].ok && c * CH + MPL * lane + u < nzh) mode_add(MV[u], a, accS, nl);
+          if (MV[u].ok && c * CH + MPL * lane + u < nzh) {
+            const int k = c * CH + MPL * lane + u;
+            mirror_accum<MODE>(MV[u], zt[4 * (k < nzh ? k : nzh - 1)], a, accS, acc2i, qinv, nl,
+                               F * NK, n2d);
+          }
         if (gn < ng) {
 #pragma unroll
           for (int r = 0; r < 4; r++)
@@ -597,19 +635,47 @@ __global__ __launch_bounds__(512) void k_bin_mirror(const T* __restrict__ gA,
             mv.kbin = e.kbin;
             mv.ok = e.ok;
           }
-          if (mv.ok) mode_add(mv, a, accS, nl);
+          if (mv.ok) mirror_accum<MODE>(mv, z.x, a, accS, acc2i, qinv, nl, F * NK, n2d);
         }
       }
     }
   }
   __syncthreads();
-  double* out = partials + (int64_t)blockIdx.x * per_copy;
-  for (int q = threadIdx.x; q < per_copy; q += blockDim.x) {
-    double t = 0.0;
-    if (q < lcp)
-      for (int c = 0; c < copies; c++) t += sm[c * lcp + q];
-    out[q] = t;
+  if constexpr (MODE == 2) {
+    long long* out = partials2 + (int64_t)blockIdx.x * lcp;
+    for (int q = threadIdx.x; q < lcp; q += blockDim.x) out[q] = (long long)acc2i[q];
+  } else {
+    double* out = partials + (int64_t)blockIdx.x * per_copy;
+    for (int q = threadIdx.x; q < per_copy; q += blockDim.x) {
+      double t = 0.0;
+      if (q < lcp)
+        for (int c = 0; c < copies; c++) t += sm[c * lcp + q];
+      out[q] = t;
+    }
   }
+}
+
+// Per-k scales of the (k, mu) quanta from the bounds (sums of NM cor
+// (|A|^2 + |B|^2) over each k bin, which no (k, mu) partial sum of any
+// spectrum exceeds in magnitude): q = B 2^-61 (exact), qinv = 2^61 / B, so
+// the int64 tables cannot overflow.
+__global__ void k_bin_scale(const double* __restrict__ bound, int nk, double* __restrict__ q) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nk; k += gridDim.x * blockDim.x) {
+    const double b = bound[k];
+    q[k] = b > 0 ? ldexp(b, -61) : 0.0;
+    q[nk + k] = b > 0 ? ldexp(1.0, 61) / b : 0.0;
+  }
+}
+
+// (k, mu) sums: the int64 workgroup tables summed exactly, times their k
+// bin's quantum, into out[0 .. nspec * n2d).
+__global__ void k_bin_final2d(const long long* __restrict__ partials2, int nwg, int len, int n2d,
+                              int nmu, const double* __restrict__ q, double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= len) return;
+  long long t = 0;
+  for (int w = 0; w < nwg; w++) t += partials2[(int64_t)w * len + i];
+  out[i] = (double)t * q[(i % n2d) / nmu];
 }
 
 // Mode counts of the branch-free form (they depend on the geometry and the
@@ -619,7 +685,8 @@ __global__ __launch_bounds__(512) void k_bin_mirror(const T* __restrict__ gA,
 __global__ __launch_bounds__(256) void k_bin_counts(const double* __restrict__ tabs,
                                                     BinArgsDev a, double* __restrict__ counts) {
   extern __shared__ double sc[];
-  for (int q = threadIdx.x; q < a.nk; q += blockDim.x) sc[q] = 0.0;
+  const int n2d = a.out2d ? a.nk * a.nmu : 0;   // (k, mu) counts after the k counts
+  for (int q = threadIdx.x; q < a.nk + n2d; q += blockDim.x) sc[q] = 0.0;
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int64_t nrows = (int64_t)a.nx * a.ny;
@@ -632,22 +699,32 @@ __global__ __launch_bounds__(256) void k_bin_counts(const double* __restrict__ t
     for (int k = lane; k < a.nzh; k += 64) {
       KBin kb = kbin_any(fxy, tabs[a.ofz + k], a);
       if (kb.near) kb = kbin_exact(fxy, tabs[a.ofz + k], a);
-      if (kb.ok) atomicAdd(&sc[kb.kbin], nm_weight(i, j, k, a.nx));
+      if (kb.ok) {
+        const double nm = nm_weight(i, j, k, a.nx);
+        atomicAdd(&sc[kb.kbin], nm);
+        if (n2d) {
+          const double fz = tabs[a.ofz + k];
+          bool in2d;
+          const int b2 = mu_bin2d(kb.kbin, a.los == 1 ? fx : (a.los == 2 ? fy : fz), kb.fktot, a,
+                                  in2d);
+          if (in2d) atomicAdd(&sc[a.nk + b2], nm);
+        }
+      }
     }
   }
   __syncthreads();
-  for (int q = threadIdx.x; q < a.nk; q += blockDim.x)
+  for (int q = threadIdx.x; q < a.nk + n2d; q += blockDim.x)
     if (sc[q] != 0.0) atomicAdd(&counts[q], sc[q]);
 }
 
 // Fixed-order sum over workgroup partials; with `counts`, the mode counts
-// [cnt_off, cnt_off + nk) are taken from there.
+// [cnt_off, cnt_off + ncnt) are taken from there.
 __global__ void k_bin_final(const double* __restrict__ partials, int nwg, int per_copy,
-                            const double* __restrict__ counts, int cnt_off, int nk,
+                            const double* __restrict__ counts, int cnt_off, int ncnt,
                             double* __restrict__ out) {
   int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= per_copy) return;
-  if (counts && q >= cnt_off && q < cnt_off + nk) {
+  if (counts && q >= cnt_off && q < cnt_off + ncnt) {
     out[q] = counts[q - cnt_off];
     return;
   }
@@ -661,24 +738,37 @@ __global__ void k_bin_final(const double* __restrict__ partials, int nwg, int pe
 static int mirror_chunks(int nzh, int ch) { return nzh / ch + (nzh % ch >= 16 ? 1 : 0); }
 
 // FP32: two modes per lane (one 16-byte load per row and grid) once a row
-// fills a 128-mode chunk; FP64: one.
-template <typename T, bool IL, bool HB>
+// fills a 128-mode chunk; FP64: one.  MODE 2 runs 8 waves on one shared
+// table.
+template <typename T, bool IL, bool HB, int MODE>
 static void launch_mirror(BinLaunch& L, size_t lds) {
   constexpr int M2 = sizeof(T) == 4 ? 2 : 1;
   const bool two = M2 == 2 && L.dev.nzh >= 128;
-  const void* fn = two ? (const void*)k_bin_mirror<T, M2, IL, HB>
-                       : (const void*)k_bin_mirror<T, 1, IL, HB>;
+  const int threads = MODE == 2 ? 512 : 64 * L.dev.copies;
+  const void* fn = two ? (const void*)k_bin_mirror<T, M2, IL, HB, MODE>
+                       : (const void*)k_bin_mirror<T, 1, IL, HB, MODE>;
   if (lds > 64 * 1024) hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  const double* qinv = MODE == 2 ? L.qscale + L.dev.nk : nullptr;
+  const char* name = MODE == 2 ? "bin_2d" : "bin";
   if (two)
-    TIMED(L.timer, "bin",
-          (k_bin_mirror<T, M2, IL, HB><<<L.nwg, 64 * L.dev.copies, lds, L.stream>>>(
+    TIMED(L.timer, name,
+          (k_bin_mirror<T, M2, IL, HB, MODE><<<L.nwg, threads, lds, L.stream>>>(
               (const T*)L.grid[0], (const T*)L.grid[1], (const T*)L.grid[2], (const T*)L.grid[3],
-              L.tabs, L.dev, mirror_chunks(L.dev.nzh, 64 * M2), L.partials)));
+              L.tabs, L.dev, mirror_chunks(L.dev.nzh, 64 * M2), L.partials, qinv, L.partials2)));
   else
-    TIMED(L.timer, "bin",
-          (k_bin_mirror<T, 1, IL, HB><<<L.nwg, 64 * L.dev.copies, lds, L.stream>>>(
+    TIMED(L.timer, name,
+          (k_bin_mirror<T, 1, IL, HB, MODE><<<L.nwg, threads, lds, L.stream>>>(
               (const T*)L.grid[0], (const T*)L.grid[1], (const T*)L.grid[2], (const T*)L.grid[3],
-              L.tabs, L.dev, mirror_chunks(L.dev.nzh, 64), L.partials)));
+              L.tabs, L.dev, mirror_chunks(L.dev.nzh, 64), L.partials, qinv, L.partials2)));
+}
+
+template <typename T, int MODE>
+static void launch_mirror_any(BinLaunch& L, size_t lds) {
+  const bool il = L.dev.interlace != 0, hb = L.dev.hasB != 0;
+  if (il && hb) launch_mirror<T, true, true, MODE>(L, lds);
+  else if (il) launch_mirror<T, true, false, MODE>(L, lds);
+  else if (hb) launch_mirror<T, false, true, MODE>(L, lds);
+  else launch_mirror<T, false, false, MODE>(L, lds);
 }
 
 template <typename T, int NUF>
@@ -706,11 +796,14 @@ static void launch_rows(BinLaunch& L, size_t lds) {
 // the geometry and the bins only).
 int run_bin_counts(BinLaunch& L) {
   if (*L.counts_valid) return CAPOW_OK;
-  hipMemsetAsync(L.counts, 0, (size_t)L.dev.nk * sizeof(double), L.stream);
+  const int ncnt = L.dev.nk + (L.dev.out2d ? L.dev.nk * L.dev.nmu : 0);
+  hipMemsetAsync(L.counts, 0, (size_t)ncnt * sizeof(double), L.stream);
   const int64_t nrows = (int64_t)L.dev.nx * L.dev.ny;
   const int nblk = (int)std::min<int64_t>((nrows + 3) / 4, 4096);
-  k_bin_counts<<<nblk, 256, (size_t)L.dev.nk * sizeof(double), L.stream>>>(L.tabs, L.dev,
-                                                                         L.counts);
+  const size_t lds = (size_t)ncnt * sizeof(double);
+  if (lds > 64 * 1024)
+    hipFuncSetAttribute((const void*)k_bin_counts, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  k_bin_counts<<<nblk, 256, lds, L.stream>>>(L.tabs, L.dev, L.counts);
   if (hipGetLastError() != hipSuccess) return CAPOW_ERR_HIP;
   *L.counts_valid = true;
   return CAPOW_OK;
@@ -724,8 +817,8 @@ int run_bin_final(BinLaunch& L, bool fast) {
   TIMED(L.timer, "bin_final",
         (k_bin_final<<<(L.per_copy + 255) / 256, 256, 0, L.stream>>>(
             L.partials, L.nwg, L.per_copy, fast ? L.counts : nullptr,
-            F * L.dev.nk + L.dev.nspec * (L.dev.out2d ? L.dev.nk * L.dev.nmu : 0), L.dev.nk,
-            L.out)));
+            F * L.dev.nk + L.dev.nspec * (L.dev.out2d ? L.dev.nk * L.dev.nmu : 0),
+            L.dev.nk + (L.dev.out2d ? L.dev.nk * L.dev.nmu : 0), L.out)));
   return hipGetLastError() == hipSuccess ? CAPOW_OK : CAPOW_ERR_HIP;
 }
 
@@ -738,23 +831,38 @@ static int run_bin_t(BinLaunch& L) {
   // bins): LDS copies of the sums only, mode counts from k_bin_counts
   const bool fast = L.row_form && !L.dev.logbins && !L.dev.out2d;
   const bool mirror = L.row_form && L.mirror && !L.dev.out2d;
-  const bool sums = fast || mirror;
+  // (k, mu) bins in the mirror form: the 1-D pass also sums the per-k bounds
+  const bool mirror2d = L.row_form && L.mirror2d;
+  const bool sums = fast || mirror || mirror2d;
   const int nl = L.dev.odd ? 5 : 3;
   const int F = L.dev.nspec * nl;
-  const size_t lcp = sums ? (size_t)F * L.dev.nk : (size_t)per_copy;
+  const size_t lcp = sums ? (size_t)F * L.dev.nk + (mirror2d ? L.dev.nk : 0) : (size_t)per_copy;
   const size_t lds = lcp * L.dev.copies * sizeof(double) + (size_t)L.dev.nzh * 4 * sizeof(double);
   if (sums) {
     const int rc = run_bin_counts(L);
     if (rc) return rc;
   }
+  if (mirror2d) {
+    const int n2d = L.dev.nk * L.dev.nmu;
+    const size_t lds2 = (size_t)L.dev.nspec * n2d * 8 + (size_t)L.dev.nzh * 4 * sizeof(double);
+    if (lds > 160 * 1024 || lds2 > 160 * 1024) return CAPOW_ERR_UNSUPPORTED;
+    launch_mirror_any<T, 1>(L, lds);
+    int rc = run_bin_final(L, true);
+    if (rc) return rc;
+    // the bounds sit where the (k, mu) sums go
+    double* sums2d = L.out + (size_t)F * L.dev.nk;
+    k_bin_scale<<<(L.dev.nk + 255) / 256, 256, 0, L.stream>>>(sums2d, L.dev.nk, L.qscale);
+    launch_mirror_any<T, 2>(L, lds2);
+    const int len = L.dev.nspec * n2d;
+    TIMED(L.timer, "bin_final",
+          (k_bin_final2d<<<(len + 255) / 256, 256, 0, L.stream>>>(L.partials2, L.nwg, len, n2d,
+                                                                 L.dev.nmu, L.qscale, sums2d)));
+    return hipGetLastError() == hipSuccess ? CAPOW_OK : CAPOW_ERR_HIP;
+  }
   if (mirror) {
     // the host sizes the copies of the sums to fit beside the z tables
     if (lds > 160 * 1024) return CAPOW_ERR_UNSUPPORTED;
-    const bool il = L.dev.interlace != 0, hb = L.dev.hasB != 0;
-    if (il && hb) launch_mirror<T, true, true>(L, lds);
-    else if (il) launch_mirror<T, true, false>(L, lds);
-    else if (hb) launch_mirror<T, false, true>(L, lds);
-    else launch_mirror<T, false, false>(L, lds);
+    launch_mirror_any<T, 0>(L, lds);
   } else if (L.row_form && lds <= 160 * 1024) {
     // the largest instantiated full-slot count not above nzh / 64
     if (nuf >= 16 && sizeof(T) == 4) launch_rows<T, 16>(L, lds);

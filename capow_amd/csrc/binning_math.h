@@ -136,6 +136,9 @@ struct ModeVals {
   bool ok, near;
   double np[3];
   double L2, L4, mu;
+  double fktot;   // 2 pi |k| (group form)
+  double bound;   // group form: NM cor (|A|^2 + |B|^2) summed over the rows,
+                  // >= |np[s]| for every spectrum
 };
 
 // Linear k bin of a mode by multiplication with 1/binwidth; `near` flags a
@@ -189,6 +192,20 @@ __device__ __forceinline__ KBin kbin_log(double fxy, double fz, const BinArgsDev
 // (near-integer quotients flagged for kbin_exact), log bins exactly.
 __device__ __forceinline__ KBin kbin_any(double fxy, double fz, const BinArgsDev& a) {
   return a.logbins ? kbin_log(fxy, fz, a) : kbin_fast(fxy, fz, a);
+}
+
+// The (k, mu) bin of a binned mode (Output2D, src/main.c:453-461): mu by the
+// exact division (it feeds the mu bin), mu bin (int)((mu + 1)/2 * NMU) capped
+// at NMU - 1, as mode_geom computes it.
+__device__ __forceinline__ int mu_bin2d(int kbin, double fl, double fktot, const BinArgsDev& a,
+                                        bool& in2d) {
+  double mu = 0.0;
+  if (fktot > 0) mu = ddiv(dmul(2.0 * 3.14159265358979323846, fl), fktot);
+  int mubin = (int)(((mu + 1.0) / 2.0) * (double)(float)a.nmu);
+  if (mubin >= a.nmu) mubin = a.nmu - 1;
+  const int b2 = kbin * a.nmu + mubin;
+  in2d = (b2 >= 0) && (b2 < a.nk * a.nmu);
+  return b2;
 }
 
 // The reference's Hermitian weight NM (src/main.c:422-429): 2, except 1 in
@@ -295,6 +312,7 @@ __device__ __forceinline__ ModeVals group_mode(const Group& G, int k, double fz,
   const double twopi = 2.0 * 3.14159265358979323846;
   const KBin kb = kbin_any(G.fxy, fz, a);
   const double fktot = kb.fktot;
+  V.fktot = fktot;
   V.near = kb.near;
   V.kbin = kb.kbin;
   V.ok = kb.ok;
@@ -339,6 +357,8 @@ __device__ __forceinline__ ModeVals group_mode(const Group& G, int k, double fz,
     }
     V.np[s] = wc * p;
   }
+  // |Im or Re(A* B)| <= |A||B| <= (|A|^2 + |B|^2) / 2 per row
+  V.bound = wc * (saa + sbb);
   const double mu2 = mu * mu;
   V.mu = mu;
   V.L2 = 1.5 * mu2 - 0.5;

@@ -68,6 +68,9 @@ struct capow_ctx {
   double* bin_partials = nullptr;
   double* bin_out = nullptr;
   double* bin_counts = nullptr;     // mode counts of the branch-free binning
+  bool bin_mirror2d = false;        // (k, mu) bins in the mirror form
+  long long* bin_partials2 = nullptr;   // their int64 workgroup tables
+  double* bin_qscale = nullptr;     // their per-k quanta
   bool bin_counts_valid = false;
   int bin_nwg = 1024;
   int bin_per_copy = 0;
@@ -438,8 +441,15 @@ static int setup(capow_ctx* c) {
     c->bin_row_form = bytes + ztab <= lds_max;
     // the branch-free row form (linear bins) and the mirror-group form
     // (linear or log bins) keep only the sums in LDS
+    // (k, mu) bins in the mirror form (line of sight along z, where the four
+    // mirror modes share mu): a 1-D pass with per-k bounds, then one int64
+    // (k, mu) table per workgroup
+    c->bin_mirror2d = c->bin_row_form && c->bin_mirror && b.out2d && p.los != 1 && p.los != 2 &&
+                      (size_t)ns * n2d * 8 + ztab <= lds_max &&
+                      (size_t)(ns * nl + 1) * p.nk * 8 + ztab <= lds_max;
     const bool fast = c->bin_row_form && !b.out2d && (!b.logbins || c->bin_mirror);
-    const size_t lbytes = fast ? (size_t)ns * nl * p.nk * 8 : bytes;
+    const size_t lbytes = c->bin_mirror2d ? (size_t)(ns * nl + 1) * p.nk * 8
+                                          : (fast ? (size_t)ns * nl * p.nk * 8 : bytes);
     int copies = (int)((lds_max - (c->bin_row_form ? ztab : 0)) / lbytes);
     b.copies = copies > 8 ? 8 : copies;
     int dev = 0, ncu = 256;
@@ -455,7 +465,11 @@ static int setup(capow_ctx* c) {
     c->bin_nwg = (int)(want < ncu ? want : ncu);
     HIPCHK(c, hipMalloc((void**)&c->bin_partials, (size_t)c->bin_nwg * bytes));
     HIPCHK(c, hipMalloc((void**)&c->bin_out, bytes));
-    HIPCHK(c, hipMalloc((void**)&c->bin_counts, (size_t)p.nk * 8));
+    HIPCHK(c, hipMalloc((void**)&c->bin_counts, (size_t)(p.nk + n2d) * 8));
+    if (c->bin_mirror2d) {
+      HIPCHK(c, hipMalloc((void**)&c->bin_partials2, (size_t)c->bin_nwg * ns * n2d * 8));
+      HIPCHK(c, hipMalloc((void**)&c->bin_qscale, (size_t)2 * p.nk * 8));
+    }
     c->bin_counts_valid = false;
     HIPCHK(c, hipHostMalloc((void**)&c->h_bins, bytes, hipHostMallocDefault));
   }
@@ -468,7 +482,7 @@ static void release(capow_ctx* c) {
   void* ps[] = {c->own_parts, c->own_w, c->sorted, c->sorted_w, c->counts, c->offsets,
                 c->cursor, c->bsum, c->boff, c->partials, c->gmax, c->errs, c->stats,
                 c->tw[0], c->tw[1], c->tw[2], c->bin_tabs, c->bin_partials, c->bin_out,
-                c->bin_counts};
+                c->bin_counts, c->bin_partials2, c->bin_qscale};
   for (void* q : ps) if (q) hipFree(q);
   void* bs[] = {c->bwd_G, c->bwd_g0, c->bwd_out, c->bwd_w};
   for (void* q : bs) if (q) hipFree(q);
@@ -732,6 +746,9 @@ static void bin_launch(capow_ctx* c, BinLaunch& L) {
   L.tabs = c->bin_tabs;
   L.row_form = c->bin_row_form;
   L.mirror = c->bin_mirror;
+  L.mirror2d = c->bin_mirror2d;
+  L.partials2 = c->bin_partials2;
+  L.qscale = c->bin_qscale;
   L.nwg = c->bin_nwg;
   L.per_copy = c->bin_per_copy;
   L.partials = c->bin_partials;
@@ -890,19 +907,7 @@ extern "C" int capow_backward(capow_ctx* c, const double* dL_draw, void* grad_xy
   if (!c->bwd_g0) HIPCHK(c, hipMalloc((void**)&c->bwd_g0, 8 * sizeof(double)));
   HIPCHK(c, hipMemcpyAsync(c->bwd_G, dL_draw, gbytes, hipMemcpyHostToDevice, c->stream));
   BinLaunch L;
-  L.stream = c->stream;
-  L.timer = &c->timer;
-  for (int s = 0; s < 4; s++) L.grid[s] = c->grid[s] ? c->grid[s] : c->grid[0];
-  L.dev = c->bin;
-  L.tabs = c->bin_tabs;
-  L.row_form = c->bin_row_form;
-  L.mirror = c->bin_mirror;
-  L.nwg = c->bin_nwg;
-  L.per_copy = c->bin_per_copy;
-  L.partials = c->bin_partials;
-  L.out = c->bin_out;
-  L.counts = c->bin_counts;
-  L.counts_valid = &c->bin_counts_valid;
+  bin_launch(c, L);
   rc = run_bin_backward(L, c->bwd_G, c->bwd_g0, c->fp64);
   if (rc) return fail(c, rc, std::string("binning backward failed: ") + hipGetErrorString(hipGetLastError()));
   FftArgs f;

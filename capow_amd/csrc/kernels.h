@@ -163,6 +163,9 @@ struct BinLaunch {
   const double* tabs;     // per-axis tables (offsets in dev)
   bool row_form;          // row-walking kernel (z tables in LDS)
   bool mirror;            // mirror-group form (symmetric tables, even NX, NY)
+  bool mirror2d;          // (k, mu) bins in the mirror form (line of sight z)
+  long long* partials2;   // nwg * nspec * NK * NMU int64 (k, mu) tables
+  double* qscale;         // 2 NK: per-k quanta and their inverses
   int nwg;
   int per_copy;
   double* partials;       // nwg * per_copy

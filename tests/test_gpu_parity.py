@@ -387,6 +387,12 @@ MIRROR_CASES = [
     ("fp32_log_two_per_lane", api.FP32, (32, 32, 256),
      dict(interp_order=3, do_interlacing=1, momentum=0, output_log=1, mink=-2.5, maxk=-0.9,
           nk=20)),
+    # (k, mu) bins with the line of sight along z: per-k bounds, int64 quanta
+    ("fp64_2d_los_z", api.FP64, (64, 48, 64),
+     dict(interp_order=3, do_interlacing=1, momentum=1, extra_spectra=3, output_2d=1, nmu=7,
+          odd_multipoles=1)),
+    ("fp32_2d_two_per_lane", api.FP32, (32, 32, 256),
+     dict(interp_order=2, do_interlacing=1, momentum=0, output_2d=1, nmu=10)),
 ]
 
 
@@ -446,6 +452,15 @@ def test_mirror_group_binning(case, oracle, monkeypatch):
         for l in ls:
             d = np.abs(a[code]["pk"][l] - b[code]["pk"][l])
             assert np.all(d <= 1e-12 * (scale + np.abs(b[code]["pk"][0])) + 1e-300), (name, code, l)
+    if p.output_2d:
+        for form, r in res.items():
+            assert np.array_equal(r["nmodes2d"].ravel(), ob.nmodes2d), (name, form)
+            d = np.abs(r[p.momentum]["pk2d"].ravel() - ob.pk2d)
+            assert d.max() <= 1e-12 * np.abs(ob.pk2d).max() * 10, (name, form)
+        for code in a["codes"]:
+            ref2 = b[code]["pk2d"]
+            d = np.abs(a[code]["pk2d"] - ref2)
+            assert d.max() <= 1e-12 * np.abs(ref2).max() * 10, (name, code)
 
 
 @pytest.mark.parametrize("prec", [api.FP64, api.FP32])

@@ -592,3 +592,245 @@ int orc_run(const orc_params *p, const double *pos, const double *vel,
   for (int q = 0; q < 4; q++) free(g[q]);
   return rc;
 }
+
+/* ---------------------------------------------------------------------------
+ * Survey (Yamamoto) multipole estimator: reference src/main.c:472-776
+ * (compute_survey_power) and src/main.c:781-967 (assign_survey_power), at
+ * NTask = 1.  The grids hold the real-space fields before any transform; they
+ * are not modified.  Work copies are weighted by products of the unit vector
+ * r^ = (cell position - origin) / |.| and transformed; the monopole pass keeps
+ * its k-space field (F0, F0_mom) for the cross products of the later passes.
+ * --------------------------------------------------------------------------- */
+
+/* One weighted copy of a real-space grid: dst = src * (product of the r^
+ * components in idx[0..n-1]) on the real cells, as src/main.c:551-560
+ * (dipole), 603-612 (quadrupole), 663-672 (octopole) and 734-743
+ * (hexadecapole, whose first index appears twice). */
+static void survey_weight(const orc_params *p, const double *origin, const double *src,
+                          double *dst, int n, const int *idx) {
+  const unsigned long long NX = (unsigned long long)p->nx, NY = (unsigned long long)p->ny,
+                           NZ = (unsigned long long)p->nz;
+  const double dx = dxof(p), dy = dyof(p), dz = dzof(p);
+  for (unsigned long long ix = 0; ix < NX; ix++)
+    for (unsigned long long iy = 0; iy < NY; iy++)
+      for (unsigned long long iz = 0; iz < NZ; iz++) {
+        const unsigned long long ind = iz + ROWLEN(p) * (iy + NY * ix);
+        if (n == 0) {
+          dst[ind] = src[ind];
+          continue;
+        }
+        double vec[3] = {(double)ix * dx + p->xmin - origin[0], (double)iy * dy + p->ymin - origin[1],
+                         (double)iz * dz + p->zmin - origin[2]};
+        double r2 = vec[0] * vec[0] + vec[1] * vec[1] + vec[2] * vec[2];
+        if (r2 == 0.0) r2 = 1.0;
+        double v;
+        if (n == 1) v = src[ind] * vec[idx[0]] / sqrt(r2);
+        else if (n == 2) v = src[ind] * vec[idx[0]] * vec[idx[1]] / r2;
+        else if (n == 3) v = src[ind] * vec[idx[0]] * vec[idx[1]] * vec[idx[2]] / (r2 * sqrt(r2));
+        else v = src[ind] * vec[idx[0]] * vec[idx[0]] * vec[idx[1]] * vec[idx[2]] / (r2 * r2);
+        dst[ind] = v;
+      }
+}
+
+/* assign_survey_power (src/main.c:781-967) for one weighted, transformed
+ * field.  w[0..3] = ddg_2, ddg_interlace_2, ddg_mom_2, ddg_mom_interlace_2. */
+static void survey_assign(const orc_params *p, int multipole, int ii, int jj, int kk, double **w,
+                          double *F0, double *F0m, orc_bins *b) {
+  const unsigned long long NX = (unsigned long long)p->nx, NY = (unsigned long long)p->ny,
+                           NZ = (unsigned long long)p->nz;
+  const double dx = dxof(p), dy = dyof(p), dz = dzof(p);
+  const int M = p->momentum, NK = p->nk;
+  const int mom = (M != 0) && (M != 1);
+  const double binwidth = (p->maxk - p->mink) / (double)NK;
+  double min_nyquist = M_PI / dx;
+  if (M_PI / dy < min_nyquist) min_nyquist = M_PI / dy;
+  if (M_PI / dz < min_nyquist) min_nyquist = M_PI / dz;
+  const double sx = 1.0 / ((float)NX * dx), sy = 1.0 / ((float)NY * dy), sz = 1.0 / ((float)NZ * dz);
+  for (unsigned long long i = 0; i < NX; i++)
+    for (unsigned long long j = 0; j < NY; j++)
+      for (unsigned long long k = 0; k <= NZ / 2; k++) {
+        const double fx = i <= NX / 2 ? (float)i * sx : ((float)i - (float)NX) * sx;
+        const double fy = j <= NY / 2 ? (float)j * sy : ((float)j - (float)NY) * sy;
+        const double fz = (float)k * sz;
+        const double fktot = 2.0 * M_PI * sqrt(fx * fx + fy * fy + fz * fz);
+        int kbin;
+        if (p->output_log) {
+          const double lk = log10(fktot);
+          if (!(lk > -1e308)) continue;   /* (int) of -inf: rejected */
+          kbin = (int)((float)(lk - p->mink) / binwidth);
+        } else {
+          kbin = (int)((float)(fktot - p->mink) / binwidth);
+        }
+        if (!((kbin >= 0) && (kbin < NK) && (fktot <= min_nyquist))) continue;
+        double sinc_x = 1.0, sinc_y = 1.0, sinc_z = 1.0;
+        const double ax = M_PI * fx * dx, ay = M_PI * fy * dy, az = M_PI * fz * dz;
+        if (fx != 0.0) sinc_x = sin(ax) / ax;
+        if (fy != 0.0) sinc_y = sin(ay) / ay;
+        if (fz != 0.0) sinc_z = sin(az) / az;
+        double grid_cor = pow(1.0 / (sinc_x * sinc_y * sinc_z), 2 * p->interp_order);
+        const double kvec[3] = {2. * M_PI * fx, 2. * M_PI * fy, 2. * M_PI * fz};
+        const unsigned long long ind = 2 * k + ROWLEN(p) * (j + NY * i);
+        double dkr = w[0][ind], dki = w[0][ind + 1];
+        const double kh = ax + ay + az, sink = sin(kh), cosk = cos(kh);
+        if (p->do_interlacing) {
+          grid_cor *= 0.25;
+          dkr += w[1][ind] * cosk - w[1][ind + 1] * sink;
+          dki += w[1][ind] * sink + w[1][ind + 1] * cosk;
+        }
+        double dkr_mom = 0.0, dki_mom = 0.0;
+        if (mom) {
+          dkr_mom = w[2][ind];
+          dki_mom = w[2][ind + 1];
+          if (p->do_interlacing) {
+            dkr_mom += w[3][ind] * cosk - w[3][ind + 1] * sink;
+            dki_mom += w[3][ind] * sink + w[3][ind + 1] * cosk;
+          }
+        }
+        double power;
+        if (!mom) {
+          if (multipole == 0) {
+            F0[ind] = dkr;
+            F0[ind + 1] = dki;
+            power = (dkr * dkr + dki * dki) * grid_cor;
+          } else if (multipole % 2) {
+            power = (dkr * F0[ind + 1] - dki * F0[ind]) * grid_cor;
+          } else {
+            power = (dkr * F0[ind] + dki * F0[ind + 1]) * grid_cor;
+          }
+        } else {
+          if (multipole == 0) {
+            F0[ind] = dkr;
+            F0[ind + 1] = dki;
+            F0m[ind] = dkr_mom;
+            F0m[ind + 1] = dki_mom;
+            power = (dkr * dki_mom - dki * dkr_mom) * grid_cor;
+          } else if (multipole % 2) {
+            power = 0.5 * (dkr * F0m[ind] + dki * F0m[ind + 1] + dkr_mom * F0[ind] +
+                           dki_mom * F0[ind + 1]) * grid_cor;
+          } else {
+            power = 0.5 * (dkr * F0m[ind + 1] - dki * F0m[ind] + dkr_mom * F0[ind + 1] -
+                           dki_mom * F0[ind]) * grid_cor;
+          }
+        }
+        int NM = 2;
+        if (k == 0) {
+          if (j == 0) {
+            if ((i != 0) && (i != NX / 2)) NM = 1;
+          } else {
+            if (i != NX / 2) NM = 1;
+          }
+        }
+        power *= NM;
+        if (multipole == 0) {
+          b->pk0[kbin] += power;
+          b->pk2[kbin] -= 0.5 * power;
+          b->pk4[kbin] += 0.375 * power;
+          b->nmodes[kbin] += NM;
+        } else if (!(fktot > 0.0)) {
+          continue;
+        } else if (multipole == 1) {
+          power *= kvec[ii] / fktot;
+          b->pk1[kbin] += power;
+          b->pk3[kbin] -= 1.5 * power;
+        } else if (multipole == 2) {
+          const double pre = ii != jj ? 2.0 : 1.0;
+          power *= pre * kvec[ii] * kvec[jj] / (fktot * fktot);
+          b->pk2[kbin] += 1.5 * power;
+          b->pk4[kbin] -= 3.75 * power;
+        } else if (multipole == 3) {
+          const double pre = ii != kk ? (ii != jj ? 6.0 : 3.0) : 1.0;
+          b->pk3[kbin] += 2.5 * pre * kvec[ii] * kvec[jj] * kvec[kk] / (fktot * fktot * fktot) * power;
+        } else {
+          const double pre = ii == jj ? (ii != kk ? 4.0 : 1.0) : (jj == kk ? 6.0 : 12.0);
+          b->pk4[kbin] += 4.375 * pre * kvec[ii] * kvec[ii] * kvec[jj] * kvec[kk] /
+                          (fktot * fktot * fktot * fktot) * power;
+        }
+      }
+}
+
+/* The index sets of the reference's loops: quadrupole (ii <= jj),
+ * octopole (src/main.c:649-657) and hexadecapole (src/main.c:709-726). */
+int orc_survey_combos(int multipole, int *idx) {
+  int n = 0;
+  if (multipole == 0) return 1;
+  if (multipole == 1) {
+    for (int ii = 0; ii < 3; ii++) idx[3 * n++] = ii;
+    return n;
+  }
+  if (multipole == 2) {
+    for (int ii = 0; ii < 3; ii++)
+      for (int jj = ii; jj < 3; jj++) {
+        idx[3 * n] = ii;
+        idx[3 * n + 1] = jj;
+        n++;
+      }
+    return n;
+  }
+  if (multipole == 3) {
+    for (int ii = 0; ii < 3; ii++) {
+      const int jjhigh = ii == 0 ? ii + 2 : ii + 1;
+      for (int jj = ii; jj < jjhigh; jj++)
+        for (int kk = jj != ii ? 2 : 0; kk < 3; kk++) {
+          idx[3 * n] = ii;
+          idx[3 * n + 1] = jj;
+          idx[3 * n + 2] = kk;
+          n++;
+        }
+    }
+    return n;
+  }
+  for (int ii = 0; ii < 3; ii++)
+    for (int jj = 0; jj < 3; jj++) {
+      int kklow = 0, kkhigh = 3;
+      if (jj != ii) {
+        if (ii == 1) {
+          kklow = 2;
+        } else if (ii == 2) {
+          if (jj == 1) continue;
+          kklow = 1;
+          kkhigh = 2;
+        } else {
+          kklow = jj;
+        }
+      }
+      for (int kk = kklow; kk < kkhigh; kk++) {
+        idx[3 * n] = ii;
+        idx[3 * n + 1] = jj;
+        idx[3 * n + 2] = kk;
+        n++;
+      }
+    }
+  return n;
+}
+
+int orc_survey_power(const orc_params *p, const double *origin, double **grids, orc_bins *b) {
+  const int M = p->momentum;
+  if (M == 2 || M == 5) return -1;   /* no ddg_mom grid (src/main.c:1198) */
+  const int mom = (M != 0) && (M != 1);
+  const int il = p->do_interlacing != 0;
+  const long long sz = orc_grid_size(p);
+  double *w[4] = {NULL, NULL, NULL, NULL};
+  const int used[4] = {1, il, mom, mom && il};
+  for (int s = 0; s < 4; s++)
+    if (used[s]) w[s] = (double *)calloc((size_t)sz, sizeof(double));
+  double *F0 = (double *)calloc((size_t)sz, sizeof(double));
+  double *F0m = mom ? (double *)calloc((size_t)sz, sizeof(double)) : NULL;
+  const int step = p->odd_multipoles ? 1 : 2;
+  int idx[3 * 15];
+  for (int m = 0; m <= 4; m += step) {
+    const int nc = orc_survey_combos(m, idx);
+    for (int c = 0; c < nc; c++) {
+      const int *ix = idx + 3 * c;
+      for (int s = 0; s < 4; s++)
+        if (used[s]) {
+          survey_weight(p, origin, grids[s], w[s], m, ix);
+          orc_fft_r2c_3d(w[s], p->nx, p->ny, p->nz);
+        }
+      survey_assign(p, m, ix[0], ix[1], ix[2], w, F0, F0m, b);
+    }
+  }
+  for (int s = 0; s < 4; s++) free(w[s]);
+  free(F0);
+  free(F0m);
+  return 0;
+}

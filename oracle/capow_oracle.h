@@ -25,6 +25,11 @@
  *                          (tests/test_oracle_power.py)
  *   orc_output_power       byte comparison with hand-built %g rows
  *                          (tests/test_oracle_output.py)
+ *   orc_survey_power       closed forms for one and two point sources at
+ *                          generic directions (sums of NM cor L_l(k.n)
+ *                          and of their sin/cos cross terms), the combos'
+ *                          multiplicities against the multinomial
+ *                          coefficients (tests/test_oracle_survey.py)
  */
 #ifndef CAPOW_ORACLE_H
 #define CAPOW_ORACLE_H
@@ -111,6 +116,19 @@ int orc_finalize(const orc_params *p, double shot, double norm, orc_bins *b);
  * writes %.17g instead of %g (a parity side dump).  Returns 0 or -1. */
 int orc_write_output(const orc_params *p, double shot, double norm,
                      const orc_bins *b, const char *path, int precise);
+
+/* Survey (Yamamoto) estimator, reference src/main.c:472-967 at NTask = 1:
+ * grids[0..3] are the real-space fields (ddg, ddg_interlace, ddg_mom,
+ * ddg_mom_interlace; unused entries may be NULL), untouched; origin =
+ * X_Origin, Y_Origin, Z_Origin.  Accumulates Pk0..Pk4 (pk1/pk3 when
+ * Odd_Multipoles) and Nmodes into b (zeroed by the caller).  Returns 0, or -1
+ * for Momentum 2 or 5, where the reference has no ddg_mom grid. */
+int orc_survey_power(const orc_params *p, const double *origin, double **grids, orc_bins *b);
+
+/* The (ii, jj, kk) index triples of the reference's multipole loops
+ * (src/main.c:487-776) for one multipole, in its order; returns their count
+ * (1, 3, 6, 10, 15). */
+int orc_survey_combos(int multipole, int *idx);
 
 /* Whole periodic pipeline (reference src/main.c:28-298) on arrays. */
 int orc_run(const orc_params *p, const double *pos, const double *vel,

@@ -73,6 +73,10 @@ def lib():
         L.orc_run.argtypes = [P, _D, _D, _D, C.c_longlong, C.POINTER(OrcBins), C.POINTER(OrcStats),
                               _D, _D, _D]
         L.orc_run.restype = C.c_int
+        L.orc_survey_power.argtypes = [P, _D, C.POINTER(_D), C.POINTER(OrcBins)]
+        L.orc_survey_power.restype = C.c_int
+        L.orc_survey_combos.argtypes = [C.c_int, C.POINTER(C.c_int)]
+        L.orc_survey_combos.restype = C.c_int
         _lib = L
     return _lib
 
@@ -168,6 +172,22 @@ def compute_power(p, grids):
     b = Bins(p)
     lib().orc_compute_power(C.byref(p), _grid_ptrs(grids), C.byref(b.c))
     return b
+
+
+def survey_power(p, origin, grids):
+    """Survey (Yamamoto) estimator on real-space grids (not modified)."""
+    b = Bins(p)
+    o = np.ascontiguousarray(origin, dtype=np.float64)
+    rc = lib().orc_survey_power(C.byref(p), _dp(o), _grid_ptrs(grids), C.byref(b.c))
+    if rc != 0:
+        raise ValueError("survey estimator: Momentum %d has no ddg_mom grid" % p.momentum)
+    return b
+
+
+def survey_combos(multipole):
+    idx = (C.c_int * 45)()
+    n = lib().orc_survey_combos(multipole, idx)
+    return [tuple(idx[3 * c:3 * c + 3]) for c in range(n)]
 
 
 def finalize(p, shot, norm, b):

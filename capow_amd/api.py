@@ -96,6 +96,7 @@ SIGNATURES = {
     "capow_set_grid": ([_V, C.c_int32, _V, C.c_int64], C.c_int),
     "capow_mark_assigned": ([_V, C.POINTER(AssignStats)], C.c_int),
     "capow_mark_transformed": ([_V], C.c_int),
+    "capow_survey_power": ([_V, _D, _D, _I64], C.c_int),
     "capow_kernel_times": ([_V, C.c_char_p, C.c_int32, _D, C.POINTER(C.c_int32), C.c_int32,
                             C.POINTER(C.c_int32)], C.c_int),
     "capow_reset_kernel_times": ([_V], C.c_int),
@@ -318,6 +319,16 @@ class PowerSpectrum:
 
     def mark_transformed(self):
         self.check(lib().capow_mark_transformed(self.ctx))
+
+    def survey_power(self, origin):
+        """Survey (Yamamoto) estimator on the real-space grids (capow_survey_power):
+        {'pk': (5, nk) raw sums P0..P4, 'nmodes': (nk,)}."""
+        o = np.ascontiguousarray(origin, dtype=np.float64)
+        nk = self.p.nk
+        pk = np.zeros(5 * nk)
+        nm = np.zeros(nk, dtype=np.int64)
+        self.check(lib().capow_survey_power(self.ctx, _dptr(o), _dptr(pk), _iptr(nm)))
+        return {"pk": pk.reshape(5, nk), "nmodes": nm}
 
     def synchronize(self):
         self.check(lib().capow_synchronize(self.ctx))

@@ -25,7 +25,8 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result", "-Wno-unused-value",
 ]
-HIP_SOURCES = ["capow_api.hip", "deposit.hip", "fft.hip", "binning.hip", "adjoint.hip"]
+HIP_SOURCES = ["capow_api.hip", "deposit.hip", "fft.hip", "binning.hip", "adjoint.hip",
+               "survey.hip"]
 C_SOURCES = ["capow_host.c", "capow_ascii.c"]
 
 

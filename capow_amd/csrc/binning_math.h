@@ -64,6 +64,7 @@ struct ModeGeom {
   bool ok;       // inside the k range and below the Nyquist frequency
   int kbin;
   double mu, NM, cor, cr, ci;
+  double fktot;  // 2 pi |k|
   int b2;        // 2-D bin
   bool in2d;
 };
@@ -107,6 +108,7 @@ __device__ __forceinline__ ModeGeom mode_geom(const RowVals& R, int k, double fz
   }
   G.ok = true;
   G.kbin = kbin;
+  G.fktot = fktot;
   G.mu = mu;
   G.NM = NM;
   G.cor = R.corxy * corz;  // x0.25 folded in corz

@@ -94,6 +94,15 @@ struct capow_ctx {
   void* bwd_w = nullptr;
   size_t bwd_w_cap = 0;
   int fft_col_mul = 2;   // column batch multiplier of the y/x FFT passes (CAPOW_FFT_COL_MUL=1 halves it)
+  // survey estimator: copies of the real-space fields, the monopole k-space
+  // fields, bin partials and sums (allocated on first use)
+  void* sv_src[4] = {nullptr, nullptr, nullptr, nullptr};
+  void* sv_F0 = nullptr;
+  void* sv_F0m = nullptr;
+  double* sv_partials = nullptr;
+  double* sv_acc = nullptr;
+  double* h_sv = nullptr;
+  int sv_nwg = 0;
 };
 
 static std::string g_create_err;
@@ -486,8 +495,11 @@ static void release(capow_ctx* c) {
   for (void* q : ps) if (q) hipFree(q);
   void* bs[] = {c->bwd_G, c->bwd_g0, c->bwd_out, c->bwd_w};
   for (void* q : bs) if (q) hipFree(q);
-  void* hs[] = {c->h_stats, c->h_errs, c->h_bins};
+  void* hs[] = {c->h_stats, c->h_errs, c->h_bins, c->h_sv};
   for (void* q : hs) if (q) hipHostFree(q);
+  void* sv[] = {c->sv_src[0], c->sv_src[1], c->sv_src[2], c->sv_src[3], c->sv_F0, c->sv_F0m,
+                c->sv_partials, c->sv_acc};
+  for (void* q : sv) if (q) hipFree(q);
 }
 
 extern "C" int capow_create(const capow_params* p, capow_ctx** out) {
@@ -816,6 +828,135 @@ extern "C" int capow_power(capow_ctx* c, capow_assign_stats* st, capow_bins* out
   rc = capow_fft(c);
   if (rc) return rc;
   return capow_bin(c, out);
+}
+
+// The (ii, jj, kk) index sets of the survey multipole loops, in the
+// reference's order: src/main.c:540 (l = 1), 594-595 (l = 2, ii <= jj),
+// 649-657 (l = 3), 709-726 (l = 4).
+static int survey_combos(int l, int idx[][3]) {
+  int n = 0;
+  auto put = [&](int a, int b, int d) { idx[n][0] = a; idx[n][1] = b; idx[n][2] = d; n++; };
+  if (l == 0) {
+    put(0, 0, 0);
+  } else if (l == 1) {
+    for (int ii = 0; ii < 3; ii++) put(ii, 0, 0);
+  } else if (l == 2) {
+    for (int ii = 0; ii < 3; ii++)
+      for (int jj = ii; jj < 3; jj++) put(ii, jj, 0);
+  } else if (l == 3) {
+    for (int ii = 0; ii < 3; ii++)
+      for (int jj = ii; jj < (ii == 0 ? 2 : ii + 1); jj++)
+        for (int kk = (jj != ii ? 2 : 0); kk < 3; kk++) put(ii, jj, kk);
+  } else {
+    for (int ii = 0; ii < 3; ii++)
+      for (int jj = 0; jj < 3; jj++) {
+        int lo = 0, hi = 3;
+        if (jj != ii) {
+          if (ii == 0) lo = jj;
+          else if (ii == 1) lo = 2;
+          else if (jj == 1) continue;
+          else { lo = 1; hi = 2; }
+        }
+        for (int kk = lo; kk < hi; kk++) put(ii, jj, kk);
+      }
+  }
+  return n;
+}
+
+extern "C" int capow_survey_power(capow_ctx* c, const double origin[3], double* pk,
+                                  int64_t* nmodes) {
+  if (!c || !origin || !pk || !nmodes) return CAPOW_ERR_ARG;
+  const capow_params& p = c->p;
+  if (c->state < 2) return fail(c, CAPOW_ERR_STATE, "capow_survey_power before the grids are filled");
+  if (c->state >= 3) return fail(c, CAPOW_ERR_STATE, "grids already transformed");
+  if (p.momentum == 2 || p.momentum == 5)
+    return fail(c, CAPOW_ERR_UNSUPPORTED, "survey estimator: Momentum 2 and 5 have no ddg_mom grid");
+  const int NK = p.nk;
+  const int per_copy = 6 * NK;
+  if ((size_t)4 * per_copy * 8 > (size_t)160 * 1024)
+    return fail(c, CAPOW_ERR_UNSUPPORTED, "survey estimator: NK too large for the LDS bins");
+  const bool mom = p.momentum != 0 && p.momentum != 1;
+  const bool used[4] = {true, p.do_interlacing != 0, mom, mom && p.do_interlacing != 0};
+  const size_t rbytes = (size_t)p.nx * p.ny * c->g.rowlen * c->esz;
+  for (int s = 0; s < 4; s++)
+    if (used[s] && !c->sv_src[s]) HIPCHK(c, hipMalloc(&c->sv_src[s], rbytes));
+  if (!c->sv_F0) HIPCHK(c, hipMalloc(&c->sv_F0, rbytes));
+  if (mom && !c->sv_F0m) HIPCHK(c, hipMalloc(&c->sv_F0m, rbytes));
+  if (!c->sv_acc) {
+    const int64_t modes = (int64_t)p.nx * p.ny * c->g.rowc;
+    int64_t want = (modes + 255) / 256;
+    if (want > 2 * (int64_t)c->ncu) want = 2 * (int64_t)c->ncu;
+    c->sv_nwg = (int)(want < 1 ? 1 : want);
+    HIPCHK(c, hipMalloc((void**)&c->sv_partials, (size_t)c->sv_nwg * per_copy * 8));
+    HIPCHK(c, hipMalloc((void**)&c->sv_acc, (size_t)per_copy * 8));
+    HIPCHK(c, hipHostMalloc((void**)&c->h_sv, (size_t)per_copy * 8, hipHostMallocDefault));
+  }
+  for (int s = 0; s < 4; s++)
+    if (used[s])
+      HIPCHK(c, hipMemcpyAsync(c->sv_src[s], c->grid[s], rbytes, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->sv_acc, 0, (size_t)per_copy * 8, c->stream));
+  SurveyLaunch L;
+  L.stream = c->stream;
+  L.timer = &c->timer;
+  L.g = &c->g;
+  for (int s = 0; s < 4; s++) {
+    L.src[s] = used[s] ? c->sv_src[s] : nullptr;
+    L.work[s] = used[s] ? c->grid[s] : nullptr;
+  }
+  L.F0 = c->sv_F0;
+  L.F0m = mom ? c->sv_F0m : nullptr;
+  L.dev = c->bin;
+  L.dev.out2d = 0;
+  L.dev.nmu = 1;
+  L.tabs = c->bin_tabs;
+  for (int a = 0; a < 3; a++) L.origin[a] = origin[a];
+  L.mom = mom ? 1 : 0;
+  L.nwg = c->sv_nwg;
+  L.partials = c->sv_partials;
+  L.acc = c->sv_acc;
+  int idx[15][3];
+  for (int l = 0; l <= 4; l += p.odd_multipoles ? 1 : 2) {
+    const int nc = survey_combos(l, idx);
+    for (int q = 0; q < nc; q++) {
+      int rc = run_survey_weight(L, c->fp64, l, idx[q]);
+      if (rc) return fail(c, rc, std::string("survey weighting failed: ") + hipGetErrorString(hipGetLastError()));
+      // transform the weighted copies (the slots in use only)
+      FftArgs a;
+      fft_args(c, a);
+      a.ngrids = 0;
+      for (int s = 0; s < 4; s++)
+        if (used[s]) {
+          a.grid[a.ngrids] = c->grid[s];
+          a.scratch[a.ngrids] = c->scratch[s];
+          a.ngrids++;
+        }
+      if (!c->scratch[0]) a.scratch[0] = nullptr;
+      rc = run_fft(a, c->fp64);
+      if (rc) return fail(c, rc, std::string("fft launch failed: ") + hipGetErrorString(hipGetLastError()));
+      if (a.swapped)
+        for (int s = 0; s < 4; s++)
+          if (used[s]) {
+            void* t = c->grid[s];
+            c->grid[s] = c->scratch[s];
+            c->scratch[s] = t;
+            L.work[s] = c->grid[s];
+          }
+      rc = run_survey_bin(L, c->fp64, l, idx[q]);
+      if (rc) return fail(c, rc, std::string("survey binning failed: ") + hipGetErrorString(hipGetLastError()));
+    }
+  }
+  HIPCHK(c, hipMemcpyAsync(c->h_sv, c->sv_acc, (size_t)per_copy * 8, hipMemcpyDeviceToHost, c->stream));
+  // the grids hold the real-space fields again afterwards
+  for (int s = 0; s < 4; s++)
+    if (used[s])
+      HIPCHK(c, hipMemcpyAsync(c->grid[s], c->sv_src[s], rbytes, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->timer.collect();
+  for (int l = 0; l < 5; l++)
+    for (int k = 0; k < NK; k++)
+      pk[l * NK + k] = (l % 2 && !p.odd_multipoles) ? 0.0 : c->h_sv[l * NK + k];
+  for (int k = 0; k < NK; k++) nmodes[k] = (int64_t)c->h_sv[5 * NK + k];
+  return CAPOW_OK;
 }
 
 extern "C" int capow_grid_elems(const capow_ctx* c, int64_t* n) {

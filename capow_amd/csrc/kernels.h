@@ -135,6 +135,7 @@ struct FftArgs {
   int order = 0;          // bit per pass (z, y, x): batches with the outer index fastest
 };
 int run_fft(FftArgs& a, bool fp64);
+
 // adjoint of run_fft on gradient grids (register FFT sizes only)
 int run_ifft(FftArgs& a, bool fp64);
 
@@ -178,6 +179,29 @@ int run_bin(BinLaunch& L, bool fp64);
 // mode counts of the branch-free forms, and the final sum of the partials.
 int run_bin_counts(BinLaunch& L);
 int run_bin_final(BinLaunch& L, bool fast);
+
+// Survey (Yamamoto) estimator passes (survey.hip).
+struct SurveyLaunch {
+  hipStream_t stream;
+  Timer* timer;
+  const Geom* g;
+  const void* src[4];     // real-space fields A, A_int, B, B_int (null: unused)
+  void* work[4];          // weighted copies, transformed in place by run_fft
+  void* F0;               // monopole k-space fields kept for the cross terms
+  void* F0m;
+  BinArgsDev dev;         // geometry and k bins (2-D bins off)
+  const double* tabs;
+  double origin[3];
+  int mom;                // momentum cross spectrum (B grids)
+  int nwg;
+  double* partials;       // nwg * 6 * NK
+  double* acc;            // 6 * NK: P0..P4, mode counts
+};
+// work[s] = src[s] * (product of r^ components idx[0..n-1]; idx[0] twice
+// for n = 4) on the real cells
+int run_survey_weight(SurveyLaunch& L, bool fp64, int n, const int idx[3]);
+// one multipole pass over the transformed work grids, added into L.acc
+int run_survey_bin(SurveyLaunch& L, bool fp64, int multipole, const int idx[3]);
 
 // Backward of the binning: dL/draw (G: [n_spectra][5][NK]) -> gradients of
 // the k-space grids, in place; g0: the k = 0 gradients (8 doubles).

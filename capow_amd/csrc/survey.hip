@@ -1,0 +1,252 @@
+// survey.hip -- the survey (Yamamoto) multipole estimator for gfx950.
+//
+// Reference: compute_survey_power (src/main.c:472-776) and
+// assign_survey_power (src/main.c:781-967).  For each multipole l the field
+// is weighted in real space by the products of the unit vector
+// r^ = (cell - origin)/|cell - origin| that span degree l, transformed, and
+// each transform is paired with the monopole's k-space field times the
+// matching k^ product:
+//   l = 0: F0 = FFT(field); P0 += |F0|^2, P2 -= 1/2 of it, P4 += 3/8 of it
+//   l = 1: 3 fields r_i;           P1 += k_i/k Im-part, P3 -= 3/2 of it
+//   l = 2: 6 fields r_i r_j;       P2 += 3/2 m k_i k_j/k^2 Re-part, P4 -= 15/4
+//   l = 3: 10 fields r_i r_j r_k;  P3 += 5/2 m k_i k_j k_k/k^3 Im-part
+//   l = 4: 15 fields r_i^2 r_j r_k; P4 += 35/8 m k_i^2 k_j k_k/k^4 Re-part
+// (m: the multiplicity of the index combination).  With a B grid
+// (momentum cross spectra) the products are the symmetrised cross terms of
+// src/main.c:884-898.  The transforms are the periodic path's FFT; the
+// geometry (frequencies with the reference's float casts, k bins, window
+// deconvolution, interlace phase, Hermitian weights) is the periodic
+// binning's, so the mode counts agree with it bit for bit.
+//
+// Layout: every pass reads the real-space sources and writes the weighted
+// copies once (HBM-bound, 2 G N_g s bytes), the FFT runs its three passes,
+// and the binning reads the transformed copies plus F0 (and, for l = 0,
+// writes F0).  Per-wave FP64 bin copies in LDS, reduced in a fixed order,
+// keep the sums reproducible.
+#include "binning_math.h"
+
+namespace capow {
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_survey_weight(const T* __restrict__ s0,
+                                                       const T* __restrict__ s1,
+                                                       const T* __restrict__ s2,
+                                                       const T* __restrict__ s3,
+                                                       T* __restrict__ w0, T* __restrict__ w1,
+                                                       T* __restrict__ w2, T* __restrict__ w3,
+                                                       Geom g, double ox, double oy, double oz,
+                                                       int n, int i0, int i1, int i2) {
+  const int64_t row = blockIdx.x;                  // ix * NY + iy
+  const int ix = (int)(row / g.n[1]);
+  const int iy = (int)(row - (int64_t)ix * g.n[1]);
+  const int64_t base = row * g.rowlen;
+  const double vx = (double)ix * g.d[0] + g.lo[0] - ox;
+  const double vy = (double)iy * g.d[1] + g.lo[1] - oy;
+  for (int iz = threadIdx.x; iz < g.n[2]; iz += blockDim.x) {
+    const double vec[3] = {vx, vy, (double)iz * g.d[2] + g.lo[2] - oz};
+    double r2 = vec[0] * vec[0] + vec[1] * vec[1] + vec[2] * vec[2];
+    if (r2 == 0.0) r2 = 1.0;   // src/main.c:557
+    // the reference's operation order: field * vec... / r^l
+    double num = 1.0, den = 1.0;
+    if (n == 1) {
+      num = vec[i0];
+      den = sqrt(r2);
+    } else if (n == 2) {
+      num = vec[i0] * vec[i1];
+      den = r2;
+    } else if (n == 3) {
+      num = vec[i0] * vec[i1] * vec[i2];
+      den = r2 * sqrt(r2);
+    } else if (n == 4) {
+      num = vec[i0] * vec[i0] * vec[i1] * vec[i2];
+      den = r2 * r2;
+    }
+    const int64_t q = base + iz;
+    if (n == 0) {
+      w0[q] = s0[q];
+      if (s1) w1[q] = s1[q];
+      if (s2) w2[q] = s2[q];
+      if (s3) w3[q] = s3[q];
+    } else {
+      w0[q] = (T)((double)s0[q] * num / den);
+      if (s1) w1[q] = (T)((double)s1[q] * num / den);
+      if (s2) w2[q] = (T)((double)s2[q] * num / den);
+      if (s3) w3[q] = (T)((double)s3[q] * num / den);
+    }
+  }
+}
+
+// One multipole pass.  A wave owns one LDS copy of [P0..P4][NK] and the NK
+// mode counts; lanes walk the padded complex layout with a grid stride.
+template <typename T>
+__global__ __launch_bounds__(256) void k_survey_bin(const T* __restrict__ gA,
+                                                    const T* __restrict__ gAi,
+                                                    const T* __restrict__ gB,
+                                                    const T* __restrict__ gBi,
+                                                    T* __restrict__ F0, T* __restrict__ F0m,
+                                                    const double* __restrict__ tabs,
+                                                    BinArgsDev a, int mom, int multipole, int ii,
+                                                    int jj, int kk, double* __restrict__ partials) {
+  typedef typename CT<T>::t C;
+  extern __shared__ double sm[];
+  const int NK = a.nk;
+  const int per_copy = 6 * NK;
+  const int nwave = blockDim.x >> 6;
+  for (int q = threadIdx.x; q < per_copy * nwave; q += blockDim.x) sm[q] = 0.0;
+  __syncthreads();
+  double* acc = sm + (threadIdx.x >> 6) * per_copy;
+  const C* A = reinterpret_cast<const C*>(gA);
+  const C* Ai = reinterpret_cast<const C*>(gAi);
+  const C* B = reinterpret_cast<const C*>(gB);
+  const C* Bi = reinterpret_cast<const C*>(gBi);
+  C* f0 = reinterpret_cast<C*>(F0);
+  C* f0m = reinterpret_cast<C*>(F0m);
+  const uint32_t plane = (uint32_t)a.ny * (uint32_t)a.rowc;
+  const int64_t total = (int64_t)a.nx * plane;
+  const double twopi = 2.0 * 3.14159265358979323846;
+  for (int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; m < total;
+       m += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(m / plane);
+    const uint32_t r = (uint32_t)(m - (int64_t)i * plane);
+    const uint32_t j = a.fd_rowc.div(r);
+    const int k = (int)(r - j * (uint32_t)a.rowc);
+    if (k >= a.nzh) continue;   // row padding
+    const RowVals R = row_vals(i, (int)j, tabs, a);
+    const double fz = tabs[a.ofz + k];
+    const ModeGeom G = mode_geom(R, k, fz, tabs[a.ocz + k], tabs[a.opz + 2 * k],
+                                 tabs[a.opz + 2 * k + 1], a);
+    if (!G.ok) continue;
+    double dkr = (double)A[m].x, dki = (double)A[m].y;
+    if (a.interlace) {
+      const C v = Ai[m];
+      dkr += (double)v.x * G.cr - (double)v.y * G.ci;
+      dki += (double)v.x * G.ci + (double)v.y * G.cr;
+    }
+    double dkr_m = 0.0, dki_m = 0.0;
+    if (mom) {
+      dkr_m = (double)B[m].x;
+      dki_m = (double)B[m].y;
+      if (a.interlace) {
+        const C v = Bi[m];
+        dkr_m += (double)v.x * G.cr - (double)v.y * G.ci;
+        dki_m += (double)v.x * G.ci + (double)v.y * G.cr;
+      }
+    }
+    const double cor = G.cor;
+    double power;
+    if (multipole == 0) {
+      C o;
+      o.x = (T)dkr;
+      o.y = (T)dki;
+      f0[m] = o;
+      if (mom) {
+        o.x = (T)dkr_m;
+        o.y = (T)dki_m;
+        f0m[m] = o;
+        power = (dkr * dki_m - dki * dkr_m) * cor;
+      } else {
+        power = (dkr * dkr + dki * dki) * cor;
+      }
+    } else {
+      const C z = f0[m];
+      const double zr = (double)z.x, zi = (double)z.y;
+      if (!mom) {
+        power = (multipole & 1) ? (dkr * zi - dki * zr) * cor : (dkr * zr + dki * zi) * cor;
+      } else {
+        const C zm = f0m[m];
+        const double mr = (double)zm.x, mi = (double)zm.y;
+        power = (multipole & 1)
+                    ? 0.5 * (dkr * mr + dki * mi + dkr_m * zr + dki_m * zi) * cor
+                    : 0.5 * (dkr * mi - dki * mr + dkr_m * zi - dki_m * zr) * cor;
+      }
+    }
+    power *= G.NM;
+    const int kb = G.kbin;
+    if (multipole == 0) {
+      atomicAdd(&acc[kb], power);
+      atomicAdd(&acc[2 * NK + kb], -0.5 * power);
+      atomicAdd(&acc[4 * NK + kb], 0.375 * power);
+      atomicAdd(&acc[5 * NK + kb], G.NM);
+      continue;
+    }
+    const double fk = G.fktot;
+    if (!(fk > 0.0)) continue;
+    const double kv[3] = {twopi * R.fx, twopi * R.fy, twopi * fz};
+    if (multipole == 1) {
+      power *= kv[ii] / fk;
+      atomicAdd(&acc[NK + kb], power);
+      atomicAdd(&acc[3 * NK + kb], -1.5 * power);
+    } else if (multipole == 2) {
+      const double pre = ii != jj ? 2.0 : 1.0;
+      power *= pre * kv[ii] * kv[jj] / (fk * fk);
+      atomicAdd(&acc[2 * NK + kb], 1.5 * power);
+      atomicAdd(&acc[4 * NK + kb], -3.75 * power);
+    } else if (multipole == 3) {
+      const double pre = ii != kk ? (ii != jj ? 6.0 : 3.0) : 1.0;
+      atomicAdd(&acc[3 * NK + kb], 2.5 * pre * kv[ii] * kv[jj] * kv[kk] / (fk * fk * fk) * power);
+    } else {
+      const double pre = ii == jj ? (ii != kk ? 4.0 : 1.0) : (jj == kk ? 6.0 : 12.0);
+      atomicAdd(&acc[4 * NK + kb],
+                4.375 * pre * kv[ii] * kv[ii] * kv[jj] * kv[kk] / (fk * fk * fk * fk) * power);
+    }
+  }
+  __syncthreads();
+  double* out = partials + (int64_t)blockIdx.x * per_copy;
+  for (int q = threadIdx.x; q < per_copy; q += blockDim.x) {
+    double t = 0.0;
+    for (int c = 0; c < nwave; c++) t += sm[c * per_copy + q];
+    out[q] = t;
+  }
+}
+
+// acc += sum of the workgroup partials, in workgroup order.
+__global__ void k_survey_final(const double* __restrict__ partials, int nwg, int len,
+                               double* __restrict__ acc) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= len) return;
+  double t = 0.0;
+  for (int w = 0; w < nwg; w++) t += partials[(int64_t)w * len + q];
+  acc[q] += t;
+}
+
+template <typename T>
+static int weight_t(SurveyLaunch& L, int n, const int idx[3]) {
+  const Geom& g = *L.g;
+  const int64_t rows = (int64_t)g.n[0] * g.n[1];
+  TIMED(L.timer, "survey_weight",
+        (k_survey_weight<T><<<(unsigned)rows, 256, 0, L.stream>>>(
+            (const T*)L.src[0], (const T*)L.src[1], (const T*)L.src[2], (const T*)L.src[3],
+            (T*)L.work[0], (T*)L.work[1], (T*)L.work[2], (T*)L.work[3], g, L.origin[0],
+            L.origin[1], L.origin[2], n, idx[0], idx[1], idx[2])));
+  return hipGetLastError() == hipSuccess ? CAPOW_OK : CAPOW_ERR_HIP;
+}
+
+int run_survey_weight(SurveyLaunch& L, bool fp64, int n, const int idx[3]) {
+  return fp64 ? weight_t<double>(L, n, idx) : weight_t<float>(L, n, idx);
+}
+
+template <typename T>
+static int bin_t(SurveyLaunch& L, int multipole, const int idx[3]) {
+  const int NK = L.dev.nk;
+  const int per_copy = 6 * NK;
+  const size_t lds = (size_t)4 * per_copy * sizeof(double);
+  const void* fn = (const void*)k_survey_bin<T>;
+  if (lds > 64 * 1024) hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  const T* w1 = (const T*)(L.work[1] ? L.work[1] : L.work[0]);
+  const T* w2 = (const T*)(L.work[2] ? L.work[2] : L.work[0]);
+  const T* w3 = (const T*)(L.work[3] ? L.work[3] : L.work[0]);
+  TIMED(L.timer, "survey_bin",
+        (k_survey_bin<T><<<L.nwg, 256, lds, L.stream>>>(
+            (const T*)L.work[0], w1, w2, w3, (T*)L.F0, (T*)(L.F0m ? L.F0m : L.F0), L.tabs, L.dev,
+            L.mom, multipole, idx[0], idx[1], idx[2], L.partials)));
+  TIMED(L.timer, "survey_final",
+        (k_survey_final<<<(per_copy + 255) / 256, 256, 0, L.stream>>>(L.partials, L.nwg, per_copy,
+                                                                     L.acc)));
+  return hipGetLastError() == hipSuccess ? CAPOW_OK : CAPOW_ERR_HIP;
+}
+
+int run_survey_bin(SurveyLaunch& L, bool fp64, int multipole, const int idx[3]) {
+  return fp64 ? bin_t<double>(L, multipole, idx) : bin_t<float>(L, multipole, idx);
+}
+
+}  // namespace capow

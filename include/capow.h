@@ -183,6 +183,20 @@ int capow_bin(capow_ctx *ctx, capow_bins *out);
 /* capow_assign + capow_fft + capow_bin. */
 int capow_power(capow_ctx *ctx, capow_assign_stats *stats, capow_bins *out);
 
+/* Survey (Yamamoto) multipole estimator on the context's real-space grids
+ * (filled by capow_assign or capow_set_grid, not transformed); replaces
+ * compute_survey_power and assign_survey_power (src/main.c:472-967) for the
+ * grids the survey ingest deposits.  origin = X_Origin, Y_Origin, Z_Origin.
+ * For each multipole the grids are weighted by the r^ products of degree l,
+ * transformed and paired with the monopole's k-space field times the k^
+ * products.  pk (host, [5][nk]) receives the raw sums Pk0..Pk4 (Pk1, Pk3
+ * zero without Odd_Multipoles), nmodes (host, nk) the mode counts; Output2D
+ * bins stay empty, as in the reference's survey branch.  Momentum 2 and 5
+ * are CAPOW_ERR_UNSUPPORTED (the reference has no ddg_mom grid for them);
+ * Momentum 1 uses the density grids only (src/main.c:519).  The grids hold
+ * the same real-space fields afterwards.  Synchronises. */
+int capow_survey_power(capow_ctx *ctx, const double origin[3], double *pk, int64_t *nmodes);
+
 /* Number of spectra capow_bin produces and their reference Momentum codes. */
 int capow_spectra(const capow_params *p, int32_t *n_spectra, int32_t codes[3]);
 

@@ -172,6 +172,62 @@ def run_adjoint(args):
     return out
 
 
+def run_survey(args, parts, geom):
+    """Survey (Yamamoto) estimator: the workload's Zel'dovich catalogue
+    deposited on the 1024^3 grid (TSC + interlacing, density), observer at
+    (-L/2, -L/2, -L/2) outside the box, P_0..4 with odd multipoles: 35
+    weighted transforms per call (1 + 3 + 6 + 10 + 15), FP32 form."""
+    import torch
+    from capow_amd import api
+    n, L = geom["n"], geom["L"]
+    p = api.make_params(nx=n, ny=n, nz=n, xmax=L, ymax=L, zmax=L, interp_order=3,
+                        do_interlacing=1, momentum=0, nk=geom["nk"], maxk=geom["nk"] * geom["dk"],
+                        odd_multipoles=1, precision=api.FP32)
+    ps = api.PowerSpectrum(p)
+    ps.set_particles(parts)
+    ps.assign()
+    origin = [-L / 2, -L / 2, -L / 2]
+    ps.survey_power(origin)            # warmup (allocates the survey buffers)
+    torch.cuda.synchronize()
+    reps = 2
+    ps.set_timing(True)
+    ps.reset_kernel_times()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        r = ps.survey_power(origin)
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    kt = ps.kernel_times()
+    ps.close()
+    torch.cuda.empty_cache()
+    G, s = 2, 4
+    real = n * n * n * G * s                       # real cells of the used grids
+    cplx = n * n * (n // 2 + 1) * 2 * s            # one complex half-grid
+    kern = {}
+    for k, (t, la) in kt.items():
+        avg = t / la
+        b = None
+        if k == "survey_weight":
+            b = 2 * real
+        elif k == "survey_bin":
+            b = (G + 1) * cplx
+        elif k.startswith("fft_"):
+            b = 2 * G * cplx
+        row = dict(ms_per_call=round(t / reps, 3), launches_per_call=la // reps,
+                   ms_per_launch=round(avg, 4))
+        if b:
+            row["alg_bytes"] = b
+            row["frac_hbm"] = round(b / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+        kern[k] = row
+    log("survey: %.1f ms per call (35 weighted transforms)" % ms)
+    for k, v in sorted(kern.items(), key=lambda kv: -kv[1]["ms_per_call"]):
+        log("    %-16s %9.3f ms/call  %9.4f ms/launch  %s" % (k, v["ms_per_call"], v["ms_per_launch"],
+                                                          v.get("frac_hbm", "")))
+    return dict(workload=("survey estimator: the workload's catalogue on %d^3, TSC+interlacing, "
+                          "density, observer (-L/2,-L/2,-L/2), P_0..4 incl. odd, FP32" % n),
+                ms_per_call=round(ms, 2), transforms_per_call=35,
+                nmodes_total=int(r["nmodes"].sum()), kernels=kern)
+
+
 def profile_pass(parts, geom):
     """One FP32 step of the workload, then a hipMemcpy of COPY_BYTES (the
     counter calibration), for rocprofv3 --pmc runs."""
@@ -267,6 +323,7 @@ def main():
     ap.add_argument("--mocks", type=int, default=512, help="batched mocks (0: skip)")
     ap.add_argument("--mock-streams", type=int, default=6)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-survey", action="store_true")
     ap.add_argument("--profile-pass", action="store_true",
                     help="for rocprofv3 counter runs: one FP32 step and a %d-byte calibration "
                          "copy, no timing" % COPY_BYTES)
@@ -336,6 +393,7 @@ def main():
                                         "deposit") if k in kt) / f32["steps"]
     assign_bytes = n_p * 16 + G * ng * 4
 
+    survey = None if args.no_survey else run_survey(args, parts32, geom)
     del parts32
     torch.cuda.empty_cache()
     batch = run_batch(args) if args.mocks > 0 else None
@@ -381,6 +439,7 @@ def main():
         "roofline": roofline,
         "batched_mocks": batch,
         "adjoint": adjoint,
+        "survey": survey,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

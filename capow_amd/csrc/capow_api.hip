@@ -872,8 +872,9 @@ extern "C" int capow_survey_power(capow_ctx* c, const double origin[3], double* 
   if (p.momentum == 2 || p.momentum == 5)
     return fail(c, CAPOW_ERR_UNSUPPORTED, "survey estimator: Momentum 2 and 5 have no ddg_mom grid");
   const int NK = p.nk;
-  const int per_copy = 6 * NK;
-  if ((size_t)4 * per_copy * 8 > (size_t)160 * 1024)
+  const int per_copy = 6 * NK;   // accumulated P0..P4 and mode counts
+  // LDS per workgroup: 4 waves x up to 4 sums per k bin
+  if ((size_t)4 * 4 * NK * 8 > (size_t)160 * 1024)
     return fail(c, CAPOW_ERR_UNSUPPORTED, "survey estimator: NK too large for the LDS bins");
   const bool mom = p.momentum != 0 && p.momentum != 1;
   const bool used[4] = {true, p.do_interlacing != 0, mom, mom && p.do_interlacing != 0};
@@ -884,10 +885,11 @@ extern "C" int capow_survey_power(capow_ctx* c, const double origin[3], double* 
   if (mom && !c->sv_F0m) HIPCHK(c, hipMalloc(&c->sv_F0m, rbytes));
   if (!c->sv_acc) {
     const int64_t modes = (int64_t)p.nx * p.ny * c->g.rowc;
+    // up to 5 workgroups of 4 waves per CU (the registers' limit)
     int64_t want = (modes + 255) / 256;
-    if (want > 2 * (int64_t)c->ncu) want = 2 * (int64_t)c->ncu;
+    if (want > 5 * (int64_t)c->ncu) want = 5 * (int64_t)c->ncu;
     c->sv_nwg = (int)(want < 1 ? 1 : want);
-    HIPCHK(c, hipMalloc((void**)&c->sv_partials, (size_t)c->sv_nwg * per_copy * 8));
+    HIPCHK(c, hipMalloc((void**)&c->sv_partials, (size_t)c->sv_nwg * 4 * NK * 8));
     HIPCHK(c, hipMalloc((void**)&c->sv_acc, (size_t)per_copy * 8));
     HIPCHK(c, hipHostMalloc((void**)&c->h_sv, (size_t)per_copy * 8, hipHostMallocDefault));
   }

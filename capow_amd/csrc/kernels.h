@@ -194,7 +194,7 @@ struct SurveyLaunch {
   double origin[3];
   int mom;                // momentum cross spectrum (B grids)
   int nwg;
-  double* partials;       // nwg * 6 * NK
+  double* partials;       // nwg * (up to 4) * NK: the sums one pass touches
   double* acc;            // 6 * NK: P0..P4, mode counts
 };
 // work[s] = src[s] * (product of r^ components idx[0..n-1]; idx[0] twice

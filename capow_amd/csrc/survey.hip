@@ -27,6 +27,22 @@
 
 namespace capow {
 
+// Weight of one cell: vec[i0]...vec[i_{n-1}] / r^n (idx[0] twice for
+// n = 4), the reference's r2 == 0 guard included.
+__device__ __forceinline__ double survey_w(double vx, double vy, double vz, int n, int i0, int i1,
+                                           int i2) {
+  const double vec[3] = {vx, vy, vz};
+  double r2 = vx * vx + vy * vy + vz * vz;
+  if (r2 == 0.0) r2 = 1.0;   // src/main.c:557
+  if (n == 1) return vec[i0] / sqrt(r2);
+  if (n == 2) return vec[i0] * vec[i1] / r2;
+  if (n == 3) return vec[i0] * vec[i1] * vec[i2] / (r2 * sqrt(r2));
+  return vec[i0] * vec[i0] * vec[i1] * vec[i2] / (r2 * r2);
+}
+
+// One workgroup per (ix, iy) row; a thread weights V consecutive cells of
+// every field with one 16-byte load and store each (rows start on 128-byte
+// boundaries).  n = 0 copies.
 template <typename T>
 __global__ __launch_bounds__(256) void k_survey_weight(const T* __restrict__ s0,
                                                        const T* __restrict__ s1,
@@ -36,48 +52,56 @@ __global__ __launch_bounds__(256) void k_survey_weight(const T* __restrict__ s0,
                                                        T* __restrict__ w2, T* __restrict__ w3,
                                                        Geom g, double ox, double oy, double oz,
                                                        int n, int i0, int i1, int i2) {
+  constexpr int V = 16 / sizeof(T);
+  typedef T vec_t __attribute__((ext_vector_type(V)));
   const int64_t row = blockIdx.x;                  // ix * NY + iy
   const int ix = (int)(row / g.n[1]);
   const int iy = (int)(row - (int64_t)ix * g.n[1]);
   const int64_t base = row * g.rowlen;
   const double vx = (double)ix * g.d[0] + g.lo[0] - ox;
   const double vy = (double)iy * g.d[1] + g.lo[1] - oy;
-  for (int iz = threadIdx.x; iz < g.n[2]; iz += blockDim.x) {
-    const double vec[3] = {vx, vy, (double)iz * g.d[2] + g.lo[2] - oz};
-    double r2 = vec[0] * vec[0] + vec[1] * vec[1] + vec[2] * vec[2];
-    if (r2 == 0.0) r2 = 1.0;   // src/main.c:557
-    // the reference's operation order: field * vec... / r^l
-    double num = 1.0, den = 1.0;
-    if (n == 1) {
-      num = vec[i0];
-      den = sqrt(r2);
-    } else if (n == 2) {
-      num = vec[i0] * vec[i1];
-      den = r2;
-    } else if (n == 3) {
-      num = vec[i0] * vec[i1] * vec[i2];
-      den = r2 * sqrt(r2);
-    } else if (n == 4) {
-      num = vec[i0] * vec[i0] * vec[i1] * vec[i2];
-      den = r2 * r2;
-    }
-    const int64_t q = base + iz;
-    if (n == 0) {
-      w0[q] = s0[q];
-      if (s1) w1[q] = s1[q];
-      if (s2) w2[q] = s2[q];
-      if (s3) w3[q] = s3[q];
+  const int nz = g.n[2];
+  const T* src[4] = {s0, s1, s2, s3};
+  T* dst[4] = {w0, w1, w2, w3};
+  for (int z0 = threadIdx.x * V; z0 < nz; z0 += blockDim.x * V) {
+    double w[V];
+#pragma unroll
+    for (int u = 0; u < V; u++)
+      w[u] = n == 0 ? 1.0 : survey_w(vx, vy, (double)(z0 + u) * g.d[2] + g.lo[2] - oz, n, i0, i1, i2);
+    const int64_t q = base + z0;
+    if (z0 + V <= nz) {
+#pragma unroll
+      for (int f = 0; f < 4; f++) {
+        if (!src[f]) continue;
+        vec_t v = *reinterpret_cast<const vec_t*>(src[f] + q);
+        if (n != 0) {
+#pragma unroll
+          for (int u = 0; u < V; u++) v[u] = (T)((double)v[u] * w[u]);
+        }
+        *reinterpret_cast<vec_t*>(dst[f] + q) = v;
+      }
     } else {
-      w0[q] = (T)((double)s0[q] * num / den);
-      if (s1) w1[q] = (T)((double)s1[q] * num / den);
-      if (s2) w2[q] = (T)((double)s2[q] * num / den);
-      if (s3) w3[q] = (T)((double)s3[q] * num / den);
+      for (int u = 0; z0 + u < nz; u++)
+        for (int f = 0; f < 4; f++)
+          if (src[f]) dst[f][q + u] = n == 0 ? src[f][q + u] : (T)((double)src[f][q + u] * w[u]);
     }
   }
 }
 
-// One multipole pass.  A wave owns one LDS copy of [P0..P4][NK] and the NK
-// mode counts; lanes walk the padded complex layout with a grid stride.
+// Sums one multipole pass touches, in the order of its LDS slots (indices
+// into the accumulated [P0, P1, P2, P3, P4, counts][NK]).
+__host__ __device__ __forceinline__ int survey_slots(int multipole) {
+  return multipole == 0 ? 4 : (multipole <= 2 ? 2 : 1);
+}
+__host__ __device__ __forceinline__ int survey_slot_target(int multipole, int slot) {
+  // l = 0: P0 P2 P4 counts; l = 1: P1 P3; l = 2: P2 P4; l = 3: P3; l = 4: P4
+  return multipole == 0 ? (slot == 3 ? 5 : 2 * slot)
+                        : (multipole == 1 ? 1 + 2 * slot : (multipole == 2 ? 2 + 2 * slot : multipole));
+}
+
+// One multipole pass.  A wave owns one LDS copy of the pass's sums
+// (survey_slots(l) x NK doubles); lanes walk the padded complex layout with a
+// grid stride.
 template <typename T>
 __global__ __launch_bounds__(256) void k_survey_bin(const T* __restrict__ gA,
                                                     const T* __restrict__ gAi,
@@ -90,7 +114,7 @@ __global__ __launch_bounds__(256) void k_survey_bin(const T* __restrict__ gA,
   typedef typename CT<T>::t C;
   extern __shared__ double sm[];
   const int NK = a.nk;
-  const int per_copy = 6 * NK;
+  const int per_copy = survey_slots(multipole) * NK;
   const int nwave = blockDim.x >> 6;
   for (int q = threadIdx.x; q < per_copy * nwave; q += blockDim.x) sm[q] = 0.0;
   __syncthreads();
@@ -162,31 +186,31 @@ __global__ __launch_bounds__(256) void k_survey_bin(const T* __restrict__ gA,
     }
     power *= G.NM;
     const int kb = G.kbin;
-    if (multipole == 0) {
+    if (multipole == 0) {   // slots P0 P2 P4 counts
       atomicAdd(&acc[kb], power);
-      atomicAdd(&acc[2 * NK + kb], -0.5 * power);
-      atomicAdd(&acc[4 * NK + kb], 0.375 * power);
-      atomicAdd(&acc[5 * NK + kb], G.NM);
+      atomicAdd(&acc[NK + kb], -0.5 * power);
+      atomicAdd(&acc[2 * NK + kb], 0.375 * power);
+      atomicAdd(&acc[3 * NK + kb], G.NM);
       continue;
     }
     const double fk = G.fktot;
     if (!(fk > 0.0)) continue;
     const double kv[3] = {twopi * R.fx, twopi * R.fy, twopi * fz};
-    if (multipole == 1) {
+    if (multipole == 1) {   // slots P1 P3
       power *= kv[ii] / fk;
-      atomicAdd(&acc[NK + kb], power);
-      atomicAdd(&acc[3 * NK + kb], -1.5 * power);
-    } else if (multipole == 2) {
+      atomicAdd(&acc[kb], power);
+      atomicAdd(&acc[NK + kb], -1.5 * power);
+    } else if (multipole == 2) {   // slots P2 P4
       const double pre = ii != jj ? 2.0 : 1.0;
       power *= pre * kv[ii] * kv[jj] / (fk * fk);
-      atomicAdd(&acc[2 * NK + kb], 1.5 * power);
-      atomicAdd(&acc[4 * NK + kb], -3.75 * power);
-    } else if (multipole == 3) {
+      atomicAdd(&acc[kb], 1.5 * power);
+      atomicAdd(&acc[NK + kb], -3.75 * power);
+    } else if (multipole == 3) {   // slot P3
       const double pre = ii != kk ? (ii != jj ? 6.0 : 3.0) : 1.0;
-      atomicAdd(&acc[3 * NK + kb], 2.5 * pre * kv[ii] * kv[jj] * kv[kk] / (fk * fk * fk) * power);
-    } else {
+      atomicAdd(&acc[kb], 2.5 * pre * kv[ii] * kv[jj] * kv[kk] / (fk * fk * fk) * power);
+    } else {   // slot P4
       const double pre = ii == jj ? (ii != kk ? 4.0 : 1.0) : (jj == kk ? 6.0 : 12.0);
-      atomicAdd(&acc[4 * NK + kb],
+      atomicAdd(&acc[kb],
                 4.375 * pre * kv[ii] * kv[ii] * kv[jj] * kv[kk] / (fk * fk * fk * fk) * power);
     }
   }
@@ -199,14 +223,26 @@ __global__ __launch_bounds__(256) void k_survey_bin(const T* __restrict__ gA,
   }
 }
 
-// acc += sum of the workgroup partials, in workgroup order.
-__global__ void k_survey_final(const double* __restrict__ partials, int nwg, int len,
-                               double* __restrict__ acc) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= len) return;
+// acc[target] += sum over the workgroup partials of one pass: one block per
+// sum, each thread a fixed stride of workgroups, then a fixed-shape tree, so
+// the order of the additions never changes.
+__global__ __launch_bounds__(256) void k_survey_final(const double* __restrict__ partials, int nwg,
+                                                      int len, int nk, int multipole,
+                                                      double* __restrict__ acc) {
+  __shared__ double red[256];
+  const int q = blockIdx.x;   // < len
   double t = 0.0;
-  for (int w = 0; w < nwg; w++) t += partials[(int64_t)w * len + q];
-  acc[q] += t;
+  for (int w = threadIdx.x; w < nwg; w += 256) t += partials[(int64_t)w * len + q];
+  red[threadIdx.x] = t;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const int slot = q / nk, k = q - slot * nk;
+    acc[survey_slot_target(multipole, slot) * nk + k] += red[0];
+  }
 }
 
 template <typename T>
@@ -228,7 +264,7 @@ int run_survey_weight(SurveyLaunch& L, bool fp64, int n, const int idx[3]) {
 template <typename T>
 static int bin_t(SurveyLaunch& L, int multipole, const int idx[3]) {
   const int NK = L.dev.nk;
-  const int per_copy = 6 * NK;
+  const int per_copy = survey_slots(multipole) * NK;
   const size_t lds = (size_t)4 * per_copy * sizeof(double);
   const void* fn = (const void*)k_survey_bin<T>;
   if (lds > 64 * 1024) hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -240,8 +276,8 @@ static int bin_t(SurveyLaunch& L, int multipole, const int idx[3]) {
             (const T*)L.work[0], w1, w2, w3, (T*)L.F0, (T*)(L.F0m ? L.F0m : L.F0), L.tabs, L.dev,
             L.mom, multipole, idx[0], idx[1], idx[2], L.partials)));
   TIMED(L.timer, "survey_final",
-        (k_survey_final<<<(per_copy + 255) / 256, 256, 0, L.stream>>>(L.partials, L.nwg, per_copy,
-                                                                     L.acc)));
+        (k_survey_final<<<per_copy, 256, 0, L.stream>>>(L.partials, L.nwg, per_copy, NK, multipole,
+                                                       L.acc)));
   return hipGetLastError() == hipSuccess ? CAPOW_OK : CAPOW_ERR_HIP;
 }
 

@@ -128,10 +128,22 @@ __global__ __launch_bounds__(256) void k_survey_bin(const T* __restrict__ gA,
   const uint32_t plane = (uint32_t)a.ny * (uint32_t)a.rowc;
   const int64_t total = (int64_t)a.nx * plane;
   const double twopi = 2.0 * 3.14159265358979323846;
-  for (int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; m < total;
-       m += (int64_t)gridDim.x * blockDim.x) {
-    const int i = (int)(m / plane);
-    const uint32_t r = (uint32_t)(m - (int64_t)i * plane);
+  // grid-stride walk; the plane index i and offset r advance with carries
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int s_i = (int)(stride / plane);
+  const uint32_t s_r = (uint32_t)(stride - (int64_t)s_i * plane);
+  int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int i_next = (int)(m / plane);
+  uint32_t r_next = (uint32_t)(m - (int64_t)i_next * plane);
+  for (; m < total; m += stride) {
+    const int i = i_next;
+    const uint32_t r = r_next;
+    r_next += s_r;
+    i_next += s_i;
+    if (r_next >= plane) {
+      r_next -= plane;
+      i_next++;
+    }
     const uint32_t j = a.fd_rowc.div(r);
     const int k = (int)(r - j * (uint32_t)a.rowc);
     if (k >= a.nzh) continue;   // row padding

@@ -100,8 +100,7 @@ __host__ __device__ __forceinline__ int survey_slot_target(int multipole, int sl
 }
 
 // One multipole pass.  A wave owns one LDS copy of the pass's sums
-// (survey_slots(l) x NK doubles); lanes walk the padded complex layout with a
-// grid stride.
+// (survey_slots(l) x NK doubles) and walks whole (i, j) rows.
 template <typename T>
 __global__ __launch_bounds__(256) void k_survey_bin(const T* __restrict__ gA,
                                                     const T* __restrict__ gAi,
@@ -125,105 +124,96 @@ __global__ __launch_bounds__(256) void k_survey_bin(const T* __restrict__ gA,
   const C* Bi = reinterpret_cast<const C*>(gBi);
   C* f0 = reinterpret_cast<C*>(F0);
   C* f0m = reinterpret_cast<C*>(F0m);
-  const uint32_t plane = (uint32_t)a.ny * (uint32_t)a.rowc;
-  const int64_t total = (int64_t)a.nx * plane;
   const double twopi = 2.0 * 3.14159265358979323846;
-  // grid-stride walk; the plane index i and offset r advance with carries
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  const int s_i = (int)(stride / plane);
-  const uint32_t s_r = (uint32_t)(stride - (int64_t)s_i * plane);
-  int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int i_next = (int)(m / plane);
-  uint32_t r_next = (uint32_t)(m - (int64_t)i_next * plane);
-  for (; m < total; m += stride) {
-    const int i = i_next;
-    const uint32_t r = r_next;
-    r_next += s_r;
-    i_next += s_i;
-    if (r_next >= plane) {
-      r_next -= plane;
-      i_next++;
-    }
-    const uint32_t j = a.fd_rowc.div(r);
-    const int k = (int)(r - j * (uint32_t)a.rowc);
-    if (k >= a.nzh) continue;   // row padding
-    const RowVals R = row_vals(i, (int)j, tabs, a);
-    const double fz = tabs[a.ofz + k];
-    const ModeGeom G = mode_geom(R, k, fz, tabs[a.ocz + k], tabs[a.opz + 2 * k],
-                                 tabs[a.opz + 2 * k + 1], a);
-    if (!G.ok) continue;
-    double dkr = (double)A[m].x, dki = (double)A[m].y;
-    if (a.interlace) {
-      const C v = Ai[m];
-      dkr += (double)v.x * G.cr - (double)v.y * G.ci;
-      dki += (double)v.x * G.ci + (double)v.y * G.cr;
-    }
-    double dkr_m = 0.0, dki_m = 0.0;
-    if (mom) {
-      dkr_m = (double)B[m].x;
-      dki_m = (double)B[m].y;
+  // one wave per (i, j) row: the row's x and y quantities are wave-uniform,
+  // lanes take kz = lane, lane + 64, ...
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63;
+  const int64_t nrows = (int64_t)a.nx * a.ny;
+  for (int64_t row = (int64_t)blockIdx.x * nwave + wave; row < nrows;
+       row += (int64_t)gridDim.x * nwave) {
+    const int i = (int)(row / a.ny);
+    const int j = (int)(row - (int64_t)i * a.ny);
+    const RowVals R = row_vals(i, j, tabs, a);
+    for (int k = lane; k < a.nzh; k += 64) {
+      const int64_t m = row * a.rowc + k;
+      const double fz = tabs[a.ofz + k];
+      const ModeGeom G = mode_geom(R, k, fz, tabs[a.ocz + k], tabs[a.opz + 2 * k],
+                                   tabs[a.opz + 2 * k + 1], a);
+      if (!G.ok) continue;
+      double dkr = (double)A[m].x, dki = (double)A[m].y;
       if (a.interlace) {
-        const C v = Bi[m];
-        dkr_m += (double)v.x * G.cr - (double)v.y * G.ci;
-        dki_m += (double)v.x * G.ci + (double)v.y * G.cr;
+        const C v = Ai[m];
+        dkr += (double)v.x * G.cr - (double)v.y * G.ci;
+        dki += (double)v.x * G.ci + (double)v.y * G.cr;
       }
-    }
-    const double cor = G.cor;
-    double power;
-    if (multipole == 0) {
-      C o;
-      o.x = (T)dkr;
-      o.y = (T)dki;
-      f0[m] = o;
+      double dkr_m = 0.0, dki_m = 0.0;
       if (mom) {
-        o.x = (T)dkr_m;
-        o.y = (T)dki_m;
-        f0m[m] = o;
-        power = (dkr * dki_m - dki * dkr_m) * cor;
-      } else {
-        power = (dkr * dkr + dki * dki) * cor;
+        dkr_m = (double)B[m].x;
+        dki_m = (double)B[m].y;
+        if (a.interlace) {
+          const C v = Bi[m];
+          dkr_m += (double)v.x * G.cr - (double)v.y * G.ci;
+          dki_m += (double)v.x * G.ci + (double)v.y * G.cr;
+        }
       }
-    } else {
-      const C z = f0[m];
-      const double zr = (double)z.x, zi = (double)z.y;
-      if (!mom) {
-        power = (multipole & 1) ? (dkr * zi - dki * zr) * cor : (dkr * zr + dki * zi) * cor;
+      const double cor = G.cor;
+      double power;
+      if (multipole == 0) {
+        C o;
+        o.x = (T)dkr;
+        o.y = (T)dki;
+        f0[m] = o;
+        if (mom) {
+          o.x = (T)dkr_m;
+          o.y = (T)dki_m;
+          f0m[m] = o;
+          power = (dkr * dki_m - dki * dkr_m) * cor;
+        } else {
+          power = (dkr * dkr + dki * dki) * cor;
+        }
       } else {
-        const C zm = f0m[m];
-        const double mr = (double)zm.x, mi = (double)zm.y;
-        power = (multipole & 1)
-                    ? 0.5 * (dkr * mr + dki * mi + dkr_m * zr + dki_m * zi) * cor
-                    : 0.5 * (dkr * mi - dki * mr + dkr_m * zi - dki_m * zr) * cor;
+        const C z = f0[m];
+        const double zr = (double)z.x, zi = (double)z.y;
+        if (!mom) {
+          power = (multipole & 1) ? (dkr * zi - dki * zr) * cor : (dkr * zr + dki * zi) * cor;
+        } else {
+          const C zm = f0m[m];
+          const double mr = (double)zm.x, mi = (double)zm.y;
+          power = (multipole & 1)
+                      ? 0.5 * (dkr * mr + dki * mi + dkr_m * zr + dki_m * zi) * cor
+                      : 0.5 * (dkr * mi - dki * mr + dkr_m * zi - dki_m * zr) * cor;
+        }
       }
-    }
-    power *= G.NM;
-    const int kb = G.kbin;
-    if (multipole == 0) {   // slots P0 P2 P4 counts
-      atomicAdd(&acc[kb], power);
-      atomicAdd(&acc[NK + kb], -0.5 * power);
-      atomicAdd(&acc[2 * NK + kb], 0.375 * power);
-      atomicAdd(&acc[3 * NK + kb], G.NM);
-      continue;
-    }
-    const double fk = G.fktot;
-    if (!(fk > 0.0)) continue;
-    const double kv[3] = {twopi * R.fx, twopi * R.fy, twopi * fz};
-    if (multipole == 1) {   // slots P1 P3
-      power *= kv[ii] / fk;
-      atomicAdd(&acc[kb], power);
-      atomicAdd(&acc[NK + kb], -1.5 * power);
-    } else if (multipole == 2) {   // slots P2 P4
-      const double pre = ii != jj ? 2.0 : 1.0;
-      power *= pre * kv[ii] * kv[jj] / (fk * fk);
-      atomicAdd(&acc[kb], 1.5 * power);
-      atomicAdd(&acc[NK + kb], -3.75 * power);
-    } else if (multipole == 3) {   // slot P3
-      const double pre = ii != kk ? (ii != jj ? 6.0 : 3.0) : 1.0;
-      atomicAdd(&acc[kb], 2.5 * pre * kv[ii] * kv[jj] * kv[kk] / (fk * fk * fk) * power);
-    } else {   // slot P4
-      const double pre = ii == jj ? (ii != kk ? 4.0 : 1.0) : (jj == kk ? 6.0 : 12.0);
-      atomicAdd(&acc[kb],
-                4.375 * pre * kv[ii] * kv[ii] * kv[jj] * kv[kk] / (fk * fk * fk * fk) * power);
+      power *= G.NM;
+      const int kb = G.kbin;
+      if (multipole == 0) {   // slots P0 P2 P4 counts
+        atomicAdd(&acc[kb], power);
+        atomicAdd(&acc[NK + kb], -0.5 * power);
+        atomicAdd(&acc[2 * NK + kb], 0.375 * power);
+        atomicAdd(&acc[3 * NK + kb], G.NM);
+        continue;
+      }
+      const double fk = G.fktot;
+      if (!(fk > 0.0)) continue;
+      const double kv[3] = {twopi * R.fx, twopi * R.fy, twopi * fz};
+      if (multipole == 1) {   // slots P1 P3
+        power *= kv[ii] / fk;
+        atomicAdd(&acc[kb], power);
+        atomicAdd(&acc[NK + kb], -1.5 * power);
+      } else if (multipole == 2) {   // slots P2 P4
+        const double pre = ii != jj ? 2.0 : 1.0;
+        power *= pre * kv[ii] * kv[jj] / (fk * fk);
+        atomicAdd(&acc[kb], 1.5 * power);
+        atomicAdd(&acc[NK + kb], -3.75 * power);
+      } else if (multipole == 3) {   // slot P3
+        const double pre = ii != kk ? (ii != jj ? 6.0 : 3.0) : 1.0;
+        atomicAdd(&acc[kb], 2.5 * pre * kv[ii] * kv[jj] * kv[kk] / (fk * fk * fk) * power);
+      } else {   // slot P4
+        const double pre = ii == jj ? (ii != kk ? 4.0 : 1.0) : (jj == kk ? 6.0 : 12.0);
+        atomicAdd(&acc[kb],
+                  4.375 * pre * kv[ii] * kv[ii] * kv[jj] * kv[kk] / (fk * fk * fk * fk) * power);
+      }
     }
   }
   __syncthreads();

@@ -393,6 +393,10 @@ MIRROR_CASES = [
           odd_multipoles=1)),
     ("fp32_2d_two_per_lane", api.FP32, (32, 32, 256),
      dict(interp_order=2, do_interlacing=1, momentum=0, output_2d=1, nmu=10)),
+    # (k, mu) bins over log k bins, momentum cross spectrum with its autos
+    ("fp64_2d_log_bins", api.FP64, (48, 64, 96),
+     dict(interp_order=3, do_interlacing=1, momentum=3, extra_spectra=3, output_2d=1, nmu=5,
+          output_log=1, mink=-2.4, maxk=-0.7, nk=17, los=3)),
 ]
 
 

@@ -228,6 +228,38 @@ def run_survey(args, parts, geom):
                 nmodes_total=int(r["nmodes"].sum()), kernels=kern)
 
 
+def run_bin_variants(parts, geom):
+    """Binning of the workload's grids (1024^3, TSC + interlacing, P00/P11/P01)
+    with log k bins (OutputLog, 100 bins from 10^-2.3 to 10^0.2) and with
+    (k, mu) bins (Output2D, NMU = 10 and 4), FP32 form: the binning kernels'
+    HIP-event times of one timed step after one warmup step."""
+    import torch
+    from capow_amd import api
+    n, L = geom["n"], geom["L"]
+    out = {}
+    for name, kw in (("log_k", dict(output_log=1, mink=-2.3, maxk=0.2, nk=100)),
+                     ("k_mu_nmu10", dict(output_2d=1, nmu=10)),
+                     ("k_mu_nmu4", dict(output_2d=1, nmu=4))):
+        base = dict(nx=n, ny=n, nz=n, xmax=L, ymax=L, zmax=L, interp_order=3, do_interlacing=1,
+                    los=0, momentum=1, nk=geom["nk"], maxk=geom["nk"] * geom["dk"],
+                    precision=api.FP32, extra_spectra=api.SPEC_AUTO_A | api.SPEC_AUTO_B)
+        base.update(kw)
+        ps = api.PowerSpectrum(api.make_params(**base))
+        ps.set_particles(parts)
+        ps.power()
+        ps.set_timing(True)
+        ps.reset_kernel_times()
+        ps.power()
+        kt = ps.kernel_times()
+        ps.close()
+        torch.cuda.empty_cache()
+        bins = {k: round(v[0] / v[1], 3) for k, v in kt.items() if k.startswith("bin")}
+        out[name] = dict(kernels_ms=bins, bin_total_ms=round(sum(v[0] for k, v in kt.items()
+                                                                  if k.startswith("bin")), 3))
+        log("binning %s: %s" % (name, bins))
+    return out
+
+
 def profile_pass(parts, geom):
     """One FP32 step of the workload, then a hipMemcpy of COPY_BYTES (the
     counter calibration), for rocprofv3 --pmc runs."""
@@ -324,6 +356,7 @@ def main():
     ap.add_argument("--mock-streams", type=int, default=6)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-survey", action="store_true")
+    ap.add_argument("--no-bin-variants", action="store_true")
     ap.add_argument("--profile-pass", action="store_true",
                     help="for rocprofv3 counter runs: one FP32 step and a %d-byte calibration "
                          "copy, no timing" % COPY_BYTES)
@@ -394,6 +427,7 @@ def main():
     assign_bytes = n_p * 16 + G * ng * 4
 
     survey = None if args.no_survey else run_survey(args, parts32, geom)
+    bin_variants = None if args.no_bin_variants else run_bin_variants(parts32, geom)
     del parts32
     torch.cuda.empty_cache()
     batch = run_batch(args) if args.mocks > 0 else None
@@ -440,6 +474,7 @@ def main():
         "batched_mocks": batch,
         "adjoint": adjoint,
         "survey": survey,
+        "bin_variants": bin_variants,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

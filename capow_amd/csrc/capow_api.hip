@@ -39,6 +39,7 @@ struct capow_ctx {
   int ncu = 256;                    // compute units of the context's device
   int fft_rows_mul = 0;             // CAPOW_FFT_ROWS_MUL=1|2 forces 64|128-byte write runs
   int64_t grid_elems = 0;
+  size_t grid_bytes = 0;            // capacity of every grid and scratch buffer
   // particles
   const void* parts = nullptr;
   const void* wts = nullptr;
@@ -303,6 +304,7 @@ static int setup(capow_ctx* c) {
   if (l1 > cap_c) cap_c = l1;
   if (l2 > cap_c) cap_c = l2;
   const size_t gbytes = (size_t)cap_c * 2 * c->esz;
+  c->grid_bytes = gbytes;
   int ngr = 0;
   for (int s = 0; s < 4; s++) {
     bool need = (s == 0) || (s == 1 && g.interlace) || (s == 2 && hasB) || (s == 3 && hasB && g.interlace);
@@ -879,8 +881,11 @@ extern "C" int capow_survey_power(capow_ctx* c, const double origin[3], double* 
   const bool mom = p.momentum != 0 && p.momentum != 1;
   const bool used[4] = {true, p.do_interlacing != 0, mom, mom && p.do_interlacing != 0};
   const size_t rbytes = (size_t)p.nx * p.ny * c->g.rowlen * c->esz;
+  // one spare buffer per used grid, of the grids' capacity: it takes the
+  // grid's place for the weighted copies while the real-space field stays
+  // untouched in the original buffer, which becomes the grid again at the end
   for (int s = 0; s < 4; s++)
-    if (used[s] && !c->sv_src[s]) HIPCHK(c, hipMalloc(&c->sv_src[s], rbytes));
+    if (used[s] && !c->sv_src[s]) HIPCHK(c, hipMalloc(&c->sv_src[s], c->grid_bytes));
   if (!c->sv_F0) HIPCHK(c, hipMalloc(&c->sv_F0, rbytes));
   if (mom && !c->sv_F0m) HIPCHK(c, hipMalloc(&c->sv_F0m, rbytes));
   if (!c->sv_acc) {
@@ -893,10 +898,13 @@ extern "C" int capow_survey_power(capow_ctx* c, const double origin[3], double* 
     HIPCHK(c, hipMalloc((void**)&c->sv_acc, (size_t)per_copy * 8));
     HIPCHK(c, hipHostMalloc((void**)&c->h_sv, (size_t)per_copy * 8, hipHostMallocDefault));
   }
-  for (int s = 0; s < 4; s++)
-    if (used[s])
-      HIPCHK(c, hipMemcpyAsync(c->sv_src[s], c->grid[s], rbytes, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(c, hipMemsetAsync(c->sv_acc, 0, (size_t)per_copy * 8, c->stream));
+  // swap the spare buffers in: sv_src holds the fields, grid the work copies
+  auto swap_in_out = [&]() {
+    for (int s = 0; s < 4; s++)
+      if (used[s]) std::swap(c->grid[s], c->sv_src[s]);
+  };
+  swap_in_out();
   SurveyLaunch L;
   L.stream = c->stream;
   L.timer = &c->timer;
@@ -921,7 +929,10 @@ extern "C" int capow_survey_power(capow_ctx* c, const double origin[3], double* 
     const int nc = survey_combos(l, idx);
     for (int q = 0; q < nc; q++) {
       int rc = run_survey_weight(L, c->fp64, l, idx[q]);
-      if (rc) return fail(c, rc, std::string("survey weighting failed: ") + hipGetErrorString(hipGetLastError()));
+      if (rc) {
+        swap_in_out();
+        return fail(c, rc, std::string("survey weighting failed: ") + hipGetErrorString(hipGetLastError()));
+      }
       // transform the weighted copies (the slots in use only)
       FftArgs a;
       fft_args(c, a);
@@ -934,7 +945,10 @@ extern "C" int capow_survey_power(capow_ctx* c, const double origin[3], double* 
         }
       if (!c->scratch[0]) a.scratch[0] = nullptr;
       rc = run_fft(a, c->fp64);
-      if (rc) return fail(c, rc, std::string("fft launch failed: ") + hipGetErrorString(hipGetLastError()));
+      if (rc) {
+        swap_in_out();
+        return fail(c, rc, std::string("fft launch failed: ") + hipGetErrorString(hipGetLastError()));
+      }
       if (a.swapped)
         for (int s = 0; s < 4; s++)
           if (used[s]) {
@@ -944,14 +958,15 @@ extern "C" int capow_survey_power(capow_ctx* c, const double origin[3], double* 
             L.work[s] = c->grid[s];
           }
       rc = run_survey_bin(L, c->fp64, l, idx[q]);
-      if (rc) return fail(c, rc, std::string("survey binning failed: ") + hipGetErrorString(hipGetLastError()));
+      if (rc) {
+        swap_in_out();
+        return fail(c, rc, std::string("survey binning failed: ") + hipGetErrorString(hipGetLastError()));
+      }
     }
   }
+  // the grids are the untouched real-space fields again
+  swap_in_out();
   HIPCHK(c, hipMemcpyAsync(c->h_sv, c->sv_acc, (size_t)per_copy * 8, hipMemcpyDeviceToHost, c->stream));
-  // the grids hold the real-space fields again afterwards
-  for (int s = 0; s < 4; s++)
-    if (used[s])
-      HIPCHK(c, hipMemcpyAsync(c->grid[s], c->sv_src[s], rbytes, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->timer.collect();
   for (int l = 0; l < 5; l++)

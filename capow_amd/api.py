@@ -23,7 +23,7 @@ ERRORS = {1: "CAPOW_ERR_ARG", 2: "CAPOW_ERR_HIP", 3: "CAPOW_ERR_NOMEM", 4: "CAPO
           5: "CAPOW_ERR_UNSUPPORTED", 6: "CAPOW_ERR_INTERNAL", 7: "CAPOW_ERR_IO",
           8: "CAPOW_ERR_PARSE", 110: "CAPOW_ERR_OUT_OF_BOX"}
 FP32, FP64 = 0, 1
-EDGE_REFERENCE, EDGE_PERIODIC = 0, 1
+EDGE_REFERENCE, EDGE_PERIODIC, EDGE_SURVEY = 0, 1, 2
 SPEC_AUTO_A, SPEC_AUTO_B = 1, 2
 HOST, DEVICE = 0, 1
 
@@ -58,6 +58,25 @@ class Bins(C.Structure):
     _fields_ = [("n_spectra", C.c_int32), ("codes", C.c_int32 * 3),
                 ("pk", C.POINTER(C.c_double)), ("nmodes", C.POINTER(C.c_int64)),
                 ("pk2d", C.POINTER(C.c_double)), ("nmodes2d", C.POINTER(C.c_int64))]
+
+
+class SurveyObj(C.Structure):
+    _fields_ = [(k, C.c_double) for k in
+                ("x", "y", "z", "redshift", "weight", "weight_pv", "nbar", "pv", "pverr")]
+
+
+class SurveyParams(C.Structure):
+    _fields_ = [("coord_type", C.c_int32), ("momentum", C.c_int32), ("nbar_column", C.c_int32),
+                ("fkp_column", C.c_int32), ("omega_m", C.c_double), ("redmin_in", C.c_double),
+                ("redmax_in", C.c_double), ("sky_area", C.c_double), ("fkp_pk", C.c_double),
+                ("fkp_pk_mom", C.c_double), ("fkp_pk_cross", C.c_double)]
+
+
+class SurveyTotals(C.Structure):
+    _fields_ = [(k, C.c_double) for k in
+                ("data_nbw", "data_nbwsq", "data_nbsqwsq", "data_nbsqwsq_pv", "data_vr",
+                 "data_vrsq", "rand_nbw", "rand_nbwsq", "rand_nbsqwsq", "rand_nbsqwsq_pv",
+                 "alpha", "shot", "norm")]
 
 
 class Mock(C.Structure):
@@ -106,6 +125,26 @@ SIGNATURES = {
                          C.POINTER(AssignStats), C.POINTER(C.c_float)], C.c_int),
     "capow_batch_destroy": ([_V], C.c_int),
     "capow_batch_last_error": ([_V], C.c_char_p),
+    "capow_cspline_init": ([C.c_int32, _D, _D, _D], C.c_int),
+    "capow_cspline_eval": ([C.c_int32, _D, _D, _D, C.c_double, _D], C.c_int),
+    "capow_zr_table": ([C.c_double, C.c_int32, C.c_double, _D, _D], C.c_int),
+    "capow_cosmo_create": ([C.c_double, C.POINTER(_V)], C.c_int),
+    "capow_cosmo_destroy": ([_V], C.c_int),
+    "capow_cosmo_r_of_z": ([_V, C.c_double, _D], C.c_int),
+    "capow_cosmo_z_of_r": ([_V, C.c_double, _D], C.c_int),
+    "capow_read_survey": ([C.c_char_p, C.c_int32, _P, C.POINTER(SurveyParams), _V, C.c_int64,
+                           C.POINTER(C.POINTER(SurveyObj)), _I64, _I64, _D, _D, C.c_char_p,
+                           C.c_int32, _D], C.c_int),
+    "capow_free_survey": ([C.POINTER(SurveyObj)], None),
+    "capow_survey_nbar_bins": ([C.POINTER(SurveyParams)], C.c_int32),
+    "capow_survey_nbar": ([C.POINTER(SurveyParams), _V, _D, C.POINTER(SurveyObj), C.c_int64,
+                           C.POINTER(SurveyObj), C.c_int64, _D, _D, C.c_int32], C.c_int),
+    "capow_survey_fkp": ([C.POINTER(SurveyParams), C.POINTER(SurveyObj), C.c_int64], C.c_int),
+    "capow_survey_sums": ([C.POINTER(SurveyParams), C.POINTER(SurveyObj), C.c_int64,
+                           C.POINTER(SurveyObj), C.c_int64, C.POINTER(SurveyTotals)], C.c_int),
+    "capow_survey_pack": ([C.POINTER(SurveyParams), C.POINTER(SurveyObj), C.c_int64,
+                           C.POINTER(SurveyObj), C.c_int64, C.c_double, C.c_int32, _V, _V,
+                           _I64], C.c_int),
 }
 
 _lib = None

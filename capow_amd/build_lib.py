@@ -27,7 +27,7 @@ HIP_FLAGS = [
 ]
 HIP_SOURCES = ["capow_api.hip", "deposit.hip", "fft.hip", "binning.hip", "adjoint.hip",
                "survey.hip"]
-C_SOURCES = ["capow_host.c", "capow_ascii.c"]
+C_SOURCES = ["capow_host.c", "capow_ascii.c", "capow_survey_ingest.c"]
 
 
 def _run(cmd):

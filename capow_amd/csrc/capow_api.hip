@@ -194,7 +194,12 @@ static int validate(const capow_params* p, std::string& m) {
   if (p->maxk < p->mink) { m = "`Maxk' is less than `Mink'."; return CAPOW_ERR_ARG; }
   if (p->momentum < 0 || p->momentum > 8) { m = "`Momentum' must be 0..8."; return CAPOW_ERR_ARG; }
   if (p->precision != CAPOW_FP32 && p->precision != CAPOW_FP64) { m = "precision must be CAPOW_FP32 or CAPOW_FP64."; return CAPOW_ERR_ARG; }
-  if (p->edge_mode != CAPOW_EDGE_REFERENCE && p->edge_mode != CAPOW_EDGE_PERIODIC) { m = "edge_mode must be CAPOW_EDGE_REFERENCE or CAPOW_EDGE_PERIODIC."; return CAPOW_ERR_ARG; }
+  if (p->edge_mode != CAPOW_EDGE_REFERENCE && p->edge_mode != CAPOW_EDGE_PERIODIC &&
+      p->edge_mode != CAPOW_EDGE_SURVEY) { m = "edge_mode must be CAPOW_EDGE_REFERENCE, CAPOW_EDGE_PERIODIC or CAPOW_EDGE_SURVEY."; return CAPOW_ERR_ARG; }
+  if (p->edge_mode == CAPOW_EDGE_SURVEY && p->momentum != 0 && p->momentum != 1 && p->momentum != 3) {
+    m = "survey `Momentum' must be 0, 1 or 3 (2 has no ddg_mom grid in the reference).";
+    return CAPOW_ERR_UNSUPPORTED;
+  }
   if (p->nz % 2 != 0) { m = "NZ must be even (r2c row transform)."; return CAPOW_ERR_UNSUPPORTED; }
   if (p->nx > 65536 || p->ny > 65536 || p->nz > 65536) { m = "grid dimension above 65536."; return CAPOW_ERR_UNSUPPORTED; }
   FftPlan pl;
@@ -260,6 +265,19 @@ static int setup(capow_ctx* c) {
   field_exps(p.momentum, &g.fa, &g.fb);
   g.mean_sub_a = (g.fa == 0);
   g.rsd_axis = (p.los >= 1 && p.los <= 3) ? p.los - 1 : -1;
+  if (p.edge_mode == CAPOW_EDGE_SURVEY) {
+    // assign_survey_data (src/read_data.c:461-486): ddg = w (codes 0, 3) or
+    // w v (code 1); ddg_mom = w v (code 3); randoms arrive with -alpha w.
+    // Non-periodic taps (the reference-rule path with LOS != 0), no RSD
+    // shift and no mean subtraction (src/main.c:226-228).
+    g.los = 3;
+    g.edge = CAPOW_EDGE_REFERENCE;
+    g.periodic = false;
+    g.rsd_axis = -1;
+    g.fa = (p.momentum == 1) ? 1 : 0;
+    g.fb = (p.momentum == 3) ? 1 : -1;
+    g.mean_sub_a = 0;
+  }
   g.rsd_f1 = 1.0 + p.redshift;
   g.rsd_den = 100.0 * sqrt(p.omega_m * pow((1.0 + p.redshift), 3.0) + 1.0 - p.omega_m);
   if (g.order == 3) { g.reach_lo = -1; g.reach_hi = g.interlace ? 2 : 1; }
@@ -736,6 +754,8 @@ static void fft_args(capow_ctx* c, FftArgs& a) {
 
 extern "C" int capow_fft(capow_ctx* c) {
   if (!c) return CAPOW_ERR_ARG;
+  if (c->p.edge_mode == CAPOW_EDGE_SURVEY)
+    return fail(c, CAPOW_ERR_UNSUPPORTED, "survey context: use capow_survey_power");
   if (c->state < 2) return fail(c, CAPOW_ERR_STATE, "capow_fft before capow_assign");
   if (c->state >= 3) return fail(c, CAPOW_ERR_STATE, "grids already transformed");
   FftArgs a;
@@ -815,6 +835,8 @@ static void bin_finish(capow_ctx* c, const double* h, capow_bins* out) {
 
 extern "C" int capow_bin(capow_ctx* c, capow_bins* out) {
   if (!c || !out) return CAPOW_ERR_ARG;
+  if (c->p.edge_mode == CAPOW_EDGE_SURVEY)
+    return fail(c, CAPOW_ERR_UNSUPPORTED, "survey context: use capow_survey_power");
   int rc = bin_enqueue(c, c->h_bins);
   if (rc) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1050,6 +1072,8 @@ extern "C" int capow_reset_kernel_times(capow_ctx* c) {
 extern "C" int capow_backward(capow_ctx* c, const double* dL_draw, void* grad_xyzv, void* grad_w,
                               int32_t where) {
   if (!c) return CAPOW_ERR_ARG;
+  if (c->p.edge_mode == CAPOW_EDGE_SURVEY)
+    return fail(c, CAPOW_ERR_UNSUPPORTED, "capow_backward covers the periodic estimator only");
   if (!dL_draw || !grad_xyzv) return fail(c, CAPOW_ERR_ARG, "null gradient array");
   if (where != CAPOW_HOST && where != CAPOW_DEVICE)
     return fail(c, CAPOW_ERR_ARG, "where must be CAPOW_HOST or CAPOW_DEVICE");
@@ -1162,6 +1186,9 @@ extern "C" int capow_batch_create(const capow_params* configs, int32_t nconf,
     return bfail(nullptr, CAPOW_ERR_ARG, "bad batch arguments");
   if (max_particles > (1LL << 29))
     return bfail(nullptr, CAPOW_ERR_UNSUPPORTED, "more than 2^29 particles per catalogue");
+  for (int q = 0; q < nconf; q++)
+    if (configs[q].edge_mode == CAPOW_EDGE_SURVEY)
+      return bfail(nullptr, CAPOW_ERR_UNSUPPORTED, "batches cover the periodic estimator only");
   *out = nullptr;
   capow_batch* b = new capow_batch();
   b->cfg.assign(configs, configs + nconf);

@@ -61,7 +61,20 @@ enum { CAPOW_FP32 = 0, CAPOW_FP64 = 1 };
  * interlaced CIC/NGP reject shifted coordinates >= MAX, LOS != 0 is
  * non-periodic: src/read_data.c:514-745 with the halo add of
  * src/read_data.c:152-175).  PERIODIC is textbook periodic assignment. */
-enum { CAPOW_EDGE_REFERENCE = 0, CAPOW_EDGE_PERIODIC = 1 };
+enum { CAPOW_EDGE_REFERENCE = 0, CAPOW_EDGE_PERIODIC = 1, CAPOW_EDGE_SURVEY = 2 };
+/* CAPOW_EDGE_SURVEY deposits a survey catalogue (data with weight w, randoms
+ * with weight -alpha w) the way assign_survey_data does
+ * (src/read_data.c:461-486, src/main.c:94, 228): `momentum` is the survey
+ * code (0 density: A = w; 1 velocity: A = w v; 3 density + velocity: A = w,
+ * B = w v, where the caller passes v = weight_pv pv / weight so that B holds
+ * weight_pv pv), no mean subtraction, no RSD shift, non-periodic edges.  The
+ * survey reader admits only objects whose every tap lies inside the grid
+ * (src/read_data.c:196-205, 287-290), so the reference's survey edge clamps
+ * (src/read_data.c:577-589) never apply.  Survey codes 2 and 4..8 are
+ * CAPOW_ERR_UNSUPPORTED (the reference allocates no ddg_mom for 2,
+ * src/main.c:1197, and has no survey field for the rest).  capow_fft,
+ * capow_bin and capow_backward reject a survey context; capow_survey_power
+ * is its estimator. */
 
 /* Element type of arrays handed to capow_set_particles. */
 enum { CAPOW_DTYPE_F32 = 0, CAPOW_DTYPE_F64 = 1 };
@@ -269,6 +282,101 @@ int capow_read_ascii(const char *path, int32_t los, const double lo[3], const do
                      int32_t fp64, int32_t nthreads, void **rows, int64_t *n,
                      char *bad_line, int32_t bad_len, double bad_xyz[3]);
 void capow_free_rows(void *rows);
+
+/* ---- survey ingest (no GPU) -------------------------------------------- */
+
+/* One kept survey object (the reference's struct survey_data,
+ * src/vars.h): Cartesian position after the Coord_Type conversion,
+ * redshift, weight (FKP-weighted after capow_survey_fkp), weight_pv, nbar,
+ * pv and pverr (both zero unless Momentum != 0; pv zero for randoms, whose
+ * file holds no logdist column). */
+typedef struct capow_survey_obj {
+  double x, y, z, redshift, weight, weight_pv, nbar, pv, pverr;
+} capow_survey_obj;
+
+/* The survey tags of run_parameters.dat (src/read_params.c). */
+typedef struct capow_survey_params {
+  int32_t coord_type;      /* Coord_Type: 0 Cartesian, 1 RA/Dec degrees, else radians */
+  int32_t momentum;        /* Momentum (survey code)                       */
+  int32_t nbar_column;     /* NBAR_Column: > 0 keeps the file's nbar        */
+  int32_t fkp_column;      /* FKP_Column: < 0 computes FKP weights          */
+  double omega_m;          /* Omega_m                                       */
+  double redmin_in, redmax_in; /* REDMININ REDMAXIN                          */
+  double sky_area;         /* SkyArea (deg^2)                               */
+  double fkp_pk, fkp_pk_mom, fkp_pk_cross;  /* FKP_Pk FKP_Pk_mom FKP_Pk_cross */
+} capow_survey_params;
+
+/* Sums of src/main.c:149-204 (randoms' sums already scaled by alpha). */
+typedef struct capow_survey_totals {
+  double data_nbw, data_nbwsq, data_nbsqwsq, data_nbsqwsq_pv, data_vr, data_vrsq;
+  double rand_nbw, rand_nbwsq, rand_nbsqwsq, rand_nbsqwsq_pv;
+  double alpha, shot, norm;
+} capow_survey_totals;
+
+/* Natural cubic spline (gsl_interp_cspline in the reference): m receives the
+ * n second derivatives; eval returns CAPOW_ERR_ARG outside [x0, x(n-1)],
+ * where GSL reports a domain error. */
+int capow_cspline_init(int32_t n, const double *x, const double *y, double *m);
+int capow_cspline_eval(int32_t n, const double *x, const double *y, const double *m, double xq,
+                       double *out);
+
+/* r(z) = c/100 int_0^z dz'/sqrt(Om (1+z')^3 + 1 - Om) in Mpc/h at n nodes
+ * z_i = i zmax/(n-1); replaces comoving_distance (src/read_data.c:363-378)
+ * over the table of create_grids (src/main.c:1230-1242). */
+int capow_zr_table(double omega_m, int32_t n, double zmax, double *z, double *r);
+
+/* The table at 10000 nodes on [0, 10] with its two splines, r(z) and z(r)
+ * (src/main.c:1230-1245: dist_spline, red_spline). */
+typedef struct capow_cosmo capow_cosmo;
+int capow_cosmo_create(double omega_m, capow_cosmo **out);
+int capow_cosmo_destroy(capow_cosmo *c);
+int capow_cosmo_r_of_z(const capow_cosmo *c, double z, double *r);
+int capow_cosmo_z_of_r(const capow_cosmo *c, double r, double *z);
+
+/* Reads a survey data (randoms = 0: twelve numbers a line) or randoms file
+ * (eight or more) the way read_survey_serial_ascii does
+ * (src/read_data.c:184-361): leading '#' lines skipped, nbar from the fifth
+ * column, the redshift cut [REDMININ, REDMAXIN] (Coord_Type 0 through z(r)),
+ * zero-weight objects dropped for Coord_Type != 0, (RA, Dec, z) converted to
+ * Cartesian, and an object outside the grid shrunk by the interpolation
+ * reach fatal (CAPOW_ERR_OUT_OF_BOX; bad = x, y, z, redshift, bad_line =
+ * the line).  A malformed line is CAPOW_ERR_PARSE, more than nobj_max kept
+ * objects CAPOW_ERR_NOMEM, a distance or redshift outside the z-r table
+ * CAPOW_ERR_ARG.  zrange (in/out) widens to the kept redshifts (the
+ * reference's REDMIN, REDMAX); xyz_range (or NULL) gets the X, Y, Z
+ * low/high pairs the reference prints.  *rows is released with
+ * capow_free_survey. */
+int capow_read_survey(const char *path, int32_t randoms, const capow_params *g,
+                      const capow_survey_params *sp, const capow_cosmo *cosmo, int64_t nobj_max,
+                      capow_survey_obj **rows, int64_t *n_keep, int64_t *n_read, double zrange[2],
+                      double xyz_range[6], char *bad_line, int32_t bad_len, double bad[4]);
+void capow_free_survey(capow_survey_obj *rows);
+
+/* compute_nbar (src/read_data.c:383-440): a histogram of the first ndata
+ * randoms' redshifts in ceil((REDMAXIN - REDMININ)/0.002) bins over zrange,
+ * divided by the shell volumes over SkyArea, a natural spline through the
+ * bin centres with zero end points, evaluated at every object.  znbar and
+ * nbar (capacity cap >= capow_survey_nbar_bins + 2) receive the table. */
+int32_t capow_survey_nbar_bins(const capow_survey_params *sp);
+int capow_survey_nbar(const capow_survey_params *sp, const capow_cosmo *cosmo, const double zrange[2],
+                      capow_survey_obj *data, int64_t ndata, capow_survey_obj *randoms, int64_t nrand,
+                      double *znbar, double *nbar, int32_t cap);
+
+/* compute_fkp (src/read_data.c:444-463), in place. */
+int capow_survey_fkp(const capow_survey_params *sp, capow_survey_obj *o, int64_t n);
+
+/* alpha, shot noise and normalisation (src/main.c:149-204). */
+int capow_survey_sums(const capow_survey_params *sp, const capow_survey_obj *data, int64_t ndata,
+                      const capow_survey_obj *randoms, int64_t nrand, capow_survey_totals *t);
+
+/* Rows for capow_set_particles on a CAPOW_EDGE_SURVEY context: the data,
+ * then (Momentum != 1) the randoms with weight -alpha w
+ * (assign_survey_data, src/read_data.c:464-486, src/main.c:94, 228).
+ * xyzv (4 per row) and w (1 per row) have room for ndata + nrand rows of
+ * dtype. */
+int capow_survey_pack(const capow_survey_params *sp, const capow_survey_obj *data, int64_t ndata,
+                      const capow_survey_obj *randoms, int64_t nrand, double alpha, int32_t dtype,
+                      void *xyzv, void *w, int64_t *n_out);
 
 /* ---- host-side output math (no GPU) ------------------------------------ */
 

@@ -254,6 +254,198 @@ static void check(capow_ctx *ctx, int rc, const char *routine, int line) {
   FatalError(routine, line);
 }
 
+/* Finalisation and output shared by both branches (src/main.c:971-1119). */
+static void write_spectra(const capow_params *p, double shot, double norm, double *pk,
+                          int64_t *nmodes, double *pk2d, int64_t *nmodes2d, const char *outfile) {
+  banner("Outputting power spectra");
+  printf("Writing multipoles to file: %s\n", outfile);
+  if (Output2D) printf("Writing 2D power spectrum to file:%s_2D\n", outfile);
+  fflush(stdout);
+  int32_t last = -1;
+  capow_finalize(p, shot, norm, pk, nmodes, Output2D ? pk2d : NULL, Output2D ? nmodes2d : NULL, &last);
+  if (last >= 0) {
+    double bw = (Maxk - Mink) / (double)NK;
+    double kmax = OutputLog ? pow(10.0, Mink + ((float)last + 0.5) * bw) : Mink + ((float)last + 0.5) * bw;
+    printf("Last full bin: %d (k=%g)\n", last, kmax);
+    fflush(stdout);
+  }
+  int rc = capow_write_output(p, shot, norm, pk, nmodes, Output2D ? pk2d : NULL,
+                              Output2D ? nmodes2d : NULL, outfile, 0);
+  if (rc) {
+    printf("cannot open output file: %s\n", outfile);
+    FatalError("read_data", 142);
+  }
+  const char *pr = getenv("CAPOW_PRECISE_OUTPUT");
+  if (pr && pr[0] == '1') {
+    size_t L = strlen(outfile);
+    char *q = (char *)malloc(L + 16);
+    snprintf(q, L + 16, "%s.precise", outfile);
+    capow_write_output(p, shot, norm, pk, nmodes, Output2D ? pk2d : NULL,
+                       Output2D ? nmodes2d : NULL, q, 1);
+    free(q);
+  }
+}
+
+/* read_survey_serial_ascii for one file (src/read_data.c:184-361), with
+ * the reference's messages for its fatal cases. */
+static capow_survey_obj *read_survey_file(const char *path, int randoms, const capow_params *p,
+                                          const capow_survey_params *sp, const capow_cosmo *cosmo,
+                                          double zr[2], long long *n_out) {
+  printf("Reading: %s\n", path);
+  fflush(stdout);
+  capow_survey_obj *rows = NULL;
+  int64_t n = 0, nread = 0;
+  double xr[6], bad[4] = {0, 0, 0, 0};
+  char line[2000];
+  line[0] = 0;
+  const int rc = capow_read_survey(path, randoms, p, sp, cosmo, NOBJ_Max, &rows, &n, &nread, zr, xr,
+                                   line, (int32_t)sizeof line, bad);
+  if (rc == CAPOW_ERR_IO) {
+    printf("Task %d cannot open input file\n", 0);
+    FatalError("read_data", 113);
+  }
+  if (rc == CAPOW_ERR_PARSE) {
+    printf("Task %d has error reading file: %s\n", 0, line);
+    FatalError("read_data", 102);
+  }
+  if (rc == CAPOW_ERR_OUT_OF_BOX) {
+    printf("Task %d has object out of grid bounds for chosen InterpOrder: x=%lf, y=%lf, z=%lf, red=%lf\n",
+           0, bad[0], bad[1], bad[2], bad[3]);
+    FatalError("read_data", 155);
+  }
+  if (rc == CAPOW_ERR_NOMEM) {
+    printf("Task %d has more objects than NOBJ_Max (%d)\n", 0, NOBJ_Max);
+    FatalError("main", 72);
+  }
+  if (rc) {
+    printf("Task %d has an object outside the redshift-distance table: %s\n", 0, line);
+    FatalError("read_data", 261);
+  }
+  printf("Have %llu objects\n", (unsigned long long)n);
+  printf("%12.6lf <   X  < %12.6lf\n", xr[0], xr[1]);
+  printf("%12.6lf <   Y  < %12.6lf\n", xr[2], xr[3]);
+  printf("%12.6lf <   Z  < %12.6lf\n", xr[4], xr[5]);
+  fflush(stdout);
+  *n_out = (long long)n;
+  return rows;
+}
+
+/* The survey branch of src/main.c:77-94, 149-204, 226-228 and 279-280:
+ * data and randoms read, n(z) and FKP weights on the host, the data and the
+ * -alpha randoms deposited on the GPU, the Yamamoto estimator, and the
+ * shared output. */
+static int run_survey(const char *datafile, const char *randfile, const char *outfile, int fp64) {
+  capow_params p;
+  memset(&p, 0, sizeof p);
+  p.nx = NX; p.ny = NY; p.nz = NZ;
+  p.xmin = XMIN; p.xmax = XMAX; p.ymin = YMIN; p.ymax = YMAX; p.zmin = ZMIN; p.zmax = ZMAX;
+  p.interp_order = InterpOrder; p.do_interlacing = DoInterlacing; p.los = 0;
+  p.momentum = Momentum;
+  p.nk = NK; p.nmu = NMU; p.output_2d = Output2D; p.output_log = OutputLog;
+  p.odd_multipoles = Odd_Multipoles; p.mink = Mink; p.maxk = Maxk;
+  p.omega_m = Omega_m; p.redshift = Redshift;
+  p.precision = fp64 ? CAPOW_FP64 : CAPOW_FP32;
+  p.edge_mode = CAPOW_EDGE_SURVEY;
+  capow_survey_params sp;
+  memset(&sp, 0, sizeof sp);
+  sp.coord_type = Coord_Type; sp.momentum = Momentum; sp.nbar_column = NBAR_Column;
+  sp.fkp_column = FKP_Column; sp.omega_m = Omega_m; sp.redmin_in = REDMININ;
+  sp.redmax_in = REDMAXIN; sp.sky_area = SkyArea; sp.fkp_pk = FKP_Pk;
+  sp.fkp_pk_mom = FKP_Pk_mom; sp.fkp_pk_cross = FKP_Pk_cross;
+  if (Momentum != 0 && Momentum != 1 && Momentum != 3) {
+    printf("\nERROR: survey `Momentum' must be 0, 1 or 3 (2 has no ddg_mom grid).\n\n");
+    FatalError("main", 1197);
+  }
+
+  banner("Allocating grid");
+  printf("Creating redshift-distance lookup table...\n");
+  fflush(stdout);
+  capow_cosmo *cosmo = NULL;
+  if (capow_cosmo_create(Omega_m, &cosmo)) FatalError("main", 1242);
+
+  banner("Reading input data and assigning to grid");
+  double zr[2] = {1.0e30, -1.0e30};
+  long long nd = 0, nr = 0;
+  capow_survey_obj *data = read_survey_file(datafile, 0, &p, &sp, cosmo, zr, &nd);
+  capow_survey_obj *rand = read_survey_file(randfile, 1, &p, &sp, cosmo, zr, &nr);
+  if (NBAR_Column < 0) {
+    printf("Calculating Number Density...\n");
+    const int32_t nb = capow_survey_nbar_bins(&sp);
+    double *zn = (double *)calloc((size_t)nb + 2, sizeof(double));
+    double *nbv = (double *)calloc((size_t)nb + 2, sizeof(double));
+    if (capow_survey_nbar(&sp, cosmo, zr, data, nd, rand, nr, zn, nbv, nb + 2)) {
+      printf("Task 0 cannot compute the number density (fewer randoms than data, or a bin outside the table)\n");
+      FatalError("read_data", 384);
+    }
+    for (int32_t i = 1; i <= nb; i++) printf("%12.6lf, %12.6lf\n", zn[i], nbv[i]);
+    free(zn);
+    free(nbv);
+  }
+  if (FKP_Column < 0) {
+    printf("Computing FKP Weights...\n");
+    capow_survey_fkp(&sp, data, nd);
+    capow_survey_fkp(&sp, rand, nr);
+  }
+
+  banner("Computing shot-noise and normalisation");
+  capow_survey_totals t;
+  capow_survey_sums(&sp, data, nd, rand, nr, &t);
+  if (Momentum == 0) {
+    printf("alpha = %g\n", t.alpha);
+    printf("shot-noise [data, randoms]    = %g, %g\n", t.data_nbwsq, t.rand_nbwsq);
+    printf("normalisation [data, randoms] = %g, %g\n", t.data_nbsqwsq, t.rand_nbsqwsq);
+  } else if (Momentum == 1) {
+    printf("shot-noise [data]    = %g \n", t.data_vrsq);
+    printf("normalisation [data, randoms] = %g, %g\n", t.data_nbsqwsq, t.rand_nbsqwsq);
+  } else {
+    printf("alpha = %g\n", t.alpha);
+    printf("shot-noise    = %g\n", t.data_vr);
+    printf("normalisation = %g\n", t.norm);
+  }
+  fflush(stdout);
+
+  banner("Subtracting off expected density");
+  const int64_t ntot = (int64_t)nd + (int64_t)nr;
+  const size_t es = fp64 ? sizeof(double) : sizeof(float);
+  void *xyzv = malloc((size_t)(ntot > 0 ? ntot : 1) * 4 * es);
+  void *w = malloc((size_t)(ntot > 0 ? ntot : 1) * es);
+  int64_t np = 0;
+  if (!xyzv || !w) FatalError("main", 72);
+  if (capow_survey_pack(&sp, data, nd, rand, nr, t.alpha, fp64 ? CAPOW_DTYPE_F64 : CAPOW_DTYPE_F32,
+                        xyzv, w, &np)) {
+    printf("Task 0 has a data object with zero weight and a velocity\n");
+    FatalError("read_data", 480);
+  }
+  capow_free_survey(data);
+  capow_free_survey(rand);
+  capow_cosmo_destroy(cosmo);
+  capow_ctx *ctx = NULL;
+  check(NULL, capow_create(&p, &ctx), "main", 1123);
+  check(ctx, capow_set_particles(ctx, xyzv, w, np, fp64 ? CAPOW_DTYPE_F64 : CAPOW_DTYPE_F32, CAPOW_HOST),
+        "read_data", 471);
+  capow_assign_stats st;
+  check(ctx, capow_assign(ctx, &st), "read_data", 471);
+  free(xyzv);
+  free(w);
+  printf("Task %d has %llu objects, gridded %lf\n", 0, (unsigned long long)nd, (double)nd);
+  fflush(stdout);
+
+  banner("Fourier transforming overdensity field");
+  banner("Iterating over grid cells");
+  const int nk = NK, nmu = Output2D ? NMU : 1;
+  double *pk = (double *)calloc((size_t)5 * nk, sizeof(double));
+  int64_t *nmodes = (int64_t *)calloc((size_t)nk, sizeof(int64_t));
+  double *pk2d = (double *)calloc((size_t)nk * nmu, sizeof(double));
+  int64_t *nmodes2d = (int64_t *)calloc((size_t)nk * nmu, sizeof(int64_t));
+  const double origin[3] = {X_Origin, Y_Origin, Z_Origin};
+  check(ctx, capow_survey_power(ctx, origin, pk, nmodes), "main", 280);
+  write_spectra(&p, t.shot, t.norm, pk, nmodes, pk2d, nmodes2d, outfile);
+  banner("Cleaning up");
+  capow_destroy(ctx);
+  free(pk); free(nmodes); free(pk2d); free(nmodes2d);
+  return 0;
+}
+
 int main(int argc, char **argv) {
   if (argc < 5) {
     fprintf(stdout, "Input parameters not found\n");
@@ -265,12 +457,13 @@ int main(int argc, char **argv) {
   read_parameterfile(argv[1]);
   check_inputs();
   const char *datafile = argv[2], *outfile = argv[4];
-  if (!Periodic) {
-    printf("\nERROR: only the periodic-box path (Periodic = 1) is built here.\n\n");
-    FatalError("main", 75);
-  }
   const char *pe = getenv("CAPOW_PRECISION");
   const int fp64 = !(pe && (strcmp(pe, "fp32") == 0 || strcmp(pe, "FP32") == 0));
+  if (!Periodic) {
+    if (Survey) return run_survey(datafile, argv[3], outfile, fp64);
+    printf("\nERROR: neither `Periodic' nor `Survey' is set.\n\n");
+    FatalError("main", 75);
+  }
   const char *ee = getenv("CAPOW_EDGE");
   const int edge = (ee && strcmp(ee, "periodic") == 0) ? CAPOW_EDGE_PERIODIC : CAPOW_EDGE_REFERENCE;
 
@@ -336,33 +529,7 @@ int main(int argc, char **argv) {
   b.pk = pk; b.nmodes = nmodes; b.pk2d = pk2d; b.nmodes2d = nmodes2d;
   check(ctx, capow_bin(ctx, &b), "main", 302);
 
-  banner("Outputting power spectra");
-  printf("Writing multipoles to file: %s\n", outfile);
-  if (Output2D) printf("Writing 2D power spectrum to file:%s_2D\n", outfile);
-  fflush(stdout);
-  int32_t last = -1;
-  capow_finalize(&p, shot, norm, pk, nmodes, Output2D ? pk2d : NULL, Output2D ? nmodes2d : NULL, &last);
-  if (last >= 0) {
-    double bw = (Maxk - Mink) / (double)NK;
-    double kmax = OutputLog ? pow(10.0, Mink + ((float)last + 0.5) * bw) : Mink + ((float)last + 0.5) * bw;
-    printf("Last full bin: %d (k=%g)\n", last, kmax);
-    fflush(stdout);
-  }
-  rc = capow_write_output(&p, shot, norm, pk, nmodes, Output2D ? pk2d : NULL,
-                          Output2D ? nmodes2d : NULL, outfile, 0);
-  if (rc) {
-    printf("cannot open output file: %s\n", outfile);
-    FatalError("read_data", 142);
-  }
-  const char *pr = getenv("CAPOW_PRECISE_OUTPUT");
-  if (pr && pr[0] == '1') {
-    size_t L = strlen(outfile);
-    char *q = (char *)malloc(L + 16);
-    snprintf(q, L + 16, "%s.precise", outfile);
-    capow_write_output(&p, shot, norm, pk, nmodes, Output2D ? pk2d : NULL,
-                       Output2D ? nmodes2d : NULL, q, 1);
-    free(q);
-  }
+  write_spectra(&p, shot, norm, pk, nmodes, pk2d, nmodes2d, outfile);
   banner("Cleaning up");
   capow_destroy(ctx);
   free(pk); free(nmodes); free(pk2d); free(nmodes2d);

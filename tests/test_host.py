@@ -106,8 +106,35 @@ def test_catalogue_errors(tmp_path):
     assert "Fatal Error at line 110 in routine read_data" in out
 
 
-def test_survey_mode_is_rejected(tmp_path):
-    pf = write_params(str(tmp_path / "p.dat"), Periodic=0, Survey=1)
+def test_neither_periodic_nor_survey_is_rejected(tmp_path):
+    pf = write_params(str(tmp_path / "p.dat"), Periodic=0, Survey=0)
     rc, out = run(pf, "d", "r", "o")
     assert rc == 1
-    assert "only the periodic-box path" in out
+    assert "neither `Periodic' nor `Survey'" in out
+
+
+def test_survey_ingest_failures(tmp_path):
+    """The survey branch's fatal cases before any GPU work
+    (src/read_data.c:113, 102, 155; src/main.c:72)."""
+    kw = dict(Periodic=0, Survey=1, XMIN=-1000.0, XMAX=1000.0, YMIN=-1000.0, YMAX=1000.0,
+              ZMIN=-1000.0, ZMAX=1000.0, Coord_Type=1, REDMININ=0.0, REDMAXIN=0.5, NOBJ_Max=2)
+    pf = write_params(str(tmp_path / "p.dat"), **kw)
+    rc, out = run(pf, str(tmp_path / "missing"), "r", "o")
+    assert rc == 1 and "Creating redshift-distance lookup table" in out
+    assert "cannot open input file" in out and "Fatal Error at line 113 in routine read_data" in out
+    row = "10 20 %s 1 1e-4 1e-4 0 0 0 0 0 0\n"
+    bad = tmp_path / "bad.txt"
+    bad.write_text(row % 0.1 + "1 2 3\n")
+    rc, out = run(pf, str(bad), "r", "o")
+    assert rc == 1 and "has error reading file: 1 2 3" in out
+    assert "Fatal Error at line 102 in routine read_data" in out
+    far = tmp_path / "far.txt"
+    far.write_text(row % 0.45)     # r(0.45) ~ 1210 Mpc/h > 1000
+    rc, out = run(pf, str(far), "r", "o")
+    assert rc == 1 and "object out of grid bounds for chosen InterpOrder" in out
+    assert "Fatal Error at line 155 in routine read_data" in out
+    many = tmp_path / "many.txt"
+    many.write_text(row % 0.1 * 3)
+    rc, out = run(pf, str(many), "r", "o")
+    assert rc == 1 and "more objects than NOBJ_Max" in out
+    assert "Fatal Error at line 72 in routine main" in out

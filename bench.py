@@ -228,6 +228,106 @@ def run_survey(args, parts, geom):
                 nmodes_total=int(r["nmodes"].sum()), kernels=kern)
 
 
+def run_survey_ingest(n_data=250000, n_rand=750000, nside=256):
+    """Survey ingest on the host (capow_read_survey for a data and a randoms
+    file in (RA, Dec, z), the n(z) estimate, FKP weights, the sums, the
+    packing) and the survey deposit on the GPU (data and -alpha randoms,
+    TSC + interlacing, FP32) of a seeded full-sky catalogue, z in
+    [0.05, 0.6], Omega_m 0.31, on a nside^3 grid of side 4000 Mpc/h."""
+    import ctypes as C
+    import tempfile
+    import torch
+    from capow_amd import api
+    L = api.lib()
+    rng = np.random.default_rng(2024)
+    tmp = tempfile.mkdtemp(prefix="capow_survey_")
+    paths = []
+    for n, ncol in ((n_data, 12), (n_rand, 8)):
+        cols = np.zeros((n, ncol))
+        cols[:, 0] = rng.uniform(0, 360, n)
+        cols[:, 1] = np.degrees(np.arcsin(rng.uniform(-1, 1, n)))
+        cols[:, 2] = rng.uniform(0.05, 0.6, n)
+        cols[:, 3] = rng.uniform(0.5, 1.5, n)
+        cols[:, 4] = 1e-4
+        cols[:, 6] = rng.normal(0, 0.05, n)
+        cols[:, 7] = rng.uniform(0.01, 0.1, n)
+        path = os.path.join(tmp, "cat%d.txt" % ncol)
+        with open(path, "w") as f:
+            f.write("\n".join(" ".join("%.9g" % v for v in row) for row in cols.tolist()) + "\n")
+        paths.append(path)
+    mb = sum(os.path.getsize(q) for q in paths) / 1e6
+    g = api.make_params(nx=nside, ny=nside, nz=nside, xmin=-2000.0, xmax=2000.0, ymin=-2000.0,
+                        ymax=2000.0, zmin=-2000.0, zmax=2000.0, interp_order=3, do_interlacing=1,
+                        momentum=0, nk=40, maxk=0.2, omega_m=0.31, precision=api.FP32,
+                        edge_mode=api.EDGE_SURVEY)
+    sp = api.SurveyParams(coord_type=1, momentum=0, nbar_column=-1, fkp_column=-1, omega_m=0.31,
+                          redmin_in=0.05, redmax_in=0.6, sky_area=41253.0, fkp_pk=1e4,
+                          fkp_pk_mom=1e9, fkp_pk_cross=1e6)
+    t = {}
+    t0 = time.perf_counter()
+    cosmo = C.c_void_p()
+    assert L.capow_cosmo_create(0.31, C.byref(cosmo)) == 0
+    t["zr_table_ms"] = (time.perf_counter() - t0) * 1e3
+    zr = np.array([1e30, -1e30])
+    objs = []
+    for k, path in enumerate(paths):
+        rows, nk_, nr_ = C.POINTER(api.SurveyObj)(), C.c_int64(), C.c_int64()
+        t0 = time.perf_counter()
+        rc = L.capow_read_survey(path.encode(), k, C.byref(g), C.byref(sp), cosmo, 10 ** 8,
+                                 C.byref(rows), C.byref(nk_), C.byref(nr_),
+                                 zr.ctypes.data_as(C.POINTER(C.c_double)), None, None, 0, None)
+        t["read_%s_ms" % ("randoms" if k else "data")] = (time.perf_counter() - t0) * 1e3
+        assert rc == 0, rc
+        objs.append((rows, nk_.value))
+    (d, nd), (r, nr) = objs
+    nb = L.capow_survey_nbar_bins(C.byref(sp))
+    zn, nbar = np.zeros(nb + 2), np.zeros(nb + 2)
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    t0 = time.perf_counter()
+    assert L.capow_survey_nbar(C.byref(sp), cosmo, dp(zr), d, nd, r, nr, dp(zn), dp(nbar), nb + 2) == 0
+    t["nbar_ms"] = (time.perf_counter() - t0) * 1e3
+    t0 = time.perf_counter()
+    assert L.capow_survey_fkp(C.byref(sp), d, nd) == 0 and L.capow_survey_fkp(C.byref(sp), r, nr) == 0
+    tot = api.SurveyTotals()
+    assert L.capow_survey_sums(C.byref(sp), d, nd, r, nr, C.byref(tot)) == 0
+    xyzv = np.zeros((nd + nr, 4), dtype=np.float32)
+    w = np.zeros(nd + nr, dtype=np.float32)
+    npk = C.c_int64()
+    assert L.capow_survey_pack(C.byref(sp), d, nd, r, nr, tot.alpha, api.FP32, xyzv.ctypes.data,
+                               w.ctypes.data, C.byref(npk)) == 0
+    t["fkp_sums_pack_ms"] = (time.perf_counter() - t0) * 1e3
+    host_ms = sum(t.values())
+    ps = api.PowerSpectrum(g)
+    xg = torch.from_numpy(xyzv).cuda()
+    wg = torch.from_numpy(w).cuda()
+    ps.set_particles(xg, wg)
+    ps.assign()                                          # warm-up
+    torch.cuda.synchronize()
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ps.set_particles(xg, wg)
+        ps.assign()
+    torch.cuda.synchronize()
+    dep_ms = (time.perf_counter() - t0) * 1e3 / reps
+    ps.close()
+    L.capow_free_survey(d)
+    L.capow_free_survey(r)
+    L.capow_cosmo_destroy(cosmo)
+    for q in paths:
+        os.remove(q)
+    os.rmdir(tmp)
+    res = {k: round(v, 2) for k, v in t.items()}
+    res.update(workload=("survey ingest: %d data + %d randoms, full sky (RA, Dec, z), z 0.05-0.6, "
+                         "%.0f MB of text, one host thread; deposit of data and -alpha randoms on "
+                         "%d^3, TSC+interlacing, FP32" % (nd, nr, mb, nside)),
+               host_ms=round(host_ms, 1), host_mb_per_s=round(mb / (host_ms * 1e-3), 1),
+               deposit_ms=round(dep_ms, 3), alpha=tot.alpha)
+    log("survey ingest: host %.0f ms (%.0f MB/s), deposit %.2f ms" % (host_ms, res["host_mb_per_s"],
+                                                                      dep_ms))
+    return res
+
+
 def run_bin_variants(parts, geom):
     """Binning of the workload's grids (1024^3, TSC + interlacing, P00/P11/P01)
     with log k bins (OutputLog, 100 bins from 10^-2.3 to 10^0.2) and with
@@ -427,6 +527,7 @@ def main():
     assign_bytes = n_p * 16 + G * ng * 4
 
     survey = None if args.no_survey else run_survey(args, parts32, geom)
+    survey_ingest = None if args.no_survey else run_survey_ingest()
     bin_variants = None if args.no_bin_variants else run_bin_variants(parts32, geom)
     del parts32
     torch.cuda.empty_cache()
@@ -474,6 +575,7 @@ def main():
         "batched_mocks": batch,
         "adjoint": adjoint,
         "survey": survey,
+        "survey_ingest": survey_ingest,
         "bin_variants": bin_variants,
         "cpu_baseline": cpu,
     }

@@ -10,11 +10,18 @@
  * splines (gsl_interp_cspline); here both are written from the mathematics:
  * Gauss-Legendre quadrature on each table interval, accumulated, and the
  * natural cubic spline through its second derivatives (tridiagonal solve). */
+#define _GNU_SOURCE
 #include <errno.h>
+#include <fcntl.h>
 #include <math.h>
+#include <pthread.h>
+#include <sched.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
 
 #include "capow.h"
 
@@ -246,6 +253,88 @@ static int survey_line(const char *line, int randoms, const capow_params *g,
   return 0;
 }
 
+/* One thread's share of the file: whole lines in [beg, end). */
+typedef struct {
+  const char *base;
+  size_t beg, end;
+  int randoms;
+  const capow_params *g;
+  const capow_survey_params *sp;
+  const capow_cosmo *cosmo;
+  const double *lo, *hi;
+  capow_survey_obj *objs;
+  int64_t nkeep, cap, nread;
+  int err;               /* first error in the range (file order), 0 if none */
+  size_t err_line;       /* its offset */
+  double bad[4];
+  double zr[2], xl[3], xh[3];
+} survey_chunk;
+
+static void *survey_range(void *arg) {
+  survey_chunk *c = (survey_chunk *)arg;
+  char buf[2048];
+  char *big = NULL;
+  c->zr[0] = 1.0e30; c->zr[1] = -1.0e30;
+  for (int a = 0; a < 3; a++) { c->xl[a] = 1.0e-30; c->xh[a] = -1.0e30; }
+  size_t i = c->beg;
+  while (i < c->end) {
+    const char *nl = (const char *)memchr(c->base + i, '\n', c->end - i);
+    if (!nl) break;                      /* the unterminated last piece is not read */
+    const size_t len = (size_t)(nl - (c->base + i));
+    if (len == 0) { i++; continue; }     /* empty lines: strtok skips them */
+    char *line = buf;
+    if (len + 1 > sizeof buf) {
+      free(big);
+      big = (char *)malloc(len + 1);
+      if (!big) { c->err = CAPOW_ERR_NOMEM; c->err_line = i; break; }
+      line = big;
+    }
+    memcpy(line, c->base + i, len);
+    line[len] = '\0';
+    capow_survey_obj o;
+    const int st = survey_line(line, c->randoms, c->g, c->sp, c->cosmo, c->lo, c->hi, &o, c->bad);
+    c->nread++;
+    if (st != 0 && st != 1) { c->err = st; c->err_line = i; break; }
+    i += len + 1;
+    if (st == 1) continue;
+    if (c->nkeep == c->cap) {
+      const int64_t nc = c->cap ? 2 * c->cap : 4096;
+      capow_survey_obj *t = (capow_survey_obj *)realloc(c->objs, (size_t)nc * sizeof *t);
+      if (!t) { c->err = CAPOW_ERR_NOMEM; c->err_line = i; break; }
+      c->objs = t;
+      c->cap = nc;
+    }
+    c->objs[c->nkeep++] = o;
+    if (o.redshift < c->zr[0]) c->zr[0] = o.redshift;
+    if (o.redshift > c->zr[1]) c->zr[1] = o.redshift;
+    const double p[3] = {o.x, o.y, o.z};
+    for (int a = 0; a < 3; a++) {
+      if (p[a] < c->xl[a]) c->xl[a] = p[a];
+      if (p[a] > c->xh[a]) c->xh[a] = p[a];
+    }
+  }
+  free(big);
+  return NULL;
+}
+
+static int survey_threads(size_t size) {
+  /* OMP_NUM_THREADS, else the CPUs this process may run on; at most 16,
+   * and at least 1 MB of text per thread */
+  int nt;
+  const char *e = getenv("OMP_NUM_THREADS");
+  if (e && atoi(e) > 0) {
+    nt = atoi(e);
+  } else {
+    cpu_set_t set;
+    CPU_ZERO(&set);
+    nt = sched_getaffinity(0, sizeof set, &set) == 0 ? CPU_COUNT(&set) : 1;
+  }
+  if (nt > 16) nt = 16;
+  const size_t max_nt = size / (1u << 20) + 1;
+  if ((size_t)nt > max_nt) nt = (int)max_nt;
+  return nt < 1 ? 1 : nt;
+}
+
 int capow_read_survey(const char *path, int32_t randoms, const capow_params *g,
                       const capow_survey_params *sp, const capow_cosmo *cosmo, int64_t nobj_max,
                       capow_survey_obj **rows, int64_t *n_keep, int64_t *n_read, double zrange[2],
@@ -255,8 +344,6 @@ int capow_read_survey(const char *path, int32_t randoms, const capow_params *g,
   *rows = NULL;
   *n_keep = 0;
   if (n_read) *n_read = 0;
-  double bad_local[4];
-  if (!bad) bad = bad_local;
   /* src/read_data.c:196-205: the box shrunk by the interpolation reach */
   const double d[3] = {(g->xmax - g->xmin) / (double)g->nx, (g->ymax - g->ymin) / (double)g->ny,
                        (g->zmax - g->zmin) / (double)g->nz};
@@ -267,83 +354,116 @@ int capow_read_survey(const char *path, int32_t randoms, const capow_params *g,
     hi[a] = mx[a] - (g->interp_order - 1) * d[a];
     if (g->do_interlacing) hi[a] -= 0.5 * d[a];
   }
-  FILE *fp = fopen(path, "rb");
-  if (!fp) return CAPOW_ERR_IO;
-  size_t cap = 1 << 16, len = 0;
-  char *line = (char *)malloc(cap);
-  int64_t keep_cap = 1024;
-  capow_survey_obj *out = (capow_survey_obj *)malloc((size_t)keep_cap * sizeof *out);
-  if (!line || !out) {
-    free(line); free(out); fclose(fp);
+  const int fd = open(path, O_RDONLY);
+  if (fd < 0) return CAPOW_ERR_IO;
+  struct stat sb;
+  if (fstat(fd, &sb) != 0) { close(fd); return CAPOW_ERR_IO; }
+  const size_t size = (size_t)sb.st_size;
+  const char *base = NULL;
+  if (size > 0) {
+    base = (const char *)mmap(NULL, size, PROT_READ, MAP_PRIVATE, fd, 0);
+    if (base == MAP_FAILED) { close(fd); return CAPOW_ERR_IO; }
+    madvise((void *)base, size, MADV_SEQUENTIAL);
+  }
+  close(fd);
+  /* Lines are the newline-terminated runs of the file; empty ones are
+   * skipped (strtok on "\n", src/read_data.c:237); the '#' lines before the
+   * first other line are skipped (src/read_data.c:215-217); a last line
+   * without a newline is not read (the reference's chunk loop stops before
+   * it, src/read_data.c:238, 335). */
+  size_t start = 0;
+  while (start < size && base[start] == '#') {
+    const char *nl = (const char *)memchr(base + start, '\n', size - start);
+    start = nl ? (size_t)(nl - base) + 1 : size;
+  }
+  const int nt = survey_threads(size - start);
+  survey_chunk *ch = (survey_chunk *)calloc((size_t)nt, sizeof *ch);
+  pthread_t *th = (pthread_t *)calloc((size_t)nt, sizeof *th);
+  if (!ch || !th) {
+    free(ch); free(th);
+    if (base) munmap((void *)base, size);
     return CAPOW_ERR_NOMEM;
   }
+  size_t prev = start;
+  for (int t = 0; t < nt; t++) {
+    size_t e = (t == nt - 1) ? size : start + (size_t)((unsigned __int128)(size - start) * (t + 1) / nt);
+    if (e < prev) e = prev;
+    while (e < size && e > 0 && base[e - 1] != '\n') e++;
+    survey_chunk *c = &ch[t];
+    c->base = base; c->beg = prev; c->end = e; c->randoms = randoms;
+    c->g = g; c->sp = sp; c->cosmo = cosmo; c->lo = lo; c->hi = hi;
+    prev = e;
+  }
+  int started = 0;
+  for (int t = 1; t < nt; t++) {
+    if (pthread_create(&th[t], NULL, survey_range, &ch[t]) != 0) break;
+    started = t;
+  }
+  survey_range(&ch[0]);
+  for (int t = started + 1; t < nt; t++) survey_range(&ch[t]);
+  for (int t = 1; t <= started; t++) pthread_join(th[t], NULL);
+
+  /* the first error in file order; the objects kept before it decide
+   * whether the sequential reader would have passed NOBJ_Max first */
+  int rc = CAPOW_OK, ferr = -1;
+  int64_t kept = 0, nread = 0;
+  for (int t = 0; t < nt; t++) {
+    kept += ch[t].nkeep;
+    nread += ch[t].nread;
+    if (ch[t].err) { ferr = t; break; }
+  }
+  if (n_read) *n_read = nread;
   double xl[3] = {1.0e-30, 1.0e-30, 1.0e-30}, xh[3] = {-1.0e30, -1.0e30, -1.0e30};
-  int64_t nread = 0, nkeep = 0;
-  int rc = CAPOW_OK, leading = 1, ch;
-  /* Lines are the newline-terminated runs of the file; empty ones are
-   * skipped (strtok on "\n", src/read_data.c:237), leading '#' lines are
-   * skipped (src/read_data.c:215-217), and a last line without a newline is
-   * not read (the reference's chunk loop stops before it,
-   * src/read_data.c:238, 335). */
-  while ((ch = getc(fp)) != EOF) {
-    if (ch != '\n') {
-      if (len + 1 >= cap) {
-        char *t = (char *)realloc(line, cap * 2);
-        if (!t) { rc = CAPOW_ERR_NOMEM; break; }
-        line = t;
-        cap *= 2;
-      }
-      line[len++] = (char)ch;
-      continue;
-    }
-    line[len] = '\0';
-    if (len == 0) {
-      leading = 0; /* the '#' skip ends at the first line not starting with '#' */
-      continue;
-    }
-    len = 0;
-    if (leading && line[0] == '#') continue;
-    leading = 0;
-    capow_survey_obj o;
-    const int s = survey_line(line, randoms, g, sp, cosmo, lo, hi, &o, bad);
-    nread++;
-    if (s == 1) continue;
-    if (s != 0) {
-      rc = s;
-      bad_copy(bad_line, bad_len, line);
-      break;
-    }
-    if (nkeep >= nobj_max) {
-      rc = CAPOW_ERR_NOMEM;
-      bad_copy(bad_line, bad_len, "more objects than NOBJ_Max");
-      break;
-    }
-    if (nkeep == keep_cap) {
-      capow_survey_obj *t = (capow_survey_obj *)realloc(out, (size_t)keep_cap * 2 * sizeof *out);
-      if (!t) { rc = CAPOW_ERR_NOMEM; break; }
-      out = t;
-      keep_cap *= 2;
-    }
-    out[nkeep++] = o;
-    if (o.redshift < zrange[0]) zrange[0] = o.redshift;
-    if (o.redshift > zrange[1]) zrange[1] = o.redshift;
-    const double p[3] = {o.x, o.y, o.z};
+  const int tlast = ferr >= 0 ? ferr : nt - 1;
+  for (int t = 0; t <= tlast; t++) {
+    if (ch[t].nkeep == 0) continue;
+    if (ch[t].zr[0] < zrange[0]) zrange[0] = ch[t].zr[0];
+    if (ch[t].zr[1] > zrange[1]) zrange[1] = ch[t].zr[1];
     for (int a = 0; a < 3; a++) {
-      if (p[a] < xl[a]) xl[a] = p[a];
-      if (p[a] > xh[a]) xh[a] = p[a];
+      if (ch[t].xl[a] < xl[a]) xl[a] = ch[t].xl[a];
+      if (ch[t].xh[a] > xh[a]) xh[a] = ch[t].xh[a];
     }
   }
-  free(line);
-  fclose(fp);
-  if (n_read) *n_read = nread;
   if (xyz_range)
     for (int a = 0; a < 3; a++) { xyz_range[2 * a] = xl[a]; xyz_range[2 * a + 1] = xh[a]; }
+  capow_survey_obj *out = NULL;
+  if (kept > nobj_max) {
+    rc = CAPOW_ERR_NOMEM;
+    bad_copy(bad_line, bad_len, "more objects than NOBJ_Max");
+  } else if (ferr >= 0) {
+    const survey_chunk *c = &ch[ferr];
+    rc = c->err;
+    if (bad) memcpy(bad, c->bad, sizeof c->bad);
+    if (bad_line && bad_len > 0) {
+      const char *l = base + c->err_line;
+      const char *nl = (const char *)memchr(l, '\n', size - c->err_line);
+      size_t n = nl ? (size_t)(nl - l) : size - c->err_line;
+      if (n > (size_t)bad_len - 1) n = (size_t)bad_len - 1;
+      memcpy(bad_line, l, n);
+      bad_line[n] = '\0';
+    }
+  } else {
+    out = (capow_survey_obj *)malloc((size_t)(kept > 0 ? kept : 1) * sizeof *out);
+    if (!out) {
+      rc = CAPOW_ERR_NOMEM;
+    } else {
+      int64_t k = 0;
+      for (int t = 0; t < nt; t++) {
+        if (ch[t].nkeep) memcpy(out + k, ch[t].objs, (size_t)ch[t].nkeep * sizeof *out);
+        k += ch[t].nkeep;
+      }
+    }
+  }
+  for (int t = 0; t < nt; t++) free(ch[t].objs);
+  free(ch);
+  free(th);
+  if (base) munmap((void *)base, size);
   if (rc) {
     free(out);
     return rc;
   }
   *rows = out;
-  *n_keep = nkeep;
+  *n_keep = kept;
   return CAPOW_OK;
 }
 

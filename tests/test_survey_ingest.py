@@ -352,3 +352,47 @@ def test_pack_rows(momentum):
     assert api.lib().capow_survey_pack(C.byref(csurvey(survey(momentum=2))), to_c(d), 1, to_c(r), 2,
                                        0.25, api.FP64, xyzv.ctypes.data, w.ctypes.data,
                                        C.byref(n)) == 5
+
+
+def test_threaded_reader_keeps_file_order(tmp_path, monkeypatch):
+    """The reader splits the text over threads (>= 1 MB each): objects come
+    back in file order, and the reported failure is the first in file order,
+    as the reference's sequential loop meets it."""
+    g, s = grid(), survey()
+    cosmo = Cosmo(0.31)
+    rng = np.random.default_rng(8)
+    n = 60000
+    z = rng.uniform(0.05, 0.4, n)
+    ra, dec = rng.uniform(0, 360, n), rng.uniform(-80, 80, n)
+    lines = [_data_line(float(a), float(b), float(c)) for a, b, c in zip(ra, dec, z)]
+    f = tmp_path / "big.txt"
+    f.write_text("# h\n" + "".join(lines))
+    assert f.stat().st_size > 5 << 20     # five threads of ~12000 lines
+    res = {}
+    for nt in ("1", "8"):
+        monkeypatch.setenv("OMP_NUM_THREADS", nt)
+        rc, objs, zr, nread, _, _ = read(str(f), 0, g, s, cosmo)
+        assert rc == 0 and nread == n
+        res[nt] = (np.array([[o.x, o.y, o.z, o.redshift] for o in objs]), tuple(zr))
+    assert np.array_equal(res["1"][0], res["8"][0]) and res["1"][1] == res["8"][1]
+    assert np.allclose(res["1"][0][:, 3], z[(z >= 0.0) & (z <= 0.5)])
+    bad_parse = "1 2 0.1 1\n"
+    far = _data_line(0.0, 0.0, 0.45)
+    g_small = grid(xmin=-1300.0, xmax=1300.0)
+    monkeypatch.setenv("OMP_NUM_THREADS", "8")
+    for first, second, want in ((bad_parse, far, api.CAPOW_ERR_PARSE),
+                                (far, bad_parse, api.CAPOW_ERR_OUT_OF_BOX)):
+        L2 = list(lines)
+        L2[25000] = first
+        L2[50000] = second
+        f.write_text("".join(L2))
+        rc, _, _, nread, badl, _ = read(str(f), 0, g_small, s, cosmo)
+        assert rc == want
+        if want == api.CAPOW_ERR_PARSE:
+            assert badl == bad_parse.strip()
+    # NOBJ_Max passed before the first failing line: the capacity error wins
+    L2 = list(lines)
+    L2[50000] = bad_parse
+    f.write_text("".join(L2))
+    assert read(str(f), 0, g, s, cosmo, nobj_max=20000)[0] == 3
+    assert read(str(f), 0, g, s, cosmo, nobj_max=55000)[0] == api.CAPOW_ERR_PARSE

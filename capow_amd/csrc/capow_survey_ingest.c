@@ -471,6 +471,26 @@ void capow_free_survey(capow_survey_obj *rows) { free(rows); }
 
 /* ---- n(z), FKP weights, sums -------------------------------------------- */
 
+/* n(z) spline evaluations for objects [i0, i1) of data followed by randoms. */
+typedef struct {
+  int32_t n;
+  const double *x, *y, *m;
+  capow_survey_obj *data;
+  int64_t ndata;
+  capow_survey_obj *randoms;
+  int64_t nrand, i0, i1;
+  int rc;
+} nbar_job;
+
+static void *nbar_eval(void *arg) {
+  nbar_job *j = (nbar_job *)arg;
+  for (int64_t i = j->i0; i < j->i1 && !j->rc; i++) {
+    capow_survey_obj *o = i < j->ndata ? &j->data[i] : &j->randoms[i - j->ndata];
+    j->rc = capow_cspline_eval(j->n, j->x, j->y, j->m, o->redshift, &o->nbar);
+  }
+  return NULL;
+}
+
 int32_t capow_survey_nbar_bins(const capow_survey_params *sp) {
   if (!sp) return -1;
   return (int32_t)ceil((sp->redmax_in - sp->redmin_in) / 0.002); /* src/read_data.c:387 */
@@ -507,10 +527,25 @@ int capow_survey_nbar(const capow_survey_params *sp, const capow_cosmo *cosmo, c
   double *m = (double *)malloc((size_t)(nbins + 2) * sizeof(double));
   if (!m) return CAPOW_ERR_NOMEM;
   int rc = capow_cspline_init(nbins + 2, znbar, nbar, m);
-  for (int64_t i = 0; !rc && i < ndata; i++)
-    rc = capow_cspline_eval(nbins + 2, znbar, nbar, m, data[i].redshift, &data[i].nbar);
-  for (int64_t i = 0; !rc && i < nrand; i++)
-    rc = capow_cspline_eval(nbins + 2, znbar, nbar, m, randoms[i].redshift, &randoms[i].nbar);
+  if (!rc) {
+    /* the evaluations are independent: split data and randoms over threads */
+    const int nt = survey_threads((size_t)((ndata + nrand) / 65536) << 20);
+    nbar_job job[16];
+    pthread_t th[16];
+    for (int t = 0; t < nt; t++) {
+      job[t] = (nbar_job){nbins + 2, znbar, nbar, m, data, ndata, randoms, nrand,
+                          (ndata + nrand) * t / nt, (ndata + nrand) * (t + 1) / nt, 0};
+    }
+    int started = 0;
+    for (int t = 1; t < nt; t++) {
+      if (pthread_create(&th[t], NULL, nbar_eval, &job[t]) != 0) break;
+      started = t;
+    }
+    nbar_eval(&job[0]);
+    for (int t = started + 1; t < nt; t++) nbar_eval(&job[t]);
+    for (int t = 1; t <= started; t++) pthread_join(th[t], NULL);
+    for (int t = 0; t < nt && !rc; t++) rc = job[t].rc;
+  }
   free(m);
   return rc;
 }

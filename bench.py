@@ -318,9 +318,15 @@ def run_survey_ingest(n_data=250000, n_rand=750000, nside=256):
         os.remove(q)
     os.rmdir(tmp)
     res = {k: round(v, 2) for k, v in t.items()}
+    # the readers and the n(z) step use OMP_NUM_THREADS, else the CPUs this
+    # process may run on, at most 16 (the same rule as the library)
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    nthreads = int(env) if env.isdigit() and int(env) > 0 else len(os.sched_getaffinity(0))
+    nthreads = min(nthreads, 16)
     res.update(workload=("survey ingest: %d data + %d randoms, full sky (RA, Dec, z), z 0.05-0.6, "
-                         "%.0f MB of text, one host thread; deposit of data and -alpha randoms on "
-                         "%d^3, TSC+interlacing, FP32" % (nd, nr, mb, nside)),
+                         "%.0f MB of text, up to %d host threads; deposit of data and -alpha "
+                         "randoms on %d^3, TSC+interlacing, FP32" % (nd, nr, mb, nthreads, nside)),
+               host_threads=nthreads,
                host_ms=round(host_ms, 1), host_mb_per_s=round(mb / (host_ms * 1e-3), 1),
                deposit_ms=round(dep_ms, 3), alpha=tot.alpha)
     log("survey ingest: host %.0f ms (%.0f MB/s), deposit %.2f ms" % (host_ms, res["host_mb_per_s"],

@@ -794,10 +794,15 @@ static void launch_rows(BinLaunch& L, size_t lds) {
 
 // Mode counts of the branch-free forms, once per context (they depend on
 // the geometry and the bins only).
+__global__ void k_fill_zero(double* __restrict__ p, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0.0;
+}
+
 int run_bin_counts(BinLaunch& L) {
   if (*L.counts_valid) return CAPOW_OK;
   const int ncnt = L.dev.nk + (L.dev.out2d ? L.dev.nk * L.dev.nmu : 0);
-  hipMemsetAsync(L.counts, 0, (size_t)ncnt * sizeof(double), L.stream);
+  // cleared by a kernel (no memset node when a caller captures the stream)
+  k_fill_zero<<<(ncnt + 255) / 256, 256, 0, L.stream>>>(L.counts, ncnt);
   const int64_t nrows = (int64_t)L.dev.nx * L.dev.ny;
   const int nblk = (int)std::min<int64_t>((nrows + 3) / 4, 4096);
   const size_t lds = (size_t)ncnt * sizeof(double);

@@ -57,7 +57,6 @@ struct capow_ctx {
            *boff = nullptr;
   int count_blocks = 3072;   // bucket-kernel blocks: 12 per CU (measured best at C2)
   double* partials = nullptr;
-  unsigned long long* gmax = nullptr;
   unsigned int* errs = nullptr;
   double* stats = nullptr;
   // fft
@@ -181,6 +180,13 @@ static int pick_tile(int n, int tmax, bool even = false) {
 
 // check_inputs (src/read_params.c:294-428) for the periodic tags, plus the
 // shapes the kernels handle.
+static int validate(const capow_params* p, std::string& m);
+
+namespace capow {
+int validate_params(const capow_params* p, std::string& m) { return validate(p, m); }
+void set_global_error(const std::string& m) { g_create_err = m; }
+}  // namespace capow
+
 static int validate(const capow_params* p, std::string& m) {
   if (p->los < 0 || p->los > 3) { m = "`LOS' is neither 0 (real-space), 1, 2 nor 3."; return CAPOW_ERR_ARG; }
   if (p->interp_order < 1 || p->interp_order > 3) { m = "`InterpOrder' is neither 1, 2 nor 3."; return CAPOW_ERR_ARG; }
@@ -236,6 +242,155 @@ static std::vector<T> twiddles(int64_t N) {
   }
   return v;
 }
+
+// One table entry of axis `axis` (0 x, 1 y, 2 z), as src/main.c:329-352
+// and 382-389 form them: frequency with the float casts, the window
+// deconvolution pow(1/sinc, 2 InterpOrder) and the interlace phase.
+static void bin_table_entry(int axis, unsigned long long N, int i, double s, double d, int po,
+                            double* f_out, double* cor, double* c_ph, double* s_ph) {
+  double f;
+  if (axis == 2) f = (float)i * s;
+  else if ((unsigned long long)i <= N / 2) f = (float)i * s;
+  else f = ((float)i - (float)N) * s;
+  *f_out = f;
+  const double a = M_PI * f * d;
+  double sn = 1.0;
+  if (f != 0.0) sn = sin(a) / a;
+  *cor = pow(1.0 / sn, po);
+  *c_ph = cos(a);
+  *s_ph = sin(a);
+}
+
+namespace capow {
+
+// Table layout: fx[NX] fy[NY] fz[nzh] corx[NX] cory[NY] corz[nzh]
+// phx[2 NX] phy[2 NY] phz[2 nzh]; entry idx computed on its own (the
+// interlacing factor 0.25 of src/main.c:395 folded into corz).
+double bin_table_at(const capow_params& p, size_t idx) {
+  const unsigned long long N[3] = {(unsigned long long)p.nx, (unsigned long long)p.ny,
+                                   (unsigned long long)p.nz};
+  const double d[3] = {(p.xmax - p.xmin) / (double)N[0], (p.ymax - p.ymin) / (double)N[1],
+                       (p.zmax - p.zmin) / (double)N[2]};
+  const size_t cnt[3] = {(size_t)N[0], (size_t)N[1], (size_t)(N[2] / 2 + 1)};
+  const size_t sect = cnt[0] + cnt[1] + cnt[2];
+  // 0: frequencies, 1: deconvolution, 2: phases (two doubles per entry)
+  int kind, axis = 0;
+  size_t r;
+  if (idx < sect) { kind = 0; r = idx; }
+  else if (idx < 2 * sect) { kind = 1; r = idx - sect; }
+  else { kind = 2; r = idx - 2 * sect; }
+  const size_t per = kind == 2 ? 2 : 1;
+  while (axis < 2 && r >= cnt[axis] * per) r -= cnt[axis++] * per;
+  const int i = (int)(r / per);
+  const double s = 1.0 / ((float)N[axis] * d[axis]);
+  double f, cor, c, sn;
+  bin_table_entry(axis, N[axis], i, s, d[axis], 2 * p.interp_order, &f, &cor, &c, &sn);
+  if (kind == 0) return f;
+  if (kind == 1) return (axis == 2 && p.do_interlacing) ? cor * 0.25 : cor;
+  return (r % 2) ? sn : c;
+}
+
+size_t bin_table_len(const capow_params& p) { return 4 * (size_t)(p.nx + p.ny + p.nz / 2 + 1); }
+
+void fill_bin_tables(const capow_params& p, double* tab) {
+  const size_t n = bin_table_len(p);
+  for (size_t i = 0; i < n; i++) tab[i] = bin_table_at(p, i);
+}
+
+int make_bin_plan(const capow_params& p, int64_t rowc, bool mirror_allowed, int ncu, BinPlan& bp,
+                  std::string& m) {
+  memset(&bp, 0, sizeof(bp));
+  const unsigned long long NX = p.nx, NY = p.ny, NZ = p.nz;
+  const double dx = (p.xmax - p.xmin) / (double)NX, dy = (p.ymax - p.ymin) / (double)NY,
+               dz = (p.zmax - p.zmin) / (double)NZ;
+  const double sx = 1.0 / ((float)NX * dx), sy = 1.0 / ((float)NY * dy);
+  const int nzh = (int)(NZ / 2 + 1);
+  const size_t ofx = 0, ofy = NX, ofz = NX + NY, ocx = ofz + nzh, ocy = ocx + NX, ocz = ocy + NY,
+               opx = ocz + nzh, opy = opx + 2 * NX, opz = opy + 2 * NY, total = opz + 2 * nzh;
+  bp.ntab = total;
+  int fa, fb;
+  field_exps(p.momentum, &fa, &fb);
+  if (p.edge_mode == CAPOW_EDGE_SURVEY) fb = (p.momentum == 3) ? 1 : -1;
+  const bool hasB = fb >= 0;
+  // mirror-group binning: f(N - i) = -f(i) and cor(N - i) = cor(i) exactly
+  // on x and y, even NX and NY, and mu's sign irrelevant (line of sight
+  // along z, or even multipoles only)
+  {
+    const int po = 2 * p.interp_order;
+    bool sym = NX % 2 == 0 && NY % 2 == 0;
+    double f1, c1, f2, c2, t0, t1;
+    for (unsigned long long i = 1; sym && 2 * i < NX; i++) {
+      bin_table_entry(0, NX, (int)i, sx, dx, po, &f1, &c1, &t0, &t1);
+      bin_table_entry(0, NX, (int)(NX - i), sx, dx, po, &f2, &c2, &t0, &t1);
+      sym = f2 == -f1 && c2 == c1;
+    }
+    for (unsigned long long j = 1; sym && 2 * j < NY; j++) {
+      bin_table_entry(1, NY, (int)j, sy, dy, po, &f1, &c1, &t0, &t1);
+      bin_table_entry(1, NY, (int)(NY - j), sy, dy, po, &f2, &c2, &t0, &t1);
+      sym = f2 == -f1 && c2 == c1;
+    }
+    const bool mu_even = !p.odd_multipoles || (p.los != 1 && p.los != 2);
+    bp.mirror = sym && mu_even && mirror_allowed;
+  }
+  BinArgsDev& b = bp.b;
+  b.nx = (int)NX; b.ny = (int)NY; b.nzh = nzh;
+  b.fd_nzh.init((uint32_t)nzh);
+  b.rowc = (int)rowc;
+  b.fd_rowc.init((uint32_t)rowc);
+  b.nk = p.nk; b.nmu = p.output_2d ? p.nmu : 1;
+  b.out2d = p.output_2d; b.odd = p.odd_multipoles; b.logbins = p.output_log;
+  b.los = p.los; b.interlace = p.do_interlacing; b.hasB = hasB;
+  b.mink = p.mink;
+  b.binwidth = (p.maxk - p.mink) / (double)p.nk;  // src/main.c:305
+  b.inv_binwidth = 1.0 / b.binwidth;
+  double nyx = M_PI / dx, nyy = M_PI / dy, nyz = M_PI / dz, mn = nyx;
+  if (nyy < mn) mn = nyy;
+  if (nyz < mn) mn = nyz;
+  b.min_nyq = mn;
+  b.ofx = (int64_t)ofx; b.ofy = (int64_t)ofy; b.ofz = (int64_t)ofz;
+  b.ocx = (int64_t)ocx; b.ocy = (int64_t)ocy; b.ocz = (int64_t)ocz;
+  b.opx = (int64_t)opx; b.opy = (int64_t)opy; b.opz = (int64_t)opz;
+  int32_t ns, codes[3];
+  capow_spectra(&p, &ns, codes);
+  b.nspec = ns;
+  for (int s = 0; s < 3; s++) bp.codes[s] = s < ns ? codes[s] : 0;
+  for (int s = 0; s < ns; s++) b.kind[s] = spec_kind(codes[s], p.momentum);
+  const int nl = p.odd_multipoles ? 5 : 3;
+  const int n2d = p.output_2d ? p.nk * p.nmu : 0;
+  bp.per_copy = ns * nl * p.nk + ns * n2d + p.nk + n2d;
+  const size_t bytes = (size_t)bp.per_copy * 8;
+  const size_t lds_max = 160 * 1024;
+  if (bytes > lds_max) {
+    m = "NK/NMU too large for the LDS bin arrays";
+    return CAPOW_ERR_UNSUPPORTED;
+  }
+  // one wavefront per private bin copy, as many as fit in one CU's LDS
+  // beside the z tables (up to 8); a persistent grid of one workgroup per CU.
+  // The branch-free row form (linear bins, no 2-D bins) and the mirror-group
+  // form (linear or log bins) keep only the sums in LDS: their mode counts
+  // are computed once.  (k, mu) bins in the mirror form (line of sight
+  // along z, where the four mirror modes share mu): a 1-D pass with per-k
+  // bounds, then one int64 (k, mu) table per workgroup.
+  const size_t ztab = (size_t)nzh * 4 * 8;
+  bp.row_form = bytes + ztab <= lds_max;
+  bp.mirror2d = bp.row_form && bp.mirror && b.out2d && p.los != 1 && p.los != 2 &&
+                (size_t)ns * n2d * 8 + ztab <= lds_max &&
+                (size_t)(ns * nl + 1) * p.nk * 8 + ztab <= lds_max;
+  const bool fast = bp.row_form && !b.out2d && (!b.logbins || bp.mirror);
+  const size_t lbytes = bp.mirror2d ? (size_t)(ns * nl + 1) * p.nk * 8
+                                    : (fast ? (size_t)ns * nl * p.nk * 8 : bytes);
+  const int copies = (int)((lds_max - (bp.row_form ? ztab : 0)) / lbytes);
+  b.copies = copies > 8 ? 8 : copies;
+  const bool fp64 = p.precision == CAPOW_FP64;
+  const int64_t modes = (int64_t)NX * NY * rowc;
+  const int64_t per_wg = (int64_t)64 * b.copies * (fp64 ? 4 : 8);   // modes per iteration
+  const int64_t nbatch = ((int64_t)NX * NY + 63) / 64;              // row form: 64-row batches
+  const int64_t want = bp.row_form ? (nbatch + b.copies - 1) / b.copies : (modes + per_wg - 1) / per_wg;
+  bp.nwg = (int)(want < ncu ? want : ncu);
+  return CAPOW_OK;
+}
+
+}  // namespace capow
 
 static int setup(capow_ctx* c) {
   const capow_params& p = c->p;
@@ -356,7 +511,6 @@ static int setup(capow_ctx* c) {
   HIPCHK(c, hipMalloc((void**)&c->bsum, (size_t)(nblk + 1) * 4));
   HIPCHK(c, hipMalloc((void**)&c->boff, (size_t)(nblk + 1) * 4));
   HIPCHK(c, hipMalloc((void**)&c->partials, (size_t)c->count_blocks * STAT_N * 8));
-  HIPCHK(c, hipMalloc((void**)&c->gmax, 2 * sizeof(unsigned long long)));
   HIPCHK(c, hipMalloc((void**)&c->errs, 4 * sizeof(unsigned int)));
   HIPCHK(c, hipMalloc((void**)&c->stats, STAT_TOTAL * 8));
   HIPCHK(c, hipHostMalloc((void**)&c->h_stats, STAT_TOTAL * 8, hipHostMallocDefault));
@@ -381,122 +535,36 @@ static int setup(capow_ctx* c) {
     }
   }
 
-  // binning tables, built exactly as src/main.c:305-352 / 382-389
+  // binning plan and tables (src/main.c:305-352, 382-389)
   {
-    const unsigned long long NX = p.nx, NY = p.ny, NZ = p.nz;
-    const double dx = g.d[0], dy = g.d[1], dz = g.d[2];
-    double sx = 1.0 / ((float)NX * dx), sy = 1.0 / ((float)NY * dy), sz = 1.0 / ((float)NZ * dz);
-    const int nzh = g.nzh;
-    // layout: fx[NX] fy[NY] fz[nzh] corx corY corz phx(2NX) phy(2NY) phz(2nzh)
-    std::vector<double> tab;
-    size_t ofx = 0, ofy = NX, ofz = NX + NY, ocx = ofz + nzh, ocy = ocx + NX, ocz = ocy + NY,
-           opx = ocz + nzh, opy = opx + 2 * NX, opz = opy + 2 * NY, total = opz + 2 * nzh;
-    tab.resize(total);
-    const int po = 2 * p.interp_order;
-    auto fill = [&](int axis, size_t of, size_t oc, size_t op, unsigned long long N, int cnt,
-                    double s, double d) {
-      for (int i = 0; i < cnt; i++) {
-        double f;
-        if (axis == 2) f = (float)i * s;
-        else if ((unsigned long long)i <= N / 2) f = (float)i * s;
-        else f = ((float)i - (float)N) * s;
-        tab[of + i] = f;
-        double a = M_PI * f * d;
-        double sn = 1.0;
-        if (f != 0.0) sn = sin(a) / a;
-        tab[oc + i] = pow(1.0 / sn, po);
-        tab[op + 2 * i] = cos(a);
-        tab[op + 2 * i + 1] = sin(a);
-      }
-    };
-    fill(0, ofx, ocx, opx, NX, (int)NX, sx, dx);
-    fill(1, ofy, ocy, opy, NY, (int)NY, sy, dy);
-    fill(2, ofz, ocz, opz, NZ, nzh, sz, dz);
-    if (p.do_interlacing)  // grid_cor *= 0.25 (src/main.c:395), folded into the z table
-      for (int k = 0; k < nzh; k++) tab[ocz + k] *= 0.25;
-    // mirror-group binning: f(N - i) = -f(i) and cor(N - i) = cor(i) exactly
-    // on x and y, even NX and NY, and mu's sign irrelevant (line of sight
-    // along z, or even multipoles only)
-    {
-      bool sym = NX % 2 == 0 && NY % 2 == 0;
-      for (unsigned long long i = 1; sym && 2 * i < NX; i++)
-        sym = tab[ofx + NX - i] == -tab[ofx + i] && tab[ocx + NX - i] == tab[ocx + i];
-      for (unsigned long long j = 1; sym && 2 * j < NY; j++)
-        sym = tab[ofy + NY - j] == -tab[ofy + j] && tab[ocy + NY - j] == tab[ocy + j];
-      const bool mu_even = !p.odd_multipoles || (p.los != 1 && p.los != 2);
-      c->bin_mirror = sym && mu_even && c->bin_mirror_env != 0;
-    }
-    HIPCHK(c, hipMalloc((void**)&c->bin_tabs, total * 8));
-    HIPCHK(c, hipMemcpy(c->bin_tabs, tab.data(), total * 8, hipMemcpyHostToDevice));
-    BinArgsDev& b = c->bin;
-    memset(&b, 0, sizeof(b));
-    b.nx = (int)NX; b.ny = (int)NY; b.nzh = nzh;
-    b.fd_nzh.init((uint32_t)nzh);
-    b.rowc = (int)g.rowc;
-    b.fd_rowc.init((uint32_t)g.rowc);
-    b.nk = p.nk; b.nmu = p.output_2d ? p.nmu : 1;
-    b.out2d = p.output_2d; b.odd = p.odd_multipoles; b.logbins = p.output_log;
-    b.los = p.los; b.interlace = p.do_interlacing; b.hasB = hasB;
-    b.mink = p.mink;
-    b.binwidth = (p.maxk - p.mink) / (double)p.nk;  // src/main.c:305
-    b.inv_binwidth = 1.0 / b.binwidth;
-    double nyx = M_PI / dx, nyy = M_PI / dy, nyz = M_PI / dz, mn = nyx;
-    if (nyy < mn) mn = nyy;
-    if (nyz < mn) mn = nyz;
-    b.min_nyq = mn;
-    b.ofx = (int64_t)ofx; b.ofy = (int64_t)ofy; b.ofz = (int64_t)ofz;
-    b.ocx = (int64_t)ocx; b.ocy = (int64_t)ocy; b.ocz = (int64_t)ocz;
-    b.opx = (int64_t)opx; b.opy = (int64_t)opy; b.opz = (int64_t)opz;
-    int32_t ns, codes[3];
-    capow_spectra(&p, &ns, codes);
-    c->nspec = ns;
-    b.nspec = ns;
-    for (int s = 0; s < ns; s++) {
-      c->codes[s] = codes[s];
-      b.kind[s] = spec_kind(codes[s], p.momentum);
-    }
-    const int nl = p.odd_multipoles ? 5 : 3;
-    const int n2d = p.output_2d ? p.nk * p.nmu : 0;
-    c->bin_per_copy = ns * nl * p.nk + ns * n2d + p.nk + n2d;
-    const size_t bytes = (size_t)c->bin_per_copy * 8;
-    const size_t lds_max = 160 * 1024;
-    if (bytes > lds_max)
-      return fail(c, CAPOW_ERR_UNSUPPORTED, "NK/NMU too large for the LDS bin arrays");
-    // one wavefront per private bin copy, as many as fit in one CU's LDS
-    // beside the z tables (up to 8); a persistent grid of one workgroup per CU.
-    // The branch-free row form (linear bins, no 2-D bins) keeps only the
-    // sums in LDS: its mode counts are computed once per context.
-    const size_t ztab = (size_t)nzh * 4 * 8;
-    c->bin_row_form = bytes + ztab <= lds_max;
-    // the branch-free row form (linear bins) and the mirror-group form
-    // (linear or log bins) keep only the sums in LDS
-    // (k, mu) bins in the mirror form (line of sight along z, where the four
-    // mirror modes share mu): a 1-D pass with per-k bounds, then one int64
-    // (k, mu) table per workgroup
-    c->bin_mirror2d = c->bin_row_form && c->bin_mirror && b.out2d && p.los != 1 && p.los != 2 &&
-                      (size_t)ns * n2d * 8 + ztab <= lds_max &&
-                      (size_t)(ns * nl + 1) * p.nk * 8 + ztab <= lds_max;
-    const bool fast = c->bin_row_form && !b.out2d && (!b.logbins || c->bin_mirror);
-    const size_t lbytes = c->bin_mirror2d ? (size_t)(ns * nl + 1) * p.nk * 8
-                                          : (fast ? (size_t)ns * nl * p.nk * 8 : bytes);
-    int copies = (int)((lds_max - (c->bin_row_form ? ztab : 0)) / lbytes);
-    b.copies = copies > 8 ? 8 : copies;
     int dev = 0, ncu = 256;
     if (hipGetDevice(&dev) == hipSuccess)
       hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
     if (ncu < 1) ncu = 256;
     c->ncu = ncu;
-    const int64_t modes = (int64_t)NX * NY * g.rowc;
-    const int64_t per_wg = (int64_t)64 * b.copies * (c->fp64 ? 4 : 8);   // modes per iteration
-    const int64_t nbatch = ((int64_t)NX * NY + 63) / 64;                   // row form: 64-row batches
-    const int64_t want = c->bin_row_form ? (nbatch + b.copies - 1) / b.copies
-                                         : (modes + per_wg - 1) / per_wg;
-    c->bin_nwg = (int)(want < ncu ? want : ncu);
+    BinPlan bp;
+    std::string m;
+    const int rc = make_bin_plan(p, g.rowc, c->bin_mirror_env != 0, ncu, bp, m);
+    if (rc) return fail(c, rc, m);
+    std::vector<double> tab(bp.ntab);
+    fill_bin_tables(p, tab.data());
+    HIPCHK(c, hipMalloc((void**)&c->bin_tabs, bp.ntab * 8));
+    HIPCHK(c, hipMemcpy(c->bin_tabs, tab.data(), bp.ntab * 8, hipMemcpyHostToDevice));
+    c->bin = bp.b;
+    c->bin_mirror = bp.mirror;
+    c->bin_row_form = bp.row_form;
+    c->bin_mirror2d = bp.mirror2d;
+    c->nspec = bp.b.nspec;
+    for (int s = 0; s < 3; s++) c->codes[s] = bp.codes[s];
+    c->bin_per_copy = bp.per_copy;
+    c->bin_nwg = bp.nwg;
+    const size_t bytes = (size_t)bp.per_copy * 8;
+    const int n2d = p.output_2d ? p.nk * p.nmu : 0;
     HIPCHK(c, hipMalloc((void**)&c->bin_partials, (size_t)c->bin_nwg * bytes));
     HIPCHK(c, hipMalloc((void**)&c->bin_out, bytes));
     HIPCHK(c, hipMalloc((void**)&c->bin_counts, (size_t)(p.nk + n2d) * 8));
     if (c->bin_mirror2d) {
-      HIPCHK(c, hipMalloc((void**)&c->bin_partials2, (size_t)c->bin_nwg * ns * n2d * 8));
+      HIPCHK(c, hipMalloc((void**)&c->bin_partials2, (size_t)c->bin_nwg * bp.b.nspec * n2d * 8));
       HIPCHK(c, hipMalloc((void**)&c->bin_qscale, (size_t)2 * p.nk * 8));
     }
     c->bin_counts_valid = false;
@@ -509,7 +577,7 @@ static void release(capow_ctx* c) {
   for (int s = 0; s < 4; s++) if (c->grid[s]) hipFree(c->grid[s]);
   for (int s = 0; s < 4; s++) if (c->scratch[s]) hipFree(c->scratch[s]);
   void* ps[] = {c->own_parts, c->own_w, c->sorted, c->sorted_w, c->counts, c->offsets,
-                c->cursor, c->bsum, c->boff, c->partials, c->gmax, c->errs, c->stats,
+                c->cursor, c->bsum, c->boff, c->partials, c->errs, c->stats,
                 c->tw[0], c->tw[1], c->tw[2], c->bin_tabs, c->bin_partials, c->bin_out,
                 c->bin_counts, c->bin_partials2, c->bin_qscale};
   for (void* q : ps) if (q) hipFree(q);
@@ -645,7 +713,6 @@ static int assign_enqueue(capow_ctx* c, double* h_stats, unsigned int* h_errs, b
   Geom& g = c->g;
   g.has_w = c->wts != nullptr;
   HIPCHK(c, hipMemsetAsync(c->counts, 0, (size_t)g.nbuckets * 4, c->stream));
-  HIPCHK(c, hipMemsetAsync(c->gmax, 0, 2 * sizeof(unsigned long long), c->stream));
   HIPCHK(c, hipMemsetAsync(c->errs, 0, 4 * sizeof(unsigned int), c->stream));
   DepositArgs a;
   a.g = &g;
@@ -662,7 +729,6 @@ static int assign_enqueue(capow_ctx* c, double* h_stats, unsigned int* h_errs, b
   a.bsum = c->bsum;
   a.boff = c->boff;
   a.partials = c->partials;
-  a.gmax = c->gmax;
   a.errs = c->errs;
   a.stats = c->stats;
   for (int s = 0; s < 4; s++) a.grid[s] = c->grid[s];

@@ -631,8 +631,17 @@ int capow_survey_pack(const capow_survey_params *sp, const capow_survey_obj *dat
       double v = 0.0, wt = pass ? -alpha * o->weight : o->weight;
       if (!pass && sp->momentum == 1) v = o->pv;
       if (!pass && sp->momentum == 3) {
-        if (o->weight == 0.0) return CAPOW_ERR_ARG;
-        v = o->weight_pv * o->pv / o->weight;
+        /* B = weight_pv pv is carried as w v.  A zero-weight object (kept
+         * for Coord_Type 0, src/read_data.c:259-266) has weight_pv = 0
+         * too (tw, or weight tp tc after FKP, src/read_data.c:460), so it
+         * deposits nothing into either field, as in the reference.  A
+         * nonzero weight_pv pv on a zero weight cannot be carried. */
+        if (o->weight == 0.0) {
+          if (o->weight_pv * o->pv != 0.0) return CAPOW_ERR_UNSUPPORTED;
+          v = 0.0;
+        } else {
+          v = o->weight_pv * o->pv / o->weight;
+        }
       }
       const double row[4] = {o->x, o->y, o->z, v};
       if (dtype == CAPOW_DTYPE_F64) {

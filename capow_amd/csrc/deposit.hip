@@ -297,13 +297,11 @@ __global__ __launch_bounds__(BK_T) void k_bucket(const T* __restrict__ parts,
                                                  T* __restrict__ sorted,
                                                  T* __restrict__ sorted_w, int64_t cap,
                                                  double* __restrict__ partials,
-                                                 unsigned long long* __restrict__ gmax,
                                                  unsigned int* __restrict__ errs) {
   __shared__ uint32_t hkey[BK_HS];
   __shared__ uint32_t hcnt[BK_HS];   // copies per slot; in the scatter pass, then the slot's base
   __shared__ double sh[BK_T];
   double s_ngrid = 0, s_w = 0, s_v1 = 0, s_v2 = 0, s_v3 = 0, s_v4 = 0, s_out = 0;
-  double m_a = 0, m_b = 0;
   unsigned int oob = 0, reach_bad = 0;
   const int64_t nchunks = (n + BK_CHUNK - 1) / BK_CHUNK;
   for (int64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
@@ -360,10 +358,6 @@ __global__ __launch_bounds__(BK_T) void k_bucket(const T* __restrict__ parts,
         s_v2 += (double)(w * v * v);
         s_v3 += (double)(w * v * v * v);
         s_v4 += (double)(w * v * v * v * v);
-        if (c.acc_main || c.acc_int) {
-          m_a = fmax(m_a, fabs((double)(w * vpow<T>(v, g.fa))));
-          if (g.fb >= 0) m_b = fmax(m_b, fabs((double)(w * vpow<T>(v, g.fb))));
-        }
       }
     }
     __syncthreads();
@@ -398,17 +392,13 @@ __global__ __launch_bounds__(BK_T) void k_bucket(const T* __restrict__ parts,
       double t = block_sum<BK_T>(r[q], sh);
       if (threadIdx.x == 0) partials[(int64_t)blockIdx.x * STAT_N + q] = t;
     }
-    // max |field| (non-negative doubles order as their bit patterns)
-    atomicMax(&gmax[0], (unsigned long long)__double_as_longlong(m_a));
-    atomicMax(&gmax[1], (unsigned long long)__double_as_longlong(m_b));
     if (oob) atomicAdd(&errs[0], oob);
     if (reach_bad) atomicAdd(&errs[1], reach_bad);
   }
 }
 
-// Fixed-order reduction of block partials -> stats[STAT_N], alpha, scales.
-__global__ void k_stats_final(const double* __restrict__ partials, int nblocks,
-                              const unsigned long long* __restrict__ gmax, double ncells,
+// Fixed-order reduction of block partials -> stats[STAT_N] and alpha.
+__global__ void k_stats_final(const double* __restrict__ partials, int nblocks, double ncells,
                               double* __restrict__ stats) {
   __shared__ double sh[256];
   for (int q = 0; q < 7; q++) {
@@ -420,18 +410,6 @@ __global__ void k_stats_final(const double* __restrict__ partials, int nblocks,
   }
   if (threadIdx.x == 0) {
     stats[STAT_ALPHA] = stats[STAT_NGRID] / ncells;  // src/main.c:117
-    for (int f = 0; f < 2; f++) {
-      double m = __longlong_as_double((long long)gmax[f]);
-      int e = 0;
-      if (m > 0.0) {
-        int ex;
-        frexp(m, &ex);           // m < 2^ex
-        e = 40 - ex;             // |tap| <= m < 2^(ex) -> scaled < 2^40
-      }
-      stats[STAT_SCALE_A + f] = ldexp(1.0, e);
-      stats[STAT_INVSCALE_A + f] = ldexp(1.0, -e);
-      stats[STAT_GMAX_A + f] = m;
-    }
   }
 }
 
@@ -578,24 +556,28 @@ __constant__ DepRanges c_dep_ranges = kDepRanges;
 #define DEP_K 4       // particle copies per thread kept in registers for both phases
 #define DEP_TRASH 64  // per-lane sink cells for taps outside the tile
 // LDS bytes of the largest tile: FP32 16x16x32 (int64 per cell), FP64
-// 16x16x16 (two doubles per cell), each with the sink cells
+// 16x16x16 (two int64 per cell), each with the sink cells
 #define DEP_LDS_MAX (2 * (16 * 16 * 16 + DEP_TRASH) * 8)
 static_assert((16 * 16 * 32 + DEP_TRASH) * 8 <= DEP_LDS_MAX, "FP32 tile");
 
-// Exact accumulation.  FP32 form: each tap is rounded to an int32 multiple
-// of a per-tile quantum q chosen so that (copies in the tile) x max|field| x
-// (max tap weight) / q <= 2^30, so no partial sum of either field leaves
-// (-2^31, 2^31); fields A and B of a cell share one int64 accumulator,
-// A * 2^32 + B, updated with one 64-bit LDS atomic per tap (the low word
-// read as int32 is B, the rest is A).  FP64 form: each tap is rounded to a
-// multiple of 2^-e, e from the field's max |value| so that one tap is below
-// 2^40 quanta, and summed with ds_add_f64; sums below 2^53 quanta are exact.
-// Integer sums do not depend on the order of the additions.
-__device__ __forceinline__ double tile_scale(uint32_t count, double gmax, double wmax) {
+// Exact accumulation with a per-tile quantum q (a power of two) chosen so
+// that (copies in the tile) x (the tile's largest |field| value) x (max tap
+// weight) / q < 2^BITS:
+// a cell takes at most one tap per copy, so no partial sum of a field
+// leaves (-2^BITS - copies/2, 2^BITS + copies/2) whatever the order.
+// FP32 form (BITS 30): fields A and B of a cell share one int64
+// accumulator, A * 2^32 + B, updated with one 64-bit LDS atomic per tap
+// (the low word read as int32 is B, the rest is A).  FP64 form (BITS 62):
+// one int64 accumulator per field; a tile of c copies rounds each tap to
+// about c x (largest |field|) x 2^-63, below the FP64 rounding of the
+// tile's largest values for every tile of fewer than 2^10 copies, no bound
+// on how many taps one cell may take.  Integer sums do not depend on the
+// order of the additions, so the grids are deterministic.
+__device__ __forceinline__ double tile_scale(uint32_t count, double gmax, double wmax, int bits) {
   const double bound = (double)(count ? count : 1) * (gmax > 0 ? gmax : 1.0) * wmax;
   int ex;
   frexp(bound, &ex);               // bound < 2^ex
-  return ldexp(1.0, 30 - ex);      // bound * scale < 2^30
+  return ldexp(1.0, bits - ex);    // bound * scale < 2^bits
 }
 
 // floor(x + 0.5) as int32 in one instruction (ties round up); the rounding
@@ -651,10 +633,8 @@ __global__ __launch_bounds__(DEP_T) void k_deposit(const T* __restrict__ sorted,
   const int TY = TYC ? TYC : g.tdim[1], TZ = TZC ? TZC : g.tdim[2];
   const int ncell = TX * TY * TZ;
   const int nacc = ncell + DEP_TRASH;
-  // FP32: int64 [nacc]; FP64: double A [nacc], B [nacc]
+  // FP32: packed int64 [nacc]; FP64: int64 A [nacc], then B [nacc]
   unsigned long long* acc64 = reinterpret_cast<unsigned long long*>(smem);
-  double* accA = reinterpret_cast<double*>(smem);
-  double* accB = accA + nacc;
   const int lane = threadIdx.x & 63;
   const int trash = ncell + lane;
 
@@ -668,8 +648,12 @@ __global__ __launch_bounds__(DEP_T) void k_deposit(const T* __restrict__ sorted,
   if (tileid == 0 && threadIdx.x == 0 && (int64_t)off[g.nbuckets] > cap) errs[2] = 1u;
   // the tile's 63 ranges (wave 0): start and exclusive prefix of the lengths
   __shared__ uint32_t r_start[64], r_pre[65];
+  // the tile's max |w v^a|, |w v^b| as bit patterns (non-negative doubles
+  // order as their bits)
+  __shared__ unsigned long long s_fmax[2];
   if (threadIdx.x < 64) {
     const int r = threadIdx.x;
+    if (r == 0) s_fmax[0] = s_fmax[1] = 0ull;
     uint32_t st = 0, len = 0;
     if (r < c_dep_ranges.n) {
       const DepRange d = c_dep_ranges.r[r];
@@ -719,25 +703,80 @@ __global__ __launch_bounds__(DEP_T) void k_deposit(const T* __restrict__ sorted,
     return r_start[lo] + (f - r_pre[lo]);
   };
 
-  const double wmax = ORDER == 3 ? 0.421875 : 1.0;
-  const double scA = F32 ? tile_scale(total, stats[STAT_GMAX_A], wmax) : stats[STAT_SCALE_A];
-  const double scB = F32 ? tile_scale(total, stats[STAT_GMAX_B], wmax) : stats[STAT_SCALE_B];
-  const double invA = 1.0 / scA, invB = 1.0 / scB;   // powers of two: exact
   const T d0 = (T)g.d[0], d1 = (T)g.d[1], d2 = (T)g.d[2];
   const double alpha = stats[STAT_ALPHA];
   const bool single = total <= (uint32_t)(DEP_T * DEP_K);
 
+  // The quanta come from this tile's own largest field values, so a large
+  // value elsewhere in the catalogue does not coarsen this tile's rounding.
+  // A tile of at most DEP_T * DEP_K copies keeps them in registers for both
+  // phases; a larger one is read once more here.
   T px[DEP_K], py[DEP_K], pz[DEP_K], pa[DEP_K], pb[DEP_K];
   bool ph_has[DEP_K];
-  for (int ph = 0; ph < (g.interlace ? 2 : 1); ph++) {
-    {
-      uint4* z4 = reinterpret_cast<uint4*>(smem);
-      const int n4 = HALO ? (TX * TY * RZ + 2) / 2 : (F32 ? nacc / 2 : nacc);   // 16-byte words
-      for (int c = threadIdx.x; c < n4; c += DEP_T) z4[c] = make_uint4(0u, 0u, 0u, 0u);
+  // phase 0's accumulators are cleared while the particles load; the
+  // barrier after the maximum covers both
+  auto clear_acc = [&]() {
+    uint4* z4 = reinterpret_cast<uint4*>(smem);
+    const int n4 = HALO ? (TX * TY * RZ + 2) / 2 : (F32 ? nacc / 2 : nacc);   // 16-byte words
+    for (int c = threadIdx.x; c < n4; c += DEP_T) z4[c] = make_uint4(0u, 0u, 0u, 0u);
+  };
+  clear_acc();
+  double ma = 0.0, mb = 0.0;
+  if (single) {
+#pragma unroll
+    for (int k = 0; k < DEP_K; k++) {
+      const uint32_t f = (uint32_t)(k * DEP_T) + threadIdx.x;
+      ph_has[k] = f < total;
+      T x = T(0), y = T(0), z = T(0), v = T(0), w = T(1);
+      if (ph_has[k]) {
+        const uint32_t i = pidx(f);
+        load4<T>(sorted, i, x, y, z, v);
+        if (g.has_w) w = sorted_w[i];
+      }
+      px[k] = x; py[k] = y; pz[k] = z;
+      pa[k] = ph_has[k] ? w * vpow<T>(v, g.fa) : T(0);
+      pb[k] = (HASB && ph_has[k]) ? w * vpow<T>(v, g.fb) : T(0);
+      ma = fmax(ma, fabs((double)pa[k]));
+      mb = fmax(mb, fabs((double)pb[k]));
     }
-    __syncthreads();
+  } else {
+    for (uint32_t f = threadIdx.x; f < total; f += DEP_T) {
+      T x, y, z, v, w = T(1);
+      const uint32_t i = pidx(f);
+      load4<T>(sorted, i, x, y, z, v);
+      if (g.has_w) w = sorted_w[i];
+      ma = fmax(ma, fabs((double)(w * vpow<T>(v, g.fa))));
+      if (HASB) mb = fmax(mb, fabs((double)(w * vpow<T>(v, g.fb))));
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    ma = fmax(ma, __shfl_xor(ma, o));
+    mb = fmax(mb, __shfl_xor(mb, o));
+  }
+  if (lane == 0) {
+    atomicMax(&s_fmax[0], (unsigned long long)__double_as_longlong(ma));
+    atomicMax(&s_fmax[1], (unsigned long long)__double_as_longlong(mb));
+  }
+  __syncthreads();
+  const double wmax = ORDER == 3 ? 0.421875 : 1.0;
+  const double scA = tile_scale(total, __longlong_as_double((long long)s_fmax[0]), wmax, F32 ? 30 : 62);
+  const double scB = tile_scale(total, __longlong_as_double((long long)s_fmax[1]), wmax, F32 ? 30 : 62);
+  const double invA = 1.0 / scA, invB = 1.0 / scB;   // powers of two: exact
+  if (single) {
+#pragma unroll
+    for (int k = 0; k < DEP_K; k++) {
+      pa[k] *= (T)scA;
+      pb[k] *= (T)scB;
+    }
+  }
+  for (int ph = 0; ph < (g.interlace ? 2 : 1); ph++) {
+    if (ph) {
+      clear_acc();
+      __syncthreads();
+    }
     for (uint32_t base = 0; base < total; base += DEP_T * DEP_K) {
-      if (ph == 0 || !single) {
+      if (!single) {
 #pragma unroll
         for (int k = 0; k < DEP_K; k++) {
           const uint32_t f = base + (uint32_t)(k * DEP_T) + threadIdx.x;
@@ -864,10 +903,12 @@ __global__ __launch_bounds__(DEP_T) void k_deposit(const T* __restrict__ sorted,
                 const unsigned long long packed = ((unsigned long long)hi << 32) | (uint32_t)qb;
                 atomicAdd(reinterpret_cast<unsigned long long*>(smem + cell8), packed);
               } else {
-                atomicAdd(reinterpret_cast<double*>(smem + cell8), rint((double)qa0 * (double)az.w[tz]));
+                // |tap| < 2^62 quanta: the rounded double converts exactly
+                atomicAdd(reinterpret_cast<unsigned long long*>(smem + cell8),
+                          (unsigned long long)(long long)rint((double)qa0 * (double)az.w[tz]));
                 if (HASB)
-                  atomicAdd(reinterpret_cast<double*>(smem + (size_t)nacc * 8 + cell8),
-                            rint((double)qb0 * (double)az.w[tz]));
+                  atomicAdd(reinterpret_cast<unsigned long long*>(smem + (size_t)nacc * 8 + cell8),
+                            (unsigned long long)(long long)rint((double)qb0 * (double)az.w[tz]));
               }
             }
           }
@@ -916,12 +957,12 @@ __global__ __launch_bounds__(DEP_T) void k_deposit(const T* __restrict__ sorted,
         store2<T>(dA + gi, (T)((double)a0 * invA - meanA), (T)((double)a1 * invA - meanA));
         if (HASB) store2<T>(dB + gi, (T)((double)b0 * invB), (T)((double)b1 * invB));
       } else {
-        const double2 qa = *reinterpret_cast<const double2*>(accA + cs);
-        const double a0 = swap ? qa.y : qa.x, a1 = swap ? qa.x : qa.y;
+        const longlong2 qa = *reinterpret_cast<const longlong2*>(acc64 + cs);
+        const double a0 = (double)(swap ? qa.y : qa.x), a1 = (double)(swap ? qa.x : qa.y);
         store2<T>(dA + gi, (T)(a0 * invA - meanA), (T)(a1 * invA - meanA));
         if (HASB) {
-          const double2 qb = *reinterpret_cast<const double2*>(accB + cs);
-          const double b0 = swap ? qb.y : qb.x, b1 = swap ? qb.x : qb.y;
+          const longlong2 qb = *reinterpret_cast<const longlong2*>(acc64 + nacc + cs);
+          const double b0 = (double)(swap ? qb.y : qb.x), b1 = (double)(swap ? qb.x : qb.y);
           store2<T>(dB + gi, (T)(b0 * invB), (T)(b1 * invB));
         }
       }
@@ -941,9 +982,9 @@ static int run_count_t(DepositArgs& a) {
   TIMED(a.timer, "bucket_count",
         (k_bucket<T, ORDER, false><<<nb, BK_T, 0, st>>>((const T*)a.parts, (const T*)a.wts, a.n, g,
                                                         a.counts, nullptr, nullptr, nullptr, nullptr, 0,
-                                                        a.partials, a.gmax, a.errs)));
+                                                        a.partials, a.errs)));
   TIMED(a.timer, "stats_final",
-        (k_stats_final<<<1, 256, 0, st>>>(a.partials, nb, a.gmax,
+        (k_stats_final<<<1, 256, 0, st>>>(a.partials, nb,
                                          (double)g.n[0] * (double)g.n[1] * (double)g.n[2],
                                          a.stats)));
   // scan counts -> offsets; zero the own/ghost cursors
@@ -967,7 +1008,7 @@ static int run_scatter_t(DepositArgs& a) {
         (k_bucket<T, ORDER, true><<<nb, BK_T, 0, st>>>((const T*)a.parts, (const T*)a.wts, a.n, g,
                                                        nullptr, a.cursor, a.offsets, (T*)a.sorted,
                                                        (T*)a.sorted_w, a.sorted_cap, nullptr,
-                                                       nullptr, nullptr)));
+                                                       nullptr)));
   const int ncell = g.tdim[0] * g.tdim[1] * g.tdim[2];
   const size_t nacc = (size_t)ncell + DEP_TRASH;
   const size_t lds = sizeof(T) == 4 ? nacc * 8 : 2 * nacc * 8;

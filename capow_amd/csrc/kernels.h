@@ -19,9 +19,7 @@ enum {
   STAT_NOUT = 6,
   STAT_N = 8,       // stride of block partials
   STAT_ALPHA = 8,
-  STAT_SCALE_A = 9, STAT_SCALE_B = 10,
-  STAT_INVSCALE_A = 11, STAT_INVSCALE_B = 12,
-  STAT_GMAX_A = 13, STAT_GMAX_B = 14,
+  // 9..14 unused
   STAT_TOTAL = 16
 };
 
@@ -98,7 +96,6 @@ struct DepositArgs {
   void* sorted_w;         // sorted_cap or null
   int64_t sorted_cap;     // rows the sorted buffers hold
   double* partials;       // count_blocks * STAT_N
-  unsigned long long* gmax;  // 2 (zeroed)
   unsigned int* errs;     // 2 (zeroed): [0] LOS=0 out-of-box, [1] reach violations
   double* stats;          // STAT_TOTAL
   void* grid[4];          // A, A_int, B, B_int (null when unused)
@@ -174,6 +171,26 @@ struct BinLaunch {
   double* counts;         // nk: mode counts of the branch-free form
   bool* counts_valid;     // counts computed for this geometry
 };
+// The binning plan of a parameter set for grids of `rowc` complex values
+// per row: the kernels' arguments, which form runs, the workgroup count of a
+// device with `ncu` compute units and the buffer sizes (capow_create and
+// capow_bin_noalloc share it); the per-axis tables it points into.
+struct BinPlan {
+  BinArgsDev b;
+  bool mirror, row_form, mirror2d;
+  int codes[3];
+  int per_copy;           // doubles per bin copy
+  int nwg;                // workgroups of the accumulation kernels
+  size_t ntab;            // doubles of the per-axis tables
+};
+int make_bin_plan(const capow_params& p, int64_t rowc, bool mirror_allowed, int ncu, BinPlan& bp,
+                  std::string& err);
+void fill_bin_tables(const capow_params& p, double* tab);
+double bin_table_at(const capow_params& p, size_t idx);   // one entry of fill_bin_tables
+size_t bin_table_len(const capow_params& p);
+// capow_create's parameter check, and the message capow_last_error(NULL) returns
+int validate_params(const capow_params* p, std::string& err);
+void set_global_error(const std::string& err);
 int run_bin(BinLaunch& L, bool fp64);
 // The pieces of run_bin around the accumulation kernel: the once-per-context
 // mode counts of the branch-free forms, and the final sum of the partials.

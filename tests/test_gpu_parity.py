@@ -2,8 +2,9 @@
 
 Bars (stated per test):
   * deposit grids (FP64 form): |gpu - oracle| <= 1e-10 * max|field value|
-    per cell (each tap is rounded to 2^-40 of the largest |field value|
-    before exact accumulation);
+    per cell (each tap is rounded to an int64 quantum of about 2^-63 x
+    copies in the tile x the largest |field value| before exact
+    accumulation);
   * FFT (FP64): <= 1e-12 * max|X|; (FP32): <= 2e-5 * max|X|;
   * binning on identical k-space grids: mode counts bit-exact, sums within
     1e-12 of the per-bin Cauchy-Schwarz scale;
@@ -603,3 +604,40 @@ def test_batch_matches_single_contexts():
         for code in ref["codes"]:
             assert np.array_equal(res[i][code]["pk"], ref[code]["pk"]), (i, code)
         assert np.array_equal(res[i]["nmodes"], ref["nmodes"]), i
+
+
+def test_fp32_quanta_follow_each_tiles_own_values():
+    """One particle with a line-of-sight velocity 10^5 times the others'
+    coarsens the FP32 rounding only in the tiles that deposit it: each tile
+    takes its quantum from its own largest |w v|.  Away from the outlier the
+    FP32 momentum grid agrees with the FP64 one within 1e-4 of the largest
+    |w v| there (the FP32 tap weights of float32 positions alone leave about
+    1e-5; a quantum from the global maximum would leave errors of order
+    1e-3)."""
+    say("per-tile quanta with an outlier")
+    n = 128
+    rng = np.random.default_rng(17)
+    pos = rng.uniform(0, L, size=(60000, 3)).astype(np.float32).astype(np.float64)
+    vel = rng.normal(0, 300.0, size=(60000, 3)).astype(np.float32).astype(np.float64)
+    pos[0] = (0.1 * L, 0.1 * L, 0.1 * L)
+    vel[0, 2] = 3.0e7
+    grids = {}
+    for prec in (api.FP32, api.FP64):
+        p = api.make_params(nx=n, ny=n, nz=n, interp_order=3, do_interlacing=1, momentum=1,
+                            precision=prec, nk=10, maxk=0.2)
+        ps = api.PowerSpectrum(p)
+        xyzv = mocks.xyzv(pos, vel, 0)
+        ps.set_particles(xyzv.astype(np.float32) if prec == api.FP32 else xyzv)
+        ps.assign()
+        grids[prec] = ps.get_grid(2).astype(np.float64).reshape(n, n, -1)[:, :, :n]
+        ps.close()
+    # the outlier sits in cell 12 of the first 16-cell tile along x; its taps
+    # stay in that tile, whose neighbours are x tiles 1 and 7: compare x
+    # planes 32 .. 111, whose tiles never see it
+    far = np.zeros((n, n, n), bool)
+    far[32:112] = True
+    b64 = grids[api.FP64]
+    scale = np.abs(b64[far]).max()
+    err = np.abs(grids[api.FP32][far] - b64[far]).max() / scale
+    say("  max |B32 - B64| / max |B| away from the outlier: %.2e" % err)
+    assert scale < 1e5 and err <= 1e-4

@@ -396,3 +396,73 @@ def test_threaded_reader_keeps_file_order(tmp_path, monkeypatch):
     f.write_text("".join(L2))
     assert read(str(f), 0, g, s, cosmo, nobj_max=20000)[0] == 3
     assert read(str(f), 0, g, s, cosmo, nobj_max=55000)[0] == api.CAPOW_ERR_PARSE
+
+
+def test_zero_weight_coord0_row_at_momentum_3(tmp_path):
+    """Coord_Type 0 keeps zero-weight objects (src/read_data.c:259-266 has no
+    weight test there); at Momentum 3 such an object has weight_pv = 0 too
+    (tw, or weight tp tc after FKP, src/read_data.c:460) and deposits nothing
+    into either field, so the packed row carries w = 0 and v = 0.  A zero
+    weight with a nonzero weight_pv pv cannot be carried as w v: refused."""
+    g, s = grid(), survey(coord_type=0, momentum=3, nbar_column=1)
+    cosmo = Cosmo(0.31)
+    f = tmp_path / "d.txt"
+    rr = so.comoving_distance(0.2, 0.31)
+    f.write_text("%r 0.0 0.0 1.0 1e-4 2e-4 0.02 0.003 0 0 0 0\n" % rr
+                 + "0.0 %r 0.0 0.0 1e-4 2e-4 0.05 0.003 0 0 0 0\n" % rr)
+    rc, objs, zr, nread, _, _ = read(str(f), 0, g, s, cosmo)
+    assert rc == 0 and nread == 2 and len(objs) == 2
+    assert objs[1].weight == 0.0 and objs[1].pv != 0.0
+    L = api.lib()
+    assert L.capow_survey_fkp(C.byref(csurvey(s)), objs, 2) == 0
+    assert objs[1].weight == 0.0 and objs[1].weight_pv == 0.0
+    xyzv, w = np.zeros(8), np.zeros(2)
+    n = C.c_int64()
+    assert L.capow_survey_pack(C.byref(csurvey(s)), objs, 2, to_c([]), 0, 0.5, api.FP64,
+                               xyzv.ctypes.data, w.ctypes.data, C.byref(n)) == 0
+    assert n.value == 2 and w[1] == 0.0 and xyzv[7] == 0.0
+    assert xyzv[3] == pytest.approx(objs[0].weight_pv * objs[0].pv / objs[0].weight, rel=1e-15)
+    bad = to_c([dict(x=1.0, y=1.0, z=1.0, weight=0.0, weight_pv=0.5, pv=10.0)])
+    assert L.capow_survey_pack(C.byref(csurvey(s)), bad, 1, to_c([]), 0, 0.5, api.FP64,
+                               xyzv.ctypes.data, w.ctypes.data, C.byref(n)) == \
+        5   # CAPOW_ERR_UNSUPPORTED
+
+
+def test_threaded_nbar_is_bitwise_thread_independent(monkeypatch):
+    """The n(z) spline is evaluated over host threads, one per 65,536
+    objects up to OMP_NUM_THREADS: 131,072 data and 262,144 randoms give
+    bitwise the same table and per-object nbar with OMP_NUM_THREADS = 1
+    (one thread) and 16 (six threads)."""
+    s = survey(nbar_column=-1, redmin_in=0.05, redmax_in=0.45, sky_area=20000.0)
+    cosmo = Cosmo(0.31)
+    rng = np.random.default_rng(12)
+    nd, nr = 131072, 262144
+
+    def objs(n):
+        arr = (api.SurveyObj * n)()
+        z = rng.uniform(0.05, 0.45, n)
+        for i in range(n):
+            arr[i].redshift = z[i]
+            arr[i].weight = 1.0
+        return arr
+
+    d0, r0 = objs(nd), objs(nr)
+    zr = np.array([0.05, 0.45])
+    L = api.lib()
+    nb = L.capow_survey_nbar_bins(C.byref(csurvey(s)))
+    res = {}
+    for nt in ("1", "16"):
+        monkeypatch.setenv("OMP_NUM_THREADS", nt)
+        d = (api.SurveyObj * nd)()
+        r = (api.SurveyObj * nr)()
+        C.memmove(d, d0, C.sizeof(d0))
+        C.memmove(r, r0, C.sizeof(r0))
+        zn, nbar = np.zeros(nb + 2), np.zeros(nb + 2)
+        assert L.capow_survey_nbar(C.byref(csurvey(s)), cosmo.h, _d(zr), d, nd, r, nr, _d(zn),
+                                   _d(nbar), nb + 2) == 0
+        per = np.frombuffer(d, dtype=np.float64).reshape(nd, 9)[:, 6].copy()
+        perr = np.frombuffer(r, dtype=np.float64).reshape(nr, 9)[:, 6].copy()
+        res[nt] = (zn, nbar, per, perr)
+    for a, b in zip(res["1"], res["16"]):
+        assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
+    assert np.all(res["1"][2] > 0)

@@ -100,6 +100,7 @@ SIGNATURES = {
     "capow_assign": ([_V, C.POINTER(AssignStats)], C.c_int),
     "capow_fft": ([_V], C.c_int),
     "capow_bin": ([_V, C.POINTER(Bins)], C.c_int),
+    "capow_bin_noalloc": ([_P, C.POINTER(_V), C.c_int64, _V, C.POINTER(Bins)], C.c_int),
     "capow_power": ([_V, C.POINTER(AssignStats), C.POINTER(Bins)], C.c_int),
     "capow_spectra": ([_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)], C.c_int),
     "capow_scalars": ([_P, C.POINTER(AssignStats), C.c_int32, _D, _D], C.c_int),

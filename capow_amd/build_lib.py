@@ -26,7 +26,7 @@ HIP_FLAGS = [
     "-Wno-unused-result", "-Wno-unused-value",
 ]
 HIP_SOURCES = ["capow_api.hip", "deposit.hip", "fft.hip", "binning.hip", "adjoint.hip",
-               "survey.hip"]
+               "survey.hip", "noalloc.hip"]
 C_SOURCES = ["capow_host.c", "capow_ascii.c", "capow_survey_ingest.c"]
 
 

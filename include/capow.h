@@ -210,6 +210,27 @@ int capow_power(capow_ctx *ctx, capow_assign_stats *stats, capow_bins *out);
  * the same real-space fields afterwards.  Synchronises. */
 int capow_survey_power(capow_ctx *ctx, const double origin[3], double *pk, int64_t *nmodes);
 
+/* No counterpart in the reference.  capow_bin for callers that cannot
+ * allocate device memory (code under stream capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the same binning kernels on k-space
+ * grids the caller owns, with no context.  p: the parameters (validated as
+ * capow_create does; survey parameters are refused).  grids: device
+ * pointers, slots as capow_get_grid's (0 A, 1 A interlaced, 2 B, 3 B
+ * interlaced; the slots p does not use are ignored), each [NX][NY][row_pitch]
+ * complex values of the precision of p, 256-byte aligned, holding the r2c
+ * half spectrum in the first NZ/2+1 of every row (the layout capow_fft
+ * leaves); row_pitch >= NZ/2+1 and a whole number of 128-byte lines (a
+ * multiple of 16 in FP32, of 8 in FP64).  hip_stream: the hipStream_t the
+ * work runs on (NULL: the default stream).  out: host arrays as for
+ * capow_bin, filled with the same raw sums and mode counts; synchronises
+ * the stream.  Every buffer the kernels need (tables, partial bins, counts)
+ * lives in one device array of 64 MiB per GPU that the library holds until
+ * the process ends; bins that need more are CAPOW_ERR_UNSUPPORTED.  Calls
+ * that overlap on one GPU share that array and race.  Errors are reported
+ * by capow_last_error(NULL). */
+int capow_bin_noalloc(const capow_params *p, const void *const grids[4], int64_t row_pitch,
+                      void *hip_stream, capow_bins *out);
+
 /* Number of spectra capow_bin produces and their reference Momentum codes. */
 int capow_spectra(const capow_params *p, int32_t *n_spectra, int32_t codes[3]);
 

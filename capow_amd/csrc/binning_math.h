@@ -306,7 +306,9 @@ __device__ __forceinline__ Group group_of(int g, int hy1, const double* __restri
 // context's flags as template arguments; the four products |A|^2, |B|^2,
 // Im(A* B), Re(A* B) are summed over the rows and mapped onto the spectra
 // once per group.
-template <typename C, bool IL, bool HB>
+// F32: the interlace combine and the four products in FP32 (FP32 grids;
+// the geometry, NM cor and L_l stay in FP64): the fused x-pass binning.
+template <typename C, bool IL, bool HB, bool F32 = false>
 __device__ __forceinline__ ModeVals group_mode(const Group& G, int k, double fz, double corz,
                                                double pzr, double pzi, const C (&v)[4][4],
                                                const BinArgsDev& a) {
@@ -325,6 +327,36 @@ __device__ __forceinline__ ModeVals group_mode(const Group& G, int k, double fz,
   const double mu = fktot > 0 ? (twopi * fl) * rk : 0.0;
   const double wc = nm_weight(G.i, G.j, k, a.nx) * (G.corxy * corz);
   double saa = 0.0, sbb = 0.0, sim = 0.0, sre = 0.0;
+  if constexpr (F32) {
+    float faa = 0.f, fbb = 0.f, fim = 0.f, fre = 0.f;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      float ar = (float)v[r][0].x, ai = (float)v[r][0].y, br = 0.f, bi = 0.f;
+      float cr = 1.f, ci = 0.f;
+      if constexpr (IL) {
+        cr = (float)(G.qr[r] * pzr - G.qi[r] * pzi);
+        ci = (float)(G.qr[r] * pzi + G.qi[r] * pzr);
+        ar += (float)v[r][1].x * cr - (float)v[r][1].y * ci;
+        ai += (float)v[r][1].x * ci + (float)v[r][1].y * cr;
+      }
+      faa += ar * ar + ai * ai;
+      if constexpr (HB) {
+        br = (float)v[r][2].x;
+        bi = (float)v[r][2].y;
+        if constexpr (IL) {
+          br += (float)v[r][3].x * cr - (float)v[r][3].y * ci;
+          bi += (float)v[r][3].x * ci + (float)v[r][3].y * cr;
+        }
+        fbb += br * br + bi * bi;
+        fim += ar * bi - ai * br;
+        fre += ar * br + ai * bi;
+      }
+    }
+    saa = faa;
+    sbb = fbb;
+    sim = fim;
+    sre = fre;
+  } else {
 #pragma unroll
   for (int r = 0; r < 4; r++) {
     double ar = (double)v[r][0].x, ai = (double)v[r][0].y, br = 0, bi = 0;
@@ -347,6 +379,7 @@ __device__ __forceinline__ ModeVals group_mode(const Group& G, int k, double fz,
       sim += ar * bi - ai * br;
       sre += ar * br + ai * bi;
     }
+  }
   }
 #pragma unroll
   for (int s = 0; s < 3; s++) {

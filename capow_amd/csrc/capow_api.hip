@@ -77,6 +77,19 @@ struct capow_ctx {
   bool bin_row_form = true;
   bool bin_mirror = false;  // mirror-group binning (set up in setup())
   int bin_mirror_env = -1;  // CAPOW_BIN_MIRROR: 0 off, 1 on where it applies
+  // x pass fused with the binning (FP32 out-of-place FFT, mirror form, 1-D
+  // bins): capow_fft runs the z and y passes and leaves the x pass pending;
+  // capow_bin transforms and bins at once; anything else that reads the
+  // k-space grids runs the pending pass first (finish_x)
+  bool xbin_ok = false;
+  int xbin_env = -1;        // CAPOW_XBIN: 0 off
+  bool x_pending = false;
+  double xbin_factor = 0.0; // 4 cor_max NX NY
+  double* xbin_ssq = nullptr;
+  double* xbin_q = nullptr;
+  long long* xbin_part = nullptr;
+  int xbin_rows = 0;
+  int xbin_ssq_rows = 0;    // rows of xbin_ssq the last deferred transform wrote
   int nspec = 1;
   int codes[3] = {0, 0, 0};
   capow_assign_stats last{};
@@ -569,6 +582,33 @@ static int setup(capow_ctx* c) {
     }
     c->bin_counts_valid = false;
     HIPCHK(c, hipHostMalloc((void**)&c->h_bins, bytes, hipHostMallocDefault));
+    // the fused x pass + binning: FP32 out-of-place transform (scratch
+    // grids), persistent row passes, mirror-group 1-D bins
+    const size_t xl = xbin_lds(g, bp.b);
+    c->xbin_ok = !c->fp64 && c->scratch[0] && c->fft_persist && bp.mirror && bp.row_form &&
+                 !p.output_2d && xl > 0 && c->xbin_env != 0 && p.edge_mode != CAPOW_EDGE_SURVEY;
+    if (c->xbin_ok) {
+      // cor_max: the largest deconvolution factor of each axis (x 0.25 with
+      // interlacing, folded into the z table)
+      double cm[3] = {0.0, 0.0, 0.0};
+      const size_t nax[3] = {(size_t)p.nx, (size_t)p.ny, (size_t)(p.nz / 2 + 1)};
+      const size_t sect = nax[0] + nax[1] + nax[2];
+      size_t off = sect;   // the deconvolution tables follow the frequencies
+      for (int ax = 0; ax < 3; ax++) {
+        for (size_t i = 0; i < nax[ax]; i++) {
+          const double v = bin_table_at(p, off + i);
+          if (v > cm[ax]) cm[ax] = v;
+        }
+        off += nax[ax];
+      }
+      c->xbin_factor = 4.0 * cm[0] * cm[1] * cm[2] * (double)p.nx * (double)p.ny;
+      const int per_cu = (int)((160 * 1024) / xl) > 0 ? (int)((160 * 1024) / xl) : 1;
+      c->xbin_rows = c->ncu * (per_cu < 8 ? per_cu : 8);
+      const int F = bp.b.nspec * (p.odd_multipoles ? 5 : 3);
+      HIPCHK(c, hipMalloc((void**)&c->xbin_ssq, (size_t)XBIN_SSQ_MAX * 4 * 8));
+      HIPCHK(c, hipMalloc((void**)&c->xbin_q, 6 * 8));
+      HIPCHK(c, hipMalloc((void**)&c->xbin_part, (size_t)c->xbin_rows * F * p.nk * 8));
+    }
   }
   return CAPOW_OK;
 }
@@ -579,7 +619,8 @@ static void release(capow_ctx* c) {
   void* ps[] = {c->own_parts, c->own_w, c->sorted, c->sorted_w, c->counts, c->offsets,
                 c->cursor, c->bsum, c->boff, c->partials, c->errs, c->stats,
                 c->tw[0], c->tw[1], c->tw[2], c->bin_tabs, c->bin_partials, c->bin_out,
-                c->bin_counts, c->bin_partials2, c->bin_qscale};
+                c->bin_counts, c->bin_partials2, c->bin_qscale, c->xbin_ssq, c->xbin_q,
+                c->xbin_part};
   for (void* q : ps) if (q) hipFree(q);
   void* bs[] = {c->bwd_G, c->bwd_g0, c->bwd_out, c->bwd_w};
   for (void* q : bs) if (q) hipFree(q);
@@ -604,6 +645,7 @@ extern "C" int capow_create(const capow_params* p, capow_ctx** out) {
   if (const char* e = getenv("CAPOW_DEPOSIT_HALO")) c->dep_halo = atoi(e) != 0;
   if (const char* e = getenv("CAPOW_FFT_ORDER")) c->fft_order = atoi(e) & 7;
   if (const char* e = getenv("CAPOW_BIN_MIRROR")) c->bin_mirror_env = atoi(e) != 0;
+  if (const char* e = getenv("CAPOW_XBIN")) c->xbin_env = atoi(e) != 0;
   rc = setup(c);
   if (rc) {
     g_create_err = c->err;
@@ -710,6 +752,7 @@ static void fill_stats(capow_ctx* c, const double* s) {
 // h_errs[2] (the caller re-runs that catalogue).
 static int assign_enqueue(capow_ctx* c, double* h_stats, unsigned int* h_errs, bool exact) {
   if (c->state < 1) return fail(c, CAPOW_ERR_STATE, "capow_assign before capow_set_particles");
+  c->x_pending = false;   // the deposit rewrites every grid
   Geom& g = c->g;
   g.has_w = c->wts != nullptr;
   HIPCHK(c, hipMemsetAsync(c->counts, 0, (size_t)g.nbuckets * 4, c->stream));
@@ -826,6 +869,12 @@ extern "C" int capow_fft(capow_ctx* c) {
   if (c->state >= 3) return fail(c, CAPOW_ERR_STATE, "grids already transformed");
   FftArgs a;
   fft_args(c, a);
+  if (c->xbin_ok) {
+    a.defer_x = true;
+    a.ssq = c->xbin_ssq;
+    a.xbin_factor = c->xbin_factor;
+    a.xq = c->xbin_q;
+  }
   int rc = run_fft(a, c->fp64);
   if (rc) return fail(c, rc, std::string("fft launch failed: ") + hipGetErrorString(hipGetLastError()));
   if (a.swapped)
@@ -834,7 +883,26 @@ extern "C" int capow_fft(capow_ctx* c) {
       c->grid[s] = c->scratch[s];
       c->scratch[s] = t;
     }
+  c->x_pending = a.x_deferred;
+  c->xbin_ssq_rows = a.ssq_rows;
   c->state = 3;
+  return CAPOW_OK;
+}
+
+// Runs the x pass capow_fft left pending, so that the grids hold the
+// k-space fields (for anything but the fused binning).
+static int finish_x(capow_ctx* c) {
+  if (!c->x_pending) return CAPOW_OK;
+  FftArgs a;
+  fft_args(c, a);
+  const int rc = run_fft_x(a);
+  if (rc) return fail(c, rc, std::string("fft x pass failed: ") + hipGetErrorString(hipGetLastError()));
+  for (int s = 0; s < 4; s++) {
+    void* t = c->grid[s];
+    c->grid[s] = c->scratch[s];
+    c->scratch[s] = t;
+  }
+  c->x_pending = false;
   return CAPOW_OK;
 }
 
@@ -861,8 +929,25 @@ static void bin_launch(capow_ctx* c, BinLaunch& L) {
 static int bin_enqueue(capow_ctx* c, double* h_bins) {
   if (c->state < 3) return fail(c, CAPOW_ERR_STATE, "capow_bin before capow_fft");
   BinLaunch L;
-  bin_launch(c, L);
-  int rc = run_bin(L, c->fp64);
+  int rc = CAPOW_ERR_UNSUPPORTED;
+  if (c->x_pending) {
+    FftArgs a;
+    fft_args(c, a);
+    a.ssq = c->xbin_ssq;
+    a.ssq_rows = c->xbin_ssq_rows;
+    a.xbin_factor = c->xbin_factor;
+    a.xq = c->xbin_q;
+    bin_launch(c, L);
+    rc = run_fft_x_bin(a, L, c->xbin_part, c->xbin_rows);
+    if (rc && rc != CAPOW_ERR_UNSUPPORTED)
+      return fail(c, rc, std::string("fused x pass and binning failed: ") + hipGetErrorString(hipGetLastError()));
+  }
+  if (rc == CAPOW_ERR_UNSUPPORTED) {
+    rc = finish_x(c);
+    if (rc) return rc;
+    bin_launch(c, L);
+    rc = run_bin(L, c->fp64);
+  }
   if (rc) return fail(c, rc, std::string("binning launch failed: ") + hipGetErrorString(hipGetLastError()));
   HIPCHK(c, hipMemcpyAsync(h_bins, c->bin_out, (size_t)c->bin_per_copy * 8, hipMemcpyDeviceToHost,
                            c->stream));
@@ -1074,6 +1159,8 @@ extern "C" int capow_get_grid(capow_ctx* c, int32_t slot, void* host, int64_t n)
   if (!c || !host || slot < 0 || slot > 3) return CAPOW_ERR_ARG;
   if (!c->grid[slot]) return fail(c, CAPOW_ERR_ARG, "grid slot not allocated");
   if (n < c->grid_elems) return fail(c, CAPOW_ERR_ARG, "host buffer too small");
+  int rc = finish_x(c);
+  if (rc) return rc;
   const size_t w = (size_t)2 * c->g.nzh * c->esz, pitch = (size_t)c->g.rowlen * c->esz;
   HIPCHK(c, hipMemcpy2DAsync(host, w, c->grid[slot], pitch, w, (size_t)c->p.nx * c->p.ny,
                              hipMemcpyDeviceToHost, c->stream));
@@ -1085,6 +1172,8 @@ extern "C" int capow_set_grid(capow_ctx* c, int32_t slot, const void* host, int6
   if (!c || !host || slot < 0 || slot > 3) return CAPOW_ERR_ARG;
   if (!c->grid[slot]) return fail(c, CAPOW_ERR_ARG, "grid slot not allocated");
   if (n < c->grid_elems) return fail(c, CAPOW_ERR_ARG, "host buffer too small");
+  int rc = finish_x(c);   // the other slots stay consistent
+  if (rc) return rc;
   const size_t w = (size_t)2 * c->g.nzh * c->esz, pitch = (size_t)c->g.rowlen * c->esz;
   HIPCHK(c, hipMemcpy2DAsync(c->grid[slot], pitch, host, w, w, (size_t)c->p.nx * c->p.ny,
                              hipMemcpyHostToDevice, c->stream));
@@ -1094,6 +1183,7 @@ extern "C" int capow_set_grid(capow_ctx* c, int32_t slot, const void* host, int6
 
 extern "C" int capow_mark_assigned(capow_ctx* c, const capow_assign_stats* st) {
   if (!c) return CAPOW_ERR_ARG;
+  c->x_pending = false;   // the caller's grids replace the pending transform
   if (st) c->last = *st;
   c->state = 2;
   return CAPOW_OK;
@@ -1101,6 +1191,7 @@ extern "C" int capow_mark_assigned(capow_ctx* c, const capow_assign_stats* st) {
 
 extern "C" int capow_mark_transformed(capow_ctx* c) {
   if (!c) return CAPOW_ERR_ARG;
+  c->x_pending = false;
   c->state = 3;
   return CAPOW_OK;
 }
@@ -1148,6 +1239,7 @@ extern "C" int capow_backward(capow_ctx* c, const double* dL_draw, void* grad_xy
   int rc = capow_assign(c, nullptr);
   if (rc) return rc;
   rc = capow_fft(c);
+  if (!rc) rc = finish_x(c);   // the binning backward needs the k-space grids
   if (rc) return rc;
   const size_t gbytes = (size_t)c->nspec * 5 * c->p.nk * 8;
   rc = ensure(c, (void**)&c->bwd_G, &c->bwd_G_cap, gbytes);

@@ -15,6 +15,7 @@
 //      written back in place.
 // Twiddles: exp(-2 pi i j / N) generated in double on the host, stored in the
 // working precision.
+#include "binning_math.h"
 #include "common.h"
 #include "kernels.h"
 #include "twiddle_table.h"
@@ -677,7 +678,7 @@ __global__ __launch_bounds__((N1 * N2 / clcm(N1, N2)) * B) void k_fft_rows_t(
     const T* __restrict__ i3, T* __restrict__ o0, T* __restrict__ o1, T* __restrict__ o2,
     T* __restrict__ o3, int64_t in_so, int64_t in_sb, int64_t out_so, int64_t out_sk,
     int64_t n_inner, int64_t inner_groups, const cx<T>* __restrict__ tw, int64_t nbx,
-    int ngrids, int outer_fast) {
+    int ngrids, int outer_fast, double* __restrict__ ssq) {
   constexpr int N = N1 * N2;
   constexpr int E = clcm(N1, N2);
   constexpr int TS = N / E;
@@ -751,9 +752,21 @@ __global__ __launch_bounds__((N1 * N2 / clcm(N1, N2)) * B) void k_fft_rows_t(
     cx<T>* ob = dest(L0, nlive);
     rt_rest<T, N1, N2, B, R2C>(tw, nullptr, lds, t, c, ob, out_sk, nlive);
   } else {
+    // ssq: sum of |v|^2 of the rows this workgroup reads, per grid (fixed
+    // order: per thread in item order, then the workgroup's threads in order)
+    double sq[4] = {0.0, 0.0, 0.0, 0.0};
     if (L0 < Lend) load(L0);
     for (int64_t item = L0; item < Lend; item += step) {
       if (!PF && item != L0) load(item);
+      if (ssq) {
+        float s2 = 0.0f;
+#pragma unroll
+        for (int e = 0; e < E; e++) s2 += (float)v[e].x * (float)v[e].x + (float)v[e].y * (float)v[e].y;
+        const int gsel = (int)(item / nbx);
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+          if (q == gsel) sq[q] += (double)s2;
+      }
       rt_stage1<T, N1, N2, R2C, TW_REG>(v, twr, tw, lds, t, c);
       __syncthreads();
       // v is free: the next batch's loads overlap the rest of this one
@@ -763,6 +776,18 @@ __global__ __launch_bounds__((N1 * N2 / clcm(N1, N2)) * B) void k_fft_rows_t(
       rt_rest<T, N1, N2, B, R2C>(tw, tws, lds, t, c, ob, out_sk, nlive);
       // the next batch's first stage overwrites the LDS
       __syncthreads();
+    }
+    if (ssq) {
+      // the LDS holds B (N + N1) >= 32 TS B bytes: room for 4 doubles a thread
+      double* red = reinterpret_cast<double*>(smem);
+#pragma unroll
+      for (int q = 0; q < 4; q++) red[threadIdx.x * 4 + q] = sq[q];
+      __syncthreads();
+      if (threadIdx.x < 4) {
+        double tot = 0.0;
+        for (int i = 0; i < (int)blockDim.x; i++) tot += red[i * 4 + threadIdx.x];
+        ssq[(int64_t)blockIdx.x * 4 + threadIdx.x] = tot;
+      }
     }
   }
 }
@@ -901,7 +926,7 @@ static bool try_rows2(FftArgs& a, T** gp, int M, int64_t nrows, const void* twz,
 template <typename T, int N1, int N2, bool R2C, int MUL>
 static void launch_rows_t(FftArgs& a, const char* name, T* const* in, T* const* out, int64_t n_outer,
                           int64_t n_inner, int64_t in_so, int64_t in_sb, int64_t out_so,
-                          int64_t out_sk, const void* tw) {
+                          int64_t out_sk, const void* tw, double* ssq) {
   constexpr int N = N1 * N2, E = clcm(N1, N2), TS = N / E;
   constexpr int B = (sizeof(T) == 4 ? 8 : 4) * MUL;   // 64- (MUL 2: 128-) byte write runs
   // FP32: persistent, prefetch and register twiddles (FP64 has no registers
@@ -920,17 +945,19 @@ static void launch_rows_t(FftArgs& a, const char* name, T* const* in, T* const* 
     per_cu = 1;
   int64_t nblk = (int64_t)(a.ncu > 0 ? a.ncu : 256) * per_cu;
   if (!PS || a.persist == 0 || nblk > total) nblk = total;
+  if (ssq) a.ssq_rows = (int)nblk;
   const int ofast = (a.order >> (name[4] == 'z' ? 0 : name[4] == 'y' ? 1 : 2)) & 1;
   TIMED(a.timer, name,
         (k_fft_rows_t<T, N1, N2, B, R2C, PS, PF, TWR><<<(unsigned)nblk, TS * B, lds, a.stream>>>(
             in[0], in[1], in[2], in[3], out[0], out[1], out[2], out[3], in_so, in_sb, out_so,
-            out_sk, n_inner, groups, (const cx<T>*)tw, nbx, a.ngrids, ofast)));
+            out_sk, n_inner, groups, (const cx<T>*)tw, nbx, a.ngrids, ofast, ssq)));
 }
 
 template <typename T, bool R2C>
 static bool try_rows_t(FftArgs& a, const char* name, int N, T* const* in, T* const* out,
                        int64_t n_outer, int64_t n_inner, int64_t in_so, int64_t in_sb,
-                       int64_t out_so, int64_t out_sk, const void* tw, int mul) {
+                       int64_t out_so, int64_t out_sk, const void* tw, int mul,
+                       double* ssq = nullptr) {
   int n1, n2;
   if (!reg_split(N, &n1, &n2)) return false;
 #define X(A, B_)                                                                               \
@@ -938,15 +965,358 @@ static bool try_rows_t(FftArgs& a, const char* name, int N, T* const* in, T* con
     if (mul == 2 && (size_t)(sizeof(T) == 4 ? 16 : 8) * sizeof(cx<T>) * (A * B_ + A) +           \
                         (R2C ? (size_t)(A * B_ / 2 + 1) * sizeof(cx<T>) : 0) <= 160 * 1024)      \
       launch_rows_t<T, A, B_, R2C, 2>(a, name, in, out, n_outer, n_inner, in_so, in_sb, out_so,  \
-                                      out_sk, tw);                                             \
+                                      out_sk, tw, ssq);                                        \
     else                                                                                       \
       launch_rows_t<T, A, B_, R2C, 1>(a, name, in, out, n_outer, n_inner, in_so, in_sb, out_so,  \
-                                      out_sk, tw);                                             \
+                                      out_sk, tw, ssq);                                        \
     return true;                                                                               \
   }
   CAPOW_FFT_SIZES(X)
 #undef X
   return false;
+}
+
+// ---------------------------------------------------------------------------
+// The x pass fused with the mirror-group binning (FP32, 1-D bins).
+//
+// After the z and y passes the grids hold u = [ky][kz][x] (x rows
+// contiguous, pitch px).  A work item is the x rows (j, kz) and (NY - j, kz)
+// of every grid for KZ consecutive kz (B = 2 G KZ rows, G grids): they are
+// transformed along x as the x pass does (two-level register FFT, natural
+// order in LDS), and the mirror groups (i, j), (NX - i, j), (i, NY - j),
+// (NX - i, NY - j) of those kz -- all in LDS -- are binned at once, so the
+// x pass output is never written.  The sums are exact int64 quanta, one
+// quantum per spectrum, added into ONE table per workgroup (integer adds do
+// not depend on their order: the bins are bit-reproducible).  With S_A the
+// sum of |A(k)|^2 + |A_int(k)|^2 over the stored modes (NX NY times the sum
+// of |v|^2 over the y pass input of those grids: Parseval along y and x),
+// S_B the same for the B grids, and |delta|^2 <= 2 (|A|^2 + |A_int|^2):
+//   auto A:  sum |NM cor L_l |delta|^2|       <= 4 cor_max S_A
+//   auto B:  sum |NM cor L_l |beta|^2|        <= 4 cor_max S_B
+//   cross:   sum |NM cor L_l Re/Im delta* beta| <= 2 cor_max sum |delta||beta|
+//                                              <= 4 cor_max sqrt(S_A S_B)
+// (Cauchy-Schwarz), so with that bound T_s < 2^e and q_s = 2^(e - 62) no
+// partial sum of any bin leaves int64 (each rounded add moves it by at most
+// 1/2), and q_s is about 2^-62 T_s: a spectrum's quantum follows its own
+// power (the density spectrum is not rounded on the momentum's scale).
+// ---------------------------------------------------------------------------
+
+// ssq rows (4 per workgroup of the y pass, one per transformed grid in the
+// order A, A_int, B, B_int of the grids present) -> xq[2 s], xq[2 s + 1] =
+// q_s, 1/q_s per spectrum s, fixed order.  factor = 4 cor_max NX NY.
+__global__ void k_xbin_scale(const double* __restrict__ ssq, int rows, double factor, int il,
+                             int hb, BinArgsDev a, double* __restrict__ xq) {
+  if (threadIdx.x != 0) return;
+  double g[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int r = 0; r < rows; r++)
+    for (int q = 0; q < 4; q++) g[q] += ssq[r * 4 + q];
+  const double sa = g[0] + (il ? g[1] : 0.0);
+  const double sb = hb ? (il ? g[2] + g[3] : g[1]) : 0.0;
+  for (int s = 0; s < 3; s++) {
+    double T = 0.0;
+    if (s < a.nspec) {
+      const int k = a.kind[s];
+      T = factor * (k == SPEC_AUTO_A ? sa : (k == SPEC_AUTO_B ? sb : sqrt(sa * sb)));
+      T *= 1.0 + 1e-5;   // the float partial sums of |v|^2 round by < 1e-5
+    }
+    int e = 0;
+    if (T > 0) frexp(T, &e);   // T < 2^e
+    xq[2 * s] = T > 0 ? ldexp(1.0, e - 62) : 1.0;
+    xq[2 * s + 1] = T > 0 ? ldexp(1.0, 62 - e) : 1.0;
+  }
+}
+
+template <int N1, int N2, int G, int KZ, bool IL, bool HB, bool ODD>
+__global__ __launch_bounds__((N1 * N2 / clcm(N1, N2)) * (2 * G * KZ)) void k_fft_x_bin(
+    const float* __restrict__ i0, const float* __restrict__ i1, const float* __restrict__ i2,
+    const float* __restrict__ i3, int64_t px, int64_t plane, const cx<float>* __restrict__ tw,
+    const double* __restrict__ tabs, BinArgsDev a, const double* __restrict__ xq,
+    long long* __restrict__ xpart, int nkzb) {
+  constexpr int N = N1 * N2, E = clcm(N1, N2), TS = N / E, G1 = E / N2;
+  constexpr int B = 2 * G * KZ;
+  constexpr bool TW_REG = G1 == 1;
+  extern __shared__ unsigned char smem[];
+  cx<float>* lds = reinterpret_cast<cx<float>*>(smem);
+  unsigned long long* tab = reinterpret_cast<unsigned long long*>(smem + (size_t)B * (N + N1) * 8);
+  constexpr int NL = ODD ? 5 : 3;   // multipoles summed: 0, 2, 4 [, 1, 3]
+  const int NK = a.nk, nl = NL, F = a.nspec * nl;
+  for (int q = threadIdx.x; q < F * NK; q += blockDim.x) tab[q] = 0ull;
+  const int t = threadIdx.x % TS, c = threadIdx.x / TS;
+  // row c of an item: grid c % G, j slot (c / G) % 2 (0: j, 1: NY - j),
+  // kz slot c / (2 G)
+  const int cg = c % G, cjs = (c / G) & 1, cks = c / (2 * G);
+  const int hy1 = a.ny / 2 + 1;
+  const int64_t total = (int64_t)hy1 * nkzb;
+  int64_t L0, Lend, step;
+  if ((gridDim.x & 7) == 0) {
+    const int64_t M = gridDim.x >> 3;
+    const int64_t chunk = (total + 7) >> 3;
+    const int64_t xcd = blockIdx.x & 7;
+    L0 = xcd * chunk + (blockIdx.x >> 3);
+    Lend = min(total, (xcd + 1) * chunk);
+    step = M;
+  } else {
+    L0 = blockIdx.x;
+    Lend = total;
+    step = gridDim.x;
+  }
+  cx<float> twr[TW_REG ? N2 : 1];
+  if constexpr (TW_REG) {
+#pragma unroll
+    for (int k2 = 0; k2 < N2; k2++) twr[k2] = tw[t * k2];
+  }
+  const float* gin = cg == 0 ? i0 : cg == 1 ? i1 : cg == 2 ? i2 : i3;
+  cx<float> v[E];
+  auto load = [&](int64_t item) {
+    const int jc = (int)(item / nkzb), kz = (int)(item - (int64_t)jc * nkzb) * KZ + cks;
+    const bool live = kz < a.nzh && (cjs == 0 || (jc > 0 && 2 * jc < a.ny));
+    const int j = cjs ? a.ny - jc : jc;
+    const cx<float>* row = reinterpret_cast<const cx<float>*>(gin) +
+                           (live ? (int64_t)j * plane + (int64_t)kz * px : 0);
+#pragma unroll
+    for (int g = 0; g < G1; g++)
+#pragma unroll
+      for (int n2 = 0; n2 < N2; n2++) {
+        const int n = (t + TS * g) + N1 * n2;
+        v[g * N2 + n2] = live ? row[n] : mk<float>(0.f, 0.f);
+      }
+  };
+  // binning split: KZ x PARTS units (kz slot, part of the canonical
+  // i = 0 .. NX/2), unit u to wave u % NW; lane l takes S consecutive i (S
+  // odd: the lanes' LDS reads, S complex apart, fall into distinct banks)
+  static_assert((TS * B) % 64 == 0, "whole waves");
+  constexpr int NW = TS * B / 64;
+  constexpr int PARTS = NW / KZ > 0 ? NW / KZ : 1;
+  constexpr int HX1 = N / 2 + 1;
+  constexpr int PART = (HX1 + PARTS - 1) / PARTS;
+  constexpr int S0 = (PART + 63) / 64;
+  constexpr int S = (S0 % 2) ? S0 : S0 + 1;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63;
+  const double iqs[3] = {xq[1], xq[3], xq[5]};
+  const typename CT<float>::t zero = {0.f, 0.f};
+  if (L0 < Lend) load(L0);
+  for (int64_t item = L0; item < Lend; item += step) {
+    rt_stage1<float, N1, N2, false, TW_REG>(v, twr, tw, lds, t, c);
+    __syncthreads();
+    if (item + step < Lend) load(item + step);
+    rt_stage2<float, N1, N2>(lds, t, c);
+    // bin the item's groups from LDS.  The item's y quantities (rows j and
+    // NY - j) are the same for every group; a lane takes S consecutive i,
+    // whose modes mostly share a k bin, and sums them in FP64 registers
+    // until the bin changes (one conversion to quanta and one set of LDS
+    // atomics per run).
+    const int jc = (int)(item / nkzb);
+    const bool my = jc > 0 && 2 * jc < a.ny;
+    const int j2 = my ? a.ny - jc : jc;
+    const double fy = tabs[a.ofy + jc], cory = tabs[a.ocy + jc];
+    const double pyr = tabs[a.opy + 2 * jc], pyi = tabs[a.opy + 2 * jc + 1];
+    const double pyr2 = tabs[a.opy + 2 * j2], pyi2 = tabs[a.opy + 2 * j2 + 1];
+    for (int unit = wave; unit < KZ * PARTS; unit += NW) {
+      const int wks = unit % KZ, ib = (unit / KZ) * PART, ie = min(HX1, ib + PART);
+      const int kz = (int)(item - (int64_t)jc * nkzb) * KZ + wks;
+      if (kz >= a.nzh) continue;
+      const double fz = tabs[a.ofz + kz], corz = tabs[a.ocz + kz];
+      const double pzr = tabs[a.opz + 2 * kz], pzi = tabs[a.opz + 2 * kz + 1];
+      int cur = -1;
+      double acc[3][NL];
+#pragma unroll
+      for (int sp = 0; sp < 3; sp++)
+#pragma unroll
+        for (int q = 0; q < NL; q++) acc[sp][q] = 0.0;
+      auto flush = [&]() {
+        if (cur < 0) return;
+#pragma unroll
+        for (int sp = 0; sp < 3; sp++) {
+          if (sp >= a.nspec) break;
+          unsigned long long* b = tab + sp * nl * NK + cur;
+#pragma unroll
+          for (int q = 0; q < NL; q++) {
+            atomicAdd(&b[q * NK], (unsigned long long)__double2ll_rn(acc[sp][q] * iqs[sp]));
+            acc[sp][q] = 0.0;
+          }
+        }
+      };
+      for (int u = 0; u < S; u++) {
+        const int i = ib + lane * S + u;
+        if (i >= ie) break;
+        const bool mx = i > 0 && 2 * i < a.nx;
+        const int i2 = mx ? a.nx - i : i;
+        Group Gp;
+        Gp.i = i;
+        Gp.j = jc;
+        Gp.fx = tabs[a.ofx + i];
+        Gp.fy = fy;
+        Gp.fxy = dadd(dmul(Gp.fx, Gp.fx), dmul(fy, fy));
+        Gp.corxy = tabs[a.ocx + i] * cory;
+        Gp.live[0] = true;
+        Gp.live[1] = mx;
+        Gp.live[2] = my;
+        Gp.live[3] = mx && my;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          Gp.qr[r] = 1.0;
+          Gp.qi[r] = 0.0;
+        }
+        if constexpr (IL) {
+          const double pxr = tabs[a.opx + 2 * i], pxi = tabs[a.opx + 2 * i + 1];
+          const double pxr2 = tabs[a.opx + 2 * i2], pxi2 = tabs[a.opx + 2 * i2 + 1];
+          const double xr[4] = {pxr, pxr2, pxr, pxr2}, xi[4] = {pxi, pxi2, pxi, pxi2};
+          const double yr[4] = {pyr, pyr, pyr2, pyr2}, yi[4] = {pyi, pyi, pyi2, pyi2};
+#pragma unroll
+          for (int r = 0; r < 4; r++) {
+            Gp.qr[r] = xr[r] * yr[r] - xi[r] * yi[r];
+            Gp.qi[r] = xr[r] * yi[r] + xi[r] * yr[r];
+          }
+        }
+        typename CT<float>::t vv[4][4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int js = r >> 1, ii = (r & 1) ? i2 : i;
+          const bool lv = Gp.live[r];
+#pragma unroll
+          for (int q = 0; q < 4; q++) {
+            // grid slot q (A, A_int, B, B_int) -> loaded grid index
+            const int gi = q == 0 ? 0 : q == 1 ? (IL ? 1 : -1) : q == 2 ? (HB ? (IL ? 2 : 1) : -1)
+                                                                        : (IL && HB ? 3 : -1);
+            vv[r][q] = (lv && gi >= 0) ? lds[((wks * 2 + js) * G + gi) * (N + 1) + ii] : zero;
+          }
+        }
+        ModeVals mv =
+            group_mode<typename CT<float>::t, IL, HB, true>(Gp, kz, fz, corz, pzr, pzi, vv, a);
+        if (mv.near) {
+          const KBin e = kbin_exact(Gp.fxy, fz, a);
+          mv.kbin = e.kbin;
+          mv.ok = e.ok;
+        }
+        if (!mv.ok) continue;
+        if (mv.kbin != cur) {
+          flush();
+          cur = mv.kbin;
+        }
+#pragma unroll
+        for (int sp = 0; sp < 3; sp++) {
+          const double np = mv.np[sp];
+          acc[sp][0] += np;
+          acc[sp][1] += mv.L2 * np;
+          acc[sp][2] += mv.L4 * np;
+          if constexpr (ODD) {
+            acc[sp][3] += mv.mu * np;
+            acc[sp][4] += (2.5 * mv.mu * mv.mu * mv.mu - 1.5 * mv.mu) * np;
+          }
+        }
+      }
+      flush();
+    }
+    // the next item's first stage overwrites the rows
+    __syncthreads();
+  }
+  long long* out = xpart + (int64_t)blockIdx.x * F * NK;
+  for (int q = threadIdx.x; q < F * NK; q += blockDim.x) out[q] = (long long)tab[q];
+}
+
+// Workgroup tables summed exactly, times their spectrum's quantum: the sums
+// of L.out ([s][l][k], nlk = nl NK per spectrum), then the mode counts (the
+// k_bin_final layout of the branch-free forms).
+__global__ void k_xbin_final(const long long* __restrict__ xpart, int nwg, int fnk, int nk,
+                             int nlk, const double* __restrict__ counts,
+                             const double* __restrict__ xq, double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < fnk) {
+    long long t = 0;
+    for (int w = 0; w < nwg; w++) t += xpart[(int64_t)w * fnk + i];
+    out[i] = (double)t * xq[2 * (i / nlk)];
+  } else if (i < fnk + nk) {
+    out[i] = counts[i - fnk];
+  }
+}
+
+// rows per item for G grids: 16 (KZ = 8/G) when the rows and the table fit
+// the LDS beside each other, else 8
+static int xbin_rows(int N, int N1, const BinArgsDev& b, size_t* lds) {
+  const int F = b.nspec * (b.odd ? 5 : 3);
+  const size_t tabb = (size_t)F * b.nk * 8;
+  for (int B = 16; B >= 8; B /= 2) {
+    const size_t l = (size_t)B * (N + N1) * 8 + tabb;
+    if (l <= 160 * 1024) {
+      *lds = l;
+      return B;
+    }
+  }
+  *lds = 0;
+  return 0;
+}
+
+size_t xbin_lds(const Geom& g, const BinArgsDev& b) {
+  int n1, n2;
+  if (!reg_split(g.n[0], &n1, &n2) || b.out2d) return 0;
+  size_t l;
+  return xbin_rows(g.n[0], n1, b, &l) ? l : 0;
+}
+
+template <int N1, int N2, int G, int KZ, bool IL, bool HB, bool ODD>
+static int launch_xbin(FftArgs& f, BinLaunch& L, long long* xpart, int xpart_rows, size_t lds) {
+  constexpr int N = N1 * N2, E = clcm(N1, N2), TS = N / E, B = 2 * G * KZ;
+  const void* fn = (const void*)k_fft_x_bin<N1, N2, G, KZ, IL, HB, ODD>;
+  if (lds > 64 * 1024) hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  int per_cu = 1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, TS * B, lds) != hipSuccess || per_cu < 1)
+    per_cu = 1;
+  const int nzh = f.g->n[2] / 2 + 1;
+  const int nkzb = (nzh + KZ - 1) / KZ;
+  const int64_t total = (int64_t)(f.g->n[1] / 2 + 1) * nkzb;
+  int64_t nblk = (int64_t)(f.ncu > 0 ? f.ncu : 256) * per_cu;
+  if (nblk > total) nblk = total;
+  if (nblk > xpart_rows) nblk = xpart_rows;
+  const float* g[4];
+  for (int q = 0; q < 4; q++) g[q] = (const float*)(q < f.ngrids ? f.grid[q] : f.grid[0]);
+  TIMED(f.timer, "fft_x_bin",
+        (k_fft_x_bin<N1, N2, G, KZ, IL, HB, ODD><<<(unsigned)nblk, TS * B, lds, f.stream>>>(
+            g[0], g[1], g[2], g[3], f.px, (int64_t)nzh * f.px, (const cx<float>*)f.tw[0], L.tabs,
+            L.dev, f.xq, xpart, nkzb)));
+  const int nlk = (L.dev.odd ? 5 : 3) * L.dev.nk, fnk = L.dev.nspec * nlk;
+  TIMED(f.timer, "bin_final",
+        (k_xbin_final<<<(fnk + L.dev.nk + 255) / 256, 256, 0, f.stream>>>(
+            xpart, (int)nblk, fnk, L.dev.nk, nlk, L.counts, f.xq, L.out)));
+  return hipGetLastError() == hipSuccess ? CAPOW_OK : CAPOW_ERR_HIP;
+}
+
+template <int N1, int N2, bool IL, bool HB>
+static int launch_xbin_rows(FftArgs& f, BinLaunch& L, long long* xpart, int xpart_rows) {
+  constexpr int G = 1 + (IL ? 1 : 0) + (HB ? 1 : 0) + (IL && HB ? 1 : 0);
+  constexpr int TS = N1 * N2 / clcm(N1, N2);
+  size_t lds;
+  const int B = xbin_rows(N1 * N2, N1, L.dev, &lds);
+  const bool odd = L.dev.odd != 0;
+  if constexpr ((TS * 16) % 64 == 0)
+    if (B == 16)
+      return odd ? launch_xbin<N1, N2, G, 8 / G, IL, HB, true>(f, L, xpart, xpart_rows, lds)
+                 : launch_xbin<N1, N2, G, 8 / G, IL, HB, false>(f, L, xpart, xpart_rows, lds);
+  if constexpr ((TS * 8) % 64 == 0 && G <= 4)
+    if (B == 8)
+      return odd ? launch_xbin<N1, N2, G, 4 / G, IL, HB, true>(f, L, xpart, xpart_rows, lds)
+                 : launch_xbin<N1, N2, G, 4 / G, IL, HB, false>(f, L, xpart, xpart_rows, lds);
+  return CAPOW_ERR_UNSUPPORTED;
+}
+
+int run_fft_x_bin(FftArgs& f, BinLaunch& L, long long* xpart, int xpart_rows) {
+  int n1, n2;
+  if (!reg_split(f.g->n[0], &n1, &n2) || L.dev.out2d) return CAPOW_ERR_UNSUPPORTED;
+  if (!f.ssq || f.ssq_rows < 1 || f.ssq_rows > XBIN_SSQ_MAX || !f.xq) return CAPOW_ERR_UNSUPPORTED;
+  const int rc = run_bin_counts(L);
+  if (rc) return rc;
+  const bool il = L.dev.interlace != 0, hb = L.dev.hasB != 0;
+  k_xbin_scale<<<1, 64, 0, f.stream>>>(f.ssq, f.ssq_rows, f.xbin_factor, il, hb, L.dev, f.xq);
+#define X(A_, B_)                                                                          \
+  if (n1 == A_ && n2 == B_) {                                                              \
+    if (il && hb) return launch_xbin_rows<A_, B_, true, true>(f, L, xpart, xpart_rows);    \
+    if (il) return launch_xbin_rows<A_, B_, true, false>(f, L, xpart, xpart_rows);         \
+    if (hb) return launch_xbin_rows<A_, B_, false, true>(f, L, xpart, xpart_rows);         \
+    return launch_xbin_rows<A_, B_, false, false>(f, L, xpart, xpart_rows);                \
+  }
+  CAPOW_FFT_SIZES(X)
+#undef X
+  return CAPOW_ERR_UNSUPPORTED;
 }
 
 // Out-of-place pipeline: every pass reads whole rows and writes runs of the
@@ -984,13 +1354,39 @@ static bool run_fft_transposed(FftArgs& a) {
   const bool kyz = sizeof(T) == 4;
   const int64_t y_so = kyz ? px : (int64_t)ny * px, y_sk = kyz ? nzh * px : px;
   const int64_t x_so = kyz ? nzh * px : px, x_sb = kyz ? px : (int64_t)ny * px;
+  const bool defer = a.defer_x && kyz && a.ssq && a.xq;
   if (!try_rows_t<T, false>(a, "fft_y", ny, S, G, nzh, nx, py, nzh * py, y_so, y_sk, a.tw[1],
-                            my))
+                            my, defer ? a.ssq : nullptr))
     return false;
+  if (defer && a.ssq_rows <= XBIN_SSQ_MAX) {
+    a.x_deferred = true;
+    return true;
+  }
+  // (more y-pass workgroups than ssq rows: the x pass runs now)
   if (!try_rows_t<T, false>(a, "fft_x", nx, G, S, ny, nzh, x_so, x_sb, rowc, (int64_t)ny * rowc,
                             a.tw[0], mx))
     return false;
   return true;
+}
+
+// The x pass left pending by a deferred run_fft: grid [ky][kz][x] ->
+// scratch [kx][ky][kz] (FP32 layout of run_fft_transposed).
+int run_fft_x(FftArgs& a) {
+  const Geom& g = *a.g;
+  const int nx = g.n[0], ny = g.n[1];
+  const int64_t nzh = g.n[2] / 2 + 1, rowc = g.rowc, px = a.px;
+  float* G[4];
+  float* S[4];
+  for (int q = 0; q < 4; q++) {
+    G[q] = q < a.ngrids ? (float*)a.grid[q] : (float*)a.grid[0];
+    S[q] = q < a.ngrids ? (float*)a.scratch[q] : (float*)a.scratch[0];
+  }
+  const int mx = a.rows_mul ? a.rows_mul : 2;
+  if (!try_rows_t<float, false>(a, "fft_x", nx, G, S, ny, nzh, nzh * px, px, rowc,
+                                (int64_t)ny * rowc, a.tw[0], mx))
+    return CAPOW_ERR_UNSUPPORTED;
+  a.x_deferred = false;
+  return hipGetLastError() == hipSuccess ? CAPOW_OK : CAPOW_ERR_HIP;
 }
 
 template <typename T>
@@ -998,8 +1394,9 @@ static int run_fft_t(FftArgs& a) {
   const Geom& g = *a.g;
   hipStream_t st = a.stream;
   a.swapped = false;
+  a.x_deferred = false;
   if (a.fast && a.scratch[0] && run_fft_transposed<T>(a)) {
-    a.swapped = true;
+    a.swapped = !a.x_deferred;
     return hipGetLastError() == hipSuccess ? CAPOW_OK : CAPOW_ERR_HIP;
   }
   T* gp[4];

@@ -130,8 +130,25 @@ struct FftArgs {
   int ncu = 256;          // compute units (persistent grids)
   int persist = 1;        // 0: one workgroup per batch instead of a persistent grid
   int order = 0;          // bit per pass (z, y, x): batches with the outer index fastest
+  // FP32 out-of-place pipeline with the x pass deferred to the binning
+  // (run_fft_x_bin): defer_x asks for the z and y passes only; x_deferred
+  // reports that they ran (the y pass output stays in grid[]).  The y pass
+  // then sums |u|^2 of its input per grid and workgroup into ssq (XBIN_SSQ_MAX
+  // rows of 4 doubles) and k_xbin_scale turns them into the binning quantum
+  // xq = {q, 1/q}: xbin_factor x NX NY sum_u |u|^2 bounds the sum of the
+  // magnitudes of every binned contribution (Parseval).
+  bool defer_x = false;
+  bool x_deferred = false;
+  double* ssq = nullptr;
+  double xbin_factor = 0.0;
+  double* xq = nullptr;
+  int ssq_rows = 0;       // out: workgroups of the y pass (rows of ssq)
 };
+#define XBIN_SSQ_MAX 8192
 int run_fft(FftArgs& a, bool fp64);
+// The pending x pass after run_fft with x_deferred (grid -> scratch; the
+// caller swaps the buffers as after a whole run_fft).
+int run_fft_x(FftArgs& a);
 
 // adjoint of run_fft on gradient grids (register FFT sizes only)
 int run_ifft(FftArgs& a, bool fp64);
@@ -192,6 +209,14 @@ size_t bin_table_len(const capow_params& p);
 int validate_params(const capow_params* p, std::string& err);
 void set_global_error(const std::string& err);
 int run_bin(BinLaunch& L, bool fp64);
+// The x pass fused with the mirror-group binning (FP32, 1-D bins): the y
+// pass output in f.grid[] is transformed along x and binned without being
+// written; int64 quanta of f.xq per workgroup into xpart, summed into L.out
+// (sums, then the mode counts from L.counts, which must be valid).
+// xbin_lds: the LDS the fused kernel needs (0: this geometry has no fused
+// kernel).
+size_t xbin_lds(const Geom& g, const BinArgsDev& b);
+int run_fft_x_bin(FftArgs& f, BinLaunch& L, long long* xpart, int xpart_rows);
 // The pieces of run_bin around the accumulation kernel: the once-per-context
 // mode counts of the branch-free forms, and the final sum of the partials.
 int run_bin_counts(BinLaunch& L);

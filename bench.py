@@ -47,6 +47,7 @@ def algorithmic_bytes(kernel, n_p, ng_reals, G, s):
         "fft_z": 2 * G * grid,                 # read + write every grid
         "fft_y": 2 * G * grid,
         "fft_x": 2 * G * grid,
+        "fft_x_bin": G * grid,                 # x pass fused with the binning: read every grid once
         "bin": G * grid,                       # read every grid once
         "bucket_count": n_p * bp,
         "bucket_scatter": 2 * n_p * bp,
@@ -448,6 +449,75 @@ def cpu_baseline(geom_small, nl_small):
                          fft_scale, scale_g)))
 
 
+def host_cpu():
+    """The host CPU model and the CPUs this process may run on."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return dict(model=model, nproc=os.cpu_count(), affinity=len(os.sched_getaffinity(0)))
+
+
+C0_N, C0_L, C0_NP, C0_NK, C0_SEED = 256, 1000.0, 1_000_000, 80, 20
+
+
+def c0_params(mod, prec=None):
+    kw = dict(nx=C0_N, ny=C0_N, nz=C0_N, xmax=C0_L, ymax=C0_L, zmax=C0_L, interp_order=2,
+              do_interlacing=0, los=0, momentum=0, nk=C0_NK, maxk=C0_NK * 0.01)
+    if prec is not None:
+        kw["precision"] = prec
+    return mod.make_params(**kw) if hasattr(mod, "make_params") else mod.params(**kw)
+
+
+def run_c0_gpu(steps=20):
+    """The reference's CPU-runnable configuration on the GPU: 1,000,000
+    uniform particles, 256^3, CIC, P_0, dk = 0.01, FP64 and FP32 forms;
+    ms per step (inputs resident)."""
+    import torch
+    from capow_amd import api, mocks
+    pos, vel = mocks.uniform(C0_NP, C0_L, seed=C0_SEED)
+    out = {}
+    for name, prec, dt in (("fp64", api.FP64, torch.float64), ("fp32", api.FP32, torch.float32)):
+        x = torch.tensor(mocks.xyzv(pos, vel, 0), device="cuda", dtype=dt)
+        ps = api.PowerSpectrum(c0_params(api, prec))
+        ps.set_particles(x)
+        for _ in range(3):
+            ps.power()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ps.power()
+        torch.cuda.synchronize()
+        out[name + "_ms"] = round((time.perf_counter() - t0) * 1e3 / steps, 3)
+        ps.close()
+    return out
+
+
+def cpu_c0():
+    """The FP64 oracle (one thread) at the reference's CPU configuration
+    exactly: stage times of assignment + mean subtraction, the r2c FFT and
+    the binning."""
+    from capow_amd import mocks
+    from oracle import pyoracle as O
+    pos, vel = mocks.uniform(C0_NP, C0_L, seed=C0_SEED)
+    p = c0_params(O)
+    t0 = time.perf_counter()
+    rc, grids, st = O.assign(p, pos, vel)
+    alpha, shot, norm = O.scalars(p, st)
+    O.mean_subtract(p, alpha, grids)
+    t1 = time.perf_counter()
+    O.fft_r2c_3d(grids[0], C0_N, C0_N, C0_N)
+    t2 = time.perf_counter()
+    O.compute_power(p, grids)
+    t3 = time.perf_counter()
+    return dict(assign_ms=round((t1 - t0) * 1e3, 1), fft_ms=round((t2 - t1) * 1e3, 1),
+                bin_ms=round((t3 - t2) * 1e3, 1), total_ms=round((t3 - t0) * 1e3, 1))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -541,11 +611,24 @@ def main():
     adjoint = None if args.no_adjoint else run_adjoint(args)
 
     cpu = None
+    c0 = run_c0_gpu()
+    log("C0 on the GPU: FP64 %.3f ms, FP32 %.3f ms per step" % (c0["fp64_ms"], c0["fp32_ms"]))
     if not args.no_cpu_baseline:
         log("CPU baseline (oracle sample)")
         small = dict(n=256, L=L, dk=dk, nk=int(np.floor(np.pi * 256 / L / dk)))
         cpu = cpu_baseline(small, 200)
         log("  est %.0f ms" % cpu["value"])
+        oc0 = cpu_c0()
+        log("  C0 measured: %.0f ms (assign %.0f, FFT %.0f, binning %.0f)" %
+            (oc0["total_ms"], oc0["assign_ms"], oc0["fft_ms"], oc0["bin_ms"]))
+        cpu["value_is"] = ("an estimate for the workload: the sample's measured stage times "
+                           "scaled to 1024^3 / 134M particles")
+        cpu["host"] = host_cpu()
+        cpu["c0_measured"] = dict(
+            config=("C0: 1,000,000 uniform particles (seed %d), L=1000, 256^3, CIC, no "
+                    "interlacing, P_0, dk=0.01, NK=80, FP64" % C0_SEED),
+            oracle_ms=oc0["total_ms"], oracle_stages_ms=oc0, cores=1,
+            gpu_fp64_ms=c0["fp64_ms"], gpu_fp32_ms=c0["fp32_ms"])
 
     res = f32["res"]
     out = {

@@ -1004,12 +1004,26 @@ static bool try_rows_t(FftArgs& a, const char* name, int N, T* const* in, T* con
 // ssq rows (4 per workgroup of the y pass, one per transformed grid in the
 // order A, A_int, B, B_int of the grids present) -> xq[2 s], xq[2 s + 1] =
 // q_s, 1/q_s per spectrum s, fixed order.  factor = 4 cor_max NX NY.
-__global__ void k_xbin_scale(const double* __restrict__ ssq, int rows, double factor, int il,
-                             int hb, BinArgsDev a, double* __restrict__ xq) {
+__global__ __launch_bounds__(256) void k_xbin_scale(const double* __restrict__ ssq, int rows,
+                                                    double factor, int il, int hb, BinArgsDev a,
+                                                    double* __restrict__ xq) {
+  // each thread sums a strided subset of the rows, then a fixed tree
+  __shared__ double red[4][256];
+  double part[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int r = threadIdx.x; r < rows; r += 256)
+#pragma unroll
+    for (int q = 0; q < 4; q++) part[q] += ssq[r * 4 + q];
+#pragma unroll
+  for (int q = 0; q < 4; q++) red[q][threadIdx.x] = part[q];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+#pragma unroll
+      for (int q = 0; q < 4; q++) red[q][threadIdx.x] += red[q][threadIdx.x + w];
+    __syncthreads();
+  }
   if (threadIdx.x != 0) return;
-  double g[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int r = 0; r < rows; r++)
-    for (int q = 0; q < 4; q++) g[q] += ssq[r * 4 + q];
+  const double g[4] = {red[0][0], red[1][0], red[2][0], red[3][0]};
   const double sa = g[0] + (il ? g[1] : 0.0);
   const double sb = hb ? (il ? g[2] + g[3] : g[1]) : 0.0;
   for (int s = 0; s < 3; s++) {
@@ -1215,20 +1229,30 @@ __global__ __launch_bounds__((N1 * N2 / clcm(N1, N2)) * (2 * G * KZ)) void k_fft
   for (int q = threadIdx.x; q < F * NK; q += blockDim.x) out[q] = (long long)tab[q];
 }
 
-// Workgroup tables summed exactly, times their spectrum's quantum: the sums
-// of L.out ([s][l][k], nlk = nl NK per spectrum), then the mode counts (the
-// k_bin_final layout of the branch-free forms).
-__global__ void k_xbin_final(const long long* __restrict__ xpart, int nwg, int fnk, int nk,
-                             int nlk, const double* __restrict__ counts,
-                             const double* __restrict__ xq, double* __restrict__ out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < fnk) {
-    long long t = 0;
-    for (int w = 0; w < nwg; w++) t += xpart[(int64_t)w * fnk + i];
-    out[i] = (double)t * xq[2 * (i / nlk)];
-  } else if (i < fnk + nk) {
-    out[i] = counts[i - fnk];
+// Workgroup tables summed exactly (integer sums: any order gives the same
+// result), times their spectrum's quantum: the sums of L.out ([s][l][k],
+// nlk = nl NK per spectrum), then the mode counts (the k_bin_final layout of
+// the branch-free forms).  One block per entry.
+__global__ __launch_bounds__(256) void k_xbin_final(const long long* __restrict__ xpart, int nwg,
+                                                    int fnk, int nk, int nlk,
+                                                    const double* __restrict__ counts,
+                                                    const double* __restrict__ xq,
+                                                    double* __restrict__ out) {
+  const int i = blockIdx.x;
+  if (i >= fnk) {
+    if (threadIdx.x == 0 && i < fnk + nk) out[i] = counts[i - fnk];
+    return;
   }
+  __shared__ long long red[256];
+  long long t = 0;
+  for (int w = threadIdx.x; w < nwg; w += 256) t += xpart[(int64_t)w * fnk + i];
+  red[threadIdx.x] = t;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[i] = (double)red[0] * xq[2 * (i / nlk)];
 }
 
 // rows per item for G grids: 16 (KZ = 8/G) when the rows and the table fit
@@ -1265,8 +1289,11 @@ static int launch_xbin(FftArgs& f, BinLaunch& L, long long* xpart, int xpart_row
   const int nzh = f.g->n[2] / 2 + 1;
   const int nkzb = (nzh + KZ - 1) / KZ;
   const int64_t total = (int64_t)(f.g->n[1] / 2 + 1) * nkzb;
+  // at least a few items per workgroup (each workgroup's table is summed at
+  // the end), at most what fits the CUs at once
   int64_t nblk = (int64_t)(f.ncu > 0 ? f.ncu : 256) * per_cu;
-  if (nblk > total) nblk = total;
+  if (nblk > (total + 3) / 4) nblk = (total + 3) / 4;
+  if (nblk < 1) nblk = 1;
   if (nblk > xpart_rows) nblk = xpart_rows;
   const float* g[4];
   for (int q = 0; q < 4; q++) g[q] = (const float*)(q < f.ngrids ? f.grid[q] : f.grid[0]);
@@ -1276,8 +1303,8 @@ static int launch_xbin(FftArgs& f, BinLaunch& L, long long* xpart, int xpart_row
             L.dev, f.xq, xpart, nkzb)));
   const int nlk = (L.dev.odd ? 5 : 3) * L.dev.nk, fnk = L.dev.nspec * nlk;
   TIMED(f.timer, "bin_final",
-        (k_xbin_final<<<(fnk + L.dev.nk + 255) / 256, 256, 0, f.stream>>>(
-            xpart, (int)nblk, fnk, L.dev.nk, nlk, L.counts, f.xq, L.out)));
+        (k_xbin_final<<<fnk + L.dev.nk, 256, 0, f.stream>>>(xpart, (int)nblk, fnk, L.dev.nk,
+                                                             nlk, L.counts, f.xq, L.out)));
   return hipGetLastError() == hipSuccess ? CAPOW_OK : CAPOW_ERR_HIP;
 }
 
@@ -1306,7 +1333,7 @@ int run_fft_x_bin(FftArgs& f, BinLaunch& L, long long* xpart, int xpart_rows) {
   const int rc = run_bin_counts(L);
   if (rc) return rc;
   const bool il = L.dev.interlace != 0, hb = L.dev.hasB != 0;
-  k_xbin_scale<<<1, 64, 0, f.stream>>>(f.ssq, f.ssq_rows, f.xbin_factor, il, hb, L.dev, f.xq);
+  k_xbin_scale<<<1, 256, 0, f.stream>>>(f.ssq, f.ssq_rows, f.xbin_factor, il, hb, L.dev, f.xq);
 #define X(A_, B_)                                                                          \
   if (n1 == A_ && n2 == B_) {                                                              \
     if (il && hb) return launch_xbin_rows<A_, B_, true, true>(f, L, xpart, xpart_rows);    \

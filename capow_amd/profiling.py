@@ -32,6 +32,11 @@ STAGE_PATTERNS = [
     ("scan", r"k_scan_"),
     ("bucket_scatter", r"k_bucket<.*true>"),
     ("deposit", r"k_deposit"),
+    # the x pass fused with the binning (FP32), its quanta and final sums
+    ("fft_x_bin", r"k_fft_x_bin"),
+    ("xbin_scale", r"k_xbin_scale"),
+    ("bin_final", r"k_xbin_final"),
+    ("bin_counts_clear", r"k_fill_zero"),
     # transposed pipeline: the r2c row pass (5th template argument true) is
     # z, the two c2c passes y then x
     ("fft_yx", r"k_fft_rows_t<[^,]+, \d+, \d+, \d+, false"),

@@ -90,3 +90,44 @@ def test_program_output_matches_oracle(case, oracle, tmp_path):
     if tags["Output2D"]:
         a2, b2 = _rows(out + "_2D"), _rows(ref + ".g_2D")
         assert a2.shape == b2.shape and np.allclose(a2, b2, rtol=2e-5, atol=1e-5 * scale)
+
+
+@pytest.mark.parametrize("case", CASES[:2], ids=[c[0] for c in CASES[:2]])
+def test_program_fp32_form_within_1e4_of_fp64(case, tmp_path):
+    """CAPOW_PRECISION=fp32 runs the program's FP32 form (FP32 grids and FFT;
+    on a 64^3 grid the x pass fused with the binning): same mode counts as
+    the FP64 program output, P_l within 1e-4 of the bin's scale for
+    k < k_Nyq/2, on a catalogue whose numbers are float32-representable."""
+    name, over = case
+    over = dict(over, NX=64, NY=64, NZ=64)
+    print("host program fp32 case", name, flush=True)
+    pos, _ = mocks.uniform(40000, L, 23)
+    vel = np.random.default_rng(6).normal(0, 200.0, size=pos.shape)
+    pos = pos.astype(np.float32).astype(np.float64)
+    vel = vel.astype(np.float32).astype(np.float64)
+    if over.get("LOS", 0) == 0:
+        pos = np.clip(pos, 0, np.float32(np.nextafter(np.float32(L), 0)))
+    params = write_params(str(tmp_path / "run_parameters.dat"), **over)
+    cat = write_catalogue(str(tmp_path / "cat.txt"), pos, vel)
+    outs = {}
+    for prec in ("fp64", "fp32"):
+        out = str(tmp_path / ("pk_%s.txt" % prec))
+        env = dict(os.environ, CAPOW_PRECISE_OUTPUT="1", CAPOW_PRECISION=prec)
+        r = subprocess.run([EXE, params, cat, "unused.txt", out], capture_output=True, text=True,
+                           env=env, timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        outs[prec] = _rows(out + ".precise")
+        if prec == "fp64":
+            with open(out + ".precise") as f:
+                head = [ln for ln in f if ln.startswith("# shot")]
+            shot_norm = float(head[0].split("=")[1])
+    a, b = outs["fp32"], outs["fp64"]
+    assert a.shape == b.shape
+    assert np.array_equal(a[:, -1], b[:, -1]), "mode counts"
+    knyq = np.pi * 64 / L
+    sel = (b[:, 0] < knyq / 2) & (b[:, -1] > 0)
+    assert sel.sum() >= 4
+    scale = np.abs(b[sel, 1]) + abs(shot_norm)      # |P_0| + shot/norm per bin
+    rel = (np.abs(a[sel, 1:-1] - b[sel, 1:-1]).max(axis=1) / scale).max()
+    print("  max rel", rel, flush=True)
+    assert rel <= 1e-4

@@ -64,6 +64,9 @@ def run_form(prec, parts, n_p, args, geom):
     ps = api.PowerSpectrum(p)
     ps.set_particles(parts)
     import torch
+    # the project's autotuner picks the sort, deposit and FFT settings on
+    # this catalogue before the warmup (untimed; its report is recorded)
+    tune = None if args.no_autotune else ps.autotune(reps=2)
     for _ in range(args.warmup):
         res = ps.power()
     torch.cuda.synchronize()
@@ -81,7 +84,8 @@ def run_form(prec, parts, n_p, args, geom):
     ms = (time.perf_counter() - t0) * 1e3 / steps
     ktimes = ps.kernel_times()
     ps.set_timing(False)
-    out = dict(ms=ms, kernels=ktimes, res=res, stats=ps.stats, params=p, steps=steps)
+    out = dict(ms=ms, kernels=ktimes, res=res, stats=ps.stats, params=p, steps=steps,
+               autotune=(tune.strip().splitlines()[-1] if tune else None))
     ps.close()
     return out
 
@@ -108,15 +112,29 @@ def run_batch(args):
     log("batch: %d mocks, %d particles generated in %.1f s" % (len(cats), ntot, time.perf_counter() - t0))
     b = api.Batch(cfgs, max_particles=int(counts.max()), nstreams=args.mock_streams)
     b.run(cats, which)                    # warmup: one untimed pass over every catalogue
+    tune = None if args.no_autotune else b.autotune(cats, which, reps=2)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     res, stats, ms = b.run(cats, which)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    # graph mode: the same list captured into one HIP graph (untimed first
+    # run: eager + capture), then one timed replay, checked against the
+    # eager run's bins
+    b.set_graph(True)
+    b.run(cats, which)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    gres, _, _ = b.run(cats, which)
+    torch.cuda.synchronize()
+    gwall = time.perf_counter() - t0
+    same = all(np.array_equal(a[c]["pk"], g[c]["pk"]) for a, g in zip(res, gres) for c in a["codes"])
     b.close()
     del cats
     torch.cuda.empty_cache()
-    out = dict(mocks=len(counts), mocks_per_s=round(len(counts) / wall, 1),
+    chosen = [ln.strip() for ln in tune.splitlines() if ln.startswith("chosen")] if tune else None
+    out = dict(mocks=len(counts), mocks_per_s=round(len(counts) / wall, 1), autotune=chosen,
+               graph_mocks_per_s=round(len(counts) / gwall, 1), graph_bins_equal_eager=bool(same),
                wall_s=round(wall, 4), p50_ms=round(float(np.percentile(ms, 50)), 3),
                p99_ms=round(float(np.percentile(ms, 99)), 3), particles=ntot,
                mparticles_per_s=round(ntot / wall / 1e6, 1), streams=args.mock_streams,
@@ -124,8 +142,8 @@ def run_batch(args):
                interp="CIC+interlacing", dtype="f32 grids, f64 bins",
                per_mock_ms=("GPU time from a mock's first to last stage on its stream, "
                             "sharing the GPU with the other streams"))
-    log("batch: %.1f mocks/s, p50 %.3f ms, p99 %.3f ms per mock" % (out["mocks_per_s"],
-                                                                   out["p50_ms"], out["p99_ms"]))
+    log("batch: %.1f mocks/s (graph replay %.1f, bins equal %s), p50 %.3f ms, p99 %.3f ms per mock"
+        % (out["mocks_per_s"], out["graph_mocks_per_s"], same, out["p50_ms"], out["p99_ms"]))
     return out
 
 
@@ -529,8 +547,10 @@ def main():
     ap.add_argument("--grid", type=int, default=1024, help="grid size (1024 = the workload)")
     ap.add_argument("--lattice", type=int, default=512)
     ap.add_argument("--mocks", type=int, default=512, help="batched mocks (0: skip)")
-    ap.add_argument("--mock-streams", type=int, default=6)
+    ap.add_argument("--mock-streams", type=int, default=8)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default settings (no capow_autotune before the timed steps)")
     ap.add_argument("--no-survey", action="store_true")
     ap.add_argument("--no-bin-variants", action="store_true")
     ap.add_argument("--profile-pass", action="store_true",
@@ -652,6 +672,7 @@ def main():
             "spectra": res["codes"], "nk": nk,
         },
         "fp64_ms_per_step": round(f64["ms"], 3) if f64 else None,
+        "autotune": {"fp32": f32["autotune"], "fp64": f64["autotune"] if f64 else None},
         "fp32_speedup_vs_fp64": round(f64["ms"] / f32["ms"], 3) if f64 else None,
         "mparticles_per_s": round(n_p / (f32["ms"] * 1e-3) / 1e6, 1),
         "assign_ms": round(assign_ms, 3),

@@ -125,6 +125,10 @@ SIGNATURES = {
     "capow_batch_run": ([_V, C.POINTER(Mock), C.c_int32, C.POINTER(Bins),
                          C.POINTER(AssignStats), C.POINTER(C.c_float)], C.c_int),
     "capow_batch_destroy": ([_V], C.c_int),
+    "capow_batch_set_graph": ([_V, C.c_int32], C.c_int),
+    "capow_autotune": ([_V, C.c_int32, C.c_char_p, C.c_int32], C.c_int),
+    "capow_batch_autotune": ([_V, C.POINTER(Mock), C.c_int32, C.c_int32, C.c_char_p, C.c_int32],
+                             C.c_int),
     "capow_batch_last_error": ([_V], C.c_char_p),
     "capow_cspline_init": ([C.c_int32, _D, _D, _D], C.c_int),
     "capow_cspline_eval": ([C.c_int32, _D, _D, _D, C.c_double, _D], C.c_int),
@@ -385,6 +389,13 @@ class PowerSpectrum:
     def reset_kernel_times(self):
         self.check(lib().capow_reset_kernel_times(self.ctx))
 
+    def autotune(self, reps=3):
+        """Times the tunables on the particles set (capow_autotune); returns
+        the report text (every timing and the choice)."""
+        buf = C.create_string_buffer(8192)
+        self.check(lib().capow_autotune(self.ctx, int(reps), buf, 8192))
+        return buf.value.decode()
+
 
 def scalars(p, stats, code):
     shot, norm = C.c_double(), C.c_double()
@@ -476,6 +487,25 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+    def set_graph(self, on=True):
+        """capow_batch_set_graph: capture a catalogue list into one HIP graph
+        after its first run and replay it while the list is unchanged."""
+        rc = lib().capow_batch_set_graph(self.h, 1 if on else 0)
+        if rc:
+            raise CapowError(rc, lib().capow_batch_last_error(self.h).decode())
+
+    def autotune(self, catalogues, configs, reps=3):
+        """capow_batch_autotune on these catalogues; returns the report."""
+        n = len(catalogues)
+        mocks = (Mock * n)()
+        for i, (t, q) in enumerate(zip(catalogues, configs)):
+            mocks[i] = Mock(t.data_ptr(), None, int(t.shape[0]), int(q), 0)
+        buf = C.create_string_buffer(16384)
+        rc = lib().capow_batch_autotune(self.h, mocks, n, int(reps), buf, 16384)
+        if rc:
+            raise CapowError(rc, lib().capow_batch_last_error(self.h).decode())
+        return buf.value.decode()
 
     def run(self, catalogues, configs, weights=None):
         """catalogues: list of device tensors; configs: list of config indices.

@@ -5,6 +5,10 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <stdarg.h>
+
+#include <algorithm>
+#include <chrono>
 #include <string>
 #include <vector>
 
@@ -755,8 +759,10 @@ static int assign_enqueue(capow_ctx* c, double* h_stats, unsigned int* h_errs, b
   c->x_pending = false;   // the deposit rewrites every grid
   Geom& g = c->g;
   g.has_w = c->wts != nullptr;
-  HIPCHK(c, hipMemsetAsync(c->counts, 0, (size_t)g.nbuckets * 4, c->stream));
-  HIPCHK(c, hipMemsetAsync(c->errs, 0, 4 * sizeof(unsigned int), c->stream));
+  // cleared by kernels (no memset nodes when the stream is captured)
+  if (zero_u32(c->counts, g.nbuckets, c->stream) || zero_u32(c->errs, 4, c->stream))
+    return fail(c, CAPOW_ERR_HIP, std::string("clearing the sort counters failed: ") +
+                                      hipGetErrorString(hipGetLastError()));
   DepositArgs a;
   a.g = &g;
   a.stream = c->stream;
@@ -1307,6 +1313,7 @@ extern "C" int capow_backward(capow_ctx* c, const double* dL_draw, void* grad_xy
 struct capow_batch {
   std::vector<capow_params> cfg;
   int nconf = 0, nstreams = 0;
+  int nactive = 0;               // streams the runs use (<= nstreams; the autotuner picks)
   int64_t max_n = 0;
   std::vector<hipStream_t> streams;
   std::vector<capow_ctx*> ctx;   // [stream * nconf + config]
@@ -1315,6 +1322,16 @@ struct capow_batch {
   size_t stage_mocks = 0;
   std::vector<hipEvent_t> ev;
   std::string err;
+  // graph mode: the whole run (every stream's stages of every catalogue)
+  // captured into one hipGraph, replayed while the catalogue list (device
+  // pointers, counts, parameter sets), the streams in use and the staging
+  // buffer are unchanged
+  bool graph = false;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<capow_mock> gkey;
+  int gactive = 0;
+  double* gstage = nullptr;
+  std::vector<hipEvent_t> gev;   // fork / join of the captured streams
 };
 
 static int bfail(capow_batch* b, int code, const std::string& m) {
@@ -1326,11 +1343,23 @@ static int bfail(capow_batch* b, int code, const std::string& m) {
 extern "C" int capow_batch_destroy(capow_batch* b) {
   if (!b) return CAPOW_OK;
   for (hipStream_t s : b->streams) hipStreamSynchronize(s);
+  if (b->gexec) hipGraphExecDestroy(b->gexec);
+  for (hipEvent_t e : b->gev) hipEventDestroy(e);
   for (capow_ctx* c : b->ctx) capow_destroy(c);
   for (hipEvent_t e : b->ev) hipEventDestroy(e);
   for (hipStream_t s : b->streams) hipStreamDestroy(s);
   if (b->stage) hipHostFree(b->stage);
   delete b;
+  return CAPOW_OK;
+}
+
+static void batch_drop_graph(capow_batch* b);
+
+extern "C" int capow_batch_set_graph(capow_batch* b, int32_t on) {
+  if (!b) return bfail(b, CAPOW_ERR_ARG, "null batch");
+  for (hipStream_t s : b->streams) hipStreamSynchronize(s);
+  batch_drop_graph(b);
+  b->graph = on != 0;
   return CAPOW_OK;
 }
 
@@ -1352,6 +1381,7 @@ extern "C" int capow_batch_create(const capow_params* configs, int32_t nconf,
   b->cfg.assign(configs, configs + nconf);
   b->nconf = nconf;
   b->nstreams = nstreams;
+  b->nactive = nstreams;
   b->max_n = max_particles;
   size_t per_copy_max = 0;
   for (int s = 0; s < nstreams; s++) {
@@ -1389,6 +1419,86 @@ extern "C" int capow_batch_create(const capow_params* configs, int32_t nconf,
   return CAPOW_OK;
 }
 
+// Enqueues every catalogue's stages (particles, assignment, transform,
+// binning) round-robin over the active streams; timed: events around each
+// catalogue's stages (not while capturing).  Returns the first failure.
+static int batch_enqueue(capow_batch* b, const capow_mock* mocks, int n, bool timed,
+                         std::vector<int>& launched) {
+  int first = CAPOW_OK;
+  for (int i = 0; i < n; i++) {
+    const int s = i % b->nactive;
+    capow_ctx* c = b->ctx[(size_t)s * b->nconf + mocks[i].config];
+    double* st = b->stage + (size_t)i * b->stride;
+    unsigned int* er = reinterpret_cast<unsigned int*>(st + STAT_TOTAL);
+    double* bins = st + STAT_TOTAL + 2;
+    if (timed) hipEventRecord(b->ev[2 * i], b->streams[s]);
+    int rc = capow_set_particles(c, mocks[i].xyzv, mocks[i].w, mocks[i].n,
+                                 c->fp64 ? CAPOW_DTYPE_F64 : CAPOW_DTYPE_F32, CAPOW_DEVICE);
+    if (!rc) rc = assign_enqueue(c, st, er, false);
+    if (!rc) rc = capow_fft(c);
+    if (!rc) rc = bin_enqueue(c, bins);
+    if (timed) hipEventRecord(b->ev[2 * i + 1], b->streams[s]);
+    if (rc) {
+      if (!first) {
+        first = rc;
+        b->err = "mock " + std::to_string(i) + ": " + c->err;
+      }
+      continue;
+    }
+    launched[i] = 1;
+  }
+  return first;
+}
+
+static void batch_drop_graph(capow_batch* b) {
+  if (b->gexec) hipGraphExecDestroy(b->gexec);
+  b->gexec = nullptr;
+  b->gkey.clear();
+}
+
+// Captures the enqueue of `mocks` on every active stream into one graph
+// (stream 0 forks to the others through an event and joins them at the
+// end) and instantiates it; nothing runs.  The enqueue allocates nothing
+// and synchronises nothing once the list has run eagerly, and every clear
+// inside it is a kernel, so the graph holds kernels and copies only.
+static int batch_capture(capow_batch* b, const capow_mock* mocks, int n) {
+  batch_drop_graph(b);
+  while (b->gev.size() < (size_t)b->nstreams) {
+    hipEvent_t e;
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+      return bfail(b, CAPOW_ERR_HIP, "hipEventCreate failed");
+    b->gev.push_back(e);
+  }
+  hipStream_t s0 = b->streams[0];
+  if (hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal) != hipSuccess)
+    return bfail(b, CAPOW_ERR_HIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(hipGetLastError()));
+  hipError_t e = hipEventRecord(b->gev[0], s0);
+  for (int s = 1; s < b->nactive && e == hipSuccess; s++) e = hipStreamWaitEvent(b->streams[s], b->gev[0], 0);
+  std::vector<int> launched(n, 0);
+  int rc = e == hipSuccess ? batch_enqueue(b, mocks, n, false, launched) : CAPOW_ERR_HIP;
+  for (int s = 1; s < b->nactive && e == hipSuccess; s++) {
+    e = hipEventRecord(b->gev[s], b->streams[s]);
+    if (e == hipSuccess) e = hipStreamWaitEvent(s0, b->gev[s], 0);
+  }
+  hipGraph_t graph = nullptr;
+  const hipError_t e1 = hipStreamEndCapture(s0, &graph);
+  if (e == hipSuccess) e = e1;
+  if (e == hipSuccess && !rc && graph) e = hipGraphInstantiate(&b->gexec, graph, nullptr, nullptr, 0);
+  if (graph) hipGraphDestroy(graph);
+  if (rc) {
+    b->gexec = nullptr;
+    return rc;
+  }
+  if (e != hipSuccess || !b->gexec) {
+    b->gexec = nullptr;
+    return bfail(b, CAPOW_ERR_HIP, std::string("batch graph capture failed: ") + hipGetErrorString(e));
+  }
+  b->gkey.assign(mocks, mocks + n);
+  b->gactive = b->nactive;
+  b->gstage = b->stage;
+  return CAPOW_OK;
+}
+
 extern "C" int capow_batch_run(capow_batch* b, const capow_mock* mocks, int32_t n,
                                capow_bins* out, capow_assign_stats* stats, float* ms) {
   if (!b || (n > 0 && (!mocks || !out))) return bfail(b, CAPOW_ERR_ARG, "bad batch run arguments");
@@ -1412,35 +1522,24 @@ extern "C" int capow_batch_run(capow_batch* b, const capow_mock* mocks, int32_t 
     b->ev.push_back(e);
   }
   int first = CAPOW_OK;
+  bool fell_back = false;
   std::vector<int> launched(n, 0);
-  for (int i = 0; i < n; i++) {
-    const int s = i % b->nstreams;
-    capow_ctx* c = b->ctx[(size_t)s * b->nconf + mocks[i].config];
-    double* st = b->stage + (size_t)i * b->stride;
-    unsigned int* er = reinterpret_cast<unsigned int*>(st + STAT_TOTAL);
-    double* bins = st + STAT_TOTAL + 2;
-    hipEventRecord(b->ev[2 * i], b->streams[s]);
-    int rc = capow_set_particles(c, mocks[i].xyzv, mocks[i].w, mocks[i].n,
-                                 c->fp64 ? CAPOW_DTYPE_F64 : CAPOW_DTYPE_F32, CAPOW_DEVICE);
-    if (!rc) rc = assign_enqueue(c, st, er, false);
-    if (!rc) rc = capow_fft(c);
-    if (!rc) rc = bin_enqueue(c, bins);
-    hipEventRecord(b->ev[2 * i + 1], b->streams[s]);
-    if (rc) {
-      if (!first) {
-        first = rc;
-        b->err = "mock " + std::to_string(i) + ": " + c->err;
-      }
-      continue;
-    }
-    launched[i] = 1;
+  const bool replay = b->graph && b->gexec && b->gkey.size() == (size_t)n &&
+                      b->gactive == b->nactive && b->gstage == b->stage &&
+                      memcmp(b->gkey.data(), mocks, (size_t)n * sizeof(capow_mock)) == 0;
+  if (replay) {
+    if (hipGraphLaunch(b->gexec, b->streams[0]) != hipSuccess)
+      return bfail(b, CAPOW_ERR_HIP, std::string("hipGraphLaunch: ") + hipGetErrorString(hipGetLastError()));
+    for (int i = 0; i < n; i++) launched[i] = 1;
+  } else {
+    first = batch_enqueue(b, mocks, n, true, launched);
   }
   for (hipStream_t s : b->streams)
     if (hipStreamSynchronize(s) != hipSuccess)
       return bfail(b, CAPOW_ERR_HIP, std::string("batch stream: ") + hipGetErrorString(hipGetLastError()));
   for (int i = 0; i < n; i++) {
     if (!launched[i]) continue;
-    const int s = i % b->nstreams;
+    const int s = i % b->nactive;
     capow_ctx* c = b->ctx[(size_t)s * b->nconf + mocks[i].config];
     const double* st = b->stage + (size_t)i * b->stride;
     const unsigned int* er = reinterpret_cast<const unsigned int*>(st + STAT_TOTAL);
@@ -1449,6 +1548,9 @@ extern "C" int capow_batch_run(capow_batch* b, const capow_mock* mocks, int32_t 
     if (rc == CAPOW_ERR_NOMEM) {
       // the catalogue needed more sorted copies than reserved: run it alone
       // with an exactly sized buffer
+      // (its exactly sized buffer replaces the one a captured graph uses)
+      batch_drop_graph(b);
+      fell_back = true;
       int r = capow_set_particles(c, mocks[i].xyzv, mocks[i].w, mocks[i].n,
                                   c->fp64 ? CAPOW_DTYPE_F64 : CAPOW_DTYPE_F32, CAPOW_DEVICE);
       if (!r) r = capow_assign(c, stats ? &stats[i] : nullptr);
@@ -1459,8 +1561,8 @@ extern "C" int capow_batch_run(capow_batch* b, const capow_mock* mocks, int32_t 
       bin_finish(c, st + STAT_TOTAL + 2, &out[i]);
     }
     if (ms) {
-      float t = 0.f;
-      hipEventElapsedTime(&t, b->ev[2 * i], b->ev[2 * i + 1]);
+      float t = -1.f;   // a replayed graph does not time the catalogues one by one
+      if (!replay) hipEventElapsedTime(&t, b->ev[2 * i], b->ev[2 * i + 1]);
       ms[i] = t;
     }
     if (rc && !first) {
@@ -1468,5 +1570,185 @@ extern "C" int capow_batch_run(capow_batch* b, const capow_mock* mocks, int32_t 
       b->err = "mock " + std::to_string(i) + ": " + c->err;
     }
   }
+  // graph mode: after an eager run of a new list (which left every buffer
+  // sized and the mode counts computed), capture the same list for replay
+  if (b->graph && !replay && !first && !fell_back) return batch_capture(b, mocks, n);
   return first;
+}
+
+// ---- autotuner ------------------------------------------------------------
+// The reference plans its FFTs with FFTW_ESTIMATE (src/main.c:1200-1217) and
+// has no other tunable stage.  Here each tunable of a context is timed on
+// the particles it holds, one tunable at a time (coordinate descent), and
+// the fastest candidate kept:
+//   sort_blocks  workgroups of the two sort passes per CU (4, 8, 12)
+//   sort_lds     LDS run aggregation in the sort (on) or the global
+//                counters for every run (off)
+//   dep_halo     halo rows in the FP32 deposit tiles
+//   fft_runs     write-run width of the out-of-place FFT passes (auto:
+//                128 bytes, or 64)
+//   fft_order    batch order of the z/y/x passes (bit per pass: outer
+//                index fastest)
+//   xbin         the x pass fused with the binning (FP32)
+// Every choice but xbin gives bitwise the same grids and bins (integer
+// deposit sums, the same per-row transforms); xbin's sums differ from the
+// separate binning's at the FP32 rounding level.
+
+struct TuneKnob {
+  const char* name;
+  int* value;
+  std::vector<int> cand;
+};
+
+static int tune_time(capow_ctx* c, int reps, float* med) {
+  hipEvent_t a, b;
+  if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return CAPOW_ERR_HIP;
+  std::vector<float> t;
+  capow_bins out;
+  memset(&out, 0, sizeof out);   // null arrays: the sums stay on the device
+  int rc = CAPOW_OK;
+  for (int r = 0; r <= reps && !rc; r++) {   // r = 0: warm-up
+    hipEventRecord(a, c->stream);
+    rc = capow_assign(c, nullptr);
+    if (!rc) rc = capow_fft(c);
+    if (!rc) rc = capow_bin(c, &out);
+    hipEventRecord(b, c->stream);
+    hipEventSynchronize(b);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, a, b);
+    if (r > 0) t.push_back(ms);
+  }
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  if (rc) return rc;
+  std::sort(t.begin(), t.end());
+  *med = t[t.size() / 2];
+  return CAPOW_OK;
+}
+
+static void tune_note(std::string& rep, const char* fmt, ...) {
+  char buf[256];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  rep += buf;
+}
+
+extern "C" int capow_autotune(capow_ctx* c, int32_t reps, char* report, int32_t report_len) {
+  if (!c) return CAPOW_ERR_ARG;
+  if (c->p.edge_mode == CAPOW_EDGE_SURVEY)
+    return fail(c, CAPOW_ERR_UNSUPPORTED, "survey context: nothing to tune");
+  if (c->state < 1) return fail(c, CAPOW_ERR_STATE, "capow_autotune before capow_set_particles");
+  if (reps < 1) reps = 3;
+  const bool xbin_avail = c->xbin_ok || c->xbin_part != nullptr;
+  int sort_blocks = c->count_blocks / (c->ncu > 0 ? c->ncu : 256);
+  int sort_lds = c->g.sort_probes > 0;
+  int halo = c->dep_halo ? 1 : 0, runs = c->fft_rows_mul, order = c->fft_order;
+  int xbin = c->xbin_ok ? 1 : 0;
+  std::vector<TuneKnob> knobs = {
+      {"sort_blocks", &sort_blocks, {12, 8, 4}},
+      {"sort_lds", &sort_lds, {1, 0}},
+      {"dep_halo", &halo, {1, 0}},
+      {"fft_runs", &runs, {0, 1}},
+      {"fft_order", &order, {c->fp64 ? 0 : 5, 0, 7, 2}},
+  };
+  if (xbin_avail) knobs.push_back({"xbin", &xbin, {1, 0}});
+  auto apply = [&]() {
+    c->count_blocks = sort_blocks * (c->ncu > 0 ? c->ncu : 256);
+    c->g.sort_probes = sort_lds ? 32 : 0;
+    c->dep_halo = halo != 0;
+    c->fft_rows_mul = runs;
+    c->fft_order = order;
+    c->xbin_ok = xbin_avail && xbin != 0;
+  };
+  std::string rep;
+  tune_note(rep, "autotune %lldx%lldx%lld %s, %lld particles, median of %d\n", (long long)c->p.nx,
+            (long long)c->p.ny, (long long)c->p.nz, c->fp64 ? "FP64" : "FP32", (long long)c->n, reps);
+  for (auto& k : knobs) {
+    int best = *k.value;
+    float best_ms = 1e30f;
+    for (int v : k.cand) {
+      *k.value = v;
+      apply();
+      float ms = 0.f;
+      const int rc = tune_time(c, reps, &ms);
+      if (rc) return rc;
+      tune_note(rep, "  %s=%d: %.3f ms\n", k.name, v, ms);
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = v;
+      }
+    }
+    *k.value = best;
+    apply();
+  }
+  tune_note(rep, "chosen: sort_blocks=%d sort_lds=%d dep_halo=%d fft_runs=%d fft_order=%d xbin=%d\n",
+            sort_blocks, sort_lds, halo, runs, order, c->xbin_ok ? 1 : 0);
+  if (report && report_len > 0) {
+    strncpy(report, rep.c_str(), (size_t)report_len - 1);
+    report[report_len - 1] = 0;
+  }
+  return CAPOW_OK;
+}
+
+extern "C" int capow_batch_autotune(capow_batch* b, const capow_mock* mocks, int32_t n,
+                                    int32_t reps, char* report, int32_t report_len) {
+  if (!b || !mocks || n < 1) return bfail(b, CAPOW_ERR_ARG, "bad batch autotune arguments");
+  if (reps < 1) reps = 3;
+  for (hipStream_t st : b->streams) hipStreamSynchronize(st);
+  batch_drop_graph(b);   // the launches it holds follow the old choices
+  std::string rep;
+  std::vector<char> buf(4096);
+  for (int q = 0; q < b->nconf; q++) {
+    int m = -1;
+    for (int i = 0; i < n && m < 0; i++)
+      if (mocks[i].config == q) m = i;
+    if (m < 0) continue;
+    capow_ctx* c0 = b->ctx[q];
+    int rc = capow_set_particles(c0, mocks[m].xyzv, mocks[m].w, mocks[m].n,
+                                 c0->fp64 ? CAPOW_DTYPE_F64 : CAPOW_DTYPE_F32, CAPOW_DEVICE);
+    if (!rc) rc = capow_autotune(c0, reps, buf.data(), (int32_t)buf.size());
+    if (rc) return bfail(b, rc, "autotune config " + std::to_string(q) + ": " + c0->err);
+    rep += buf.data();
+    for (int s = 1; s < b->nstreams; s++) {
+      capow_ctx* c = b->ctx[(size_t)s * b->nconf + q];
+      c->count_blocks = c0->count_blocks;
+      c->g.sort_probes = c0->g.sort_probes;
+      c->dep_halo = c0->dep_halo;
+      c->fft_rows_mul = c0->fft_rows_mul;
+      c->fft_order = c0->fft_order;
+      c->xbin_ok = c0->xbin_ok;
+    }
+  }
+  // streams in use: the whole batch timed with each candidate count
+  std::vector<capow_bins> out(n);
+  for (auto& o : out) memset(&o, 0, sizeof o);
+  int best = b->nactive;
+  double best_s = 1e30;
+  for (int ns = 2; ns <= b->nstreams; ns += 2) {
+    b->nactive = ns;
+    std::vector<double> t;
+    for (int r = 0; r <= reps; r++) {
+      for (hipStream_t st : b->streams) hipStreamSynchronize(st);
+      const auto t0 = std::chrono::steady_clock::now();
+      const int rc = capow_batch_run(b, mocks, n, out.data(), nullptr, nullptr);
+      const auto t1 = std::chrono::steady_clock::now();
+      if (rc) return rc;
+      if (r > 0) t.push_back(std::chrono::duration<double>(t1 - t0).count());
+    }
+    std::sort(t.begin(), t.end());
+    tune_note(rep, "  streams=%d: %.3f ms for %d catalogues\n", ns, t[t.size() / 2] * 1e3, n);
+    if (t[t.size() / 2] < best_s) {
+      best_s = t[t.size() / 2];
+      best = ns;
+    }
+  }
+  b->nactive = best;
+  tune_note(rep, "chosen: streams=%d\n", best);
+  if (report && report_len > 0) {
+    strncpy(report, rep.c_str(), (size_t)report_len - 1);
+    report[report_len - 1] = 0;
+  }
+  return CAPOW_OK;
 }

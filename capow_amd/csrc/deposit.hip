@@ -971,6 +971,22 @@ __global__ __launch_bounds__(DEP_T) void k_deposit(const T* __restrict__ sorted,
   }
 }
 
+// Zero fill by a kernel: a stream a caller captures into a graph holds no
+// memset nodes.
+__global__ void k_zero_u32(uint32_t* __restrict__ p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0u;
+}
+
+int zero_u32(uint32_t* p, int64_t n, hipStream_t st) {
+  if (n <= 0) return CAPOW_OK;
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  k_zero_u32<<<(unsigned)blocks, 256, 0, st>>>(p, n);
+  return hipGetLastError() == hipSuccess ? CAPOW_OK : CAPOW_ERR_HIP;
+}
+
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
@@ -994,7 +1010,7 @@ static int run_count_t(DepositArgs& a) {
   TIMED(a.timer, "scan", (k_scan_carry<<<1, SCAN_T, 0, st>>>(a.bsum, nblk, a.boff)));
   TIMED(a.timer, "scan", (k_scan_apply<<<(int)nblk, SCAN_T, 0, st>>>(a.counts, nbk, a.boff, a.offsets,
                                                                  nullptr)));
-  if (hipMemsetAsync(a.cursor, 0, (size_t)nbk * sizeof(uint32_t), st) != hipSuccess)
+  if (zero_u32(a.cursor, nbk, st) != CAPOW_OK)
     return CAPOW_ERR_HIP;
   return hipGetLastError() == hipSuccess ? CAPOW_OK : CAPOW_ERR_HIP;
 }

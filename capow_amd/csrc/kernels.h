@@ -146,6 +146,8 @@ struct FftArgs {
 };
 #define XBIN_SSQ_MAX 8192
 int run_fft(FftArgs& a, bool fp64);
+// zero fill by a kernel (deposit.hip)
+int zero_u32(uint32_t* p, int64_t n, hipStream_t st);
 // The pending x pass after run_fft with x_deferred (grid -> scratch; the
 // caller swaps the buffers as after a whole run_fft).
 int run_fft_x(FftArgs& a);

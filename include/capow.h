@@ -280,8 +280,39 @@ int capow_batch_create(const capow_params *configs, int32_t nconf,
  * Returns the first failing mock's status (all mocks are attempted). */
 int capow_batch_run(capow_batch *b, const capow_mock *mocks, int32_t n,
                     capow_bins *out, capow_assign_stats *stats, float *ms);
+/* Graph mode (no counterpart in the reference; for callers that run the
+ * same catalogue list many times, e.g. a covariance loop over mocks whose
+ * contents are rewritten in place): on = 1 makes capow_batch_run capture a
+ * list's launches on every stream into one HIP graph after running it once
+ * eagerly, and replay that graph whenever the next list is the same (the
+ * same device pointers, counts and parameter sets, in the same order, and
+ * the same streams in use); a different list runs eagerly and is captured
+ * in turn.  Replayed runs read the catalogues' current contents; ms[i] is
+ * then -1.  A catalogue re-run alone (sort buffer too small) or an
+ * autotune drops the graph.  on = 0 (the default) drops it.  Synchronises. */
+int capow_batch_set_graph(capow_batch *b, int32_t on);
 int capow_batch_destroy(capow_batch *b);
 const char *capow_batch_last_error(const capow_batch *b);
+
+/* ---- autotuner ------------------------------------------------------------ */
+/* No counterpart in the reference, whose only planning is FFTW_ESTIMATE
+ * (src/main.c:1200-1217).  capow_autotune times the pipeline (capow_assign,
+ * capow_fft, capow_bin) on the particles last set with capow_set_particles,
+ * trying the candidates of one tunable at a time and keeping the fastest
+ * (median of `reps` runs, <= 0: 3): the sort's workgroups per CU and its LDS
+ * run aggregation, the deposit's halo rows, the FFT passes' write-run width
+ * and batch order, and the x pass fused with the binning.  The choices stay
+ * with the context; `report` (may be NULL) receives every timing and the
+ * choice as text.  The results do not depend on the choices, except that
+ * the fused binning's sums differ from the separate binning's at the FP32
+ * rounding level.  Synchronises. */
+int capow_autotune(capow_ctx *ctx, int32_t reps, char *report, int32_t report_len);
+/* The same for a batch: each parameter set is tuned on its first catalogue
+ * in `mocks` and the choice copied to every stream's context of that set;
+ * then the number of streams in use (2, 4, ... up to the batch's) is timed
+ * on the whole list. */
+int capow_batch_autotune(capow_batch *b, const capow_mock *mocks, int32_t n, int32_t reps,
+                         char *report, int32_t report_len);
 
 /* ---- catalogue input (no GPU) ------------------------------------------ */
 

@@ -1,0 +1,74 @@
+"""The autotuner (capow_autotune, capow_batch_autotune): it reports every
+candidate it timed and its choice, and the results after tuning equal the
+untuned ones -- bitwise when the fused x-pass binning keeps its setting,
+within the FP32 rounding level (2e-6 of the bin's scale) when it changes.
+The reference has nothing to tune (FFTW_ESTIMATE plans,
+src/main.c:1200-1217)."""
+import sys
+
+import numpy as np
+import pytest
+
+from capow_amd import api, mocks
+
+pytestmark = pytest.mark.gpu
+
+L = 1000.0
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def test_autotune_reports_and_keeps_results():
+    say("autotune 96^3")
+    n = 96
+    p = api.make_params(nx=n, ny=n, nz=n, xmax=L, ymax=L, zmax=L, interp_order=3, do_interlacing=1,
+                        momentum=1, extra_spectra=3, nk=20, maxk=0.2, precision=api.FP32)
+    pos, vel = mocks.zeldovich(48, L, 3)
+    xyzv = mocks.xyzv(pos, vel, 0).astype(np.float32)
+    ps = api.PowerSpectrum(p)
+    ps.set_particles(xyzv)
+    before = ps.power()
+    rep = ps.autotune(reps=2)
+    say(rep)
+    for k in ("sort_blocks=", "sort_lds=", "dep_halo=", "fft_runs=", "fft_order=", "chosen:"):
+        assert k in rep, k
+    xbin_now = "xbin=1" in rep.split("chosen:")[1]
+    after = ps.power()
+    ps.close()
+    assert np.array_equal(before["nmodes"], after["nmodes"])
+    for code in before["codes"]:
+        a, b = before[code]["pk"], after[code]["pk"]
+        if xbin_now:
+            assert np.array_equal(a.view(np.uint64), b.view(np.uint64)), code
+        else:
+            scale = np.abs(a[0]).max()
+            assert np.abs(a - b).max() <= 2e-6 * scale, code
+
+
+def test_batch_autotune_keeps_results():
+    import torch
+    say("batch autotune")
+    Lb = 500.0
+    cfgs = [dict(nx=32, ny=32, nz=32, xmax=Lb, ymax=Lb, zmax=Lb, interp_order=2, do_interlacing=1,
+                 momentum=0, nk=20, maxk=0.2, precision=api.FP32),
+            dict(nx=48, ny=48, nz=48, xmax=Lb, ymax=Lb, zmax=Lb, interp_order=2, do_interlacing=1,
+                 momentum=0, nk=30, maxk=0.3, precision=api.FP32)]
+    counts = [3000, 5000, 4000, 6000, 2500, 4500, 3500, 5500]
+    cats = [mocks.mixed_mock_torch(c, Lb, seed=300 + i) for i, c in enumerate(counts)]
+    which = [i % 2 for i in range(len(cats))]
+    b = api.Batch(cfgs, max_particles=6000, nstreams=4)
+    r0, _, _ = b.run(cats, which)
+    rep = b.autotune(cats, which, reps=2)
+    say(rep)
+    assert "streams=2" in rep and "chosen: streams=" in rep
+    r1, _, _ = b.run(cats, which)
+    b.close()
+    torch.cuda.synchronize()
+    for a, c in zip(r0, r1):
+        assert np.array_equal(a["nmodes"], c["nmodes"])
+        for code in a["codes"]:
+            scale = np.abs(a[code]["pk"][0]).max()
+            assert np.abs(a[code]["pk"] - c[code]["pk"]).max() <= 2e-6 * scale

@@ -95,6 +95,7 @@ __device__ __forceinline__ int fl2i(T v) { return (int)v; }  // trunc toward 0
 // of a cell edge is part of the FP32 form's error).
 template <typename T>
 __device__ __forceinline__ T scaled(T r, T d, T invd) {
+#pragma clang fp contract(off)
   if constexpr (sizeof(T) == 4) return r * invd;
   else return r / d;
 }
@@ -112,6 +113,9 @@ struct AxisCell {
 template <typename T, int ORDER>
 __device__ __forceinline__ AxisCell<T> axis_cell(T u, T lo, T hi, T d, T L, int n, bool is_x,
                                                  int los, int edge, T invd) {
+  // No multiply-add contraction: the sort, the deposit and the gradient
+  // gather inline this into different code, and must round alike.
+#pragma clang fp contract(off)
   AxisCell<T> o;
   if (edge == CAPOW_EDGE_PERIODIC) {
     // wrap into [lo, hi)
@@ -151,6 +155,7 @@ __device__ __forceinline__ AxisCell<T> axis_cell(T u, T lo, T hi, T d, T L, int 
 template <typename T, int ORDER>
 __device__ __forceinline__ AxisOut<T> axis_taps(T u, T lo, T hi, T d, T L, int n,
                                                 bool is_x, int los, int edge, T invd) {
+#pragma clang fp contract(off)
   const AxisCell<T> ac = axis_cell<T, ORDER>(u, lo, hi, d, L, n, is_x, los, edge, invd);
   AxisOut<T> o;
   o.ok = ac.ok;
@@ -193,6 +198,31 @@ __device__ __forceinline__ AxisOut<T> axis_taps(T u, T lo, T hi, T d, T L, int n
     o.w[0] = (periodic || c < n) ? T(1) : T(0);
   }
   return o;
+}
+
+// Edge rules fixed at compile time for the sort and deposit kernels, so
+// that the per-particle rules above compile to straight-line code:
+// EK_RUNTIME reads Geom::edge, ::los and ::periodic; EK_PERIODIC is
+// CAPOW_EDGE_PERIODIC; EK_REF_LOS0 the reference rules with LOS = 0 (taps
+// wrap, the TSC x shift); EK_REF_LOS the reference rules with LOS != 0
+// (taps outside [0, N) dropped; the survey deposit too).
+enum { EK_RUNTIME = 0, EK_PERIODIC = 1, EK_REF_LOS0 = 2, EK_REF_LOS = 3 };
+template <int EK>
+__device__ __forceinline__ int ek_edge(const Geom& g) {
+  return EK == EK_RUNTIME ? g.edge : (EK == EK_PERIODIC ? CAPOW_EDGE_PERIODIC : CAPOW_EDGE_REFERENCE);
+}
+// only "LOS == 0 or not" matters to the reference rules (the periodic rules
+// ignore it; the out-of-box check reads it)
+template <int EK>
+__device__ __forceinline__ int ek_los(const Geom& g) {
+  return EK == EK_REF_LOS0 ? 0 : (EK == EK_REF_LOS ? 3 : g.los);
+}
+template <int EK>
+__device__ __forceinline__ bool ek_periodic(const Geom& g) {
+  return EK == EK_RUNTIME ? g.periodic != 0 : EK != EK_REF_LOS;
+}
+inline int edge_kind(const Geom& g) {
+  return g.edge == CAPOW_EDGE_PERIODIC ? EK_PERIODIC : (g.los == 0 ? EK_REF_LOS0 : EK_REF_LOS);
 }
 
 // Bucket key cell along an axis: the cell of axis_cell, wrapped mod N on

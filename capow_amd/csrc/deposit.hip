@@ -116,9 +116,9 @@ __device__ __forceinline__ int canon_off(int off, int n) {
 // [key + reach_lo, key + reach_hi], otherwise a tile that owns a tap would
 // not hold a copy of the particle.
 template <int ORDER>
-__device__ __forceinline__ bool reach_ok(int first, int key, int n, const Geom& g) {
+__device__ __forceinline__ bool reach_ok(int first, int key, int n, const Geom& g, bool per) {
   int a = first, b = first + ORDER - 1;
-  if (!g.periodic) {
+  if (!per) {
     a = a < 0 ? 0 : a;
     b = b > n - 1 ? n - 1 : b;
     if (a > b) return true;   // every tap dropped
@@ -127,8 +127,9 @@ __device__ __forceinline__ bool reach_ok(int first, int key, int n, const Geom& 
   return canon_off(a - key, n) >= g.reach_lo && canon_off(b - key, n) <= g.reach_hi;
 }
 
-template <typename T, int ORDER>
+template <typename T, int ORDER, int EK>
 __device__ __forceinline__ PClass<T, ORDER> classify(T x, T y, T z, T v, const Geom& g) {
+  const int edge = ek_edge<EK>(g), los = ek_los<EK>(g);
   PClass<T, ORDER> c;
   if (g.rsd_axis >= 0) {
     // tx += tvr*(1+z)/(100 sqrt(Om (1+z)^3 + 1 - Om))   src/read_data.c:67-76
@@ -144,13 +145,13 @@ __device__ __forceinline__ PClass<T, ORDER> classify(T x, T y, T z, T v, const G
 #pragma unroll
   for (int a = 0; a < 3; a++) {
     const T lo = (T)g.lo[a], hi = (T)g.hi[a], d = (T)g.d[a], L = (T)g.L[a], invd = (T)g.invd[a];
-    const AxisCell<T> m = axis_cell<T, ORDER>(u[a], lo, hi, d, L, g.n[a], a == 0, g.los, g.edge, invd);
+    const AxisCell<T> m = axis_cell<T, ORDER>(u[a], lo, hi, d, L, g.n[a], a == 0, los, edge, invd);
     okm = okm && m.ok;
     cm[a] = m.c;
     ci[a] = m.c;
     if (g.interlace) {
-      const AxisCell<T> i = axis_cell<T, ORDER>(u[a] + d / T(2), lo, hi, d, L, g.n[a], a == 0, g.los,
-                                                g.edge, invd);
+      const AxisCell<T> i = axis_cell<T, ORDER>(u[a] + d / T(2), lo, hi, d, L, g.n[a], a == 0, los,
+                                                edge, invd);
       oki = oki && i.ok;
       ci[a] = i.c;
     }
@@ -163,7 +164,7 @@ __device__ __forceinline__ PClass<T, ORDER> classify(T x, T y, T z, T v, const G
   // (the key lies within reach_hi of its upper edge) or below (within
   // -reach_lo of its lower edge)
   const int lw = -g.reach_lo, hw = g.reach_hi;
-  const bool per = g.periodic != 0;
+  const bool per = ek_periodic<EK>(g);
   int tc[3], cl[3];
 #pragma unroll
   for (int a = 0; a < 3; a++) {
@@ -175,8 +176,8 @@ __device__ __forceinline__ PClass<T, ORDER> classify(T x, T y, T z, T v, const G
     if (nt > 1) {
       const int first_m = ORDER == 3 ? cm[a] - 1 : cm[a];
       const int first_i = ORDER == 3 ? ci[a] - 1 : ci[a];
-      const bool r = (!okm || reach_ok<ORDER>(first_m, key, n, g)) &&
-                     (!oki || reach_ok<ORDER>(first_i, key, n, g));
+      const bool r = (!okm || reach_ok<ORDER>(first_m, key, n, g, per)) &&
+                     (!oki || reach_ok<ORDER>(first_i, key, n, g, per));
       c.reach_ok = c.reach_ok && r;
       cl[a] = local >= td - hw ? 2 : (local < lw ? 0 : 1);
     }
@@ -288,7 +289,7 @@ __device__ __forceinline__ int hs_add(uint32_t* __restrict__ hkey, uint32_t* __r
   return slot;
 }
 
-template <typename T, int ORDER, bool SCATTER>
+template <typename T, int ORDER, bool SCATTER, int EK>
 __global__ __launch_bounds__(BK_T) void k_bucket(const T* __restrict__ parts,
                                                  const T* __restrict__ wts, int64_t n,
                                                  Geom g, uint32_t* __restrict__ counts,
@@ -323,7 +324,7 @@ __global__ __launch_bounds__(BK_T) void k_bucket(const T* __restrict__ parts,
         load4<T>(parts, i, x, y, z, v);
         if (g.has_w) w = wts[i];
       }
-      PClass<T, ORDER> c = classify<T, ORDER>(x, y, z, v, g);
+      PClass<T, ORDER> c = classify<T, ORDER, EK>(x, y, z, v, g);
       const bool take = valid && c.take;
       px[k] = c.x; py[k] = c.y; pz[k] = c.z; pv[k] = v; pw[k] = w;
       pr[k] = BK_EMPTY;
@@ -348,7 +349,7 @@ __global__ __launch_bounds__(BK_T) void k_bucket(const T* __restrict__ parts,
         // src/read_data.c:81-88 (on the shifted position)
         if ((c.x < (T)g.lo[0]) || (c.x > (T)g.hi[0]) || (c.y < (T)g.lo[1]) ||
             (c.y > (T)g.hi[1]) || (c.z < (T)g.lo[2]) || (c.z > (T)g.hi[2])) {
-          if (g.los == 0) oob++;
+          if (ek_los<EK>(g) == 0) oob++;
           else s_out += 1.0;
         }
         if (c.acc_main) s_ngrid += (double)w;
@@ -609,7 +610,7 @@ __device__ __forceinline__ int zmask(int rowi, int TZ) {
 // borrow.
 #define DEP_ZLO 4   // halo cells below the tile's z range (taps reach -3)
 #define DEP_ZHI 2   // above (taps reach TZ + 2: the next row's first cells)
-template <typename T, int ORDER, bool HASB, int TYC, int TZC, bool HALO, bool APOS>
+template <typename T, int ORDER, bool HASB, int TYC, int TZC, bool HALO, bool APOS, int EK>
 __global__ __launch_bounds__(DEP_T) void k_deposit(const T* __restrict__ sorted,
                                                    const T* __restrict__ sorted_w,
                                                    const uint32_t* __restrict__ off,
@@ -664,7 +665,7 @@ __global__ __launch_bounds__(DEP_T) void k_deposit(const T* __restrict__ sorted,
       for (int a = 0; a < 3; a++) {
         int q = tt[a] + d.d[a];
         if (q < 0 || q >= g.ntile[a]) {
-          if (g.periodic) q += q < 0 ? g.ntile[a] : -g.ntile[a];
+          if (ek_periodic<EK>(g)) q += q < 0 ? g.ntile[a] : -g.ntile[a];
           else ok = false;
         }
         nb[a] = q;
@@ -801,47 +802,53 @@ __global__ __launch_bounds__(DEP_T) void k_deposit(const T* __restrict__ sorted,
           y += d1 / T(2);
           z += d2 / T(2);
         }
-        const AxisOut<T> ax = axis_taps<T, ORDER>(x, (T)g.lo[0], (T)g.hi[0], d0, (T)g.L[0], g.n[0], true, g.los, g.edge, (T)g.invd[0]);
-        const AxisOut<T> ay = axis_taps<T, ORDER>(y, (T)g.lo[1], (T)g.hi[1], d1, (T)g.L[1], g.n[1], false, g.los, g.edge, (T)g.invd[1]);
-        const AxisOut<T> az = axis_taps<T, ORDER>(z, (T)g.lo[2], (T)g.hi[2], d2, (T)g.L[2], g.n[2], false, g.los, g.edge, (T)g.invd[2]);
+        const int edge = ek_edge<EK>(g), los = ek_los<EK>(g);
+        const bool per = ek_periodic<EK>(g);
+        const AxisOut<T> ax = axis_taps<T, ORDER>(x, (T)g.lo[0], (T)g.hi[0], d0, (T)g.L[0], g.n[0], true, los, edge, (T)g.invd[0]);
+        const AxisOut<T> ay = axis_taps<T, ORDER>(y, (T)g.lo[1], (T)g.hi[1], d1, (T)g.L[1], g.n[1], false, los, edge, (T)g.invd[1]);
+        const AxisOut<T> az = axis_taps<T, ORDER>(z, (T)g.lo[2], (T)g.hi[2], d2, (T)g.L[2], g.n[2], false, los, edge, (T)g.invd[2]);
         const bool ok = ax.ok && ay.ok && az.ok;
         const T va = ok ? pa[k] : T(0), vb = ok ? pb[k] : T(0);
         if constexpr (HALO) {
           // x, y: tile-local tap rows; rows outside the tile add zeros to row
-          // `lane` (a row of the tile: TX * TY >= 64)
-          int lx[ORDER], ly[ORDER];
-          bool vx[ORDER], vy[ORDER];
-#pragma unroll
-          for (int t = 0; t < ORDER; t++) {
-            int gx = ax.first + t, gy = ay.first + t;
-            if (g.periodic) {
-              gx += gx < 0 ? g.n[0] : (gx >= g.n[0] ? -g.n[0] : 0);
-              gy += gy < 0 ? g.n[1] : (gy >= g.n[1] ? -g.n[1] : 0);
-            }
-            lx[t] = gx - t0x;
-            ly[t] = gy - t0y;
-            vx[t] = (unsigned)lx[t] < (unsigned)TX;
-            vy[t] = (unsigned)ly[t] < (unsigned)TY;
-          }
+          // `lane` (a row of the tile: TX * TY >= 64).  With two or more
+          // tiles along x and y (checked on the host), the first tap's
+          // offset from the tile taken as the nearest periodic image places
+          // every tap: tap t is in the tile iff that offset + t is in
+          // [0, T) (an image one period away lies at least N - T - 2 >= T - 2
+          // cells off, outside the tile for every tap).
+          // Only a tile on the box's border reads copies across the periodic
+          // boundary (a uniform branch).
+          int lx0 = ax.first - t0x, ly0 = ay.first - t0y;
+          if (per && (ttx == 0 || ttx == g.ntile[0] - 1)) lx0 = canon_off(lx0, g.n[0]);
+          if (per && (tty == 0 || tty == g.ntile[1] - 1)) ly0 = canon_off(ly0, g.n[1]);
           // z: the first tap relative to the tile (periodic: the nearest
           // image); a particle of this tile's lists has it in [-3, TZ], the
           // clamp only keeps any other in the halo
           int lz0 = az.first - t0z;
-          if (g.periodic) lz0 = canon_off(lz0, g.n[2]);
+          if (per && (ttz == 0 || ttz == g.ntile[2] - 1)) lz0 = canon_off(lz0, g.n[2]);
           lz0 = lz0 < -3 ? -3 : (lz0 > TZ ? TZ : lz0);
-          const int zb8 = (lz0 + DEP_ZLO) * 8;
+          // LDS byte offsets stay below 2^17: 32-bit unsigned arithmetic
+          const uint32_t zb8 = (uint32_t)(lz0 + DEP_ZLO) * 8u;
+          const uint32_t lane_row = __umul24((uint32_t)lane, (uint32_t)(RZ * 8)) + zb8;
+          // row (lx0 + tx, ly0 + ty) at base + a compile-time step
+          int base_b = (lx0 * TY + ly0) * (RZ * 8) + (int)zb8;
+          // kept in a register (not recomputed per row with a 32-bit
+          // multiply): an empty asm the compiler cannot see through
+          asm volatile("" : "+v"(base_b));
 #pragma unroll
           for (int tx = 0; tx < ORDER; tx++) {
+            const bool vx = (unsigned)(lx0 + tx) < (unsigned)TX;
 #pragma unroll
             for (int ty = 0; ty < ORDER; ty++) {
-              const bool vxy = vx[tx] && vy[ty];
+              const bool vxy = vx && (unsigned)(ly0 + ty) < (unsigned)TY;
               // a row outside the tile for every lane (ghost particles of
               // one border class share the pattern): nothing to add
-              if (!__any(vxy)) continue;
-              const int rowi = vxy ? lx[tx] * TY + ly[ty] : lane;
+              if (__builtin_amdgcn_ballot_w64(vxy) == 0) continue;
+              const uint32_t rowb = (uint32_t)(base_b + (tx * TY + ty) * (RZ * 8));
               const T wxy = vxy ? ax.w[tx] * ay.w[ty] : T(0);
               const T qa0 = va * wxy, qb0 = vb * wxy;
-              unsigned char* rb = smem + rowi * (RZ * 8) + zb8;
+              unsigned char* rb = smem + (vxy ? rowb : lane_row);
 #pragma unroll
               for (int tz = 0; tz < ORDER; tz++) {
                 const int qa = round_rpi((float)(qa0 * az.w[tz]));
@@ -867,7 +874,7 @@ __global__ __launch_bounds__(DEP_T) void k_deposit(const T* __restrict__ sorted,
 #pragma unroll
         for (int t = 0; t < ORDER; t++) {
           int gx = ax.first + t, gy = ay.first + t, gz = az.first + t;
-          if (g.periodic) {
+          if (per) {
             gx += gx < 0 ? g.n[0] : (gx >= g.n[0] ? -g.n[0] : 0);
             gy += gy < 0 ? g.n[1] : (gy >= g.n[1] ? -g.n[1] : 0);
             gz += gz < 0 ? g.n[2] : (gz >= g.n[2] ? -g.n[2] : 0);
@@ -990,15 +997,24 @@ int zero_u32(uint32_t* p, int64_t n, hipStream_t st) {
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
+// Calls f with the edge kind as a compile-time constant.
+template <typename F>
+static void launch_edge(int ek, F&& f) {
+  if (ek == EK_PERIODIC) f(std::integral_constant<int, EK_PERIODIC>());
+  else if (ek == EK_REF_LOS0) f(std::integral_constant<int, EK_REF_LOS0>());
+  else f(std::integral_constant<int, EK_REF_LOS>());
+}
 template <typename T, int ORDER>
 static int run_count_t(DepositArgs& a) {
   const Geom& g = *a.g;
   const int nb = a.count_blocks;
   hipStream_t st = a.stream;
   TIMED(a.timer, "bucket_count",
-        (k_bucket<T, ORDER, false><<<nb, BK_T, 0, st>>>((const T*)a.parts, (const T*)a.wts, a.n, g,
-                                                        a.counts, nullptr, nullptr, nullptr, nullptr, 0,
-                                                        a.partials, a.errs)));
+        (launch_edge(edge_kind(g), [&](auto ek) {
+          k_bucket<T, ORDER, false, decltype(ek)::value><<<nb, BK_T, 0, st>>>(
+              (const T*)a.parts, (const T*)a.wts, a.n, g, a.counts, nullptr, nullptr, nullptr, nullptr,
+              0, a.partials, a.errs);
+        })));
   TIMED(a.timer, "stats_final",
         (k_stats_final<<<1, 256, 0, st>>>(a.partials, nb,
                                          (double)g.n[0] * (double)g.n[1] * (double)g.n[2],
@@ -1021,10 +1037,11 @@ static int run_scatter_t(DepositArgs& a) {
   const int nb = a.count_blocks;
   hipStream_t st = a.stream;
   TIMED(a.timer, "bucket_scatter",
-        (k_bucket<T, ORDER, true><<<nb, BK_T, 0, st>>>((const T*)a.parts, (const T*)a.wts, a.n, g,
-                                                       nullptr, a.cursor, a.offsets, (T*)a.sorted,
-                                                       (T*)a.sorted_w, a.sorted_cap, nullptr,
-                                                       nullptr)));
+        (launch_edge(edge_kind(g), [&](auto ek) {
+          k_bucket<T, ORDER, true, decltype(ek)::value><<<nb, BK_T, 0, st>>>(
+              (const T*)a.parts, (const T*)a.wts, a.n, g, nullptr, a.cursor, a.offsets, (T*)a.sorted,
+              (T*)a.sorted_w, a.sorted_cap, nullptr, nullptr);
+        })));
   const int ncell = g.tdim[0] * g.tdim[1] * g.tdim[2];
   const size_t nacc = (size_t)ncell + DEP_TRASH;
   const size_t lds = sizeof(T) == 4 ? nacc * 8 : 2 * nacc * 8;
@@ -1035,22 +1052,32 @@ static int run_scatter_t(DepositArgs& a) {
   // integer multiplies in the tap addressing)
   const int ty = g.tdim[1], tz = g.tdim[2];
   const bool t1632 = ty == 16 && tz == 32, t1616 = ty == 16 && tz == 16;
-#define DEP_LAUNCH(HB, TYC, TZC, HALO, APOS)                                                   \
+#define DEP_LAUNCH_EK(HB, TYC, TZC, HALO, APOS, EK)                                            \
   TIMED(a.timer, "deposit",                                                                    \
-        (k_deposit<T, ORDER, HB, TYC, TZC, HALO, APOS><<<(unsigned)ntiles, DEP_T, 0, st>>>(     \
+        (k_deposit<T, ORDER, HB, TYC, TZC, HALO, APOS, EK><<<(unsigned)ntiles, DEP_T, 0, st>>>( \
             (const T*)a.sorted, (const T*)a.sorted_w, a.offsets, g, (T*)a.grid[0],              \
             (T*)a.grid[1], (T*)a.grid[2], (T*)a.grid[3], a.stats, a.sorted_cap, a.errs)))
-  // halo rows: FP32 16x16x32 tiles with two or more tiles along z (one tile
-  // wraps its own z taps); A >= 0 when it is the plain count (no weights)
-  const bool halo = a.halo && sizeof(T) == 4 && t1632 && g.ntile[2] >= 2 && g.tdim[0] * 16 >= 64;
+#define DEP_LAUNCH(HB, TYC, TZC, HALO, APOS) DEP_LAUNCH_EK(HB, TYC, TZC, HALO, APOS, EK_RUNTIME)
+#define DEP_LAUNCH_HALO(HB, APOS)                                                               \
+  switch (edge_kind(g)) {                                                                      \
+    case EK_PERIODIC: DEP_LAUNCH_EK(HB, 16, 32, true, APOS, EK_PERIODIC); break;               \
+    case EK_REF_LOS0: DEP_LAUNCH_EK(HB, 16, 32, true, APOS, EK_REF_LOS0); break;               \
+    default: DEP_LAUNCH_EK(HB, 16, 32, true, APOS, EK_REF_LOS); break;                         \
+  }
+  // halo rows: FP32 16x16x32 tiles with two or more tiles along every axis
+  // (one tile along z wraps its own z taps; along x or y the first tap's
+  // nearest image does not place every tap); A >= 0 when it is the plain
+  // count (no weights)
+  const bool halo = a.halo && sizeof(T) == 4 && t1632 && g.ntile[0] >= 2 && g.ntile[1] >= 2 &&
+                    g.ntile[2] >= 2 && g.tdim[0] * 16 >= 64;
   const bool apos = !g.has_w && g.fa == 0;
   bool done = false;
   if constexpr (sizeof(T) == 4) {
     if (halo) {
-      if (hasB && apos) DEP_LAUNCH(true, 16, 32, true, true);
-      else if (hasB) DEP_LAUNCH(true, 16, 32, true, false);
-      else if (apos) DEP_LAUNCH(false, 16, 32, true, true);
-      else DEP_LAUNCH(false, 16, 32, true, false);
+      if (hasB && apos) { DEP_LAUNCH_HALO(true, true) }
+      else if (hasB) { DEP_LAUNCH_HALO(true, false) }
+      else if (apos) { DEP_LAUNCH_HALO(false, true) }
+      else { DEP_LAUNCH_HALO(false, false) }
       done = true;
     }
   }
@@ -1065,6 +1092,8 @@ static int run_scatter_t(DepositArgs& a) {
     else DEP_LAUNCH(false, 0, 0, false, false);
   }
 #undef DEP_LAUNCH
+#undef DEP_LAUNCH_EK
+#undef DEP_LAUNCH_HALO
   return hipGetLastError() == hipSuccess ? CAPOW_OK : CAPOW_ERR_HIP;
 }
 

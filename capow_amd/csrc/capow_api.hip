@@ -35,6 +35,7 @@ struct capow_ctx {
   int64_t fft_px = 0, fft_py = 0;   // row pitches of the intermediate layouts
   int fft_persist = 1;              // persistent row passes (CAPOW_FFT_PERSIST=0: off)
   bool dep_halo = true;             // halo-row deposit (CAPOW_DEPOSIT_HALO=0: off)
+  bool dep_bank = true;             // its bank-balanced copy order (CAPOW_DEPOSIT_BANK=0: off)
   // batch order of the row passes, bit per pass (z, y, x): outer index
   // fastest (FP32 default z and x: 128-byte write runs of neighbouring
   // outer rows land close together; measured 45.7 -> 42.7-44.9 ms at 1024^3;
@@ -647,6 +648,7 @@ extern "C" int capow_create(const capow_params* p, capow_ctx** out) {
   if (const char* e = getenv("CAPOW_FFT_ROWS_MUL")) c->fft_rows_mul = atoi(e) == 2 ? 2 : (atoi(e) == 1 ? 1 : 0);
   if (const char* e = getenv("CAPOW_FFT_PERSIST")) c->fft_persist = atoi(e) != 0;
   if (const char* e = getenv("CAPOW_DEPOSIT_HALO")) c->dep_halo = atoi(e) != 0;
+  if (const char* e = getenv("CAPOW_DEPOSIT_BANK")) c->dep_bank = atoi(e) != 0;
   if (const char* e = getenv("CAPOW_FFT_ORDER")) c->fft_order = atoi(e) & 7;
   if (const char* e = getenv("CAPOW_BIN_MIRROR")) c->bin_mirror_env = atoi(e) != 0;
   if (const char* e = getenv("CAPOW_XBIN")) c->xbin_env = atoi(e) != 0;
@@ -782,6 +784,7 @@ static int assign_enqueue(capow_ctx* c, double* h_stats, unsigned int* h_errs, b
   a.stats = c->stats;
   for (int s = 0; s < 4; s++) a.grid[s] = c->grid[s];
   a.halo = c->dep_halo;
+  a.bank = c->dep_bank;
   int rc = run_count(a, c->fp64);
   if (rc) return fail(c, rc, std::string("deposit count failed: ") + hipGetErrorString(hipGetLastError()));
   if (exact) {
@@ -1585,6 +1588,7 @@ extern "C" int capow_batch_run(capow_batch* b, const capow_mock* mocks, int32_t 
 //   sort_lds     LDS run aggregation in the sort (on) or the global
 //                counters for every run (off)
 //   dep_halo     halo rows in the FP32 deposit tiles
+//   dep_bank     bank-balanced copy order and masked rows in those tiles
 //   fft_runs     write-run width of the out-of-place FFT passes (auto:
 //                128 bytes, or 64)
 //   fft_order    batch order of the z/y/x passes (bit per pass: outer
@@ -1645,11 +1649,13 @@ extern "C" int capow_autotune(capow_ctx* c, int32_t reps, char* report, int32_t 
   int sort_blocks = c->count_blocks / (c->ncu > 0 ? c->ncu : 256);
   int sort_lds = c->g.sort_probes > 0;
   int halo = c->dep_halo ? 1 : 0, runs = c->fft_rows_mul, order = c->fft_order;
+  int bank = c->dep_bank ? 1 : 0;
   int xbin = c->xbin_ok ? 1 : 0;
   std::vector<TuneKnob> knobs = {
       {"sort_blocks", &sort_blocks, {12, 8, 4}},
       {"sort_lds", &sort_lds, {1, 0}},
       {"dep_halo", &halo, {1, 0}},
+      {"dep_bank", &bank, {1, 0}},
       {"fft_runs", &runs, {0, 1}},
       {"fft_order", &order, {c->fp64 ? 0 : 5, 0, 7, 2}},
   };
@@ -1658,6 +1664,7 @@ extern "C" int capow_autotune(capow_ctx* c, int32_t reps, char* report, int32_t 
     c->count_blocks = sort_blocks * (c->ncu > 0 ? c->ncu : 256);
     c->g.sort_probes = sort_lds ? 32 : 0;
     c->dep_halo = halo != 0;
+    c->dep_bank = bank != 0;
     c->fft_rows_mul = runs;
     c->fft_order = order;
     c->xbin_ok = xbin_avail && xbin != 0;
@@ -1683,8 +1690,10 @@ extern "C" int capow_autotune(capow_ctx* c, int32_t reps, char* report, int32_t 
     *k.value = best;
     apply();
   }
-  tune_note(rep, "chosen: sort_blocks=%d sort_lds=%d dep_halo=%d fft_runs=%d fft_order=%d xbin=%d\n",
-            sort_blocks, sort_lds, halo, runs, order, c->xbin_ok ? 1 : 0);
+  tune_note(rep,
+            "chosen: sort_blocks=%d sort_lds=%d dep_halo=%d dep_bank=%d fft_runs=%d fft_order=%d "
+            "xbin=%d\n",
+            sort_blocks, sort_lds, halo, bank, runs, order, c->xbin_ok ? 1 : 0);
   if (report && report_len > 0) {
     strncpy(report, rep.c_str(), (size_t)report_len - 1);
     report[report_len - 1] = 0;
@@ -1716,6 +1725,7 @@ extern "C" int capow_batch_autotune(capow_batch* b, const capow_mock* mocks, int
       c->count_blocks = c0->count_blocks;
       c->g.sort_probes = c0->g.sort_probes;
       c->dep_halo = c0->dep_halo;
+      c->dep_bank = c0->dep_bank;
       c->fft_rows_mul = c0->fft_rows_mul;
       c->fft_order = c0->fft_order;
       c->xbin_ok = c0->xbin_ok;

@@ -608,9 +608,24 @@ __device__ __forceinline__ int zmask(int rowi, int TZ) {
 // to a row of the tile.  APOS (field A = w v^0 with no weights: A >= 0):
 // the int64 accumulator is B * 2^32 + A, so the int32 taps need no sign
 // borrow.
+//
+// BANK (HALO only): before each phase the tile's own copies are reordered
+// in LDS so that the 64 lanes of a wave start their taps in evenly spread
+// LDS banks, and tap rows outside the tile are masked off instead of adding
+// zeros.  A tap is the lane's base cell plus an offset common to all lanes
+// ((tx TY + ty) RZ + tz cells), so lanes whose base cells are spread evenly
+// over the 16 8-byte bank pairs stay spread over every tap: 64-bit LDS
+// atomics from such lanes take about half the time of random ones.  The
+// base cell's bank pair is (lx0 TY RZ + ly0 RZ + lz0) mod 16 = (6 ly0 + lz0)
+// mod 16 up to a constant (TY RZ = 608 = 0 mod 16, RZ = 38 = 6 mod 16).  A
+// counting sort by that key with slots in (rank within key, key) order makes
+// consecutive slots cycle through the keys.  The accumulation is exact
+// integer arithmetic, so the order changes no result.
 #define DEP_ZLO 4   // halo cells below the tile's z range (taps reach -3)
 #define DEP_ZHI 2   // above (taps reach TZ + 2: the next row's first cells)
-template <typename T, int ORDER, bool HASB, int TYC, int TZC, bool HALO, bool APOS, int EK>
+#define DEP_NKEY 16 // bank-pair keys of the BANK order
+template <typename T, int ORDER, bool HASB, int TYC, int TZC, bool HALO, bool APOS, int EK,
+          bool BANK>
 __global__ __launch_bounds__(DEP_T) void k_deposit(const T* __restrict__ sorted,
                                                    const T* __restrict__ sorted_w,
                                                    const uint32_t* __restrict__ off,
@@ -628,6 +643,10 @@ __global__ __launch_bounds__(DEP_T) void k_deposit(const T* __restrict__ sorted,
                             : ((TYC && TZC) ? (16 * TYC * TZC + DEP_TRASH) * 8 * (F32 ? 1 : 2)
                                             : DEP_LDS_MAX);
   static_assert(!HALO || (F32 && TYC && TZC), "halo rows: FP32, compile-time extents");
+  static_assert(!BANK || HALO, "bank order: halo rows only");
+  // BANK scratch (before a phase's clear): key counts, the (rank, key) slot
+  // tables and the reordered copies (x, y, z, a, b)
+  static_assert(!BANK || (DEP_NKEY + 2 * DEP_T) * 4 + 5 * DEP_T * DEP_K * 4 <= LDSB, "bank scratch");
   __shared__ __align__(16) unsigned char smem[LDSB];
   // TYC, TZC: compile-time tile extents (0: the runtime ones)
   const int TX = g.tdim[0];
@@ -721,22 +740,34 @@ __global__ __launch_bounds__(DEP_T) void k_deposit(const T* __restrict__ sorted,
     const int n4 = HALO ? (TX * TY * RZ + 2) / 2 : (F32 ? nacc / 2 : nacc);   // 16-byte words
     for (int c = threadIdx.x; c < n4; c += DEP_T) z4[c] = make_uint4(0u, 0u, 0u, 0u);
   };
-  clear_acc();
+  // (BANK: the LDS holds the reorder scratch first; each phase clears)
+  if (!BANK) clear_acc();
   double ma = 0.0, mb = 0.0;
   if (single) {
+    // every index first, then every load, then the first use: one memory
+    // latency per tile instead of one per copy slot
+    uint32_t idx[DEP_K];
 #pragma unroll
     for (int k = 0; k < DEP_K; k++) {
       const uint32_t f = (uint32_t)(k * DEP_T) + threadIdx.x;
       ph_has[k] = f < total;
+      idx[k] = ph_has[k] ? pidx(f) : 0u;
+    }
+    T pv[DEP_K], pw[DEP_K];
+#pragma unroll
+    for (int k = 0; k < DEP_K; k++) {
       T x = T(0), y = T(0), z = T(0), v = T(0), w = T(1);
       if (ph_has[k]) {
-        const uint32_t i = pidx(f);
-        load4<T>(sorted, i, x, y, z, v);
-        if (g.has_w) w = sorted_w[i];
+        load4<T>(sorted, idx[k], x, y, z, v);
+        if (g.has_w) w = sorted_w[idx[k]];
       }
       px[k] = x; py[k] = y; pz[k] = z;
-      pa[k] = ph_has[k] ? w * vpow<T>(v, g.fa) : T(0);
-      pb[k] = (HASB && ph_has[k]) ? w * vpow<T>(v, g.fb) : T(0);
+      pv[k] = v; pw[k] = w;
+    }
+#pragma unroll
+    for (int k = 0; k < DEP_K; k++) {
+      pa[k] = ph_has[k] ? pw[k] * vpow<T>(pv[k], g.fa) : T(0);
+      pb[k] = (HASB && ph_has[k]) ? pw[k] * vpow<T>(pv[k], g.fb) : T(0);
       ma = fmax(ma, fabs((double)pa[k]));
       mb = fmax(mb, fabs((double)pb[k]));
     }
@@ -772,7 +803,80 @@ __global__ __launch_bounds__(DEP_T) void k_deposit(const T* __restrict__ sorted,
     }
   }
   for (int ph = 0; ph < (g.interlace ? 2 : 1); ph++) {
-    if (ph) {
+    if constexpr (BANK) {
+      if (single && own > 0) {
+        uint32_t* s_cnt = reinterpret_cast<uint32_t*>(smem);   // [DEP_NKEY]
+        uint32_t* s_T = s_cnt + DEP_NKEY;                      // [DEP_T]
+        uint32_t* s_M = s_T + DEP_T;                           // [DEP_T]
+        T* s_p = reinterpret_cast<T*>(s_M + DEP_T);            // [5][DEP_T DEP_K]
+        constexpr int NS = DEP_T * DEP_K;
+        if (threadIdx.x < DEP_NKEY) s_cnt[threadIdx.x] = 0u;
+        __syncthreads();
+        // key of each own copy: the bank pair of its first tap (for this
+        // phase's shift); an approximate cell is enough, the key only
+        // spreads the lanes
+        const T sh1 = ph ? d1 / T(2) : T(0), sh2 = ph ? d2 / T(2) : T(0);
+        const T rnd = ORDER == 3 ? T(0.5) : T(0);
+        uint32_t key[DEP_K], rk[DEP_K];
+#pragma unroll
+        for (int k = 0; k < DEP_K; k++) {
+          const uint32_t f = (uint32_t)(k * DEP_T) + threadIdx.x;
+          key[k] = 0u;
+          rk[k] = 0u;
+          if (f < own) {
+            const int cy = (int)floor((py[k] + sh1 - (T)g.lo[1]) * (T)g.invd[1] + rnd);
+            const int cz = (int)floor((pz[k] + sh2 - (T)g.lo[2]) * (T)g.invd[2] + rnd);
+            key[k] = (uint32_t)(6 * cy + cz) & (DEP_NKEY - 1);
+            rk[k] = atomicAdd(&s_cnt[key[k]], 1u);
+          }
+        }
+        __syncthreads();
+        // slot of (rank r, key b): T[r] = sum_b min(cnt_b, r) copies of lower
+        // rank, then the keys below b that still have a copy of rank r
+        // (mask M[r]); thread r fills row r
+        uint32_t tr = 0u, mr = 0u, mx = 0u;
+#pragma unroll
+        for (int b = 0; b < DEP_NKEY; b++) {
+          const uint32_t cb = s_cnt[b];
+          tr += min(cb, (uint32_t)threadIdx.x);
+          mr |= (cb > (uint32_t)threadIdx.x ? 1u : 0u) << b;
+          mx = max(mx, cb);
+        }
+        s_T[threadIdx.x] = tr;
+        s_M[threadIdx.x] = mr;
+        __syncthreads();
+        // mx is the same in every thread; ranks beyond the table keep the
+        // sorted order
+        if (mx <= (uint32_t)DEP_T) {
+#pragma unroll
+          for (int k = 0; k < DEP_K; k++) {
+            const uint32_t f = (uint32_t)(k * DEP_T) + threadIdx.x;
+            if (f < own) {
+              const uint32_t pos = s_T[rk[k]] + (uint32_t)__popc(s_M[rk[k]] & ((1u << key[k]) - 1u));
+              s_p[pos] = px[k];
+              s_p[NS + pos] = py[k];
+              s_p[2 * NS + pos] = pz[k];
+              s_p[3 * NS + pos] = pa[k];
+              if (HASB) s_p[4 * NS + pos] = pb[k];
+            }
+          }
+          __syncthreads();
+#pragma unroll
+          for (int k = 0; k < DEP_K; k++) {
+            const uint32_t f = (uint32_t)(k * DEP_T) + threadIdx.x;
+            if (f < own) {
+              px[k] = s_p[f];
+              py[k] = s_p[NS + f];
+              pz[k] = s_p[2 * NS + f];
+              pa[k] = s_p[3 * NS + f];
+              if (HASB) pb[k] = s_p[4 * NS + f];
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    if (ph || BANK) {
       clear_acc();
       __syncthreads();
     }
@@ -846,9 +950,13 @@ __global__ __launch_bounds__(DEP_T) void k_deposit(const T* __restrict__ sorted,
               // one border class share the pattern): nothing to add
               if (__builtin_amdgcn_ballot_w64(vxy) == 0) continue;
               const uint32_t rowb = (uint32_t)(base_b + (tx * TY + ty) * (RZ * 8));
+              // BANK: lanes outside the tile sit this row out (the row's
+              // three atomics run under a narrower mask); otherwise they add
+              // zeros to a row of the tile
+              if (BANK && !vxy) continue;
               const T wxy = vxy ? ax.w[tx] * ay.w[ty] : T(0);
               const T qa0 = va * wxy, qb0 = vb * wxy;
-              unsigned char* rb = smem + (vxy ? rowb : lane_row);
+              unsigned char* rb = smem + ((BANK || vxy) ? rowb : lane_row);
 #pragma unroll
               for (int tz = 0; tz < ORDER; tz++) {
                 const int qa = round_rpi((float)(qa0 * az.w[tz]));
@@ -1052,18 +1160,22 @@ static int run_scatter_t(DepositArgs& a) {
   // integer multiplies in the tap addressing)
   const int ty = g.tdim[1], tz = g.tdim[2];
   const bool t1632 = ty == 16 && tz == 32, t1616 = ty == 16 && tz == 16;
-#define DEP_LAUNCH_EK(HB, TYC, TZC, HALO, APOS, EK)                                            \
+#define DEP_LAUNCH_EK(HB, TYC, TZC, HALO, APOS, EK, BK)                                        \
   TIMED(a.timer, "deposit",                                                                    \
-        (k_deposit<T, ORDER, HB, TYC, TZC, HALO, APOS, EK><<<(unsigned)ntiles, DEP_T, 0, st>>>( \
+        (k_deposit<T, ORDER, HB, TYC, TZC, HALO, APOS, EK, BK><<<(unsigned)ntiles, DEP_T, 0, st>>>( \
             (const T*)a.sorted, (const T*)a.sorted_w, a.offsets, g, (T*)a.grid[0],              \
             (T*)a.grid[1], (T*)a.grid[2], (T*)a.grid[3], a.stats, a.sorted_cap, a.errs)))
-#define DEP_LAUNCH(HB, TYC, TZC, HALO, APOS) DEP_LAUNCH_EK(HB, TYC, TZC, HALO, APOS, EK_RUNTIME)
-#define DEP_LAUNCH_HALO(HB, APOS)                                                               \
+#define DEP_LAUNCH(HB, TYC, TZC, HALO, APOS) \
+  DEP_LAUNCH_EK(HB, TYC, TZC, HALO, APOS, EK_RUNTIME, false)
+#define DEP_LAUNCH_HALO_B(HB, APOS, BK)                                                         \
   switch (edge_kind(g)) {                                                                      \
-    case EK_PERIODIC: DEP_LAUNCH_EK(HB, 16, 32, true, APOS, EK_PERIODIC); break;               \
-    case EK_REF_LOS0: DEP_LAUNCH_EK(HB, 16, 32, true, APOS, EK_REF_LOS0); break;               \
-    default: DEP_LAUNCH_EK(HB, 16, 32, true, APOS, EK_REF_LOS); break;                         \
+    case EK_PERIODIC: DEP_LAUNCH_EK(HB, 16, 32, true, APOS, EK_PERIODIC, BK); break;           \
+    case EK_REF_LOS0: DEP_LAUNCH_EK(HB, 16, 32, true, APOS, EK_REF_LOS0, BK); break;           \
+    default: DEP_LAUNCH_EK(HB, 16, 32, true, APOS, EK_REF_LOS, BK); break;                     \
   }
+#define DEP_LAUNCH_HALO(HB, APOS)                 \
+  if (a.bank) { DEP_LAUNCH_HALO_B(HB, APOS, true) } \
+  else { DEP_LAUNCH_HALO_B(HB, APOS, false) }
   // halo rows: FP32 16x16x32 tiles with two or more tiles along every axis
   // (one tile along z wraps its own z taps; along x or y the first tap's
   // nearest image does not place every tap); A >= 0 when it is the plain
@@ -1094,6 +1206,7 @@ static int run_scatter_t(DepositArgs& a) {
 #undef DEP_LAUNCH
 #undef DEP_LAUNCH_EK
 #undef DEP_LAUNCH_HALO
+#undef DEP_LAUNCH_HALO_B
   return hipGetLastError() == hipSuccess ? CAPOW_OK : CAPOW_ERR_HIP;
 }
 

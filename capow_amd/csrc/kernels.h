@@ -100,6 +100,7 @@ struct DepositArgs {
   double* stats;          // STAT_TOTAL
   void* grid[4];          // A, A_int, B, B_int (null when unused)
   bool halo = true;       // halo-row deposit where the tile shape allows
+  bool bank = true;       // halo-row deposit: bank-balanced copy order, masked rows
 };
 // count + stats + scan; then scatter + tile deposit
 int run_count(DepositArgs& a, bool fp64);

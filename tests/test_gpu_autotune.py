@@ -33,7 +33,8 @@ def test_autotune_reports_and_keeps_results():
     before = ps.power()
     rep = ps.autotune(reps=2)
     say(rep)
-    for k in ("sort_blocks=", "sort_lds=", "dep_halo=", "fft_runs=", "fft_order=", "chosen:"):
+    for k in ("sort_blocks=", "sort_lds=", "dep_halo=", "dep_bank=", "fft_runs=", "fft_order=",
+              "chosen:"):
         assert k in rep, k
     xbin_now = "xbin=1" in rep.split("chosen:")[1]
     after = ps.power()

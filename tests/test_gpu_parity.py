@@ -120,13 +120,18 @@ def test_deposit_matches_oracle(case, oracle):
 
 
 HALO_CASES = [
-    # name, (nx, ny, nz), order, interlace, los, momentum, weights
-    ("tsc_int_los0_m1", (64, 64, 64), 3, 1, 0, 1, False),
-    ("cic_int_los3_m3", (96, 64, 128), 2, 1, 3, 3, False),
-    ("ngp_los0_m2", (64, 48, 64), 1, 0, 0, 2, False),
-    ("tsc_los1_m1_w", (64, 32, 96), 3, 0, 1, 1, True),
-    ("tsc_int_los2_m6", (48, 64, 64), 3, 1, 2, 6, False),
-    ("cic_los0_m0_w", (32, 64, 64), 2, 0, 0, 0, True),
+    # name, (nx, ny, nz), order, interlace, los, momentum, weights, clustered copies
+    ("tsc_int_los0_m1", (64, 64, 64), 3, 1, 0, 1, False, 0),
+    ("cic_int_los3_m3", (96, 64, 128), 2, 1, 3, 3, False, 0),
+    ("ngp_los0_m2", (64, 48, 64), 1, 0, 0, 2, False, 0),
+    ("tsc_los1_m1_w", (64, 32, 96), 3, 0, 1, 1, True, 0),
+    ("tsc_int_los2_m6", (48, 64, 64), 3, 1, 2, 6, False, 0),
+    ("cic_los0_m0_w", (32, 64, 64), 2, 0, 0, 0, True, 0),
+    # one tile with 900 copies of one bank key (beyond the 512 ranks of the
+    # reorder table: that tile keeps the sorted order) ...
+    ("tsc_int_cluster900", (64, 64, 64), 3, 1, 0, 1, False, 900),
+    # ... and one with more copies than the registers hold (reloaded per phase)
+    ("tsc_int_cluster3000_w", (64, 64, 64), 3, 1, 0, 3, True, 3000),
 ]
 
 
@@ -139,7 +144,7 @@ def test_fp32_halo_deposit_matches_plain_tiles(case, monkeypatch):
     oracle, and integer sums do not depend on the order: the grids must be
     bitwise equal.  Particles sit on every edge rule of every axis, and
     weights (when used) take both signs."""
-    name, (nx, ny, nz), order, inter, los, mom, weighted = case
+    name, (nx, ny, nz), order, inter, los, mom, weighted, ncl = case
     say("halo deposit case", name)
     p = api.make_params(nx=nx, ny=ny, nz=nz, xmax=L, ymax=L, zmax=L, interp_order=order,
                         do_interlacing=inter, los=los, momentum=mom, precision=api.FP32,
@@ -148,7 +153,11 @@ def test_fp32_halo_deposit_matches_plain_tiles(case, monkeypatch):
     pos, vel = make_catalogue(nx, 30000, los, nz, mom)
     extra = np.concatenate([edge_particles(ny, L, los, rng), edge_particles(nz, L, los, rng)])
     pos = np.concatenate([pos, extra])
-    vel = np.concatenate([vel, rng.normal(0, 300.0, size=(len(extra), 3))])
+    if ncl:
+        # inside one cell of an interior tile: every copy has the same key
+        c0 = np.array([20.3, 21.6, 40.2]) * (L / np.array([nx, ny, nz]))
+        pos = np.concatenate([pos, c0 + rng.uniform(0, 0.2 * L / nx, size=(ncl, 3))])
+    vel = np.concatenate([vel, rng.normal(0, 300.0, size=(len(pos) - len(vel), 3))])
     xyzv = mocks.xyzv(pos, vel, los)
     w = rng.uniform(-0.5, 2.0, size=len(pos)) if weighted else None
 
@@ -166,15 +175,19 @@ def test_fp32_halo_deposit_matches_plain_tiles(case, monkeypatch):
         ps.close()
         return out
 
-    halo = grids()
+    halo = grids()   # bank-balanced copy order, masked rows
+    monkeypatch.setenv("CAPOW_DEPOSIT_BANK", "0")
+    halo_sorted = grids()
+    monkeypatch.delenv("CAPOW_DEPOSIT_BANK")
     monkeypatch.setenv("CAPOW_DEPOSIT_HALO", "0")
     plain = grids()
     monkeypatch.delenv("CAPOW_DEPOSIT_HALO")
     assert any(g is not None for g in halo)
-    for s in range(4):
-        assert (halo[s] is None) == (plain[s] is None), s
-        if halo[s] is not None:
-            assert np.array_equal(halo[s].view(np.uint32), plain[s].view(np.uint32)), (name, s)
+    for other in (halo_sorted, plain):
+        for s in range(4):
+            assert (halo[s] is None) == (other[s] is None), s
+            if halo[s] is not None:
+                assert np.array_equal(halo[s].view(np.uint32), other[s].view(np.uint32)), (name, s)
 
 
 @pytest.mark.parametrize("prec", [api.FP64, api.FP32])

@@ -180,13 +180,27 @@ def run_adjoint(args):
     torch.cuda.synchronize()
     bwd = (time.perf_counter() - t0) * 1e3 / reps
     ok = bool(torch.isfinite(gx).all().item() and torch.isfinite(gw).all().item())
+    # per-kernel HIP-event times of one more backward (not in the figures above)
+    ps.set_timing(True)
+    ps.reset_kernel_times()
+    ps.set_particles(parts, w)
+    ps.backward(G)
+    kt = {k: round(v[0], 4) for k, v in ps.kernel_times().items()}
+    ps.set_timing(False)
+    # gather: sorted rows and weights, the two gradient grids (density, main
+    # and interlaced) with each
+    # tile's halo, the index and the gradient rows (algorithmic: the grids
+    # once, without the halo)
+    gbytes = 2 * n * n * (n + 2) * 4 + parts.shape[0] * (16 + 4 + 4 + 16 + 4)
+    if "gather" in kt:
+        kt["gather_gbs"] = round(gbytes / (kt["gather"] * 1e-3) / 1e9, 1)
     ps.close()
     del parts, w, gx, gw
     torch.cuda.empty_cache()
     out = dict(workload="16,777,216 Zel'dovich particles, 512^3, TSC+interlacing, P_0,2,4, FP32",
                forward_ms=round(fwd, 3), backward_ms=round(bwd, 3),
                backward_includes="assignment + FFT recomputed, binning backward, inverse FFT, gather",
-               gradients_finite=ok)
+               backward_kernels_ms=kt, gradients_finite=ok)
     log("adjoint: forward %.2f ms, backward %.2f ms" % (fwd, bwd))
     return out
 

@@ -210,6 +210,222 @@ __global__ __launch_bounds__(256) void k_gather(const T* __restrict__ parts,
   }
 }
 
+// Tile gather: the same transposed deposit, one workgroup per tile of the
+// forward's counting sort.  The sort leaves a tile's own particles (those
+// whose main cell lies in the tile: one row per particle) contiguous, with
+// every tap of either deposit within one cell below and two above the tile
+// per axis (the sort flags any other).  So per phase the workgroup stages
+// the tile's gradient region -- (T + 3) cells per axis, the A and B values
+// of a cell side by side -- in LDS with coalesced loads, and its particles
+// read their taps there: the grids are read once per tile (plus the halo)
+// instead of once per tap.  A tap row's three z values are summed with the
+// weights and their derivatives first (separable weights), so per row
+//   dL/dA += wx wy S_a,  dL/dx += dwx wy F,  dL/dy += wx dwy F,
+//   dL/dz += wx wy F'    (S_a = sum_z wz g_A, F = fa S_a + fb S_b, F' with dwz)
+// in the working precision, with the rows summed in FP64.  Particles the
+// sort dropped (outside every deposit) keep the zero gradient written
+// before.  Rows are written back at their input index.
+#define GT_T 1024  // threads per tile workgroup (16 waves: at most 128 VGPRs)
+#define GT_K (sizeof(T) == 8 ? 1 : 2)   // own particles per thread per pass over the region
+#define GT_U 4     // region cells per thread whose loads are in flight together
+template <typename T, int ORDER, bool HASB>
+__global__ __launch_bounds__(GT_T) void k_gather_tiles(
+    const T* __restrict__ sorted, const T* __restrict__ sorted_w,
+    const uint32_t* __restrict__ sidx, const uint32_t* __restrict__ off, Geom g,
+    const T* __restrict__ gA, const T* __restrict__ gAi, const T* __restrict__ gB,
+    const T* __restrict__ gBi, const double* __restrict__ g0, T* __restrict__ gout,
+    T* __restrict__ gw, unsigned int* __restrict__ errs, FastDiv divz, FastDiv divy) {
+  typedef typename CT<T>::t C2;
+  extern __shared__ __align__(16) unsigned char smem[];
+  C2* regAB = reinterpret_cast<C2*>(smem);   // HASB: (A, B) per cell
+  T* regA = reinterpret_cast<T*>(smem);      // otherwise A only
+  const int TX = g.tdim[0], TY = g.tdim[1], TZ = g.tdim[2];
+  const int RX = TX + 3, RY = TY + 3, RZ = TZ + 3;
+  const int ncell = RX * RY * RZ;
+  const int64_t tile = blockIdx.x;
+  const int ttx = (int)(tile % g.ntile[0]);
+  const int tty = (int)((tile / g.ntile[0]) % g.ntile[1]);
+  const int ttz = (int)(tile / ((int64_t)g.ntile[0] * g.ntile[1]));
+  const int t0x = ttx * TX, t0y = tty * TY, t0z = ttz * TZ;
+  const uint32_t s0 = off[tile * 27], own = off[tile * 27 + 27] - s0;
+  if (own == 0) return;   // the whole workgroup: no barrier reached yet
+  const bool per = g.periodic != 0;
+  const double dalpha_w = g.mean_sub_a ? -(g0[0] + (g.interlace ? g0[2] : 0.0)) : 0.0;
+  const T d0 = (T)g.d[0], d1 = (T)g.d[1], d2 = (T)g.d[2];
+  // region index of a tile-relative first tap (the nearest periodic image)
+  auto loc = [&](int first, int t0, int T_, int n) {
+    int o = first - t0;
+    if (per) {
+      if (o > T_ + 2) o -= n;
+      else if (o < -1) o += n;
+    }
+    return o + 1;
+  };
+  // grid index of region cell l along an axis (-1: outside a non-periodic grid)
+  auto gidx = [&](int l, int t0, int n) {
+    int q = t0 - 1 + l;
+    if (q < 0 || q >= n) {
+      if (!per) return -1;
+      q += q < 0 ? n : -n;
+    }
+    return q;
+  };
+  for (uint32_t base = 0; base < own; base += GT_T * GT_K) {
+    double dfa[GT_K], dfb[GT_K], ds0[GT_K], ds1[GT_K], ds2[GT_K];
+    bool mok[GT_K];
+    // the chunk's particles, loaded once for both phases (every slot's load
+    // in flight before the first use)
+    T px[GT_K], py[GT_K], pz[GT_K], pv[GT_K], pw[GT_K];
+#pragma unroll
+    for (int k = 0; k < GT_K; k++) {
+      dfa[k] = dfb[k] = ds0[k] = ds1[k] = ds2[k] = 0.0;
+      mok[k] = false;
+      const uint32_t f = base + (uint32_t)(k * GT_T) + threadIdx.x;
+      T x = T(0), y = T(0), z = T(0), v = T(0), w = T(1);
+      if (f < own) {
+        load4<T>(sorted, s0 + f, x, y, z, v);
+        if (g.has_w) w = sorted_w[s0 + f];
+      }
+      px[k] = x; py[k] = y; pz[k] = z; pv[k] = v; pw[k] = w;
+    }
+    for (int ph = 0; ph < (g.interlace ? 2 : 1); ph++) {
+      const T* ga = ph ? gAi : gA;
+      const T* gb = ph ? gBi : gB;
+      __syncthreads();   // the region's previous readers are done
+      // GT_U cells per thread per round, their loads issued together
+      for (int c0 = threadIdx.x; c0 < ncell; c0 += GT_T * GT_U) {
+        T av[GT_U], bv[GT_U];
+#pragma unroll
+        for (int u = 0; u < GT_U; u++) {
+          const int c = c0 + u * GT_T;
+          const uint32_t r = divz.div((uint32_t)c);
+          const int lz = c - (int)r * RZ;
+          const int lx = (int)divy.div(r), ly = (int)r - lx * RY;
+          const int qx = gidx(lx, t0x, g.n[0]), qy = gidx(ly, t0y, g.n[1]), qz = gidx(lz, t0z, g.n[2]);
+          av[u] = bv[u] = T(0);
+          if (c < ncell && qx >= 0 && qy >= 0 && qz >= 0) {
+            const int64_t e = ((int64_t)qx * g.n[1] + qy) * g.rowlen + qz;
+            av[u] = ga[e];
+            if (HASB) bv[u] = gb[e];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < GT_U; u++) {
+          const int c = c0 + u * GT_T;
+          if (c < ncell) {
+            if (HASB) regAB[c] = C2{av[u], bv[u]};
+            else regA[c] = av[u];
+          }
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < GT_K; k++) {
+        const uint32_t f = base + (uint32_t)(k * GT_T) + threadIdx.x;
+        if (f >= own) continue;
+        T x = px[k], y = py[k], z = pz[k];
+        const T v = pv[k], w = pw[k];
+        if (ph) {
+          x += d0 / T(2);
+          y += d1 / T(2);
+          z += d2 / T(2);
+        }
+        const AxisOut<T> ax = axis_taps<T, ORDER>(x, (T)g.lo[0], (T)g.hi[0], d0, (T)g.L[0], g.n[0], true, g.los, g.edge, (T)g.invd[0]);
+        const AxisOut<T> ay = axis_taps<T, ORDER>(y, (T)g.lo[1], (T)g.hi[1], d1, (T)g.L[1], g.n[1], false, g.los, g.edge, (T)g.invd[1]);
+        const AxisOut<T> az = axis_taps<T, ORDER>(z, (T)g.lo[2], (T)g.hi[2], d2, (T)g.L[2], g.n[2], false, g.los, g.edge, (T)g.invd[2]);
+        if (!(ax.ok && ay.ok && az.ok)) continue;
+        if (ph == 0) mok[k] = true;
+        const int lx0 = loc(ax.first, t0x, TX, g.n[0]);
+        const int ly0 = loc(ay.first, t0y, TY, g.n[1]);
+        const int lz0 = loc(az.first, t0z, TZ, g.n[2]);
+        // Taps outside the region can only be taps the edge rules drop
+        // (zero weight; e.g. TSC x in [MAX, MAX + dx) with a line of sight:
+        // its first tap is the last grid slice): they read zero.  A live
+        // tap outside means the sort's reach check failed.
+        bool vx[ORDER], vy[ORDER], vz[ORDER];
+        bool bad = false;
+#pragma unroll
+        for (int t = 0; t < ORDER; t++) {
+          vx[t] = (unsigned)(lx0 + t) < (unsigned)RX;
+          vy[t] = (unsigned)(ly0 + t) < (unsigned)RY;
+          vz[t] = (unsigned)(lz0 + t) < (unsigned)RZ;
+          bad = bad || (!vx[t] && (ax.w[t] != T(0) || ax.dw[t] != T(0))) ||
+                (!vy[t] && (ay.w[t] != T(0) || ay.dw[t] != T(0))) ||
+                (!vz[t] && (az.w[t] != T(0) || az.dw[t] != T(0)));
+        }
+        if (bad) {
+          atomicOr(&errs[0], 1u);
+          continue;
+        }
+        const T fa = w * vpow_b<T>(v, g.fa);
+        const T fb = HASB ? w * vpow_b<T>(v, g.fb) : T(0);
+#pragma unroll
+        for (int tx = 0; tx < ORDER; tx++) {
+#pragma unroll
+          for (int ty = 0; ty < ORDER; ty++) {
+            if (!(vx[tx] && vy[ty])) continue;
+            const int rb = ((lx0 + tx) * RY + (ly0 + ty)) * RZ + lz0;
+            T sa = T(0), sb = T(0), da = T(0), db = T(0);
+#pragma unroll
+            for (int tz = 0; tz < ORDER; tz++) {
+              T a = T(0), b = T(0);
+              if (vz[tz]) {
+                if (HASB) {
+                  const C2 ab = regAB[rb + tz];
+                  a = ab.x;
+                  b = ab.y;
+                } else {
+                  a = regA[rb + tz];
+                }
+              }
+              sa += az.w[tz] * a;
+              da += az.dw[tz] * a;
+              if (HASB) {
+                sb += az.w[tz] * b;
+                db += az.dw[tz] * b;
+              }
+            }
+            const T wxy = ax.w[tx] * ay.w[ty];
+            const double fw = (double)(fa * sa + fb * sb), fd = (double)(fa * da + fb * db);
+            dfa[k] += (double)(wxy * sa);
+            if (HASB) dfb[k] += (double)(wxy * sb);
+            ds0[k] += (double)(ax.dw[tx] * ay.w[ty]) * fw;
+            ds1[k] += (double)(ax.w[tx] * ay.dw[ty]) * fw;
+            ds2[k] += (double)wxy * fd;
+          }
+        }
+      }
+    }
+    // the chain through the fields, the RSD shift and alpha, as k_gather
+#pragma unroll
+    for (int k = 0; k < GT_K; k++) {
+      const uint32_t f = base + (uint32_t)(k * GT_T) + threadIdx.x;
+      if (f >= own) continue;
+      const T v = pv[k], w = pw[k];
+      const double gx = ds0[k] / g.d[0], gy = ds1[k] / g.d[1], gz = ds2[k] / g.d[2];
+      const int ea = g.fa, eb = g.fb;
+      double dw = dfa[k] * (double)vpow_b<T>(v, ea) + (HASB ? dfb[k] * (double)vpow_b<T>(v, eb) : 0.0);
+      double dv = 0.0;
+      if (ea > 0) dv += dfa[k] * (double)w * ea * (double)vpow_b<T>(v, ea - 1);
+      if (HASB && eb > 0) dv += dfb[k] * (double)w * eb * (double)vpow_b<T>(v, eb - 1);
+      if (g.rsd_axis >= 0) {
+        const double gl = g.rsd_axis == 0 ? gx : (g.rsd_axis == 1 ? gy : gz);
+        dv += gl * g.rsd_f1 / g.rsd_den;
+      }
+      if (mok[k]) dw += dalpha_w;
+      const int64_t p = (int64_t)sidx[s0 + f];
+      store4<T>(gout, p, (T)gx, (T)gy, (T)gz, (T)dv);
+      if (gw) gw[p] = (T)dw;
+    }
+  }
+}
+
+// LDS bytes of the tile gather's region for the context's tiles
+size_t gather_tiles_lds(const Geom& g, bool fp64) {
+  const size_t cells = (size_t)(g.tdim[0] + 3) * (g.tdim[1] + 3) * (g.tdim[2] + 3);
+  return cells * (fp64 ? 8 : 4) * (g.fb >= 0 ? 2 : 1);
+}
+
 int run_bin_backward(const BinLaunch& L, const double* G, double* g0, bool fp64) {
   const int blocks = 2048;
   if (fp64)
@@ -235,7 +451,48 @@ static void launch_gather(const GatherArgs& a) {
             (const T*)a.grid[2], (const T*)a.grid[3], a.g0, (T*)a.gout, (T*)a.gw)));
 }
 
+template <typename T, int ORDER>
+static int launch_gather_tiles(const GatherArgs& a) {
+  const Geom& g = *a.g;
+  const size_t lds = gather_tiles_lds(g, sizeof(T) == 8);
+  const int64_t ntiles = (int64_t)g.ntile[0] * g.ntile[1] * g.ntile[2];
+  // particles outside every deposit are not in the sort: zero gradients
+  const int64_t words = a.n * (int64_t)(sizeof(T) / 4);
+  if (zero_u32((uint32_t*)a.gout, 4 * words, a.stream) != CAPOW_OK) return CAPOW_ERR_HIP;
+  if (a.gw && zero_u32((uint32_t*)a.gw, words, a.stream) != CAPOW_OK) return CAPOW_ERR_HIP;
+  FastDiv divz, divy;   // region cell -> (row, z), row -> (x, y)
+  divz.init((uint32_t)(g.tdim[2] + 3));
+  divy.init((uint32_t)(g.tdim[1] + 3));
+  const bool hb = g.fb >= 0;
+  const void* fn = hb ? (const void*)k_gather_tiles<T, ORDER, true> : (const void*)k_gather_tiles<T, ORDER, false>;
+  if (lds > 64 * 1024 &&
+      hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+    return CAPOW_ERR_HIP;
+#define GT_LAUNCH(HB)                                                                           \
+  TIMED(a.timer, "gather",                                                                      \
+        (k_gather_tiles<T, ORDER, HB><<<(unsigned)ntiles, GT_T, lds, a.stream>>>(                \
+            (const T*)a.sorted, (const T*)a.sorted_w, a.sorted_idx, a.offsets, g,                \
+            (const T*)a.grid[0], (const T*)a.grid[1], (const T*)a.grid[2], (const T*)a.grid[3],  \
+            a.g0, (T*)a.gout, (T*)a.gw, a.errs, divz, divy)))
+  if (hb) GT_LAUNCH(true);
+  else GT_LAUNCH(false);
+#undef GT_LAUNCH
+  return hipGetLastError() == hipSuccess ? CAPOW_OK : CAPOW_ERR_HIP;
+}
+
 int run_gather(const GatherArgs& a, bool fp64) {
+  if (a.sorted && a.sorted_idx && a.offsets && a.errs &&
+      gather_tiles_lds(*a.g, fp64) <= 160 * 1024) {
+    switch (a.g->order * 2 + (fp64 ? 1 : 0)) {
+      case 2: return launch_gather_tiles<float, 1>(a);
+      case 3: return launch_gather_tiles<double, 1>(a);
+      case 4: return launch_gather_tiles<float, 2>(a);
+      case 5: return launch_gather_tiles<double, 2>(a);
+      case 6: return launch_gather_tiles<float, 3>(a);
+      case 7: return launch_gather_tiles<double, 3>(a);
+      default: return CAPOW_ERR_ARG;
+    }
+  }
   switch (a.g->order * 2 + (fp64 ? 1 : 0)) {
     case 2: launch_gather<float, 1>(a); break;
     case 3: launch_gather<double, 1>(a); break;

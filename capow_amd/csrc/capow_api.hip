@@ -111,6 +111,13 @@ struct capow_ctx {
   size_t bwd_out_cap = 0;
   void* bwd_w = nullptr;
   size_t bwd_w_cap = 0;
+  // input index of every sorted row, written by the sort while a backward
+  // recomputes the assignment (the tile gather writes gradients back there)
+  uint32_t* sorted_idx = nullptr;
+  size_t sorted_idx_cap = 0;
+  bool want_idx = false;
+  unsigned int* bwd_err = nullptr;   // tile gather: a tap outside its region
+  bool bwd_tiles = true;             // tile gather (CAPOW_GATHER_TILES=0: per particle)
   int fft_col_mul = 2;   // column batch multiplier of the y/x FFT passes (CAPOW_FFT_COL_MUL=1 halves it)
   // survey estimator: copies of the real-space fields, the monopole k-space
   // fields, bin partials and sums (allocated on first use)
@@ -627,7 +634,7 @@ static void release(capow_ctx* c) {
                 c->bin_counts, c->bin_partials2, c->bin_qscale, c->xbin_ssq, c->xbin_q,
                 c->xbin_part};
   for (void* q : ps) if (q) hipFree(q);
-  void* bs[] = {c->bwd_G, c->bwd_g0, c->bwd_out, c->bwd_w};
+  void* bs[] = {c->bwd_G, c->bwd_g0, c->bwd_out, c->bwd_w, c->sorted_idx, c->bwd_err};
   for (void* q : bs) if (q) hipFree(q);
   void* hs[] = {c->h_stats, c->h_errs, c->h_bins, c->h_sv};
   for (void* q : hs) if (q) hipHostFree(q);
@@ -649,6 +656,7 @@ extern "C" int capow_create(const capow_params* p, capow_ctx** out) {
   if (const char* e = getenv("CAPOW_FFT_PERSIST")) c->fft_persist = atoi(e) != 0;
   if (const char* e = getenv("CAPOW_DEPOSIT_HALO")) c->dep_halo = atoi(e) != 0;
   if (const char* e = getenv("CAPOW_DEPOSIT_BANK")) c->dep_bank = atoi(e) != 0;
+  if (const char* e = getenv("CAPOW_GATHER_TILES")) c->bwd_tiles = atoi(e) != 0;
   if (const char* e = getenv("CAPOW_FFT_ORDER")) c->fft_order = atoi(e) & 7;
   if (const char* e = getenv("CAPOW_BIN_MIRROR")) c->bin_mirror_env = atoi(e) != 0;
   if (const char* e = getenv("CAPOW_XBIN")) c->xbin_env = atoi(e) != 0;
@@ -805,6 +813,11 @@ static int assign_enqueue(capow_ctx* c, double* h_stats, unsigned int* h_errs, b
   if (c->wts) {
     const int64_t capw = (int64_t)(c->sorted_w_cap / c->esz);
     cap = capw < cap ? capw : cap;
+  }
+  if (c->want_idx) {
+    rc = ensure(c, (void**)&c->sorted_idx, &c->sorted_idx_cap, (size_t)(cap > 0 ? cap : 1) * 4);
+    if (rc) return rc;
+    a.sorted_idx = c->sorted_idx;
   }
   a.sorted_cap = cap;
   rc = run_scatter_deposit(a, c->fp64);
@@ -1244,8 +1257,13 @@ extern "C" int capow_backward(capow_ctx* c, const double* dL_draw, void* grad_xy
   if (where != CAPOW_HOST && where != CAPOW_DEVICE)
     return fail(c, CAPOW_ERR_ARG, "where must be CAPOW_HOST or CAPOW_DEVICE");
   if (c->state < 1) return fail(c, CAPOW_ERR_STATE, "capow_backward before capow_set_particles");
-  // the forward grids, recomputed (nothing is stored between the passes)
+  // the forward grids, recomputed (nothing is stored between the passes);
+  // the sort also records each row's input index for the tile gather
+  const bool tiles = c->bwd_tiles && c->n < ((int64_t)1 << 32) &&
+                     gather_tiles_lds(c->g, c->fp64) <= 160 * 1024;
+  c->want_idx = tiles;
   int rc = capow_assign(c, nullptr);
+  c->want_idx = false;
   if (rc) return rc;
   rc = capow_fft(c);
   if (!rc) rc = finish_x(c);   // the binning backward needs the k-space grids
@@ -1287,6 +1305,16 @@ extern "C" int capow_backward(capow_ctx* c, const double* dL_draw, void* grad_xy
   ga.n = c->n;
   for (int s = 0; s < 4; s++) ga.grid[s] = c->grid[s] ? c->grid[s] : c->grid[0];
   ga.g0 = c->bwd_g0;
+  if (tiles) {
+    if (!c->bwd_err) HIPCHK(c, hipMalloc((void**)&c->bwd_err, sizeof(unsigned int)));
+    if (zero_u32(c->bwd_err, 1, c->stream) != CAPOW_OK)
+      return fail(c, CAPOW_ERR_HIP, "clearing the gather flag failed");
+    ga.sorted = c->sorted;
+    ga.sorted_w = c->sorted_w;
+    ga.sorted_idx = c->sorted_idx;
+    ga.offsets = c->offsets;
+    ga.errs = c->bwd_err;
+  }
   const size_t ob = (size_t)(c->n > 0 ? c->n : 1) * 4 * c->esz, wb = (size_t)(c->n > 0 ? c->n : 1) * c->esz;
   if (where == CAPOW_DEVICE) {
     ga.gout = grad_xyzv;
@@ -1305,7 +1333,10 @@ extern "C" int capow_backward(capow_ctx* c, const double* dL_draw, void* grad_xy
     HIPCHK(c, hipMemcpyAsync(grad_xyzv, c->bwd_out, (size_t)c->n * 4 * c->esz, hipMemcpyDeviceToHost, c->stream));
     if (grad_w) HIPCHK(c, hipMemcpyAsync(grad_w, c->bwd_w, (size_t)c->n * c->esz, hipMemcpyDeviceToHost, c->stream));
   }
+  unsigned int gerr = 0;
+  if (tiles) HIPCHK(c, hipMemcpyAsync(&gerr, c->bwd_err, sizeof gerr, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (gerr) return fail(c, CAPOW_ERR_INTERNAL, "tile gather: a tap outside its tile's region");
   c->timer.collect();
   return CAPOW_OK;
 }

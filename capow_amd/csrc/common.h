@@ -100,6 +100,29 @@ __device__ __forceinline__ T scaled(T r, T d, T invd) {
   else return r / d;
 }
 
+// A particle row (x, y, z, v_los) as one 16- or 32-byte access.
+template <typename T>
+struct Vec4;
+template <>
+struct Vec4<float> { typedef float4 type; };
+template <>
+struct Vec4<double> { typedef double4 type; };
+
+template <typename T>
+__device__ __forceinline__ void load4(const T* p, int64_t i, T& x, T& y, T& z, T& v) {
+  typedef typename Vec4<T>::type V;
+  V q = reinterpret_cast<const V*>(p)[i];
+  x = q.x; y = q.y; z = q.z; v = q.w;
+}
+
+template <typename T>
+__device__ __forceinline__ void store4(T* p, int64_t i, T x, T y, T z, T v) {
+  typedef typename Vec4<T>::type V;
+  V q;
+  q.x = x; q.y = y; q.z = z; q.w = v;
+  reinterpret_cast<V*>(p)[i] = q;
+}
+
 // Centre (TSC) or floor (CIC, NGP) cell of one coordinate and whether the
 // reference accepts it; the one place where the cell arithmetic lives, so
 // the sort keys and the deposit taps agree bit for bit.

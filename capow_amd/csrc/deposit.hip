@@ -45,28 +45,6 @@ __device__ __forceinline__ T vpow(T v, int k) {
   }
 }
 
-template <typename T>
-struct Vec4;
-template <>
-struct Vec4<float> { typedef float4 type; };
-template <>
-struct Vec4<double> { typedef double4 type; };
-
-template <typename T>
-__device__ __forceinline__ void load4(const T* p, int64_t i, T& x, T& y, T& z, T& v) {
-  typedef typename Vec4<T>::type V;
-  V q = reinterpret_cast<const V*>(p)[i];
-  x = q.x; y = q.y; z = q.z; v = q.w;
-}
-
-template <typename T>
-__device__ __forceinline__ void store4(T* p, int64_t i, T x, T y, T z, T v) {
-  typedef typename Vec4<T>::type V;
-  V q;
-  q.x = x; q.y = y; q.z = z; q.w = v;
-  reinterpret_cast<V*>(p)[i] = q;
-}
-
 // FP32: a non-temporal store (the grid is read back only by the FFT's first
 // pass, long after the write-out; streaming it past the caches measured
 // about 3% faster for the whole deposit).  FP64: a plain store (no gain).
@@ -298,7 +276,8 @@ __global__ __launch_bounds__(BK_T) void k_bucket(const T* __restrict__ parts,
                                                  T* __restrict__ sorted,
                                                  T* __restrict__ sorted_w, int64_t cap,
                                                  double* __restrict__ partials,
-                                                 unsigned int* __restrict__ errs) {
+                                                 unsigned int* __restrict__ errs,
+                                                 uint32_t* __restrict__ sorted_idx) {
   __shared__ uint32_t hkey[BK_HS];
   __shared__ uint32_t hcnt[BK_HS];   // copies per slot; in the scatter pass, then the slot's base
   __shared__ double sh[BK_T];
@@ -339,6 +318,7 @@ __global__ __launch_bounds__(BK_T) void k_bucket(const T* __restrict__ parts,
           if ((int64_t)pos < cap) {
             store4<T>(sorted, pos, c.x, c.y, c.z, v);
             if (g.has_w) sorted_w[pos] = w;
+            if (sorted_idx) sorted_idx[pos] = (uint32_t)i;
           }
         } else {
           pr[k] = (uint32_t)slot | (rank << BK_HS_BITS);
@@ -381,6 +361,8 @@ __global__ __launch_bounds__(BK_T) void k_bucket(const T* __restrict__ parts,
         if ((int64_t)pos < cap) {
           store4<T>(sorted, pos, px[k], py[k], pz[k], pv[k]);
           if (g.has_w) sorted_w[pos] = pw[k];
+          if (sorted_idx)
+            sorted_idx[pos] = (uint32_t)(ch * BK_CHUNK + (int64_t)k * BK_T + threadIdx.x);
         }
       }
     }
@@ -1121,7 +1103,7 @@ static int run_count_t(DepositArgs& a) {
         (launch_edge(edge_kind(g), [&](auto ek) {
           k_bucket<T, ORDER, false, decltype(ek)::value><<<nb, BK_T, 0, st>>>(
               (const T*)a.parts, (const T*)a.wts, a.n, g, a.counts, nullptr, nullptr, nullptr, nullptr,
-              0, a.partials, a.errs);
+              0, a.partials, a.errs, nullptr);
         })));
   TIMED(a.timer, "stats_final",
         (k_stats_final<<<1, 256, 0, st>>>(a.partials, nb,
@@ -1148,7 +1130,7 @@ static int run_scatter_t(DepositArgs& a) {
         (launch_edge(edge_kind(g), [&](auto ek) {
           k_bucket<T, ORDER, true, decltype(ek)::value><<<nb, BK_T, 0, st>>>(
               (const T*)a.parts, (const T*)a.wts, a.n, g, nullptr, a.cursor, a.offsets, (T*)a.sorted,
-              (T*)a.sorted_w, a.sorted_cap, nullptr, nullptr);
+              (T*)a.sorted_w, a.sorted_cap, nullptr, nullptr, a.sorted_idx);
         })));
   const int ncell = g.tdim[0] * g.tdim[1] * g.tdim[2];
   const size_t nacc = (size_t)ncell + DEP_TRASH;

@@ -94,6 +94,7 @@ struct DepositArgs {
   uint32_t* boff;         // scan block offsets (+1)
   void* sorted;           // sorted_cap x 4 (one row per particle copy)
   void* sorted_w;         // sorted_cap or null
+  uint32_t* sorted_idx = nullptr;   // sorted_cap input indices of the rows, or null (not written)
   int64_t sorted_cap;     // rows the sorted buffers hold
   double* partials;       // count_blocks * STAT_N
   unsigned int* errs;     // 2 (zeroed): [0] LOS=0 out-of-box, [1] reach violations
@@ -259,11 +260,19 @@ struct GatherArgs {
   const void* parts;      // n x 4
   const void* wts;        // n or null
   int64_t n;
+  // the forward's tile sort (rows in tile order with their input indices):
+  // when set, the tile gather runs; otherwise the per-particle gather
+  const void* sorted = nullptr;
+  const void* sorted_w = nullptr;
+  const uint32_t* sorted_idx = nullptr;
+  const uint32_t* offsets = nullptr;
+  unsigned int* errs = nullptr;   // [1]: a tap outside its tile's region
   const void* grid[4];    // real-space gradient grids (A, A_int, B, B_int)
   const double* g0;       // k = 0 gradients from the binning backward
   void* gout;             // n x 4: dL/dx, dL/dy, dL/dz, dL/dv_los
   void* gw;               // n: dL/dw (or null)
 };
 int run_gather(const GatherArgs& a, bool fp64);
+size_t gather_tiles_lds(const Geom& g, bool fp64);
 
 }  // namespace capow

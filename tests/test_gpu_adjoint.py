@@ -158,3 +158,78 @@ def test_autograd_loss_gradient():
         fd = (loss_of(x, wp) - loss_of(x, wm)).item() / (2 * hw)
         assert abs(fd - gw[i].item()) <= 1e-5 * sw, (i, fd, gw[i].item())
     ps.close()
+
+
+TILE_CASES = [
+    # name, (nx, ny, nz), order, interlace, los, momentum, extra, weights, precision
+    # (register FFT lengths only: NX, NY, NZ/2 in 16, 32, 64, 96, ...)
+    ("tsc_int_los0_m1_fp64", (64, 96, 128), 3, 1, 0, 1, 3, False, api.FP64),
+    ("tsc_int_los0_m1_fp32", (64, 64, 64), 3, 1, 0, 1, 3, False, api.FP32),
+    ("cic_int_los3_m3_w_fp32", (96, 64, 128), 2, 1, 3, 3, 1, True, api.FP32),
+    ("ngp_los1_m0_w_fp64", (32, 64, 64), 1, 0, 1, 0, 0, True, api.FP64),
+    ("tsc_los2_m2_fp32", (32, 64, 192), 3, 0, 2, 2, 0, False, api.FP32),
+]
+
+
+@pytest.mark.parametrize("case", TILE_CASES, ids=[c[0] for c in TILE_CASES])
+def test_tile_gather_matches_particle_gather(case, monkeypatch):
+    """The tile gather (the forward's sort tiles, gradient regions staged in
+    LDS, rows written back at the input index) against the per-particle
+    gather (CAPOW_GATHER_TILES=0) on the same catalogue: the same sums in
+    another order, so within 1e-12 (FP64) or 2e-5 (FP32) of max|gradient|
+    per component.  Shuffled input order, particles on every edge rule and
+    outside the box (line of sight != 0: dropped by the sort, zero gradient
+    in both), signed weights, and one tile with 3000 particles (more than
+    one pass over its region)."""
+    from tests.test_gpu_parity import edge_particles
+    name, (nx, ny, nz), order, inter, los, mom, extra, weighted, prec = case
+    say("tile gather", name)
+    Lb = 1000.0
+    rng = np.random.default_rng(nx + ny + nz + order)
+    pos = rng.uniform(-0.02 * Lb, 1.02 * Lb, size=(40000, 3)) if los else rng.uniform(0, Lb, size=(40000, 3))
+    pos = np.concatenate([pos, edge_particles(nx, Lb, los, rng), edge_particles(nz, Lb, los, rng)])
+    c0 = np.array([20.3, 21.6, 40.2]) * (Lb / np.array([nx, ny, nz]))
+    pos = np.concatenate([pos, c0 + rng.uniform(0, 0.3 * Lb / nx, size=(3000, 3))])
+    if los == 0:
+        pos = np.clip(pos, 0, Lb)
+    vel = rng.normal(0, 200.0, size=pos.shape)
+    perm = rng.permutation(len(pos))
+    pos, vel = pos[perm], vel[perm]
+    w = rng.uniform(-0.5, 2.0, size=len(pos)) if weighted else None
+    kw = dict(nx=nx, ny=ny, nz=nz, xmax=Lb, ymax=Lb, zmax=Lb, interp_order=order,
+              do_interlacing=inter, los=los, momentum=mom, nk=16, maxk=0.16,
+              extra_spectra=extra, precision=prec)
+    p = api.make_params(**kw)
+    ns = len(AR.spectra(mom, extra))
+    G = np.random.default_rng(6).normal(size=(ns, 5, 16))
+    xyzv = mocks.xyzv(pos, vel, los)
+
+    def grads():
+        ps = api.PowerSpectrum(p)
+        ps.set_particles(xyzv, w)
+        gx, gw = ps.backward(G)
+        ps.close()
+        return gx, gw
+
+    tiles = grads()
+    monkeypatch.setenv("CAPOW_GATHER_TILES", "0")
+    plain = grads()
+    monkeypatch.delenv("CAPOW_GATHER_TILES")
+    tol = 1e-12 if prec == api.FP64 else 2e-5
+    for c in range(4):
+        s = np.abs(plain[0][:, c]).max()
+        assert np.isfinite(tiles[0][:, c]).all()
+        if s == 0:
+            assert np.abs(tiles[0][:, c]).max() == 0, c
+            continue
+        err = np.abs(tiles[0][:, c] - plain[0][:, c]).max() / s
+        say("  component", c, "max rel diff", err)
+        assert err <= tol, (c, err)
+    if weighted or mom:
+        s = np.abs(plain[1]).max()
+        err = np.abs(tiles[1] - plain[1]).max() / s
+        say("  weights max rel diff", err)
+        assert err <= tol
+    # particles the sort drops have zero gradients in both
+    dead = np.all(plain[0] == 0, axis=1)
+    assert np.all(tiles[0][dead] == 0)

@@ -138,7 +138,6 @@ __global__ __launch_bounds__(256) void k_gather(const T* __restrict__ parts,
                                                 T* __restrict__ gout, T* __restrict__ gw) {
   const bool hasB = g.fb >= 0;
   const double dalpha_w = g.mean_sub_a ? -(g0[0] + (g.interlace ? g0[2] : 0.0)) : 0.0;
-  const T d0 = (T)g.d[0], d1 = (T)g.d[1], d2 = (T)g.d[2];
   for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
        p += (int64_t)gridDim.x * blockDim.x) {
     const T x0 = parts[4 * p], y0 = parts[4 * p + 1], z0 = parts[4 * p + 2], v = parts[4 * p + 3];
@@ -155,12 +154,9 @@ __global__ __launch_bounds__(256) void k_gather(const T* __restrict__ parts,
     for (int ph = 0; ph < (g.interlace ? 2 : 1); ph++) {
       const T* ga = ph ? gAi : gA;
       const T* gb = ph ? gBi : gB;
-      const T x = pos[0] + (ph ? d0 / T(2) : T(0));
-      const T y = pos[1] + (ph ? d1 / T(2) : T(0));
-      const T z = pos[2] + (ph ? d2 / T(2) : T(0));
-      const AxisOut<T> ax = axis_taps<T, ORDER>(x, (T)g.lo[0], (T)g.hi[0], d0, (T)g.L[0], g.n[0], true, g.los, g.edge, (T)g.invd[0]);
-      const AxisOut<T> ay = axis_taps<T, ORDER>(y, (T)g.lo[1], (T)g.hi[1], d1, (T)g.L[1], g.n[1], false, g.los, g.edge, (T)g.invd[1]);
-      const AxisOut<T> az = axis_taps<T, ORDER>(z, (T)g.lo[2], (T)g.hi[2], d2, (T)g.L[2], g.n[2], false, g.los, g.edge, (T)g.invd[2]);
+      const AxisOut<T> ax = axis_taps<T, ORDER>(pos[0], ph ? g.d[0] / 2.0 : 0.0, g, 0, g.los, g.edge);
+      const AxisOut<T> ay = axis_taps<T, ORDER>(pos[1], ph ? g.d[1] / 2.0 : 0.0, g, 1, g.los, g.edge);
+      const AxisOut<T> az = axis_taps<T, ORDER>(pos[2], ph ? g.d[2] / 2.0 : 0.0, g, 2, g.los, g.edge);
       if (!(ax.ok && ay.ok && az.ok)) continue;
       if (ph == 0) main_ok = true;
       for (int tx = 0; tx < ORDER; tx++) {
@@ -222,7 +218,7 @@ __global__ __launch_bounds__(256) void k_gather(const T* __restrict__ parts,
 // weights and their derivatives first (separable weights), so per row
 //   dL/dA += wx wy S_a,  dL/dx += dwx wy F,  dL/dy += wx dwy F,
 //   dL/dz += wx wy F'    (S_a = sum_z wz g_A, F = fa S_a + fb S_b, F' with dwz)
-// in the working precision, with the rows summed in FP64.  Particles the
+// all in FP64 (a derivative is a difference of neighbouring cells' values).  Particles the
 // sort dropped (outside every deposit) keep the zero gradient written
 // before.  Rows are written back at their input index.
 #define GT_T 1024  // threads per tile workgroup (16 waves: at most 128 VGPRs)
@@ -251,7 +247,6 @@ __global__ __launch_bounds__(GT_T) void k_gather_tiles(
   if (own == 0) return;   // the whole workgroup: no barrier reached yet
   const bool per = g.periodic != 0;
   const double dalpha_w = g.mean_sub_a ? -(g0[0] + (g.interlace ? g0[2] : 0.0)) : 0.0;
-  const T d0 = (T)g.d[0], d1 = (T)g.d[1], d2 = (T)g.d[2];
   // region index of a tile-relative first tap (the nearest periodic image)
   auto loc = [&](int first, int t0, int T_, int n) {
     int o = first - t0;
@@ -323,16 +318,10 @@ __global__ __launch_bounds__(GT_T) void k_gather_tiles(
       for (int k = 0; k < GT_K; k++) {
         const uint32_t f = base + (uint32_t)(k * GT_T) + threadIdx.x;
         if (f >= own) continue;
-        T x = px[k], y = py[k], z = pz[k];
         const T v = pv[k], w = pw[k];
-        if (ph) {
-          x += d0 / T(2);
-          y += d1 / T(2);
-          z += d2 / T(2);
-        }
-        const AxisOut<T> ax = axis_taps<T, ORDER>(x, (T)g.lo[0], (T)g.hi[0], d0, (T)g.L[0], g.n[0], true, g.los, g.edge, (T)g.invd[0]);
-        const AxisOut<T> ay = axis_taps<T, ORDER>(y, (T)g.lo[1], (T)g.hi[1], d1, (T)g.L[1], g.n[1], false, g.los, g.edge, (T)g.invd[1]);
-        const AxisOut<T> az = axis_taps<T, ORDER>(z, (T)g.lo[2], (T)g.hi[2], d2, (T)g.L[2], g.n[2], false, g.los, g.edge, (T)g.invd[2]);
+        const AxisOut<T> ax = axis_taps<T, ORDER>(px[k], ph ? g.d[0] / 2.0 : 0.0, g, 0, g.los, g.edge);
+        const AxisOut<T> ay = axis_taps<T, ORDER>(py[k], ph ? g.d[1] / 2.0 : 0.0, g, 1, g.los, g.edge);
+        const AxisOut<T> az = axis_taps<T, ORDER>(pz[k], ph ? g.d[2] / 2.0 : 0.0, g, 2, g.los, g.edge);
         if (!(ax.ok && ay.ok && az.ok)) continue;
         if (ph == 0) mok[k] = true;
         const int lx0 = loc(ax.first, t0x, TX, g.n[0]);
@@ -357,15 +346,23 @@ __global__ __launch_bounds__(GT_T) void k_gather_tiles(
           atomicOr(&errs[0], 1u);
           continue;
         }
-        const T fa = w * vpow_b<T>(v, g.fa);
-        const T fb = HASB ? w * vpow_b<T>(v, g.fb) : T(0);
+        const double fa = (double)(w * vpow_b<T>(v, g.fa));
+        const double fb = HASB ? (double)(w * vpow_b<T>(v, g.fb)) : 0.0;
+        // weights and the sums in FP64: a derivative is a difference of
+        // neighbouring cells' values, which cancels on fine grids
+        double wzd[ORDER], dwzd[ORDER];
+#pragma unroll
+        for (int t = 0; t < ORDER; t++) {
+          wzd[t] = (double)az.w[t];
+          dwzd[t] = (double)az.dw[t];
+        }
 #pragma unroll
         for (int tx = 0; tx < ORDER; tx++) {
 #pragma unroll
           for (int ty = 0; ty < ORDER; ty++) {
             if (!(vx[tx] && vy[ty])) continue;
             const int rb = ((lx0 + tx) * RY + (ly0 + ty)) * RZ + lz0;
-            T sa = T(0), sb = T(0), da = T(0), db = T(0);
+            double sa = 0.0, sb = 0.0, da = 0.0, db = 0.0;
 #pragma unroll
             for (int tz = 0; tz < ORDER; tz++) {
               T a = T(0), b = T(0);
@@ -378,20 +375,21 @@ __global__ __launch_bounds__(GT_T) void k_gather_tiles(
                   a = regA[rb + tz];
                 }
               }
-              sa += az.w[tz] * a;
-              da += az.dw[tz] * a;
+              sa += wzd[tz] * (double)a;
+              da += dwzd[tz] * (double)a;
               if (HASB) {
-                sb += az.w[tz] * b;
-                db += az.dw[tz] * b;
+                sb += wzd[tz] * (double)b;
+                db += dwzd[tz] * (double)b;
               }
             }
-            const T wxy = ax.w[tx] * ay.w[ty];
-            const double fw = (double)(fa * sa + fb * sb), fd = (double)(fa * da + fb * db);
-            dfa[k] += (double)(wxy * sa);
-            if (HASB) dfb[k] += (double)(wxy * sb);
-            ds0[k] += (double)(ax.dw[tx] * ay.w[ty]) * fw;
-            ds1[k] += (double)(ax.w[tx] * ay.dw[ty]) * fw;
-            ds2[k] += (double)wxy * fd;
+            const double wx = (double)ax.w[tx], wy = (double)ay.w[ty];
+            const double wxy = wx * wy;
+            const double fw = fa * sa + fb * sb, fd = fa * da + fb * db;
+            dfa[k] += wxy * sa;
+            if (HASB) dfb[k] += wxy * sb;
+            ds0[k] += (double)ax.dw[tx] * wy * fw;
+            ds1[k] += wx * (double)ay.dw[ty] * fw;
+            ds2[k] += wxy * fd;
           }
         }
       }

@@ -720,7 +720,9 @@ extern "C" int capow_set_particles(capow_ctx* c, const void* xyzv, const void* w
   if (n < 0 || (n > 0 && !xyzv)) return fail(c, CAPOW_ERR_ARG, "bad particle array");
   if ((dtype == CAPOW_DTYPE_F64) != c->fp64)
     return fail(c, CAPOW_ERR_ARG, "particle dtype must match the context precision");
-  if (n > (1LL << 29)) return fail(c, CAPOW_ERR_UNSUPPORTED, "more than 2^29 particles (32-bit copy offsets)");
+  // one sorted row per particle, addressed by 32-bit bucket offsets
+  if (n > (int64_t)UINT32_MAX)
+    return fail(c, CAPOW_ERR_UNSUPPORTED, "more than 2^32 - 1 particles (32-bit row offsets)");
   const size_t pb = (size_t)n * 4 * c->esz, wb = (size_t)n * c->esz;
   if (where == CAPOW_HOST) {
     int rc = ensure(c, &c->own_parts, &c->own_parts_cap, pb > 0 ? pb : 16);
@@ -1405,8 +1407,8 @@ extern "C" int capow_batch_create(const capow_params* configs, int32_t nconf,
                                   int64_t max_particles, int32_t nstreams, capow_batch** out) {
   if (!configs || !out || nconf < 1 || nstreams < 1 || max_particles < 0)
     return bfail(nullptr, CAPOW_ERR_ARG, "bad batch arguments");
-  if (max_particles > (1LL << 29))
-    return bfail(nullptr, CAPOW_ERR_UNSUPPORTED, "more than 2^29 particles per catalogue");
+  if (max_particles > (int64_t)UINT32_MAX - 4096)
+    return bfail(nullptr, CAPOW_ERR_UNSUPPORTED, "more than 2^32 - 4097 particles per catalogue");
   for (int q = 0; q < nconf; q++)
     if (configs[q].edge_mode == CAPOW_EDGE_SURVEY)
       return bfail(nullptr, CAPOW_ERR_UNSUPPORTED, "batches cover the periodic estimator only");

@@ -90,15 +90,6 @@ struct AxisOut {
 template <typename T>
 __device__ __forceinline__ int fl2i(T v) { return (int)v; }  // trunc toward 0
 
-// Scaled coordinate (u - lo)/d.  FP64 divides exactly as the reference;
-// the FP32 form multiplies by the reciprocal (cell membership within an ulp
-// of a cell edge is part of the FP32 form's error).
-template <typename T>
-__device__ __forceinline__ T scaled(T r, T d, T invd) {
-#pragma clang fp contract(off)
-  if constexpr (sizeof(T) == 4) return r * invd;
-  else return r / d;
-}
 
 // A particle row (x, y, z, v_los) as one 16- or 32-byte access.
 template <typename T>
@@ -125,20 +116,35 @@ __device__ __forceinline__ void store4(T* p, int64_t i, T x, T y, T z, T v) {
 
 // Centre (TSC) or floor (CIC, NGP) cell of one coordinate and whether the
 // reference accepts it; the one place where the cell arithmetic lives, so
-// the sort keys and the deposit taps agree bit for bit.
+// the sort keys and the deposit taps agree bit for bit.  The arithmetic runs
+// in the working precision T: the box geometry (FP64 in Geom) is rounded to
+// T first, and the interlacing shift sh (0 or d/2) is added to u in T.
 template <typename T>
 struct AxisCell {
-  T s;      // scaled coordinate (u - MIN)/d after the TSC periodic x shift
+  T s;      // scaled coordinate (u + sh - MIN)/d after the TSC periodic x shift
   int c;    // unwrapped cell: TSC (int)(s + 0.5), CIC/NGP (int)s
   bool ok;
 };
 
+// Scaled coordinate (u - lo)/d.  FP64 divides exactly as the reference;
+// the FP32 form multiplies by the reciprocal (cell membership within an ulp
+// of a cell edge is part of the FP32 form's error).
+template <typename T>
+__device__ __forceinline__ T scaled(T r, T d, T invd) {
+#pragma clang fp contract(off)
+  if constexpr (sizeof(T) == 4) return r * invd;
+  else return r / d;
+}
+
 template <typename T, int ORDER>
-__device__ __forceinline__ AxisCell<T> axis_cell(T u, T lo, T hi, T d, T L, int n, bool is_x,
-                                                 int los, int edge, T invd) {
+__device__ __forceinline__ AxisCell<T> axis_cell(T u_in, double sh_d, double lo_d, double hi_d,
+                                                 double d_d, double L_d, int n, bool is_x,
+                                                 int los, int edge, double invd_d) {
   // No multiply-add contraction: the sort, the deposit and the gradient
   // gather inline this into different code, and must round alike.
 #pragma clang fp contract(off)
+  const T lo = (T)lo_d, hi = (T)hi_d, d = (T)d_d, L = (T)L_d, invd = (T)invd_d;
+  T u = u_in + (T)sh_d;
   AxisCell<T> o;
   if (edge == CAPOW_EDGE_PERIODIC) {
     // wrap into [lo, hi)
@@ -175,11 +181,14 @@ __device__ __forceinline__ AxisCell<T> axis_cell(T u, T lo, T hi, T d, T L, int 
   return o;
 }
 
+// Taps of one coordinate (with the interlacing shift sh) along axis a of g.
 template <typename T, int ORDER>
-__device__ __forceinline__ AxisOut<T> axis_taps(T u, T lo, T hi, T d, T L, int n,
-                                                bool is_x, int los, int edge, T invd) {
+__device__ __forceinline__ AxisOut<T> axis_taps(T u, double sh, const Geom& g, int a, int los,
+                                                int edge) {
 #pragma clang fp contract(off)
-  const AxisCell<T> ac = axis_cell<T, ORDER>(u, lo, hi, d, L, n, is_x, los, edge, invd);
+  const int n = g.n[a];
+  const AxisCell<T> ac = axis_cell<T, ORDER>(u, sh, g.lo[a], g.hi[a], g.d[a], g.L[a], n, a == 0,
+                                             los, edge, g.invd[a]);
   AxisOut<T> o;
   o.ok = ac.ok;
   o.w[0] = o.w[1] = o.w[2] = T(0);

@@ -122,14 +122,14 @@ __device__ __forceinline__ PClass<T, ORDER> classify(T x, T y, T z, T v, const G
   int cm[3], ci[3];
 #pragma unroll
   for (int a = 0; a < 3; a++) {
-    const T lo = (T)g.lo[a], hi = (T)g.hi[a], d = (T)g.d[a], L = (T)g.L[a], invd = (T)g.invd[a];
-    const AxisCell<T> m = axis_cell<T, ORDER>(u[a], lo, hi, d, L, g.n[a], a == 0, los, edge, invd);
+    const AxisCell<T> m = axis_cell<T, ORDER>(u[a], 0.0, g.lo[a], g.hi[a], g.d[a], g.L[a], g.n[a],
+                                              a == 0, los, edge, g.invd[a]);
     okm = okm && m.ok;
     cm[a] = m.c;
     ci[a] = m.c;
     if (g.interlace) {
-      const AxisCell<T> i = axis_cell<T, ORDER>(u[a] + d / T(2), lo, hi, d, L, g.n[a], a == 0, los,
-                                                edge, invd);
+      const AxisCell<T> i = axis_cell<T, ORDER>(u[a], g.d[a] / 2.0, g.lo[a], g.hi[a], g.d[a], g.L[a],
+                                                g.n[a], a == 0, los, edge, g.invd[a]);
       oki = oki && i.ok;
       ci[a] = i.c;
     }
@@ -267,7 +267,9 @@ __device__ __forceinline__ int hs_add(uint32_t* __restrict__ hkey, uint32_t* __r
   return slot;
 }
 
-template <typename T, int ORDER, bool SCATTER, int EK>
+// IDX (scatter only): also write each sorted row's input index (the
+// backward's tile gather needs it; the forward does not pay for it).
+template <typename T, int ORDER, bool SCATTER, int EK, bool IDX = false>
 __global__ __launch_bounds__(BK_T) void k_bucket(const T* __restrict__ parts,
                                                  const T* __restrict__ wts, int64_t n,
                                                  Geom g, uint32_t* __restrict__ counts,
@@ -318,7 +320,7 @@ __global__ __launch_bounds__(BK_T) void k_bucket(const T* __restrict__ parts,
           if ((int64_t)pos < cap) {
             store4<T>(sorted, pos, c.x, c.y, c.z, v);
             if (g.has_w) sorted_w[pos] = w;
-            if (sorted_idx) sorted_idx[pos] = (uint32_t)i;
+            if (IDX) sorted_idx[pos] = (uint32_t)i;
           }
         } else {
           pr[k] = (uint32_t)slot | (rank << BK_HS_BITS);
@@ -361,8 +363,7 @@ __global__ __launch_bounds__(BK_T) void k_bucket(const T* __restrict__ parts,
         if ((int64_t)pos < cap) {
           store4<T>(sorted, pos, px[k], py[k], pz[k], pv[k]);
           if (g.has_w) sorted_w[pos] = pw[k];
-          if (sorted_idx)
-            sorted_idx[pos] = (uint32_t)(ch * BK_CHUNK + (int64_t)k * BK_T + threadIdx.x);
+          if (IDX) sorted_idx[pos] = (uint32_t)(ch * BK_CHUNK + (int64_t)k * BK_T + threadIdx.x);
         }
       }
     }
@@ -705,7 +706,7 @@ __global__ __launch_bounds__(DEP_T) void k_deposit(const T* __restrict__ sorted,
     return r_start[lo] + (f - r_pre[lo]);
   };
 
-  const T d0 = (T)g.d[0], d1 = (T)g.d[1], d2 = (T)g.d[2];
+  const T d1 = (T)g.d[1], d2 = (T)g.d[2];   // bank keys (the taps shift in FP64)
   const double alpha = stats[STAT_ALPHA];
   const bool single = total <= (uint32_t)(DEP_T * DEP_K);
 
@@ -882,17 +883,13 @@ __global__ __launch_bounds__(DEP_T) void k_deposit(const T* __restrict__ sorted,
 #pragma unroll
       for (int k = 0; k < DEP_K; k++) {
         if (!ph_has[k]) continue;
-        T x = px[k], y = py[k], z = pz[k];
-        if (ph) {
-          x += d0 / T(2);
-          y += d1 / T(2);
-          z += d2 / T(2);
-        }
         const int edge = ek_edge<EK>(g), los = ek_los<EK>(g);
         const bool per = ek_periodic<EK>(g);
-        const AxisOut<T> ax = axis_taps<T, ORDER>(x, (T)g.lo[0], (T)g.hi[0], d0, (T)g.L[0], g.n[0], true, los, edge, (T)g.invd[0]);
-        const AxisOut<T> ay = axis_taps<T, ORDER>(y, (T)g.lo[1], (T)g.hi[1], d1, (T)g.L[1], g.n[1], false, los, edge, (T)g.invd[1]);
-        const AxisOut<T> az = axis_taps<T, ORDER>(z, (T)g.lo[2], (T)g.hi[2], d2, (T)g.L[2], g.n[2], false, los, edge, (T)g.invd[2]);
+        // the interlaced phase's half-cell shift goes into the FP64 cell
+        // arithmetic
+        const AxisOut<T> ax = axis_taps<T, ORDER>(px[k], ph ? g.d[0] / 2.0 : 0.0, g, 0, los, edge);
+        const AxisOut<T> ay = axis_taps<T, ORDER>(py[k], ph ? g.d[1] / 2.0 : 0.0, g, 1, los, edge);
+        const AxisOut<T> az = axis_taps<T, ORDER>(pz[k], ph ? g.d[2] / 2.0 : 0.0, g, 2, los, edge);
         const bool ok = ax.ok && ay.ok && az.ok;
         const T va = ok ? pa[k] : T(0), vb = ok ? pb[k] : T(0);
         if constexpr (HALO) {
@@ -1128,9 +1125,14 @@ static int run_scatter_t(DepositArgs& a) {
   hipStream_t st = a.stream;
   TIMED(a.timer, "bucket_scatter",
         (launch_edge(edge_kind(g), [&](auto ek) {
-          k_bucket<T, ORDER, true, decltype(ek)::value><<<nb, BK_T, 0, st>>>(
-              (const T*)a.parts, (const T*)a.wts, a.n, g, nullptr, a.cursor, a.offsets, (T*)a.sorted,
-              (T*)a.sorted_w, a.sorted_cap, nullptr, nullptr, a.sorted_idx);
+          if (a.sorted_idx)
+            k_bucket<T, ORDER, true, decltype(ek)::value, true><<<nb, BK_T, 0, st>>>(
+                (const T*)a.parts, (const T*)a.wts, a.n, g, nullptr, a.cursor, a.offsets,
+                (T*)a.sorted, (T*)a.sorted_w, a.sorted_cap, nullptr, nullptr, a.sorted_idx);
+          else
+            k_bucket<T, ORDER, true, decltype(ek)::value, false><<<nb, BK_T, 0, st>>>(
+                (const T*)a.parts, (const T*)a.wts, a.n, g, nullptr, a.cursor, a.offsets,
+                (T*)a.sorted, (T*)a.sorted_w, a.sorted_cap, nullptr, nullptr, nullptr);
         })));
   const int ncell = g.tdim[0] * g.tdim[1] * g.tdim[2];
   const size_t nacc = (size_t)ncell + DEP_TRASH;

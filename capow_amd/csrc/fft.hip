@@ -792,9 +792,14 @@ __global__ __launch_bounds__((N1 * N2 / clcm(N1, N2)) * B) void k_fft_rows_t(
   }
 }
 
-// Sizes served by the register kernels: N -> (N1, N2).
-static bool reg_split(int N, int* n1, int* n2) {
+// Sizes served by the register kernels: N -> (N1, N2).  2048 (64 x 32, 64
+// complex values a thread) in FP32 only: FP64 would hold 256 VGPRs of data
+// alone, so its 2048-point passes take the Stockham kernels.
+static bool reg_split(int N, int* n1, int* n2, bool fp64 = false) {
   switch (N) {
+    case 2048:
+      if (fp64) return false;
+      *n1 = 64; *n2 = 32; return true;
     case 16: *n1 = 4; *n2 = 4; return true;
     case 32: *n1 = 4; *n2 = 8; return true;
     case 64: *n1 = 8; *n2 = 8; return true;
@@ -889,14 +894,18 @@ static void launch_rows2(FftArgs& a, T** gp, int64_t nrows, const void* twz, int
 }
 
 #define CAPOW_FFT_SIZES(X) X(4, 4) X(4, 8) X(8, 8) X(8, 12) X(8, 16) X(8, 24) X(16, 16) \
-  X(16, 24) X(16, 32) X(32, 32)
+  X(16, 24) X(16, 32) X(32, 32) X(64, 32)
+// split (A, B_) instantiated for precision T (64 x 32: FP32 only)
+template <typename T, int A, int B_>
+constexpr bool split_for() { return sizeof(T) == 4 || A * B_ != 2048; }
 
 template <typename T, int AX>
 static bool try_cols2(FftArgs& a, const char* name, T** gp, int N, int64_t astride,
                       int64_t pstride, int64_t ncols, int64_t nplanes, const void* tw) {
   int n1, n2;
-  if (!reg_split(N, &n1, &n2)) return false;
+  if (!reg_split(N, &n1, &n2, sizeof(T) == 8)) return false;
 #define X(A, B_)                                                                     \
+  if constexpr (split_for<T, A, B_>())                                                \
   if (n1 == A && n2 == B_) {                                                          \
     if (a.col_mul == 2 && (size_t)A * B_ * cols_B<T, A, B_>() * 2 * sizeof(cx<T>) <= 160 * 1024) \
       launch_cols2<T, A, B_, 2, AX>(a, name, gp, astride, pstride, ncols, nplanes, tw); \
@@ -912,8 +921,9 @@ static bool try_cols2(FftArgs& a, const char* name, T** gp, int N, int64_t astri
 template <typename T>
 static bool try_rows2(FftArgs& a, T** gp, int M, int64_t nrows, const void* twz, int64_t rowlen) {
   int n1, n2;
-  if (!reg_split(M, &n1, &n2)) return false;
+  if (!reg_split(M, &n1, &n2, sizeof(T) == 8)) return false;
 #define X(A, B_)                                   \
+  if constexpr (split_for<T, A, B_>())              \
   if (n1 == A && n2 == B_) {                        \
     launch_rows2<T, A, B_>(a, gp, nrows, twz, rowlen); \
     return true;                                    \
@@ -959,8 +969,9 @@ static bool try_rows_t(FftArgs& a, const char* name, int N, T* const* in, T* con
                        int64_t out_so, int64_t out_sk, const void* tw, int mul,
                        double* ssq = nullptr) {
   int n1, n2;
-  if (!reg_split(N, &n1, &n2)) return false;
+  if (!reg_split(N, &n1, &n2, sizeof(T) == 8)) return false;
 #define X(A, B_)                                                                               \
+  if constexpr (split_for<T, A, B_>())                                                          \
   if (n1 == A && n2 == B_) {                                                                    \
     if (mul == 2 && (size_t)(sizeof(T) == 4 ? 16 : 8) * sizeof(cx<T>) * (A * B_ + A) +           \
                         (R2C ? (size_t)(A * B_ / 2 + 1) * sizeof(cx<T>) : 0) <= 160 * 1024)      \
@@ -1361,7 +1372,8 @@ static bool run_fft_transposed(FftArgs& a) {
   const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
   const int64_t nzh = nz / 2 + 1, rowc = g.rowc, px = a.px, py = a.py;
   int n1, n2;
-  if (!reg_split(nz / 2, &n1, &n2) || !reg_split(ny, &n1, &n2) || !reg_split(nx, &n1, &n2))
+  constexpr bool F64 = sizeof(T) == 8;
+  if (!reg_split(nz / 2, &n1, &n2, F64) || !reg_split(ny, &n1, &n2, F64) || !reg_split(nx, &n1, &n2, F64))
     return false;
   T* G[4];
   T* S[4];
@@ -1489,7 +1501,8 @@ static int run_ifft_t(FftArgs& a) {
   const int nzh = nz / 2 + 1;
   const int64_t rowc = g.rowc, nrows = (int64_t)nx * ny;
   int n1, n2;
-  if (!reg_split(nx, &n1, &n2) || !reg_split(ny, &n1, &n2) || !reg_split(nz / 2, &n1, &n2))
+  constexpr bool F64 = sizeof(T) == 8;
+  if (!reg_split(nx, &n1, &n2, F64) || !reg_split(ny, &n1, &n2, F64) || !reg_split(nz / 2, &n1, &n2, F64))
     return CAPOW_ERR_UNSUPPORTED;
   const int64_t ncols = (int64_t)ny * rowc;
   if (!try_cols2<T, 2>(a, "ifft_x", gp, nx, ncols, 0, ncols, 1, a.tw[0])) return CAPOW_ERR_UNSUPPORTED;
@@ -1498,6 +1511,7 @@ static int run_ifft_t(FftArgs& a) {
   reg_split(nz / 2, &n1, &n2);
   bool done = false;
 #define X(A, B_)                                                   \
+  if constexpr (split_for<T, A, B_>())                             \
   if (!done && n1 == A && n2 == B_) {                              \
     launch_rows2_adj<T, A, B_>(a, gp, nrows, a.tw[2], g.rowlen);   \
     done = true;                                                   \

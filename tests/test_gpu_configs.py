@@ -384,10 +384,34 @@ def test_c4_reference_gradient_64(oracle):
 @pytest.mark.parametrize("prec", [api.FP32, api.FP64])
 @pytest.mark.parametrize("kind", ["cic_int_reference_edges", "tsc_int_periodic_edges"])
 def test_lattice_known_answer_full_size(n, prec, kind):
+    _lattice_known_answer(n, prec, kind)
+
+
+@pytest.mark.timeout(300)
+@pytest.mark.parametrize("kind", ["cic_int_reference_edges", "tsc_int_periodic_edges"])
+def test_lattice_known_answer_2048(kind):
+    """The largest grid: 2048^3 with a 1024^3 lattice (1,073,741,824
+    particles), FP32 form (its 2048-point x and y passes are the 64 x 32
+    register kernels; FP64 takes the Stockham kernels at 2048 and two FP64
+    grids alone would hold 138 GB).
+
+    The box (L = 4096, so N/L = 1/2) and the lattice offsets (1/4, 1/8, 3/8
+    of a cell) make every position and every FP32 scaled coordinate exact,
+    so the FP32 form must give the known answer to rounding.  With
+    L = 4000 (N/L not a float) the FP32 form's reciprocal stretches the
+    scaled coordinates by up to 2^-24 N cells across the box, and the FFT's
+    rounding on the perfectly periodic lattice adds up coherently over its
+    million identical rows: P_4 of the fundamental bin then carries 1.5e-4
+    V/N (2.7e-6 at 1024^3); the FP64 form is exact either way."""
+    _lattice_known_answer(2048, api.FP32, kind, L=4096.0, dk=0.0025, offs=(0.25, 0.125, 0.375),
+                          tol=1e-6)
+
+
+def _lattice_known_answer(n, prec, kind, L=None, dk=None, offs=(0.3, 0.1, 0.45), tol=None):
     import torch
     say("lattice known answer", n, "fp64" if prec == api.FP64 else "fp32", kind)
-    L = 2000.0 * n / 1024
-    dk = 0.005 * 1024 / n
+    L = 2000.0 * n / 1024 if L is None else L
+    dk = 0.005 * 1024 / n if dk is None else dk
     nk = int(np.floor(np.pi * n / L / dk))
     order, edge = (2, api.EDGE_REFERENCE) if kind.startswith("cic") else (3, api.EDGE_PERIODIC)
     p = api.make_params(nx=n, ny=n, nz=n, xmax=L, ymax=L, zmax=L, interp_order=order,
@@ -397,10 +421,15 @@ def test_lattice_known_answer_full_size(n, prec, kind):
     dx = L / n
     m = n // 2
     q = torch.arange(m, device="cuda", dtype=torch.float64) * 2 * dx
-    X, Y, Z = torch.meshgrid(q + 0.3 * dx, q + 0.1 * dx, q + 0.45 * dx, indexing="ij")
-    xyzv = torch.stack([X.reshape(-1), Y.reshape(-1), Z.reshape(-1),
-                        torch.zeros(m ** 3, device="cuda", dtype=torch.float64)], dim=1).to(dt)
-    del X, Y, Z
+    # rows written column by column in the working precision (a 2048^3 grid
+    # takes 1024^3 particles: no float64 copies of the whole catalogue)
+    xyzv = torch.zeros((m ** 3, 4), device="cuda", dtype=dt)
+    for a, off in enumerate(offs):
+        shape = [1, 1, 1]
+        shape[a] = m
+        col = (q + off * dx).to(dt).reshape(shape).expand(m, m, m)
+        xyzv[:, a] = col.reshape(-1)
+        del col
     ps = api.PowerSpectrum(p)
     ps.set_particles(xyzv.contiguous())
     res = ps.power()
@@ -413,7 +442,8 @@ def test_lattice_known_answer_full_size(n, prec, kind):
     assert shot / norm == pytest.approx(vn, rel=1e-14)
     sel = np.asarray(nm) > 0
     assert sel.sum() >= nk - 2
-    tol = 1e-10 if prec == api.FP64 else 1e-4
+    if tol is None:
+        tol = 1e-10 if prec == api.FP64 else 1e-4
     d0 = np.abs(pk[0][sel] + vn).max() / vn
     d24 = max(np.abs(pk[2][sel]).max(), np.abs(pk[4][sel]).max()) / vn
     say("  max |P0 + V/N| / (V/N) %.2e, max |P2|,|P4| / (V/N) %.2e" % (d0, d24))

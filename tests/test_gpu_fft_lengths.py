@@ -2,8 +2,10 @@
 src/main.c:266-271, plans src/main.c:1200-1217).
 
 The register kernels serve NX, NY and NZ/2 in {16, 32, 64, 96, 128, 192,
-256, 384, 512, 1024}, each as a two-level split N1 x N2 (4x4 ... 32x32; the
-24- and 32-point in-register DFTs appear in 8x24, 16x24, 16x32, 32x32).
+256, 384, 512, 1024} and, in FP32, 2048, each as a two-level split N1 x N2
+(4x4 ... 32x32, 64x32; the 24-, 32- and 64-point in-register DFTs appear in
+8x24, 16x24, 16x32, 32x32, 64x32).  FP64 takes the Stockham kernels at 2048
+(and the backward, which needs register sizes, refuses it).
 Thin grids put every length on every axis -- x (column pass), y (column
 pass) and z (r2c row pass) -- so each kernel template the 256^3-2048^3
 workloads run is compared with the oracle's FFT at least once, through
@@ -29,7 +31,7 @@ from oracle import adjoint_ref as AR
 
 pytestmark = pytest.mark.gpu
 
-REG = (16, 32, 64, 96, 128, 192, 256, 384, 512, 1024)
+REG = (16, 32, 64, 96, 128, 192, 256, 384, 512, 1024, 2048)
 
 # (NX, NY, NZ): x, y and z = NZ/2 lengths
 SHAPES = [
@@ -44,6 +46,9 @@ SHAPES = [
     (64, 32, 1024),    # x 64, y 32, z 512
     (128, 64, 512),    # x 128, y 64, z 256
     (16, 128, 256),    # y 128 (8x16), z 128
+    (2048, 16, 32),    # x 2048 (64x32), y 16, z 16
+    (16, 2048, 64),    # y 2048, z 32
+    (32, 16, 4096),    # z 2048 (64x32 r2c)
 ]
 
 
@@ -93,6 +98,8 @@ def test_inverse_fft_length_through_backward(shape, prec, oracle):
     """capow_backward = binning backward, inverse FFT (x, y column adjoints
     and the z-row adjoint at these lengths), transposed deposit."""
     nx, ny, nz = shape
+    if prec == api.FP64 and max(nx, ny, nz // 2) > 1024:
+        pytest.skip("FP64 register kernels stop at 1024; the backward needs register sizes")
     say("ifft length case", shape, "fp64" if prec == api.FP64 else "fp32")
     L = 1000.0
     rng = np.random.default_rng(nx * 5 + ny)
@@ -111,12 +118,26 @@ def test_inverse_fft_length_through_backward(shape, prec, oracle):
     gx, gw = ps.backward(G)
     ps.close()
     tol = 1e-9 if prec == api.FP64 else 2e-3
+    keep = np.ones(n, dtype=bool)
+    if prec == api.FP32:
+        # The FP32 form scales float32 coordinates in FP32: s = (x - MIN)/dx
+        # carries an ulp of s (2.4e-4 cells at s ~ 2048), so a particle
+        # that close to a cell edge (s, or s + 1/2 when interlaced, an
+        # integer) can take the neighbouring cell, where the CIC weights'
+        # derivative jumps; the FP64 reference, on the same float32
+        # positions, does not.  Those particles are left out.
+        for a, na in enumerate((nx, ny, nz)):
+            sa = pos[:, a] / (L / na)
+            for sh in (0.0, 0.5):
+                fr = (sa + sh) % 1.0
+                keep &= np.minimum(fr, 1.0 - fr) > 1e-3
+        say("  particles within 1e-3 cells of an edge, left out:", int((~keep).sum()))
     for c in range(4):
-        s = np.abs(ref_x[:, c]).max()
+        s = np.abs(ref_x[keep, c]).max()
         assert s > 0, c
-        err = np.abs(gx[:, c] - ref_x[:, c]).max() / s
+        err = np.abs(gx[keep, c] - ref_x[keep, c]).max() / s
         say("  component", c, "max rel err", err)
         assert err <= tol, (c, err)
-    err = np.abs(gw - ref_w).max() / np.abs(ref_w).max()
+    err = np.abs(gw[keep] - ref_w[keep]).max() / np.abs(ref_w[keep]).max()
     say("  weights max rel err", err)
     assert err <= tol

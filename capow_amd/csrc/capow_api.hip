@@ -33,6 +33,10 @@ struct capow_ctx {
   // the transform); null when the in-place passes are used
   void* scratch[4] = {nullptr, nullptr, nullptr, nullptr};
   int64_t fft_px = 0, fft_py = 0;   // row pitches of the intermediate layouts
+  // one spare buffer (scratch[0]) instead of one per grid: the grids are too
+  // large for a second set (e.g. four FP32 grids at 2048^3); the FFT passes
+  // then run grid by grid and rotate the buffers
+  bool fft_one_spare = false;
   int fft_persist = 1;              // persistent row passes (CAPOW_FFT_PERSIST=0: off)
   bool dep_halo = true;             // halo-row deposit (CAPOW_DEPOSIT_HALO=0: off)
   bool dep_bank = true;             // its bank-balanced copy order (CAPOW_DEPOSIT_BANK=0: off)
@@ -513,6 +517,15 @@ static int setup(capow_ctx* c) {
   }
   bool want_scratch = true;
   if (const char* e = getenv("CAPOW_FFT_TRANSPOSED")) want_scratch = atoi(e) != 0;
+  // CAPOW_FFT_ONE_SPARE=1: the one-spare pipeline even when a full second
+  // set of grids would fit (for tests at small sizes)
+  bool force_one = false;
+  if (const char* e = getenv("CAPOW_FFT_ONE_SPARE")) force_one = atoi(e) != 0;
+  if (force_one && want_scratch && ngr > 1) {
+    HIPCHK(c, hipMalloc(&c->scratch[0], gbytes));
+    c->fft_one_spare = true;
+    want_scratch = false;
+  }
   if (want_scratch) {
     // only with room to spare: the second set of grids plus 16 GB
     size_t fre = 0, tot = 0;
@@ -526,6 +539,11 @@ static int setup(capow_ctx* c) {
           break;
         }
       }
+    } else if (ngr > 1 && hipMemGetInfo(&fre, &tot) == hipSuccess &&
+               fre > gbytes + ((size_t)16 << 30)) {
+      // room for one spare: the transposed passes run grid by grid
+      if (hipMalloc(&c->scratch[0], gbytes) == hipSuccess) c->fft_one_spare = true;
+      else hipGetLastError();
     }
   }
   // sort buffers
@@ -877,12 +895,31 @@ static void fft_args(capow_ctx* c, FftArgs& a) {
   for (int s = 0; s < 4; s++)
     if (c->grid[s]) a.scratch[ns++] = c->scratch[s];
   if (!c->scratch[0]) a.scratch[0] = nullptr;
+  a.one_spare = c->fft_one_spare;
+  if (c->fft_one_spare) {
+    a.scratch[0] = c->scratch[0];
+    for (int q = 1; q < 4; q++) a.scratch[q] = nullptr;
+  }
   a.px = c->fft_px;
   a.py = c->fft_py;
   a.rows_mul = c->fft_rows_mul;
   a.ncu = c->ncu;
   a.persist = c->fft_persist;
   a.order = c->fft_order;
+}
+
+// Takes back the buffers a transform rotated through the one spare: the
+// i-th transformed grid belongs to slot slot_of[i].
+static void keep_rotated(capow_ctx* c, const FftArgs& a, const int* slot_of) {
+  for (int i = 0; i < a.ngrids; i++) c->grid[slot_of[i]] = a.grid[i];
+  c->scratch[0] = a.scratch[0];
+}
+
+static int grid_slots(const capow_ctx* c, int* slot_of) {
+  int n = 0;
+  for (int s = 0; s < 4; s++)
+    if (c->grid[s]) slot_of[n++] = s;
+  return n;
 }
 
 extern "C" int capow_fft(capow_ctx* c) {
@@ -901,12 +938,17 @@ extern "C" int capow_fft(capow_ctx* c) {
   }
   int rc = run_fft(a, c->fp64);
   if (rc) return fail(c, rc, std::string("fft launch failed: ") + hipGetErrorString(hipGetLastError()));
-  if (a.swapped)
+  if (a.rotated) {
+    int slot_of[4];
+    grid_slots(c, slot_of);
+    keep_rotated(c, a, slot_of);
+  } else if (a.swapped) {
     for (int s = 0; s < 4; s++) {
       void* t = c->grid[s];
       c->grid[s] = c->scratch[s];
       c->scratch[s] = t;
     }
+  }
   c->x_pending = a.x_deferred;
   c->xbin_ssq_rows = a.ssq_rows;
   c->state = 3;
@@ -921,10 +963,16 @@ static int finish_x(capow_ctx* c) {
   fft_args(c, a);
   const int rc = run_fft_x(a);
   if (rc) return fail(c, rc, std::string("fft x pass failed: ") + hipGetErrorString(hipGetLastError()));
-  for (int s = 0; s < 4; s++) {
-    void* t = c->grid[s];
-    c->grid[s] = c->scratch[s];
-    c->scratch[s] = t;
+  if (a.rotated) {
+    int slot_of[4];
+    grid_slots(c, slot_of);
+    keep_rotated(c, a, slot_of);
+  } else {
+    for (int s = 0; s < 4; s++) {
+      void* t = c->grid[s];
+      c->grid[s] = c->scratch[s];
+      c->scratch[s] = t;
+    }
   }
   c->x_pending = false;
   return CAPOW_OK;
@@ -1134,19 +1182,26 @@ extern "C" int capow_survey_power(capow_ctx* c, const double origin[3], double* 
       FftArgs a;
       fft_args(c, a);
       a.ngrids = 0;
+      int slot_of[4];
       for (int s = 0; s < 4; s++)
         if (used[s]) {
+          slot_of[a.ngrids] = s;
           a.grid[a.ngrids] = c->grid[s];
-          a.scratch[a.ngrids] = c->scratch[s];
+          a.scratch[a.ngrids] = c->fft_one_spare ? nullptr : c->scratch[s];
           a.ngrids++;
         }
+      if (c->fft_one_spare) a.scratch[0] = c->scratch[0];
       if (!c->scratch[0]) a.scratch[0] = nullptr;
       rc = run_fft(a, c->fp64);
       if (rc) {
         swap_in_out();
         return fail(c, rc, std::string("fft launch failed: ") + hipGetErrorString(hipGetLastError()));
       }
-      if (a.swapped)
+      if (a.rotated) {
+        keep_rotated(c, a, slot_of);
+        for (int s = 0; s < 4; s++)
+          if (used[s]) L.work[s] = c->grid[s];
+      } else if (a.swapped) {
         for (int s = 0; s < 4; s++)
           if (used[s]) {
             void* t = c->grid[s];
@@ -1154,6 +1209,7 @@ extern "C" int capow_survey_power(capow_ctx* c, const double origin[3], double* 
             c->scratch[s] = t;
             L.work[s] = c->grid[s];
           }
+      }
       rc = run_survey_bin(L, c->fp64, l, idx[q]);
       if (rc) {
         swap_in_out();

@@ -678,7 +678,7 @@ __global__ __launch_bounds__((N1 * N2 / clcm(N1, N2)) * B) void k_fft_rows_t(
     const T* __restrict__ i3, T* __restrict__ o0, T* __restrict__ o1, T* __restrict__ o2,
     T* __restrict__ o3, int64_t in_so, int64_t in_sb, int64_t out_so, int64_t out_sk,
     int64_t n_inner, int64_t inner_groups, const cx<T>* __restrict__ tw, int64_t nbx,
-    int ngrids, int outer_fast, double* __restrict__ ssq) {
+    int ngrids, int outer_fast, double* __restrict__ ssq, int ssq_slot0) {
   constexpr int N = N1 * N2;
   constexpr int E = clcm(N1, N2);
   constexpr int TS = N / E;
@@ -783,10 +783,12 @@ __global__ __launch_bounds__((N1 * N2 / clcm(N1, N2)) * B) void k_fft_rows_t(
 #pragma unroll
       for (int q = 0; q < 4; q++) red[threadIdx.x * 4 + q] = sq[q];
       __syncthreads();
-      if (threadIdx.x < 4) {
+      // the slots of this launch's grids only (grid-by-grid launches fill
+      // the others)
+      if ((int)threadIdx.x < ngrids) {
         double tot = 0.0;
         for (int i = 0; i < (int)blockDim.x; i++) tot += red[i * 4 + threadIdx.x];
-        ssq[(int64_t)blockIdx.x * 4 + threadIdx.x] = tot;
+        ssq[(int64_t)blockIdx.x * 4 + ssq_slot0 + threadIdx.x] = tot;
       }
     }
   }
@@ -960,7 +962,7 @@ static void launch_rows_t(FftArgs& a, const char* name, T* const* in, T* const* 
   TIMED(a.timer, name,
         (k_fft_rows_t<T, N1, N2, B, R2C, PS, PF, TWR><<<(unsigned)nblk, TS * B, lds, a.stream>>>(
             in[0], in[1], in[2], in[3], out[0], out[1], out[2], out[3], in_so, in_sb, out_so,
-            out_sk, n_inner, groups, (const cx<T>*)tw, nbx, a.ngrids, ofast, ssq)));
+            out_sk, n_inner, groups, (const cx<T>*)tw, nbx, a.ngrids, ofast, ssq, a.ssq_slot0)));
 }
 
 template <typename T, bool R2C>
@@ -1375,16 +1377,69 @@ static bool run_fft_transposed(FftArgs& a) {
   constexpr bool F64 = sizeof(T) == 8;
   if (!reg_split(nz / 2, &n1, &n2, F64) || !reg_split(ny, &n1, &n2, F64) || !reg_split(nx, &n1, &n2, F64))
     return false;
+  // write runs: 128 bytes (FP64: 64 for the y pass; measured best at 1024^3);
+  // rows_mul overrides all three
+  const int mz = a.rows_mul ? a.rows_mul : 2, my = a.rows_mul ? a.rows_mul : (sizeof(T) == 4 ? 2 : 1),
+            mx = a.rows_mul ? a.rows_mul : 2;
+  if (a.one_spare && a.ngrids > 1) {
+    // grid by grid through the one spare buffer: z grid -> spare, y spare
+    // -> grid, [x grid -> spare, then the two swap]
+    const bool kyz = sizeof(T) == 4;
+    const int64_t y_so = kyz ? px : (int64_t)ny * px, y_sk = kyz ? nzh * px : px;
+    const int64_t x_so = kyz ? nzh * px : px, x_sb = kyz ? px : (int64_t)ny * px;
+    const bool defer = a.defer_x && kyz && a.ssq && a.xq;
+    FftArgs b = a;
+    b.ngrids = 1;
+    int rows = 0;
+    if (defer && zero_u32((uint32_t*)a.ssq, (int64_t)XBIN_SSQ_MAX * 4 * 2, a.stream) != CAPOW_OK)
+      return false;
+    for (int q = 0; q < a.ngrids; q++) {
+      T* Gq[4] = {(T*)a.grid[q], (T*)a.grid[q], (T*)a.grid[q], (T*)a.grid[q]};
+      T* Sp[4] = {(T*)a.scratch[0], (T*)a.scratch[0], (T*)a.scratch[0], (T*)a.scratch[0]};
+      if (!try_rows_t<T, true>(b, "fft_z", nz / 2, Gq, Sp, nx, ny, (int64_t)ny * rowc, rowc,
+                               nzh * py, py, a.tw[2], mz))
+        return false;
+      b.ssq_slot0 = q;
+      if (!try_rows_t<T, false>(b, "fft_y", ny, Sp, Gq, nzh, nx, py, nzh * py, y_so, y_sk, a.tw[1],
+                                my, defer ? a.ssq : nullptr))
+        return false;
+      rows = max(rows, b.ssq_rows);
+      if (!defer) {
+        if (!try_rows_t<T, false>(b, "fft_x", nx, Gq, Sp, ny, nzh, x_so, x_sb, rowc,
+                                  (int64_t)ny * rowc, a.tw[0], mx))
+          return false;
+        void* t = a.grid[q];
+        a.grid[q] = a.scratch[0];
+        a.scratch[0] = t;
+      }
+    }
+    a.ssq_rows = rows;
+    a.rotated = true;
+    if (defer && rows <= XBIN_SSQ_MAX) {
+      a.x_deferred = true;
+      return true;
+    }
+    if (defer) {
+      // (more y-pass workgroups than ssq rows: the x passes run now)
+      for (int q = 0; q < a.ngrids; q++) {
+        T* Gq[4] = {(T*)a.grid[q], (T*)a.grid[q], (T*)a.grid[q], (T*)a.grid[q]};
+        T* Sp[4] = {(T*)a.scratch[0], (T*)a.scratch[0], (T*)a.scratch[0], (T*)a.scratch[0]};
+        if (!try_rows_t<T, false>(b, "fft_x", nx, Gq, Sp, ny, nzh, x_so, x_sb, rowc,
+                                  (int64_t)ny * rowc, a.tw[0], mx))
+          return false;
+        void* t = a.grid[q];
+        a.grid[q] = a.scratch[0];
+        a.scratch[0] = t;
+      }
+    }
+    return true;
+  }
   T* G[4];
   T* S[4];
   for (int q = 0; q < 4; q++) {
     G[q] = q < a.ngrids ? (T*)a.grid[q] : (T*)a.grid[0];
     S[q] = q < a.ngrids ? (T*)a.scratch[q] : (T*)a.scratch[0];
   }
-  // write runs: 128 bytes (FP64: 64 for the y pass; measured best at 1024^3);
-  // rows_mul overrides all three
-  const int mz = a.rows_mul ? a.rows_mul : 2, my = a.rows_mul ? a.rows_mul : (sizeof(T) == 4 ? 2 : 1),
-            mx = a.rows_mul ? a.rows_mul : 2;
   if (!try_rows_t<T, true>(a, "fft_z", nz / 2, G, S, nx, ny, (int64_t)ny * rowc, rowc,
                            nzh * py, py, a.tw[2], mz))
     return false;
@@ -1414,13 +1469,31 @@ int run_fft_x(FftArgs& a) {
   const Geom& g = *a.g;
   const int nx = g.n[0], ny = g.n[1];
   const int64_t nzh = g.n[2] / 2 + 1, rowc = g.rowc, px = a.px;
+  const int mx = a.rows_mul ? a.rows_mul : 2;
+  if (a.one_spare && a.ngrids > 1) {
+    FftArgs b = a;
+    b.ngrids = 1;
+    for (int q = 0; q < a.ngrids; q++) {
+      float* Gq[4] = {(float*)a.grid[q], (float*)a.grid[q], (float*)a.grid[q], (float*)a.grid[q]};
+      float* Sp[4] = {(float*)a.scratch[0], (float*)a.scratch[0], (float*)a.scratch[0],
+                      (float*)a.scratch[0]};
+      if (!try_rows_t<float, false>(b, "fft_x", nx, Gq, Sp, ny, nzh, nzh * px, px, rowc,
+                                    (int64_t)ny * rowc, a.tw[0], mx))
+        return CAPOW_ERR_UNSUPPORTED;
+      void* t = a.grid[q];
+      a.grid[q] = a.scratch[0];
+      a.scratch[0] = t;
+    }
+    a.rotated = true;
+    a.x_deferred = false;
+    return hipGetLastError() == hipSuccess ? CAPOW_OK : CAPOW_ERR_HIP;
+  }
   float* G[4];
   float* S[4];
   for (int q = 0; q < 4; q++) {
     G[q] = q < a.ngrids ? (float*)a.grid[q] : (float*)a.grid[0];
     S[q] = q < a.ngrids ? (float*)a.scratch[q] : (float*)a.scratch[0];
   }
-  const int mx = a.rows_mul ? a.rows_mul : 2;
   if (!try_rows_t<float, false>(a, "fft_x", nx, G, S, ny, nzh, nzh * px, px, rowc,
                                 (int64_t)ny * rowc, a.tw[0], mx))
     return CAPOW_ERR_UNSUPPORTED;
@@ -1435,7 +1508,7 @@ static int run_fft_t(FftArgs& a) {
   a.swapped = false;
   a.x_deferred = false;
   if (a.fast && a.scratch[0] && run_fft_transposed<T>(a)) {
-    a.swapped = !a.x_deferred;
+    a.swapped = !a.x_deferred && !a.rotated;
     return hipGetLastError() == hipSuccess ? CAPOW_OK : CAPOW_ERR_HIP;
   }
   T* gp[4];

@@ -145,6 +145,13 @@ struct FftArgs {
   double xbin_factor = 0.0;
   double* xq = nullptr;
   int ssq_rows = 0;       // out: workgroups of the y pass (rows of ssq)
+  int ssq_slot0 = 0;      // ssq slot of grid[0] (grid-by-grid passes)
+  // one spare buffer (scratch[0] only, grids too large for a scratch each):
+  // the passes run grid by grid, z: grid -> spare, y: spare -> grid, and an
+  // x pass grid -> spare swaps the two; on return grid[] and scratch[0]
+  // hold the rotated pointers (rotated = true), for the caller to keep
+  bool one_spare = false;
+  bool rotated = false;
 };
 #define XBIN_SSQ_MAX 8192
 int run_fft(FftArgs& a, bool fp64);

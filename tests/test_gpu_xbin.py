@@ -123,3 +123,42 @@ def test_fused_binning_twice_and_after_backward(monkeypatch):
     for code in a["codes"]:
         assert np.array_equal(a[code]["pk"].view(np.uint64), c[code]["pk"].view(np.uint64))
     ps.close()
+
+
+@pytest.mark.parametrize("prec", [api.FP32, api.FP64])
+def test_one_spare_pipeline_matches_full_scratch(prec, monkeypatch):
+    """Grids too large for a second set (four FP32 grids at 2048^3) run the
+    transposed FFT grid by grid through one spare buffer
+    (CAPOW_FFT_ONE_SPARE=1 forces it here): the same kernels on the same
+    rows, so the bins, the k-space grids (the pending x pass run grid by
+    grid) and a backward after them are bitwise those of the full-scratch
+    pipeline.  Four grids (TSC + interlacing, density and momentum)."""
+    n = 64   # NZ/2 = 32: register sizes on every axis (the backward needs them)
+    p = api.make_params(nx=n, ny=n, nz=n, interp_order=3, do_interlacing=1, momentum=1,
+                        extra_spectra=3, precision=prec, nk=30, maxk=0.3)
+    pos, vel = mocks.zeldovich(40, 1000.0, 17)
+    xyzv = mocks.xyzv(pos, vel, 0)
+    G = np.random.default_rng(2).normal(size=(3, 5, 30))
+
+    def run():
+        ps = api.PowerSpectrum(p)
+        ps.set_particles(xyzv)
+        res = ps.power()
+        grids = [ps.get_grid(s) for s in range(4)]
+        gx, gw = ps.backward(G)
+        res2 = ps.power()
+        ps.close()
+        return res, grids, gx, gw, res2
+
+    full = run()
+    monkeypatch.setenv("CAPOW_FFT_ONE_SPARE", "1")
+    one = run()
+    monkeypatch.delenv("CAPOW_FFT_ONE_SPARE")
+    u = np.uint32 if prec == api.FP32 else np.uint64
+    for r in (0, 4):
+        for code in full[r]["codes"]:
+            assert np.array_equal(full[r][code]["pk"], one[r][code]["pk"]), (r, code)
+        assert np.array_equal(full[r]["nmodes"], one[r]["nmodes"])
+    for s in range(4):
+        assert np.array_equal(full[1][s].view(u), one[1][s].view(u)), s
+    assert np.array_equal(full[2], one[2]) and np.array_equal(full[3], one[3])

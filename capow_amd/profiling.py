@@ -27,10 +27,12 @@ import sys
 
 # stage names of one step, in launch order (capow_power)
 STAGE_PATTERNS = [
-    ("bucket_count", r"k_bucket<.*false>"),
+    # k_bucket<T, ORDER, SCATTER, EDGE[, IDX]>: the third argument is the pass
+    ("bucket_count", r"k_bucket<[^,]+, \d+, false"),
+    ("sort_clear", r"k_zero_u32"),
     ("stats_final", r"k_stats_final"),
     ("scan", r"k_scan_"),
-    ("bucket_scatter", r"k_bucket<.*true>"),
+    ("bucket_scatter", r"k_bucket<[^,]+, \d+, true"),
     ("deposit", r"k_deposit"),
     # the x pass fused with the binning (FP32), its quanta and final sums
     ("fft_x_bin", r"k_fft_x_bin"),

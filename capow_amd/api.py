@@ -126,6 +126,7 @@ SIGNATURES = {
                          C.POINTER(AssignStats), C.POINTER(C.c_float)], C.c_int),
     "capow_batch_destroy": ([_V], C.c_int),
     "capow_batch_set_graph": ([_V, C.c_int32], C.c_int),
+    "capow_batch_set_stack": ([_V, C.c_int32], C.c_int),
     "capow_autotune": ([_V, C.c_int32, C.c_char_p, C.c_int32], C.c_int),
     "capow_batch_autotune": ([_V, C.POINTER(Mock), C.c_int32, C.c_int32, C.c_char_p, C.c_int32],
                              C.c_int),
@@ -492,6 +493,14 @@ class Batch:
         """capow_batch_set_graph: capture a catalogue list into one HIP graph
         after its first run and replay it while the list is unchanged."""
         rc = lib().capow_batch_set_graph(self.h, 1 if on else 0)
+        if rc:
+            raise CapowError(rc, lib().capow_batch_last_error(self.h).decode())
+
+    def set_stack(self, m):
+        """capow_batch_set_stack: run each parameter set's catalogues in
+        stacks of up to m, one launch per stage per stack (0: one catalogue
+        per launch)."""
+        rc = lib().capow_batch_set_stack(self.h, int(m))
         if rc:
             raise CapowError(rc, lib().capow_batch_last_error(self.h).decode())
 

@@ -132,6 +132,13 @@ struct capow_ctx {
   double* sv_acc = nullptr;
   double* h_sv = nullptr;
   int sv_nwg = 0;
+  // stacked batch context: nstack catalogues of this parameter set share
+  // one launch per stage; every per-catalogue buffer (grids, sort counters,
+  // sums, flags, bins, the fused binning's rows) holds nstack copies
+  int nstack = 1;
+  MockDev* stack_tab = nullptr;     // device table of the catalogues of a run
+  MockDev* h_stack_tab = nullptr;   // its pinned staging
+  int64_t stack_rows = 0;           // sorted rows (the sum of the catalogue sizes)
 };
 
 static std::string g_create_err;
@@ -478,6 +485,11 @@ static int setup(capow_ctx* c) {
   }
   // one bucket per (tile, border class)
   g.nbuckets = (int64_t)g.ntile[0] * g.ntile[1] * g.ntile[2] * 27;
+  g.ntiles_mock = (int64_t)g.ntile[0] * g.ntile[1] * g.ntile[2];
+  g.mock_stride = 0;
+  const int M = c->nstack;
+  if ((double)g.nbuckets * M > (double)UINT32_MAX - 1)
+    return fail(c, CAPOW_ERR_UNSUPPORTED, "too many sort buckets for this stack");
   // LDS hash probes per key in the counting sort; CAPOW_SORT_PROBES=0 sends
   // every copy through the global counters (the table-full path)
   g.sort_probes = 32;
@@ -508,14 +520,21 @@ static int setup(capow_ctx* c) {
   const size_t gbytes = (size_t)cap_c * 2 * c->esz;
   c->grid_bytes = gbytes;
   int ngr = 0;
+  // a stack: the catalogues' grids back to back in one buffer per slot
+  if (M > 1) g.mock_stride = cap_c * 2;
   for (int s = 0; s < 4; s++) {
     bool need = (s == 0) || (s == 1 && g.interlace) || (s == 2 && hasB) || (s == 3 && hasB && g.interlace);
     if (need) {
-      HIPCHK(c, hipMalloc(&c->grid[s], gbytes));
+      HIPCHK(c, hipMalloc(&c->grid[s], gbytes * M));
       ngr++;
     }
   }
-  bool want_scratch = true;
+  if (M > 1) {
+    // a scratch buffer per grid slot, or no stack
+    for (int s = 0; s < 4; s++)
+      if (c->grid[s]) HIPCHK(c, hipMalloc(&c->scratch[s], gbytes * M));
+  }
+  bool want_scratch = M == 1;
   if (const char* e = getenv("CAPOW_FFT_TRANSPOSED")) want_scratch = atoi(e) != 0;
   // CAPOW_FFT_ONE_SPARE=1: the one-spare pipeline even when a full second
   // set of grids would fit (for tests at small sizes)
@@ -547,17 +566,22 @@ static int setup(capow_ctx* c) {
     }
   }
   // sort buffers
-  HIPCHK(c, hipMalloc((void**)&c->counts, (size_t)g.nbuckets * 4));
-  HIPCHK(c, hipMalloc((void**)&c->offsets, (size_t)(g.nbuckets + 1) * 4));
-  HIPCHK(c, hipMalloc((void**)&c->cursor, (size_t)g.nbuckets * 4));
-  int64_t nblk = (g.nbuckets + 4095) / 4096;
+  const int64_t nbk = g.nbuckets * M;
+  HIPCHK(c, hipMalloc((void**)&c->counts, (size_t)nbk * 4));
+  HIPCHK(c, hipMalloc((void**)&c->offsets, (size_t)(nbk + 1) * 4));
+  HIPCHK(c, hipMalloc((void**)&c->cursor, (size_t)nbk * 4));
+  int64_t nblk = (nbk + 4095) / 4096;
   HIPCHK(c, hipMalloc((void**)&c->bsum, (size_t)(nblk + 1) * 4));
   HIPCHK(c, hipMalloc((void**)&c->boff, (size_t)(nblk + 1) * 4));
-  HIPCHK(c, hipMalloc((void**)&c->partials, (size_t)c->count_blocks * STAT_N * 8));
-  HIPCHK(c, hipMalloc((void**)&c->errs, 4 * sizeof(unsigned int)));
-  HIPCHK(c, hipMalloc((void**)&c->stats, STAT_TOTAL * 8));
-  HIPCHK(c, hipHostMalloc((void**)&c->h_stats, STAT_TOTAL * 8, hipHostMallocDefault));
-  HIPCHK(c, hipHostMalloc((void**)&c->h_errs, 4 * sizeof(unsigned int), hipHostMallocDefault));
+  HIPCHK(c, hipMalloc((void**)&c->partials, (size_t)c->count_blocks * STAT_N * 8 * M));
+  HIPCHK(c, hipMalloc((void**)&c->errs, 4 * sizeof(unsigned int) * M));
+  HIPCHK(c, hipMalloc((void**)&c->stats, STAT_TOTAL * 8 * M));
+  HIPCHK(c, hipHostMalloc((void**)&c->h_stats, STAT_TOTAL * 8 * M, hipHostMallocDefault));
+  HIPCHK(c, hipHostMalloc((void**)&c->h_errs, 4 * sizeof(unsigned int) * M, hipHostMallocDefault));
+  if (M > 1) {
+    HIPCHK(c, hipMalloc((void**)&c->stack_tab, sizeof(MockDev) * M));
+    HIPCHK(c, hipHostMalloc((void**)&c->h_stack_tab, sizeof(MockDev) * M, hipHostMallocDefault));
+  }
 
   // FFT plans + twiddles
   if (!make_plan(p.nx, &c->plan[0]) || !make_plan(p.ny, &c->plan[1]) ||
@@ -604,14 +628,14 @@ static int setup(capow_ctx* c) {
     const size_t bytes = (size_t)bp.per_copy * 8;
     const int n2d = p.output_2d ? p.nk * p.nmu : 0;
     HIPCHK(c, hipMalloc((void**)&c->bin_partials, (size_t)c->bin_nwg * bytes));
-    HIPCHK(c, hipMalloc((void**)&c->bin_out, bytes));
+    HIPCHK(c, hipMalloc((void**)&c->bin_out, bytes * M));
     HIPCHK(c, hipMalloc((void**)&c->bin_counts, (size_t)(p.nk + n2d) * 8));
     if (c->bin_mirror2d) {
       HIPCHK(c, hipMalloc((void**)&c->bin_partials2, (size_t)c->bin_nwg * bp.b.nspec * n2d * 8));
       HIPCHK(c, hipMalloc((void**)&c->bin_qscale, (size_t)2 * p.nk * 8));
     }
     c->bin_counts_valid = false;
-    HIPCHK(c, hipHostMalloc((void**)&c->h_bins, bytes, hipHostMallocDefault));
+    HIPCHK(c, hipHostMalloc((void**)&c->h_bins, bytes * M, hipHostMallocDefault));
     // the fused x pass + binning: FP32 out-of-place transform (scratch
     // grids), persistent row passes, mirror-group 1-D bins
     const size_t xl = xbin_lds(g, bp.b);
@@ -635,10 +659,13 @@ static int setup(capow_ctx* c) {
       const int per_cu = (int)((160 * 1024) / xl) > 0 ? (int)((160 * 1024) / xl) : 1;
       c->xbin_rows = c->ncu * (per_cu < 8 ? per_cu : 8);
       const int F = bp.b.nspec * (p.odd_multipoles ? 5 : 3);
-      HIPCHK(c, hipMalloc((void**)&c->xbin_ssq, (size_t)XBIN_SSQ_MAX * 4 * 8));
-      HIPCHK(c, hipMalloc((void**)&c->xbin_q, 6 * 8));
-      HIPCHK(c, hipMalloc((void**)&c->xbin_part, (size_t)c->xbin_rows * F * p.nk * 8));
+      HIPCHK(c, hipMalloc((void**)&c->xbin_ssq, (size_t)XBIN_SSQ_MAX * 4 * 8 * M));
+      HIPCHK(c, hipMalloc((void**)&c->xbin_q, 6 * 8 * M));
+      HIPCHK(c, hipMalloc((void**)&c->xbin_part, (size_t)c->xbin_rows * F * p.nk * 8 * M));
     }
+    if (M > 1 && !c->xbin_ok)
+      return fail(c, CAPOW_ERR_UNSUPPORTED,
+                  "stacked batches need the fused x-pass binning (FP32, 1-D bins, register FFT lengths)");
   }
   return CAPOW_OK;
 }
@@ -654,14 +681,16 @@ static void release(capow_ctx* c) {
   for (void* q : ps) if (q) hipFree(q);
   void* bs[] = {c->bwd_G, c->bwd_g0, c->bwd_out, c->bwd_w, c->sorted_idx, c->bwd_err};
   for (void* q : bs) if (q) hipFree(q);
-  void* hs[] = {c->h_stats, c->h_errs, c->h_bins, c->h_sv};
+  void* hs[] = {c->h_stats, c->h_errs, c->h_bins, c->h_sv, c->h_stack_tab};
+  if (c->stack_tab) hipFree(c->stack_tab);
   for (void* q : hs) if (q) hipHostFree(q);
   void* sv[] = {c->sv_src[0], c->sv_src[1], c->sv_src[2], c->sv_src[3], c->sv_F0, c->sv_F0m,
                 c->sv_partials, c->sv_acc};
   for (void* q : sv) if (q) hipFree(q);
 }
 
-extern "C" int capow_create(const capow_params* p, capow_ctx** out) {
+// nstack > 1: a stacked batch context (internal to the batches)
+static int create_ctx(const capow_params* p, int nstack, capow_ctx** out) {
   if (!p || !out) return fail(nullptr, CAPOW_ERR_ARG, "null argument");
   *out = nullptr;
   std::string m;
@@ -669,6 +698,7 @@ extern "C" int capow_create(const capow_params* p, capow_ctx** out) {
   if (rc) return fail(nullptr, rc, m);
   capow_ctx* c = new capow_ctx();
   c->p = *p;
+  c->nstack = nstack < 1 ? 1 : nstack;
   if (const char* e = getenv("CAPOW_FFT_COL_MUL")) c->fft_col_mul = atoi(e) == 1 ? 1 : 2;
   if (const char* e = getenv("CAPOW_FFT_ROWS_MUL")) c->fft_rows_mul = atoi(e) == 2 ? 2 : (atoi(e) == 1 ? 1 : 0);
   if (const char* e = getenv("CAPOW_FFT_PERSIST")) c->fft_persist = atoi(e) != 0;
@@ -687,6 +717,10 @@ extern "C" int capow_create(const capow_params* p, capow_ctx** out) {
   }
   *out = c;
   return CAPOW_OK;
+}
+
+extern "C" int capow_create(const capow_params* p, capow_ctx** out) {
+  return create_ctx(p, 1, out);
 }
 
 extern "C" int capow_destroy(capow_ctx* c) {
@@ -1424,7 +1458,47 @@ struct capow_batch {
   int gactive = 0;
   double* gstage = nullptr;
   std::vector<hipEvent_t> gev;   // fork / join of the captured streams
+  // stacked mode (capow_batch_set_stack): one stack context per parameter
+  // set, on stream config % nactive; a run orders the catalogues by
+  // (parameter set, weights or not) and sends each such group through its
+  // set's context in stacks of up to `stack` catalogues, one launch per
+  // stage per stack.  The staging then holds n stats rows, n flag rows and
+  // the stacks' bins (stack at position j0: from j0 * stride_bins on).
+  int stack = 0;
+  std::vector<capow_ctx*> sctx;  // [config]
+  MockDev* tab = nullptr;        // device table, one entry per catalogue position
+  MockDev* htab = nullptr;       // its pinned staging
+  size_t tab_mocks = 0;
+  int pc_max = 0;                // largest bin copy (doubles) over the sets
+  int gstack = 0;
 };
+
+// A run's stacks: the catalogue at each position (grouped by parameter set
+// and weights) and every stack's first position and size.
+struct StackPlan {
+  std::vector<int> order;        // position -> catalogue index
+  std::vector<int> first, size;  // per stack
+};
+
+static void stack_plan(const capow_batch* b, const capow_mock* mocks, int n, StackPlan& sp) {
+  sp.order.clear();
+  sp.first.clear();
+  sp.size.clear();
+  for (int q = 0; q < b->nconf; q++)
+    for (int hw = 0; hw < 2; hw++) {
+      int in_stack = 0;
+      for (int i = 0; i < n; i++) {
+        if (mocks[i].config != q || (mocks[i].w != nullptr) != (hw != 0)) continue;
+        if (in_stack == 0) {
+          sp.first.push_back((int)sp.order.size());
+          sp.size.push_back(0);
+        }
+        sp.order.push_back(i);
+        sp.size.back()++;
+        if (++in_stack == b->stack) in_stack = 0;
+      }
+    }
+}
 
 static int bfail(capow_batch* b, int code, const std::string& m) {
   if (b) b->err = m;
@@ -1438,6 +1512,9 @@ extern "C" int capow_batch_destroy(capow_batch* b) {
   if (b->gexec) hipGraphExecDestroy(b->gexec);
   for (hipEvent_t e : b->gev) hipEventDestroy(e);
   for (capow_ctx* c : b->ctx) capow_destroy(c);
+  for (capow_ctx* c : b->sctx) capow_destroy(c);
+  if (b->tab) hipFree(b->tab);
+  if (b->htab) hipHostFree(b->htab);
   for (hipEvent_t e : b->ev) hipEventDestroy(e);
   for (hipStream_t s : b->streams) hipStreamDestroy(s);
   if (b->stage) hipHostFree(b->stage);
@@ -1452,6 +1529,54 @@ extern "C" int capow_batch_set_graph(capow_batch* b, int32_t on) {
   for (hipStream_t s : b->streams) hipStreamSynchronize(s);
   batch_drop_graph(b);
   b->graph = on != 0;
+  return CAPOW_OK;
+}
+
+// Stacked mode: a stack context per parameter set holding `m` catalogues
+// (fewer when m times the largest catalogue would pass 2^32 - 4097 sorted
+// rows), its sort buffers sized for m of the largest catalogues and its
+// tunables copied from the set's stream-0 context.  m = 0 returns to the
+// per-catalogue streams.  Every set must support stacks, or the mode stays
+// as it was.
+extern "C" int capow_batch_set_stack(capow_batch* b, int32_t m) {
+  if (!b || m < 0) return bfail(b, CAPOW_ERR_ARG, "bad stack size");
+  for (hipStream_t s : b->streams) hipStreamSynchronize(s);
+  batch_drop_graph(b);
+  if (m == 0) {
+    b->stack = 0;
+    return CAPOW_OK;
+  }
+  const int64_t per = b->max_n > 0 ? b->max_n : 1;
+  int64_t mm = std::min<int64_t>(m, ((int64_t)UINT32_MAX - 4096) / per);
+  if (mm < 1) mm = 1;
+  std::vector<capow_ctx*> made;
+  for (int q = 0; q < b->nconf; q++) {
+    capow_ctx* c = nullptr;
+    int rc = create_ctx(&b->cfg[q], (int)mm, &c);
+    if (!rc) {
+      capow_ctx* c0 = b->ctx[q];
+      c->count_blocks = c0->count_blocks;
+      c->g.sort_probes = c0->g.sort_probes;
+      c->dep_halo = c0->dep_halo;
+      c->dep_bank = c0->dep_bank;
+      c->fft_rows_mul = c0->fft_rows_mul;
+      c->fft_order = c0->fft_order;
+      c->cap_fixed = true;
+      const size_t rows = (size_t)mm * (size_t)per + 4096;
+      rc = ensure(c, &c->sorted, &c->sorted_cap, rows * 4 * c->esz);
+      if (!rc) rc = ensure(c, &c->sorted_w, &c->sorted_w_cap, rows * c->esz);
+    }
+    if (rc) {
+      const std::string m2 = c ? c->err : g_create_err;
+      if (c) capow_destroy(c);
+      for (capow_ctx* d : made) capow_destroy(d);
+      return bfail(b, rc, "stack for config " + std::to_string(q) + ": " + m2);
+    }
+    made.push_back(c);
+  }
+  for (capow_ctx* d : b->sctx) capow_destroy(d);
+  b->sctx = made;
+  b->stack = (int)mm;
   return CAPOW_OK;
 }
 
@@ -1507,6 +1632,7 @@ extern "C" int capow_batch_create(const capow_params* configs, int32_t nconf,
     }
   }
   b->stride = STAT_TOTAL + 2 + per_copy_max;
+  b->pc_max = (int)per_copy_max;
   *out = b;
   return CAPOW_OK;
 }
@@ -1542,6 +1668,120 @@ static int batch_enqueue(capow_batch* b, const capow_mock* mocks, int n, bool ti
   return first;
 }
 
+// Enqueues one stack (positions j0 .. j0 + k - 1 of the plan, catalogues of
+// one parameter set with or without weights) on its set's stack context:
+// the table upload, the sort and deposit of every catalogue, the z and y
+// passes and the fused x pass + binning, each one launch for the stack, then
+// the copies of the sums, flags and bins into the run's staging (n
+// positions in all).
+static int stack_enqueue(capow_batch* b, capow_ctx* c, const capow_mock* mocks,
+                         const StackPlan& sp, int st, int n) {
+  const int j0 = sp.first[st], k = sp.size[st];
+  Geom& g = c->g;
+  hipStream_t s = c->stream;
+  int64_t rows = 0, want = 1;
+  for (int j = j0; j < j0 + k; j++) {
+    const capow_mock& m = mocks[sp.order[j]];
+    b->htab[j] = MockDev{m.xyzv, m.w, m.n};
+    rows += m.n;
+    want = std::max(want, (m.n + BK_CHUNK - 1) / BK_CHUNK);
+  }
+  g.has_w = mocks[sp.order[j0]].w != nullptr;
+  const int64_t cap = (int64_t)(c->sorted_cap / (4 * c->esz));
+  if (rows > cap || (g.has_w && (int64_t)(c->sorted_w_cap / c->esz) < rows))
+    return fail(c, CAPOW_ERR_NOMEM, "stack sort buffer too small");
+  HIPCHK(c, hipMemcpyAsync(b->tab + j0, b->htab + j0, (size_t)k * sizeof(MockDev),
+                           hipMemcpyHostToDevice, s));
+  if (zero_u32(c->counts, g.nbuckets * k, s) || zero_u32(c->errs, 4 * (int64_t)k, s))
+    return fail(c, CAPOW_ERR_HIP, "clearing the stack's sort counters failed");
+  DepositArgs a;
+  a.g = &g;
+  a.stream = s;
+  a.timer = &c->timer;
+  a.parts = nullptr;
+  a.wts = nullptr;
+  a.n = 0;
+  a.nmock = k;
+  a.mocks = b->tab + j0;
+  // each catalogue's k_bucket blocks: as for a context of its own
+  a.count_blocks = (int)std::min<int64_t>(want, c->count_blocks);
+  a.counts = c->counts;
+  a.offsets = c->offsets;
+  a.cursor = c->cursor;
+  a.bsum = c->bsum;
+  a.boff = c->boff;
+  a.partials = c->partials;
+  a.errs = c->errs;
+  a.stats = c->stats;
+  for (int q = 0; q < 4; q++) a.grid[q] = c->grid[q];
+  a.halo = c->dep_halo;
+  a.bank = c->dep_bank;
+  int rc = run_count(a, c->fp64);
+  if (rc) return fail(c, rc, std::string("stack count failed: ") + hipGetErrorString(hipGetLastError()));
+  a.sorted = c->sorted;
+  a.sorted_w = c->sorted_w;
+  a.sorted_cap = cap;
+  rc = run_scatter_deposit(a, c->fp64);
+  if (rc) return fail(c, rc, std::string("stack deposit failed: ") + hipGetErrorString(hipGetLastError()));
+  FftArgs f;
+  fft_args(c, f);
+  f.nmock = k;
+  f.mock_stride = g.mock_stride;
+  f.defer_x = true;
+  f.ssq = c->xbin_ssq;
+  f.xbin_factor = c->xbin_factor;
+  f.xq = c->xbin_q;
+  rc = run_fft(f, c->fp64);
+  if (!rc && !f.x_deferred) rc = CAPOW_ERR_UNSUPPORTED;
+  if (rc) return fail(c, rc, std::string("stack transform failed: ") + hipGetErrorString(hipGetLastError()));
+  BinLaunch L;
+  bin_launch(c, L);
+  L.out_stride = c->bin_per_copy;
+  rc = run_fft_x_bin(f, L, c->xbin_part, c->xbin_rows);
+  if (rc) return fail(c, rc, std::string("stack binning failed: ") + hipGetErrorString(hipGetLastError()));
+  double* stage = b->stage;
+  HIPCHK(c, hipMemcpyAsync(stage + (size_t)j0 * STAT_TOTAL, c->stats, (size_t)k * STAT_TOTAL * 8,
+                           hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipMemcpyAsync(reinterpret_cast<unsigned int*>(stage + (size_t)n * STAT_TOTAL) + (size_t)j0 * 4,
+                           c->errs, (size_t)k * 4 * sizeof(unsigned int), hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipMemcpyAsync(stage + (size_t)n * (STAT_TOTAL + 2) + (size_t)j0 * b->pc_max, c->bin_out,
+                           (size_t)k * c->bin_per_copy * 8, hipMemcpyDeviceToHost, s));
+  return CAPOW_OK;
+}
+
+// Every stack of the plan, stack contexts on stream config % nactive;
+// timed: events around each stack (not while capturing).
+static int stacks_enqueue(capow_batch* b, const capow_mock* mocks, int n, const StackPlan& sp,
+                          bool timed, std::vector<int>& launched) {
+  int first = CAPOW_OK;
+  for (size_t st = 0; st < sp.first.size(); st++) {
+    const int j0 = sp.first[st];
+    const int q = mocks[sp.order[j0]].config;
+    capow_ctx* c = b->sctx[q];
+    capow_set_stream(c, b->streams[q % b->nactive]);
+    if (timed) hipEventRecord(b->ev[2 * j0], c->stream);
+    const int rc = stack_enqueue(b, c, mocks, sp, (int)st, n);
+    if (timed) hipEventRecord(b->ev[2 * j0 + 1], c->stream);
+    if (rc) {
+      if (!first) {
+        first = rc;
+        b->err = "stack at mock " + std::to_string(sp.order[j0]) + ": " + c->err;
+      }
+      continue;
+    }
+    for (int j = j0; j < j0 + sp.size[st]; j++) launched[sp.order[j]] = 1;
+  }
+  return first;
+}
+
+static int any_enqueue(capow_batch* b, const capow_mock* mocks, int n, bool timed,
+                       std::vector<int>& launched) {
+  if (b->stack <= 0) return batch_enqueue(b, mocks, n, timed, launched);
+  StackPlan sp;
+  stack_plan(b, mocks, n, sp);
+  return stacks_enqueue(b, mocks, n, sp, timed, launched);
+}
+
 static void batch_drop_graph(capow_batch* b) {
   if (b->gexec) hipGraphExecDestroy(b->gexec);
   b->gexec = nullptr;
@@ -1567,7 +1807,7 @@ static int batch_capture(capow_batch* b, const capow_mock* mocks, int n) {
   hipError_t e = hipEventRecord(b->gev[0], s0);
   for (int s = 1; s < b->nactive && e == hipSuccess; s++) e = hipStreamWaitEvent(b->streams[s], b->gev[0], 0);
   std::vector<int> launched(n, 0);
-  int rc = e == hipSuccess ? batch_enqueue(b, mocks, n, false, launched) : CAPOW_ERR_HIP;
+  int rc = e == hipSuccess ? any_enqueue(b, mocks, n, false, launched) : CAPOW_ERR_HIP;
   for (int s = 1; s < b->nactive && e == hipSuccess; s++) {
     e = hipEventRecord(b->gev[s], b->streams[s]);
     if (e == hipSuccess) e = hipStreamWaitEvent(s0, b->gev[s], 0);
@@ -1588,6 +1828,7 @@ static int batch_capture(capow_batch* b, const capow_mock* mocks, int n) {
   b->gkey.assign(mocks, mocks + n);
   b->gactive = b->nactive;
   b->gstage = b->stage;
+  b->gstack = b->stack;
   return CAPOW_OK;
 }
 
@@ -1613,22 +1854,64 @@ extern "C" int capow_batch_run(capow_batch* b, const capow_mock* mocks, int32_t 
     if (hipEventCreate(&e) != hipSuccess) return bfail(b, CAPOW_ERR_HIP, "hipEventCreate failed");
     b->ev.push_back(e);
   }
+  if (b->stack > 0 && b->tab_mocks < (size_t)n) {
+    if (b->tab) hipFree(b->tab);
+    if (b->htab) hipHostFree(b->htab);
+    b->tab = nullptr;
+    b->htab = nullptr;
+    b->tab_mocks = 0;
+    if (hipMalloc((void**)&b->tab, (size_t)n * sizeof(MockDev)) != hipSuccess ||
+        hipHostMalloc((void**)&b->htab, (size_t)n * sizeof(MockDev), hipHostMallocDefault) != hipSuccess)
+      return bfail(b, CAPOW_ERR_NOMEM, "batch catalogue table allocation failed");
+    b->tab_mocks = (size_t)n;
+  }
   int first = CAPOW_OK;
   bool fell_back = false;
   std::vector<int> launched(n, 0);
   const bool replay = b->graph && b->gexec && b->gkey.size() == (size_t)n &&
-                      b->gactive == b->nactive && b->gstage == b->stage &&
+                      b->gactive == b->nactive && b->gstage == b->stage && b->gstack == b->stack &&
                       memcmp(b->gkey.data(), mocks, (size_t)n * sizeof(capow_mock)) == 0;
   if (replay) {
     if (hipGraphLaunch(b->gexec, b->streams[0]) != hipSuccess)
       return bfail(b, CAPOW_ERR_HIP, std::string("hipGraphLaunch: ") + hipGetErrorString(hipGetLastError()));
     for (int i = 0; i < n; i++) launched[i] = 1;
   } else {
-    first = batch_enqueue(b, mocks, n, true, launched);
+    first = any_enqueue(b, mocks, n, true, launched);
   }
   for (hipStream_t s : b->streams)
     if (hipStreamSynchronize(s) != hipSuccess)
       return bfail(b, CAPOW_ERR_HIP, std::string("batch stream: ") + hipGetErrorString(hipGetLastError()));
+  if (b->stack > 0) {
+    // stacked: per position, the sums, flags and bins its stack staged
+    StackPlan sp;
+    stack_plan(b, mocks, n, sp);
+    for (size_t st = 0; st < sp.first.size(); st++) {
+      const int j0 = sp.first[st];
+      float t = -1.f;   // the stack's GPU time (not while replaying)
+      if (ms && !replay) hipEventElapsedTime(&t, b->ev[2 * j0], b->ev[2 * j0 + 1]);
+      for (int j = j0; j < j0 + sp.size[st]; j++) {
+        const int i = sp.order[j];
+        if (ms) ms[i] = t;
+        if (!launched[i]) continue;
+        capow_ctx* c = b->sctx[mocks[i].config];
+        const double* hs = b->stage + (size_t)j * STAT_TOTAL;
+        const unsigned int* he =
+            reinterpret_cast<const unsigned int*>(b->stage + (size_t)n * STAT_TOTAL) + (size_t)j * 4;
+        c->n = mocks[i].n;
+        int rc = assign_finish(c, hs, he, stats ? &stats[i] : nullptr);
+        if (!rc)
+          bin_finish(c, b->stage + (size_t)n * (STAT_TOTAL + 2) + (size_t)j0 * b->pc_max +
+                            (size_t)(j - j0) * c->bin_per_copy,
+                     &out[i]);
+        if (rc && !first) {
+          first = rc;
+          b->err = "mock " + std::to_string(i) + ": " + c->err;
+        }
+      }
+    }
+    if (b->graph && !replay && !first) return batch_capture(b, mocks, n);
+    return first;
+  }
   for (int i = 0; i < n; i++) {
     if (!launched[i]) continue;
     const int s = i % b->nactive;

@@ -54,6 +54,12 @@ struct Geom {
   int reach_lo, reach_hi;// taps of a particle lie in [key+reach_lo, key+reach_hi]
   int64_t nbuckets;      // ntiles (one bucket per deposit tile)
   int sort_probes;       // LDS hash probes per key in the sort (0: always global)
+  // stacked batches (many catalogues of one parameter set on one set of
+  // grids, catalogue-major): the tiles and buckets of catalogue m follow
+  // those of catalogue m - 1, and its grids start mock_stride reals after
+  // those of catalogue m - 1
+  int64_t ntiles_mock;   // tiles per catalogue (every tile: one catalogue)
+  int64_t mock_stride;   // reals per grid per catalogue (0: one catalogue)
 };
 
 // ---------------------------------------------------------------------------

@@ -279,14 +279,29 @@ __global__ __launch_bounds__(BK_T) void k_bucket(const T* __restrict__ parts,
                                                  T* __restrict__ sorted_w, int64_t cap,
                                                  double* __restrict__ partials,
                                                  unsigned int* __restrict__ errs,
-                                                 uint32_t* __restrict__ sorted_idx) {
+                                                 uint32_t* __restrict__ sorted_idx,
+                                                 const MockDev* __restrict__ mocks) {
   __shared__ uint32_t hkey[BK_HS];
   __shared__ uint32_t hcnt[BK_HS];   // copies per slot; in the scatter pass, then the slot's base
   __shared__ double sh[BK_T];
+  // stacked batch: catalogue blockIdx.y from the table, its buckets after
+  // those of the catalogues before it
+  const int64_t mock = blockIdx.y;
+  if (mocks) {
+    const MockDev md = mocks[mock];
+    parts = (const T*)md.parts;
+    wts = (const T*)md.w;
+    n = md.n;
+  }
+  const uint32_t kbase = (uint32_t)(mock * g.ntiles_mock * 27);
   double s_ngrid = 0, s_w = 0, s_v1 = 0, s_v2 = 0, s_v3 = 0, s_v4 = 0, s_out = 0;
   unsigned int oob = 0, reach_bad = 0;
   const int64_t nchunks = (n + BK_CHUNK - 1) / BK_CHUNK;
-  for (int64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+  // blocks of this catalogue: one per chunk, at most the launch's width
+  // (the count a launch for this catalogue alone has)
+  const int64_t nbm = max((int64_t)1, min(nchunks, (int64_t)gridDim.x));
+  if ((int64_t)blockIdx.x >= nbm) return;
+  for (int64_t ch = blockIdx.x; ch < nchunks; ch += nbm) {
     for (int s = threadIdx.x; s < BK_HS; s += BK_T) {
       hkey[s] = BK_EMPTY;
       hcnt[s] = 0;
@@ -303,13 +318,13 @@ __global__ __launch_bounds__(BK_T) void k_bucket(const T* __restrict__ parts,
       T x = T(0), y = T(0), z = T(0), v = T(0), w = T(1);
       if (valid) {
         load4<T>(parts, i, x, y, z, v);
-        if (g.has_w) w = wts[i];
+        if (g.has_w && wts) w = wts[i];
       }
       PClass<T, ORDER> c = classify<T, ORDER, EK>(x, y, z, v, g);
       const bool take = valid && c.take;
       px[k] = c.x; py[k] = c.y; pz[k] = c.z; pv[k] = v; pw[k] = w;
       pr[k] = BK_EMPTY;
-      const uint32_t key = take ? c.key : 0u;
+      const uint32_t key = take ? kbase + c.key : 0u;
       uint32_t rank;
       uint32_t* gctr = SCATTER ? cursor + key : counts + key;
       const int slot = hs_add<SCATTER>(hkey, hcnt, key, take, gctr, g.sort_probes, rank);
@@ -374,17 +389,25 @@ __global__ __launch_bounds__(BK_T) void k_bucket(const T* __restrict__ parts,
 #pragma unroll
     for (int q = 0; q < 7; q++) {
       double t = block_sum<BK_T>(r[q], sh);
-      if (threadIdx.x == 0) partials[(int64_t)blockIdx.x * STAT_N + q] = t;
+      if (threadIdx.x == 0) partials[(mock * gridDim.x + blockIdx.x) * STAT_N + q] = t;
     }
-    if (oob) atomicAdd(&errs[0], oob);
-    if (reach_bad) atomicAdd(&errs[1], reach_bad);
+    if (oob) atomicAdd(&errs[mock * 4 + 0], oob);
+    if (reach_bad) atomicAdd(&errs[mock * 4 + 1], reach_bad);
   }
 }
 
 // Fixed-order reduction of block partials -> stats[STAT_N] and alpha.
+// Stacked batch (mocks set): catalogue blockIdx.x, whose partials are the
+// first blocks (as many as its k_bucket used) of its nblocks rows.
 __global__ void k_stats_final(const double* __restrict__ partials, int nblocks, double ncells,
-                              double* __restrict__ stats) {
+                              double* __restrict__ stats, const MockDev* __restrict__ mocks) {
   __shared__ double sh[256];
+  partials += (int64_t)blockIdx.x * nblocks * STAT_N;
+  stats += (int64_t)blockIdx.x * STAT_TOTAL;
+  if (mocks) {
+    const int64_t nch = (mocks[blockIdx.x].n + BK_CHUNK - 1) / BK_CHUNK;
+    nblocks = (int)max((int64_t)1, min(nch, (int64_t)nblocks));
+  }
   for (int q = 0; q < 7; q++) {
     double acc = 0.0;
     // each thread sums a strided subset in fixed order, then a fixed tree
@@ -641,14 +664,16 @@ __global__ __launch_bounds__(DEP_T) void k_deposit(const T* __restrict__ sorted,
   const int lane = threadIdx.x & 63;
   const int trash = ncell + lane;
 
-  const int64_t tileid = blockIdx.x;
+  // catalogue (stacked batches; 0 otherwise) and its tile
+  const int64_t mock = (int64_t)blockIdx.x / g.ntiles_mock;
+  const int64_t tileid = (int64_t)blockIdx.x - mock * g.ntiles_mock;
   const int ttx = (int)(tileid % g.ntile[0]);
   const int tty = (int)((tileid / g.ntile[0]) % g.ntile[1]);
   const int ttz = (int)(tileid / ((int64_t)g.ntile[0] * g.ntile[1]));
   const int t0x = ttx * TX, t0y = tty * TY, t0z = ttz * TZ;
   // copies beyond the sorted buffer's capacity were not stored: flag the
   // overflow and deposit only what is there (the result is then discarded)
-  if (tileid == 0 && threadIdx.x == 0 && (int64_t)off[g.nbuckets] > cap) errs[2] = 1u;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && (int64_t)off[(int64_t)gridDim.x * 27] > cap) errs[2] = 1u;
   // the tile's 63 ranges (wave 0): start and exclusive prefix of the lengths
   __shared__ uint32_t r_start[64], r_pre[65];
   // the tile's max |w v^a|, |w v^b| as bit patterns (non-negative doubles
@@ -673,7 +698,8 @@ __global__ __launch_bounds__(DEP_T) void k_deposit(const T* __restrict__ sorted,
         nb[a] = q;
       }
       if (ok) {
-        const int64_t b = (((int64_t)nb[2] * g.ntile[1] + nb[1]) * g.ntile[0] + nb[0]) * 27;
+        const int64_t b =
+            (mock * g.ntiles_mock + ((int64_t)nb[2] * g.ntile[1] + nb[1]) * g.ntile[0] + nb[0]) * 27;
         const uint32_t s0 = (uint32_t)min((int64_t)off[b + d.lo], cap);
         const uint32_t s1 = (uint32_t)min((int64_t)off[b + d.hi], cap);
         st = s0;
@@ -706,8 +732,8 @@ __global__ __launch_bounds__(DEP_T) void k_deposit(const T* __restrict__ sorted,
     return r_start[lo] + (f - r_pre[lo]);
   };
 
-  const T d1 = (T)g.d[1], d2 = (T)g.d[2];   // bank keys (the taps shift in FP64)
-  const double alpha = stats[STAT_ALPHA];
+  const T d1 = (T)g.d[1], d2 = (T)g.d[2];   // bank keys
+  const double alpha = stats[mock * STAT_TOTAL + STAT_ALPHA];
   const bool single = total <= (uint32_t)(DEP_T * DEP_K);
 
   // The quanta come from this tile's own largest field values, so a large
@@ -1011,8 +1037,8 @@ __global__ __launch_bounds__(DEP_T) void k_deposit(const T* __restrict__ sorted,
     }
     __syncthreads();
     // write this phase's grids once: value * q - alpha for density fields
-    T* dA = ph ? gAi : gA;
-    T* dB = ph ? gBi : gB;
+    T* dA = (ph ? gAi : gA) + mock * g.mock_stride;
+    T* dB = (ph ? gBi : gB) + (HASB ? mock * g.mock_stride : 0);
     const double meanA = g.mean_sub_a ? alpha : 0.0;
     // cell pair c2 = (lxx*TY + ly)*(TZ/2) + lz/2, advanced by DEP_T pairs per
     // iteration with carries instead of divisions
@@ -1096,18 +1122,20 @@ static int run_count_t(DepositArgs& a) {
   const Geom& g = *a.g;
   const int nb = a.count_blocks;
   hipStream_t st = a.stream;
+  const dim3 grid((unsigned)nb, (unsigned)a.nmock);
   TIMED(a.timer, "bucket_count",
         (launch_edge(edge_kind(g), [&](auto ek) {
-          k_bucket<T, ORDER, false, decltype(ek)::value><<<nb, BK_T, 0, st>>>(
+          k_bucket<T, ORDER, false, decltype(ek)::value><<<grid, BK_T, 0, st>>>(
               (const T*)a.parts, (const T*)a.wts, a.n, g, a.counts, nullptr, nullptr, nullptr, nullptr,
-              0, a.partials, a.errs, nullptr);
+              0, a.partials, a.errs, nullptr, a.mocks);
         })));
   TIMED(a.timer, "stats_final",
-        (k_stats_final<<<1, 256, 0, st>>>(a.partials, nb,
-                                         (double)g.n[0] * (double)g.n[1] * (double)g.n[2],
-                                         a.stats)));
-  // scan counts -> offsets; zero the own/ghost cursors
-  int64_t nbk = g.nbuckets;
+        (k_stats_final<<<a.nmock, 256, 0, st>>>(a.partials, nb,
+                                               (double)g.n[0] * (double)g.n[1] * (double)g.n[2],
+                                               a.stats, a.mocks)));
+  // scan counts -> offsets (every catalogue's buckets in one scan); zero
+  // the own/ghost cursors
+  int64_t nbk = g.nbuckets * a.nmock;
   int64_t nblk = (nbk + SCAN_CHUNK - 1) / SCAN_CHUNK;
   TIMED(a.timer, "scan", (k_scan_reduce<<<(int)nblk, SCAN_T, 0, st>>>(a.counts, nbk, a.bsum)));
   TIMED(a.timer, "scan", (k_scan_carry<<<1, SCAN_T, 0, st>>>(a.bsum, nblk, a.boff)));
@@ -1123,21 +1151,22 @@ static int run_scatter_t(DepositArgs& a) {
   const Geom& g = *a.g;
   const int nb = a.count_blocks;
   hipStream_t st = a.stream;
+  const dim3 sgrid((unsigned)nb, (unsigned)a.nmock);
   TIMED(a.timer, "bucket_scatter",
         (launch_edge(edge_kind(g), [&](auto ek) {
           if (a.sorted_idx)
-            k_bucket<T, ORDER, true, decltype(ek)::value, true><<<nb, BK_T, 0, st>>>(
+            k_bucket<T, ORDER, true, decltype(ek)::value, true><<<sgrid, BK_T, 0, st>>>(
                 (const T*)a.parts, (const T*)a.wts, a.n, g, nullptr, a.cursor, a.offsets,
-                (T*)a.sorted, (T*)a.sorted_w, a.sorted_cap, nullptr, nullptr, a.sorted_idx);
+                (T*)a.sorted, (T*)a.sorted_w, a.sorted_cap, nullptr, nullptr, a.sorted_idx, a.mocks);
           else
-            k_bucket<T, ORDER, true, decltype(ek)::value, false><<<nb, BK_T, 0, st>>>(
+            k_bucket<T, ORDER, true, decltype(ek)::value, false><<<sgrid, BK_T, 0, st>>>(
                 (const T*)a.parts, (const T*)a.wts, a.n, g, nullptr, a.cursor, a.offsets,
-                (T*)a.sorted, (T*)a.sorted_w, a.sorted_cap, nullptr, nullptr, nullptr);
+                (T*)a.sorted, (T*)a.sorted_w, a.sorted_cap, nullptr, nullptr, nullptr, a.mocks);
         })));
   const int ncell = g.tdim[0] * g.tdim[1] * g.tdim[2];
   const size_t nacc = (size_t)ncell + DEP_TRASH;
   const size_t lds = sizeof(T) == 4 ? nacc * 8 : 2 * nacc * 8;
-  const int64_t ntiles = (int64_t)g.ntile[0] * g.ntile[1] * g.ntile[2];
+  const int64_t ntiles = (int64_t)g.ntile[0] * g.ntile[1] * g.ntile[2] * a.nmock;
   const bool hasB = g.fb >= 0;
   if (lds > DEP_LDS_MAX) return CAPOW_ERR_UNSUPPORTED;
   // the default tile shapes get compile-time extents (shifts instead of

@@ -678,13 +678,21 @@ __global__ __launch_bounds__((N1 * N2 / clcm(N1, N2)) * B) void k_fft_rows_t(
     const T* __restrict__ i3, T* __restrict__ o0, T* __restrict__ o1, T* __restrict__ o2,
     T* __restrict__ o3, int64_t in_so, int64_t in_sb, int64_t out_so, int64_t out_sk,
     int64_t n_inner, int64_t inner_groups, const cx<T>* __restrict__ tw, int64_t nbx,
-    int ngrids, int outer_fast, double* __restrict__ ssq, int ssq_slot0) {
+    int ngrids, int outer_fast, double* __restrict__ ssq, int ssq_slot0, int64_t ms) {
   constexpr int N = N1 * N2;
   constexpr int E = clcm(N1, N2);
   constexpr int TS = N / E;
   constexpr int G1 = E / N2;
   constexpr bool TW_REG = TWR && G1 == 1;
   extern __shared__ unsigned char smem[];
+  // stacked batch: catalogue blockIdx.y, ms reals after the one before it
+  // in every buffer, its own ssq rows
+  if (blockIdx.y) {
+    const int64_t mo = (int64_t)blockIdx.y * ms;
+    i0 += mo; i1 += mo; i2 += mo; i3 += mo;
+    o0 += mo; o1 += mo; o2 += mo; o3 += mo;
+    if (ssq) ssq += (int64_t)blockIdx.y * gridDim.x * 4;
+  }
   cx<T>* lds = reinterpret_cast<cx<T>*>(smem);
   const int t = threadIdx.x % TS, c = threadIdx.x / TS;
   const int64_t total = nbx * ngrids;
@@ -959,10 +967,12 @@ static void launch_rows_t(FftArgs& a, const char* name, T* const* in, T* const* 
   if (!PS || a.persist == 0 || nblk > total) nblk = total;
   if (ssq) a.ssq_rows = (int)nblk;
   const int ofast = (a.order >> (name[4] == 'z' ? 0 : name[4] == 'y' ? 1 : 2)) & 1;
+  const dim3 grid((unsigned)nblk, (unsigned)a.nmock);
   TIMED(a.timer, name,
-        (k_fft_rows_t<T, N1, N2, B, R2C, PS, PF, TWR><<<(unsigned)nblk, TS * B, lds, a.stream>>>(
+        (k_fft_rows_t<T, N1, N2, B, R2C, PS, PF, TWR><<<grid, TS * B, lds, a.stream>>>(
             in[0], in[1], in[2], in[3], out[0], out[1], out[2], out[3], in_so, in_sb, out_so,
-            out_sk, n_inner, groups, (const cx<T>*)tw, nbx, a.ngrids, ofast, ssq, a.ssq_slot0)));
+            out_sk, n_inner, groups, (const cx<T>*)tw, nbx, a.ngrids, ofast, ssq, a.ssq_slot0,
+            a.mock_stride)));
 }
 
 template <typename T, bool R2C>
@@ -1017,9 +1027,13 @@ static bool try_rows_t(FftArgs& a, const char* name, int N, T* const* in, T* con
 // ssq rows (4 per workgroup of the y pass, one per transformed grid in the
 // order A, A_int, B, B_int of the grids present) -> xq[2 s], xq[2 s + 1] =
 // q_s, 1/q_s per spectrum s, fixed order.  factor = 4 cor_max NX NY.
+// Stacked batch: catalogue blockIdx.x, its ssq rows and quanta after those
+// of the catalogues before it.
 __global__ __launch_bounds__(256) void k_xbin_scale(const double* __restrict__ ssq, int rows,
                                                     double factor, int il, int hb, BinArgsDev a,
                                                     double* __restrict__ xq) {
+  ssq += (int64_t)blockIdx.x * rows * 4;
+  xq += (int64_t)blockIdx.x * 6;
   // each thread sums a strided subset of the rows, then a fixed tree
   __shared__ double red[4][256];
   double part[4] = {0.0, 0.0, 0.0, 0.0};
@@ -1058,11 +1072,18 @@ __global__ __launch_bounds__((N1 * N2 / clcm(N1, N2)) * (2 * G * KZ)) void k_fft
     const float* __restrict__ i0, const float* __restrict__ i1, const float* __restrict__ i2,
     const float* __restrict__ i3, int64_t px, int64_t plane, const cx<float>* __restrict__ tw,
     const double* __restrict__ tabs, BinArgsDev a, const double* __restrict__ xq,
-    long long* __restrict__ xpart, int nkzb) {
+    long long* __restrict__ xpart, int nkzb, int64_t ms) {
   constexpr int N = N1 * N2, E = clcm(N1, N2), TS = N / E, G1 = E / N2;
   constexpr int B = 2 * G * KZ;
   constexpr bool TW_REG = G1 == 1;
   extern __shared__ unsigned char smem[];
+  // stacked batch: catalogue blockIdx.y (grids ms reals apart, its own
+  // quanta and table rows)
+  if (blockIdx.y) {
+    const int64_t mo = (int64_t)blockIdx.y * ms;
+    i0 += mo; i1 += mo; i2 += mo; i3 += mo;
+    xq += (int64_t)blockIdx.y * 6;
+  }
   cx<float>* lds = reinterpret_cast<cx<float>*>(smem);
   unsigned long long* tab = reinterpret_cast<unsigned long long*>(smem + (size_t)B * (N + N1) * 8);
   constexpr int NL = ODD ? 5 : 3;   // multipoles summed: 0, 2, 4 [, 1, 3]
@@ -1238,20 +1259,24 @@ __global__ __launch_bounds__((N1 * N2 / clcm(N1, N2)) * (2 * G * KZ)) void k_fft
     // the next item's first stage overwrites the rows
     __syncthreads();
   }
-  long long* out = xpart + (int64_t)blockIdx.x * F * NK;
+  long long* out = xpart + ((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * F * NK;
   for (int q = threadIdx.x; q < F * NK; q += blockDim.x) out[q] = (long long)tab[q];
 }
 
 // Workgroup tables summed exactly (integer sums: any order gives the same
 // result), times their spectrum's quantum: the sums of L.out ([s][l][k],
 // nlk = nl NK per spectrum), then the mode counts (the k_bin_final layout of
-// the branch-free forms).  One block per entry.
+// the branch-free forms).  One block per entry; stacked batch: catalogue
+// blockIdx.y, its outputs out_stride doubles after the one before it.
 __global__ __launch_bounds__(256) void k_xbin_final(const long long* __restrict__ xpart, int nwg,
                                                     int fnk, int nk, int nlk,
                                                     const double* __restrict__ counts,
                                                     const double* __restrict__ xq,
-                                                    double* __restrict__ out) {
+                                                    double* __restrict__ out, int64_t out_stride) {
   const int i = blockIdx.x;
+  xpart += (int64_t)blockIdx.y * nwg * fnk;
+  xq += (int64_t)blockIdx.y * 6;
+  out += (int64_t)blockIdx.y * out_stride;
   if (i >= fnk) {
     if (threadIdx.x == 0 && i < fnk + nk) out[i] = counts[i - fnk];
     return;
@@ -1311,13 +1336,14 @@ static int launch_xbin(FftArgs& f, BinLaunch& L, long long* xpart, int xpart_row
   const float* g[4];
   for (int q = 0; q < 4; q++) g[q] = (const float*)(q < f.ngrids ? f.grid[q] : f.grid[0]);
   TIMED(f.timer, "fft_x_bin",
-        (k_fft_x_bin<N1, N2, G, KZ, IL, HB, ODD><<<(unsigned)nblk, TS * B, lds, f.stream>>>(
+        (k_fft_x_bin<N1, N2, G, KZ, IL, HB, ODD><<<dim3((unsigned)nblk, (unsigned)f.nmock), TS * B,
+                                                   lds, f.stream>>>(
             g[0], g[1], g[2], g[3], f.px, (int64_t)nzh * f.px, (const cx<float>*)f.tw[0], L.tabs,
-            L.dev, f.xq, xpart, nkzb)));
+            L.dev, f.xq, xpart, nkzb, f.mock_stride)));
   const int nlk = (L.dev.odd ? 5 : 3) * L.dev.nk, fnk = L.dev.nspec * nlk;
   TIMED(f.timer, "bin_final",
-        (k_xbin_final<<<fnk + L.dev.nk, 256, 0, f.stream>>>(xpart, (int)nblk, fnk, L.dev.nk,
-                                                             nlk, L.counts, f.xq, L.out)));
+        (k_xbin_final<<<dim3((unsigned)(fnk + L.dev.nk), (unsigned)f.nmock), 256, 0, f.stream>>>(
+            xpart, (int)nblk, fnk, L.dev.nk, nlk, L.counts, f.xq, L.out, L.out_stride)));
   return hipGetLastError() == hipSuccess ? CAPOW_OK : CAPOW_ERR_HIP;
 }
 
@@ -1346,7 +1372,7 @@ int run_fft_x_bin(FftArgs& f, BinLaunch& L, long long* xpart, int xpart_rows) {
   const int rc = run_bin_counts(L);
   if (rc) return rc;
   const bool il = L.dev.interlace != 0, hb = L.dev.hasB != 0;
-  k_xbin_scale<<<1, 256, 0, f.stream>>>(f.ssq, f.ssq_rows, f.xbin_factor, il, hb, L.dev, f.xq);
+  k_xbin_scale<<<f.nmock, 256, 0, f.stream>>>(f.ssq, f.ssq_rows, f.xbin_factor, il, hb, L.dev, f.xq);
 #define X(A_, B_)                                                                          \
   if (n1 == A_ && n2 == B_) {                                                              \
     if (il && hb) return launch_xbin_rows<A_, B_, true, true>(f, L, xpart, xpart_rows);    \
@@ -1511,6 +1537,8 @@ static int run_fft_t(FftArgs& a) {
     a.swapped = !a.x_deferred && !a.rotated;
     return hipGetLastError() == hipSuccess ? CAPOW_OK : CAPOW_ERR_HIP;
   }
+  // stacked catalogues: the transposed register passes only
+  if (a.nmock > 1) return CAPOW_ERR_UNSUPPORTED;
   T* gp[4];
   for (int q = 0; q < 4; q++) gp[q] = q < a.ngrids ? (T*)a.grid[q] : (T*)a.grid[0];
   const int nx = g.n[0], ny = g.n[1], nz = g.n[2];

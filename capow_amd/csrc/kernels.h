@@ -79,6 +79,14 @@ struct Timer {
 // particles per bucket-kernel block chunk (one LDS aggregation round)
 #define BK_CHUNK 1024
 
+// One catalogue of a stacked batch (device table, one entry per catalogue):
+// its particles (n x 4), weights (n or null) and count.
+struct MockDev {
+  const void* parts;
+  const void* w;
+  int64_t n;
+};
+
 struct DepositArgs {
   const Geom* g;
   hipStream_t stream;
@@ -102,6 +110,12 @@ struct DepositArgs {
   void* grid[4];          // A, A_int, B, B_int (null when unused)
   bool halo = true;       // halo-row deposit where the tile shape allows
   bool bank = true;       // halo-row deposit: bank-balanced copy order, masked rows
+  // stacked batch: nmock catalogues of one geometry (g->mock_stride set),
+  // their particles in the device table `mocks` (parts, wts, n unused);
+  // counts, offsets and cursors hold nmock * nbuckets entries, partials
+  // nmock * count_blocks rows, errs 4 and stats STAT_TOTAL per catalogue
+  int nmock = 1;
+  const MockDev* mocks = nullptr;
 };
 // count + stats + scan; then scatter + tile deposit
 int run_count(DepositArgs& a, bool fp64);
@@ -152,6 +166,11 @@ struct FftArgs {
   // hold the rotated pointers (rotated = true), for the caller to keep
   bool one_spare = false;
   bool rotated = false;
+  // stacked batch: nmock catalogues, each buffer's catalogue m at m *
+  // mock_stride reals (ssq: nmock blocks of ssq_rows rows; xq: 6 doubles
+  // per catalogue); transposed register pipeline only
+  int nmock = 1;
+  int64_t mock_stride = 0;
 };
 #define XBIN_SSQ_MAX 8192
 int run_fft(FftArgs& a, bool fp64);
@@ -195,7 +214,8 @@ struct BinLaunch {
   int nwg;
   int per_copy;
   double* partials;       // nwg * per_copy
-  double* out;            // per_copy
+  double* out;            // per_copy (stacked batch: per catalogue, out_stride apart)
+  int64_t out_stride = 0;
   double* counts;         // nk: mode counts of the branch-free form
   bool* counts_valid;     // counts computed for this geometry
 };

@@ -291,6 +291,20 @@ int capow_batch_run(capow_batch *b, const capow_mock *mocks, int32_t n,
  * then -1.  A catalogue re-run alone (sort buffer too small) or an
  * autotune drops the graph.  on = 0 (the default) drops it.  Synchronises. */
 int capow_batch_set_graph(capow_batch *b, int32_t on);
+/* Stacked mode (no counterpart in the reference): m >= 1 runs the
+ * catalogues of each parameter set (and of the same weighting: all weighted
+ * or all unit weights) in stacks of up to m, every stage one launch for
+ * the whole stack -- the sort, the statistics, the deposit, the z and y
+ * passes and the fused x pass + binning, each kernel taking the catalogue
+ * from its grid's second index -- on grids laid out catalogue after
+ * catalogue.  Each set's stack runs on stream (set % streams in use).  The
+ * bins are bitwise those of the per-catalogue path.  Needs the fused
+ * binning (FP32 grids, 1-D bins, register FFT lengths) for every set,
+ * else CAPOW_ERR_UNSUPPORTED and the mode is unchanged; m is capped so
+ * that m times max_particles stays below 2^32 - 4096 sorted rows.  ms[i]
+ * is then the GPU time of mock i's stack.  m = 0 (the default) returns to
+ * one catalogue per launch.  Allocates the stacks' buffers; synchronises. */
+int capow_batch_set_stack(capow_batch *b, int32_t m);
 int capow_batch_destroy(capow_batch *b);
 const char *capow_batch_last_error(const capow_batch *b);
 

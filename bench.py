@@ -112,38 +112,67 @@ def run_batch(args):
     log("batch: %d mocks, %d particles generated in %.1f s" % (len(cats), ntot, time.perf_counter() - t0))
     b = api.Batch(cfgs, max_particles=int(counts.max()), nstreams=args.mock_streams)
     b.run(cats, which)                    # warmup: one untimed pass over every catalogue
+    # the autotuner picks the per-set settings, the streams in use and the
+    # mode: stacks of 16 or 64 catalogues (one launch per stage per stack)
+    # or one catalogue per launch
     tune = None if args.no_autotune else b.autotune(cats, which, reps=2)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    res, stats, ms = b.run(cats, which)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    # graph mode: the same list captured into one HIP graph (untimed first
-    # run: eager + capture), then one timed replay, checked against the
-    # eager run's bins
+    if args.no_autotune:
+        b.set_stack(64)
+    chosen = [ln.strip() for ln in tune.splitlines() if ln.startswith("chosen")] if tune else None
+    stack = 64 if args.no_autotune else int(chosen[-1].split("=")[1])
+
+    def timed_run():
+        b.run(cats, which)                # untimed (a graph: eager + capture)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = b.run(cats, which)
+        torch.cuda.synchronize()
+        return r, time.perf_counter() - t0
+
+    (res, stats, ms), wall = timed_run()
+    # graph mode: the same list captured into one HIP graph, then one timed
+    # replay, checked against the eager run's bins
     b.set_graph(True)
-    b.run(cats, which)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    gres, _, _ = b.run(cats, which)
-    torch.cuda.synchronize()
-    gwall = time.perf_counter() - t0
+    (gres, _, _), gwall = timed_run()
+    b.set_graph(False)
     same = all(np.array_equal(a[c]["pk"], g[c]["pk"]) for a, g in zip(res, gres) for c in a["codes"])
+    # the other mode beside it (stacks <-> per-catalogue streams), bins
+    # checked bitwise across the modes
+    b.set_stack(0 if stack else 64)
+    (ores, _, _), owall = timed_run()
+    same_modes = all(np.array_equal(a[c]["pk"], g[c]["pk"])
+                     for a, g in zip(res, ores) for c in a["codes"])
     b.close()
     del cats
     torch.cuda.empty_cache()
-    chosen = [ln.strip() for ln in tune.splitlines() if ln.startswith("chosen")] if tune else None
-    out = dict(mocks=len(counts), mocks_per_s=round(len(counts) / wall, 1), autotune=chosen,
+    # algorithmic HBM bytes: per particle 16 B read by the sort's count,
+    # 16 + 16 B by its scatter, 16 B by the deposit; per cell and grid (A,
+    # A_int) 4 B deposit write, 4 + 4 B z pass, 4 + 4 B y pass, 4 B fused
+    # x pass + binning
+    cells = sum(float(cfgs[q]["nx"]) ** 3 for q in which)
+    algo = 64.0 * ntot + 2 * 24.0 * cells
+    gbs = algo / wall / 1e9
+    mode = "stacks of %d" % stack if stack else "one catalogue per launch"
+    out = dict(mocks=len(counts), mocks_per_s=round(len(counts) / wall, 1), mode=mode,
+               autotune=chosen,
                graph_mocks_per_s=round(len(counts) / gwall, 1), graph_bins_equal_eager=bool(same),
+               other_mode=("one catalogue per launch" if stack else "stacks of 64"),
+               other_mode_mocks_per_s=round(len(counts) / owall, 1),
+               bins_equal_across_modes=bool(same_modes),
                wall_s=round(wall, 4), p50_ms=round(float(np.percentile(ms, 50)), 3),
                p99_ms=round(float(np.percentile(ms, 99)), 3), particles=ntot,
                mparticles_per_s=round(ntot / wall / 1e6, 1), streams=args.mock_streams,
+               roofline={"bound": "hbm", "achieved": round(gbs, 1), "peak": 8000.0, "unit": "GB/s",
+                         "frac": round(gbs / 8000.0, 3), "algo_bytes": algo,
+                         "note": "whole-batch algorithmic bytes over the batch's wall time"},
                grids="128^3 (NK 80) / 192^3 (NK 120) alternating", box=Lb,
                interp="CIC+interlacing", dtype="f32 grids, f64 bins",
-               per_mock_ms=("GPU time from a mock's first to last stage on its stream, "
-                            "sharing the GPU with the other streams"))
-    log("batch: %.1f mocks/s (graph replay %.1f, bins equal %s), p50 %.3f ms, p99 %.3f ms per mock"
-        % (out["mocks_per_s"], out["graph_mocks_per_s"], same, out["p50_ms"], out["p99_ms"]))
+               per_mock_ms=("stacks: GPU time of the mock's stack; one catalogue per launch: "
+                            "GPU time from the mock's first to last stage on its stream"))
+    log("batch (%s): %.1f mocks/s (graph replay %.1f, bins equal %s; %s %.1f, equal %s), "
+        "%.0f GB/s algorithmic"
+        % (mode, out["mocks_per_s"], out["graph_mocks_per_s"], same, out["other_mode"],
+           out["other_mode_mocks_per_s"], same_modes, gbs))
     return out
 
 

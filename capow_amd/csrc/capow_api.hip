@@ -1544,6 +1544,8 @@ extern "C" int capow_batch_set_stack(capow_batch* b, int32_t m) {
   batch_drop_graph(b);
   if (m == 0) {
     b->stack = 0;
+    for (capow_ctx* d : b->sctx) capow_destroy(d);
+    b->sctx.clear();
     return CAPOW_OK;
   }
   const int64_t per = b->max_n > 0 ? b->max_n : 1;
@@ -2128,6 +2130,34 @@ extern "C" int capow_batch_autotune(capow_batch* b, const capow_mock* mocks, int
   }
   b->nactive = best;
   tune_note(rep, "chosen: streams=%d\n", best);
+  // then stacks (one launch per stage for up to m catalogues of a set)
+  // against the per-catalogue streams, where every set supports them
+  const int stack0 = b->stack;
+  int best_stack = 0;
+  if (stack0 > 0) capow_batch_set_stack(b, 0);
+  for (int m : {16, 64}) {
+    if (capow_batch_set_stack(b, m) != CAPOW_OK) {
+      tune_note(rep, "  stack=%d: unsupported (%s)\n", m, b->err.c_str());
+      break;
+    }
+    std::vector<double> t;
+    for (int r = 0; r <= reps; r++) {
+      for (hipStream_t st : b->streams) hipStreamSynchronize(st);
+      const auto t0 = std::chrono::steady_clock::now();
+      const int rc = capow_batch_run(b, mocks, n, out.data(), nullptr, nullptr);
+      const auto t1 = std::chrono::steady_clock::now();
+      if (rc) return rc;
+      if (r > 0) t.push_back(std::chrono::duration<double>(t1 - t0).count());
+    }
+    std::sort(t.begin(), t.end());
+    tune_note(rep, "  stack=%d: %.3f ms for %d catalogues\n", m, t[t.size() / 2] * 1e3, n);
+    if (t[t.size() / 2] < best_s) {
+      best_s = t[t.size() / 2];
+      best_stack = m;
+    }
+  }
+  capow_batch_set_stack(b, best_stack);
+  tune_note(rep, "chosen: stack=%d\n", best_stack);
   if (report && report_len > 0) {
     strncpy(report, rep.c_str(), (size_t)report_len - 1);
     report[report_len - 1] = 0;

@@ -324,7 +324,9 @@ int capow_autotune(capow_ctx *ctx, int32_t reps, char *report, int32_t report_le
 /* The same for a batch: each parameter set is tuned on its first catalogue
  * in `mocks` and the choice copied to every stream's context of that set;
  * then the number of streams in use (2, 4, ... up to the batch's) is timed
- * on the whole list. */
+ * on the whole list, then stacks of 16 and 64 catalogues
+ * (capow_batch_set_stack) against the per-catalogue streams; the fastest
+ * mode stays set. */
 int capow_batch_autotune(capow_batch *b, const capow_mock *mocks, int32_t n, int32_t reps,
                          char *report, int32_t report_len);
 

@@ -179,9 +179,11 @@ def run_batch(args):
 def run_adjoint(args):
     """Gradient configuration: 16,777,216 Zel'dovich particles (256^3
     lattice, L = 1000 Mpc/h), 512^3 grid, TSC + interlacing, density P_0,2,4,
-    FP32 form: one forward (capow_power) and one backward (capow_backward:
-    assignment and FFT recomputed, binning backward, inverse FFT, gather),
-    G = dL/d(raw bins) fixed."""
+    FP32 form: one forward (capow_power, keeping its sort and transform for
+    the backward: capow_keep_forward) and one backward right after it
+    (capow_backward: the pending x pass, binning backward, inverse FFT,
+    gather), G = dL/d(raw bins) fixed; beside it the backward that recomputes
+    the assignment and FFT (particles set again before it)."""
     import torch
     from capow_amd import api, mocks
     Lb, n = 1000.0, 512
@@ -190,29 +192,44 @@ def run_adjoint(args):
     p = api.make_params(nx=n, ny=n, nz=n, xmax=Lb, ymax=Lb, zmax=Lb, interp_order=3,
                         do_interlacing=1, momentum=0, nk=160, maxk=1.6, precision=api.FP32)
     ps = api.PowerSpectrum(p)
+    ps.keep_forward(True)
     ps.set_particles(parts, w)
     G = np.random.default_rng(1).normal(size=(1, 5, 160))
     ps.power()
     ps.backward(G)                     # warmup
     torch.cuda.synchronize()
     reps = 3
-    t0 = time.perf_counter()
+    fwd = bwd = 0.0
+    reused = True
     for _ in range(reps):
         ps.set_particles(parts, w)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
         ps.power()
-    torch.cuda.synchronize()
-    fwd = (time.perf_counter() - t0) * 1e3 / reps
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        gx, gw = ps.backward(G)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        reused = reused and ps.backward_reused()
+        fwd += (t1 - t0) * 1e3 / reps
+        bwd += (t2 - t1) * 1e3 / reps
+    ok = bool(torch.isfinite(gx).all().item() and torch.isfinite(gw).all().item())
+    # the recomputing backward (particles set again: the forward is not kept)
     t0 = time.perf_counter()
     for _ in range(reps):
         ps.set_particles(parts, w)
-        gx, gw = ps.backward(G)
+        gx2, gw2 = ps.backward(G)
     torch.cuda.synchronize()
-    bwd = (time.perf_counter() - t0) * 1e3 / reps
-    ok = bool(torch.isfinite(gx).all().item() and torch.isfinite(gw).all().item())
-    # per-kernel HIP-event times of one more backward (not in the figures above)
+    bwd_re = (time.perf_counter() - t0) * 1e3 / reps
+    same = bool(torch.equal(gx, gx2) and torch.equal(gw, gw2))
+    # per-kernel HIP-event times of one more forward + backward (not in the
+    # figures above)
     ps.set_timing(True)
     ps.reset_kernel_times()
     ps.set_particles(parts, w)
+    ps.power()
+    ps.reset_kernel_times()
     ps.backward(G)
     kt = {k: round(v[0], 4) for k, v in ps.kernel_times().items()}
     ps.set_timing(False)
@@ -228,9 +245,16 @@ def run_adjoint(args):
     torch.cuda.empty_cache()
     out = dict(workload="16,777,216 Zel'dovich particles, 512^3, TSC+interlacing, P_0,2,4, FP32",
                forward_ms=round(fwd, 3), backward_ms=round(bwd, 3),
-               backward_includes="assignment + FFT recomputed, binning backward, inverse FFT, gather",
+               backward_over_forward=round(bwd / fwd, 3),
+               forward_includes="sort (recording the rows' input indices), deposit, z/y passes, "
+                                "fused x pass + binning",
+               backward_includes="the forward's sort and transform reused: x pass, binning "
+                                 "backward, inverse FFT, gather",
+               backward_reused_forward=bool(reused),
+               backward_recompute_ms=round(bwd_re, 3), recompute_gradients_equal=same,
                backward_kernels_ms=kt, gradients_finite=ok)
-    log("adjoint: forward %.2f ms, backward %.2f ms" % (fwd, bwd))
+    log("adjoint: forward %.2f ms, backward %.2f ms (reused %s; recomputing %.2f ms, equal %s)"
+        % (fwd, bwd, reused, bwd_re, same))
     return out
 
 

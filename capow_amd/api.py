@@ -121,6 +121,8 @@ SIGNATURES = {
                             C.POINTER(C.c_int32)], C.c_int),
     "capow_reset_kernel_times": ([_V], C.c_int),
     "capow_backward": ([_V, _D, _V, _V, C.c_int32], C.c_int),
+    "capow_keep_forward": ([_V, C.c_int32], C.c_int),
+    "capow_backward_reused": ([_V, C.POINTER(C.c_int32)], C.c_int),
     "capow_batch_create": ([_P, C.c_int32, C.c_int64, C.c_int32, C.POINTER(_V)], C.c_int),
     "capow_batch_run": ([_V, C.POINTER(Mock), C.c_int32, C.POINTER(Bins),
                          C.POINTER(AssignStats), C.POINTER(C.c_float)], C.c_int),
@@ -325,7 +327,8 @@ class PowerSpectrum:
         """dL/d(raw bins) (array [n_spectra][5][nk], PowerSpectrum.bin()'s
         order) -> (dL/d(x, y, z, v_los) as (n, 4), dL/dw as (n,) or None) for
         the particles last set: torch tensors on the device for device
-        particles, numpy arrays otherwise.  Recomputes the forward."""
+        particles, numpy arrays otherwise.  Recomputes the forward unless
+        keep_forward(True) kept it (see backward_reused())."""
         G = np.ascontiguousarray(dL_draw, dtype=np.float64).ravel()
         ns = len(self.codes)
         if G.size != ns * 5 * self.p.nk:
@@ -344,6 +347,17 @@ class PowerSpectrum:
         self.check(lib().capow_backward(self.ctx, _dptr(G), gx.ctypes.data_as(_V),
                                         gw.ctypes.data_as(_V) if gw is not None else None, HOST))
         return gx, gw
+
+    def keep_forward(self, on=True):
+        """capow_keep_forward: the forward keeps its sort (with the rows'
+        input indices) and transform for the next backward."""
+        self.check(lib().capow_keep_forward(self.ctx, 1 if on else 0))
+
+    def backward_reused(self):
+        """True when the last backward reused the forward."""
+        r = C.c_int32()
+        self.check(lib().capow_backward_reused(self.ctx, C.byref(r)))
+        return bool(r.value)
 
     def grid_elems(self):
         n = C.c_int64()

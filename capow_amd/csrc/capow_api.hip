@@ -122,6 +122,15 @@ struct capow_ctx {
   bool want_idx = false;
   unsigned int* bwd_err = nullptr;   // tile gather: a tap outside its region
   bool bwd_tiles = true;             // tile gather (CAPOW_GATHER_TILES=0: per particle)
+  // capow_keep_forward: the forward's sort records the rows' input indices
+  // and capow_backward reuses the forward's sort and transform while
+  // fwd_kept holds: the grids are the transform (capow_fft) of the
+  // deposit of the current particles (fwd_assigned: the last assignment
+  // came from capow_assign; fwd_idx: it recorded the indices), unchanged
+  // since
+  bool keep_fwd = false;
+  bool fwd_assigned = false, fwd_idx = false, fwd_kept = false;
+  bool bwd_reused = false;           // the last backward reused the forward
   int fft_col_mul = 2;   // column batch multiplier of the y/x FFT passes (CAPOW_FFT_COL_MUL=1 halves it)
   // survey estimator: copies of the real-space fields, the monopole k-space
   // fields, bin partials and sums (allocated on first use)
@@ -797,6 +806,7 @@ extern "C" int capow_set_particles(capow_ctx* c, const void* xyzv, const void* w
   }
   c->n = n;
   c->state = 1;
+  c->fwd_assigned = c->fwd_kept = false;
   return CAPOW_OK;
 }
 
@@ -821,6 +831,7 @@ static void fill_stats(capow_ctx* c, const double* s) {
 static int assign_enqueue(capow_ctx* c, double* h_stats, unsigned int* h_errs, bool exact) {
   if (c->state < 1) return fail(c, CAPOW_ERR_STATE, "capow_assign before capow_set_particles");
   c->x_pending = false;   // the deposit rewrites every grid
+  c->fwd_kept = false;
   Geom& g = c->g;
   g.has_w = c->wts != nullptr;
   // cleared by kernels (no memset nodes when the stream is captured)
@@ -868,17 +879,20 @@ static int assign_enqueue(capow_ctx* c, double* h_stats, unsigned int* h_errs, b
     const int64_t capw = (int64_t)(c->sorted_w_cap / c->esz);
     cap = capw < cap ? capw : cap;
   }
-  if (c->want_idx) {
+  const bool idx = c->want_idx || (c->keep_fwd && !c->cap_fixed);
+  if (idx) {
     rc = ensure(c, (void**)&c->sorted_idx, &c->sorted_idx_cap, (size_t)(cap > 0 ? cap : 1) * 4);
     if (rc) return rc;
     a.sorted_idx = c->sorted_idx;
   }
+  c->fwd_idx = idx;
   a.sorted_cap = cap;
   rc = run_scatter_deposit(a, c->fp64);
   if (rc) return fail(c, rc, std::string("deposit launch failed: ") + hipGetErrorString(hipGetLastError()));
   HIPCHK(c, hipMemcpyAsync(h_stats, c->stats, STAT_TOTAL * 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipMemcpyAsync(h_errs, c->errs, 4 * sizeof(unsigned int), hipMemcpyDeviceToHost, c->stream));
   c->state = 2;
+  c->fwd_assigned = true;
   return CAPOW_OK;
 }
 
@@ -986,6 +1000,7 @@ extern "C" int capow_fft(capow_ctx* c) {
   c->x_pending = a.x_deferred;
   c->xbin_ssq_rows = a.ssq_rows;
   c->state = 3;
+  c->fwd_kept = c->fwd_assigned;
   return CAPOW_OK;
 }
 
@@ -1288,6 +1303,7 @@ extern "C" int capow_set_grid(capow_ctx* c, int32_t slot, const void* host, int6
   if (n < c->grid_elems) return fail(c, CAPOW_ERR_ARG, "host buffer too small");
   int rc = finish_x(c);   // the other slots stay consistent
   if (rc) return rc;
+  c->fwd_assigned = c->fwd_kept = false;
   const size_t w = (size_t)2 * c->g.nzh * c->esz, pitch = (size_t)c->g.rowlen * c->esz;
   HIPCHK(c, hipMemcpy2DAsync(c->grid[slot], pitch, host, w, w, (size_t)c->p.nx * c->p.ny,
                              hipMemcpyHostToDevice, c->stream));
@@ -1300,6 +1316,7 @@ extern "C" int capow_mark_assigned(capow_ctx* c, const capow_assign_stats* st) {
   c->x_pending = false;   // the caller's grids replace the pending transform
   if (st) c->last = *st;
   c->state = 2;
+  c->fwd_assigned = c->fwd_kept = false;
   return CAPOW_OK;
 }
 
@@ -1307,6 +1324,19 @@ extern "C" int capow_mark_transformed(capow_ctx* c) {
   if (!c) return CAPOW_ERR_ARG;
   c->x_pending = false;
   c->state = 3;
+  c->fwd_assigned = c->fwd_kept = false;
+  return CAPOW_OK;
+}
+
+extern "C" int capow_keep_forward(capow_ctx* c, int32_t on) {
+  if (!c) return CAPOW_ERR_ARG;
+  c->keep_fwd = on != 0;
+  return CAPOW_OK;
+}
+
+extern "C" int capow_backward_reused(const capow_ctx* c, int32_t* reused) {
+  if (!c || !reused) return CAPOW_ERR_ARG;
+  *reused = c->bwd_reused ? 1 : 0;
   return CAPOW_OK;
 }
 
@@ -1349,15 +1379,21 @@ extern "C" int capow_backward(capow_ctx* c, const double* dL_draw, void* grad_xy
   if (where != CAPOW_HOST && where != CAPOW_DEVICE)
     return fail(c, CAPOW_ERR_ARG, "where must be CAPOW_HOST or CAPOW_DEVICE");
   if (c->state < 1) return fail(c, CAPOW_ERR_STATE, "capow_backward before capow_set_particles");
-  // the forward grids, recomputed (nothing is stored between the passes);
-  // the sort also records each row's input index for the tile gather
+  // the forward grids: kept from the forward (capow_keep_forward, nothing
+  // changed since), or recomputed; the sort also records each row's input
+  // index for the tile gather
   const bool tiles = c->bwd_tiles && c->n < ((int64_t)1 << 32) &&
                      gather_tiles_lds(c->g, c->fp64) <= 160 * 1024;
-  c->want_idx = tiles;
-  int rc = capow_assign(c, nullptr);
-  c->want_idx = false;
-  if (rc) return rc;
-  rc = capow_fft(c);
+  const bool reuse = c->fwd_kept && c->state >= 3 && (!tiles || c->fwd_idx);
+  int rc = CAPOW_OK;
+  if (!reuse) {
+    c->want_idx = tiles;
+    rc = capow_assign(c, nullptr);
+    c->want_idx = false;
+    if (rc) return rc;
+    rc = capow_fft(c);
+  }
+  c->bwd_reused = reuse;
   if (!rc) rc = finish_x(c);   // the binning backward needs the k-space grids
   if (rc) return rc;
   const size_t gbytes = (size_t)c->nspec * 5 * c->p.nk * 8;
@@ -1385,6 +1421,7 @@ extern "C" int capow_backward(capow_ctx* c, const double* dL_draw, void* grad_xy
   f.persist = c->fft_persist;
   rc = run_ifft(f, c->fp64);
   c->state = 1;   // the grids now hold gradients
+  c->fwd_assigned = c->fwd_kept = false;
   if (rc == CAPOW_ERR_UNSUPPORTED)
     return fail(c, rc, "the backward needs register FFT sizes (16..1024 of 2^a 3^b with a split)");
   if (rc) return fail(c, rc, std::string("inverse fft failed: ") + hipGetErrorString(hipGetLastError()));

@@ -249,6 +249,19 @@ int capow_spectra(const capow_params *p, int32_t *n_spectra, int32_t codes[3]);
  * grids hold gradients (capow_assign must run again before capow_fft). */
 int capow_backward(capow_ctx *ctx, const double *dL_draw, void *grad_xyzv,
                    void *grad_w, int32_t where);
+/* Forward kept for the backward (no counterpart in the reference): with
+ * on = 1, capow_assign also records each sorted row's input index (4 bytes
+ * a row), and capow_backward reuses the forward's sort and transform
+ * instead of recomputing them while the grids still hold the transform
+ * (capow_fft) of the assignment (capow_assign) of the particles last set:
+ * capow_set_particles, capow_set_grid, capow_mark_assigned,
+ * capow_mark_transformed and capow_backward itself end that.  A caller that
+ * rewrites the particles in place between the forward and the backward
+ * sets them again (and the backward then recomputes).  The gradients are
+ * bitwise those of the recomputing backward.  Off by default. */
+int capow_keep_forward(capow_ctx *ctx, int32_t on);
+/* Whether the last capow_backward reused the forward (1) or recomputed it. */
+int capow_backward_reused(const capow_ctx *ctx, int32_t *reused);
 
 /* ---- batches of catalogues (mock runs) --------------------------------- */
 /* No counterpart in the reference, which measures one catalogue per

@@ -233,3 +233,63 @@ def test_tile_gather_matches_particle_gather(case, monkeypatch):
     # particles the sort drops have zero gradients in both
     dead = np.all(plain[0] == 0, axis=1)
     assert np.all(tiles[0][dead] == 0)
+
+
+@pytest.mark.parametrize("case", [TILE_CASES[1], TILE_CASES[2], TILE_CASES[0]],
+                         ids=[TILE_CASES[1][0], TILE_CASES[2][0], TILE_CASES[0][0]])
+def test_kept_forward_matches_recompute(case):
+    """capow_keep_forward: a backward right after the forward (capow_power)
+    reuses its sort and transform; the gradients must be bitwise those of
+    the backward that recomputes them (the same deposit sums, transform and
+    per-particle tap sums), the forward's bins unchanged by the recorded
+    indices, and anything that changes the particles or grids in between
+    must make the backward recompute."""
+    name, (nx, ny, nz), order, inter, los, mom, extra, weighted, prec = case
+    say("kept forward", name)
+    Lb = 1000.0
+    rng = np.random.default_rng(11)
+    pos = rng.uniform(0, Lb, size=(30000, 3))
+    vel = rng.normal(0, 200.0, size=pos.shape)
+    w = rng.uniform(-0.5, 2.0, size=len(pos)) if weighted else None
+    p = api.make_params(nx=nx, ny=ny, nz=nz, xmax=Lb, ymax=Lb, zmax=Lb, interp_order=order,
+                        do_interlacing=inter, los=los, momentum=mom, nk=16, maxk=0.16,
+                        extra_spectra=extra, precision=prec)
+    ns = len(AR.spectra(mom, extra))
+    G = np.random.default_rng(6).normal(size=(ns, 5, 16))
+    xyzv = mocks.xyzv(pos, vel, los)
+
+    ref = api.PowerSpectrum(p)
+    ref.set_particles(xyzv, w)
+    fwd_ref = ref.power()
+    ref.set_particles(xyzv, w)
+    gref = ref.backward(G)
+    assert not ref.backward_reused()
+
+    ps = api.PowerSpectrum(p)
+    ps.keep_forward(True)
+    ps.set_particles(xyzv, w)
+    fwd = ps.power()
+    for code in fwd_ref["codes"]:
+        assert np.array_equal(fwd[code]["pk"], fwd_ref[code]["pk"]), code
+    g = ps.backward(G)
+    assert ps.backward_reused()
+    assert np.array_equal(g[0], gref[0])
+    if gref[1] is not None:
+        assert np.array_equal(g[1], gref[1])
+    # the backward leaves gradients in the grids: the next one recomputes
+    g2 = ps.backward(G)
+    assert not ps.backward_reused()
+    assert np.array_equal(g2[0], gref[0])
+    # a forward, then new particles: recompute, on the new particles
+    ps.power()
+    pos2 = rng.uniform(0, Lb, size=(20000, 3))
+    xyzv2 = mocks.xyzv(pos2, rng.normal(0, 200.0, size=pos2.shape), los)
+    w2 = rng.uniform(-0.5, 2.0, size=len(pos2)) if weighted else None
+    ps.set_particles(xyzv2, w2)
+    g3 = ps.backward(G)
+    assert not ps.backward_reused()
+    ref.set_particles(xyzv2, w2)
+    g3r = ref.backward(G)
+    assert np.array_equal(g3[0], g3r[0])
+    ref.close()
+    ps.close()

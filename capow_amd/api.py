@@ -264,6 +264,8 @@ class PowerSpectrum:
         """xyzv: (n, 4) rows of x y z v_los; numpy (copied) or a torch CUDA
         tensor (used in place)."""
         dt = 1 if self.fp64 else 0
+        # every call starts a new generation (the autograd wrapper checks it)
+        self.generation = getattr(self, "generation", 0) + 1
         if hasattr(xyzv, "data_ptr") and getattr(xyzv, "is_cuda", False):
             assert xyzv.is_contiguous() and xyzv.shape[-1] == 4
             wp = _V(w.data_ptr()) if w is not None else None

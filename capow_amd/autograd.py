@@ -19,7 +19,11 @@ from . import api
 class _RawBins(torch.autograd.Function):
     @staticmethod
     def forward(ctx, xyzv, w, ps):
+        # the forward keeps its sort and transform: a backward that follows
+        # with nothing run on `ps` in between reuses them
+        ps.keep_forward(True)
         ps.set_particles(xyzv.detach().contiguous(), None if w is None else w.detach().contiguous())
+        ctx.generation = ps.generation
         res = ps.power()
         ns = len(res["codes"])
         raw = np.stack([res[c]["pk"] for c in res["codes"]])            # [ns][5][nk]
@@ -34,7 +38,9 @@ class _RawBins(torch.autograd.Function):
     def backward(ctx, graw, _gnm):
         xyzv, w = ctx.saved_tensors
         ps = ctx.ps
-        ps.set_particles(xyzv.detach().contiguous(), w.detach().contiguous() if ctx.has_w else None)
+        if ps.generation != ctx.generation:
+            # other particles were set since: recompute on these
+            ps.set_particles(xyzv.detach().contiguous(), w.detach().contiguous() if ctx.has_w else None)
         gx, gw = ps.backward(graw.detach().cpu().numpy(), want_w=True)
         if not ctx.has_w:
             # unit weights: the gradient with respect to them is still returned

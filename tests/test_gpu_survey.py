@@ -44,6 +44,10 @@ CASES = [
                                                    output_log=1, mink=-2.2, maxk=-0.6, nk=16)),
     ("fp32_tsc_int_odd", api.FP32, (64, 48, 32), dict(interp_order=3, do_interlacing=1,
                                                       odd_multipoles=1)),
+    # register FFT lengths on every axis (the transposed passes)
+    ("fp32_reg_cic_int_odd", api.FP32, (64, 32, 64), dict(interp_order=2, do_interlacing=1,
+                                                          odd_multipoles=1)),
+    ("fp32_reg_tsc_mom3", api.FP32, (32, 64, 32), dict(interp_order=3, momentum=3)),
 ]
 
 

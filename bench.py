@@ -119,15 +119,20 @@ def run_batch(args):
     if args.no_autotune:
         b.set_stack(64)
     chosen = [ln.strip() for ln in tune.splitlines() if ln.startswith("chosen")] if tune else None
-    stack = 64 if args.no_autotune else int(chosen[-1].split("=")[1])
+    stack = 64 if args.no_autotune else int(
+        [c for c in chosen if c.startswith("chosen: stack=")][0].split("=")[1])
 
     def timed_run():
-        b.run(cats, which)                # untimed (a graph: eager + capture)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        r = b.run(cats, which)
-        torch.cuda.synchronize()
-        return r, time.perf_counter() - t0
+        # one untimed run (a graph: eager + capture), then the median of 3
+        b.run(cats, which)
+        ts = []
+        for _ in range(3):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            r = b.run(cats, which)
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return r, sorted(ts)[1]
 
     (res, stats, ms), wall = timed_run()
     # graph mode: the same list captured into one HIP graph, then one timed

@@ -339,7 +339,9 @@ int capow_autotune(capow_ctx *ctx, int32_t reps, char *report, int32_t report_le
  * then the number of streams in use (2, 4, ... up to the batch's) is timed
  * on the whole list, then stacks of 16 and 64 catalogues
  * (capow_batch_set_stack) against the per-catalogue streams; the fastest
- * mode stays set. */
+ * mode stays set, and with stacks each set's stack context is tuned again
+ * (sort blocks, halo rows, bank order, FFT batch order) on its own
+ * catalogues run stacked. */
 int capow_batch_autotune(capow_batch *b, const capow_mock *mocks, int32_t n, int32_t reps,
                          char *report, int32_t report_len);
 

@@ -111,13 +111,13 @@ def run_batch(args):
     ntot = int(sum(c.shape[0] for c in cats))
     log("batch: %d mocks, %d particles generated in %.1f s" % (len(cats), ntot, time.perf_counter() - t0))
     b = api.Batch(cfgs, max_particles=int(counts.max()), nstreams=args.mock_streams)
+    if args.no_autotune:
+        b.set_stack(64)
     b.run(cats, which)                    # warmup: one untimed pass over every catalogue
     # the autotuner picks the per-set settings, the streams in use and the
     # mode: stacks of 16 or 64 catalogues (one launch per stage per stack)
     # or one catalogue per launch
     tune = None if args.no_autotune else b.autotune(cats, which, reps=2)
-    if args.no_autotune:
-        b.set_stack(64)
     chosen = [ln.strip() for ln in tune.splitlines() if ln.startswith("chosen")] if tune else None
     stack = 64 if args.no_autotune else int(
         [c for c in chosen if c.startswith("chosen: stack=")][0].split("=")[1])
@@ -143,10 +143,12 @@ def run_batch(args):
     same = all(np.array_equal(a[c]["pk"], g[c]["pk"]) for a, g in zip(res, gres) for c in a["codes"])
     # the other mode beside it (stacks <-> per-catalogue streams), bins
     # checked bitwise across the modes
-    b.set_stack(0 if stack else 64)
-    (ores, _, _), owall = timed_run()
-    same_modes = all(np.array_equal(a[c]["pk"], g[c]["pk"])
-                     for a, g in zip(res, ores) for c in a["codes"])
+    owall, same_modes = None, None
+    if not args.no_other_mode:
+        b.set_stack(0 if stack else 64)
+        (ores, _, _), owall = timed_run()
+        same_modes = all(np.array_equal(a[c]["pk"], g[c]["pk"])
+                         for a, g in zip(res, ores) for c in a["codes"])
     b.close()
     del cats
     torch.cuda.empty_cache()
@@ -162,8 +164,8 @@ def run_batch(args):
                autotune=chosen,
                graph_mocks_per_s=round(len(counts) / gwall, 1), graph_bins_equal_eager=bool(same),
                other_mode=("one catalogue per launch" if stack else "stacks of 64"),
-               other_mode_mocks_per_s=round(len(counts) / owall, 1),
-               bins_equal_across_modes=bool(same_modes),
+               other_mode_mocks_per_s=round(len(counts) / owall, 1) if owall else None,
+               bins_equal_across_modes=same_modes,
                wall_s=round(wall, 4), p50_ms=round(float(np.percentile(ms, 50)), 3),
                p99_ms=round(float(np.percentile(ms, 99)), 3), particles=ntot,
                mparticles_per_s=round(ntot / wall / 1e6, 1), streams=args.mock_streams,
@@ -174,7 +176,7 @@ def run_batch(args):
                interp="CIC+interlacing", dtype="f32 grids, f64 bins",
                per_mock_ms=("stacks: GPU time of the mock's stack; one catalogue per launch: "
                             "GPU time from the mock's first to last stage on its stream"))
-    log("batch (%s): %.1f mocks/s (graph replay %.1f, bins equal %s; %s %.1f, equal %s), "
+    log("batch (%s): %.1f mocks/s (graph replay %.1f, bins equal %s; %s %s, equal %s), "
         "%.0f GB/s algorithmic"
         % (mode, out["mocks_per_s"], out["graph_mocks_per_s"], same, out["other_mode"],
            out["other_mode_mocks_per_s"], same_modes, gbs))
@@ -624,6 +626,10 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default settings (no capow_autotune before the timed steps)")
     ap.add_argument("--no-survey", action="store_true")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="run the batched mocks alone and print their figures (not the bench line)")
+    ap.add_argument("--no-other-mode", action="store_true",
+                    help="batched mocks: time the chosen mode only (not the other one beside it)")
     ap.add_argument("--no-bin-variants", action="store_true")
     ap.add_argument("--profile-pass", action="store_true",
                     help="for rocprofv3 counter runs: one FP32 step and a %d-byte calibration "
@@ -637,6 +643,10 @@ def main():
     import torch
     from capow_amd import api, mocks
     api.lib()
+    if args.batch_only:
+        # (for profiling the batched mocks alone: prints their figures only)
+        print(json.dumps({"batched_mocks": run_batch(args)}), flush=True)
+        return
     L = 2000.0
     n = args.grid
     dk = 0.005

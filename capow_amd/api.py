@@ -18,7 +18,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libcapow.so")
 
 CAPOW_OK = 0
-CAPOW_ERR_IO, CAPOW_ERR_PARSE, CAPOW_ERR_OUT_OF_BOX = 7, 8, 110
+CAPOW_ERR_IO, CAPOW_ERR_PARSE, CAPOW_ERR_STALE, CAPOW_ERR_OUT_OF_BOX = 7, 8, 9, 110
 ERRORS = {1: "CAPOW_ERR_ARG", 2: "CAPOW_ERR_HIP", 3: "CAPOW_ERR_NOMEM", 4: "CAPOW_ERR_STATE",
           5: "CAPOW_ERR_UNSUPPORTED", 6: "CAPOW_ERR_INTERNAL", 7: "CAPOW_ERR_IO",
           8: "CAPOW_ERR_PARSE", 110: "CAPOW_ERR_OUT_OF_BOX"}
@@ -108,6 +108,9 @@ SIGNATURES = {
                         C.POINTER(C.c_int32)], C.c_int),
     "capow_write_output": ([_P, C.c_double, C.c_double, _D, _I64, _D, _I64, C.c_char_p,
                             C.c_int32], C.c_int),
+    "capow_write_sidecar": ([C.c_char_p, C.c_char_p, C.c_int32, C.c_char_p, C.c_int32], C.c_int),
+    "capow_read_sidecar": ([C.c_char_p, C.c_char_p, C.c_int32, _D, _D, C.c_int32, C.c_int32,
+                            C.POINTER(_V), C.POINTER(C.c_int64), _D], C.c_int),
     "capow_read_ascii": ([C.c_char_p, C.c_int32, _D, _D, C.c_int32, C.c_int32, C.POINTER(_V),
                           _I64, C.c_char_p, C.c_int32, _D], C.c_int),
     "capow_free_rows": ([_V], None),
@@ -443,6 +446,53 @@ def write_output(p, shot, norm, pk5, nmodes, path, pk2d=None, nmodes2d=None, pre
         _dptr(np.ascontiguousarray(pk2d, dtype=np.float64)) if pk2d is not None else None,
         _iptr(np.ascontiguousarray(nmodes2d, dtype=np.int64)) if nmodes2d is not None else None,
         path.encode(), 1 if precise else 0))
+
+
+def write_sidecar(text_path, bin_path, nthreads=0):
+    """capow_write_sidecar: parse the periodic ASCII catalogue at text_path
+    and write its binary sidecar to bin_path."""
+    bad = C.create_string_buffer(2048)
+    rc = lib().capow_write_sidecar(os.fsencode(text_path), os.fsencode(bin_path), nthreads, bad,
+                                   len(bad))
+    if rc != CAPOW_OK:
+        msg = ("error reading file: %s" % bad.value.decode(errors="replace")
+               if rc == CAPOW_ERR_PARSE else "cannot write the sidecar of %s" % text_path)
+        raise CapowError(rc, msg)
+
+
+def read_sidecar(bin_path, text_path=None, los=0, lo=(0.0, 0.0, 0.0), hi=(1000.0, 1000.0, 1000.0),
+                 precision=FP64, nthreads=0):
+    """capow_read_sidecar: the rows read_ascii gives for the text catalogue,
+    from its binary sidecar (refused as stale when text_path is given and
+    the text changed since the sidecar was written)."""
+    lo_a = (C.c_double * 3)(*lo)
+    hi_a = (C.c_double * 3)(*hi)
+    rows = _V()
+    n = C.c_int64()
+    xyz = (C.c_double * 3)()
+    fp64 = precision == FP64
+    rc = lib().capow_read_sidecar(os.fsencode(bin_path),
+                                  os.fsencode(text_path) if text_path is not None else None, los,
+                                  lo_a, hi_a, 1 if fp64 else 0, nthreads, C.byref(rows),
+                                  C.byref(n), xyz)
+    if rc != CAPOW_OK:
+        if rc == CAPOW_ERR_OUT_OF_BOX:
+            msg = "particle out of grid bounds: x=%f, y=%f, z=%f" % tuple(xyz)
+        elif rc == CAPOW_ERR_STALE:
+            msg = "sidecar %s is older than %s" % (bin_path, text_path)
+        elif rc == CAPOW_ERR_PARSE:
+            msg = "%s is not a catalogue sidecar" % bin_path
+        else:
+            msg = "cannot read %s" % bin_path
+        raise CapowError(rc, msg)
+    try:
+        dt = np.float64 if fp64 else np.float32
+        count = n.value * 4
+        buf = (C.c_char * max(count * np.dtype(dt).itemsize, 1)).from_address(rows.value)
+        out = np.frombuffer(buf, dtype=dt, count=count).reshape(n.value, 4).copy()
+    finally:
+        lib().capow_free_rows(rows)
+    return out
 
 
 def read_ascii(path, los=0, lo=(0.0, 0.0, 0.0), hi=(1000.0, 1000.0, 1000.0), precision=FP64,

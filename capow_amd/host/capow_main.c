@@ -221,13 +221,31 @@ static void banner(const char *t) {
  * box for LOS = 0 (capow_read_ascii: the same line semantics, parsed over
  * up to 16 threads).  Returns rows of (x, y, z, v_los) in the working
  * precision.  The RSD shift and the deposit run on the GPU. */
+/* CAPOW_SIDECAR=1: the rows come from the binary sidecar <path>.capowbin
+ * when it is current (written from this text file, unchanged since);
+ * otherwise the text is parsed and the sidecar written for the next run.
+ * Both give the same rows and the same errors. */
 static void *read_catalogue(const char *path, int fp64, long long *n_out) {
   const double lo[3] = {XMIN, YMIN, ZMIN}, hi[3] = {XMAX, YMAX, ZMAX};
   void *rows = NULL;
   int64_t n = 0;
   char bad[2000];
   double xyz[3] = {0, 0, 0};
-  const int rc = capow_read_ascii(path, LOS, lo, hi, fp64, 0, &rows, &n, bad, (int32_t)sizeof bad, xyz);
+  const char *sc = getenv("CAPOW_SIDECAR");
+  const int use_side = sc && atoi(sc) != 0;
+  char side[4200];
+  int rc = -1;
+  if (use_side && snprintf(side, sizeof side, "%s.capowbin", path) < (int)sizeof side) {
+    rc = capow_read_sidecar(side, path, LOS, lo, hi, fp64, 0, &rows, &n, xyz);
+    if (rc != CAPOW_OK && rc != CAPOW_ERR_OUT_OF_BOX) {
+      /* absent, stale or damaged: parse the text and write a new one */
+      rc = capow_write_sidecar(path, side, 0, bad, (int32_t)sizeof bad);
+      if (rc == CAPOW_OK) rc = capow_read_sidecar(side, path, LOS, lo, hi, fp64, 0, &rows, &n, xyz);
+      else if (rc != CAPOW_ERR_PARSE) rc = -1;   /* cannot write it: the text reader */
+    }
+  }
+  if (rc == -1)
+    rc = capow_read_ascii(path, LOS, lo, hi, fp64, 0, &rows, &n, bad, (int32_t)sizeof bad, xyz);
   if (rc == CAPOW_ERR_IO) {
     printf("Task %d cannot open input file\n", 0);
     FatalError("read_data", 96);

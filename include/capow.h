@@ -47,6 +47,7 @@ enum {
   CAPOW_ERR_INTERNAL = 6,   /* a kernel self-check failed                  */
   CAPOW_ERR_IO = 7,         /* file could not be opened / written          */
   CAPOW_ERR_PARSE = 8,      /* catalogue line without six numbers          */
+  CAPOW_ERR_STALE = 9,      /* binary sidecar older than its text catalogue */
   CAPOW_ERR_OUT_OF_BOX = 110
 };
 
@@ -365,6 +366,28 @@ int capow_read_ascii(const char *path, int32_t los, const double lo[3], const do
                      int32_t fp64, int32_t nthreads, void **rows, int64_t *n,
                      char *bad_line, int32_t bad_len, double bad_xyz[3]);
 void capow_free_rows(void *rows);
+
+/* Binary sidecar of the periodic catalogue (no counterpart in the
+ * reference, which parses the text on every run: src/read_data.c:56-64).
+ * capow_write_sidecar parses the text file at text_path with
+ * capow_read_ascii's line rules (nthreads as there) and writes bin_path: a
+ * 64-byte header ("CAPOWCAT", version 1, 6 columns, the row count, the text
+ * file's size and modification time) and the six numbers of every line as
+ * little-endian doubles in file order; written beside bin_path and renamed
+ * over it.  Errors: CAPOW_ERR_IO, CAPOW_ERR_PARSE (bad_line as for
+ * capow_read_ascii), CAPOW_ERR_NOMEM.
+ * capow_read_sidecar returns what capow_read_ascii returns on the text --
+ * the same rows bit for bit for any LOS, box and precision, the same first
+ * particle outside the box (LOS = 0) -- from the sidecar alone.  With
+ * text_path (may be NULL) it refuses a sidecar whose text file's size or
+ * modification time differ from those it recorded: CAPOW_ERR_STALE.  A
+ * file that is not a whole sidecar: CAPOW_ERR_PARSE.  Release the rows with
+ * capow_free_rows. */
+int capow_write_sidecar(const char *text_path, const char *bin_path, int32_t nthreads,
+                        char *bad_line, int32_t bad_len);
+int capow_read_sidecar(const char *bin_path, const char *text_path, int32_t los,
+                       const double lo[3], const double hi[3], int32_t fp64, int32_t nthreads,
+                       void **rows, int64_t *n, double bad_xyz[3]);
 
 /* ---- survey ingest (no GPU) -------------------------------------------- */
 

@@ -12,6 +12,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <unistd.h>
 
 #include "capow.h"
 
@@ -81,6 +82,36 @@ static void periodic_reader(const char *dir) {
   CHECK(strstr(bad, "oops") != NULL);
   snprintf(path, sizeof path, "%s/missing.txt", dir);
   CHECK(capow_read_ascii(path, 0, lo, hi, 1, 4, &rows8, &n8, NULL, 0, NULL) == CAPOW_ERR_IO);
+}
+
+/* The binary sidecar: written from the text by several threads, read back
+ * by several threads into the text reader's rows; the first out-of-box
+ * particle; a truncated sidecar. */
+static void periodic_sidecar(const char *dir) {
+  char path[4096], side[4200];
+  snprintf(path, sizeof path, "%s/cat.txt", dir);
+  snprintf(side, sizeof side, "%s/cat.capowbin", dir);
+  const double lo[3] = {0, 0, 0}, hi[3] = {1000, 1000, 1000};
+  periodic_file(path, 60000, -1, -1);
+  char bad[256];
+  double bxyz[3];
+  void *rt = NULL, *rs = NULL;
+  int64_t nt = 0, ns = 0;
+  CHECK(capow_write_sidecar(path, side, 8, bad, sizeof bad) == CAPOW_OK);
+  CHECK(capow_read_ascii(path, 2, lo, hi, 0, 8, &rt, &nt, bad, sizeof bad, bxyz) == CAPOW_OK);
+  CHECK(capow_read_sidecar(side, path, 2, lo, hi, 0, 8, &rs, &ns, bxyz) == CAPOW_OK);
+  CHECK(nt == 60000 && ns == nt && memcmp(rt, rs, (size_t)nt * 16) == 0);
+  capow_free_rows(rt);
+  capow_free_rows(rs);
+  periodic_file(path, 60000, -1, 20000);
+  CHECK(capow_write_sidecar(path, side, 8, bad, sizeof bad) == CAPOW_OK);
+  CHECK(capow_read_sidecar(side, path, 0, lo, hi, 1, 8, &rs, &ns, bxyz) == CAPOW_ERR_OUT_OF_BOX);
+  CHECK(fabs(bxyz[0] - 1200.0) < 1e-9);
+  /* truncated: not a whole sidecar */
+  FILE *f = fopen(side, "r+b");
+  CHECK(f != NULL && ftruncate(fileno(f), 64 + 48 * 100 - 8) == 0);
+  fclose(f);
+  CHECK(capow_read_sidecar(side, NULL, 0, lo, hi, 1, 8, &rs, &ns, bxyz) == CAPOW_ERR_PARSE);
 }
 
 static void survey_file(const char *path, int n, int randoms) {
@@ -210,6 +241,7 @@ int main(int argc, char **argv) {
     return 2;
   }
   periodic_reader(argv[1]);
+  periodic_sidecar(argv[1]);
   survey(argv[1]);
   output(argv[1]);
   printf("ok\n");

@@ -106,6 +106,32 @@ def test_catalogue_errors(tmp_path):
     assert "Fatal Error at line 110 in routine read_data" in out
 
 
+def test_catalogue_errors_through_the_sidecar(tmp_path):
+    """CAPOW_SIDECAR=1: the catalogue errors are the text reader's (the
+    first malformed line, the first particle out of the box), and a
+    catalogue whose parse succeeds leaves a current sidecar beside it (its
+    box error then comes from the sidecar on the next run)."""
+    from capow_amd import api
+    pf = write_params(str(tmp_path / "p.dat"))
+    env = {"CAPOW_SIDECAR": "1"}
+    bad = tmp_path / "bad.txt"
+    bad.write_text("# header\n1 2 3 4 5 6\n1 2 3\n")
+    rc, out = run(pf, str(bad), "r", "o", env=env)
+    assert rc == 1
+    assert "Task 0 has error reading file: 1 2 3" in out
+    assert "Fatal Error at line 102 in routine read_data" in out
+    assert not os.path.exists(str(bad) + ".capowbin")
+    oob = write_catalogue(str(tmp_path / "oob.txt"), np.array([[1.0, 2.0, 3.0], [1.0, 2.0, 1000.5]]),
+                          np.zeros((2, 3)))
+    for attempt in range(2):   # the first run writes the sidecar, the second reads it
+        rc, out = run(pf, oob, "r", "o", env=env)
+        assert rc == 1
+        assert "has particle out of grid bounds: x=1.000000, y=2.000000, z=1000.500000" in out
+        assert "Fatal Error at line 110 in routine read_data" in out
+        assert os.path.exists(oob + ".capowbin")
+        assert api.read_sidecar(oob + ".capowbin", oob, los=3).shape == (2, 4)
+
+
 def test_neither_periodic_nor_survey_is_rejected(tmp_path):
     pf = write_params(str(tmp_path / "p.dat"), Periodic=0, Survey=0)
     rc, out = run(pf, "d", "r", "o")

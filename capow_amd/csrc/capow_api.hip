@@ -85,6 +85,7 @@ struct capow_ctx {
   int bin_per_copy = 0;
   bool bin_row_form = true;
   bool bin_mirror = false;  // mirror-group binning (set up in setup())
+  bool bin_sym = false;     // symmetric x and y tables (the survey's mirror groups)
   int bin_mirror_env = -1;  // CAPOW_BIN_MIRROR: 0 off, 1 on where it applies
   // x pass fused with the binning (FP32 out-of-place FFT, mirror form, 1-D
   // bins): capow_fft runs the z and y passes and leaves the x pass pending;
@@ -376,6 +377,7 @@ int make_bin_plan(const capow_params& p, int64_t rowc, bool mirror_allowed, int 
     }
     const bool mu_even = !p.odd_multipoles || (p.los != 1 && p.los != 2);
     bp.mirror = sym && mu_even && mirror_allowed;
+    bp.sym = sym;
   }
   BinArgsDev& b = bp.b;
   b.nx = (int)NX; b.ny = (int)NY; b.nzh = nzh;
@@ -628,6 +630,7 @@ static int setup(capow_ctx* c) {
     HIPCHK(c, hipMemcpy(c->bin_tabs, tab.data(), bp.ntab * 8, hipMemcpyHostToDevice));
     c->bin = bp.b;
     c->bin_mirror = bp.mirror;
+    c->bin_sym = bp.sym && c->bin_mirror_env != 0;
     c->bin_row_form = bp.row_form;
     c->bin_mirror2d = bp.mirror2d;
     c->nspec = bp.b.nspec;
@@ -1215,6 +1218,7 @@ extern "C" int capow_survey_power(capow_ctx* c, const double origin[3], double* 
   L.tabs = c->bin_tabs;
   for (int a = 0; a < 3; a++) L.origin[a] = origin[a];
   L.mom = mom ? 1 : 0;
+  L.mirror = c->bin_sym;
   L.nwg = c->sv_nwg;
   L.partials = c->sv_partials;
   L.acc = c->sv_acc;

@@ -226,6 +226,7 @@ struct BinLaunch {
 struct BinPlan {
   BinArgsDev b;
   bool mirror, row_form, mirror2d;
+  bool sym;               // f(N - i) = -f(i), cor(N - i) = cor(i) on x and y, even NX, NY
   int codes[3];
   int per_copy;           // doubles per bin copy
   int nwg;                // workgroups of the accumulation kernels
@@ -266,6 +267,7 @@ struct SurveyLaunch {
   const double* tabs;
   double origin[3];
   int mom;                // momentum cross spectrum (B grids)
+  bool mirror = false;    // mirror groups (symmetric x and y tables: BinPlan::sym)
   int nwg;
   double* partials;       // nwg * (up to 4) * NK: the sums one pass touches
   double* acc;            // 6 * NK: P0..P4, mode counts

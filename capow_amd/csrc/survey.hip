@@ -225,6 +225,181 @@ __global__ __launch_bounds__(256) void k_survey_bin(const T* __restrict__ gA,
   }
 }
 
+// The same pass over mirror groups (symmetric x and y tables): the modes
+// (i, j, k), (NX - i, j, k), (i, NY - j, k), (NX - i, NY - j, k) of a
+// canonical (i, j) (0 <= i <= NX/2, 0 <= j <= NY/2) share |k|, the bin, NM
+// and the window deconvolution, and their k^ components differ only in the
+// signs of x and y.  One wave per canonical (i, j), lanes over k: the
+// geometry and the pass's k^ factor F are computed once per group, each
+// live mirror m contributes s_m NM cor (its power) with its own grid values
+// and interlace phase (s_m: the sign the flipped axes give the k^
+// product), and the group's sum times F goes to the wave's LDS copy in one
+// set of adds.
+template <typename T>
+__global__ __launch_bounds__(256) void k_survey_bin_mirror(const T* __restrict__ gA,
+                                                           const T* __restrict__ gAi,
+                                                           const T* __restrict__ gB,
+                                                           const T* __restrict__ gBi,
+                                                           T* __restrict__ F0, T* __restrict__ F0m,
+                                                           const double* __restrict__ tabs,
+                                                           BinArgsDev a, int mom, int multipole,
+                                                           int ii, int jj, int kk,
+                                                           double* __restrict__ partials) {
+  typedef typename CT<T>::t C;
+  extern __shared__ double sm[];
+  const int NK = a.nk;
+  const int per_copy = survey_slots(multipole) * NK;
+  const int nwave = blockDim.x >> 6;
+  for (int q = threadIdx.x; q < per_copy * nwave; q += blockDim.x) sm[q] = 0.0;
+  __syncthreads();
+  double* acc = sm + (threadIdx.x >> 6) * per_copy;
+  const C* A = reinterpret_cast<const C*>(gA);
+  const C* Ai = reinterpret_cast<const C*>(gAi);
+  const C* B = reinterpret_cast<const C*>(gB);
+  const C* Bi = reinterpret_cast<const C*>(gBi);
+  C* f0 = reinterpret_cast<C*>(F0);
+  C* f0m = reinterpret_cast<C*>(F0m);
+  const double twopi = 2.0 * 3.14159265358979323846;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63;
+  const int hy = a.ny / 2 + 1;
+  const int64_t ngroups = (int64_t)(a.nx / 2 + 1) * hy;
+  // the sign a flipped axis gives the pass's k^ product, per mirror
+  // (bit 0: x flipped, bit 1: y flipped): ax = axes of the product, x or y
+  // counted once per odd power
+  int odd_x = 0, odd_y = 0;
+  {
+    int e[3] = {0, 0, 0};
+    if (multipole >= 1) e[ii]++;
+    if (multipole >= 2) e[jj]++;
+    if (multipole == 3) e[kk]++;
+    if (multipole == 4) { e[ii]++; e[kk]++; }
+    odd_x = e[0] & 1;
+    odd_y = e[1] & 1;
+  }
+  for (int64_t grp = (int64_t)blockIdx.x * nwave + wave; grp < ngroups;
+       grp += (int64_t)gridDim.x * nwave) {
+    const int i = (int)(grp / hy);
+    const int j = (int)(grp - (int64_t)i * hy);
+    const bool mx = i > 0 && 2 * i < a.nx, my = j > 0 && 2 * j < a.ny;
+    const int i2 = mx ? a.nx - i : i, j2 = my ? a.ny - j : j;
+    const RowVals R = row_vals(i, j, tabs, a);
+    // each mirror's row and interlace phase (x and y part)
+    const int mi[4] = {i, i2, i, i2}, mj[4] = {j, j, j2, j2};
+    const bool live[4] = {true, mx, my, mx && my};
+    int64_t rowb[4];
+    double qr[4], qi[4], sgn[4];
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+      rowb[m] = ((int64_t)mi[m] * a.ny + mj[m]) * a.rowc;
+      qr[m] = 1.0;
+      qi[m] = 0.0;
+      if (a.interlace) {
+        const double pxr = tabs[a.opx + 2 * mi[m]], pxi = tabs[a.opx + 2 * mi[m] + 1];
+        const double pyr = tabs[a.opy + 2 * mj[m]], pyi = tabs[a.opy + 2 * mj[m] + 1];
+        qr[m] = pxr * pyr - pxi * pyi;
+        qi[m] = pxr * pyi + pxi * pyr;
+      }
+      sgn[m] = ((odd_x && (m & 1)) != (odd_y && (m & 2))) ? -1.0 : 1.0;
+    }
+    for (int k = lane; k < a.nzh; k += 64) {
+      const double fz = tabs[a.ofz + k];
+      const double pzr = tabs[a.opz + 2 * k], pzi = tabs[a.opz + 2 * k + 1];
+      const ModeGeom G = mode_geom(R, k, fz, tabs[a.ocz + k], pzr, pzi, a);
+      if (!G.ok) continue;
+      const double cor = G.cor;
+      double sum = 0.0, nmodes = 0.0;
+#pragma unroll
+      for (int m = 0; m < 4; m++) {
+        if (!live[m]) continue;
+        const double cr = qr[m] * pzr - qi[m] * pzi, ci = qr[m] * pzi + qi[m] * pzr;
+        const int64_t q = rowb[m] + k;
+        double dkr = (double)A[q].x, dki = (double)A[q].y;
+        if (a.interlace) {
+          const C w = Ai[q];
+          dkr += (double)w.x * cr - (double)w.y * ci;
+          dki += (double)w.x * ci + (double)w.y * cr;
+        }
+        double dkr_m = 0.0, dki_m = 0.0;
+        if (mom) {
+          dkr_m = (double)B[q].x;
+          dki_m = (double)B[q].y;
+          if (a.interlace) {
+            const C w = Bi[q];
+            dkr_m += (double)w.x * cr - (double)w.y * ci;
+            dki_m += (double)w.x * ci + (double)w.y * cr;
+          }
+        }
+        double power;
+        if (multipole == 0) {
+          C o;
+          o.x = (T)dkr;
+          o.y = (T)dki;
+          f0[q] = o;
+          if (mom) {
+            o.x = (T)dkr_m;
+            o.y = (T)dki_m;
+            f0m[q] = o;
+            power = (dkr * dki_m - dki * dkr_m) * cor;
+          } else {
+            power = (dkr * dkr + dki * dki) * cor;
+          }
+        } else {
+          const C z = f0[q];
+          const double zr = (double)z.x, zi = (double)z.y;
+          if (!mom) {
+            power = (multipole & 1) ? (dkr * zi - dki * zr) * cor : (dkr * zr + dki * zi) * cor;
+          } else {
+            const C zm = f0m[q];
+            const double mr = (double)zm.x, mi2 = (double)zm.y;
+            power = (multipole & 1)
+                        ? 0.5 * (dkr * mr + dki * mi2 + dkr_m * zr + dki_m * zi) * cor
+                        : 0.5 * (dkr * mi2 - dki * mr + dkr_m * zi - dki_m * zr) * cor;
+          }
+        }
+        sum += sgn[m] * (power * G.NM);
+        nmodes += G.NM;
+      }
+      const int kb = G.kbin;
+      if (multipole == 0) {   // slots P0 P2 P4 counts
+        atomicAdd(&acc[kb], sum);
+        atomicAdd(&acc[NK + kb], -0.5 * sum);
+        atomicAdd(&acc[2 * NK + kb], 0.375 * sum);
+        atomicAdd(&acc[3 * NK + kb], nmodes);
+        continue;
+      }
+      const double fk = G.fktot;
+      if (!(fk > 0.0)) continue;
+      // the canonical mode's k^ factor (the mirrors' signs are in sum)
+      const double kv[3] = {twopi * R.fx, twopi * R.fy, twopi * fz};
+      if (multipole == 1) {   // slots P1 P3
+        const double p1 = sum * (kv[ii] / fk);
+        atomicAdd(&acc[kb], p1);
+        atomicAdd(&acc[NK + kb], -1.5 * p1);
+      } else if (multipole == 2) {   // slots P2 P4
+        const double pre = ii != jj ? 2.0 : 1.0;
+        const double p2 = sum * (pre * kv[ii] * kv[jj] / (fk * fk));
+        atomicAdd(&acc[kb], 1.5 * p2);
+        atomicAdd(&acc[NK + kb], -3.75 * p2);
+      } else if (multipole == 3) {   // slot P3
+        const double pre = ii != kk ? (ii != jj ? 6.0 : 3.0) : 1.0;
+        atomicAdd(&acc[kb], 2.5 * pre * kv[ii] * kv[jj] * kv[kk] / (fk * fk * fk) * sum);
+      } else {   // slot P4
+        const double pre = ii == jj ? (ii != kk ? 4.0 : 1.0) : (jj == kk ? 6.0 : 12.0);
+        atomicAdd(&acc[kb],
+                  4.375 * pre * kv[ii] * kv[ii] * kv[jj] * kv[kk] / (fk * fk * fk * fk) * sum);
+      }
+    }
+  }
+  __syncthreads();
+  double* out = partials + (int64_t)blockIdx.x * per_copy;
+  for (int q = threadIdx.x; q < per_copy; q += blockDim.x) {
+    double t = 0.0;
+    for (int c = 0; c < nwave; c++) t += sm[c * per_copy + q];
+    out[q] = t;
+  }
+}
+
 // acc[target] += sum over the workgroup partials of one pass: one block per
 // sum, each thread a fixed stride of workgroups, then a fixed-shape tree, so
 // the order of the additions never changes.
@@ -268,15 +443,22 @@ static int bin_t(SurveyLaunch& L, int multipole, const int idx[3]) {
   const int NK = L.dev.nk;
   const int per_copy = survey_slots(multipole) * NK;
   const size_t lds = (size_t)4 * per_copy * sizeof(double);
-  const void* fn = (const void*)k_survey_bin<T>;
+  const void* fn = L.mirror ? (const void*)k_survey_bin_mirror<T> : (const void*)k_survey_bin<T>;
   if (lds > 64 * 1024) hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   const T* w1 = (const T*)(L.work[1] ? L.work[1] : L.work[0]);
   const T* w2 = (const T*)(L.work[2] ? L.work[2] : L.work[0]);
   const T* w3 = (const T*)(L.work[3] ? L.work[3] : L.work[0]);
-  TIMED(L.timer, "survey_bin",
-        (k_survey_bin<T><<<L.nwg, 256, lds, L.stream>>>(
-            (const T*)L.work[0], w1, w2, w3, (T*)L.F0, (T*)(L.F0m ? L.F0m : L.F0), L.tabs, L.dev,
-            L.mom, multipole, idx[0], idx[1], idx[2], L.partials)));
+  if (L.mirror) {
+    TIMED(L.timer, "survey_bin",
+          (k_survey_bin_mirror<T><<<L.nwg, 256, lds, L.stream>>>(
+              (const T*)L.work[0], w1, w2, w3, (T*)L.F0, (T*)(L.F0m ? L.F0m : L.F0), L.tabs, L.dev,
+              L.mom, multipole, idx[0], idx[1], idx[2], L.partials)));
+  } else {
+    TIMED(L.timer, "survey_bin",
+          (k_survey_bin<T><<<L.nwg, 256, lds, L.stream>>>(
+              (const T*)L.work[0], w1, w2, w3, (T*)L.F0, (T*)(L.F0m ? L.F0m : L.F0), L.tabs, L.dev,
+              L.mom, multipole, idx[0], idx[1], idx[2], L.partials)));
+  }
   TIMED(L.timer, "survey_final",
         (k_survey_final<<<per_copy, 256, 0, L.stream>>>(L.partials, L.nwg, per_copy, NK, multipole,
                                                        L.acc)));

@@ -192,7 +192,7 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
 // per (chunk, key).  A key that finds no free slot within BK_PROBE probes
 // falls back to the global counter.
 #define BK_T 256
-#define BK_K 4
+#define BK_K 8
 #define BK_HS_BITS 11
 #define BK_HS (1 << BK_HS_BITS)
 #define BK_PROBE 32

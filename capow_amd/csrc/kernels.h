@@ -77,7 +77,7 @@ struct Timer {
   } while (0)
 
 // particles per bucket-kernel block chunk (one LDS aggregation round)
-#define BK_CHUNK 1024
+#define BK_CHUNK 2048
 
 // One catalogue of a stacked batch (device table, one entry per catalogue):
 // its particles (n x 4), weights (n or null) and count.

@@ -142,3 +142,26 @@ def test_stack_unsupported_sets_keep_streams():
     r, _, _ = b.run(cats, [0])
     assert r[0]["nmodes"].sum() > 0
     b.close()
+
+
+def test_stack_at_c3_sizes():
+    """The C3 geometry (128^3 with NK 80 and 192^3 with NK 120, CIC +
+    interlacing, L = 500, density) with lognormal catalogue sizes: stacks
+    of 3 catalogues against one catalogue per launch, bitwise."""
+    say("stack at C3 sizes: 4 + 4 catalogues of 128^3 / 192^3")
+    Lb = 500.0
+    cfgs = [dict(nx=g, ny=g, nz=g, xmax=Lb, ymax=Lb, zmax=Lb, interp_order=2, do_interlacing=1,
+                 momentum=0, nk=nk, maxk=nk * 0.01, precision=api.FP32)
+            for g, nk in ((128, 80), (192, 120))]
+    counts = mocks.lognormal_counts(8, seed=77, median=1.2e5, lo=6e4, hi=3e5)
+    cats = [mocks.mixed_mock_torch(int(n), Lb, seed=2000 + i) for i, n in enumerate(counts)]
+    which = [i % 2 for i in range(len(cats))]
+    ref_b = api.Batch(cfgs, max_particles=int(max(counts)), nstreams=2)
+    ref, ref_stats, _ = ref_b.run(cats, which)
+    b = api.Batch(cfgs, max_particles=int(max(counts)), nstreams=2)
+    b.set_stack(3)
+    r, st, _ = b.run(cats, which)
+    _same(ref, r, "C3 sizes")
+    _same_stats(ref_stats, st, "C3 sizes")
+    ref_b.close()
+    b.close()

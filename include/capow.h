@@ -311,8 +311,12 @@ int capow_batch_set_graph(capow_batch *b, int32_t on);
  * the whole stack -- the sort, the statistics, the deposit, the z and y
  * passes and the fused x pass + binning, each kernel taking the catalogue
  * from its grid's second index -- on grids laid out catalogue after
- * catalogue.  Each set's stack runs on stream (set % streams in use).  The
- * bins are bitwise those of the per-catalogue path.  Needs the fused
+ * catalogue.  Each set's stack runs on stream (set % streams in use).  With
+ * the same settings (the stack context starts from the set's stream
+ * contexts'; capow_batch_autotune may re-tune it) the bins and sums are
+ * bitwise those of the per-catalogue path; a different sort block count
+ * changes only the rounding of the FP64 moment sums (and, for weighted
+ * catalogues, of alpha).  Needs the fused
  * binning (FP32 grids, 1-D bins, register FFT lengths) for every set,
  * else CAPOW_ERR_UNSUPPORTED and the mode is unchanged; m is capped so
  * that m times max_particles stays below 2^32 - 4096 sorted rows.  ms[i]

@@ -1129,18 +1129,21 @@ __global__ __launch_bounds__((N1 * N2 / clcm(N1, N2)) * (2 * G * KZ)) void k_fft
         v[g * N2 + n2] = live ? row[n] : mk<float>(0.f, 0.f);
       }
   };
-  // binning split: KZ x PARTS units (kz slot, part of the canonical
-  // i = 0 .. NX/2), unit u to wave u % NW; lane l takes S consecutive i (S
-  // odd: the lanes' LDS reads, S complex apart, fall into distinct banks)
+  // binning split: the item's KZ x HX groups (kz slot, canonical
+  // i = 0 .. NX/2 - 1) numbered q = slot HX + i, thread t taking the S
+  // consecutive q from t S.  An even S visits them with the lowest bit
+  // flipped in every other run of 8 lanes, so that the lanes' 8-byte LDS
+  // reads (S complex apart) spread over the banks.  The x Nyquist groups
+  // (i = NX/2, one per kz slot) lie above every Nyquist frequency except at
+  // ky = kz = 0; threads 0 .. KZ-1 take them after an exact check of the bin
+  // condition, so they cost no round of their own.
   static_assert((TS * B) % 64 == 0, "whole waves");
-  constexpr int NW = TS * B / 64;
-  constexpr int PARTS = NW / KZ > 0 ? NW / KZ : 1;
-  constexpr int HX1 = N / 2 + 1;
-  constexpr int PART = (HX1 + PARTS - 1) / PARTS;
-  constexpr int S0 = (PART + 63) / 64;
-  constexpr int S = (S0 % 2) ? S0 : S0 + 1;
-  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  static_assert(N % 2 == 0, "even x extent: one x Nyquist group per kz");
+  constexpr int HX = N / 2;
+  constexpr int GI = KZ * HX;
+  constexpr int S = (GI + TS * B - 1) / (TS * B);
   const int lane = threadIdx.x & 63;
+  const int uflip = (S % 2) ? 0 : (((lane >> 3) ^ (lane >> 4)) & 1);
   const double iqs[3] = {xq[1], xq[3], xq[5]};
   const typename CT<float>::t zero = {0.f, 0.f};
   if (L0 < Lend) load(L0);
@@ -1150,112 +1153,123 @@ __global__ __launch_bounds__((N1 * N2 / clcm(N1, N2)) * (2 * G * KZ)) void k_fft
     if (item + step < Lend) load(item + step);
     rt_stage2<float, N1, N2>(lds, t, c);
     // bin the item's groups from LDS.  The item's y quantities (rows j and
-    // NY - j) are the same for every group; a lane takes S consecutive i,
-    // whose modes mostly share a k bin, and sums them in FP64 registers
-    // until the bin changes (one conversion to quanta and one set of LDS
-    // atomics per run).
+    // NY - j) are the same for every group; a thread takes S consecutive
+    // groups, whose modes mostly share a k bin, and sums them in FP64
+    // registers until the bin changes (one conversion to quanta and one set
+    // of LDS atomics per run).
     const int jc = (int)(item / nkzb);
+    const int kz0 = (int)(item - (int64_t)jc * nkzb) * KZ;
     const bool my = jc > 0 && 2 * jc < a.ny;
     const int j2 = my ? a.ny - jc : jc;
     const double fy = tabs[a.ofy + jc], cory = tabs[a.ocy + jc];
     const double pyr = tabs[a.opy + 2 * jc], pyi = tabs[a.opy + 2 * jc + 1];
     const double pyr2 = tabs[a.opy + 2 * j2], pyi2 = tabs[a.opy + 2 * j2 + 1];
-    for (int unit = wave; unit < KZ * PARTS; unit += NW) {
-      const int wks = unit % KZ, ib = (unit / KZ) * PART, ie = min(HX1, ib + PART);
-      const int kz = (int)(item - (int64_t)jc * nkzb) * KZ + wks;
-      if (kz >= a.nzh) continue;
-      const double fz = tabs[a.ofz + kz], corz = tabs[a.ocz + kz];
-      const double pzr = tabs[a.opz + 2 * kz], pzi = tabs[a.opz + 2 * kz + 1];
-      int cur = -1;
-      double acc[3][NL];
+    int cur = -1;
+    double acc[3][NL];
 #pragma unroll
-      for (int sp = 0; sp < 3; sp++)
+    for (int sp = 0; sp < 3; sp++)
 #pragma unroll
-        for (int q = 0; q < NL; q++) acc[sp][q] = 0.0;
-      auto flush = [&]() {
-        if (cur < 0) return;
+      for (int q = 0; q < NL; q++) acc[sp][q] = 0.0;
+    auto flush = [&]() {
+      if (cur < 0) return;
 #pragma unroll
-        for (int sp = 0; sp < 3; sp++) {
-          if (sp >= a.nspec) break;
-          unsigned long long* b = tab + sp * nl * NK + cur;
+      for (int sp = 0; sp < 3; sp++) {
+        if (sp >= a.nspec) break;
+        unsigned long long* b = tab + sp * nl * NK + cur;
 #pragma unroll
-          for (int q = 0; q < NL; q++) {
-            atomicAdd(&b[q * NK], (unsigned long long)__double2ll_rn(acc[sp][q] * iqs[sp]));
-            acc[sp][q] = 0.0;
-          }
-        }
-      };
-      for (int u = 0; u < S; u++) {
-        const int i = ib + lane * S + u;
-        if (i >= ie) break;
-        const bool mx = i > 0 && 2 * i < a.nx;
-        const int i2 = mx ? a.nx - i : i;
-        Group Gp;
-        Gp.i = i;
-        Gp.j = jc;
-        Gp.fx = tabs[a.ofx + i];
-        Gp.fy = fy;
-        Gp.fxy = dadd(dmul(Gp.fx, Gp.fx), dmul(fy, fy));
-        Gp.corxy = tabs[a.ocx + i] * cory;
-        Gp.live[0] = true;
-        Gp.live[1] = mx;
-        Gp.live[2] = my;
-        Gp.live[3] = mx && my;
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-          Gp.qr[r] = 1.0;
-          Gp.qi[r] = 0.0;
-        }
-        if constexpr (IL) {
-          const double pxr = tabs[a.opx + 2 * i], pxi = tabs[a.opx + 2 * i + 1];
-          const double pxr2 = tabs[a.opx + 2 * i2], pxi2 = tabs[a.opx + 2 * i2 + 1];
-          const double xr[4] = {pxr, pxr2, pxr, pxr2}, xi[4] = {pxi, pxi2, pxi, pxi2};
-          const double yr[4] = {pyr, pyr, pyr2, pyr2}, yi[4] = {pyi, pyi, pyi2, pyi2};
-#pragma unroll
-          for (int r = 0; r < 4; r++) {
-            Gp.qr[r] = xr[r] * yr[r] - xi[r] * yi[r];
-            Gp.qi[r] = xr[r] * yi[r] + xi[r] * yr[r];
-          }
-        }
-        typename CT<float>::t vv[4][4];
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-          const int js = r >> 1, ii = (r & 1) ? i2 : i;
-          const bool lv = Gp.live[r];
-#pragma unroll
-          for (int q = 0; q < 4; q++) {
-            // grid slot q (A, A_int, B, B_int) -> loaded grid index
-            const int gi = q == 0 ? 0 : q == 1 ? (IL ? 1 : -1) : q == 2 ? (HB ? (IL ? 2 : 1) : -1)
-                                                                        : (IL && HB ? 3 : -1);
-            vv[r][q] = (lv && gi >= 0) ? lds[((wks * 2 + js) * G + gi) * (N + 1) + ii] : zero;
-          }
-        }
-        ModeVals mv =
-            group_mode<typename CT<float>::t, IL, HB, true>(Gp, kz, fz, corz, pzr, pzi, vv, a);
-        if (mv.near) {
-          const KBin e = kbin_exact(Gp.fxy, fz, a);
-          mv.kbin = e.kbin;
-          mv.ok = e.ok;
-        }
-        if (!mv.ok) continue;
-        if (mv.kbin != cur) {
-          flush();
-          cur = mv.kbin;
-        }
-#pragma unroll
-        for (int sp = 0; sp < 3; sp++) {
-          const double np = mv.np[sp];
-          acc[sp][0] += np;
-          acc[sp][1] += mv.L2 * np;
-          acc[sp][2] += mv.L4 * np;
-          if constexpr (ODD) {
-            acc[sp][3] += mv.mu * np;
-            acc[sp][4] += (2.5 * mv.mu * mv.mu * mv.mu - 1.5 * mv.mu) * np;
-          }
+        for (int q = 0; q < NL; q++) {
+          atomicAdd(&b[q * NK], (unsigned long long)__double2ll_rn(acc[sp][q] * iqs[sp]));
+          acc[sp][q] = 0.0;
         }
       }
-      flush();
+    };
+    // group (i, jc) of kz slot wks (kz = kz0 + wks < NZ/2 + 1)
+    auto bin_group = [&](int wks, int i) {
+      const int kz = kz0 + wks;
+      const double fz = tabs[a.ofz + kz], corz = tabs[a.ocz + kz];
+      const double pzr = tabs[a.opz + 2 * kz], pzi = tabs[a.opz + 2 * kz + 1];
+      const bool mx = i > 0 && 2 * i < a.nx;
+      const int i2 = mx ? a.nx - i : i;
+      Group Gp;
+      Gp.i = i;
+      Gp.j = jc;
+      Gp.fx = tabs[a.ofx + i];
+      Gp.fy = fy;
+      Gp.fxy = dadd(dmul(Gp.fx, Gp.fx), dmul(fy, fy));
+      Gp.corxy = tabs[a.ocx + i] * cory;
+      Gp.live[0] = true;
+      Gp.live[1] = mx;
+      Gp.live[2] = my;
+      Gp.live[3] = mx && my;
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        Gp.qr[r] = 1.0;
+        Gp.qi[r] = 0.0;
+      }
+      if constexpr (IL) {
+        const double pxr = tabs[a.opx + 2 * i], pxi = tabs[a.opx + 2 * i + 1];
+        const double pxr2 = tabs[a.opx + 2 * i2], pxi2 = tabs[a.opx + 2 * i2 + 1];
+        const double xr[4] = {pxr, pxr2, pxr, pxr2}, xi[4] = {pxi, pxi2, pxi, pxi2};
+        const double yr[4] = {pyr, pyr, pyr2, pyr2}, yi[4] = {pyi, pyi, pyi2, pyi2};
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          Gp.qr[r] = xr[r] * yr[r] - xi[r] * yi[r];
+          Gp.qi[r] = xr[r] * yi[r] + xi[r] * yr[r];
+        }
+      }
+      typename CT<float>::t vv[4][4];
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int js = r >> 1, ii = (r & 1) ? i2 : i;
+        const bool lv = Gp.live[r];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          // grid slot q (A, A_int, B, B_int) -> loaded grid index
+          const int gi = q == 0 ? 0 : q == 1 ? (IL ? 1 : -1) : q == 2 ? (HB ? (IL ? 2 : 1) : -1)
+                                                                      : (IL && HB ? 3 : -1);
+          vv[r][q] = (lv && gi >= 0) ? lds[((wks * 2 + js) * G + gi) * (N + 1) + ii] : zero;
+        }
+      }
+      ModeVals mv =
+          group_mode<typename CT<float>::t, IL, HB, true>(Gp, kz, fz, corz, pzr, pzi, vv, a);
+      if (mv.near) {
+        const KBin e = kbin_exact(Gp.fxy, fz, a);
+        mv.kbin = e.kbin;
+        mv.ok = e.ok;
+      }
+      if (!mv.ok) return;
+      if (mv.kbin != cur) {
+        flush();
+        cur = mv.kbin;
+      }
+#pragma unroll
+      for (int sp = 0; sp < 3; sp++) {
+        const double np = mv.np[sp];
+        acc[sp][0] += np;
+        acc[sp][1] += mv.L2 * np;
+        acc[sp][2] += mv.L4 * np;
+        if constexpr (ODD) {
+          acc[sp][3] += mv.mu * np;
+          acc[sp][4] += (2.5 * mv.mu * mv.mu * mv.mu - 1.5 * mv.mu) * np;
+        }
+      }
+    };
+#pragma unroll
+    for (int u = 0; u < S; u++) {
+      const int q = (int)threadIdx.x * S + (u ^ uflip);
+      if (q >= GI) continue;
+      const int wks = q / HX;
+      if (kz0 + wks >= a.nzh) continue;
+      bin_group(wks, q - wks * HX);
     }
+    // the x Nyquist group of kz slot t: binned only where its |k| passes the
+    // bin condition (the same arithmetic as the group's own)
+    if ((int)threadIdx.x < KZ && kz0 + (int)threadIdx.x < a.nzh) {
+      const double fxn = tabs[a.ofx + HX];
+      const KBin kb = kbin_any(dadd(dmul(fxn, fxn), dmul(fy, fy)), tabs[a.ofz + kz0 + threadIdx.x], a);
+      if (kb.ok || kb.near) bin_group((int)threadIdx.x, HX);
+    }
+    flush();
     // the next item's first stage overwrites the rows
     __syncthreads();
   }

@@ -12,8 +12,9 @@ per-wave FP64 copies).  Mode counts are bit-exact and the sums agree within
 2e-6 of the bin's scale (FP32 rounding; the FP32 form's bar against FP64 is
 1e-4); the k-space grids read afterwards (the pending x pass run on demand)
 are bitwise those of the separate path.  Cases cover one,
-two and four grids (16 and 8 rows per item), odd multipoles, log bins and
-non-cubic grids.
+two and four grids (16 and 8 rows per item), odd multipoles, log bins,
+non-cubic grids and a grid whose x Nyquist frequency is the lowest (the x
+Nyquist group binned at ky = kz = 0 only).
 """
 import sys
 
@@ -32,6 +33,9 @@ CASES = [
     ("g4_odd_los_z", (64, 32, 128), dict(interp_order=3, do_interlacing=1, momentum=1,
                                          extra_spectra=3, odd_multipoles=1, los=3)),
     ("g2_cic_int_m0", (128, 96, 64), dict(interp_order=2, do_interlacing=1, momentum=0)),
+    # x has the lowest Nyquist frequency (within maxk): the x Nyquist group
+    # (NX/2, 0, 0) is binned, every other x Nyquist group is not
+    ("g2_x_nyquist", (64, 96, 128), dict(interp_order=2, do_interlacing=1, momentum=0)),
     ("g2_hb_no_int_m3", (96, 64, 32), dict(interp_order=2, do_interlacing=0, momentum=3,
                                             extra_spectra=3)),
     ("g1_ngp_m2", (64, 128, 64), dict(interp_order=1, do_interlacing=0, momentum=2)),

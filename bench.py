@@ -119,6 +119,8 @@ def run_batch(args):
     # or one catalogue per launch
     tune = None if args.no_autotune else b.autotune(cats, which, reps=2)
     chosen = [ln.strip() for ln in tune.splitlines() if ln.startswith("chosen")] if tune else None
+    if tune:
+        log("batch autotune:\n" + tune.rstrip())
     stack = 64 if args.no_autotune else int(
         [c for c in chosen if c.startswith("chosen: stack=")][0].split("=")[1])
 

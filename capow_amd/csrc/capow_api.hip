@@ -2200,44 +2200,48 @@ extern "C" int capow_batch_autotune(capow_batch* b, const capow_mock* mocks, int
   capow_batch_set_stack(b, best_stack);
   tune_note(rep, "chosen: stack=%d\n", best_stack);
   if (best_stack > 0) {
-    // the stack contexts start from the per-catalogue choices; a stack's
-    // best settings can differ (many catalogues per launch), so each set's
-    // stack context is tuned again on its own catalogues, stacked
+    // a stack's best settings can differ from one catalogue's (many
+    // catalogues per launch), so each set's stack context is tuned again,
+    // starting from the library defaults and timing the whole batch: the
+    // sets' stacks run on their own streams and share the GPU, and a
+    // setting timed on one set alone does not carry over to the shared run
+    auto time_all = [&](double* med) -> int {
+      std::vector<double> t;
+      for (int r = 0; r <= reps; r++) {
+        for (hipStream_t st : b->streams) hipStreamSynchronize(st);
+        const auto t0 = std::chrono::steady_clock::now();
+        const int rc = capow_batch_run(b, mocks, n, out.data(), nullptr, nullptr);
+        const auto t1 = std::chrono::steady_clock::now();
+        if (rc) return rc;
+        if (r > 0) t.push_back(std::chrono::duration<double>(t1 - t0).count());
+      }
+      std::sort(t.begin(), t.end());
+      *med = t[t.size() / 2];
+      return CAPOW_OK;
+    };
     for (int q = 0; q < b->nconf; q++) {
-      std::vector<capow_mock> sub;
-      for (int i = 0; i < n; i++)
-        if (mocks[i].config == q) sub.push_back(mocks[i]);
-      if (sub.empty()) continue;
+      bool used = false;
+      for (int i = 0; i < n && !used; i++) used = mocks[i].config == q;
+      if (!used) continue;
       capow_ctx* c = b->sctx[q];
       const int ncu = c->ncu > 0 ? c->ncu : 256;
-      int sort_blocks = c->count_blocks / ncu;
-      int halo = c->dep_halo ? 1 : 0, bank = c->dep_bank ? 1 : 0, order = c->fft_order;
-      std::vector<TuneKnob> knobs = {{"sort_blocks", &sort_blocks, {sort_blocks, 4, 8, 12}},
-                                     {"dep_halo", &halo, {halo, 1, 0}},
-                                     {"dep_bank", &bank, {bank, 1, 0}},
-                                     {"fft_order", &order, {order, 5, 0, 7, 2}}};
+      const int order0 = c->fp64 ? 0 : 5;
+      int sort_blocks = 12, sort_lds = 1, halo = 1, bank = 1, runs = 0, order = order0;
+      std::vector<TuneKnob> knobs = {{"sort_blocks", &sort_blocks, {12, 8, 4}},
+                                     {"sort_lds", &sort_lds, {1, 0}},
+                                     {"dep_halo", &halo, {1, 0}},
+                                     {"dep_bank", &bank, {1, 0}},
+                                     {"fft_runs", &runs, {0, 1}},
+                                     {"fft_order", &order, {order0, 0, 7, 2}}};
       auto apply = [&]() {
         c->count_blocks = sort_blocks * ncu;
+        c->g.sort_probes = sort_lds ? 32 : 0;
         c->dep_halo = halo != 0;
         c->dep_bank = bank != 0;
+        c->fft_rows_mul = runs;
         c->fft_order = order;
       };
-      std::vector<capow_bins> sout(sub.size());
-      for (auto& o : sout) memset(&o, 0, sizeof o);
-      auto time_sub = [&](double* med) -> int {
-        std::vector<double> t;
-        for (int r = 0; r <= reps; r++) {
-          for (hipStream_t st : b->streams) hipStreamSynchronize(st);
-          const auto t0 = std::chrono::steady_clock::now();
-          const int rc = capow_batch_run(b, sub.data(), (int32_t)sub.size(), sout.data(), nullptr, nullptr);
-          const auto t1 = std::chrono::steady_clock::now();
-          if (rc) return rc;
-          if (r > 0) t.push_back(std::chrono::duration<double>(t1 - t0).count());
-        }
-        std::sort(t.begin(), t.end());
-        *med = t[t.size() / 2];
-        return CAPOW_OK;
-      };
+      apply();
       for (TuneKnob& k : knobs) {
         int best_v = *k.value;
         double best_t = 1e30;
@@ -2247,7 +2251,7 @@ extern "C" int capow_batch_autotune(capow_batch* b, const capow_mock* mocks, int
           *k.value = v;
           apply();
           double med = 0.0;
-          const int rc = time_sub(&med);
+          const int rc = time_all(&med);
           if (rc) return rc;
           tune_note(rep, "  stack config %d %s=%d: %.3f ms\n", q, k.name, v, med * 1e3);
           if (med < best_t) {
@@ -2258,8 +2262,10 @@ extern "C" int capow_batch_autotune(capow_batch* b, const capow_mock* mocks, int
         *k.value = best_v;
         apply();
       }
-      tune_note(rep, "chosen: stack config %d sort_blocks=%d dep_halo=%d dep_bank=%d fft_order=%d\n",
-                q, sort_blocks, halo, bank, order);
+      tune_note(rep,
+                "chosen: stack config %d sort_blocks=%d sort_lds=%d dep_halo=%d dep_bank=%d "
+                "fft_runs=%d fft_order=%d\n",
+                q, sort_blocks, sort_lds, halo, bank, runs, order);
     }
   }
   if (report && report_len > 0) {

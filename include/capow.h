@@ -345,8 +345,9 @@ int capow_autotune(capow_ctx *ctx, int32_t reps, char *report, int32_t report_le
  * on the whole list, then stacks of 16 and 64 catalogues
  * (capow_batch_set_stack) against the per-catalogue streams; the fastest
  * mode stays set, and with stacks each set's stack context is tuned again
- * (sort blocks, halo rows, bank order, FFT batch order) on its own
- * catalogues run stacked. */
+ * (sort blocks, LDS run aggregation, halo rows, bank order, FFT write runs
+ * and batch order), starting from the library defaults and timing the
+ * whole list (the sets' stacks share the GPU). */
 int capow_batch_autotune(capow_batch *b, const capow_mock *mocks, int32_t n, int32_t reps,
                          char *report, int32_t report_len);
 

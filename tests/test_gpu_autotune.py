@@ -65,6 +65,16 @@ def test_batch_autotune_keeps_results():
     rep = b.autotune(cats, which, reps=2)
     say(rep)
     assert "streams=2" in rep and "chosen: streams=" in rep
+    assert "chosen: stack=" in rep
+    if "chosen: stack=0" not in rep:
+        # stacks chosen: each set's stack context is tuned again from the
+        # library defaults, every knob (the sort's LDS aggregation and the
+        # FFT write runs included) timed on the whole list
+        for q in (0, 1):
+            for knob in ("sort_blocks=12", "sort_lds=1", "sort_lds=0", "dep_halo=1", "dep_bank=1",
+                         "fft_runs=0", "fft_runs=1", "fft_order=5"):
+                assert "stack config %d %s:" % (q, knob) in rep, (q, knob)
+            assert "chosen: stack config %d sort_blocks=" % q in rep
     r1, _, _ = b.run(cats, which)
     b.close()
     torch.cuda.synchronize()
@@ -73,3 +83,39 @@ def test_batch_autotune_keeps_results():
         for code in a["codes"]:
             scale = np.abs(a[code]["pk"][0]).max()
             assert np.abs(a[code]["pk"] - c[code]["pk"]).max() <= 2e-6 * scale
+
+
+def test_batch_autotune_retunes_stacks():
+    """Many small catalogues on grids every stage can stack (32^3, 64^3): the
+    per-catalogue launches cost more than their work, so stacks win and each set's stack context is tuned again (every
+    knob from the library defaults, timed on the whole list); the stacked
+    bins after tuning are bitwise those of the untuned per-catalogue run
+    (every re-tuned knob keeps the sums; the stack keeps the fused binning)."""
+    import torch
+    say("batch autotune, 48 small catalogues")
+    Lb = 500.0
+    cfgs = [dict(nx=32, ny=32, nz=32, xmax=Lb, ymax=Lb, zmax=Lb, interp_order=2, do_interlacing=1,
+                 momentum=0, nk=20, maxk=0.2, precision=api.FP32),
+            dict(nx=64, ny=64, nz=64, xmax=Lb, ymax=Lb, zmax=Lb, interp_order=2, do_interlacing=1,
+                 momentum=0, nk=30, maxk=0.3, precision=api.FP32)]
+    counts = [1500 + 37 * i for i in range(48)]
+    cats = [mocks.mixed_mock_torch(c, Lb, seed=500 + i) for i, c in enumerate(counts)]
+    which = [i % 2 for i in range(len(cats))]
+    b = api.Batch(cfgs, max_particles=max(counts), nstreams=4)
+    r0, _, _ = b.run(cats, which)
+    rep = b.autotune(cats, which, reps=2)
+    say(rep)
+    assert "chosen: stack=0" not in rep and "chosen: stack=" in rep
+    for q in (0, 1):
+        for knob in ("sort_blocks=12", "sort_blocks=8", "sort_blocks=4", "sort_lds=1", "sort_lds=0",
+                     "dep_halo=1", "dep_halo=0", "dep_bank=1", "dep_bank=0", "fft_runs=0",
+                     "fft_runs=1", "fft_order=5", "fft_order=0", "fft_order=7", "fft_order=2"):
+            assert "stack config %d %s:" % (q, knob) in rep, (q, knob)
+        assert "chosen: stack config %d sort_blocks=" % q in rep
+    r1, _, _ = b.run(cats, which)
+    b.close()
+    torch.cuda.synchronize()
+    for i, (a, c) in enumerate(zip(r0, r1)):
+        assert np.array_equal(a["nmodes"], c["nmodes"]), i
+        for code in a["codes"]:
+            assert np.array_equal(a[code]["pk"].view(np.uint64), c[code]["pk"].view(np.uint64)), (i, code)

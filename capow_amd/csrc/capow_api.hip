@@ -2219,6 +2219,16 @@ extern "C" int capow_batch_autotune(capow_batch* b, const capow_mock* mocks, int
       *med = t[t.size() / 2];
       return CAPOW_OK;
     };
+    // every stack context at the defaults first: a set is timed against
+    // the others' defaults until their own turn
+    for (capow_ctx* c : b->sctx) {
+      c->count_blocks = 12 * (c->ncu > 0 ? c->ncu : 256);
+      c->g.sort_probes = 32;
+      c->dep_halo = true;
+      c->dep_bank = true;
+      c->fft_rows_mul = 0;
+      c->fft_order = c->fp64 ? 0 : 5;
+    }
     for (int q = 0; q < b->nconf; q++) {
       bool used = false;
       for (int i = 0; i < n && !used; i++) used = mocks[i].config == q;

@@ -574,7 +574,8 @@ static_assert((16 * 16 * 32 + DEP_TRASH) * 8 <= DEP_LDS_MAX, "FP32 tile");
 // leaves (-2^BITS - copies/2, 2^BITS + copies/2) whatever the order.
 // FP32 form (BITS 30): fields A and B of a cell share one int64
 // accumulator, A * 2^32 + B, updated with one 64-bit LDS atomic per tap
-// (the low word read as int32 is B, the rest is A).  FP64 form (BITS 62):
+// (the low word read as int32 is B, the rest is A); halo-row tiles without
+// a B field hold one int32 per cell (half the LDS, more workgroups per CU).  FP64 form (BITS 62):
 // one int64 accumulator per field; a tile of c copies rounds each tap to
 // about c x (largest |field|) x 2^-63, below the FP64 rounding of the
 // tile's largest values for every tile of fewer than 2^10 copies, no bound
@@ -645,14 +646,19 @@ __global__ __launch_bounds__(DEP_T) void k_deposit(const T* __restrict__ sorted,
   // the largest tile (TX <= 16; with compile-time extents, of this shape),
   // checked on the host
   constexpr int RZ = TZC + DEP_ZLO + DEP_ZHI;   // HALO row length (cells)
-  constexpr int LDSB = HALO ? (16 * TYC * RZ + 2) * 8
+  // HALO without a B field: one int32 accumulator per cell (A alone; the
+  // tile's quantum keeps every partial sum inside int32), else the packed
+  // int64 of A and B
+  constexpr int CB = (HALO && !HASB) ? 4 : 8;   // HALO bytes per cell
+  constexpr int LDSB = HALO ? ((16 * TYC * RZ + 2) * CB + 15) / 16 * 16
                             : ((TYC && TZC) ? (16 * TYC * TZC + DEP_TRASH) * 8 * (F32 ? 1 : 2)
                                             : DEP_LDS_MAX);
   static_assert(!HALO || (F32 && TYC && TZC), "halo rows: FP32, compile-time extents");
   static_assert(!BANK || HALO, "bank order: halo rows only");
   // BANK scratch (before a phase's clear): key counts, the (rank, key) slot
-  // tables and the reordered copies (x, y, z, a, b)
-  static_assert(!BANK || (DEP_NKEY + 2 * DEP_T) * 4 + 5 * DEP_T * DEP_K * 4 <= LDSB, "bank scratch");
+  // tables and the reordered copies (x, y, z, a[, b])
+  static_assert(!BANK || (DEP_NKEY + 2 * DEP_T) * 4 + (HASB ? 5 : 4) * DEP_T * DEP_K * 4 <= LDSB,
+                "bank scratch");
   __shared__ __align__(16) unsigned char smem[LDSB];
   // TYC, TZC: compile-time tile extents (0: the runtime ones)
   const int TX = g.tdim[0];
@@ -746,7 +752,7 @@ __global__ __launch_bounds__(DEP_T) void k_deposit(const T* __restrict__ sorted,
   // barrier after the maximum covers both
   auto clear_acc = [&]() {
     uint4* z4 = reinterpret_cast<uint4*>(smem);
-    const int n4 = HALO ? (TX * TY * RZ + 2) / 2 : (F32 ? nacc / 2 : nacc);   // 16-byte words
+    const int n4 = HALO ? ((TX * TY * RZ + 2) * CB + 15) / 16 : (F32 ? nacc / 2 : nacc);   // 16-byte words
     for (int c = threadIdx.x; c < n4; c += DEP_T) z4[c] = make_uint4(0u, 0u, 0u, 0u);
   };
   // (BANK: the LDS holds the reorder scratch first; each phase clears)
@@ -938,10 +944,10 @@ __global__ __launch_bounds__(DEP_T) void k_deposit(const T* __restrict__ sorted,
           if (per && (ttz == 0 || ttz == g.ntile[2] - 1)) lz0 = canon_off(lz0, g.n[2]);
           lz0 = lz0 < -3 ? -3 : (lz0 > TZ ? TZ : lz0);
           // LDS byte offsets stay below 2^17: 32-bit unsigned arithmetic
-          const uint32_t zb8 = (uint32_t)(lz0 + DEP_ZLO) * 8u;
-          const uint32_t lane_row = __umul24((uint32_t)lane, (uint32_t)(RZ * 8)) + zb8;
+          const uint32_t zb8 = (uint32_t)(lz0 + DEP_ZLO) * (uint32_t)CB;
+          const uint32_t lane_row = __umul24((uint32_t)lane, (uint32_t)(RZ * CB)) + zb8;
           // row (lx0 + tx, ly0 + ty) at base + a compile-time step
-          int base_b = (lx0 * TY + ly0) * (RZ * 8) + (int)zb8;
+          int base_b = (lx0 * TY + ly0) * (RZ * CB) + (int)zb8;
           // kept in a register (not recomputed per row with a 32-bit
           // multiply): an empty asm the compiler cannot see through
           asm volatile("" : "+v"(base_b));
@@ -954,7 +960,7 @@ __global__ __launch_bounds__(DEP_T) void k_deposit(const T* __restrict__ sorted,
               // a row outside the tile for every lane (ghost particles of
               // one border class share the pattern): nothing to add
               if (__builtin_amdgcn_ballot_w64(vxy) == 0) continue;
-              const uint32_t rowb = (uint32_t)(base_b + (tx * TY + ty) * (RZ * 8));
+              const uint32_t rowb = (uint32_t)(base_b + (tx * TY + ty) * (RZ * CB));
               // BANK: lanes outside the tile sit this row out (the row's
               // three atomics run under a narrower mask); otherwise they add
               // zeros to a row of the tile
@@ -965,6 +971,11 @@ __global__ __launch_bounds__(DEP_T) void k_deposit(const T* __restrict__ sorted,
 #pragma unroll
               for (int tz = 0; tz < ORDER; tz++) {
                 const int qa = round_rpi((float)(qa0 * az.w[tz]));
+                if constexpr (!HASB) {
+                  // two's complement: the int32 sum wraps back into range
+                  atomicAdd(reinterpret_cast<unsigned int*>(rb + tz * 4), (unsigned int)qa);
+                  continue;
+                }
                 const int qb = HASB ? round_rpi((float)(qb0 * az.w[tz])) : 0;
                 unsigned long long packed;
                 if constexpr (APOS) {
@@ -1059,7 +1070,11 @@ __global__ __launch_bounds__(DEP_T) void k_deposit(const T* __restrict__ sorted,
       lxx += step_x;
       if (zp >= TZ2) { zp -= TZ2; ly += 1; }
       if (ly >= TY) { ly -= TY; lxx += 1; }
-      if constexpr (F32) {
+      if constexpr (F32 && HALO && !HASB) {
+        // int32 cells: the pair at an even cell index (8-byte aligned)
+        const uint2 q = *reinterpret_cast<const uint2*>(smem + (size_t)cs * 4);
+        store2<T>(dA + gi, (T)((double)(int)q.x * invA - meanA), (T)((double)(int)q.y * invA - meanA));
+      } else if constexpr (F32) {
         const ulonglong2 q = *reinterpret_cast<const ulonglong2*>(acc64 + cs);
         const long long p0 = (long long)(swap ? q.y : q.x), p1 = (long long)(swap ? q.x : q.y);
         int a0, a1, b0, b1;

@@ -37,12 +37,25 @@ __global__ __launch_bounds__(256) void k_bin_backward(T* __restrict__ gA, T* __r
   C* Bi = reinterpret_cast<C*>(gBi);
   const bool il = a.interlace != 0, hb = a.hasB != 0;
   const int NK = a.nk;
-  for (int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; m < total;
-       m += (int64_t)gridDim.x * blockDim.x) {
-    const int i = (int)(m / plane);
-    const uint32_t r = (uint32_t)(m - (int64_t)i * plane);
-    const uint32_t j = a.fd_rowc.div(r);
-    const int k = (int)(r - j * (uint32_t)a.rowc);
+  // (x plane, offset in the plane) of the thread's mode, advanced by the
+  // grid stride with a carry: one 64-bit division per thread, not per mode
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t step_i = stride / plane;
+  const uint32_t step_r = (uint32_t)(stride - step_i * plane);
+  int64_t i64 = m / plane;
+  uint32_t r = (uint32_t)(m - i64 * plane);
+  for (; m < total; m += stride) {
+    const int i = (int)i64;
+    i64 += step_i;
+    const uint32_t rr = r;
+    r += step_r;
+    if (r >= plane) {
+      r -= plane;
+      i64++;
+    }
+    const uint32_t j = a.fd_rowc.div(rr);
+    const int k = (int)(rr - j * (uint32_t)a.rowc);
     double dr = 0, di = 0, er = 0, ei = 0;   // d delta, d beta
     double cr = 1, ci = 0;
     if (k < a.nzh) {

@@ -132,6 +132,10 @@ HALO_CASES = [
     ("tsc_int_cluster900", (64, 64, 64), 3, 1, 0, 1, False, 900),
     # ... and one with more copies than the registers hold (reloaded per phase)
     ("tsc_int_cluster3000_w", (64, 64, 64), 3, 1, 0, 3, True, 3000),
+    # density alone (no B field: one int32 per halo-row cell), interlaced CIC
+    # as in the batched mocks, clustered beyond the registers, and weighted
+    ("cic_int_m0_cluster3000", (64, 64, 64), 2, 1, 0, 0, False, 3000),
+    ("cic_int_m0_w", (64, 64, 96), 2, 1, 0, 0, True, 0),
 ]
 
 
